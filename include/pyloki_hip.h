@@ -1,0 +1,107 @@
+/*
+ * pyloki_hip.h -- C ABI of the MI355X (gfx950) search hot path of pyloki.
+ *
+ * The reference (pravirkr/pyloki v0.8.0) is Python + numba; its "FFI" for this path is
+ * the set of numba kernels and plugin methods listed beside each entry point below
+ * (paths relative to src/pyloki/). A Python caller binds these symbols with ctypes
+ * (pyloki_amd/_lib.py); the ctypes stub a pyloki maintainer would add is shown in
+ * INTEGRATION.md.
+ *
+ * Conventions
+ *  - Every entry point returns an int status: PYL_OK (0) or a negative code.
+ *    PYL_ERR_INVALID maps 1:1 to the reference's ValueError cases; the message is
+ *    available from pyl_last_error() (thread-local) and is re-raised as ValueError.
+ *  - Pointers named *_dev are device pointers (HBM); all other pointers are host
+ *    pointers read synchronously during the call. Arrays are C-contiguous.
+ *  - `stream` is a hipStream_t (NULL = the default stream). Kernels are enqueued on it;
+ *    the call returns without synchronising unless stated otherwise.
+ *  - A plan/handle is not re-entrant: one caller thread per handle at a time.
+ */
+#ifndef PYLOKI_HIP_H
+#define PYLOKI_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PYL_OK 0
+#define PYL_ERR_INVALID (-1) /* invalid argument: ValueError in the reference */
+#define PYL_ERR_HIP (-2)     /* HIP runtime error */
+#define PYL_ERR_ALLOC (-3)   /* device allocation failed */
+
+/* Last error message of the calling thread ("" if none). */
+const char* pyl_last_error(void);
+
+/* ABI version (major*100 + minor). */
+int pyl_abi_version(void);
+
+/* ------------------------------------------------------------------------------------
+ * Leaf brute fold.
+ * Replaces core/fold.py:108-171 brutefold_start (via ffa_taylor_init fold.py:527-569,
+ * FFATaylorDPFuncts.init dyn_pffa.py:48-55,190-204).
+ * out fold_dev: f32 [nseg][nfreq][2][nbins], nseg = ceil(nsamps/seg_len); per-bin FP32
+ * sums in sample order (bit-exact with the sequential reference loop). The plan holds the
+ * FP64-resolved phase-bin lists of one (freqs, seg_len, nbins, tsamp, t_ref) and can be
+ * reused for any number of series.
+ * ------------------------------------------------------------------------------------ */
+typedef struct pyl_bf_plan pyl_bf_plan;
+
+int pyl_brutefold_plan_create(const double* freqs, int64_t nfreq, int64_t seg_len,
+                              int32_t nbins, double tsamp, double t_ref, void* stream,
+                              pyl_bf_plan** plan_out);
+int pyl_brutefold_plan_destroy(pyl_bf_plan* plan);
+int pyl_brutefold_execute(const pyl_bf_plan* plan, const float* ts_e_dev, const float* ts_v_dev,
+                          int64_t nsamps, float* fold_dev, void* stream);
+
+/* ------------------------------------------------------------------------------------
+ * FFA level: resolve + shift-add.
+ * pyl_ffa_resolve replaces core/fold.py:617-671 ffa_taylor_resolve (with
+ * transforms.py:140-170 and psr_utils.py:12-56,466-514), evaluated in FP64 on the GPU
+ * for every output parameter set of the level (dyn_pffa.py:465-480).
+ *  params_dev: the current-level grids, concatenated (lengths counts_cur[0..P-1]);
+ *              output parameter set j is their cartesian product, last param fastest.
+ *  table_dev : int32 [ncoords_cur][4] = {row_tail, shift_tail, row_head, shift_head},
+ *              row = flattened index into the previous level's coordinates and
+ *              shift = int(f32(phase)+0.5f) with nbins -> 0 (common.py:313-318).
+ *  phases_dev: optional f64 [ncoords_cur][2] FP64 phases (tail, head); may be NULL.
+ * pyl_ffa_shift_add replaces the segment-pair loop of unify_fold (dyn_pffa.py:481-489)
+ * with core/common.py:304-324 shift_add:
+ *  out[q][j][c][b] = in[2q][row_t][c][(b-s_t) mod B] + in[2q+1][row_h][c][(b-s_h) mod B].
+ * pyl_ffa_level = resolve + shift_add in one call (table_dev is the caller's scratch).
+ * ------------------------------------------------------------------------------------ */
+int pyl_ffa_resolve(int32_t nparams, const double* params_dev, const int64_t* counts_cur,
+                    const int64_t* counts_prev, const double* param_limits, int32_t ffa_level,
+                    double tseg_brute, int32_t nbins, int32_t* table_dev, double* phases_dev,
+                    void* stream);
+int pyl_ffa_shift_add(const float* fold_in_dev, int64_t nseg_in, int64_t ncoords_prev,
+                      const int32_t* table_dev, int64_t ncoords_cur, int32_t nbins,
+                      float* fold_out_dev, void* stream);
+int pyl_ffa_level(int32_t nparams, const double* params_dev, const int64_t* counts_cur,
+                  const int64_t* counts_prev, const double* param_limits, int32_t ffa_level,
+                  double tseg_brute, int32_t nbins, const float* fold_in_dev, int64_t nseg_in,
+                  int32_t* table_dev, float* fold_out_dev, void* stream);
+
+/* ------------------------------------------------------------------------------------
+ * Boxcar matched-filter S/N.
+ * Replaces detection/scoring.py:99-165 boxcar_snr_1d/2d/nd (mode PYL_SCORE_PER_WIDTH,
+ * out f32 [n][nw]) and scoring.py:249-282 snr_score_batch_func (mode PYL_SCORE_MAX,
+ * out f32 [n]). layout PYL_LAYOUT_EV: folds_dev is [n][2][nbins] (e, v) and the profile
+ * is e/sqrt(v) in FP32 (ffa.py:411, scoring.py:265-266); PYL_LAYOUT_NORM: folds_dev is
+ * [n][nbins] already normalised. widths: host int64 [nw], 1 <= w < nbins, nw <= 64.
+ * ------------------------------------------------------------------------------------ */
+#define PYL_LAYOUT_EV 0
+#define PYL_LAYOUT_NORM 1
+#define PYL_SCORE_PER_WIDTH 0
+#define PYL_SCORE_MAX 1
+
+int pyl_score_boxcar(const float* folds_dev, int64_t n, int32_t nbins, int32_t layout,
+                     const int64_t* widths, int32_t nw, double stdnoise, int32_t mode,
+                     float* out_dev, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYLOKI_HIP_H */

@@ -1,0 +1,295 @@
+/*
+ * CPU oracle for the pyloki search hot path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * This file is the checker, never the thing measured or shipped: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it. The product
+ * path (pyloki_amd) calls the HIP library and fails loudly without it.
+ *
+ * It is a strict restatement of the reference's numba kernels in their operation
+ * order: no FMA contraction (-ffp-contract=off), IEEE division and sqrt, sequential
+ * sums in the reference's loop order. Each function names the reference lines it
+ * follows (paths relative to pyloki's src/pyloki/).
+ *
+ * Pinning (what checks each function beyond its own restatement) is listed in the
+ * header comment of oracle/oracle.py.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+
+#define ORC_C_VAL 299792458.0
+#define ORC_FLOAT_EPSILON 1e-6
+
+/* utils/psr_utils.py:12-46 get_phase_idx */
+double orc_phase_idx(double proper_time, double freq, int64_t nbins, double delay) {
+    double phase = (proper_time - delay) * freq;
+    phase -= floor(phase);
+    double iphase = phase * (double)nbins;
+    if (iphase >= (double)nbins) iphase = 0.0;
+    return iphase;
+}
+
+/* utils/psr_utils.py:49-56 get_phase_idx_int: int(f32(x) + f32(0.5)), nbins -> 0 */
+int64_t orc_shift_int(double iphase, int64_t nbins) {
+    float xf = (float)iphase;
+    float r = xf + 0.5f;
+    int64_t ibin = (int64_t)r;
+    if (ibin == nbins) ibin = 0;
+    return ibin;
+}
+
+int64_t orc_phase_idx_int(double proper_time, double freq, int64_t nbins, double delay) {
+    return orc_shift_int(orc_phase_idx(proper_time, freq, nbins, delay), nbins);
+}
+
+/* x**p for an integer power, as glibc pow evaluates it (correctly rounded in all
+ * but vanishingly rare cases). transforms.py:43 builds dt**powers with numba, which
+ * lowers to libm pow. Evaluated here in double-double and rounded once. */
+double orc_pow_int(double x, int p) {
+    if (p == 0) return 1.0;
+    if (p == 1) return x;
+    double hi = x, lo = 0.0;
+    for (int k = 2; k <= p; ++k) {
+        double ph = hi * x;
+        double pl = fma(hi, x, -ph);
+        pl += lo * x;
+        double s = ph + pl;
+        lo = pl - (s - ph);
+        hi = s;
+    }
+    return hi;
+}
+
+static double orc_fact(int n) {
+    double r = 1.0;
+    for (int i = 2; i <= n; ++i) r *= (double)i;
+    return r;
+}
+
+/* transforms.py:10-47 shift_taylor_params for one vector [d_k..d_0] (n entries):
+ * out[i] = sum_{j=0..n-1} vec[j] * T[i][j], T[i][j] = dt^(i-j)/(i-j)! for j<=i else 0,
+ * summed in ascending j. */
+void orc_shift_taylor_params(const double* vec, int n, double dt, double* out) {
+    for (int i = 0; i < n; ++i) {
+        double acc = 0.0;
+        for (int j = 0; j < n; ++j) {
+            double t = 0.0;
+            if (j <= i) t = orc_pow_int(dt, i - j) / orc_fact(i - j);
+            acc += vec[j] * t;
+        }
+        out[i] = acc;
+    }
+}
+
+/* utils/psr_utils.py:466-514 get_nearest_indices_analytical (one parameter) */
+int64_t orc_nearest_idx(double val, int64_t n_grid, double vmin, double vmax) {
+    if (n_grid <= 1) return 0;
+    double raw_idx = (double)n_grid * (val - vmin) / (vmax - vmin);
+    int64_t idx = (int64_t)(raw_idx + ORC_FLOAT_EPSILON);
+    if (idx < 0) idx = 0;
+    else if (idx >= n_grid) idx = n_grid - 1;
+    return idx;
+}
+
+/* core/fold.py:617-671 ffa_taylor_resolve (+ transforms.py:140-170 shift_taylor_params_d_f).
+ * pset: P values [.., accel, freq]; limits: P x 2. Writes P indices and the phase. */
+void orc_ffa_resolve(int P, const double* pset, const int64_t* counts_prev,
+                     const double* limits, int level, int latter, double tseg_brute,
+                     int64_t nbins, int64_t* idx_out, double* phase_out) {
+    double pnew[8];
+    double delta_t, delay;
+    if (P == 1) {
+        delta_t = (double)((int64_t)latter * ((int64_t)1 << (level - 1))) * tseg_brute;
+        pnew[0] = pset[0];
+        delay = 0.0;
+    } else {
+        delta_t = ((latter - 0.5) * (double)((int64_t)1 << (level - 1))) * tseg_brute;
+        double tv[8], tn[8];
+        for (int k = 0; k < P + 1; ++k) tv[k] = 0.0;
+        for (int k = 0; k < P - 1; ++k) tv[k] = pset[k];
+        orc_shift_taylor_params(tv, P + 1, delta_t, tn);
+        for (int k = 0; k < P - 1; ++k) pnew[k] = tn[k];
+        pnew[P - 1] = pset[P - 1] * (1.0 - tn[P - 1] / ORC_C_VAL);
+        delay = tn[P] / ORC_C_VAL;
+    }
+    *phase_out = orc_phase_idx(delta_t, pset[P - 1], nbins, delay);
+    for (int k = 0; k < P; ++k)
+        idx_out[k] = orc_nearest_idx(pnew[k], counts_prev[k], limits[2 * k], limits[2 * k + 1]);
+}
+
+/* core/fold.py:108-171 brutefold_start: out[nseg][nfreq][2][nbins], sequential
+ * per-bin FP32 sums in sample order. proper_time = i*tsamp - t_ref. */
+void orc_brutefold(const float* ts_e, const float* ts_v, int64_t nsamps,
+                   const double* freqs, int64_t nfreq, int64_t seg_len, int64_t nbins,
+                   double tsamp, double t_ref, float* out) {
+    int64_t nseg = (nsamps + seg_len - 1) / seg_len;
+    int32_t* pmap = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nfreq * seg_len));
+    for (int64_t f = 0; f < nfreq; ++f)
+        for (int64_t i = 0; i < seg_len; ++i) {
+            double pt = (double)i * tsamp - t_ref;
+            pmap[f * seg_len + i] = (int32_t)orc_phase_idx_int(pt, freqs[f], nbins, 0.0);
+        }
+    memset(out, 0, sizeof(float) * (size_t)(nseg * nfreq * 2 * nbins));
+    for (int64_t s = 0; s < nseg; ++s) {
+        int64_t start = s * seg_len;
+        int64_t len = nsamps - start < seg_len ? nsamps - start : seg_len;
+        for (int64_t f = 0; f < nfreq; ++f) {
+            float* o = out + (s * nfreq + f) * 2 * nbins;
+            for (int64_t i = 0; i < len; ++i) {
+                int32_t b = pmap[f * seg_len + i];
+                o[b] += ts_e[start + i];
+                o[nbins + b] += ts_v[start + i];
+            }
+        }
+    }
+    free(pmap);
+}
+
+/* core/common.py:304-324 shift_add for one (tail, head) pair of [2][nbins] rows */
+void orc_shift_add(const float* tail, const float* head, double sh_t, double sh_h,
+                   int64_t nbins, float* res) {
+    int64_t st = orc_shift_int(sh_t, nbins);
+    int64_t shh = orc_shift_int(sh_h, nbins);
+    for (int64_t j = 0; j < nbins; ++j) {
+        int64_t i1 = ((j - st) % nbins + nbins) % nbins;
+        int64_t i2 = ((j - shh) % nbins + nbins) % nbins;
+        res[j] = tail[i1] + head[i2];
+        res[nbins + j] = tail[nbins + i1] + head[nbins + i2];
+    }
+}
+
+/* dynamic/dyn_pffa.py:435-489 unify_fold. fold_in: [nseg_in][ncoords_prev][2][nbins];
+ * params: P concatenated current-level grids (lengths counts_cur); cartesian product
+ * with the last parameter fastest (np_utils.cartesian_prod_st). Optionally exports the
+ * resolved table: rows[ncoords][2] (flattened prev index of tail, head) and
+ * phases[ncoords][2]. */
+void orc_unify_fold(const float* fold_in, int64_t nseg_in, int P, const int64_t* counts_prev,
+                    const double* params, const int64_t* counts_cur, const double* limits,
+                    int level, double tseg_brute, int64_t nbins, float* fold_out,
+                    int64_t* rows_out, double* phases_out) {
+    int64_t ncur = 1, nprev = 1;
+    for (int k = 0; k < P; ++k) { ncur *= counts_cur[k]; nprev *= counts_prev[k]; }
+    int64_t offs[8];
+    offs[0] = 0;
+    for (int k = 1; k < P; ++k) offs[k] = offs[k - 1] + counts_cur[k - 1];
+    int64_t nseg_out = nseg_in / 2;
+    int64_t rowlen = 2 * nbins;
+    for (int64_t j = 0; j < ncur; ++j) {
+        double pset[8];
+        int64_t r = j;
+        for (int k = P - 1; k >= 0; --k) {
+            int64_t ik = r % counts_cur[k];
+            r /= counts_cur[k];
+            pset[k] = params[offs[k] + ik];
+        }
+        int64_t row[2];
+        double ph[2];
+        for (int latter = 0; latter < 2; ++latter) {
+            int64_t idx[8];
+            orc_ffa_resolve(P, pset, counts_prev, limits, level, latter, tseg_brute, nbins,
+                            idx, &ph[latter]);
+            int64_t flat = 0;
+            for (int k = 0; k < P; ++k) flat = flat * counts_prev[k] + idx[k];
+            row[latter] = flat;
+        }
+        if (rows_out) { rows_out[2 * j] = row[0]; rows_out[2 * j + 1] = row[1]; }
+        if (phases_out) { phases_out[2 * j] = ph[0]; phases_out[2 * j + 1] = ph[1]; }
+        for (int64_t q = 0; q < nseg_out; ++q) {
+            const float* tail = fold_in + ((2 * q) * nprev + row[0]) * rowlen;
+            const float* head = fold_in + ((2 * q + 1) * nprev + row[1]) * rowlen;
+            orc_shift_add(tail, head, ph[0], ph[1], nbins, fold_out + (q * ncur + j) * rowlen);
+        }
+    }
+}
+
+/* detection/scoring.py:42-71 circular_prefix_sum for nsum <= 2*nbins (one wrap) and
+ * the general multi-wrap form. */
+static void orc_circular_prefix_sum(const float* x, int64_t nbins, float* out, int64_t nsum) {
+    if (nbins == 0 || nsum == 0) return;
+    out[0] = x[0];
+    int64_t lim = nbins < nsum ? nbins : nsum;
+    for (int64_t i = 1; i < lim; ++i) out[i] = out[i - 1] + x[i];
+    if (nsum <= nbins) return;
+    float last_sum = out[nbins - 1];
+    int64_t n_wraps = nsum / nbins;
+    int64_t extra = nsum % nbins;
+    for (int64_t i = 1; i < n_wraps; ++i)
+        for (int64_t j = 0; j < nbins; ++j)
+            out[i * nbins + j] = (float)((double)out[j] + (double)i * (double)last_sum);
+    for (int64_t j = 0; j < extra; ++j)
+        out[n_wraps * nbins + j] = (float)((double)out[j] + (double)n_wraps * (double)last_sum);
+}
+
+/* detection/scoring.py:74-84 diff_max */
+static float orc_diff_max(const float* x, const float* y, int64_t n) {
+    float m = -FLT_MAX;
+    for (int64_t i = 0; i < n; ++i) {
+        float d = x[i] - y[i];
+        if (d > m) m = d;
+    }
+    return m;
+}
+
+/* detection/scoring.py:99-140 boxcar_snr_1d over n profiles (boxcar_snr_2d, :143-153).
+ * norm: [n][nbins]; out: [n][nw]. */
+void orc_boxcar_snr_2d(const float* norm, int64_t n, int64_t nbins, const int64_t* widths,
+                       int64_t nw, double stdnoise, float* out) {
+    int64_t wmax = 0;
+    for (int64_t k = 0; k < nw; ++k) if (widths[k] > wmax) wmax = widths[k];
+    float* ps = (float*)malloc(sizeof(float) * (size_t)(nbins + wmax));
+    double inv_std = 1.0 / stdnoise;
+    for (int64_t p = 0; p < n; ++p) {
+        const float* x = norm + p * nbins;
+        orc_circular_prefix_sum(x, nbins, ps, nbins + wmax);
+        float total = ps[nbins - 1];
+        for (int64_t k = 0; k < nw; ++k) {
+            int64_t w = widths[k];
+            float size_w = (float)(nbins - w);
+            double height = sqrt((double)size_w / (double)(nbins * w));
+            double b = (double)w * height / (double)size_w;
+            float dmax = orc_diff_max(ps + w, ps, nbins);
+            double t1 = (height + b) * (double)dmax;
+            double t2 = b * (double)total;
+            out[p * nw + k] = (float)((t1 - t2) * inv_std);
+        }
+    }
+    free(ps);
+}
+
+/* detection/scoring.py:249-282 snr_score_batch_func: folds [n][2][nbins] -> max S/N */
+void orc_snr_score_batch(const float* folds, int64_t n, int64_t nbins, const int64_t* widths,
+                         int64_t nw, float* out) {
+    int64_t wmax = 0;
+    for (int64_t k = 0; k < nw; ++k) if (widths[k] > wmax) wmax = widths[k];
+    float* ps = (float*)malloc(sizeof(float) * (size_t)(nbins + wmax));
+    float* fn = (float*)malloc(sizeof(float) * (size_t)nbins);
+    for (int64_t p = 0; p < n; ++p) {
+        const float* e = folds + p * 2 * nbins;
+        const float* v = e + nbins;
+        for (int64_t j = 0; j < nbins; ++j) fn[j] = e[j] / sqrtf(v[j]);
+        orc_circular_prefix_sum(fn, nbins, ps, nbins + wmax);
+        float total = ps[nbins - 1];
+        double max_snr = -(double)FLT_MAX;
+        for (int64_t k = 0; k < nw; ++k) {
+            int64_t w = widths[k];
+            int64_t size_w = nbins - w;
+            double height = sqrt((double)size_w / (double)(nbins * w));
+            double b = (double)w * height / (double)size_w;
+            float dmax = orc_diff_max(ps + w, ps, nbins);
+            double snr = (height + b) * (double)dmax - b * (double)total;
+            if (snr > max_snr) max_snr = snr;
+        }
+        out[p] = (float)max_snr;
+    }
+    free(fn);
+    free(ps);
+}
+
+/* ffa.py:382-411 get_fold_norm (real path): e / sqrt(v) in FP32 */
+void orc_fold_norm(const float* folds, int64_t n, int64_t nbins, float* out) {
+    for (int64_t p = 0; p < n; ++p)
+        for (int64_t j = 0; j < nbins; ++j)
+            out[p * nbins + j] = folds[p * 2 * nbins + j] / sqrtf(folds[p * 2 * nbins + nbins + j]);
+}

@@ -1,0 +1,92 @@
+"""ctypes binding of the HIP library ``libpylokihip.so`` (C ABI in include/pyloki_hip.h).
+
+This is the only route to the search kernels: there is no CPU fallback. Loading fails
+loudly when the library has not been built, and compute calls fail when no ROCm device
+is visible.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libpylokihip.so"
+
+PYL_OK = 0
+PYL_ERR_INVALID = -1
+PYL_ERR_HIP = -2
+PYL_ERR_ALLOC = -3
+
+PYL_LAYOUT_EV = 0
+PYL_LAYOUT_NORM = 1
+PYL_SCORE_PER_WIDTH = 0
+PYL_SCORE_MAX = 1
+
+_c_i32 = ctypes.c_int32
+_c_i64 = ctypes.c_int64
+_c_dbl = ctypes.c_double
+_vp = ctypes.c_void_p
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_f64p = ctypes.POINTER(ctypes.c_double)
+
+# (name, restype, argtypes) for every symbol declared in include/pyloki_hip.h.
+SIGNATURES: list[tuple[str, object, list[object]]] = [
+    ("pyl_last_error", ctypes.c_char_p, []),
+    ("pyl_abi_version", ctypes.c_int, []),
+    ("pyl_brutefold_plan_create", ctypes.c_int,
+     [_f64p, _c_i64, _c_i64, _c_i32, _c_dbl, _c_dbl, _vp, ctypes.POINTER(_vp)]),
+    ("pyl_brutefold_plan_destroy", ctypes.c_int, [_vp]),
+    ("pyl_brutefold_execute", ctypes.c_int, [_vp, _vp, _vp, _c_i64, _vp, _vp]),
+    ("pyl_ffa_resolve", ctypes.c_int,
+     [_c_i32, _vp, _i64p, _i64p, _f64p, _c_i32, _c_dbl, _c_i32, _vp, _vp, _vp]),
+    ("pyl_ffa_shift_add", ctypes.c_int, [_vp, _c_i64, _c_i64, _vp, _c_i64, _c_i32, _vp, _vp]),
+    ("pyl_ffa_level", ctypes.c_int,
+     [_c_i32, _vp, _i64p, _i64p, _f64p, _c_i32, _c_dbl, _c_i32, _vp, _c_i64, _vp, _vp, _vp]),
+    ("pyl_score_boxcar", ctypes.c_int,
+     [_vp, _c_i64, _c_i32, _c_i32, _i64p, _c_i32, _c_dbl, _c_i32, _vp, _vp]),
+]
+
+_lib: ctypes.CDLL | None = None
+
+
+class HipLibraryMissingError(RuntimeError):
+    """The HIP library is not built; there is no fallback path."""
+
+
+def load() -> ctypes.CDLL:
+    """Load the library and bind every C-ABI symbol (does not touch the GPU)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            msg = (
+                f"{LIB_PATH} is missing: build it with `python -m pyloki_amd.csrc.build` "
+                "(pyloki_amd has no CPU fallback)"
+            )
+            raise HipLibraryMissingError(msg)
+        lib = ctypes.CDLL(str(LIB_PATH))
+        for name, restype, argtypes in SIGNATURES:
+            fn = getattr(lib, name)
+            fn.restype = restype
+            fn.argtypes = argtypes
+        _lib = lib
+    return _lib
+
+
+def check(rc: int) -> None:
+    """Raise the reference's exception type for a non-zero status."""
+    if rc == PYL_OK:
+        return
+    msg = load().pyl_last_error().decode(errors="replace")
+    if rc == PYL_ERR_INVALID:
+        raise ValueError(msg)
+    raise RuntimeError(f"pyloki HIP error {rc}: {msg}")
+
+
+def i64_array(vals) -> ctypes.Array:
+    vals = [int(v) for v in vals]
+    return (ctypes.c_int64 * max(len(vals), 1))(*vals)
+
+
+def f64_array(vals) -> ctypes.Array:
+    vals = [float(v) for v in vals]
+    return (ctypes.c_double * max(len(vals), 1))(*vals)

@@ -1,0 +1,324 @@
+"""Search configuration and per-level grid planning.
+
+Mirrors ``pyloki.config`` (``src/pyloki/config.py``): the same class names, fields,
+validators, defaults and grid formulas, so that notebooks written against pyloki build
+the identical plan here. Everything in this module is host-side FP64 planning; the
+per-leaf work it plans runs in the HIP library.
+"""
+
+from __future__ import annotations
+
+import itertools
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from pyloki_amd.detection.scoring import generate_box_width_trials
+from pyloki_amd.utils import psr_utils, transforms
+from pyloki_amd.utils.misc import C_VAL, G_CONST, M_SUN
+
+
+def _is_power_of_two(n: int) -> bool:
+    return n > 0 and (n & (n - 1)) == 0
+
+
+class ParamLimits:
+    """Search parameter limits, ordered ``..., jerk, accel, freq``.
+
+    Restates ``ParamLimits`` (``config.py:59-321``) for the builders the notebooks use.
+    """
+
+    def __init__(self, limits: np.ndarray | list[tuple[float, float]]) -> None:
+        self.limits = np.asarray(limits, dtype=np.float64)
+
+    def get_cheby_limits(self, tobs: float) -> list[tuple[float, float]]:
+        """Chebyshev coefficient bounds from the Taylor corners; ``config.py:72-104``."""
+        t_s = tobs / 2
+        d_limits = self.limits[:-1][::-1]
+        d_limits = [(0, 0), (0, 0), *d_limits]
+        d_corners = np.array(list(itertools.product(*list(d_limits))))
+        alpha_corners = np.vstack(
+            [transforms.taylor_to_cheby(d_vec, t_s) for d_vec in d_corners],
+        )
+        alpha_bounds = list(
+            zip(np.min(alpha_corners, axis=0), np.max(alpha_corners, axis=0)),
+        )
+        return [tuple(self.limits[-1]), *alpha_bounds]
+
+    @classmethod
+    def from_taylor(
+        cls,
+        freq: tuple[float, float],
+        accel: tuple[float, float] | None = None,
+        jerk: tuple[float, float] | None = None,
+        snap: tuple[float, float] | None = None,
+    ) -> ParamLimits:
+        """``config.py:106-144``."""
+        default_limit = (0.0, 0.0)
+        all_params = [snap, jerk, accel, freq]
+        last_non_none = next(i for i, p in enumerate(all_params) if p is not None)
+        out = [p if p is not None else default_limit for p in all_params[last_non_none:]]
+        return cls([(float(a), float(b)) for a, b in out])
+
+    @classmethod
+    def from_upper(
+        cls,
+        freq: float | tuple[float, float],
+        true_d_params: list[float],
+        d_upper_limits: tuple[float, float],
+        t_obs: float,
+    ) -> ParamLimits:
+        """Limits from the range of the highest derivative; ``config.py:146-197``."""
+        nparams = len(true_d_params) + 1
+        dvec = np.zeros(nparams + 1, dtype=np.float64)
+        dvec[1:-2] = true_d_params[1:]
+        dvec[0] = d_upper_limits[0]
+        dvec_min_up = transforms.shift_taylor_params(dvec, t_obs / 2)
+        dvec_min_low = transforms.shift_taylor_params(dvec, -t_obs / 2)
+        dvec[0] = d_upper_limits[1]
+        dvec_max_up = transforms.shift_taylor_params(dvec, t_obs / 2)
+        dvec_max_low = transforms.shift_taylor_params(dvec, -t_obs / 2)
+        dvec_bound_low = np.minimum(dvec_min_low, dvec_max_low)
+        dvec_bound_up = np.maximum(dvec_min_up, dvec_max_up)
+        bounds_d = list(zip(dvec_bound_low, dvec_bound_up))
+        bounds = [(float(a), float(b)) for a, b in bounds_d[:-2]]
+        freq_shift = dvec_bound_up[-2] / C_VAL
+        if isinstance(freq, (tuple, list, np.ndarray)):
+            freq_min, freq_max = freq
+            bounds.append((freq_min * (1 - freq_shift), freq_max * (1 + freq_shift)))
+        else:
+            bounds.append((freq * (1 - freq_shift), freq * (1 + freq_shift)))
+        return cls(bounds)
+
+    @classmethod
+    def from_circular(
+        cls,
+        freq: float | tuple[float, float],
+        p_orb_min: float,
+        m_c_max: float = 10.0,
+        m_p_min: float = 1.2,
+        poly_order: int = 5,
+    ) -> ParamLimits:
+        """Limits from circular-orbit bounds; ``config.py:199-251``."""
+        poly_order = max(poly_order, 2)
+        omega_orb_max = 2 * np.pi / p_orb_min
+        x_const = float(
+            (M_SUN * G_CONST) ** (1 / 3) / (C_VAL * (2 * np.pi) ** (2 / 3)),
+        )
+        x_orb = (
+            x_const
+            * ((m_p_min + m_c_max) * p_orb_min**2) ** (1 / 3)
+            * m_c_max
+            / (m_p_min + m_c_max)
+        )
+        max_derivs = x_orb * C_VAL * omega_orb_max ** np.arange(poly_order + 1)
+        bounds = [(-d, d) for d in max_derivs[2:][::-1]]
+        freq_shift = max_derivs[1] / C_VAL
+        if isinstance(freq, (tuple, list, np.ndarray)):
+            freq_min, freq_max = freq
+            bounds.append((freq_min * (1 - freq_shift), freq_max * (1 + freq_shift)))
+        else:
+            bounds.append((freq * (1 - freq_shift), freq * (1 + freq_shift)))
+        return cls(bounds)
+
+
+_TILING = ("aggressive", "quadrature", "conservative")
+
+
+@dataclass(frozen=True, kw_only=True)
+class PulsarSearchConfig:
+    """Configuration of a polynomial (Taylor) search; ``config.py:324-436``.
+
+    Parameter limits are ordered ``..., jerk, accel, freq``. ``bseg_brute`` and
+    ``bseg_ffa`` default (when 0) to the reference's formulas
+    (``_bseg_brute_default``/``_bseg_ffa_default``, ``config.py:821-827``).
+    """
+
+    nsamps: int
+    tsamp: float
+    nbins: int
+    eta: float
+    param_limits: np.ndarray
+    ducy_max: float = 0.2
+    wtsp: float = 1.5
+    prune_poly_order: int = 3
+    p_orb_min: float = 0.0
+    m_c_max: float = 10.0
+    m_p_min: float = 1.4
+    bseg_brute: int = 0
+    bseg_ffa: int = 0
+    use_fourier: bool = True
+    branch_max: int = 16
+    propagator_significance: float = 2.0
+    validation_significance: float = 5.0
+    tiling_strategy: str = "aggressive"
+    _cache: dict = field(default_factory=dict, init=False, repr=False, compare=False)
+
+    def __post_init__(self) -> None:
+        lim = np.asarray(self.param_limits, dtype=np.float64)
+        if lim.ndim != 2 or lim.shape[1] != 2 or lim.shape[0] < 1:
+            msg = f"param_limits must have 1-5 elements, got {lim.shape[0] if lim.ndim else 0}"
+            raise ValueError(msg)
+        for i in range(lim.shape[0]):
+            if lim[i, 0] >= lim[i, 1]:
+                msg = f"param_limits must have min < max, got {lim}"
+                raise ValueError(msg)
+        object.__setattr__(self, "param_limits", lim)
+        if not isinstance(self.nsamps, (int, np.integer)) or self.nsamps <= 0:
+            msg = f"'nsamps' must be a positive int: {self.nsamps}"
+            raise ValueError(msg)
+        if not _is_power_of_two(int(self.nsamps)):
+            msg = f"'nsamps' must be a power of 2: {self.nsamps}"
+            raise ValueError(msg)
+        if not self.tsamp > 0:
+            msg = f"'tsamp' must be > 0: {self.tsamp}"
+            raise ValueError(msg)
+        if not isinstance(self.nbins, (int, np.integer)) or self.nbins <= 0:
+            msg = f"'nbins' must be a positive int: {self.nbins}"
+            raise ValueError(msg)
+        if not self.eta > 0:
+            msg = f"'eta' must be > 0: {self.eta}"
+            raise ValueError(msg)
+        if not self.ducy_max > 0 or not self.wtsp > 0:
+            msg = "'ducy_max' and 'wtsp' must be > 0"
+            raise ValueError(msg)
+        if self.prune_poly_order <= 1:
+            msg = f"'prune_poly_order' must be > 1: {self.prune_poly_order}"
+            raise ValueError(msg)
+        for name in ("bseg_brute", "bseg_ffa"):
+            val = int(getattr(self, name))
+            if val != 0 and not _is_power_of_two(val):
+                msg = f"'{name}' must be a power of 2: {val}"
+                raise ValueError(msg)
+        if self.branch_max <= 10:
+            msg = f"'branch_max' must be > 10: {self.branch_max}"
+            raise ValueError(msg)
+        if self.tiling_strategy not in _TILING:
+            msg = f"'tiling_strategy' must be in {_TILING}: {self.tiling_strategy}"
+            raise ValueError(msg)
+        if self.bseg_brute == 0:
+            object.__setattr__(self, "bseg_brute", self._bseg_brute_default())
+        if self.bseg_ffa == 0:
+            object.__setattr__(self, "bseg_ffa", self._bseg_ffa_default())
+        if self.bseg_brute > self.nsamps:
+            msg = f"bseg_brute ({self.bseg_brute}) must be < nsamps ({self.nsamps})"
+            raise ValueError(msg)
+        if self.bseg_ffa > self.nsamps:
+            msg = f"bseg_ffa ({self.bseg_ffa}) must be <= nsamps ({self.nsamps})"
+            raise ValueError(msg)
+        if self.bseg_ffa <= self.bseg_brute:
+            msg = f"bseg_ffa ({self.bseg_ffa}) must be > bseg_brute ({self.bseg_brute})"
+            raise ValueError(msg)
+        if self.prune_poly_order == 5 and self.p_orb_min == 0:
+            msg = "p_orb_min must be provided for a circular orbit search"
+            raise ValueError(msg)
+
+    @property
+    def tseg_brute(self) -> float:
+        return self.bseg_brute * self.tsamp
+
+    @property
+    def tseg_ffa(self) -> float:
+        return self.bseg_ffa * self.tsamp
+
+    @property
+    def niters_ffa(self) -> int:
+        return int(np.log2(self.bseg_ffa / self.bseg_brute))
+
+    @property
+    def nparams(self) -> int:
+        return self.param_limits.shape[0]
+
+    @property
+    def param_names(self) -> list[str]:
+        return ["crackle", "snap", "jerk", "accel", "freq"][-self.nparams :]
+
+    @property
+    def f_min(self) -> float:
+        return float(self.param_limits[-1][0])
+
+    @property
+    def f_max(self) -> float:
+        return float(self.param_limits[-1][1])
+
+    @property
+    def x_mass_const(self) -> float:
+        """``config.py:474-481`` (inflated by 10% for safety)."""
+        k_const = float((M_SUN * G_CONST) ** (1 / 3))
+        return k_const * 1.1 * self.m_c_max / (self.m_p_min + self.m_c_max) ** (2 / 3)
+
+    @property
+    def score_widths(self) -> np.ndarray:
+        return generate_box_width_trials(self.nbins, ducy_max=self.ducy_max, wtsp=self.wtsp)
+
+    def get_dparams_f(self, ffa_level: int) -> np.ndarray:
+        tseg_cur = 2**ffa_level * self.tseg_brute
+        t_ref = 0 if self.nparams == 1 else tseg_cur / 2
+        return psr_utils.poly_taylor_step_f(self.nparams, tseg_cur, self.nbins, self.eta, t_ref)
+
+    def get_dparams(self, ffa_level: int, use_cheby_coarsening: bool = True) -> np.ndarray:
+        """Step sizes at an FFA level; ``config.py:515-540``."""
+        tseg_cur = 2**ffa_level * self.tseg_brute
+        t_ref = 0 if self.nparams == 1 else tseg_cur / 2
+        return psr_utils.poly_taylor_step_d_f(
+            self.nparams,
+            tseg_cur,
+            self.nbins,
+            self.eta,
+            self.f_max,
+            t_ref=t_ref,
+            use_cheby=use_cheby_coarsening,
+        )
+
+    def get_dparams_actual(self, ffa_level: int, use_cheby_coarsening: bool = True) -> np.ndarray:
+        """Physically tiled spacings ``(vmax - vmin) / count``; ``config.py:542-570``."""
+        counts = self.get_param_grid_count(ffa_level, use_cheby_coarsening)
+        out = np.zeros(self.nparams, dtype=np.float64)
+        for i in range(self.nparams):
+            vmin, vmax = self.param_limits[i]
+            out[i] = (vmax - vmin) / counts[i]
+        return out
+
+    def get_param_grid_count(self, ffa_level: int, use_cheby_coarsening: bool = True) -> np.ndarray:
+        """Grid counts at an FFA level; ``config.py:572-597``."""
+        dparams = self.get_dparams(ffa_level, use_cheby_coarsening)
+        count = np.empty(self.nparams, dtype=np.int64)
+        for i in range(self.nparams):
+            count[i] = psr_utils.range_param_count(
+                float(self.param_limits[i, 0]),
+                float(self.param_limits[i, 1]),
+                float(dparams[i]),
+            )
+        return count
+
+    def get_param_arr(self, dparams: np.ndarray) -> list[np.ndarray]:
+        """Grid cell centres for given steps; ``config.py:599-634``."""
+        if len(dparams) != self.nparams:
+            msg = f"dparams must have length {self.nparams}, got {len(dparams)}"
+            raise ValueError(msg)
+        return [
+            psr_utils.range_param(
+                float(self.param_limits[i, 0]),
+                float(self.param_limits[i, 1]),
+                float(dparams[i]),
+            )
+            for i in range(self.nparams)
+        ]
+
+    def _bseg_brute_default(self) -> int:
+        init_levels = 1 if self.nparams == 1 else 5
+        levels = int(np.log2(self.nsamps * self.tsamp * self.f_min) - init_levels)
+        return int(self.nsamps / 2**levels)
+
+    def _bseg_ffa_default(self) -> int:
+        return int(self.nsamps)
+
+
+def ffa_level_counts(cfg: PulsarSearchConfig) -> list[np.ndarray]:
+    """Grid counts for every FFA level 0..niters_ffa (planning helper)."""
+    return [cfg.get_param_grid_count(level) for level in range(cfg.niters_ffa + 1)]
+
+
+def ncoords(counts: np.ndarray) -> int:
+    return int(math.prod(int(c) for c in counts))

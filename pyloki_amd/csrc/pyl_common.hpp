@@ -1,0 +1,97 @@
+// Shared helpers of the pyloki HIP library: error reporting, launch checks and the
+// FP64 phase/index math that every kernel of the search path uses.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/pyloki_hip.h"
+
+namespace pyl {
+
+constexpr double kCVal = 299792458.0;  // speed of light (utils/misc.py:37)
+constexpr double kFloatEps = 1e-6;      // index guard (utils/misc.py:39)
+
+int fail(int code, const char* fmt, ...);
+
+#define PYL_CHECK_HIP(call)                                                          \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return ::pyl::fail(PYL_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+#define PYL_CHECK_LAUNCH(name)                                                        \
+  do {                                                                                \
+    hipError_t e_ = hipGetLastError();                                                \
+    if (e_ != hipSuccess)                                                             \
+      return ::pyl::fail(PYL_ERR_HIP, "launch of %s failed: %s", name,                \
+                         hipGetErrorString(e_));                                      \
+  } while (0)
+
+#define PYL_REQUIRE(cond, ...)                                  \
+  do {                                                          \
+    if (!(cond)) return ::pyl::fail(PYL_ERR_INVALID, __VA_ARGS__); \
+  } while (0)
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// psr_utils.py:12-46 get_phase_idx: fractional bin of (t - delay) * f in [0, nbins).
+__device__ __host__ inline double phase_idx(double t, double f, int nbins, double delay) {
+  double phase = (t - delay) * f;
+  phase -= floor(phase);
+  double iphase = phase * (double)nbins;
+  if (iphase >= (double)nbins) iphase = 0.0;
+  return iphase;
+}
+
+// psr_utils.py:49-56 / common.py:313-318: int(f32(x) + 0.5f), nbins -> 0.
+__device__ __host__ inline int shift_int(double iphase, int nbins) {
+  float xf = (float)iphase;
+  float r = xf + 0.5f;
+  int ibin = (int)r;
+  if (ibin == nbins) ibin = 0;
+  return ibin;
+}
+
+// psr_utils.py:466-514: truncation of the grid position with a 1e-6 guard.
+__device__ __host__ inline int64_t nearest_idx(double val, int64_t n_grid, double vmin,
+                                               double vmax) {
+  if (n_grid <= 1) return 0;
+  double raw = (double)n_grid * (val - vmin) / (vmax - vmin);
+  int64_t idx = (int64_t)(raw + kFloatEps);
+  if (idx < 0) idx = 0;
+  else if (idx >= n_grid) idx = n_grid - 1;
+  return idx;
+}
+
+// x**p for a small integer power, rounded once from a double-double product: the value
+// libm pow returns for an integral exponent (transforms.py:43 dt**powers).
+__device__ __host__ inline double pow_int(double x, int p) {
+  if (p == 0) return 1.0;
+  if (p == 1) return x;
+  double hi = x, lo = 0.0;
+  for (int k = 2; k <= p; ++k) {
+    double ph = hi * x;
+    double pl = fma(hi, x, -ph);
+    pl += lo * x;
+    double s = ph + pl;
+    lo = pl - (s - ph);
+    hi = s;
+  }
+  return hi;
+}
+
+__device__ __host__ inline double fact_d(int n) {
+  double r = 1.0;
+  for (int i = 2; i <= n; ++i) r *= (double)i;
+  return r;
+}
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace pyl

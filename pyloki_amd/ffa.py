@@ -1,0 +1,290 @@
+"""FFA driver: leaf brute fold + dynamic-programming levels on the GPU.
+
+Mirrors ``pyloki.ffa`` (``src/pyloki/ffa.py``): ``DynamicProgramming`` keeps the
+reference's attributes and methods (``initialize``, ``execute``, ``get_fold_norm``,
+``fold``, ``param_arr``, ...). The fold lives in HBM as a torch tensor
+(``fold_device``); ``fold`` returns a host copy in the reference's shape
+``(nsegments, *param_counts, 2, nbins)``.
+
+``FFAPlan`` holds everything that does not depend on the data: the per-level grids
+(uploaded once), the brute-fold plan and the resolve tables. Series of different lengths
+whose configurations share ``bseg_brute`` and the grids (as the default planning gives for
+power-of-two lengths) share one plan and can be folded together by concatenating their
+segments (``ffa_batch``).
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from pyloki_amd import kernels
+from pyloki_amd.config import PulsarSearchConfig
+from pyloki_amd.device import require_gpu, to_device
+from pyloki_amd.io.timeseries import TimeSeries
+
+
+@dataclass
+class LevelPlan:
+    level: int
+    counts_prev: np.ndarray
+    counts_cur: np.ndarray
+    param_arr: list[np.ndarray]
+    params_dev: torch.Tensor
+    table: torch.Tensor
+    ncoords: int
+
+
+class FFAPlan:
+    """Data-independent plan of an FFA search (grids, resolve tables, brute-fold plan)."""
+
+    def __init__(self, cfg: PulsarSearchConfig, nlevels: int | None = None) -> None:
+        if cfg.use_fourier:
+            msg = "use_fourier=True (complex folds) is not supported by this FFA plan yet"
+            raise NotImplementedError(msg)
+        require_gpu()
+        self.cfg = cfg
+        self.nlevels = cfg.niters_ffa if nlevels is None else int(nlevels)
+        dparams0 = cfg.get_dparams(0)
+        self.param_arr0 = cfg.get_param_arr(dparams0)
+        if cfg.nparams > 1:
+            for i in range(cfg.nparams - 1):
+                if len(self.param_arr0[i]) > 1:
+                    msg = f"param_arr has {len(self.param_arr0[i])} values, should have only one"
+                    raise ValueError(msg)
+        self.counts0 = cfg.get_param_grid_count(0)
+        self.t_ref0 = 0.0 if cfg.nparams == 1 else cfg.bseg_brute * cfg.tsamp / 2
+        self.bf = kernels.BruteFoldPlan(self.param_arr0[-1], cfg.bseg_brute, cfg.nbins,
+                                        cfg.tsamp, self.t_ref0)
+        self.levels: list[LevelPlan] = []
+        counts_prev = self.counts0
+        for level in range(1, self.nlevels + 1):
+            pa = cfg.get_param_arr(cfg.get_dparams(level))
+            counts_cur = np.array([len(a) for a in pa], dtype=np.int64)
+            ncur = int(np.prod(counts_cur))
+            params_dev = to_device(np.concatenate(pa).astype(np.float64))
+            table = kernels.ffa_resolve(params_dev, counts_cur, counts_prev, cfg.param_limits,
+                                        level, cfg.tseg_brute, cfg.nbins)
+            self.levels.append(LevelPlan(level, counts_prev, counts_cur, pa, params_dev, table,
+                                         ncur))
+            counts_prev = counts_cur
+
+    def final_param_arr(self) -> list[np.ndarray]:
+        return self.levels[-1].param_arr if self.levels else self.param_arr0
+
+    def level_nseg(self, nsamps: int, level: int) -> int:
+        return -(-nsamps // self.cfg.bseg_brute) // (2**level)
+
+    def run(self, ts_e: torch.Tensor, ts_v: torch.Tensor, nlevels: int | None = None,
+            keep: bool = False) -> torch.Tensor | list[torch.Tensor]:
+        """Brute fold + ``nlevels`` levels; returns the last fold (or all, ``keep``)."""
+        nl = self.nlevels if nlevels is None else nlevels
+        fold = self.bf.fold(ts_e, ts_v)
+        folds = [fold] if keep else None
+        for lp in self.levels[:nl]:
+            if fold.shape[0] < 2:
+                msg = "not enough segments for another FFA level"
+                raise ValueError(msg)
+            fold = kernels.ffa_shift_add(fold, lp.table, lp.ncoords)
+            if keep:
+                folds.append(fold)
+        return folds if keep else fold
+
+
+class DynamicProgramming:
+    """FFA dynamic programming on the GPU; API of ``ffa.py:196-435``."""
+
+    def __init__(self, ts_data: TimeSeries, cfg: PulsarSearchConfig) -> None:
+        self._ts_data = ts_data
+        self._cfg = cfg
+        if cfg.use_fourier:
+            msg = "use_fourier=True (complex folds) is not supported yet; set use_fourier=False"
+            raise NotImplementedError(msg)
+        self._data_type = np.float32
+        self._plan: FFAPlan | None = None
+        self._fold_dev: torch.Tensor | None = None
+        self._fold_host: np.ndarray | None = None
+
+    @property
+    def ts_data(self) -> TimeSeries:
+        return self._ts_data
+
+    @property
+    def cfg(self) -> PulsarSearchConfig:
+        return self._cfg
+
+    @property
+    def data_type(self):
+        return self._data_type
+
+    @property
+    def fold_device(self) -> torch.Tensor:
+        """Current fold in HBM, flattened coords: ``[nseg, ncoords, 2, nbins]``."""
+        return self._fold_dev
+
+    @property
+    def fold(self) -> np.ndarray:
+        """Host copy of the fold in the reference's shape ``(nseg, *counts, 2, nbins)``."""
+        if self._fold_host is None:
+            f = self._fold_dev.cpu().numpy()
+            counts = [len(p) for p in self._param_arr]
+            self._fold_host = f.reshape(f.shape[0], *counts, *f.shape[-2:])
+        return self._fold_host
+
+    @property
+    def nsegments(self) -> int:
+        return self._fold_dev.shape[0]
+
+    @property
+    def nbins(self) -> int:
+        return self._fold_dev.shape[-1]
+
+    @property
+    def ffa_level(self) -> int:
+        return self._ffa_level
+
+    @property
+    def dparams(self) -> np.ndarray:
+        return self._dparams
+
+    @property
+    def dparams_actual(self) -> np.ndarray:
+        return self._dparams_actual
+
+    @property
+    def param_arr(self) -> list[np.ndarray]:
+        return self._param_arr
+
+    @property
+    def param_grid_count(self) -> np.ndarray:
+        return self._param_grid_count
+
+    @property
+    def param_arr_dict(self) -> dict[str, np.ndarray]:
+        names = self.cfg.param_names
+        return {f"{names[i]}": p for i, p in enumerate(self.param_arr)}
+
+    @property
+    def nparam_vol(self) -> int:
+        return int(np.prod([len(p) for p in self.param_arr])) if self.param_arr else 0
+
+    @property
+    def leaves_lb(self) -> float:
+        return np.round(np.log2(self.nparam_vol), 2)
+
+    @property
+    def tseg(self) -> float:
+        return self._tseg
+
+    def _set_fold(self, fold: torch.Tensor) -> None:
+        self._fold_dev = fold
+        self._fold_host = None
+
+    def initialize(self) -> None:
+        """Brute fold into level 0; ``ffa.py:305-329``."""
+        self._ffa_level = 0
+        self._plan = FFAPlan(self.cfg)
+        ts_e = to_device(self.ts_data.ts_e, torch.float32)
+        ts_v = to_device(self.ts_data.ts_v, torch.float32)
+        self._set_fold(self._plan.bf.fold(ts_e, ts_v))
+        self._param_arr = self._plan.param_arr0
+        self._param_grid_count = self.cfg.get_param_grid_count(0)
+        self._dparams = self.cfg.get_dparams(0)
+        self._dparams_actual = self.cfg.get_dparams_actual(0)
+        self._tseg = self.cfg.tseg_brute
+
+    def execute(self, *, show_progress: bool = False) -> None:  # noqa: ARG002
+        """Run every FFA level; ``ffa.py:331-347``."""
+        for _ in range(self.cfg.niters_ffa):
+            self._execute_iter()
+
+    def _execute_iter(self) -> None:
+        """One level; ``ffa.py:349-380``."""
+        lp = self._plan.levels[self._ffa_level]
+        self._ffa_level += 1
+        self._set_fold(kernels.ffa_shift_add(self._fold_dev, lp.table, lp.ncoords))
+        self._param_arr = lp.param_arr
+        self._param_grid_count = self.cfg.get_param_grid_count(self._ffa_level)
+        self._dparams = self.cfg.get_dparams(self._ffa_level)
+        self._dparams_actual = self.cfg.get_dparams_actual(self._ffa_level)
+        self._tseg *= 2
+
+    def get_fold_norm(self, iseg: int = 0, param_idx: tuple | None = None) -> np.ndarray:
+        """``e / sqrt(v)`` of one segment (host); ``ffa.py:382-411``."""
+        fold = self.fold
+        sel = fold[iseg] if param_idx is None else fold[(iseg, *param_idx)]
+        return sel[..., 0, :] / np.sqrt(sel[..., 1, :])
+
+    def get_complexity(self) -> list[int]:
+        out = []
+        for level in range(self.cfg.niters_ffa + 1):
+            dparams = self.cfg.get_dparams(level)
+            out.append(int(np.prod([len(p) for p in self.cfg.get_param_arr(dparams)])))
+        return out
+
+
+def compute_ffa(tseries: TimeSeries, search_cfg: PulsarSearchConfig, *, quiet: bool = False,
+                show_progress: bool = False) -> np.ndarray:  # noqa: ARG001
+    """``ffa.py:438-467``."""
+    dyp = DynamicProgramming(tseries, search_cfg)
+    dyp.initialize()
+    dyp.execute(show_progress=show_progress)
+    return dyp.fold
+
+
+def ffa_batch(plan: FFAPlan, series: list[tuple[torch.Tensor, torch.Tensor]],
+              widths: np.ndarray, stdnoise: float = 1.0) -> list[torch.Tensor]:
+    """Fold and score many series of (possibly) different lengths in one pass per level.
+
+    All series share ``plan`` (same ``bseg_brute`` and grids); series are ordered longest
+    first and concatenated along the segment axis, so level L works on every series that
+    still has more than one segment in one launch. A series that reaches a single segment
+    is scored there (``ffa_search``'s per-width S/N grid, ``search.py:46-47``). Returns the
+    S/N grids ``[ncoords_final(series), n_w]`` in the input order.
+    """
+    nser = len(series)
+    lens = [int(e.shape[0]) for e, _ in series]
+    order = sorted(range(nser), key=lambda i: -lens[i])
+    for i in range(nser):
+        if lens[i] % plan.cfg.bseg_brute:
+            msg = "series length must be a multiple of bseg_brute"
+            raise ValueError(msg)
+    nseg0 = [lens[i] // plan.cfg.bseg_brute for i in order]
+    for n in nseg0:
+        if n & (n - 1):
+            msg = "segments per series must be a power of two"
+            raise ValueError(msg)
+    if int(math.log2(max(nseg0))) > plan.nlevels:
+        msg = "plan has fewer levels than the longest series needs"
+        raise ValueError(msg)
+    total_seg = sum(nseg0)
+    dev = series[0][0].device
+    fold = torch.empty((total_seg, plan.bf.nfreq, 2, plan.cfg.nbins), dtype=torch.float32,
+                       device=dev)
+    off = 0
+    for i, n in zip(order, nseg0):
+        plan.bf.fold(series[i][0], series[i][1], out=fold[off:off + n])
+        off += n
+    results: dict[int, torch.Tensor] = {}
+    nseg = list(nseg0)
+    level = 0
+    while True:
+        # Series whose fold is down to one segment are finished: score them.
+        active = [k for k in range(nser) if nseg[k] > 1]
+        base = sum(nseg[k] for k in active)
+        off = base
+        for k in range(nser):
+            if nseg[k] == 1 and order[k] not in results:
+                results[order[k]] = kernels.score_boxcar(fold[off], widths, stdnoise)
+                off += 1
+        if not active:
+            break
+        lp = plan.levels[level]
+        fold = kernels.ffa_shift_add(fold[:base], lp.table, lp.ncoords)
+        for k in active:
+            nseg[k] //= 2
+        level += 1
+    return [results[i] for i in range(nser)]

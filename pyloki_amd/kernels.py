@@ -1,0 +1,143 @@
+"""Thin Python wrappers over the C ABI that take and return device tensors.
+
+Each wrapper validates shapes on the host before launching, so that a kernel never runs
+on operands it was not written for.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from pyloki_amd import _lib
+from pyloki_amd.device import ptr, require_gpu, stream_ptr
+
+
+class BruteFoldPlan:
+    """Device plan of the leaf brute fold for one (freqs, seg_len, nbins, tsamp, t_ref).
+
+    Replaces the per-call phase map of ``brutefold_start`` (``fold.py:150-158``): the map
+    is resolved once on the GPU and reused for every series folded with this plan.
+    """
+
+    def __init__(self, freqs: np.ndarray, seg_len: int, nbins: int, tsamp: float,
+                 t_ref: float) -> None:
+        require_gpu()
+        lib = _lib.load()
+        freqs = np.ascontiguousarray(freqs, dtype=np.float64)
+        self.freqs = freqs
+        self.nfreq = len(freqs)
+        self.seg_len = int(seg_len)
+        self.nbins = int(nbins)
+        self.tsamp = float(tsamp)
+        self.t_ref = float(t_ref)
+        handle = ctypes.c_void_p()
+        _lib.check(lib.pyl_brutefold_plan_create(
+            freqs.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), self.nfreq, self.seg_len,
+            self.nbins, self.tsamp, self.t_ref, stream_ptr(), ctypes.byref(handle)))
+        self._handle = handle
+
+    def fold(self, ts_e: torch.Tensor, ts_v: torch.Tensor,
+             out: torch.Tensor | None = None) -> torch.Tensor:
+        if ts_e.shape != ts_v.shape or ts_e.dim() != 1:
+            msg = "ts_e, ts_v must be 1-D and of the same length."
+            raise ValueError(msg)
+        if ts_e.dtype != torch.float32 or ts_v.dtype != torch.float32:
+            msg = "ts_e, ts_v must be float32"
+            raise ValueError(msg)
+        nsamps = ts_e.shape[0]
+        nseg = -(-nsamps // self.seg_len)
+        shape = (nseg, self.nfreq, 2, self.nbins)
+        if out is None:
+            out = torch.empty(shape, dtype=torch.float32, device=ts_e.device)
+        elif tuple(out.shape) != shape or out.dtype != torch.float32:
+            msg = f"out must be float32 of shape {shape}"
+            raise ValueError(msg)
+        _lib.check(_lib.load().pyl_brutefold_execute(
+            self._handle, ptr(ts_e), ptr(ts_v), nsamps, ptr(out), stream_ptr()))
+        return out
+
+    def __del__(self) -> None:
+        h = getattr(self, "_handle", None)
+        if h is not None and h.value:
+            try:
+                _lib.load().pyl_brutefold_plan_destroy(h)
+            except Exception:  # noqa: BLE001 - interpreter shutdown
+                pass
+            self._handle = None
+
+
+def ffa_resolve(params_dev: torch.Tensor, counts_cur, counts_prev, param_limits: np.ndarray,
+                level: int, tseg_brute: float, nbins: int, table: torch.Tensor | None = None,
+                phases: torch.Tensor | None = None) -> torch.Tensor:
+    """FP64 resolve of one FFA level -> int32 table [ncoords][4]."""
+    P = len(counts_cur)
+    ncur = int(np.prod([int(c) for c in counts_cur]))
+    if params_dev.dtype != torch.float64 or params_dev.numel() != int(sum(counts_cur)):
+        msg = "params_dev must be float64 with sum(counts_cur) entries"
+        raise ValueError(msg)
+    if table is None:
+        table = torch.empty((ncur, 4), dtype=torch.int32, device=params_dev.device)
+    if tuple(table.shape) != (ncur, 4) or table.dtype != torch.int32:
+        msg = "table must be int32 [ncoords, 4]"
+        raise ValueError(msg)
+    if phases is not None and (tuple(phases.shape) != (ncur, 2) or phases.dtype != torch.float64):
+        msg = "phases must be float64 [ncoords, 2]"
+        raise ValueError(msg)
+    lim = np.ascontiguousarray(param_limits, dtype=np.float64).reshape(-1)
+    _lib.check(_lib.load().pyl_ffa_resolve(
+        P, ptr(params_dev), _lib.i64_array(counts_cur), _lib.i64_array(counts_prev),
+        _lib.f64_array(lim), level, tseg_brute, nbins, ptr(table), ptr(phases), stream_ptr()))
+    return table
+
+
+def ffa_shift_add(fold_in: torch.Tensor, table: torch.Tensor, ncoords_cur: int,
+                  out: torch.Tensor | None = None) -> torch.Tensor:
+    """Segment-pair shift-add of one level: [nseg, nprev, 2, B] -> [nseg/2, ncur, 2, B]."""
+    if fold_in.dim() != 4 or fold_in.shape[2] != 2 or fold_in.dtype != torch.float32:
+        msg = "fold_in must be float32 [nseg, ncoords, 2, nbins]"
+        raise ValueError(msg)
+    nseg, nprev, _, nbins = fold_in.shape
+    if nseg % 2:
+        msg = f"nsegments must be even, got {nseg}"
+        raise ValueError(msg)
+    if tuple(table.shape) != (ncoords_cur, 4):
+        msg = "table does not match ncoords_cur"
+        raise ValueError(msg)
+    shape = (nseg // 2, ncoords_cur, 2, nbins)
+    if out is None:
+        out = torch.empty(shape, dtype=torch.float32, device=fold_in.device)
+    elif tuple(out.shape) != shape:
+        msg = f"out must have shape {shape}"
+        raise ValueError(msg)
+    _lib.check(_lib.load().pyl_ffa_shift_add(
+        ptr(fold_in), nseg, nprev, ptr(table), ncoords_cur, nbins, ptr(out), stream_ptr()))
+    return out
+
+
+def score_boxcar(folds: torch.Tensor, widths: np.ndarray, stdnoise: float = 1.0,
+                 layout: int = _lib.PYL_LAYOUT_EV, mode: int = _lib.PYL_SCORE_PER_WIDTH,
+                 out: torch.Tensor | None = None) -> torch.Tensor:
+    """Boxcar S/N of profiles: EV rows [..., 2, B] or normalised rows [..., B]."""
+    widths = np.ascontiguousarray(widths, dtype=np.int64)
+    nbins = folds.shape[-1]
+    if layout == _lib.PYL_LAYOUT_EV:
+        if folds.dim() < 2 or folds.shape[-2] != 2:
+            msg = "EV layout needs [..., 2, nbins]"
+            raise ValueError(msg)
+        lead = tuple(folds.shape[:-2])
+    else:
+        lead = tuple(folds.shape[:-1])
+    n = int(np.prod(lead)) if lead else 1
+    if folds.dtype != torch.float32:
+        msg = "folds must be float32"
+        raise ValueError(msg)
+    shape = (*lead, len(widths)) if mode == _lib.PYL_SCORE_PER_WIDTH else lead
+    if out is None:
+        out = torch.empty(shape, dtype=torch.float32, device=folds.device)
+    _lib.check(_lib.load().pyl_score_boxcar(
+        ptr(folds), n, nbins, layout, _lib.i64_array(widths), len(widths), float(stdnoise), mode,
+        ptr(out), stream_ptr()))
+    return out

@@ -1,0 +1,197 @@
+"""GPU parity of the FFA path (brute fold, resolve, shift-add, scoring) against the oracle.
+
+Integer/index results and FP32 folds are compared bit-exactly; S/N is compared bit-exactly
+as well (the tolerance the path promises is 1e-5 relative; the kernels reproduce the
+oracle's operation order, so equality is expected and asserted).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as orc
+from pyloki_amd import _lib, kernels
+from pyloki_amd.config import ParamLimits, PulsarSearchConfig
+from pyloki_amd.ffa import DynamicProgramming, FFAPlan, ffa_batch
+from pyloki_amd.io.timeseries import TimeSeries
+from pyloki_amd.search import ffa_search
+from pyloki_amd.simulation.pulse import PulseSignalConfig
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(x: np.ndarray) -> torch.Tensor:
+    return torch.from_numpy(np.ascontiguousarray(x)).cuda()
+
+
+def _case(msg: str) -> None:
+    print(f"[case] {msg}", flush=True)
+
+
+@pytest.mark.parametrize("nbins,seg_len,nfreq,t_ref_half,nsamps", [
+    (64, 8192, 37, False, 8192 * 6),          # LDS-staged segment, t_ref = 0
+    (64, 4096, 84, True, 4096 * 4 + 1000),    # partial last segment, t_ref = T/2
+    (32, 16384, 12, False, 16384 * 3),        # segment too large for LDS (global reads)
+    (100, 2048, 9, True, 2048 * 5),           # nbins not a power of two
+    (128, 1024, 20, False, 1024 * 8),
+])
+def test_brutefold_bit_exact(gpu_device, nbins, seg_len, nfreq, t_ref_half, nsamps):
+    _case(f"brutefold nbins={nbins} seg_len={seg_len} nfreq={nfreq} nsamps={nsamps}")
+    rng = np.random.default_rng(nbins + seg_len)
+    ts_e = rng.normal(size=nsamps).astype(np.float32)
+    ts_v = rng.uniform(0.5, 1.5, size=nsamps).astype(np.float32)
+    tsamp = 64e-6 if seg_len != 16384 else 256e-6
+    freqs = np.sort(rng.uniform(150.0, 210.0, size=nfreq)) if tsamp < 1e-4 else \
+        np.linspace(1.5, 2.5, nfreq)
+    t_ref = seg_len * tsamp / 2 if t_ref_half else 0.0
+    plan = kernels.BruteFoldPlan(freqs, seg_len, nbins, tsamp, t_ref)
+    got = plan.fold(_dev(ts_e), _dev(ts_v)).cpu().numpy()
+    exp = orc.brutefold(ts_e, ts_v, freqs, seg_len, nbins, tsamp, t_ref)
+    np.testing.assert_array_equal(got, exp)
+
+
+def _cfg(P: int, nsamps: int, nbins: int, bseg: int) -> PulsarSearchConfig:
+    if P == 1:
+        lim = ParamLimits.from_taylor((195.0, 205.0)).limits
+    elif P == 2:
+        lim = ParamLimits.from_taylor((195.0, 205.0), accel=(-50.0, 50.0)).limits
+    else:
+        lim = ParamLimits.from_taylor((495.0, 505.0), accel=(-30.0, 30.0), jerk=(-2.0, 2.0)).limits
+    return PulsarSearchConfig(nsamps=nsamps, tsamp=64e-6, nbins=nbins, eta=1.0, param_limits=lim,
+                              bseg_brute=bseg, use_fourier=False)
+
+
+@pytest.mark.parametrize("P", [1, 2, 3])
+def test_resolve_table_bit_exact(gpu_device, P):
+    _case(f"resolve P={P}")
+    cfg = _cfg(P, 2**20, 64, 2048)
+    counts_prev = cfg.get_param_grid_count(0)
+    for level in range(1, cfg.niters_ffa + 1):
+        pa = cfg.get_param_arr(cfg.get_dparams(level))
+        counts = np.array([len(a) for a in pa])
+        params = _dev(np.concatenate(pa))
+        phases = torch.empty((int(np.prod(counts)), 2), dtype=torch.float64, device="cuda")
+        table = kernels.ffa_resolve(params, counts, counts_prev, cfg.param_limits, level,
+                                    cfg.tseg_brute, cfg.nbins, phases=phases).cpu().numpy()
+        dummy = np.zeros((2, int(np.prod(counts_prev)), 2, cfg.nbins), dtype=np.float32)
+        _, rows, ph = orc.unify_fold(dummy, counts_prev, pa, cfg.param_limits, level,
+                                     cfg.tseg_brute, cfg.nbins, return_table=True)
+        np.testing.assert_array_equal(table[:, 0], rows[:, 0])
+        np.testing.assert_array_equal(table[:, 2], rows[:, 1])
+        np.testing.assert_array_equal(phases.cpu().numpy(), ph)
+        sh = np.vectorize(lambda x: orc.shift_int(x, cfg.nbins))(ph)
+        np.testing.assert_array_equal(table[:, 1], sh[:, 0])
+        np.testing.assert_array_equal(table[:, 3], sh[:, 1])
+        counts_prev = counts
+
+
+@pytest.mark.parametrize("P,nbins,nsamps,bseg", [
+    (1, 64, 2**18, 4096),
+    (2, 64, 2**19, 4096),
+    (3, 64, 2**19, 4096),
+    (2, 32, 2**18, 2048),
+    (2, 48, 2**17, 2048),   # generic (non power-of-two) shift-add kernel
+    (1, 128, 2**18, 4096),
+])
+def test_ffa_all_levels_bit_exact(gpu_device, P, nbins, nsamps, bseg):
+    _case(f"ffa levels P={P} nbins={nbins} nsamps={nsamps}")
+    cfg = _cfg(P, nsamps, nbins, bseg)
+    rng = np.random.default_rng(P * 1000 + nbins)
+    ts_e = rng.normal(size=nsamps).astype(np.float32)
+    ts_v = np.ones(nsamps, dtype=np.float32)
+    plan = FFAPlan(cfg)
+    folds = plan.run(_dev(ts_e), _dev(ts_v), keep=True)
+    exp, _ = orc.ffa_fold(ts_e, ts_v, cfg, levels=0)
+    np.testing.assert_array_equal(folds[0].cpu().numpy(), exp)
+    counts_prev = cfg.get_param_grid_count(0)
+    for level in range(1, cfg.niters_ffa + 1):
+        pa = cfg.get_param_arr(cfg.get_dparams(level))
+        exp = orc.unify_fold(folds[level - 1].cpu().numpy(), counts_prev, pa, cfg.param_limits,
+                             level, cfg.tseg_brute, cfg.nbins)
+        np.testing.assert_array_equal(folds[level].cpu().numpy(), exp)
+        counts_prev = cfg.get_param_grid_count(level)
+
+
+@pytest.mark.parametrize("nbins,n", [(64, 1000), (64, 64), (32, 7), (100, 130), (128, 300)])
+def test_score_bit_exact(gpu_device, nbins, n):
+    _case(f"score nbins={nbins} n={n}")
+    rng = np.random.default_rng(n)
+    e = rng.normal(size=(n, nbins)).astype(np.float32) * 5
+    v = rng.uniform(0.5, 50.0, size=(n, nbins)).astype(np.float32)
+    ev = np.ascontiguousarray(np.stack([e, v], axis=1))
+    widths = np.array([1, 2, 3, 5, 8, 13, nbins // 2 - 1], dtype=np.int64)
+    got = kernels.score_boxcar(_dev(ev), widths).cpu().numpy()
+    exp = orc.boxcar_snr(orc.fold_norm(ev), widths)
+    np.testing.assert_array_equal(got, exp)
+    got_max = kernels.score_boxcar(_dev(ev), widths, mode=_lib.PYL_SCORE_MAX).cpu().numpy()
+    np.testing.assert_array_equal(got_max, orc.snr_score_batch(ev, widths))
+    norm = orc.fold_norm(ev)
+    got_n = kernels.score_boxcar(_dev(norm), widths, 1.0, layout=_lib.PYL_LAYOUT_NORM)
+    np.testing.assert_array_equal(got_n.cpu().numpy(), exp)
+    np.testing.assert_allclose(got, exp, rtol=1e-5)
+
+
+def test_score_empty_and_invalid(gpu_device):
+    _case("score empty / invalid")
+    empty = torch.empty((0, 2, 64), dtype=torch.float32, device="cuda")
+    assert kernels.score_boxcar(empty, np.array([1, 2])).shape == (0, 2)
+    with pytest.raises(ValueError, match="width"):
+        kernels.score_boxcar(torch.ones((3, 2, 64), device="cuda"), np.array([64]))
+
+
+def test_ffa_search_config1_full_size(gpu_device):
+    """Plain FFA frequency search at its full size: 2^21 samples, tsamp 256 us, nbins 64,
+    f in [1.5, 2.5] Hz, widths 1..32 -- folds and the S/N grid equal the oracle."""
+    _case("ffa_search config 1 full size")
+    nsamps, tsamp = 2**21, 256e-6
+    sim = PulseSignalConfig(period=0.5, dt=tsamp, nsamps=nsamps, snr=15.0, ducy=0.05)
+    ts = sim.generate(seed=11)
+    cfg = PulsarSearchConfig(nsamps=nsamps, tsamp=tsamp, nbins=64, eta=1.0,
+                             param_limits=np.array([[1.5, 2.5]]), ducy_max=0.5, wtsp=2.0,
+                             use_fourier=False)
+    dyp, pgram = ffa_search(ts, cfg)
+    exp = orc.ffa_search_snr(ts.ts_e, ts.ts_v, cfg)
+    np.testing.assert_array_equal(pgram.snrs, exp)
+    best = pgram.find_best_params()
+    assert abs(best["freq"] - 2.0) < 1e-3 and best["snr"] > 10
+    fold_exp, _ = orc.ffa_fold(ts.ts_e, ts.ts_v, cfg)
+    np.testing.assert_array_equal(dyp.fold.reshape(fold_exp.shape), fold_exp)
+
+
+def test_dynamic_programming_api(gpu_device):
+    _case("DynamicProgramming API")
+    nsamps = 2**18
+    ts = TimeSeries(np.random.default_rng(5).normal(size=nsamps), np.ones(nsamps), 64e-6)
+    cfg = _cfg(2, nsamps, 64, 4096)
+    dyp = DynamicProgramming(ts, cfg)
+    dyp.initialize()
+    assert dyp.fold.shape == (nsamps // 4096, 1, len(dyp.param_arr[-1]), 2, 64)
+    dyp.execute()
+    counts = [len(p) for p in dyp.param_arr]
+    assert dyp.fold.shape == (1, *counts, 2, 64)
+    assert dyp.ffa_level == cfg.niters_ffa
+    np.testing.assert_allclose(dyp.tseg, cfg.tseg_ffa)
+    norm = dyp.get_fold_norm()
+    assert norm.shape == (*counts, 64)
+
+
+def test_ffa_batch_matches_single_series(gpu_device):
+    _case("ffa_batch mixed lengths")
+    cfg = _cfg(2, 2**19, 64, 4096)
+    plan = FFAPlan(cfg)
+    rng = np.random.default_rng(9)
+    lens = [2**17, 2**19, 2**18, 2**17]
+    series = []
+    for n in lens:
+        e = rng.normal(size=n).astype(np.float32)
+        series.append((_dev(e), _dev(np.ones(n, dtype=np.float32))))
+    widths = cfg.score_widths
+    out = ffa_batch(plan, series, widths)
+    for (e, v), snr, n in zip(series, out, lens):
+        sub = PulsarSearchConfig(nsamps=n, tsamp=cfg.tsamp, nbins=64, eta=1.0,
+                                 param_limits=cfg.param_limits, bseg_brute=4096,
+                                 use_fourier=False)
+        exp = orc.ffa_search_snr(e.cpu().numpy(), v.cpu().numpy(), sub)
+        np.testing.assert_array_equal(snr.cpu().numpy(), exp.reshape(-1, len(widths)))

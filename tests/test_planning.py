@@ -1,0 +1,126 @@
+"""Host planning (grid math, limits, widths, config validation) against the reference's
+notebook golden values and closed forms. CPU only."""
+
+from __future__ import annotations
+
+import json
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from pyloki_amd.config import ParamLimits, PulsarSearchConfig
+from pyloki_amd.detection.scoring import generate_box_width_trials
+from pyloki_amd.periodogram import find_nearest_sorted_idx
+from pyloki_amd.utils import psr_utils
+
+GOLD = json.loads((Path(__file__).parent / "golden" / "notebook_goldens.json").read_text())
+
+
+def test_ffa_freq_final_grid_bit_exact():
+    g = GOLD["ffa_freq"]
+    cfg = PulsarSearchConfig(
+        nsamps=g["nsamps"], tsamp=g["tsamp"], nbins=g["nbins"], eta=g["eta"],
+        param_limits=np.array([(1 / 0.0075, 1 / 0.0065)]), bseg_brute=g["bseg_brute"],
+        ducy_max=g["ducy_max"], wtsp=g["wtsp"], use_fourier=False,
+    )
+    freqs = cfg.get_param_arr(cfg.get_dparams(cfg.niters_ffa))[0]
+    assert freqs[g["best_freq_index"]] == g["best_freq"]  # exact float64
+    assert find_nearest_sorted_idx(freqs, 1 / 0.007) == g["true_param_index"]
+
+
+@pytest.mark.parametrize("name", ["ep_accel", "ep_jerk"])
+def test_ep_fold_dims_and_grid_sizes(name):
+    g = GOLD[name]
+    fu = g["from_upper"]
+    tobs = g["nsamps"] * g["tsamp"]
+    p = ParamLimits.from_upper(tuple(fu["freq"]), fu["true_d_params"],
+                               tuple(fu["d_upper_limits"]), tobs)
+    # printed in 9-significant-digit scientific notation
+    exp = np.array(g["param_limits_8dp"])
+    assert np.all(np.abs(p.limits - exp) <= 5.1e-9 * np.maximum(1.0, np.abs(exp)))
+    cfg = PulsarSearchConfig(
+        nsamps=g["nsamps"], tsamp=g["tsamp"], nbins=g["nbins"], eta=g["eta"],
+        param_limits=p.limits, bseg_brute=g["bseg_brute"], bseg_ffa=g["bseg_ffa"],
+        prune_poly_order=g["prune_poly_order"], ducy_max=g["ducy_max"], wtsp=g["wtsp"],
+        branch_max=g["branch_max"],
+    )
+    assert cfg.niters_ffa + 1 == len(g["fold_dims"])
+    for level, dims in enumerate(g["fold_dims"]):
+        counts = cfg.get_param_grid_count(level)
+        nseg = g["nsamps"] // (g["bseg_brute"] * 2**level)
+        assert [nseg, *counts.tolist()] == dims[:-2]
+        vol = int(np.prod(counts))
+        assert round(math.log2(vol), 2) == pytest.approx(g["leaves_log2"][level], abs=1e-9)
+    np.testing.assert_allclose(cfg.get_dparams(0), g["grid_sizes_level0"], rtol=5e-9)
+    np.testing.assert_allclose(cfg.get_dparams(cfg.niters_ffa), g["grid_sizes_final"], rtol=5e-9)
+
+
+@pytest.mark.parametrize("name", ["ffa_accel_limits", "ffa_jerk_limits"])
+def test_from_upper_limits(name):
+    g = GOLD[name]
+    fu = g["from_upper"]
+    p = ParamLimits.from_upper(fu["freq"], fu["true_d_params"], tuple(fu["d_upper_limits"]),
+                               fu["t_obs"])
+    exp = np.array(g["param_limits_8dp"])
+    # numpy printed these with 8 decimals
+    assert np.all(np.abs(p.limits - exp) <= 5.1e-9)
+
+
+def test_range_param_tiles_exactly():
+    for vmin, vmax, dv in [(1.5, 2.5, 0.013), (-50.0, 50.0, 7.0), (195.0, 205.0, 0.119),
+                           (0.0, 1.0, 2.0)]:
+        n = psr_utils.range_param_count(vmin, vmax, dv)
+        arr = psr_utils.range_param(vmin, vmax, dv)
+        assert len(arr) == n
+        if n > 1:
+            dva = (vmax - vmin) / n
+            assert arr[0] == vmin + dva / 2 and arr[-1] == vmax - dva / 2
+            np.testing.assert_allclose(np.diff(arr), dva, rtol=1e-9)
+        assert n == (1 if dv >= vmax - vmin else math.ceil((vmax - vmin) / dv))
+
+
+def test_step_closed_form():
+    # dphi * (k+1)! / (T - t_ref)^(k+1) * 2^k, highest derivative first.
+    out = psr_utils.poly_taylor_step_f(3, 10.0, 64, 1.0, t_ref=5.0)
+    dphi = 1 / 64
+    exp = [dphi * 6 / 5.0**3 * 4, dphi * 2 / 5.0**2 * 2, dphi / 5.0]
+    np.testing.assert_array_equal(out, np.array(exp))
+
+
+def test_score_widths():
+    assert generate_box_width_trials(64, 0.5, 2.0).tolist() == [1, 2, 4, 8, 16, 32]
+    assert generate_box_width_trials(64, 0.2, 1.5).tolist() == [1, 2, 3, 4, 6, 9]
+    assert generate_box_width_trials(64, 0.5, 1.2).tolist() == [
+        1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 14, 16, 19, 22, 26, 31]
+
+
+def test_bseg_brute_default():
+    cfg = PulsarSearchConfig(nsamps=2**21, tsamp=256e-6, nbins=64, eta=1,
+                             param_limits=np.array([(1.5, 2.5)]), use_fourier=False)
+    assert cfg.bseg_brute == 8192 and cfg.niters_ffa == 8
+    cfg2 = PulsarSearchConfig(nsamps=2**23, tsamp=64e-6, nbins=64, eta=1,
+                              param_limits=np.array([(-50.0, 50.0), (195.0, 205.0)]),
+                              use_fourier=False)
+    assert cfg2.bseg_brute == 4096 and cfg2.niters_ffa == 11
+
+
+@pytest.mark.parametrize("kwargs,msg", [
+    ({"nsamps": 1000}, "power of 2"),
+    ({"bseg_brute": 3000}, "power of 2"),
+    ({"bseg_ffa": 1024, "bseg_brute": 2048}, "must be > bseg_brute"),
+    ({"prune_poly_order": 5}, "p_orb_min must be provided"),
+    ({"branch_max": 8}, "branch_max"),
+    ({"param_limits": np.array([(2.5, 1.5)])}, "min < max"),
+])
+def test_config_validation_errors(kwargs, msg):
+    base = dict(nsamps=2**16, tsamp=1e-3, nbins=64, eta=1.0, param_limits=np.array([(1.5, 2.5)]))
+    base.update(kwargs)
+    with pytest.raises(ValueError, match=msg):
+        PulsarSearchConfig(**base)
+
+
+def test_range_param_invalid():
+    with pytest.raises(ValueError, match="vmin < vmax"):
+        psr_utils.range_param(1.0, 1.0, 0.1)
