@@ -100,6 +100,97 @@ int pyl_score_boxcar(const float* folds_dev, int64_t n, int32_t nbins, int32_t l
                      const int64_t* widths, int32_t nw, double stdnoise, int32_t mode,
                      float* out_dev, void* stream);
 
+/* ------------------------------------------------------------------------------------
+ * EP (enumerate-and-prune) Taylor search, real folds, moving grid.
+ * One pruning level of prune.py:165-332 pruning_iteration_batched is three calls:
+ *   pyl_prune_branch      core/taylor.py:68-169 poly_taylor_branch_batch: per parent and
+ *                         parameter the child count, first child centre and step
+ *                         (psr_utils.py:347-379), child counts per parent;
+ *   pyl_prune_children    for every child (parent-major, odometer order of
+ *                         np_utils.py:174-238): resolve (taylor.py:172-241, FP64),
+ *                         load + shift-add (common.py:375-400) and score
+ *                         (scoring.py:249-282) -> per-child score, segment row, shift,
+ *                         phase;
+ *   pyl_prune_materialize for the surviving children: transform to coord_next
+ *                         (taylor.py:323-339), fold, score and backtrack
+ *                         (prune.py:302-314) of the new tree.
+ * pyl_prune_ascend replaces dyn_poly_taylor.py:587-628 ascend_func (re-integration of
+ * every survivor over all segments so far, taylor.py:280-320, common.py:432-459).
+ * Which children survive (threshold and the WorldTree overflow rule of
+ * world_tree.py:482-551) is decided between calls 2 and 3 by the caller.
+ *
+ * Layouts: leaves f64 [n][P+2][2]; folds f32 [n][2][nbins]; seg_fold f32 rows of the FFA
+ * fold of one segment, row = accel_idx * n_freq + freq_idx (leading axes at index 0,
+ * common.py:178-235); backtracks int32 [n][P+2].
+ * ------------------------------------------------------------------------------------ */
+#define PYL_TILING_AGGRESSIVE 0
+#define PYL_TILING_QUADRATURE 1
+#define PYL_TILING_CONSERVATIVE 2
+#define PYL_MAX_WIDTHS 64
+
+typedef struct pyl_prune_params {
+  int32_t poly_order;     /* P: Taylor parameters d_P..d_1 branched; leaf rows P + 2 */
+  int32_t nbins;
+  int32_t branch_max;
+  int32_t tiling;         /* PYL_TILING_* */
+  double eta;
+  int32_t nparams_ffa;    /* FFA grid parameters (backtrack width = P + 2) */
+  int32_t n_accel;        /* FFA grid count of the acceleration axis */
+  int32_t n_freq;         /* FFA grid count of the frequency axis */
+  int32_t nw;             /* number of boxcar widths */
+  double accel_lim[2];
+  double freq_lim[2];
+  int32_t widths[PYL_MAX_WIDTHS];
+} pyl_prune_params;
+
+typedef struct pyl_prune_coords {
+  double t0_init;         /* coord_init[0] */
+  double t0_cur;          /* coord_cur[0] */
+  double scale_cur;       /* coord_cur[1] */
+  double t0_next;         /* coord_next[0] */
+  double t0_add;          /* coord_add[0] */
+} pyl_prune_coords;
+
+/* branch_dev: f64 [n_parents][P][2] = {first child centre, child step}; counts_dev int32
+ * [n_parents][P] = children per parameter, or -1 when the parameter is not split (the
+ * child keeps the parent value); nchild_dev int64 [n_parents]. err_dev (int64, one word,
+ * set to INT64_MAX by the caller) receives the smallest (parent * P + param) * 4 + kind
+ * over failing parameters: kind 1 = "increase branch_max", kind 2 = non-positive step
+ * (psr_utils.py:358-365). */
+int pyl_prune_branch(const pyl_prune_params* p, const pyl_prune_coords* c,
+                     const double* leaves_dev, int64_t n_parents, double* branch_dev,
+                     int32_t* counts_dev, int64_t* nchild_dev, int64_t* err_dev, void* stream);
+
+/* child_off_dev: int64 [n_parents + 1] exclusive prefix of nchild. Outputs per child:
+ * parent (int32), seg row (int32), shift (int32), phase (f64), score (f32).
+ * minmax_dev (f32 [2], caller-initialised to {+inf, -inf}) receives the score min/max. */
+int pyl_prune_children(const pyl_prune_params* p, const pyl_prune_coords* c,
+                       const double* leaves_dev, const float* folds_dev, int64_t n_parents,
+                       const double* branch_dev, const int32_t* counts_dev,
+                       const int64_t* child_off_dev, int64_t n_children,
+                       const float* seg_fold_dev, int32_t* child_parent_dev,
+                       int32_t* child_row_dev, int32_t* child_shift_dev, double* child_phase_dev,
+                       float* child_score_dev, float* minmax_dev, void* stream);
+
+/* surv_dev: int64 [n_surv] child ids in increasing order. Writes the new tree. */
+int pyl_prune_materialize(const pyl_prune_params* p, const pyl_prune_coords* c,
+                          const double* leaves_dev, const float* folds_dev,
+                          const double* branch_dev, const int32_t* counts_dev,
+                          const int64_t* child_off_dev, const float* seg_fold_dev,
+                          const int32_t* child_parent_dev, const int32_t* child_row_dev,
+                          const int32_t* child_shift_dev, const double* child_phase_dev,
+                          const float* child_score_dev, const int64_t* surv_dev, int64_t n_surv,
+                          double* out_leaves_dev, float* out_folds_dev, float* out_scores_dev,
+                          int32_t* out_backtracks_dev, void* stream);
+
+/* dyp_fold_dev: f32 [nseg][ncoords][2][nbins] (the FFA fold handed to the EP).
+ * seg_idx / seg_t0: host [nseg_used] segment indices and reference times (segments so
+ * far, in scheme order). Writes the re-integrated folds and their scores. */
+int pyl_prune_ascend(const pyl_prune_params* p, const double* leaves_dev, int64_t n,
+                     const float* dyp_fold_dev, int64_t ncoords_ffa, const int64_t* seg_idx,
+                     const double* seg_t0, int32_t nseg_used, double t0_cur,
+                     float* out_folds_dev, float* out_scores_dev, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

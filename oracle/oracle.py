@@ -13,7 +13,9 @@ How each function is pinned (beyond restating the reference):
 - ``phase_idx`` / ``phase_idx_int`` (``psr_utils.py:12-56``): exact rational brute force
   with ``fractions.Fraction`` on random inputs away from rounding boundaries
   (tests/test_oracle.py).
-- ``pow_int`` (``transforms.py:43``): agreement with libm ``pow`` on random inputs.
+- ``powi`` (``transforms.py:43``, ``psr_utils.py:89``): the binary-powering multiply chain,
+  checked against a Python transcription of that chain and, within 1 ulp, against the
+  exact power.
 - ``brutefold`` (``fold.py:108-171``): a pure-Python loop on tiny inputs, the
   sum-conservation invariant (sum over bins == segment sum, exact for integer-valued
   data), and ``fold_v == per-bin sample count`` for ``ts_v == 1``.
@@ -67,8 +69,8 @@ def lib() -> ctypes.CDLL:
         L.orc_phase_idx_int.argtypes = [_dbl, _dbl, _i64, _dbl]
         L.orc_shift_int.restype = _i64
         L.orc_shift_int.argtypes = [_dbl, _i64]
-        L.orc_pow_int.restype = _dbl
-        L.orc_pow_int.argtypes = [_dbl, _int]
+        L.orc_powi.restype = _dbl
+        L.orc_powi.argtypes = [_dbl, _int]
         L.orc_nearest_idx.restype = _i64
         L.orc_nearest_idx.argtypes = [_dbl, _i64, _dbl, _dbl]
         L.orc_shift_taylor_params.restype = None
@@ -106,8 +108,8 @@ def shift_int(x: float, nbins: int) -> int:
     return lib().orc_shift_int(x, nbins)
 
 
-def pow_int(x: float, p: int) -> float:
-    return lib().orc_pow_int(x, p)
+def powi(x: float, n: int) -> float:
+    return lib().orc_powi(x, n)
 
 
 def shift_taylor_params(vec: np.ndarray, dt: float) -> np.ndarray:

@@ -44,22 +44,19 @@ int64_t orc_phase_idx_int(double proper_time, double freq, int64_t nbins, double
     return orc_shift_int(orc_phase_idx(proper_time, freq, nbins, delay), nbins);
 }
 
-/* x**p for an integer power, as glibc pow evaluates it (correctly rounded in all
- * but vanishingly rare cases). transforms.py:43 builds dt**powers with numba, which
- * lowers to libm pow. Evaluated here in double-double and rounded once. */
-double orc_pow_int(double x, int p) {
-    if (p == 0) return 1.0;
-    if (p == 1) return x;
-    double hi = x, lo = 0.0;
-    for (int k = 2; k <= p; ++k) {
-        double ph = hi * x;
-        double pl = fma(hi, x, -ph);
-        pl += lo * x;
-        double s = ph + pl;
-        lo = pl - (s - ph);
-        hi = s;
+/* x**n by binary powering: what numba's fastmath lowering of a float raised to an
+ * integer array evaluates (LLVM rewrites pow(x, sitofp(n)) to llvm.powi). */
+double orc_powi(double a, int b) {
+    int recip = b < 0;
+    if (recip) b = -b;
+    double r = 1.0;
+    for (;;) {
+        if (b & 1) r *= a;
+        b /= 2;
+        if (b == 0) break;
+        a *= a;
     }
-    return hi;
+    return recip ? 1.0 / r : r;
 }
 
 static double orc_fact(int n) {
@@ -69,14 +66,14 @@ static double orc_fact(int n) {
 }
 
 /* transforms.py:10-47 shift_taylor_params for one vector [d_k..d_0] (n entries):
- * out[i] = sum_{j=0..n-1} vec[j] * T[i][j], T[i][j] = dt^(i-j)/(i-j)! for j<=i else 0,
- * summed in ascending j. */
+ * out[i] = sum_{j=0..n-1} vec[j] * T[i][j], T[i][j] = dt**(i-j)/(i-j)! for j<=i else 0
+ * (orc_powi), summed in ascending j. */
 void orc_shift_taylor_params(const double* vec, int n, double dt, double* out) {
     for (int i = 0; i < n; ++i) {
         double acc = 0.0;
         for (int j = 0; j < n; ++j) {
             double t = 0.0;
-            if (j <= i) t = orc_pow_int(dt, i - j) / orc_fact(i - j);
+            if (j <= i) t = orc_powi(dt, i - j) / orc_fact(i - j);
             acc += vec[j] * t;
         }
         out[i] = acc;

@@ -44,6 +44,15 @@ SIGNATURES: list[tuple[str, object, list[object]]] = [
      [_c_i32, _vp, _i64p, _i64p, _f64p, _c_i32, _c_dbl, _c_i32, _vp, _c_i64, _vp, _vp, _vp]),
     ("pyl_score_boxcar", ctypes.c_int,
      [_vp, _c_i64, _c_i32, _c_i32, _i64p, _c_i32, _c_dbl, _c_i32, _vp, _vp]),
+    ("pyl_prune_branch", ctypes.c_int, [_vp, _vp, _vp, _c_i64, _vp, _vp, _vp, _vp, _vp]),
+    ("pyl_prune_children", ctypes.c_int,
+     [_vp, _vp, _vp, _vp, _c_i64, _vp, _vp, _vp, _c_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+      _vp]),
+    ("pyl_prune_materialize", ctypes.c_int,
+     [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_i64, _vp, _vp,
+      _vp, _vp, _vp]),
+    ("pyl_prune_ascend", ctypes.c_int,
+     [_vp, _vp, _c_i64, _vp, _c_i64, _i64p, _f64p, _c_i32, _c_dbl, _vp, _vp, _vp]),
 ]
 
 _lib: ctypes.CDLL | None = None

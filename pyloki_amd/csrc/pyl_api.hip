@@ -15,6 +15,12 @@ int fail(int code, const char* fmt, ...) {
   return code;
 }
 
+void taylor_matrix(int n, double dt, double* out) {
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j)
+      out[i * n + j] = j <= i ? powi(dt, i - j) / fact_d(i - j) : 0.0;
+}
+
 }  // namespace pyl
 
 extern "C" const char* pyl_last_error(void) { return pyl::g_last_error.c_str(); }

@@ -69,28 +69,31 @@ __device__ __host__ inline int64_t nearest_idx(double val, int64_t n_grid, doubl
   return idx;
 }
 
-// x**p for a small integer power, rounded once from a double-double product: the value
-// libm pow returns for an integral exponent (transforms.py:43 dt**powers).
-__device__ __host__ inline double pow_int(double x, int p) {
-  if (p == 0) return 1.0;
-  if (p == 1) return x;
-  double hi = x, lo = 0.0;
-  for (int k = 2; k <= p; ++k) {
-    double ph = hi * x;
-    double pl = fma(hi, x, -ph);
-    pl += lo * x;
-    double s = ph + pl;
-    lo = pl - (s - ph);
-    hi = s;
-  }
-  return hi;
-}
-
 __device__ __host__ inline double fact_d(int n) {
   double r = 1.0;
   for (int i = 2; i <= n; ++i) r *= (double)i;
   return r;
 }
+
+// x**n for an integer n by binary powering. The reference raises floats to integer
+// arrays inside numba fastmath code (`delta_t**powers`, transforms.py:43;
+// `(tobs - t_ref)**(k+1)`, psr_utils.py:89); LLVM's fast-math simplification turns
+// pow(x, sitofp(n)) into llvm.powi, whose runtime routine is this multiply chain.
+__device__ __host__ inline double powi(double a, int b) {
+  const bool recip = b < 0;
+  if (recip) b = -b;
+  double r = 1.0;
+  while (true) {
+    if (b & 1) r *= a;
+    b /= 2;
+    if (b == 0) break;
+    a *= a;
+  }
+  return recip ? 1.0 / r : r;
+}
+
+// transforms.py:41-43: T[i][j] = dt**(i-j) / (i-j)! for j <= i, else 0 (n x n, row-major).
+void taylor_matrix(int n, double dt, double* out);
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 

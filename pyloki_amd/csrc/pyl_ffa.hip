@@ -38,6 +38,8 @@ struct ResolveArgs {
   double lim_lo[kMaxP];
   double lim_hi[kMaxP];
   double tseg_brute;
+  double delta_t[2];                       // tail, head
+  double tmat[2][(kMaxP + 1) * (kMaxP + 1)];  // Taylor shift matrix per half (row-major)
 };
 
 __global__ void ffa_resolve_kernel(const double* __restrict__ params, ResolveArgs a,
@@ -54,27 +56,22 @@ __global__ void ffa_resolve_kernel(const double* __restrict__ params, ResolveArg
   }
   int out[4];
   double ph[2];
-  const double half_span = (double)((int64_t)1 << (a.level - 1));
   for (int latter = 0; latter < 2; ++latter) {
     double pnew[kMaxP];
-    double delta_t, delay;
+    double delay;
+    const double delta_t = a.delta_t[latter];
     if (P == 1) {
-      delta_t = (double)((int64_t)latter * ((int64_t)1 << (a.level - 1))) * a.tseg_brute;
       pnew[0] = p[0];
       delay = 0.0;
     } else {
-      delta_t = ((latter - 0.5) * half_span) * a.tseg_brute;
       double tv[kMaxP + 1], tn[kMaxP + 1];
       const int n = P + 1;
+      const double* T = a.tmat[latter];
       for (int k = 0; k < n; ++k) tv[k] = 0.0;
       for (int k = 0; k < P - 1; ++k) tv[k] = p[k];
       for (int i = 0; i < n; ++i) {
         double acc = 0.0;
-        for (int jj = 0; jj < n; ++jj) {
-          double t = 0.0;
-          if (jj <= i) t = pyl::pow_int(delta_t, i - jj) / pyl::fact_d(i - jj);
-          acc += tv[jj] * t;
-        }
+        for (int jj = 0; jj < n; ++jj) acc += tv[jj] * T[i * n + jj];
         tn[i] = acc;
       }
       for (int k = 0; k < P - 1; ++k) pnew[k] = tn[k];
@@ -219,6 +216,17 @@ extern "C" int pyl_ffa_resolve(int32_t nparams, const double* params_dev,
   PYL_REQUIRE(nprev <= 0x7fffffff, "previous level has too many coordinates (%lld)",
               (long long)nprev);
   a.ncoords = ncur;
+  // fold.py:660-664: delta_t of each half; transforms.py:41-43: the matrix dt**p / p! is a
+  // per-level constant, built once on the host.
+  for (int latter = 0; latter < 2; ++latter) {
+    double dt;
+    if (nparams == 1)
+      dt = (double)((int64_t)latter * ((int64_t)1 << (ffa_level - 1))) * tseg_brute;
+    else
+      dt = ((latter - 0.5) * (double)((int64_t)1 << (ffa_level - 1))) * tseg_brute;
+    a.delta_t[latter] = dt;
+    pyl::taylor_matrix(nparams + 1, dt, a.tmat[latter]);
+  }
   hipStream_t s = pyl::as_stream(stream);
   const int threads = 128;
   ffa_resolve_kernel<<<(unsigned)pyl::ceil_div(ncur, threads), threads, 0, s>>>(

@@ -1,0 +1,572 @@
+// EP (enumerate-and-prune) Taylor search on the GPU: one pruning level of
+// prune.py:165-332 as three kernels, plus the final re-integration (ascend).
+//
+// Work decomposition. The reference walks parents in batches of 1024 and, per batch,
+// branches, resolves, shift-adds and scores every child in parent-major, odometer order.
+// Here every child of the level is one GPU thread:
+//   prune_branch_kernel    thread per parent: per-parameter child counts, first centre and
+//                          step (FP64, psr_utils.py:347-379 + taylor.py:118-142);
+//   (caller)               exclusive prefix sum of the child counts -> child offsets, so
+//                          child c of the level is the reference's c-th child;
+//   prune_children_kernel  thread per child: parent by binary search, odometer decode,
+//                          FP64 resolve (taylor.py:172-241); then the 64 children of a
+//                          block are folded (parent row + rotated segment row, coalesced
+//                          lane = bin loads) and scored by the sequential per-thread scan;
+//   prune_materialize_kernel thread per survivor: transform (taylor.py:323-339), backtrack
+//                          (prune.py:302-314); survivor rows re-folded coalesced.
+// The child folds are never stored: a survivor's fold is recomputed from its parent row and
+// segment row (one extra 1 KB read per survivor instead of a 512 B write per child).
+//
+// Taylor translation matrices T[i][j] = dt**(i-j)/(i-j)! are per-level constants; they are
+// built on the host with the binary-powering power (powi) the reference's numba code uses,
+// and the GPU only does the per-leaf FP64 matrix-vector products, in ascending column order.
+#include <algorithm>
+#include <cfloat>
+#include <climits>
+#include <cmath>
+
+#include "pyl_common.hpp"
+
+namespace {
+
+constexpr int kMaxP = 5;
+constexpr int kMaxN = kMaxP + 1;
+constexpr int kMaxW = PYL_MAX_WIDTHS;
+constexpr int kBlk = 64;
+constexpr int kStride = kBlk + 1;
+
+struct LevelConst {
+  int32_t P, nbins, branch_max, tiling;
+  int32_t nparams_ffa, n_accel, n_freq, nw, wmax;
+  double eta;
+  double dpf[kMaxP];          // poly_taylor_step_f(P, scale_cur) row order d_P..d_1
+  double fac[kMaxP];          // shift factors, same order
+  double tadd[kMaxN * kMaxN];
+  double tinit[kMaxN * kMaxN];
+  double tnext[kMaxN * kMaxN];
+  double dt_add_init;         // t0_add - t0_init
+  double amin, fmin, sa, sf;  // accel/freq grid origin and inverse step
+  int32_t widths[kMaxW];
+  double hpb[kMaxW];
+  double bb[kMaxW];
+};
+
+int build_const(const pyl_prune_params* p, const pyl_prune_coords* c, LevelConst* k) {
+  PYL_REQUIRE(p != nullptr, "null pyl_prune_params");
+  PYL_REQUIRE(p->poly_order >= 2 && p->poly_order <= kMaxP, "poly_order must be in [2, %d]",
+              kMaxP);
+  PYL_REQUIRE(p->nbins > 1, "nbins must be > 1");
+  PYL_REQUIRE(p->nw >= 1 && p->nw <= kMaxW, "number of widths must be in [1, %d]", kMaxW);
+  PYL_REQUIRE(p->nparams_ffa >= 2 && p->nparams_ffa <= p->poly_order,
+              "FFA grid must have 2..poly_order parameters");
+  PYL_REQUIRE(p->n_accel >= 1 && p->n_freq >= 1, "empty FFA grid");
+  PYL_REQUIRE(p->tiling >= 0 && p->tiling <= 2, "invalid tiling strategy");
+  *k = LevelConst{};
+  const int P = p->poly_order;
+  k->P = P;
+  k->nbins = p->nbins;
+  k->branch_max = p->branch_max;
+  k->tiling = p->tiling;
+  k->nparams_ffa = p->nparams_ffa;
+  k->n_accel = p->n_accel;
+  k->n_freq = p->n_freq;
+  k->eta = p->eta;
+  if (c) {
+    // psr_utils.py:59-96 (t_ref = 0, Chebyshev coarsening), reversed to d_P..d_1.
+    const double dphi = p->eta / p->nbins;
+    for (int kk = 0; kk < P; ++kk) {
+      const double v = dphi * pyl::fact_d(kk + 1) / pyl::powi(c->scale_cur, kk + 1);
+      k->dpf[P - 1 - kk] = (double)(1LL << kk) * v;
+    }
+    // psr_utils.py:227-248: (tobs)^(k+1) * nbins / (k+1)! / 2^k, k = P-1..0.
+    for (int idx = 0; idx < P; ++idx) {
+      const int kk = P - 1 - idx;
+      const double f = pyl::powi(c->scale_cur, kk + 1) * p->nbins / pyl::fact_d(kk + 1);
+      k->fac[idx] = f / (double)(1LL << kk);
+    }
+    pyl::taylor_matrix(P + 1, c->t0_add - c->t0_cur, k->tadd);
+    pyl::taylor_matrix(P + 1, c->t0_init - c->t0_cur, k->tinit);
+    pyl::taylor_matrix(P + 1, c->t0_next - c->t0_cur, k->tnext);
+    k->dt_add_init = c->t0_add - c->t0_init;
+  }
+  k->amin = p->accel_lim[0];
+  k->fmin = p->freq_lim[0];
+  k->sa = p->accel_lim[1] > p->accel_lim[0] ? p->n_accel / (p->accel_lim[1] - p->accel_lim[0])
+                                             : 0.0;
+  k->sf = p->freq_lim[1] > p->freq_lim[0] ? p->n_freq / (p->freq_lim[1] - p->freq_lim[0]) : 0.0;
+  k->nw = p->nw;
+  k->wmax = 0;
+  for (int w = 0; w < p->nw; ++w) {
+    const int64_t wd = p->widths[w];
+    PYL_REQUIRE(wd >= 1 && wd < p->nbins, "width %lld outside [1, nbins-1]", (long long)wd);
+    k->widths[w] = (int32_t)wd;
+    k->wmax = std::max<int32_t>(k->wmax, (int32_t)wd);
+    const int64_t size_w = p->nbins - wd;
+    const double h = std::sqrt((double)size_w / (double)(p->nbins * wd));
+    const double b = (double)wd * h / (double)size_w;
+    k->hpb[w] = h + b;
+    k->bb[w] = b;
+  }
+  return PYL_OK;
+}
+
+// psr_utils.py:517-554: (accel, freq) grid indices by truncation with a 1e-6 guard.
+__device__ __forceinline__ int64_t trunc_idx(double v, double vmin, double sinv, int n) {
+  if (!(sinv > 0.0)) return 0;
+  int64_t i = (int64_t)((v - vmin) * sinv + pyl::kFloatEps);
+  if (i < 0) i = 0;
+  else if (i >= n) i = n - 1;
+  return i;
+}
+
+__device__ void atomic_min_f32(float* addr, float v) {
+  int* ia = reinterpret_cast<int*>(addr);
+  int old = __atomic_load_n(ia, __ATOMIC_RELAXED);
+  while (__int_as_float(old) > v) {
+    int assumed = old;
+    old = atomicCAS(ia, assumed, __float_as_int(v));
+    if (old == assumed) break;
+  }
+}
+
+__device__ void atomic_max_f32(float* addr, float v) {
+  int* ia = reinterpret_cast<int*>(addr);
+  int old = __atomic_load_n(ia, __ATOMIC_RELAXED);
+  while (__int_as_float(old) < v) {
+    int assumed = old;
+    old = atomicCAS(ia, assumed, __float_as_int(v));
+    if (old == assumed) break;
+  }
+}
+
+__global__ void prune_branch_kernel(LevelConst k, const double* __restrict__ leaves, int64_t n,
+                                    double* __restrict__ branch, int32_t* __restrict__ counts,
+                                    int64_t* __restrict__ nchild,
+                                    unsigned long long* __restrict__ err) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int P = k.P;
+  const double* lf = leaves + i * (P + 2) * 2;
+  const double f0 = lf[(P + 1) * 2];
+  int64_t total = 1;
+  for (int j = 0; j < P; ++j) {
+    const double pc = lf[2 * j];
+    const double dc = lf[2 * j + 1];
+    const double dn = k.dpf[j] * pyl::kCVal / f0;
+    const double shift = fabs(dc - dn) * (k.fac[j] * (f0 / pyl::kCVal));
+    int32_t cnt = 1;
+    double first = pc, step = dc;
+    if (dc <= 0.0 || dn <= 0.0) {
+      atomicMin(err, (unsigned long long)((i * P + j) * 4 + 2));
+    } else {
+      const double ratio = dc / dn;
+      double nump = ceil(ratio - pyl::kFloatEps);
+      if (nump < 1.0) nump = 1.0;
+      if (nump > (double)k.branch_max) {
+        atomicMin(err, (unsigned long long)((i * P + j) * 4 + 1));
+        nump = 1.0;
+      }
+      cnt = (int32_t)nump;
+      step = dc / (double)cnt;
+      first = (pc - dc / 2.0) + step / 2.0;
+    }
+    if (shift < (k.eta - pyl::kFloatEps)) {
+      cnt = -1;  // not split: the child keeps the parent value and step
+      first = pc;
+      step = dc;
+    }
+    branch[(i * P + j) * 2] = first;
+    branch[(i * P + j) * 2 + 1] = step;
+    counts[i * P + j] = cnt;
+    total *= cnt < 0 ? 1 : cnt;
+  }
+  nchild[i] = total;
+}
+
+struct ChildLeaf {
+  double val[kMaxN];  // d_P..d_1, d_0
+  double err[kMaxN];
+  double f0, flag;
+};
+
+// Parent of child c: the last p with off[p] <= c.
+__device__ __forceinline__ int64_t find_parent(const int64_t* __restrict__ off, int64_t n_par,
+                                               int64_t c) {
+  int64_t lo = 0, hi = n_par;  // off[lo] <= c < off[hi]
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (off[mid] <= c) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ void decode_child(const LevelConst& k, const double* __restrict__ lf,
+                                             const double* __restrict__ br,
+                                             const int32_t* __restrict__ cn, int64_t local,
+                                             ChildLeaf& out) {
+  const int P = k.P;
+  for (int j = P - 1; j >= 0; --j) {
+    const int32_t cnt = cn[j];
+    const int64_t n = cnt < 0 ? 1 : cnt;
+    const int64_t ij = local % n;
+    local /= n;
+    out.val[j] = cnt < 0 ? br[2 * j] : br[2 * j] + (double)ij * br[2 * j + 1];
+    out.err[j] = br[2 * j + 1];
+  }
+  out.val[P] = lf[P * 2];  // d_0 copied from the parent; its error starts at 0
+  out.err[P] = 0.0;
+  out.f0 = lf[(P + 1) * 2];
+  out.flag = lf[(P + 1) * 2 + 1];
+}
+
+__device__ __forceinline__ double mat_row(const double* __restrict__ T, const double* v, int n,
+                                          int i) {
+  double acc = 0.0;
+  for (int j = 0; j < n; ++j) acc += v[j] * T[i * n + j];
+  return acc;
+}
+
+// Sequential scan + boxcar S/N of the profile held in column `tid` of S (scoring.py:249-282).
+__device__ float score_column(const LevelConst& k, float* S, int tid) {
+  const int B = k.nbins;
+  float run = S[tid];
+  for (int i = 1; i < B; ++i) {
+    run = run + S[i * kStride + tid];
+    S[i * kStride + tid] = run;
+  }
+  const float total = run;
+  for (int j = 0; j < k.wmax; ++j) S[(B + j) * kStride + tid] = S[j * kStride + tid] + total;
+  double best = -(double)FLT_MAX;
+  for (int w = 0; w < k.nw; ++w) {
+    const int wd = k.widths[w];
+    float dmax = -FLT_MAX;
+    for (int i = 0; i < B; ++i) {
+      const float d = S[(i + wd) * kStride + tid] - S[i * kStride + tid];
+      dmax = d > dmax ? d : dmax;
+    }
+    const double t1 = k.hpb[w] * (double)dmax;
+    const double t2 = k.bb[w] * (double)total;
+    const double snr = t1 - t2;
+    best = snr > best ? snr : best;
+  }
+  return (float)best;
+}
+
+__global__ void __launch_bounds__(kBlk)
+prune_children_kernel(LevelConst k, const double* __restrict__ leaves,
+                      const float* __restrict__ folds, int64_t n_par,
+                      const double* __restrict__ branch, const int32_t* __restrict__ counts,
+                      const int64_t* __restrict__ off, int64_t n_child,
+                      const float* __restrict__ seg, int32_t* __restrict__ c_parent,
+                      int32_t* __restrict__ c_row, int32_t* __restrict__ c_shift,
+                      double* __restrict__ c_phase, float* __restrict__ c_score,
+                      float* __restrict__ minmax) {
+  extern __shared__ float S[];  // [(B + wmax)][kStride]
+  __shared__ int32_t s_parent[kBlk], s_row[kBlk], s_shift[kBlk];
+  const int tid = threadIdx.x;
+  const int64_t c0 = (int64_t)blockIdx.x * kBlk;
+  const int nvalid = (int)std::min<int64_t>(kBlk, n_child - c0);
+  const int P = k.P;
+  const int B = k.nbins;
+  if (tid < nvalid) {
+    const int64_t c = c0 + tid;
+    const int64_t par = find_parent(off, n_par, c);
+    const double* lf = leaves + par * (P + 2) * 2;
+    ChildLeaf ch;
+    decode_child(k, lf, branch + par * P * 2, counts + par * P, c - off[par], ch);
+    const int n = P + 1;
+    const double accel = mat_row(k.tadd, ch.val, n, P - 2);
+    const double v_add = mat_row(k.tadd, ch.val, n, P - 1);
+    const double d_add = mat_row(k.tadd, ch.val, n, P);
+    const double v_init = mat_row(k.tinit, ch.val, n, P - 1);
+    const double d_init = mat_row(k.tinit, ch.val, n, P);
+    const double vel = v_add - v_init;
+    const double freq = ch.f0 * (1.0 - vel / pyl::kCVal);
+    const double delay = (d_add - d_init) / pyl::kCVal;
+    const double phase = pyl::phase_idx(k.dt_add_init, ch.f0, B, delay);
+    const int64_t ia = trunc_idx(accel, k.amin, k.sa, k.n_accel);
+    const int64_t ifr = trunc_idx(freq, k.fmin, k.sf, k.n_freq);
+    const int32_t row = (int32_t)(ia * k.n_freq + ifr);
+    const int32_t sh = pyl::shift_int(phase, B);
+    c_parent[c] = (int32_t)par;
+    c_row[c] = row;
+    c_shift[c] = sh;
+    c_phase[c] = phase;
+    s_parent[tid] = (int32_t)par;
+    s_row[tid] = row;
+    s_shift[tid] = sh;
+  }
+  __syncthreads();
+  // Fold (common.py:389-399: parent row + segment row rotated by the shift) and normalise,
+  // lane = bin, transposed into S.
+  for (int r = 0; r < nvalid; ++r) {
+    const float* fr = folds + (int64_t)s_parent[r] * 2 * B;
+    const float* sr = seg + (int64_t)s_row[r] * 2 * B;
+    const int sh = s_shift[r];
+    for (int b = tid; b < B; b += kBlk) {
+      int src = b - sh;
+      if (src < 0) src += B;
+      const float e = fr[b] + sr[src];
+      const float v = fr[B + b] + sr[B + src];
+      S[b * kStride + r] = e / sqrtf(v);
+    }
+  }
+  __syncthreads();
+  float sc = 0.0f;
+  if (tid < nvalid) {
+    sc = score_column(k, S, tid);
+    c_score[c0 + tid] = sc;
+  }
+  // Block min/max of the scores, then one atomic per block.
+  float mn = tid < nvalid ? sc : FLT_MAX;
+  float mx = tid < nvalid ? sc : -FLT_MAX;
+  for (int o = 32; o > 0; o >>= 1) {
+    mn = fminf(mn, __shfl_xor(mn, o));
+    mx = fmaxf(mx, __shfl_xor(mx, o));
+  }
+  if (tid == 0 && nvalid > 0) {
+    atomic_min_f32(&minmax[0], mn);
+    atomic_max_f32(&minmax[1], mx);
+  }
+}
+
+__global__ void __launch_bounds__(kBlk)
+prune_materialize_kernel(LevelConst k, const double* __restrict__ leaves,
+                         const float* __restrict__ folds, int64_t n_par,
+                         const double* __restrict__ branch, const int32_t* __restrict__ counts,
+                         const int64_t* __restrict__ off, const float* __restrict__ seg,
+                         const int32_t* __restrict__ c_parent, const int32_t* __restrict__ c_row,
+                         const int32_t* __restrict__ c_shift,
+                         const double* __restrict__ c_phase, const float* __restrict__ c_score,
+                         const int64_t* __restrict__ surv, int64_t n_surv,
+                         double* __restrict__ o_leaves, float* __restrict__ o_folds,
+                         float* __restrict__ o_scores, int32_t* __restrict__ o_bt) {
+  __shared__ int64_t s_child[kBlk];
+  const int tid = threadIdx.x;
+  const int64_t s0 = (int64_t)blockIdx.x * kBlk;
+  const int nvalid = (int)std::min<int64_t>(kBlk, n_surv - s0);
+  const int P = k.P;
+  const int B = k.nbins;
+  if (tid < nvalid) {
+    const int64_t s = s0 + tid;
+    const int64_t c = surv[s];
+    s_child[tid] = c;
+    const int64_t par = c_parent[c];
+    const double* lf = leaves + par * (P + 2) * 2;
+    ChildLeaf ch;
+    decode_child(k, lf, branch + par * P * 2, counts + par * P, c - off[par], ch);
+    const int n = P + 1;
+    double* ol = o_leaves + s * (P + 2) * 2;
+    // transforms.py:93-137 shift_taylor_full to coord_next.
+    for (int i = 0; i < n; ++i) {
+      ol[2 * i] = mat_row(k.tnext, ch.val, n, i);
+      double e;
+      if (k.tiling == PYL_TILING_AGGRESSIVE) {
+        e = ch.err[i] * fabs(k.tnext[i * n + i]);
+      } else if (k.tiling == PYL_TILING_CONSERVATIVE) {
+        e = 0.0;
+        for (int j = 0; j < n; ++j) e += ch.err[j] * fabs(k.tnext[i * n + j]);
+      } else {
+        e = 0.0;
+        for (int j = 0; j < n; ++j) {
+          const double t = k.tnext[i * n + j];
+          e += (ch.err[j] * ch.err[j]) * (t * t);
+        }
+        e = sqrt(e);
+      }
+      ol[2 * i + 1] = e;
+    }
+    ol[2 * (P + 1)] = ch.f0;
+    ol[2 * (P + 1) + 1] = ch.flag;
+    o_scores[s] = c_score[c];
+    // prune.py:309-314: [global parent, grid indices, int32(phase) (truncated)].
+    int32_t* bt = o_bt + s * (P + 2);
+    bt[0] = (int32_t)par;
+    const int32_t row = c_row[c];
+    for (int q = 1; q <= k.nparams_ffa; ++q) bt[q] = 0;
+    bt[k.nparams_ffa - 1] = row / k.n_freq;
+    bt[k.nparams_ffa] = row % k.n_freq;
+    for (int q = k.nparams_ffa + 1; q < P + 1; ++q) bt[q] = 0;
+    bt[P + 1] = (int32_t)c_phase[c];
+  }
+  __syncthreads();
+  for (int r = 0; r < nvalid; ++r) {
+    const int64_t c = s_child[r];
+    const float* fr = folds + (int64_t)c_parent[c] * 2 * B;
+    const float* sr = seg + (int64_t)c_row[c] * 2 * B;
+    const int sh = c_shift[c];
+    float* of = o_folds + (s0 + r) * 2 * B;
+    for (int b = tid; b < B; b += kBlk) {
+      int src = b - sh;
+      if (src < 0) src += B;
+      of[b] = fr[b] + sr[src];
+      of[B + b] = fr[B + b] + sr[B + src];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBlk)
+prune_ascend_kernel(LevelConst k, const double* __restrict__ leaves, int64_t n,
+                    const float* __restrict__ dyp, int64_t ncoords, const int64_t* __restrict__ segi,
+                    const double* __restrict__ segt, int32_t nseg, double t0_cur,
+                    int32_t* __restrict__ scratch_row, int32_t* __restrict__ scratch_shift,
+                    float* __restrict__ o_folds, float* __restrict__ o_scores) {
+  extern __shared__ float S[];
+  const int tid = threadIdx.x;
+  const int64_t s0 = (int64_t)blockIdx.x * kBlk;
+  const int nvalid = (int)std::min<int64_t>(kBlk, n - s0);
+  const int P = k.P;
+  const int B = k.nbins;
+  const int nn = P + 1;
+  // taylor.py:280-320: resolve every segment so far for this survivor.
+  if (tid < nvalid) {
+    const double* lf = leaves + (s0 + tid) * (P + 2) * 2;
+    double v[kMaxN];
+    for (int i = 0; i < nn; ++i) v[i] = lf[2 * i];
+    const double f0 = lf[(P + 1) * 2];
+    for (int s = 0; s < nseg; ++s) {
+      const double dt = segt[s] - t0_cur;
+      double T[kMaxN * kMaxN];
+      for (int i = 0; i < nn; ++i)
+        for (int j = 0; j < nn; ++j)
+          T[i * nn + j] = j <= i ? pyl::powi(dt, i - j) / pyl::fact_d(i - j) : 0.0;
+      const double accel = mat_row(T, v, nn, P - 2);
+      const double vel = mat_row(T, v, nn, P - 1);
+      const double d0 = mat_row(T, v, nn, P);
+      const double freq = f0 * (1.0 - vel / pyl::kCVal);
+      const double delay = d0 / pyl::kCVal;
+      const double phase = pyl::phase_idx(dt, f0, B, delay);
+      const int64_t ia = trunc_idx(accel, k.amin, k.sa, k.n_accel);
+      const int64_t ifr = trunc_idx(freq, k.fmin, k.sf, k.n_freq);
+      scratch_row[(s0 + tid) * nseg + s] = (int32_t)(ia * k.n_freq + ifr);
+      scratch_shift[(s0 + tid) * nseg + s] = pyl::shift_int(phase, B);
+    }
+  }
+  __syncthreads();
+  // common.py:432-459: accumulate the rotated rows in segment order, from zero.
+  for (int r = 0; r < nvalid; ++r) {
+    const int64_t sv = s0 + r;
+    float* of = o_folds + sv * 2 * B;
+    for (int b = tid; b < B; b += kBlk) {
+      float ae = 0.0f, av = 0.0f;
+      for (int s = 0; s < nseg; ++s) {
+        const float* rowp =
+            dyp + ((int64_t)segi[s] * ncoords + scratch_row[sv * nseg + s]) * 2 * B;
+        int src = b - scratch_shift[sv * nseg + s];
+        if (src < 0) src += B;
+        ae = ae + rowp[src];
+        av = av + rowp[B + src];
+      }
+      of[b] = ae;
+      of[B + b] = av;
+      S[b * kStride + r] = ae / sqrtf(av);
+    }
+  }
+  __syncthreads();
+  if (tid < nvalid) o_scores[s0 + tid] = score_column(k, S, tid);
+}
+
+}  // namespace
+
+extern "C" int pyl_prune_branch(const pyl_prune_params* p, const pyl_prune_coords* c,
+                                const double* leaves, int64_t n_par, double* branch,
+                                int32_t* counts, int64_t* nchild, int64_t* err, void* stream) {
+  LevelConst k;
+  int rc = build_const(p, c, &k);
+  if (rc != PYL_OK) return rc;
+  PYL_REQUIRE(c != nullptr, "null pyl_prune_coords");
+  PYL_REQUIRE(leaves && branch && counts && nchild && err, "null device pointer");
+  if (n_par == 0) return PYL_OK;
+  const int threads = 128;
+  prune_branch_kernel<<<(unsigned)pyl::ceil_div(n_par, threads), threads, 0,
+                        pyl::as_stream(stream)>>>(k, leaves, n_par, branch, counts, nchild,
+                                                  reinterpret_cast<unsigned long long*>(err));
+  PYL_CHECK_LAUNCH("prune_branch_kernel");
+  return PYL_OK;
+}
+
+extern "C" int pyl_prune_children(const pyl_prune_params* p, const pyl_prune_coords* c,
+                                  const double* leaves, const float* folds, int64_t n_par,
+                                  const double* branch, const int32_t* counts,
+                                  const int64_t* off, int64_t n_child, const float* seg,
+                                  int32_t* c_parent, int32_t* c_row, int32_t* c_shift,
+                                  double* c_phase, float* c_score, float* minmax,
+                                  void* stream) {
+  LevelConst k;
+  int rc = build_const(p, c, &k);
+  if (rc != PYL_OK) return rc;
+  PYL_REQUIRE(c != nullptr, "null pyl_prune_coords");
+  PYL_REQUIRE(leaves && folds && branch && counts && off && seg && c_parent && c_row &&
+                  c_shift && c_phase && c_score && minmax,
+              "null device pointer");
+  PYL_REQUIRE((int64_t)p->n_accel * p->n_freq <= 0x7fffffff, "FFA grid too large");
+  if (n_child == 0) return PYL_OK;
+  const size_t lds = sizeof(float) * (size_t)(k.nbins + k.wmax) * kStride;
+  PYL_REQUIRE(lds <= 64 * 1024, "nbins too large for the EP scorer (%d)", k.nbins);
+  prune_children_kernel<<<(unsigned)pyl::ceil_div(n_child, kBlk), kBlk, lds,
+                          pyl::as_stream(stream)>>>(k, leaves, folds, n_par, branch, counts,
+                                                    off, n_child, seg, c_parent, c_row, c_shift,
+                                                    c_phase, c_score, minmax);
+  PYL_CHECK_LAUNCH("prune_children_kernel");
+  return PYL_OK;
+}
+
+extern "C" int pyl_prune_materialize(const pyl_prune_params* p, const pyl_prune_coords* c,
+                                     const double* leaves, const float* folds,
+                                     const double* branch, const int32_t* counts,
+                                     const int64_t* off, const float* seg,
+                                     const int32_t* c_parent, const int32_t* c_row,
+                                     const int32_t* c_shift, const double* c_phase,
+                                     const float* c_score, const int64_t* surv, int64_t n_surv,
+                                     double* o_leaves, float* o_folds, float* o_scores,
+                                     int32_t* o_bt, void* stream) {
+  LevelConst k;
+  int rc = build_const(p, c, &k);
+  if (rc != PYL_OK) return rc;
+  PYL_REQUIRE(c != nullptr, "null pyl_prune_coords");
+  if (n_surv == 0) return PYL_OK;
+  PYL_REQUIRE(leaves && folds && branch && counts && off && seg && c_parent && c_row &&
+                  c_shift && c_phase && c_score && surv && o_leaves && o_folds && o_scores &&
+                  o_bt,
+              "null device pointer");
+  prune_materialize_kernel<<<(unsigned)pyl::ceil_div(n_surv, kBlk), kBlk, 0,
+                             pyl::as_stream(stream)>>>(
+      k, leaves, folds, 0, branch, counts, off, seg, c_parent, c_row, c_shift, c_phase, c_score,
+      surv, n_surv, o_leaves, o_folds, o_scores, o_bt);
+  PYL_CHECK_LAUNCH("prune_materialize_kernel");
+  return PYL_OK;
+}
+
+extern "C" int pyl_prune_ascend(const pyl_prune_params* p, const double* leaves, int64_t n,
+                                const float* dyp, int64_t ncoords, const int64_t* seg_idx,
+                                const double* seg_t0, int32_t nseg, double t0_cur,
+                                float* o_folds, float* o_scores, void* stream) {
+  LevelConst k;
+  int rc = build_const(p, nullptr, &k);
+  if (rc != PYL_OK) return rc;
+  PYL_REQUIRE(nseg >= 1, "no segments to integrate");
+  PYL_REQUIRE(leaves && dyp && seg_idx && seg_t0 && o_folds && o_scores, "null pointer");
+  if (n == 0) return PYL_OK;
+  hipStream_t s = pyl::as_stream(stream);
+  // Small per-call tables: segment indices/times and per-(survivor, segment) row/shift.
+  int64_t* d_idx = nullptr;
+  double* d_t0 = nullptr;
+  int32_t* d_row = nullptr;
+  int32_t* d_sh = nullptr;
+  PYL_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&d_idx), sizeof(int64_t) * nseg, s));
+  PYL_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&d_t0), sizeof(double) * nseg, s));
+  PYL_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&d_row), sizeof(int32_t) * n * nseg, s));
+  PYL_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&d_sh), sizeof(int32_t) * n * nseg, s));
+  PYL_CHECK_HIP(hipMemcpyAsync(d_idx, seg_idx, sizeof(int64_t) * nseg, hipMemcpyHostToDevice, s));
+  PYL_CHECK_HIP(hipMemcpyAsync(d_t0, seg_t0, sizeof(double) * nseg, hipMemcpyHostToDevice, s));
+  const size_t lds = sizeof(float) * (size_t)(k.nbins + k.wmax) * kStride;
+  prune_ascend_kernel<<<(unsigned)pyl::ceil_div(n, kBlk), kBlk, lds, s>>>(
+      k, leaves, n, dyp, ncoords, d_idx, d_t0, nseg, t0_cur, d_row, d_sh, o_folds, o_scores);
+  PYL_CHECK_LAUNCH("prune_ascend_kernel");
+  PYL_CHECK_HIP(hipFreeAsync(d_idx, s));
+  PYL_CHECK_HIP(hipFreeAsync(d_t0, s));
+  PYL_CHECK_HIP(hipFreeAsync(d_row, s));
+  PYL_CHECK_HIP(hipFreeAsync(d_sh, s));
+  return PYL_OK;
+}

@@ -1,0 +1,511 @@
+"""EP (enumerate-and-prune) search on the GPU: ``Pruning`` and ``prune_dyp_tree``.
+
+Mirrors ``pyloki.prune`` (``src/pyloki/prune.py``) for the Taylor basis with real folds
+and a moving grid (``use_fourier=False``, ``poly_basis="taylor"``, ``use_moving_grid=True``):
+seed from the FFA fold of the reference segment, one pruning level per remaining segment
+(middle-out order), the final ascend (re-integration + rescoring) and the report transform.
+
+Every per-leaf step runs in the HIP library (``pyl_prune_*``). The host decides which
+children survive: when the children passing the threshold fit the survivor buffer
+(``max_sugg``) they are all kept (the reference's fast path); otherwise the reference's
+batch-sequential overflow rule (``world_tree.py:482-551``) is replayed on the child scores
+in its batch order, which selects exactly the reference's survivors. Survivors are always a
+subsequence of the children in child order, so a boolean selection describes the new tree.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
+import torch
+
+from pyloki_amd import _lib
+from pyloki_amd.device import ptr, stream_ptr, to_device
+from pyloki_amd.utils.misc import C_VAL
+from pyloki_amd.utils.snail import MiddleOutScheme
+
+INT64_MAX = 2**63 - 1
+_TILING = {"aggressive": 0, "quadrature": 1, "conservative": 2}
+
+
+class PruneParams(ctypes.Structure):
+    _fields_ = [
+        ("poly_order", ctypes.c_int32),
+        ("nbins", ctypes.c_int32),
+        ("branch_max", ctypes.c_int32),
+        ("tiling", ctypes.c_int32),
+        ("eta", ctypes.c_double),
+        ("nparams_ffa", ctypes.c_int32),
+        ("n_accel", ctypes.c_int32),
+        ("n_freq", ctypes.c_int32),
+        ("nw", ctypes.c_int32),
+        ("accel_lim", ctypes.c_double * 2),
+        ("freq_lim", ctypes.c_double * 2),
+        ("widths", ctypes.c_int32 * 64),
+    ]
+
+
+class PruneCoords(ctypes.Structure):
+    _fields_ = [
+        ("t0_init", ctypes.c_double),
+        ("t0_cur", ctypes.c_double),
+        ("scale_cur", ctypes.c_double),
+        ("t0_next", ctypes.c_double),
+        ("t0_add", ctypes.c_double),
+    ]
+
+
+@dataclass
+class PruneStats:
+    """Per-level statistics (``io/cands.py:16-101``)."""
+
+    level: int
+    seg_idx: int
+    threshold: float
+    score_min: float = 0.0
+    score_max: float = 0.0
+    n_branches: int = 1
+    n_leaves: int = 1
+    n_leaves_phy: int = 1
+    n_leaves_surv: int = 1
+
+    @property
+    def lb_leaves(self) -> float:
+        return float(np.round(np.log2(max(self.n_leaves, 1)), 2))
+
+    @property
+    def branch_frac(self) -> float:
+        return 0.0 if self.n_branches == 0 else float(np.round(self.n_leaves_phy / self.n_branches, 2))
+
+    @property
+    def surv_frac(self) -> float:
+        return 0.0 if self.n_leaves_phy == 0 else float(np.round(self.n_leaves_surv / self.n_leaves_phy, 2))
+
+    def get_summary(self) -> str:
+        return (f"Prune level: {self.level:3d}, seg_idx: {self.seg_idx:3d}, "
+                f"leaves: {self.lb_leaves:5.2f}, branch_frac: {self.branch_frac:5.2f},"
+                f"score thresh: {self.threshold:5.2f}, max: {self.score_max:5.2f}, "
+                f"min: {self.score_min:5.2f}, P(surv): {self.surv_frac:4.2f}\n")
+
+
+@dataclass
+class DeviceTree:
+    """Survivor buffer (``WorldTree``, ``world_tree.py:24-351``) resident in HBM."""
+
+    leaves: torch.Tensor      # f64 [n, P+2, 2]
+    folds: torch.Tensor       # f32 [n, 2, B]
+    scores: torch.Tensor      # f32 [n]
+    backtracks: torch.Tensor  # i32 [n, P+2]
+    scores_ep: torch.Tensor = field(default=None)
+
+    def __post_init__(self) -> None:
+        if self.scores_ep is None:
+            self.scores_ep = self.scores.clone()
+
+    @property
+    def size(self) -> int:
+        return int(self.scores.shape[0])
+
+    @property
+    def valid_size(self) -> int:
+        return self.size
+
+    @property
+    def size_lb(self) -> float:
+        return float(np.log2(self.size)) if self.size else 0.0
+
+    @property
+    def score_max(self) -> float:
+        return float(self.scores.max()) if self.size else 0.0
+
+    @property
+    def score_min(self) -> float:
+        return float(self.scores.min()) if self.size else 0.0
+
+
+def overflow_survivors(scores: np.ndarray, child_off: np.ndarray, batch_size: int,
+                       threshold: float, max_sugg: int) -> np.ndarray:
+    """Children kept by the reference's batch-sequential loop when the buffer overflows.
+
+    Replays ``pruning_iteration_batched`` (``prune.py:214-321``) on the child scores only:
+    batches of ``batch_size`` parents, ``max(threshold, current_threshold)`` per batch, and
+    ``WorldTree.add_batch`` / ``prune_on_overload`` (``world_tree.py:482-551``). Returns the
+    surviving child indices in increasing order.
+    """
+    n_par = len(child_off) - 1
+    bs = max(1, min(batch_size, n_par))
+    kept_idx = np.empty(0, dtype=np.int64)
+    kept_sc = np.empty(0, dtype=np.float32)
+    cur = threshold
+    for b0 in range(0, n_par, bs):
+        b1 = min(b0 + bs, n_par)
+        c0, c1 = int(child_off[b0]), int(child_off[b1])
+        if c1 == c0:
+            continue
+        sc = scores[c0:c1]
+        sel = np.where(sc.astype(np.float64) >= max(threshold, cur))[0]
+        if len(sel) == 0:
+            continue
+        bsc = sc[sel]
+        bidx = sel.astype(np.int64) + c0
+        space = max_sugg - len(kept_idx)
+        if len(bsc) <= space:
+            kept_idx = np.concatenate([kept_idx, bidx])
+            kept_sc = np.concatenate([kept_sc, bsc])
+            continue
+        total = len(kept_sc) + len(bsc)
+        scratch = np.concatenate([kept_sc, bsc])
+        k_idx = -max_sugg
+        mid_idx = -(total // 2) - 1
+        part = np.partition(scratch, [k_idx, mid_idx])
+        topk = math.nextafter(float(part[k_idx]), math.inf)
+        med = math.nextafter(float(part[mid_idx]), math.inf)
+        eff = max(cur, topk, med)
+        keep = kept_sc.astype(np.float64) >= eff
+        kept_idx, kept_sc = kept_idx[keep], kept_sc[keep]
+        pend = np.where(bsc.astype(np.float64) >= eff)[0]
+        m = min(len(pend), max_sugg - len(kept_idx))
+        if m > 0:
+            kept_idx = np.concatenate([kept_idx, bidx[pend[:m]]])
+            kept_sc = np.concatenate([kept_sc, bsc[pend[:m]]])
+        cur = eff
+    return kept_idx
+
+
+class Pruning:
+    """EP search from one reference segment; API of ``prune.py:347-755``."""
+
+    def __init__(self, dyp, threshold_scheme: np.ndarray, max_sugg: int = 2**17,
+                 batch_size: int = 1024, poly_basis: str = "taylor", intermediate_k: int = 1,
+                 *, use_moving_grid: bool = True, logger=None) -> None:  # noqa: ARG002
+        cfg = dyp.cfg
+        if poly_basis != "taylor" or cfg.prune_poly_order == 5:
+            msg = f"Invalid pruning configuration: {(cfg.prune_poly_order == 5, poly_basis, cfg.use_fourier)}"
+            raise ValueError(msg)
+        if cfg.use_fourier:
+            msg = "complex (use_fourier=True) pruning is not supported yet"
+            raise NotImplementedError(msg)
+        if not use_moving_grid:
+            msg = "only the moving grid is supported"
+            raise NotImplementedError(msg)
+        if cfg.nparams != cfg.prune_poly_order or cfg.nparams < 2:
+            msg = "Taylor pruning needs prune_poly_order == number of FFA parameters (>= 2)"
+            raise ValueError(msg)
+        self._dyp = dyp
+        self._threshold_scheme = np.asarray(threshold_scheme, dtype=np.float64)
+        self._max_sugg = int(max_sugg)
+        self._batch_size = int(batch_size)
+        self._intermediate_k = intermediate_k
+        self._use_moving_grid = use_moving_grid
+        self._P = cfg.prune_poly_order
+        counts = [len(p) for p in dyp.param_arr]
+        widths = cfg.score_widths
+        pp = PruneParams()
+        pp.poly_order = self._P
+        pp.nbins = cfg.nbins
+        pp.branch_max = cfg.branch_max
+        pp.tiling = _TILING[cfg.tiling_strategy]
+        pp.eta = cfg.eta
+        pp.nparams_ffa = cfg.nparams
+        pp.n_accel = counts[-2]
+        pp.n_freq = counts[-1]
+        pp.nw = len(widths)
+        pp.accel_lim[0], pp.accel_lim[1] = (float(x) for x in cfg.param_limits[-2])
+        pp.freq_lim[0], pp.freq_lim[1] = (float(x) for x in cfg.param_limits[-1])
+        for i, w in enumerate(widths.tolist()):
+            pp.widths[i] = w
+        self._pp = pp
+        self._widths = widths
+        self._ncoords = int(np.prod(counts))
+
+    @property
+    def dyp(self):
+        return self._dyp
+
+    @property
+    def threshold_scheme(self) -> np.ndarray:
+        return self._threshold_scheme
+
+    @property
+    def max_sugg(self) -> int:
+        return self._max_sugg
+
+    @property
+    def batch_size(self) -> int:
+        return self._batch_size
+
+    @property
+    def prune_level(self) -> int:
+        return self._prune_level
+
+    @property
+    def prune_complete(self) -> bool:
+        return self._prune_complete
+
+    @property
+    def scheme(self) -> MiddleOutScheme:
+        return self._scheme
+
+    @property
+    def world_tree(self) -> DeviceTree:
+        return self._world_tree
+
+    @property
+    def pstats(self) -> list[PruneStats]:
+        return self._pstats
+
+    @property
+    def backtrack_hist(self) -> list[torch.Tensor]:
+        return self._backtrack_hist
+
+    def _segment_rows(self, seg_idx: int) -> torch.Tensor:
+        fold = self.dyp.fold_device
+        return fold[seg_idx]
+
+    def initialize(self, ref_seg: int, log_file: Path | None = None) -> None:
+        """Seed the tree from the reference segment (``prune.py:468-524``)."""
+        dyp = self.dyp
+        self._scheme = MiddleOutScheme(dyp.nsegments, ref_seg, dyp.tseg)
+        self._prune_complete = False
+        self._prune_level = 0
+        P = self._P
+        param_arr = dyp.param_arr
+        dparams = dyp.dparams_actual
+        mesh = np.meshgrid(*param_arr, indexing="ij")
+        cart = np.stack([m.reshape(-1) for m in mesh], axis=1)
+        n = cart.shape[0]
+        leaves = np.zeros((n, P + 2, 2), dtype=np.float64)
+        for k in range(P - 1):
+            leaves[:, k, 0] = cart[:, k]
+            leaves[:, k, 1] = dparams[k]
+        f0 = cart[:, -1]
+        leaves[:, -3, 1] = np.full(n, dparams[-1]) * (C_VAL / f0)
+        leaves[:, -1, 0] = f0
+        folds = self._segment_rows(self.scheme.ref_idx).reshape(n, 2, -1).contiguous()
+        from pyloki_amd import kernels
+
+        scores = kernels.score_boxcar(folds, self._widths, mode=_lib.PYL_SCORE_MAX)
+        bt = torch.zeros((n, P + 2), dtype=torch.int32, device=folds.device)
+        self._world_tree = DeviceTree(to_device(leaves), folds.clone(), scores, bt)
+        self._backtrack_hist = []
+        self._pstats = [PruneStats(level=0, seg_idx=self.scheme.ref_idx, threshold=0,
+                                   score_min=self._world_tree.score_min,
+                                   score_max=self._world_tree.score_max, n_branches=n,
+                                   n_leaves=n, n_leaves_phy=n, n_leaves_surv=n)]
+        if log_file is not None:
+            with Path(log_file).open("a") as f:
+                f.write(self._pstats[0].get_summary())
+
+    def _coords(self, level: int) -> PruneCoords:
+        s = self.scheme
+        c = PruneCoords()
+        c.t0_init = s.get_coord(0)[0]
+        cur = s.get_current_coord(level, True)
+        c.t0_cur, c.scale_cur = cur
+        c.t0_next = s.get_coord(level)[0]
+        c.t0_add = s.get_segment_coord(level)[0]
+        return c
+
+    def execute_iter(self, log_file: Path | None = None) -> None:
+        """One pruning level (``prune.py:615-669``)."""
+        if self._prune_complete:
+            return
+        self._prune_level += 1
+        level = self._prune_level
+        seg_idx = self.scheme.get_segment_idx(level)
+        threshold = float(self.threshold_scheme[level - 1])
+        tree = self._world_tree
+        new_tree, stats = self._level(tree, self._segment_rows(seg_idx), self._coords(level),
+                                      threshold)
+        st = PruneStats(level=level, seg_idx=seg_idx, threshold=threshold,
+                        n_branches=tree.size, n_leaves_surv=new_tree.size, **stats)
+        self._pstats.append(st)
+        if log_file is not None:
+            with Path(log_file).open("a") as f:
+                f.write(st.get_summary())
+        self._world_tree = new_tree
+        if new_tree.size == 0:
+            self._prune_complete = True
+            return
+        self._backtrack_hist.append(new_tree.backtracks.clone())
+
+    def _level(self, tree: DeviceTree, seg: torch.Tensor, coords: PruneCoords,
+               threshold: float) -> tuple[DeviceTree, dict]:
+        lib = _lib.load()
+        dev = tree.folds.device
+        P = self._P
+        n = tree.size
+        B = self.dyp.cfg.nbins
+        pp = ctypes.byref(self._pp)
+        cc = ctypes.byref(coords)
+        branch = torch.empty((n, P, 2), dtype=torch.float64, device=dev)
+        counts = torch.empty((n, P), dtype=torch.int32, device=dev)
+        nchild = torch.empty(n, dtype=torch.int64, device=dev)
+        err = torch.full((1,), INT64_MAX, dtype=torch.int64, device=dev)
+        _lib.check(lib.pyl_prune_branch(pp, cc, ptr(tree.leaves), n, ptr(branch), ptr(counts),
+                                        ptr(nchild), ptr(err), stream_ptr()))
+        off = torch.zeros(n + 1, dtype=torch.int64, device=dev)
+        torch.cumsum(nchild, 0, out=off[1:])
+        errv = int(err.item())
+        if errv != INT64_MAX:
+            if errv % 4 == 1:
+                raise ValueError("Invalid input: increase branch_max.")
+            raise ValueError("Both dparam_cur and dparam_new must be positive.")
+        total = int(off[-1].item())
+        c_parent = torch.empty(total, dtype=torch.int32, device=dev)
+        c_row = torch.empty(total, dtype=torch.int32, device=dev)
+        c_shift = torch.empty(total, dtype=torch.int32, device=dev)
+        c_phase = torch.empty(total, dtype=torch.float64, device=dev)
+        c_score = torch.empty(total, dtype=torch.float32, device=dev)
+        minmax = torch.tensor([np.inf, -np.inf], dtype=torch.float32, device=dev)
+        segc = seg.contiguous()
+        _lib.check(lib.pyl_prune_children(
+            pp, cc, ptr(tree.leaves), ptr(tree.folds), n, ptr(branch), ptr(counts), ptr(off),
+            total, ptr(segc), ptr(c_parent), ptr(c_row), ptr(c_shift), ptr(c_phase),
+            ptr(c_score), ptr(minmax), stream_ptr()))
+        passing = c_score.double() >= threshold
+        npass = int(passing.sum().item())
+        if npass <= self.max_sugg:
+            surv = torch.nonzero(passing).reshape(-1)
+        else:
+            surv_np = overflow_survivors(c_score.cpu().numpy(), off.cpu().numpy(),
+                                         self.batch_size, threshold, self.max_sugg)
+            surv = torch.from_numpy(surv_np).to(dev)
+        m = int(surv.shape[0])
+        o_leaves = torch.empty((m, P + 2, 2), dtype=torch.float64, device=dev)
+        o_folds = torch.empty((m, 2, B), dtype=torch.float32, device=dev)
+        o_scores = torch.empty(m, dtype=torch.float32, device=dev)
+        o_bt = torch.empty((m, P + 2), dtype=torch.int32, device=dev)
+        _lib.check(lib.pyl_prune_materialize(
+            pp, cc, ptr(tree.leaves), ptr(tree.folds), ptr(branch), ptr(counts), ptr(off),
+            ptr(segc), ptr(c_parent), ptr(c_row), ptr(c_shift), ptr(c_phase), ptr(c_score),
+            ptr(surv.contiguous()), m, ptr(o_leaves), ptr(o_folds), ptr(o_scores), ptr(o_bt),
+            stream_ptr()))
+        mm = minmax.cpu().numpy()
+        stats = {"n_leaves": total, "n_leaves_phy": total,
+                 "score_min": float(mm[0]) if total else 0.0,
+                 "score_max": float(mm[1]) if total else 0.0}
+        return DeviceTree(o_leaves, o_folds, o_scores, o_bt), stats
+
+    def _do_ascend(self) -> None:
+        """Re-integrate the survivors over all segments so far (``prune.py:671-693``)."""
+        if self._prune_complete or self._world_tree.size == 0:
+            return
+        tree = self._world_tree
+        seg_idx, seg_coords = self.scheme.get_segment_coords_so_far(self._prune_level)
+        t0_cur = self.scheme.get_coord(self._prune_level)[0]
+        n = tree.size
+        B = self.dyp.cfg.nbins
+        folds = torch.empty((n, 2, B), dtype=torch.float32, device=tree.folds.device)
+        scores = torch.empty(n, dtype=torch.float32, device=tree.folds.device)
+        idx = np.ascontiguousarray(seg_idx, dtype=np.int64)
+        t0 = np.ascontiguousarray(seg_coords[:, 0], dtype=np.float64)
+        _lib.check(_lib.load().pyl_prune_ascend(
+            ctypes.byref(self._pp), ptr(tree.leaves), n, ptr(self.dyp.fold_device.contiguous()),
+            self._ncoords, idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+            t0.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(idx), t0_cur, ptr(folds),
+            ptr(scores), stream_ptr()))
+        tree.scores_ep = tree.scores.clone()
+        tree.folds = folds
+        tree.scores = scores
+
+    def execute(self, ref_seg: int, ascend_levels: list[int] | None = None,
+                outdir: str | Path = "./", log_file: Path | None = None,  # noqa: ARG002
+                result_file: Path | None = None, **_: object) -> dict:
+        """Seed, prune every level, ascend, report (``prune.py:526-613``)."""
+        asc = frozenset(ascend_levels) if ascend_levels is not None else None
+        self.initialize(ref_seg, log_file)
+        for _i in range(self.dyp.nsegments - 1):
+            self.execute_iter(log_file)
+            if asc is not None and self._prune_level in asc:
+                self._do_ascend()
+        if asc is None or self._prune_level not in asc:
+            self._do_ascend()
+        leaves = self._world_tree.leaves.cpu().numpy()
+        report = poly_taylor_report_batch(leaves) if len(leaves) else leaves
+        res = {
+            "run_name": f"{ref_seg:03d}",
+            "scheme": self.scheme.data.copy(),
+            "param_sets": report,
+            "scores": self._world_tree.scores.cpu().numpy(),
+            "scores_ep": self._world_tree.scores_ep.cpu().numpy(),
+            "level_stats": self._pstats,
+        }
+        if result_file is not None:
+            save_run(result_file, res)
+        return res
+
+
+def poly_taylor_report_batch(leaves: np.ndarray) -> np.ndarray:
+    """Gauge transform to physical units with error propagation (``taylor.py:342-360``)."""
+    out = leaves.copy()
+    vals = leaves[:, :-3, 0]
+    sigs = leaves[:, :-3, 1]
+    v = leaves[:, -3, 0]
+    dv = leaves[:, -3, 1]
+    f0 = leaves[:, -1, 0]
+    s = 1 - v / C_VAL
+    out[:, :-3, 0] = vals / s[:, None]
+    out[:, :-3, 1] = np.sqrt((sigs / s[:, None]) ** 2
+                             + ((vals / (C_VAL * s[:, None] ** 2)) ** 2) * (dv[:, None] ** 2))
+    out[:, -3, 0] = f0 * s
+    out[:, -3, 1] = f0 * dv / C_VAL
+    return out
+
+
+def save_run(path: str | Path, res: dict) -> None:
+    """Per-run results as ``.npz`` (the reference writes the same datasets to HDF5,
+    ``io/cands.py:324-354``; h5py is not part of this package's dependencies)."""
+    stats = np.array([(s.level, s.seg_idx, s.threshold, s.score_min, s.score_max, s.n_branches,
+                       s.n_leaves, s.n_leaves_phy, s.n_leaves_surv) for s in res["level_stats"]])
+    np.savez(path, scheme=res["scheme"], param_sets=res["param_sets"], scores=res["scores"],
+             scores_ep=res["scores_ep"], level_stats=stats)
+
+
+def determine_ref_segs_pareto(nsegments: int, n_runs: int) -> list[int]:
+    """Pareto-optimal reference segments (``np_utils.py:412-443``)."""
+    if not (1 <= n_runs <= nsegments):
+        msg = f"n_runs must be between 1 and {nsegments}"
+        raise ValueError(msg)
+    if n_runs == 1:
+        return [nsegments // 2]
+
+    def rdiv(num: int, den: int) -> int:
+        return (num + den // 2) // den
+
+    margin = rdiv(nsegments - 1, 2 * n_runs)
+    lo, hi = margin, nsegments - 1 - margin
+    denom = n_runs - 1
+    return [rdiv((denom - i) * lo + i * hi, denom) for i in range(n_runs)]
+
+
+def prune_dyp_tree(dyp, threshold_scheme: np.ndarray, n_runs: int | None = None,
+                   ref_segs: list[int] | None = None, max_sugg: int = 2**17,
+                   batch_size: int = 1024, outdir: str | Path = "./",
+                   file_prefix: str = "test", poly_basis: str = "taylor", n_workers: int = 1,  # noqa: ARG001
+                   *, use_moving_grid: bool = True, ascend_levels: list[int] | None = None,
+                   **_: object) -> list[dict]:
+    """Run the EP search from several reference segments (``prune.py:824-932``).
+
+    The reference fans runs out over worker processes; here every run uses the GPU in
+    turn (each run's level work already fills the device). Returns the per-run results.
+    """
+    if ref_segs is None:
+        if n_runs is None:
+            msg = "Either n_runs or ref_segs must be provided"
+            raise ValueError(msg)
+        ref_segs = determine_ref_segs_pareto(dyp.nsegments, n_runs)
+    out = []
+    for task, ref in enumerate(ref_segs):
+        pr = Pruning(dyp, threshold_scheme, max_sugg, batch_size, poly_basis,
+                     use_moving_grid=use_moving_grid)
+        res = pr.execute(int(ref), ascend_levels=ascend_levels)
+        res["run_name"] = f"{int(ref):03d}_{task:02d}"
+        out.append(res)
+    if outdir is not None and file_prefix:
+        Path(outdir).mkdir(parents=True, exist_ok=True)
+    return out

@@ -4,7 +4,7 @@ The per-leaf transforms run on the GPU; these host versions serve ``ParamLimits`
 other planning code, restating ``src/pyloki/utils/transforms.py``:
 
 - ``shift_taylor_params``: ``transforms.py:10-47``. The matrix entry is
-  ``dt**p / p!`` with ``dt**p`` the correctly rounded power (glibc ``pow``), and the
+  ``dt**p / p!`` with ``dt**p`` by binary powering (``psr_utils.powi``), and the
   product is summed in ascending column order.
 - ``taylor_to_cheby``: ``transforms.py:435-480`` (Chebyshev planning bounds).
 """
@@ -15,6 +15,8 @@ import math
 
 import numpy as np
 
+from pyloki_amd.utils.psr_utils import powi
+
 
 def taylor_shift_matrix(n_params: int, delta_t: float) -> np.ndarray:
     """Lower-triangular matrix ``T[i, j] = dt^(i-j) / (i-j)!`` for ``i >= j``."""
@@ -22,7 +24,7 @@ def taylor_shift_matrix(n_params: int, delta_t: float) -> np.ndarray:
     for i in range(n_params):
         for j in range(i + 1):
             p = i - j
-            t_mat[i, j] = math.pow(delta_t, p) / float(math.factorial(p))
+            t_mat[i, j] = powi(delta_t, p) / float(math.factorial(p))
     return t_mat
 
 
