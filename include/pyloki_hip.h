@@ -78,6 +78,15 @@ int pyl_ffa_resolve(int32_t nparams, const double* params_dev, const int64_t* co
 int pyl_ffa_shift_add(const float* fold_in_dev, int64_t nseg_in, int64_t ncoords_prev,
                       const int32_t* table_dev, int64_t ncoords_cur, int32_t nbins,
                       float* fold_out_dev, void* stream);
+/* pyl_ffa_shift_add_score: the shift-add of one level fused with the per-width boxcar S/N
+ * of every output row (unify_fold dyn_pffa.py:481-489 + get_fold_norm ffa.py:411 +
+ * boxcar_snr_nd scoring.py:156-165), without storing the level's fold: snr_out f32
+ * [nseg_in/2][ncoords_cur][nw]. Used for a series' last level (search.py:44-47).
+ * nbins in {32, 64, 128}; widths as for pyl_score_boxcar. */
+int pyl_ffa_shift_add_score(const float* fold_in_dev, int64_t nseg_in, int64_t ncoords_prev,
+                            const int32_t* table_dev, int64_t ncoords_cur, int32_t nbins,
+                            const int64_t* widths, int32_t nw, double stdnoise,
+                            float* snr_out_dev, void* stream);
 int pyl_ffa_level(int32_t nparams, const double* params_dev, const int64_t* counts_cur,
                   const int64_t* counts_prev, const double* param_limits, int32_t ffa_level,
                   double tseg_brute, int32_t nbins, const float* fold_in_dev, int64_t nseg_in,

@@ -92,6 +92,14 @@ def lib() -> ctypes.CDLL:
         L.orc_snr_score_batch.argtypes = [_f32p, _i64, _i64, _i64p, _i64, _f32p]
         L.orc_fold_norm.restype = None
         L.orc_fold_norm.argtypes = [_f32p, _i64, _i64, _f32p]
+        L.orc_brutefold_f64.restype = None
+        L.orc_brutefold_f64.argtypes = [_f32p, _f32p, _i64, _f64p, _i64, _i64, _i64, _dbl,
+                                        _dbl, _f64p]
+        L.orc_unify_fold_f64.restype = None
+        L.orc_unify_fold_f64.argtypes = [_f64p, _i64, _int, _i64p, _f64p, _i64p, _f64p, _int,
+                                         _dbl, _i64, _f64p]
+        L.orc_boxcar_snr_f64.restype = None
+        L.orc_boxcar_snr_f64.argtypes = [_f64p, _i64, _i64, _i64p, _i64, _f64p]
         _lib = L
     return _lib
 
@@ -216,6 +224,39 @@ def ffa_search_snr(ts_e, ts_v, cfg) -> np.ndarray:
     norm = fold_norm(fold[0])
     snr = boxcar_snr(norm, cfg.score_widths, 1.0)
     return snr.reshape(*counts, len(cfg.score_widths))
+
+
+def ffa_search_snr_f64(ts_e, ts_v, cfg) -> np.ndarray:
+    """FP64 reference of the FFA S/N grid: same resolution (indices, shifts) as the FP32
+    path, FP64 data arithmetic throughout. Used to state the error of FP32 and of
+    reduced-precision fold storage."""
+    L = lib()
+    ts_e = np.ascontiguousarray(ts_e, dtype=np.float32)
+    ts_v = np.ascontiguousarray(ts_v, dtype=np.float32)
+    pa = cfg.get_param_arr(cfg.get_dparams(0))
+    t_ref = 0.0 if cfg.nparams == 1 else cfg.bseg_brute * cfg.tsamp / 2
+    freqs = np.ascontiguousarray(pa[-1], dtype=np.float64)
+    nseg = -(-len(ts_e) // cfg.bseg_brute)
+    fold = np.empty((nseg, len(freqs), 2, cfg.nbins), dtype=np.float64)
+    L.orc_brutefold_f64(ts_e, ts_v, len(ts_e), freqs, len(freqs), cfg.bseg_brute, cfg.nbins,
+                        cfg.tsamp, t_ref, fold.reshape(-1))
+    counts_prev = np.ascontiguousarray(cfg.get_param_grid_count(0), dtype=np.int64)
+    for level in range(1, cfg.niters_ffa + 1):
+        pa = cfg.get_param_arr(cfg.get_dparams(level))
+        counts_cur = np.array([len(a) for a in pa], dtype=np.int64)
+        out = np.empty((fold.shape[0] // 2, int(np.prod(counts_cur)), 2, cfg.nbins))
+        L.orc_unify_fold_f64(fold.reshape(-1), fold.shape[0], len(pa), counts_prev,
+                             np.ascontiguousarray(np.concatenate(pa)), counts_cur,
+                             np.ascontiguousarray(cfg.param_limits).reshape(-1), level,
+                             cfg.tseg_brute, cfg.nbins, out.reshape(-1))
+        fold = out
+        counts_prev = counts_cur
+    w = np.ascontiguousarray(cfg.score_widths, dtype=np.int64)
+    n = fold.shape[1]
+    snr = np.empty((n, len(w)))
+    L.orc_boxcar_snr_f64(np.ascontiguousarray(fold[0]).reshape(-1), n, cfg.nbins, w, len(w),
+                         snr.reshape(-1))
+    return snr.reshape(*[len(a) for a in pa], len(w))
 
 
 def cpu_count() -> int:

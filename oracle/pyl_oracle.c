@@ -290,3 +290,94 @@ void orc_fold_norm(const float* folds, int64_t n, int64_t nbins, float* out) {
         for (int64_t j = 0; j < nbins; ++j)
             out[p * nbins + j] = folds[p * 2 * nbins + j] / sqrtf(folds[p * 2 * nbins + nbins + j]);
 }
+
+/* ---- FP64 reference of the FFA (for the error of reduced-precision fold storage) ----
+ * Same algorithm as orc_brutefold / orc_unify_fold / boxcar S/N, with every fold
+ * accumulation, normalisation and prefix sum in FP64. Grid indices and shifts are the
+ * same FP64 resolution as the FP32 path, so only the data arithmetic differs. */
+void orc_brutefold_f64(const float* ts_e, const float* ts_v, int64_t nsamps,
+                       const double* freqs, int64_t nfreq, int64_t seg_len, int64_t nbins,
+                       double tsamp, double t_ref, double* out) {
+    int64_t nseg = (nsamps + seg_len - 1) / seg_len;
+    memset(out, 0, sizeof(double) * (size_t)(nseg * nfreq * 2 * nbins));
+    for (int64_t f = 0; f < nfreq; ++f)
+        for (int64_t i = 0; i < seg_len; ++i) {
+            double pt = (double)i * tsamp - t_ref;
+            int64_t b = orc_phase_idx_int(pt, freqs[f], nbins, 0.0);
+            for (int64_t s = 0; s < nseg; ++s) {
+                int64_t k = s * seg_len + i;
+                if (k >= nsamps) continue;
+                double* o = out + (s * nfreq + f) * 2 * nbins;
+                o[b] += (double)ts_e[k];
+                o[nbins + b] += (double)ts_v[k];
+            }
+        }
+}
+
+void orc_unify_fold_f64(const double* fold_in, int64_t nseg_in, int P, const int64_t* counts_prev,
+                        const double* params, const int64_t* counts_cur, const double* limits,
+                        int level, double tseg_brute, int64_t nbins, double* fold_out) {
+    int64_t ncur = 1, nprev = 1;
+    for (int k = 0; k < P; ++k) { ncur *= counts_cur[k]; nprev *= counts_prev[k]; }
+    int64_t offs[8];
+    offs[0] = 0;
+    for (int k = 1; k < P; ++k) offs[k] = offs[k - 1] + counts_cur[k - 1];
+    int64_t nseg_out = nseg_in / 2, rowlen = 2 * nbins;
+    for (int64_t j = 0; j < ncur; ++j) {
+        double pset[8];
+        int64_t r = j;
+        for (int k = P - 1; k >= 0; --k) {
+            int64_t ik = r % counts_cur[k];
+            r /= counts_cur[k];
+            pset[k] = params[offs[k] + ik];
+        }
+        int64_t row[2], sh[2];
+        for (int latter = 0; latter < 2; ++latter) {
+            int64_t idx[8];
+            double ph;
+            orc_ffa_resolve(P, pset, counts_prev, limits, level, latter, tseg_brute, nbins, idx, &ph);
+            int64_t flat = 0;
+            for (int k = 0; k < P; ++k) flat = flat * counts_prev[k] + idx[k];
+            row[latter] = flat;
+            sh[latter] = orc_shift_int(ph, nbins);
+        }
+        for (int64_t q = 0; q < nseg_out; ++q) {
+            const double* t = fold_in + ((2 * q) * nprev + row[0]) * rowlen;
+            const double* h = fold_in + ((2 * q + 1) * nprev + row[1]) * rowlen;
+            double* o = fold_out + (q * ncur + j) * rowlen;
+            for (int64_t b = 0; b < nbins; ++b) {
+                int64_t i1 = ((b - sh[0]) % nbins + nbins) % nbins;
+                int64_t i2 = ((b - sh[1]) % nbins + nbins) % nbins;
+                o[b] = t[i1] + h[i2];
+                o[nbins + b] = t[nbins + i1] + h[nbins + i2];
+            }
+        }
+    }
+}
+
+/* Boxcar S/N in FP64 of FP64 (e, v) rows: per width. */
+void orc_boxcar_snr_f64(const double* folds, int64_t n, int64_t nbins, const int64_t* widths,
+                        int64_t nw, double* out) {
+    int64_t wmax = 0;
+    for (int64_t k = 0; k < nw; ++k) if (widths[k] > wmax) wmax = widths[k];
+    double* ps = (double*)malloc(sizeof(double) * (size_t)(nbins + wmax));
+    for (int64_t p = 0; p < n; ++p) {
+        const double* e = folds + p * 2 * nbins;
+        const double* v = e + nbins;
+        double run = 0.0;
+        for (int64_t i = 0; i < nbins; ++i) { run += e[i] / sqrt(v[i]); ps[i] = run; }
+        for (int64_t j = 0; j < wmax; ++j) ps[nbins + j] = ps[j] + run;
+        for (int64_t k = 0; k < nw; ++k) {
+            int64_t w = widths[k];
+            double h = sqrt((double)(nbins - w) / (double)(nbins * w));
+            double b = (double)w * h / (double)(nbins - w);
+            double dmax = -DBL_MAX;
+            for (int64_t i = 0; i < nbins; ++i) {
+                double d = ps[i + w] - ps[i];
+                if (d > dmax) dmax = d;
+            }
+            out[p * nw + k] = (h + b) * dmax - b * run;
+        }
+    }
+    free(ps);
+}

@@ -95,6 +95,36 @@ __device__ __host__ inline double powi(double a, int b) {
 // transforms.py:41-43: T[i][j] = dt**(i-j) / (i-j)! for j <= i, else 0 (n x n, row-major).
 void taylor_matrix(int n, double dt, double* out);
 
+// BF16 storage (upper 16 bits of an FP32), round-to-nearest-even on the way down; the
+// FFA's reduced-precision fold storage. Values are always finite here.
+__device__ __host__ inline uint16_t f32_to_bf16(float f) {
+  uint32_t u;
+  __builtin_memcpy(&u, &f, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+__device__ __host__ inline float bf16_to_f32(uint16_t h) {
+  uint32_t u = (uint32_t)h << 16;
+  float f;
+  __builtin_memcpy(&f, &u, 4);
+  return f;
+}
+
+// Storage-type traits of a fold element: FP32 or BF16 (FP32 arithmetic either way).
+template <typename T>
+struct Store;
+template <>
+struct Store<float> {
+  __device__ __host__ static float load(float v) { return v; }
+  __device__ __host__ static float store(float v) { return v; }
+};
+template <>
+struct Store<uint16_t> {
+  __device__ __host__ static float load(uint16_t v) { return bf16_to_f32(v); }
+  __device__ __host__ static uint16_t store(float v) { return f32_to_bf16(v); }
+};
+
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 }  // namespace pyl

@@ -20,7 +20,7 @@
 //   the 8 XCDs in contiguous slabs: neighbours' rows are reused from the same L2.
 #include <algorithm>
 
-#include "pyl_common.hpp"
+#include "pyl_score.hpp"
 
 namespace {
 
@@ -182,6 +182,54 @@ __global__ void ffa_shift_add_generic_kernel(const float* __restrict__ in, int64
   }
 }
 
+
+// Last FFA level fused with scoring: the output row of (segment pair q, set j) is formed
+// in registers, normalised (e / sqrt(v), ffa.py:411) and scored (scoring.py:99-165)
+// without being stored. A block owns 64 output rows of one pair; 16 lanes cover one row
+// (4 bins x 2 components per lane, float4 loads with the rotation of rotated_chunk), so a
+// wavefront forms 4 rows per step; the normalised rows are transposed into LDS and every
+// thread then scans and scores one row.
+template <int B>
+__global__ void __launch_bounds__(pyl::kScoreBlock)
+ffa_shift_add_score_kernel(const float* __restrict__ in, int64_t ncoords_prev,
+                           const int4* __restrict__ table, int64_t ncoords_cur,
+                           int64_t nblk_j, pyl::ScoreConst k, float* __restrict__ out) {
+  constexpr int CPR = B / 4;                 // float4 chunks per component
+  constexpr int RPS = pyl::kScoreBlock / CPR;  // rows formed per step
+  extern __shared__ float S[];
+  __shared__ int4 s_tab[pyl::kScoreBlock];
+  const int tid = threadIdx.x;
+  const int64_t q = blockIdx.x / nblk_j;
+  const int64_t j0 = (blockIdx.x - q * nblk_j) * pyl::kScoreBlock;
+  const int nvalid = (int)std::min<int64_t>(pyl::kScoreBlock, ncoords_cur - j0);
+  if (tid < nvalid) s_tab[tid] = table[j0 + tid];
+  __syncthreads();
+  constexpr int64_t E = 2 * B;
+  const int c4 = tid % CPR;
+  const int rsub = tid / CPR;
+#pragma unroll 2
+  for (int r0 = 0; r0 < nvalid; r0 += RPS) {
+    const int r = r0 + rsub;
+    if (r < nvalid) {
+      const int4 tb = s_tab[r];
+      const float* tail = in + ((2 * q) * ncoords_prev + tb.x) * E;
+      const float* head = in + ((2 * q + 1) * ncoords_prev + tb.z) * E;
+      const float4 te = rotated_chunk<B>(tail, c4, tb.y);
+      const float4 he = rotated_chunk<B>(head, c4, tb.w);
+      const float4 tv = rotated_chunk<B>(tail + B, c4, tb.y);
+      const float4 hv = rotated_chunk<B>(head + B, c4, tb.w);
+      const int b = 4 * c4;
+      S[(b + 0) * pyl::kScoreStride + r] = (te.x + he.x) / sqrtf(tv.x + hv.x);
+      S[(b + 1) * pyl::kScoreStride + r] = (te.y + he.y) / sqrtf(tv.y + hv.y);
+      S[(b + 2) * pyl::kScoreStride + r] = (te.z + he.z) / sqrtf(tv.z + hv.z);
+      S[(b + 3) * pyl::kScoreStride + r] = (te.w + he.w) / sqrtf(tv.w + hv.w);
+    }
+  }
+  __syncthreads();
+  if (tid < nvalid)
+    pyl::score_column<true>(k, B, S, tid, out + (q * ncoords_cur + j0 + tid) * k.nw);
+}
+
 }  // namespace
 
 extern "C" int pyl_ffa_resolve(int32_t nparams, const double* params_dev,
@@ -270,6 +318,43 @@ extern "C" int pyl_ffa_shift_add(const float* fold_in, int64_t nseg_in, int64_t 
     }
   }
   PYL_CHECK_LAUNCH("ffa_shift_add_kernel");
+  return PYL_OK;
+}
+
+
+template <int B>
+static void launch_shift_add_score(const float* in, int64_t nprev, const int4* tb, int64_t ncur,
+                                   int64_t nseg_out, const pyl::ScoreConst& k, float* out,
+                                   hipStream_t s) {
+  const int64_t nblk_j = pyl::ceil_div(ncur, pyl::kScoreBlock);
+  const size_t lds = sizeof(float) * (size_t)(B + k.wmax) * pyl::kScoreStride;
+  ffa_shift_add_score_kernel<B><<<(unsigned)(nblk_j * nseg_out), pyl::kScoreBlock, lds, s>>>(
+      in, nprev, tb, ncur, nblk_j, k, out);
+}
+
+extern "C" int pyl_ffa_shift_add_score(const float* fold_in, int64_t nseg_in,
+                                       int64_t ncoords_prev, const int32_t* table_dev,
+                                       int64_t ncoords_cur, int32_t nbins, const int64_t* widths,
+                                       int32_t nw, double stdnoise, float* snr_out,
+                                       void* stream) {
+  PYL_REQUIRE(fold_in && table_dev && snr_out, "null device pointer");
+  PYL_REQUIRE(nseg_in >= 2 && nseg_in % 2 == 0, "nsegments must be even and >= 2, got %lld",
+              (long long)nseg_in);
+  PYL_REQUIRE(ncoords_prev > 0 && ncoords_cur > 0, "empty parameter grid");
+  PYL_REQUIRE(nbins == 32 || nbins == 64 || nbins == 128,
+              "fused level + score supports nbins in {32, 64, 128}, got %d", nbins);
+  pyl::ScoreConst k;
+  int rc = pyl::build_score_const(nbins, widths, nw, stdnoise, &k);
+  if (rc != PYL_OK) return rc;
+  hipStream_t s = pyl::as_stream(stream);
+  const int64_t nseg_out = nseg_in / 2;
+  const int4* tb = reinterpret_cast<const int4*>(table_dev);
+  switch (nbins) {
+    case 32: launch_shift_add_score<32>(fold_in, ncoords_prev, tb, ncoords_cur, nseg_out, k, snr_out, s); break;
+    case 64: launch_shift_add_score<64>(fold_in, ncoords_prev, tb, ncoords_cur, nseg_out, k, snr_out, s); break;
+    default: launch_shift_add_score<128>(fold_in, ncoords_prev, tb, ncoords_cur, nseg_out, k, snr_out, s); break;
+  }
+  PYL_CHECK_LAUNCH("ffa_shift_add_score_kernel");
   return PYL_OK;
 }
 

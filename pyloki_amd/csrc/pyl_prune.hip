@@ -25,19 +25,18 @@
 #include <climits>
 #include <cmath>
 
-#include "pyl_common.hpp"
+#include "pyl_score.hpp"
 
 namespace {
 
 constexpr int kMaxP = 5;
 constexpr int kMaxN = kMaxP + 1;
-constexpr int kMaxW = PYL_MAX_WIDTHS;
-constexpr int kBlk = 64;
-constexpr int kStride = kBlk + 1;
+constexpr int kBlk = pyl::kScoreBlock;
+constexpr int kStride = pyl::kScoreStride;
 
 struct LevelConst {
   int32_t P, nbins, branch_max, tiling;
-  int32_t nparams_ffa, n_accel, n_freq, nw, wmax;
+  int32_t nparams_ffa, n_accel, n_freq;
   double eta;
   double dpf[kMaxP];          // poly_taylor_step_f(P, scale_cur) row order d_P..d_1
   double fac[kMaxP];          // shift factors, same order
@@ -46,9 +45,7 @@ struct LevelConst {
   double tnext[kMaxN * kMaxN];
   double dt_add_init;         // t0_add - t0_init
   double amin, fmin, sa, sf;  // accel/freq grid origin and inverse step
-  int32_t widths[kMaxW];
-  double hpb[kMaxW];
-  double bb[kMaxW];
+  pyl::ScoreConst sc;         // boxcar widths and template constants
 };
 
 int build_const(const pyl_prune_params* p, const pyl_prune_coords* c, LevelConst* k) {
@@ -56,7 +53,6 @@ int build_const(const pyl_prune_params* p, const pyl_prune_coords* c, LevelConst
   PYL_REQUIRE(p->poly_order >= 2 && p->poly_order <= kMaxP, "poly_order must be in [2, %d]",
               kMaxP);
   PYL_REQUIRE(p->nbins > 1, "nbins must be > 1");
-  PYL_REQUIRE(p->nw >= 1 && p->nw <= kMaxW, "number of widths must be in [1, %d]", kMaxW);
   PYL_REQUIRE(p->nparams_ffa >= 2 && p->nparams_ffa <= p->poly_order,
               "FFA grid must have 2..poly_order parameters");
   PYL_REQUIRE(p->n_accel >= 1 && p->n_freq >= 1, "empty FFA grid");
@@ -94,19 +90,12 @@ int build_const(const pyl_prune_params* p, const pyl_prune_coords* c, LevelConst
   k->sa = p->accel_lim[1] > p->accel_lim[0] ? p->n_accel / (p->accel_lim[1] - p->accel_lim[0])
                                              : 0.0;
   k->sf = p->freq_lim[1] > p->freq_lim[0] ? p->n_freq / (p->freq_lim[1] - p->freq_lim[0]) : 0.0;
-  k->nw = p->nw;
-  k->wmax = 0;
-  for (int w = 0; w < p->nw; ++w) {
-    const int64_t wd = p->widths[w];
-    PYL_REQUIRE(wd >= 1 && wd < p->nbins, "width %lld outside [1, nbins-1]", (long long)wd);
-    k->widths[w] = (int32_t)wd;
-    k->wmax = std::max<int32_t>(k->wmax, (int32_t)wd);
-    const int64_t size_w = p->nbins - wd;
-    const double h = std::sqrt((double)size_w / (double)(p->nbins * wd));
-    const double b = (double)wd * h / (double)size_w;
-    k->hpb[w] = h + b;
-    k->bb[w] = b;
-  }
+  int64_t widths[PYL_MAX_WIDTHS];
+  PYL_REQUIRE(p->nw >= 1 && p->nw <= PYL_MAX_WIDTHS, "number of widths must be in [1, %d]",
+              PYL_MAX_WIDTHS);
+  for (int w = 0; w < p->nw; ++w) widths[w] = p->widths[w];
+  int rc = pyl::build_score_const(p->nbins, widths, p->nw, 1.0, &k->sc);
+  if (rc != PYL_OK) return rc;
   return PYL_OK;
 }
 
@@ -227,30 +216,10 @@ __device__ __forceinline__ double mat_row(const double* __restrict__ T, const do
   return acc;
 }
 
-// Sequential scan + boxcar S/N of the profile held in column `tid` of S (scoring.py:249-282).
-__device__ float score_column(const LevelConst& k, float* S, int tid) {
-  const int B = k.nbins;
-  float run = S[tid];
-  for (int i = 1; i < B; ++i) {
-    run = run + S[i * kStride + tid];
-    S[i * kStride + tid] = run;
-  }
-  const float total = run;
-  for (int j = 0; j < k.wmax; ++j) S[(B + j) * kStride + tid] = S[j * kStride + tid] + total;
-  double best = -(double)FLT_MAX;
-  for (int w = 0; w < k.nw; ++w) {
-    const int wd = k.widths[w];
-    float dmax = -FLT_MAX;
-    for (int i = 0; i < B; ++i) {
-      const float d = S[(i + wd) * kStride + tid] - S[i * kStride + tid];
-      dmax = d > dmax ? d : dmax;
-    }
-    const double t1 = k.hpb[w] * (double)dmax;
-    const double t2 = k.bb[w] * (double)total;
-    const double snr = t1 - t2;
-    best = snr > best ? snr : best;
-  }
-  return (float)best;
+// Sequential scan + boxcar S/N (max over widths) of the profile in column `tid` of S
+// (scoring.py:249-282).
+__device__ __forceinline__ float score_column(const LevelConst& k, float* S, int tid) {
+  return (float)pyl::score_column<false>(k.sc, k.nbins, S, tid, nullptr);
 }
 
 __global__ void __launch_bounds__(kBlk)
@@ -502,7 +471,7 @@ extern "C" int pyl_prune_children(const pyl_prune_params* p, const pyl_prune_coo
               "null device pointer");
   PYL_REQUIRE((int64_t)p->n_accel * p->n_freq <= 0x7fffffff, "FFA grid too large");
   if (n_child == 0) return PYL_OK;
-  const size_t lds = sizeof(float) * (size_t)(k.nbins + k.wmax) * kStride;
+  const size_t lds = sizeof(float) * (size_t)(k.nbins + k.sc.wmax) * kStride;
   PYL_REQUIRE(lds <= 64 * 1024, "nbins too large for the EP scorer (%d)", k.nbins);
   prune_children_kernel<<<(unsigned)pyl::ceil_div(n_child, kBlk), kBlk, lds,
                           pyl::as_stream(stream)>>>(k, leaves, folds, n_par, branch, counts,
@@ -560,7 +529,7 @@ extern "C" int pyl_prune_ascend(const pyl_prune_params* p, const double* leaves,
   PYL_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&d_sh), sizeof(int32_t) * n * nseg, s));
   PYL_CHECK_HIP(hipMemcpyAsync(d_idx, seg_idx, sizeof(int64_t) * nseg, hipMemcpyHostToDevice, s));
   PYL_CHECK_HIP(hipMemcpyAsync(d_t0, seg_t0, sizeof(double) * nseg, hipMemcpyHostToDevice, s));
-  const size_t lds = sizeof(float) * (size_t)(k.nbins + k.wmax) * kStride;
+  const size_t lds = sizeof(float) * (size_t)(k.nbins + k.sc.wmax) * kStride;
   prune_ascend_kernel<<<(unsigned)pyl::ceil_div(n, kBlk), kBlk, lds, s>>>(
       k, leaves, n, dyp, ncoords, d_idx, d_t0, nseg, t0_cur, d_row, d_sh, o_folds, o_scores);
   PYL_CHECK_LAUNCH("prune_ascend_kernel");

@@ -1,0 +1,66 @@
+// Boxcar S/N pieces shared by the scorer, the fused FFA level + score kernel and the EP
+// kernels (detection/scoring.py:42-165, 249-282).
+//
+// A block owns up to 64 profiles; profile r's normalised values sit in column r of an LDS
+// array S[(nbins + wmax)][kScoreStride] (stride padded to 65 words so that both the
+// row-wise fills, lane = bin, and the column-wise scans, lane = profile, are free of bank
+// conflicts). The scan is the reference's sequential FP32 prefix sum; per width
+//   snr = ((h + b) * max_i(S[i+w] - S[i]) - b * S[B-1]) * inv_stdnoise   (FP64, no FMA),
+// with h = sqrt((B-w)/(B*w)) and b = w*h/(B-w) computed on the host in FP64.
+#pragma once
+
+#include <cfloat>
+
+#include "pyl_common.hpp"
+
+namespace pyl {
+
+constexpr int kScoreBlock = 64;
+constexpr int kScoreStride = kScoreBlock + 1;
+constexpr int kScoreMaxW = 64;
+
+struct ScoreConst {
+  int32_t nw;
+  int32_t wmax;
+  int32_t widths[kScoreMaxW];
+  double hpb[kScoreMaxW];  // h + b
+  double bb[kScoreMaxW];   // b
+  double inv_std;
+};
+
+// Host: validate widths and fill the per-width constants (defined in pyl_score.hip).
+int build_score_const(int nbins, const int64_t* widths, int nw, double stdnoise,
+                      ScoreConst* sc);
+
+// In-place sequential scan of column `tid` and its one-wrap extension, then the S/N of
+// every width: written to out[w] when kPerWidth, and the FP64 maximum returned
+// (snr_score_batch_func's max, before the final FP32 rounding).
+template <bool kPerWidth>
+__device__ inline double score_column(const ScoreConst& k, int nbins, float* S, int tid,
+                                      float* out) {
+  float run = S[tid];
+  for (int i = 1; i < nbins; ++i) {
+    run = run + S[i * kScoreStride + tid];
+    S[i * kScoreStride + tid] = run;
+  }
+  const float total = run;
+  for (int j = 0; j < k.wmax; ++j)
+    S[(nbins + j) * kScoreStride + tid] = S[j * kScoreStride + tid] + total;
+  double best = -(double)FLT_MAX;
+  for (int w = 0; w < k.nw; ++w) {
+    const int wd = k.widths[w];
+    float dmax = -FLT_MAX;
+    for (int i = 0; i < nbins; ++i) {
+      const float d = S[(i + wd) * kScoreStride + tid] - S[i * kScoreStride + tid];
+      dmax = d > dmax ? d : dmax;
+    }
+    const double t1 = k.hpb[w] * (double)dmax;
+    const double t2 = k.bb[w] * (double)total;
+    const double snr = t1 - t2;
+    if (kPerWidth) out[w] = (float)(snr * k.inv_std);
+    best = snr > best ? snr : best;
+  }
+  return best;
+}
+
+}  // namespace pyl

@@ -236,14 +236,17 @@ def compute_ffa(tseries: TimeSeries, search_cfg: PulsarSearchConfig, *, quiet: b
 
 
 def ffa_batch(plan: FFAPlan, series: list[tuple[torch.Tensor, torch.Tensor]],
-              widths: np.ndarray, stdnoise: float = 1.0) -> list[torch.Tensor]:
+              widths: np.ndarray, stdnoise: float = 1.0, *, fused: bool = True
+              ) -> list[torch.Tensor]:
     """Fold and score many series of (possibly) different lengths in one pass per level.
 
     All series share ``plan`` (same ``bseg_brute`` and grids); series are ordered longest
     first and concatenated along the segment axis, so level L works on every series that
-    still has more than one segment in one launch. A series that reaches a single segment
-    is scored there (``ffa_search``'s per-width S/N grid, ``search.py:46-47``). Returns the
-    S/N grids ``[ncoords_final(series), n_w]`` in the input order.
+    still has more than one segment in one launch. A series whose last level this is gets
+    that level fused with its scoring (``pyl_ffa_shift_add_score``), so its final fold is
+    never written; with ``fused=False`` the final fold is stored and scored separately.
+    Returns the per-width S/N grids ``[ncoords_final(series), n_w]`` in input order
+    (``ffa_search``'s grid, ``search.py:46-47``).
     """
     nser = len(series)
     lens = [int(e.shape[0]) for e, _ in series]
@@ -252,39 +255,49 @@ def ffa_batch(plan: FFAPlan, series: list[tuple[torch.Tensor, torch.Tensor]],
         if lens[i] % plan.cfg.bseg_brute:
             msg = "series length must be a multiple of bseg_brute"
             raise ValueError(msg)
-    nseg0 = [lens[i] // plan.cfg.bseg_brute for i in order]
-    for n in nseg0:
+    nseg = [lens[i] // plan.cfg.bseg_brute for i in order]
+    for n in nseg:
         if n & (n - 1):
             msg = "segments per series must be a power of two"
             raise ValueError(msg)
-    if int(math.log2(max(nseg0))) > plan.nlevels:
+    if int(math.log2(max(nseg))) > plan.nlevels:
         msg = "plan has fewer levels than the longest series needs"
         raise ValueError(msg)
-    total_seg = sum(nseg0)
     dev = series[0][0].device
-    fold = torch.empty((total_seg, plan.bf.nfreq, 2, plan.cfg.nbins), dtype=torch.float32,
+    fold = torch.empty((sum(nseg), plan.bf.nfreq, 2, plan.cfg.nbins), dtype=torch.float32,
                        device=dev)
     off = 0
-    for i, n in zip(order, nseg0):
+    for i, n in zip(order, nseg):
         plan.bf.fold(series[i][0], series[i][1], out=fold[off:off + n])
         off += n
     results: dict[int, torch.Tensor] = {}
-    nseg = list(nseg0)
     level = 0
+    # A series with a single segment from the start is scored at level 0.
+    off = sum(n for n in nseg if n > 1)
+    for k in range(nser):
+        if nseg[k] == 1:
+            results[order[k]] = kernels.score_boxcar(fold[off], widths, stdnoise)
+            off += 1
     while True:
-        # Series whose fold is down to one segment are finished: score them.
-        active = [k for k in range(nser) if nseg[k] > 1]
-        base = sum(nseg[k] for k in active)
-        off = base
-        for k in range(nser):
-            if nseg[k] == 1 and order[k] not in results:
-                results[order[k]] = kernels.score_boxcar(fold[off], widths, stdnoise)
-                off += 1
-        if not active:
+        cont = [k for k in range(nser) if nseg[k] > 2]
+        fin = [k for k in range(nser) if nseg[k] == 2]
+        if not cont and not fin:
             break
         lp = plan.levels[level]
-        fold = kernels.ffa_shift_add(fold[:base], lp.table, lp.ncoords)
-        for k in active:
+        base = sum(nseg[k] for k in cont)
+        if fin:
+            part = fold[base:base + 2 * len(fin)]
+            if fused:
+                snr = kernels.ffa_shift_add_score(part, lp.table, lp.ncoords, widths, stdnoise)
+                for m, k in enumerate(fin):
+                    results[order[k]] = snr[m]
+            else:
+                last = kernels.ffa_shift_add(part, lp.table, lp.ncoords)
+                for m, k in enumerate(fin):
+                    results[order[k]] = kernels.score_boxcar(last[m], widths, stdnoise)
+        if cont:
+            fold = kernels.ffa_shift_add(fold[:base], lp.table, lp.ncoords)
+        for k in cont + fin:
             nseg[k] //= 2
         level += 1
     return [results[i] for i in range(nser)]

@@ -117,6 +117,33 @@ def ffa_shift_add(fold_in: torch.Tensor, table: torch.Tensor, ncoords_cur: int,
     return out
 
 
+def ffa_shift_add_score(fold_in: torch.Tensor, table: torch.Tensor, ncoords_cur: int,
+                        widths: np.ndarray, stdnoise: float = 1.0,
+                        out: torch.Tensor | None = None) -> torch.Tensor:
+    """Last level fused with scoring: [nseg, nprev, 2, B] -> S/N [nseg/2, ncur, n_w]."""
+    if fold_in.dim() != 4 or fold_in.shape[2] != 2 or fold_in.dtype != torch.float32:
+        msg = "fold_in must be float32 [nseg, ncoords, 2, nbins]"
+        raise ValueError(msg)
+    nseg, nprev, _, nbins = fold_in.shape
+    if nseg % 2:
+        msg = f"nsegments must be even, got {nseg}"
+        raise ValueError(msg)
+    if tuple(table.shape) != (ncoords_cur, 4):
+        msg = "table does not match ncoords_cur"
+        raise ValueError(msg)
+    widths = np.ascontiguousarray(widths, dtype=np.int64)
+    shape = (nseg // 2, ncoords_cur, len(widths))
+    if out is None:
+        out = torch.empty(shape, dtype=torch.float32, device=fold_in.device)
+    elif tuple(out.shape) != shape:
+        msg = f"out must have shape {shape}"
+        raise ValueError(msg)
+    _lib.check(_lib.load().pyl_ffa_shift_add_score(
+        ptr(fold_in), nseg, nprev, ptr(table), ncoords_cur, nbins, _lib.i64_array(widths),
+        len(widths), float(stdnoise), ptr(out), stream_ptr()))
+    return out
+
+
 def score_boxcar(folds: torch.Tensor, widths: np.ndarray, stdnoise: float = 1.0,
                  layout: int = _lib.PYL_LAYOUT_EV, mode: int = _lib.PYL_SCORE_PER_WIDTH,
                  out: torch.Tensor | None = None) -> torch.Tensor:

@@ -133,6 +133,21 @@ def test_score_bit_exact(gpu_device, nbins, n):
     np.testing.assert_allclose(got, exp, rtol=1e-5)
 
 
+@pytest.mark.parametrize("nbins", [32, 64, 128])
+def test_fused_last_level_score_bit_exact(gpu_device, nbins):
+    _case(f"fused level+score nbins={nbins}")
+    cfg = _cfg(2, 2**18, nbins, 4096)
+    rng = np.random.default_rng(nbins)
+    ts_e = rng.normal(size=cfg.nsamps).astype(np.float32)
+    plan = FFAPlan(cfg)
+    folds = plan.run(_dev(ts_e), _dev(np.ones(cfg.nsamps, dtype=np.float32)), keep=True)
+    lp = plan.levels[-1]
+    widths = np.array([1, 2, 3, 5, 8, nbins // 4], dtype=np.int64)
+    got = kernels.ffa_shift_add_score(folds[-2], lp.table, lp.ncoords, widths).cpu().numpy()
+    exp = orc.boxcar_snr(orc.fold_norm(folds[-1].cpu().numpy()), widths)
+    np.testing.assert_array_equal(got, exp)
+
+
 def test_score_empty_and_invalid(gpu_device):
     _case("score empty / invalid")
     empty = torch.empty((0, 2, 64), dtype=torch.float32, device="cuda")
@@ -189,6 +204,9 @@ def test_ffa_batch_matches_single_series(gpu_device):
         series.append((_dev(e), _dev(np.ones(n, dtype=np.float32))))
     widths = cfg.score_widths
     out = ffa_batch(plan, series, widths)
+    out_unfused = ffa_batch(plan, series, widths, fused=False)
+    for a, b in zip(out, out_unfused):
+        np.testing.assert_array_equal(a.cpu().numpy(), b.cpu().numpy())
     for (e, v), snr, n in zip(series, out, lens):
         sub = PulsarSearchConfig(nsamps=n, tsamp=cfg.tsamp, nbins=64, eta=1.0,
                                  param_limits=cfg.param_limits, bseg_brute=4096,
