@@ -54,6 +54,10 @@ int pyl_brutefold_plan_create(const double* freqs, int64_t nfreq, int64_t seg_le
 int pyl_brutefold_plan_destroy(pyl_bf_plan* plan);
 int pyl_brutefold_execute(const pyl_bf_plan* plan, const float* ts_e_dev, const float* ts_v_dev,
                           int64_t nsamps, float* fold_dev, void* stream);
+/* BF16-storage variant: FP32 accumulation, each bin rounded to BF16 (RNE) when stored. */
+int pyl_brutefold_execute_bf16(const pyl_bf_plan* plan, const float* ts_e_dev,
+                               const float* ts_v_dev, int64_t nsamps, uint16_t* fold_dev,
+                               void* stream);
 
 /* ------------------------------------------------------------------------------------
  * FFA level: resolve + shift-add.
@@ -87,6 +91,16 @@ int pyl_ffa_shift_add_score(const float* fold_in_dev, int64_t nseg_in, int64_t n
                             const int32_t* table_dev, int64_t ncoords_cur, int32_t nbins,
                             const int64_t* widths, int32_t nw, double stdnoise,
                             float* snr_out_dev, void* stream);
+/* BF16-storage / FP32-accumulate variants of the level kernels: rows are loaded as BF16,
+ * added in FP32 and stored rounded to BF16 (RNE); the fused variant normalises and scores
+ * the BF16-rounded row. nbins in {32, 64, 128, 256} ({32, 64, 128} fused). */
+int pyl_ffa_shift_add_bf16(const uint16_t* fold_in_dev, int64_t nseg_in, int64_t ncoords_prev,
+                           const int32_t* table_dev, int64_t ncoords_cur, int32_t nbins,
+                           uint16_t* fold_out_dev, void* stream);
+int pyl_ffa_shift_add_score_bf16(const uint16_t* fold_in_dev, int64_t nseg_in,
+                                 int64_t ncoords_prev, const int32_t* table_dev,
+                                 int64_t ncoords_cur, int32_t nbins, const int64_t* widths,
+                                 int32_t nw, double stdnoise, float* snr_out_dev, void* stream);
 int pyl_ffa_level(int32_t nparams, const double* params_dev, const int64_t* counts_cur,
                   const int64_t* counts_prev, const double* param_limits, int32_t ffa_level,
                   double tseg_brute, int32_t nbins, const float* fold_in_dev, int64_t nseg_in,
@@ -102,6 +116,7 @@ int pyl_ffa_level(int32_t nparams, const double* params_dev, const int64_t* coun
  * ------------------------------------------------------------------------------------ */
 #define PYL_LAYOUT_EV 0
 #define PYL_LAYOUT_NORM 1
+#define PYL_LAYOUT_EV_BF16 2 /* [n][2][nbins] BF16 (e, v); folds_dev points at uint16 */
 #define PYL_SCORE_PER_WIDTH 0
 #define PYL_SCORE_MAX 1
 

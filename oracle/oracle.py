@@ -98,6 +98,14 @@ def lib() -> ctypes.CDLL:
         L.orc_unify_fold_f64.restype = None
         L.orc_unify_fold_f64.argtypes = [_f64p, _i64, _int, _i64p, _f64p, _i64p, _f64p, _int,
                                          _dbl, _i64, _f64p]
+        _u16p = np.ctypeslib.ndpointer(dtype=np.uint16, flags="C_CONTIGUOUS")
+        L.orc_round_bf16.restype = None
+        L.orc_round_bf16.argtypes = [_f32p, _i64, _u16p]
+        L.orc_bf16_to_f32_array.restype = None
+        L.orc_bf16_to_f32_array.argtypes = [_u16p, _i64, _f32p]
+        L.orc_unify_fold_bf16.restype = None
+        L.orc_unify_fold_bf16.argtypes = [_u16p, _i64, _int, _i64p, _f64p, _i64p, _f64p, _int,
+                                          _dbl, _i64, _u16p]
         L.orc_boxcar_snr_f64.restype = None
         L.orc_boxcar_snr_f64.argtypes = [_f64p, _i64, _i64, _i64p, _i64, _f64p]
         _lib = L
@@ -224,6 +232,42 @@ def ffa_search_snr(ts_e, ts_v, cfg) -> np.ndarray:
     norm = fold_norm(fold[0])
     snr = boxcar_snr(norm, cfg.score_widths, 1.0)
     return snr.reshape(*counts, len(cfg.score_widths))
+
+
+def round_bf16(x: np.ndarray) -> np.ndarray:
+    """FP32 -> BF16 bits (uint16), round to nearest even."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.empty(x.shape, dtype=np.uint16)
+    lib().orc_round_bf16(x.reshape(-1), x.size, out.reshape(-1))
+    return out
+
+
+def bf16_to_f32(h: np.ndarray) -> np.ndarray:
+    h = np.ascontiguousarray(h, dtype=np.uint16)
+    out = np.empty(h.shape, dtype=np.float32)
+    lib().orc_bf16_to_f32_array(h.reshape(-1), h.size, out.reshape(-1))
+    return out
+
+
+def ffa_fold_bf16(ts_e, ts_v, cfg) -> tuple[np.ndarray, list]:
+    """FFA with BF16 fold storage and FP32 arithmetic (brute fold in FP32, then every
+    stored fold element rounded to BF16). Returns BF16 bits [nseg, ncoords, 2, B]."""
+    pa = cfg.get_param_arr(cfg.get_dparams(0))
+    t_ref = 0.0 if cfg.nparams == 1 else cfg.bseg_brute * cfg.tsamp / 2
+    fold = round_bf16(brutefold(ts_e, ts_v, pa[-1], cfg.bseg_brute, cfg.nbins, cfg.tsamp, t_ref))
+    counts_prev = np.ascontiguousarray(cfg.get_param_grid_count(0), dtype=np.int64)
+    for level in range(1, cfg.niters_ffa + 1):
+        pa = cfg.get_param_arr(cfg.get_dparams(level))
+        counts_cur = np.array([len(a) for a in pa], dtype=np.int64)
+        out = np.empty((fold.shape[0] // 2, int(np.prod(counts_cur)), 2, cfg.nbins),
+                       dtype=np.uint16)
+        lib().orc_unify_fold_bf16(fold.reshape(-1), fold.shape[0], len(pa), counts_prev,
+                                  np.ascontiguousarray(np.concatenate(pa)), counts_cur,
+                                  np.ascontiguousarray(cfg.param_limits).reshape(-1), level,
+                                  cfg.tseg_brute, cfg.nbins, out.reshape(-1))
+        fold = out
+        counts_prev = counts_cur
+    return fold, pa
 
 
 def ffa_search_snr_f64(ts_e, ts_v, cfg) -> np.ndarray:

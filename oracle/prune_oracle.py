@@ -434,12 +434,11 @@ def prune_level(tree: Tree, fold_segment: np.ndarray, coords: dict, threshold: f
 
 
 def ascend(tree: Tree, dyp_fold: np.ndarray, idx_segments, coord_segments, coord_cur,
-           cfg: dict) -> None:
-    """Re-integrate every survivor over all segments so far and rescore (in place)."""
+           cfg: dict) -> Tree:
+    """Re-integrate every survivor over all segments so far and rescore (new tree)."""
     n = tree.size
     if n == 0:
-        return
-    tree.scores_ep = tree.scores.copy()
+        return tree
     vec = tree.leaves[:, :-1, 0]
     f0 = tree.leaves[:, -1, 0]
     B = tree.folds.shape[-1]
@@ -456,8 +455,8 @@ def ascend(tree: Tree, dyp_fold: np.ndarray, idx_segments, coord_segments, coord
         src = (np.arange(B)[None, :] - s[:, None]) % B
         rolled = np.take_along_axis(rows, np.broadcast_to(src[:, None, :], rows.shape), axis=2)
         res = (res + rolled).astype(np.float32)
-    tree.folds = res
-    tree.scores = orc.snr_score_batch(res, cfg["widths"])
+    return Tree(tree.leaves.copy(), res, orc.snr_score_batch(res, cfg["widths"]),
+                tree.backtracks.copy(), scores_ep=tree.scores.copy())
 
 
 def report_batch(leaves: np.ndarray) -> np.ndarray:
@@ -514,6 +513,6 @@ def prune_run(dyp_fold: np.ndarray, param_arr: list[np.ndarray], dparams: np.nda
     final_level = level
     if tree.size > 0:
         idx_seg, coord_seg = scheme.get_segment_coords_so_far(final_level)
-        ascend(tree, dyp_fold, idx_seg, coord_seg, scheme.get_coord(final_level), cfg)
+        tree = ascend(tree, dyp_fold, idx_seg, coord_seg, scheme.get_coord(final_level), cfg)
     return {"tree": tree, "report": report_batch(tree.leaves) if tree.size else tree.leaves,
             "stats": stats, "survivors": survivors, "trees": trees, "scheme": scheme}

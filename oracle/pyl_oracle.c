@@ -381,3 +381,70 @@ void orc_boxcar_snr_f64(const double* folds, int64_t n, int64_t nbins, const int
     }
     free(ps);
 }
+
+/* ---- BF16 storage emulation (FP32 arithmetic, every stored fold element rounded to
+ * BF16 with round-to-nearest-even) ---- */
+static uint16_t orc_f32_to_bf16(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+static float orc_bf16_to_f32(uint16_t h) {
+    uint32_t u = (uint32_t)h << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+void orc_round_bf16(const float* in, int64_t n, uint16_t* out) {
+    for (int64_t i = 0; i < n; ++i) out[i] = orc_f32_to_bf16(in[i]);
+}
+
+void orc_bf16_to_f32_array(const uint16_t* in, int64_t n, float* out) {
+    for (int64_t i = 0; i < n; ++i) out[i] = orc_bf16_to_f32(in[i]);
+}
+
+/* unify_fold on BF16 storage: out = bf16(f32(tail) + f32(head)). */
+void orc_unify_fold_bf16(const uint16_t* fold_in, int64_t nseg_in, int P, const int64_t* counts_prev,
+                         const double* params, const int64_t* counts_cur, const double* limits,
+                         int level, double tseg_brute, int64_t nbins, uint16_t* fold_out) {
+    int64_t ncur = 1, nprev = 1;
+    for (int k = 0; k < P; ++k) { ncur *= counts_cur[k]; nprev *= counts_prev[k]; }
+    int64_t offs[8];
+    offs[0] = 0;
+    for (int k = 1; k < P; ++k) offs[k] = offs[k - 1] + counts_cur[k - 1];
+    int64_t nseg_out = nseg_in / 2, rowlen = 2 * nbins;
+    for (int64_t j = 0; j < ncur; ++j) {
+        double pset[8];
+        int64_t r = j;
+        for (int k = P - 1; k >= 0; --k) {
+            int64_t ik = r % counts_cur[k];
+            r /= counts_cur[k];
+            pset[k] = params[offs[k] + ik];
+        }
+        int64_t row[2], sh[2];
+        for (int latter = 0; latter < 2; ++latter) {
+            int64_t idx[8];
+            double ph;
+            orc_ffa_resolve(P, pset, counts_prev, limits, level, latter, tseg_brute, nbins, idx, &ph);
+            int64_t flat = 0;
+            for (int k = 0; k < P; ++k) flat = flat * counts_prev[k] + idx[k];
+            row[latter] = flat;
+            sh[latter] = orc_shift_int(ph, nbins);
+        }
+        for (int64_t q = 0; q < nseg_out; ++q) {
+            const uint16_t* t = fold_in + ((2 * q) * nprev + row[0]) * rowlen;
+            const uint16_t* h = fold_in + ((2 * q + 1) * nprev + row[1]) * rowlen;
+            uint16_t* o = fold_out + (q * ncur + j) * rowlen;
+            for (int64_t b = 0; b < 2 * nbins; ++b) {
+                int64_t c = b / nbins, bb = b % nbins;
+                int64_t i1 = ((bb - sh[0]) % nbins + nbins) % nbins;
+                int64_t i2 = ((bb - sh[1]) % nbins + nbins) % nbins;
+                float v = orc_bf16_to_f32(t[c * nbins + i1]) + orc_bf16_to_f32(h[c * nbins + i2]);
+                o[b] = orc_f32_to_bf16(v);
+            }
+        }
+    }
+}

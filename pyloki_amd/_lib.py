@@ -19,6 +19,7 @@ PYL_ERR_ALLOC = -3
 
 PYL_LAYOUT_EV = 0
 PYL_LAYOUT_NORM = 1
+PYL_LAYOUT_EV_BF16 = 2
 PYL_SCORE_PER_WIDTH = 0
 PYL_SCORE_MAX = 1
 
@@ -37,6 +38,11 @@ SIGNATURES: list[tuple[str, object, list[object]]] = [
      [_f64p, _c_i64, _c_i64, _c_i32, _c_dbl, _c_dbl, _vp, ctypes.POINTER(_vp)]),
     ("pyl_brutefold_plan_destroy", ctypes.c_int, [_vp]),
     ("pyl_brutefold_execute", ctypes.c_int, [_vp, _vp, _vp, _c_i64, _vp, _vp]),
+    ("pyl_brutefold_execute_bf16", ctypes.c_int, [_vp, _vp, _vp, _c_i64, _vp, _vp]),
+    ("pyl_ffa_shift_add_bf16", ctypes.c_int,
+     [_vp, _c_i64, _c_i64, _vp, _c_i64, _c_i32, _vp, _vp]),
+    ("pyl_ffa_shift_add_score_bf16", ctypes.c_int,
+     [_vp, _c_i64, _c_i64, _vp, _c_i64, _c_i32, _i64p, _c_i32, _c_dbl, _vp, _vp]),
     ("pyl_ffa_resolve", ctypes.c_int,
      [_c_i32, _vp, _i64p, _i64p, _f64p, _c_i32, _c_dbl, _c_i32, _vp, _vp, _vp]),
     ("pyl_ffa_shift_add", ctypes.c_int, [_vp, _c_i64, _c_i64, _vp, _c_i64, _c_i32, _vp, _vp]),

@@ -306,6 +306,33 @@ class PulsarSearchConfig:
             for i in range(self.nparams)
         ]
 
+    def generate_branching_pattern(
+        self,
+        kind: str = "poly_taylor_moving",
+        ref_seg: int = 0,
+        use_cheby_coarsening: bool = True,
+    ) -> np.ndarray:
+        """Expected children per parent at every pruning level (``config.py:758-819``).
+
+        Taylor kinds restate ``generate_bp_poly_taylor`` (``core/taylor.py:414-499``):
+        per final-grid frequency, the children per parent of each level, averaged with
+        weights equal to the running product of earlier branchings.
+        """
+        funcs = {"poly_taylor_moving": True, "poly_taylor_fixed": False}
+        if kind not in funcs:
+            msg = f"Invalid kind: {kind}"
+            raise ValueError(msg)
+        use_moving_grid = funcs[kind]
+        nsegments = int(np.ceil(self.nsamps / self.bseg_ffa))
+        dparams = self.get_dparams(self.niters_ffa, use_cheby_coarsening)
+        param_arr = self.get_param_arr(dparams)
+        dparams_act = self.get_dparams_actual(self.niters_ffa, use_cheby_coarsening)
+        return generate_bp_poly_taylor(
+            param_arr, dparams_act, self.tseg_ffa, nsegments, self.nbins, self.eta, ref_seg,
+            use_moving_grid=use_moving_grid, tiling_strategy=self.tiling_strategy,
+            use_cheby_coarsening=use_cheby_coarsening,
+        )
+
     def _bseg_brute_default(self) -> int:
         init_levels = 1 if self.nparams == 1 else 5
         levels = int(np.log2(self.nsamps * self.tsamp * self.f_min) - init_levels)
@@ -313,6 +340,65 @@ class PulsarSearchConfig:
 
     def _bseg_ffa_default(self) -> int:
         return int(self.nsamps)
+
+
+def generate_bp_poly_taylor(
+    param_arr: list[np.ndarray],
+    dparams_act: np.ndarray,
+    tseg_ffa: float,
+    nsegments: int,
+    nbins: int,
+    eta: float,
+    ref_seg: int,
+    *,
+    use_moving_grid: bool,
+    tiling_strategy: str,
+    use_cheby_coarsening: bool = True,
+) -> np.ndarray:
+    """Exact Taylor branching pattern (``core/taylor.py:414-499``), host FP64."""
+    from pyloki_amd.utils.misc import FLOAT_EPSILON
+    from pyloki_amd.utils.snail import MiddleOutScheme
+
+    n_params = len(dparams_act)
+    f0 = np.asarray(param_arr[-1], dtype=np.float64)
+    nf = len(f0)
+    scheme = MiddleOutScheme(nsegments, ref_seg, tseg_ffa)
+    weights = np.ones(nf)
+    bp = np.empty(nsegments - 1)
+    dcur = np.tile(np.asarray(dparams_act, dtype=np.float64), (nf, 1))
+    dcur[:, n_params - 1] *= C_VAL / f0
+    for level in range(1, nsegments):
+        coord_next = scheme.get_coord(level)
+        coord_cur = scheme.get_current_coord(level, use_moving_grid)
+        tobs = coord_cur[1]
+        dpf = psr_utils.poly_taylor_step_f(n_params, tobs, nbins, eta, 0.0, use_cheby_coarsening)
+        dnew = dpf[None, :] * C_VAL / f0[:, None]
+        k = np.arange(n_params - 1, -1, -1)
+        fac = np.array([psr_utils.powi(tobs, int(kk) + 1) * nbins / psr_utils.fact(int(kk) + 1)
+                        for kk in k])
+        if use_cheby_coarsening:
+            fac = fac / (2.0**k)
+        shift = np.abs(dcur - dnew) * (fac[None, :] * (f0 / C_VAL)[:, None])
+        nbr = np.ones(nf, dtype=np.int64)
+        dnext = dcur.copy()
+        split = shift >= (eta - FLOAT_EPSILON)
+        for i in range(nf):
+            for j in range(n_params):
+                if not split[i, j]:
+                    continue
+                num = max(1, math.ceil(dcur[i, j] / dnew[i, j] - FLOAT_EPSILON))
+                nbr[i] *= num
+                dnext[i, j] = dcur[i, j] / num
+        bp[level - 1] = np.sum(weights * nbr) / np.sum(weights)
+        weights = weights * nbr
+        if use_moving_grid:
+            dt = coord_next[0] - coord_cur[0]
+            dvec = np.zeros((nf, n_params + 1))
+            dvec[:, :-1] = dnext
+            dcur = transforms.shift_taylor_errors(dvec, dt, tiling_strategy)[:, :-1]
+        else:
+            dcur = dnext
+    return bp
 
 
 def ffa_level_counts(cfg: PulsarSearchConfig) -> list[np.ndarray]:

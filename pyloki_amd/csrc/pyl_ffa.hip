@@ -91,19 +91,46 @@ __global__ void ffa_resolve_kernel(const double* __restrict__ params, ResolveArg
   if (phases) phases[j] = make_double2(ph[0], ph[1]);
 }
 
-// Rotated load of the float4 chunk `c4` (elements 4*c4 .. 4*c4+3) of a row component of
-// B floats: element e of the result is src[(4*c4 + e - s) mod B].
-template <int B>
-__device__ __forceinline__ float4 rotated_chunk(const float* __restrict__ src, int c4, int s) {
+template <typename T>
+struct Vec4;
+template <>
+struct Vec4<float> {
+  using type = float4;
+};
+template <>
+struct Vec4<uint16_t> {
+  using type = ushort4;
+};
+
+template <typename T>
+__device__ __forceinline__ float4 to_f32x4(const typename Vec4<T>::type& v) {
+  return make_float4(pyl::Store<T>::load(v.x), pyl::Store<T>::load(v.y),
+                     pyl::Store<T>::load(v.z), pyl::Store<T>::load(v.w));
+}
+
+template <typename T>
+__device__ __forceinline__ void store4(T* __restrict__ dst, int c4, const float4& r) {
+  typename Vec4<T>::type v;
+  v.x = pyl::Store<T>::store(r.x);
+  v.y = pyl::Store<T>::store(r.y);
+  v.z = pyl::Store<T>::store(r.z);
+  v.w = pyl::Store<T>::store(r.w);
+  reinterpret_cast<typename Vec4<T>::type*>(dst)[c4] = v;
+}
+
+// Rotated load of the 4-element chunk `c4` (elements 4*c4 .. 4*c4+3) of a row component of
+// B elements, widened to FP32: element e of the result is src[(4*c4 + e - s) mod B].
+template <int B, typename T = float>
+__device__ __forceinline__ float4 rotated_chunk(const T* __restrict__ src, int c4, int s) {
   constexpr int C = B / 4;
   const int base = 4 * c4 - s;                 // may be negative
   const int m = (base % B + B) % B;            // first source element
   const int q = m >> 2;                        // source chunk
   const int o = m & 3;                         // offset inside the chunk
-  const float4* s4 = reinterpret_cast<const float4*>(src);
-  const float4 a = s4[q];
+  const typename Vec4<T>::type* s4 = reinterpret_cast<const typename Vec4<T>::type*>(src);
+  const float4 a = to_f32x4<T>(s4[q]);
   if (o == 0) return a;
-  const float4 b = s4[(q + 1) % C];
+  const float4 b = to_f32x4<T>(s4[(q + 1) % C]);
   float4 r;
   if (o == 1) {
     r.x = a.y; r.y = a.z; r.z = a.w; r.w = b.x;
@@ -116,11 +143,11 @@ __device__ __forceinline__ float4 rotated_chunk(const float* __restrict__ src, i
 }
 
 // Rows per block = kThreads / (B/2) (each row = 2 components x B/4 chunks).
-template <int B>
+template <int B, typename T>
 __global__ void __launch_bounds__(256)
-ffa_shift_add_kernel(const float* __restrict__ in, int64_t ncoords_prev,
+ffa_shift_add_kernel(const T* __restrict__ in, int64_t ncoords_prev,
                      const int4* __restrict__ table, int64_t ncoords_cur, int64_t nseg_out,
-                     int64_t nblk_j, float* __restrict__ out) {
+                     int64_t nblk_j, T* __restrict__ out) {
   constexpr int CPR = B / 4;          // chunks per component
   constexpr int TPR = 2 * CPR;        // threads per row
   constexpr int RPB = 256 / TPR;      // rows per block
@@ -146,16 +173,16 @@ ffa_shift_add_kernel(const float* __restrict__ in, int64_t ncoords_prev,
   const int c4 = e4 % CPR;
   const int4 tb = table[j];
   constexpr int64_t E = 2 * B;
-  const float* tail = in + ((2 * q) * ncoords_prev + tb.x) * E + comp * B;
-  const float* head = in + ((2 * q + 1) * ncoords_prev + tb.z) * E + comp * B;
-  const float4 a = rotated_chunk<B>(tail, c4, tb.y);
-  const float4 b = rotated_chunk<B>(head, c4, tb.w);
+  const T* tail = in + ((2 * q) * ncoords_prev + tb.x) * E + comp * B;
+  const T* head = in + ((2 * q + 1) * ncoords_prev + tb.z) * E + comp * B;
+  const float4 a = rotated_chunk<B, T>(tail, c4, tb.y);
+  const float4 b = rotated_chunk<B, T>(head, c4, tb.w);
   float4 r;
   r.x = a.x + b.x;
   r.y = a.y + b.y;
   r.z = a.z + b.z;
   r.w = a.w + b.w;
-  reinterpret_cast<float4*>(out + (q * ncoords_cur + j) * E + comp * B)[c4] = r;
+  store4<T>(out + (q * ncoords_cur + j) * E + comp * B, c4, r);
 }
 
 // Generic nbins: one thread per output element.
@@ -189,9 +216,9 @@ __global__ void ffa_shift_add_generic_kernel(const float* __restrict__ in, int64
 // (4 bins x 2 components per lane, float4 loads with the rotation of rotated_chunk), so a
 // wavefront forms 4 rows per step; the normalised rows are transposed into LDS and every
 // thread then scans and scores one row.
-template <int B>
+template <int B, typename T>
 __global__ void __launch_bounds__(pyl::kScoreBlock)
-ffa_shift_add_score_kernel(const float* __restrict__ in, int64_t ncoords_prev,
+ffa_shift_add_score_kernel(const T* __restrict__ in, int64_t ncoords_prev,
                            const int4* __restrict__ table, int64_t ncoords_cur,
                            int64_t nblk_j, pyl::ScoreConst k, float* __restrict__ out) {
   constexpr int CPR = B / 4;                 // float4 chunks per component
@@ -212,17 +239,20 @@ ffa_shift_add_score_kernel(const float* __restrict__ in, int64_t ncoords_prev,
     const int r = r0 + rsub;
     if (r < nvalid) {
       const int4 tb = s_tab[r];
-      const float* tail = in + ((2 * q) * ncoords_prev + tb.x) * E;
-      const float* head = in + ((2 * q + 1) * ncoords_prev + tb.z) * E;
-      const float4 te = rotated_chunk<B>(tail, c4, tb.y);
-      const float4 he = rotated_chunk<B>(head, c4, tb.w);
-      const float4 tv = rotated_chunk<B>(tail + B, c4, tb.y);
-      const float4 hv = rotated_chunk<B>(head + B, c4, tb.w);
+      const T* tail = in + ((2 * q) * ncoords_prev + tb.x) * E;
+      const T* head = in + ((2 * q + 1) * ncoords_prev + tb.z) * E;
+      const float4 te = rotated_chunk<B, T>(tail, c4, tb.y);
+      const float4 he = rotated_chunk<B, T>(head, c4, tb.w);
+      const float4 tv = rotated_chunk<B, T>(tail + B, c4, tb.y);
+      const float4 hv = rotated_chunk<B, T>(head + B, c4, tb.w);
       const int b = 4 * c4;
-      S[(b + 0) * pyl::kScoreStride + r] = (te.x + he.x) / sqrtf(tv.x + hv.x);
-      S[(b + 1) * pyl::kScoreStride + r] = (te.y + he.y) / sqrtf(tv.y + hv.y);
-      S[(b + 2) * pyl::kScoreStride + r] = (te.z + he.z) / sqrtf(tv.z + hv.z);
-      S[(b + 3) * pyl::kScoreStride + r] = (te.w + he.w) / sqrtf(tv.w + hv.w);
+      // The level's fold element as it would be stored (identity for FP32, BF16 rounding
+      // for BF16 storage), then e / sqrt(v).
+      auto st = [](float x) { return pyl::Store<T>::load(pyl::Store<T>::store(x)); };
+      S[(b + 0) * pyl::kScoreStride + r] = st(te.x + he.x) / sqrtf(st(tv.x + hv.x));
+      S[(b + 1) * pyl::kScoreStride + r] = st(te.y + he.y) / sqrtf(st(tv.y + hv.y));
+      S[(b + 2) * pyl::kScoreStride + r] = st(te.z + he.z) / sqrtf(st(tv.z + hv.z));
+      S[(b + 3) * pyl::kScoreStride + r] = st(te.w + he.w) / sqrtf(st(tv.w + hv.w));
     }
   }
   __syncthreads();
@@ -284,14 +314,14 @@ extern "C" int pyl_ffa_resolve(int32_t nparams, const double* params_dev,
   return PYL_OK;
 }
 
-template <int B>
-static void launch_shift_add(const float* in, int64_t nprev, const int4* tb, int64_t ncur,
-                             int64_t nseg_out, float* out, hipStream_t s) {
+template <int B, typename T>
+static void launch_shift_add(const T* in, int64_t nprev, const int4* tb, int64_t ncur,
+                             int64_t nseg_out, T* out, hipStream_t s) {
   constexpr int RPB = 256 / (B / 2);
   const int64_t nblk_j = pyl::ceil_div(ncur, RPB);
   const int64_t nblocks = nblk_j * nseg_out;
-  ffa_shift_add_kernel<B><<<(unsigned)nblocks, 256, 0, s>>>(in, nprev, tb, ncur, nseg_out,
-                                                            nblk_j, out);
+  ffa_shift_add_kernel<B, T><<<(unsigned)nblocks, 256, 0, s>>>(in, nprev, tb, ncur, nseg_out,
+                                                               nblk_j, out);
 }
 
 extern "C" int pyl_ffa_shift_add(const float* fold_in, int64_t nseg_in, int64_t ncoords_prev,
@@ -306,10 +336,10 @@ extern "C" int pyl_ffa_shift_add(const float* fold_in, int64_t nseg_in, int64_t 
   const int64_t nseg_out = nseg_in / 2;
   const int4* tb = reinterpret_cast<const int4*>(table_dev);
   switch (nbins) {
-    case 32: launch_shift_add<32>(fold_in, ncoords_prev, tb, ncoords_cur, nseg_out, fold_out, s); break;
-    case 64: launch_shift_add<64>(fold_in, ncoords_prev, tb, ncoords_cur, nseg_out, fold_out, s); break;
-    case 128: launch_shift_add<128>(fold_in, ncoords_prev, tb, ncoords_cur, nseg_out, fold_out, s); break;
-    case 256: launch_shift_add<256>(fold_in, ncoords_prev, tb, ncoords_cur, nseg_out, fold_out, s); break;
+    case 32: launch_shift_add<32, float>(fold_in, ncoords_prev, tb, ncoords_cur, nseg_out, fold_out, s); break;
+    case 64: launch_shift_add<64, float>(fold_in, ncoords_prev, tb, ncoords_cur, nseg_out, fold_out, s); break;
+    case 128: launch_shift_add<128, float>(fold_in, ncoords_prev, tb, ncoords_cur, nseg_out, fold_out, s); break;
+    case 256: launch_shift_add<256, float>(fold_in, ncoords_prev, tb, ncoords_cur, nseg_out, fold_out, s); break;
     default: {
       const int64_t total = nseg_out * ncoords_cur * 2 * (int64_t)nbins;
       const int64_t blocks = std::min<int64_t>(pyl::ceil_div(total, 256), 1 << 20);
@@ -322,21 +352,21 @@ extern "C" int pyl_ffa_shift_add(const float* fold_in, int64_t nseg_in, int64_t 
 }
 
 
-template <int B>
-static void launch_shift_add_score(const float* in, int64_t nprev, const int4* tb, int64_t ncur,
+template <int B, typename T>
+static void launch_shift_add_score(const T* in, int64_t nprev, const int4* tb, int64_t ncur,
                                    int64_t nseg_out, const pyl::ScoreConst& k, float* out,
                                    hipStream_t s) {
   const int64_t nblk_j = pyl::ceil_div(ncur, pyl::kScoreBlock);
   const size_t lds = sizeof(float) * (size_t)(B + k.wmax) * pyl::kScoreStride;
-  ffa_shift_add_score_kernel<B><<<(unsigned)(nblk_j * nseg_out), pyl::kScoreBlock, lds, s>>>(
+  ffa_shift_add_score_kernel<B, T><<<(unsigned)(nblk_j * nseg_out), pyl::kScoreBlock, lds, s>>>(
       in, nprev, tb, ncur, nblk_j, k, out);
 }
 
-extern "C" int pyl_ffa_shift_add_score(const float* fold_in, int64_t nseg_in,
-                                       int64_t ncoords_prev, const int32_t* table_dev,
-                                       int64_t ncoords_cur, int32_t nbins, const int64_t* widths,
-                                       int32_t nw, double stdnoise, float* snr_out,
-                                       void* stream) {
+template <typename T>
+static int shift_add_score(const T* fold_in, int64_t nseg_in, int64_t ncoords_prev,
+                           const int32_t* table_dev, int64_t ncoords_cur, int32_t nbins,
+                           const int64_t* widths, int32_t nw, double stdnoise, float* snr_out,
+                           void* stream) {
   PYL_REQUIRE(fold_in && table_dev && snr_out, "null device pointer");
   PYL_REQUIRE(nseg_in >= 2 && nseg_in % 2 == 0, "nsegments must be even and >= 2, got %lld",
               (long long)nseg_in);
@@ -350,11 +380,52 @@ extern "C" int pyl_ffa_shift_add_score(const float* fold_in, int64_t nseg_in,
   const int64_t nseg_out = nseg_in / 2;
   const int4* tb = reinterpret_cast<const int4*>(table_dev);
   switch (nbins) {
-    case 32: launch_shift_add_score<32>(fold_in, ncoords_prev, tb, ncoords_cur, nseg_out, k, snr_out, s); break;
-    case 64: launch_shift_add_score<64>(fold_in, ncoords_prev, tb, ncoords_cur, nseg_out, k, snr_out, s); break;
-    default: launch_shift_add_score<128>(fold_in, ncoords_prev, tb, ncoords_cur, nseg_out, k, snr_out, s); break;
+    case 32: launch_shift_add_score<32, T>(fold_in, ncoords_prev, tb, ncoords_cur, nseg_out, k, snr_out, s); break;
+    case 64: launch_shift_add_score<64, T>(fold_in, ncoords_prev, tb, ncoords_cur, nseg_out, k, snr_out, s); break;
+    default: launch_shift_add_score<128, T>(fold_in, ncoords_prev, tb, ncoords_cur, nseg_out, k, snr_out, s); break;
   }
   PYL_CHECK_LAUNCH("ffa_shift_add_score_kernel");
+  return PYL_OK;
+}
+
+extern "C" int pyl_ffa_shift_add_score(const float* fold_in, int64_t nseg_in,
+                                       int64_t ncoords_prev, const int32_t* table_dev,
+                                       int64_t ncoords_cur, int32_t nbins, const int64_t* widths,
+                                       int32_t nw, double stdnoise, float* snr_out,
+                                       void* stream) {
+  return shift_add_score<float>(fold_in, nseg_in, ncoords_prev, table_dev, ncoords_cur, nbins,
+                                widths, nw, stdnoise, snr_out, stream);
+}
+
+extern "C" int pyl_ffa_shift_add_score_bf16(const uint16_t* fold_in, int64_t nseg_in,
+                                            int64_t ncoords_prev, const int32_t* table_dev,
+                                            int64_t ncoords_cur, int32_t nbins,
+                                            const int64_t* widths, int32_t nw, double stdnoise,
+                                            float* snr_out, void* stream) {
+  return shift_add_score<uint16_t>(fold_in, nseg_in, ncoords_prev, table_dev, ncoords_cur, nbins,
+                                   widths, nw, stdnoise, snr_out, stream);
+}
+
+extern "C" int pyl_ffa_shift_add_bf16(const uint16_t* fold_in, int64_t nseg_in,
+                                      int64_t ncoords_prev, const int32_t* table_dev,
+                                      int64_t ncoords_cur, int32_t nbins, uint16_t* fold_out,
+                                      void* stream) {
+  PYL_REQUIRE(fold_in && table_dev && fold_out, "null device pointer");
+  PYL_REQUIRE(nseg_in >= 2 && nseg_in % 2 == 0, "nsegments must be even and >= 2, got %lld",
+              (long long)nseg_in);
+  PYL_REQUIRE(ncoords_prev > 0 && ncoords_cur > 0, "empty parameter grid");
+  PYL_REQUIRE(nbins == 32 || nbins == 64 || nbins == 128 || nbins == 256,
+              "BF16 storage supports nbins in {32, 64, 128, 256}, got %d", nbins);
+  hipStream_t s = pyl::as_stream(stream);
+  const int64_t nseg_out = nseg_in / 2;
+  const int4* tb = reinterpret_cast<const int4*>(table_dev);
+  switch (nbins) {
+    case 32: launch_shift_add<32, uint16_t>(fold_in, ncoords_prev, tb, ncoords_cur, nseg_out, fold_out, s); break;
+    case 64: launch_shift_add<64, uint16_t>(fold_in, ncoords_prev, tb, ncoords_cur, nseg_out, fold_out, s); break;
+    case 128: launch_shift_add<128, uint16_t>(fold_in, ncoords_prev, tb, ncoords_cur, nseg_out, fold_out, s); break;
+    default: launch_shift_add<256, uint16_t>(fold_in, ncoords_prev, tb, ncoords_cur, nseg_out, fold_out, s); break;
+  }
+  PYL_CHECK_LAUNCH("ffa_shift_add_kernel<bf16>");
   return PYL_OK;
 }
 

@@ -92,12 +92,12 @@ __global__ void bf_fill_kernel(const uint16_t* __restrict__ pmap, int64_t seg_le
   }
 }
 
-template <bool kUseLds>
+template <bool kUseLds, typename T>
 __global__ void __launch_bounds__(kBlock)
 bf_fold_kernel(const float* __restrict__ ts_e, const float* __restrict__ ts_v, int64_t nsamps,
                int64_t seg_len, int64_t nfreq, int nbins, int fpb,
                const int32_t* __restrict__ cnt, const int64_t* __restrict__ off,
-               const uint32_t* __restrict__ list, float* __restrict__ fold) {
+               const uint32_t* __restrict__ list, T* __restrict__ fold) {
   extern __shared__ float s_seg[];  // [2][seg_len] when kUseLds
   const int64_t seg = blockIdx.x;
   const int64_t f0 = (int64_t)blockIdx.y * fpb;
@@ -133,9 +133,9 @@ bf_fold_kernel(const float* __restrict__ ts_e, const float* __restrict__ ts_v, i
       ae += se[idx];
       av += sv[idx];
     }
-    float* o = fold + ((seg * nfreq + f) * 2) * nbins;
-    o[b] = ae;
-    o[nbins + b] = av;
+    T* o = fold + ((seg * nfreq + f) * 2) * nbins;
+    o[b] = pyl::Store<T>::store(ae);
+    o[nbins + b] = pyl::Store<T>::store(av);
   }
 }
 
@@ -222,9 +222,9 @@ extern "C" int pyl_brutefold_plan_destroy(pyl_bf_plan* p) {
   return PYL_OK;
 }
 
-extern "C" int pyl_brutefold_execute(const pyl_bf_plan* p, const float* ts_e,
-                                     const float* ts_v, int64_t nsamps, float* fold,
-                                     void* stream) {
+template <typename T>
+static int brutefold_execute(const pyl_bf_plan* p, const float* ts_e, const float* ts_v,
+                             int64_t nsamps, T* fold, void* stream) {
   PYL_REQUIRE(p != nullptr, "plan must not be NULL");
   PYL_REQUIRE(ts_e && ts_v && fold, "null device pointer");
   PYL_REQUIRE(nsamps > 0, "empty time series");
@@ -240,12 +240,26 @@ extern "C" int pyl_brutefold_execute(const pyl_bf_plan* p, const float* ts_e,
   while (fpb > 4 && nseg * pyl::ceil_div(p->nfreq, fpb) < target_blocks) fpb >>= 1;
   dim3 grid((unsigned)nseg, (unsigned)pyl::ceil_div(p->nfreq, fpb));
   if (use_lds) {
-    bf_fold_kernel<true><<<grid, kBlock, lds, s>>>(ts_e, ts_v, nsamps, p->seg_len, p->nfreq,
-                                                   p->nbins, fpb, p->cnt, p->off, p->list, fold);
+    bf_fold_kernel<true, T><<<grid, kBlock, lds, s>>>(ts_e, ts_v, nsamps, p->seg_len, p->nfreq,
+                                                      p->nbins, fpb, p->cnt, p->off, p->list,
+                                                      fold);
   } else {
-    bf_fold_kernel<false><<<grid, kBlock, 0, s>>>(ts_e, ts_v, nsamps, p->seg_len, p->nfreq,
-                                                  p->nbins, fpb, p->cnt, p->off, p->list, fold);
+    bf_fold_kernel<false, T><<<grid, kBlock, 0, s>>>(ts_e, ts_v, nsamps, p->seg_len, p->nfreq,
+                                                     p->nbins, fpb, p->cnt, p->off, p->list,
+                                                     fold);
   }
   PYL_CHECK_LAUNCH("bf_fold_kernel");
   return PYL_OK;
+}
+
+extern "C" int pyl_brutefold_execute(const pyl_bf_plan* p, const float* ts_e,
+                                     const float* ts_v, int64_t nsamps, float* fold,
+                                     void* stream) {
+  return brutefold_execute<float>(p, ts_e, ts_v, nsamps, fold, stream);
+}
+
+extern "C" int pyl_brutefold_execute_bf16(const pyl_bf_plan* p, const float* ts_e,
+                                          const float* ts_v, int64_t nsamps, uint16_t* fold,
+                                          void* stream) {
+  return brutefold_execute<uint16_t>(p, ts_e, ts_v, nsamps, fold, stream);
 }

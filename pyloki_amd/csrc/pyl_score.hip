@@ -50,13 +50,19 @@ score_boxcar_kernel(const float* __restrict__ folds, int64_t n, int nbins, int l
   const int tid = threadIdx.x;
   const int64_t p0 = (int64_t)blockIdx.x * pyl::kScoreBlock;
   const int nprof = (int)std::min<int64_t>(pyl::kScoreBlock, n - p0);
-  const int64_t rowlen = layout == PYL_LAYOUT_EV ? 2 * (int64_t)nbins : (int64_t)nbins;
+  const int64_t rowlen = layout == PYL_LAYOUT_NORM ? (int64_t)nbins : 2 * (int64_t)nbins;
   for (int r = 0; r < nprof; ++r) {
-    const float* row = folds + (p0 + r) * rowlen;
     for (int b = tid; b < nbins; b += pyl::kScoreBlock) {
       float x;
-      if (layout == PYL_LAYOUT_EV) x = row[b] / sqrtf(row[nbins + b]);
-      else x = row[b];
+      if (layout == PYL_LAYOUT_EV) {
+        const float* row = folds + (p0 + r) * rowlen;
+        x = row[b] / sqrtf(row[nbins + b]);
+      } else if (layout == PYL_LAYOUT_EV_BF16) {
+        const uint16_t* row = reinterpret_cast<const uint16_t*>(folds) + (p0 + r) * rowlen;
+        x = pyl::bf16_to_f32(row[b]) / sqrtf(pyl::bf16_to_f32(row[nbins + b]));
+      } else {
+        x = folds[(p0 + r) * rowlen + b];
+      }
       S[b * pyl::kScoreStride + r] = x;
     }
   }
@@ -74,14 +80,16 @@ score_boxcar_kernel(const float* __restrict__ folds, int64_t n, int nbins, int l
 extern "C" int pyl_score_boxcar(const float* folds, int64_t n, int32_t nbins, int32_t layout,
                                 const int64_t* widths, int32_t nw, double stdnoise, int32_t mode,
                                 float* out, void* stream) {
-  PYL_REQUIRE(folds && out && widths, "null pointer argument");
+  PYL_REQUIRE(widths, "null widths");
   PYL_REQUIRE(n >= 0, "negative profile count");
-  PYL_REQUIRE(layout == PYL_LAYOUT_EV || layout == PYL_LAYOUT_NORM, "invalid layout %d", layout);
+  PYL_REQUIRE(layout == PYL_LAYOUT_EV || layout == PYL_LAYOUT_NORM || layout == PYL_LAYOUT_EV_BF16,
+              "invalid layout %d", layout);
   PYL_REQUIRE(mode == PYL_SCORE_PER_WIDTH || mode == PYL_SCORE_MAX, "invalid mode %d", mode);
   pyl::ScoreConst k;
   int rc = pyl::build_score_const(nbins, widths, nw, stdnoise, &k);
   if (rc != PYL_OK) return rc;
   if (n == 0) return PYL_OK;
+  PYL_REQUIRE(folds && out, "null device pointer");
   const size_t lds = sizeof(float) * (size_t)(nbins + k.wmax) * pyl::kScoreStride;
   score_boxcar_kernel<<<(unsigned)pyl::ceil_div(n, pyl::kScoreBlock), pyl::kScoreBlock, lds,
                         pyl::as_stream(stream)>>>(folds, n, nbins, layout, k, mode, out);

@@ -79,10 +79,13 @@ class FFAPlan:
         return -(-nsamps // self.cfg.bseg_brute) // (2**level)
 
     def run(self, ts_e: torch.Tensor, ts_v: torch.Tensor, nlevels: int | None = None,
-            keep: bool = False) -> torch.Tensor | list[torch.Tensor]:
-        """Brute fold + ``nlevels`` levels; returns the last fold (or all, ``keep``)."""
+            keep: bool = False, dtype: torch.dtype = torch.float32
+            ) -> torch.Tensor | list[torch.Tensor]:
+        """Brute fold + ``nlevels`` levels; returns the last fold (or all, ``keep``).
+
+        ``dtype`` selects the fold storage: FP32, or BF16 with FP32 accumulation."""
         nl = self.nlevels if nlevels is None else nlevels
-        fold = self.bf.fold(ts_e, ts_v)
+        fold = self.bf.fold(ts_e, ts_v, dtype=dtype)
         folds = [fold] if keep else None
         for lp in self.levels[:nl]:
             if fold.shape[0] < 2:
@@ -236,8 +239,8 @@ def compute_ffa(tseries: TimeSeries, search_cfg: PulsarSearchConfig, *, quiet: b
 
 
 def ffa_batch(plan: FFAPlan, series: list[tuple[torch.Tensor, torch.Tensor]],
-              widths: np.ndarray, stdnoise: float = 1.0, *, fused: bool = True
-              ) -> list[torch.Tensor]:
+              widths: np.ndarray, stdnoise: float = 1.0, *, fused: bool = True,
+              dtype: torch.dtype = torch.float32) -> list[torch.Tensor]:
     """Fold and score many series of (possibly) different lengths in one pass per level.
 
     All series share ``plan`` (same ``bseg_brute`` and grids); series are ordered longest
@@ -264,8 +267,7 @@ def ffa_batch(plan: FFAPlan, series: list[tuple[torch.Tensor, torch.Tensor]],
         msg = "plan has fewer levels than the longest series needs"
         raise ValueError(msg)
     dev = series[0][0].device
-    fold = torch.empty((sum(nseg), plan.bf.nfreq, 2, plan.cfg.nbins), dtype=torch.float32,
-                       device=dev)
+    fold = torch.empty((sum(nseg), plan.bf.nfreq, 2, plan.cfg.nbins), dtype=dtype, device=dev)
     off = 0
     for i, n in zip(order, nseg):
         plan.bf.fold(series[i][0], series[i][1], out=fold[off:off + n])

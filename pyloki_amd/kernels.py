@@ -39,8 +39,9 @@ class BruteFoldPlan:
             self.nbins, self.tsamp, self.t_ref, stream_ptr(), ctypes.byref(handle)))
         self._handle = handle
 
-    def fold(self, ts_e: torch.Tensor, ts_v: torch.Tensor,
-             out: torch.Tensor | None = None) -> torch.Tensor:
+    def fold(self, ts_e: torch.Tensor, ts_v: torch.Tensor, out: torch.Tensor | None = None,
+             dtype: torch.dtype = torch.float32) -> torch.Tensor:
+        """Fold into FP32 or BF16 storage (FP32 accumulation either way)."""
         if ts_e.shape != ts_v.shape or ts_e.dim() != 1:
             msg = "ts_e, ts_v must be 1-D and of the same length."
             raise ValueError(msg)
@@ -51,12 +52,13 @@ class BruteFoldPlan:
         nseg = -(-nsamps // self.seg_len)
         shape = (nseg, self.nfreq, 2, self.nbins)
         if out is None:
-            out = torch.empty(shape, dtype=torch.float32, device=ts_e.device)
-        elif tuple(out.shape) != shape or out.dtype != torch.float32:
-            msg = f"out must be float32 of shape {shape}"
+            out = torch.empty(shape, dtype=dtype, device=ts_e.device)
+        elif tuple(out.shape) != shape or out.dtype not in (torch.float32, torch.bfloat16):
+            msg = f"out must be float32 or bfloat16 of shape {shape}"
             raise ValueError(msg)
-        _lib.check(_lib.load().pyl_brutefold_execute(
-            self._handle, ptr(ts_e), ptr(ts_v), nsamps, ptr(out), stream_ptr()))
+        fn = (_lib.load().pyl_brutefold_execute if out.dtype == torch.float32
+              else _lib.load().pyl_brutefold_execute_bf16)
+        _lib.check(fn(self._handle, ptr(ts_e), ptr(ts_v), nsamps, ptr(out), stream_ptr()))
         return out
 
     def __del__(self) -> None:
@@ -95,9 +97,12 @@ def ffa_resolve(params_dev: torch.Tensor, counts_cur, counts_prev, param_limits:
 
 def ffa_shift_add(fold_in: torch.Tensor, table: torch.Tensor, ncoords_cur: int,
                   out: torch.Tensor | None = None) -> torch.Tensor:
-    """Segment-pair shift-add of one level: [nseg, nprev, 2, B] -> [nseg/2, ncur, 2, B]."""
-    if fold_in.dim() != 4 or fold_in.shape[2] != 2 or fold_in.dtype != torch.float32:
-        msg = "fold_in must be float32 [nseg, ncoords, 2, nbins]"
+    """Segment-pair shift-add of one level: [nseg, nprev, 2, B] -> [nseg/2, ncur, 2, B].
+
+    FP32 or BF16 storage (FP32 arithmetic); the output has the input's dtype."""
+    if fold_in.dim() != 4 or fold_in.shape[2] != 2 or fold_in.dtype not in (
+            torch.float32, torch.bfloat16):
+        msg = "fold_in must be float32/bfloat16 [nseg, ncoords, 2, nbins]"
         raise ValueError(msg)
     nseg, nprev, _, nbins = fold_in.shape
     if nseg % 2:
@@ -108,12 +113,14 @@ def ffa_shift_add(fold_in: torch.Tensor, table: torch.Tensor, ncoords_cur: int,
         raise ValueError(msg)
     shape = (nseg // 2, ncoords_cur, 2, nbins)
     if out is None:
-        out = torch.empty(shape, dtype=torch.float32, device=fold_in.device)
-    elif tuple(out.shape) != shape:
-        msg = f"out must have shape {shape}"
+        out = torch.empty(shape, dtype=fold_in.dtype, device=fold_in.device)
+    elif tuple(out.shape) != shape or out.dtype != fold_in.dtype:
+        msg = f"out must have shape {shape} and dtype {fold_in.dtype}"
         raise ValueError(msg)
-    _lib.check(_lib.load().pyl_ffa_shift_add(
-        ptr(fold_in), nseg, nprev, ptr(table), ncoords_cur, nbins, ptr(out), stream_ptr()))
+    fn = (_lib.load().pyl_ffa_shift_add if fold_in.dtype == torch.float32
+          else _lib.load().pyl_ffa_shift_add_bf16)
+    _lib.check(fn(ptr(fold_in), nseg, nprev, ptr(table), ncoords_cur, nbins, ptr(out),
+                  stream_ptr()))
     return out
 
 
@@ -121,8 +128,9 @@ def ffa_shift_add_score(fold_in: torch.Tensor, table: torch.Tensor, ncoords_cur:
                         widths: np.ndarray, stdnoise: float = 1.0,
                         out: torch.Tensor | None = None) -> torch.Tensor:
     """Last level fused with scoring: [nseg, nprev, 2, B] -> S/N [nseg/2, ncur, n_w]."""
-    if fold_in.dim() != 4 or fold_in.shape[2] != 2 or fold_in.dtype != torch.float32:
-        msg = "fold_in must be float32 [nseg, ncoords, 2, nbins]"
+    if fold_in.dim() != 4 or fold_in.shape[2] != 2 or fold_in.dtype not in (
+            torch.float32, torch.bfloat16):
+        msg = "fold_in must be float32/bfloat16 [nseg, ncoords, 2, nbins]"
         raise ValueError(msg)
     nseg, nprev, _, nbins = fold_in.shape
     if nseg % 2:
@@ -138,9 +146,10 @@ def ffa_shift_add_score(fold_in: torch.Tensor, table: torch.Tensor, ncoords_cur:
     elif tuple(out.shape) != shape:
         msg = f"out must have shape {shape}"
         raise ValueError(msg)
-    _lib.check(_lib.load().pyl_ffa_shift_add_score(
-        ptr(fold_in), nseg, nprev, ptr(table), ncoords_cur, nbins, _lib.i64_array(widths),
-        len(widths), float(stdnoise), ptr(out), stream_ptr()))
+    fn = (_lib.load().pyl_ffa_shift_add_score if fold_in.dtype == torch.float32
+          else _lib.load().pyl_ffa_shift_add_score_bf16)
+    _lib.check(fn(ptr(fold_in), nseg, nprev, ptr(table), ncoords_cur, nbins,
+                  _lib.i64_array(widths), len(widths), float(stdnoise), ptr(out), stream_ptr()))
     return out
 
 
@@ -150,7 +159,7 @@ def score_boxcar(folds: torch.Tensor, widths: np.ndarray, stdnoise: float = 1.0,
     """Boxcar S/N of profiles: EV rows [..., 2, B] or normalised rows [..., B]."""
     widths = np.ascontiguousarray(widths, dtype=np.int64)
     nbins = folds.shape[-1]
-    if layout == _lib.PYL_LAYOUT_EV:
+    if layout in (_lib.PYL_LAYOUT_EV, _lib.PYL_LAYOUT_EV_BF16):
         if folds.dim() < 2 or folds.shape[-2] != 2:
             msg = "EV layout needs [..., 2, nbins]"
             raise ValueError(msg)
@@ -158,8 +167,10 @@ def score_boxcar(folds: torch.Tensor, widths: np.ndarray, stdnoise: float = 1.0,
     else:
         lead = tuple(folds.shape[:-1])
     n = int(np.prod(lead)) if lead else 1
-    if folds.dtype != torch.float32:
-        msg = "folds must be float32"
+    if folds.dtype == torch.bfloat16 and layout == _lib.PYL_LAYOUT_EV:
+        layout = _lib.PYL_LAYOUT_EV_BF16
+    elif folds.dtype != torch.float32:
+        msg = "folds must be float32 (or bfloat16 (e, v) rows)"
         raise ValueError(msg)
     shape = (*lead, len(widths)) if mode == _lib.PYL_SCORE_PER_WIDTH else lead
     if out is None:

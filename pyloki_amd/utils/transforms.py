@@ -45,6 +45,32 @@ def shift_taylor_params(taylor_param_vec: np.ndarray, delta_t: float, n_out: int
     return out.reshape(*vec.shape[:-1], n_out)
 
 
+def shift_taylor_errors(taylor_error_vec: np.ndarray, delta_t: float,
+                        tiling_strategy: str = "aggressive") -> np.ndarray:
+    """Propagate cell widths ``[dd_k .. dd_0]`` by dt (``transforms.py:50-90``)."""
+    err = np.asarray(taylor_error_vec, dtype=np.float64)
+    n = err.shape[-1]
+    t_mat = taylor_shift_matrix(n, delta_t)
+    flat = err.reshape(-1, n)
+    if tiling_strategy == "aggressive":
+        out = flat * np.abs(np.diag(t_mat))[None, :]
+    elif tiling_strategy == "conservative":
+        out = np.zeros_like(flat)
+        for i in range(n):
+            for j in range(n):
+                out[:, i] = out[:, i] + flat[:, j] * abs(t_mat[i, j])
+    elif tiling_strategy == "quadrature":
+        out = np.zeros_like(flat)
+        for i in range(n):
+            for j in range(n):
+                out[:, i] = out[:, i] + flat[:, j] ** 2 * t_mat[i, j] ** 2
+        out = np.sqrt(out)
+    else:
+        msg = f"Invalid tiling strategy: {tiling_strategy}"
+        raise ValueError(msg)
+    return out.reshape(err.shape)
+
+
 def taylor_to_cheby(taylor_vec: np.ndarray, t_s: float) -> np.ndarray:
     """Taylor ``[d_k, ..., d_0]`` to Chebyshev ``[alpha_k, ..., alpha_0]`` on [-t_s, t_s].
 

@@ -124,3 +124,23 @@ def test_config_validation_errors(kwargs, msg):
 def test_range_param_invalid():
     with pytest.raises(ValueError, match="vmin < vmax"):
         psr_utils.range_param(1.0, 1.0, 0.1)
+
+
+@pytest.mark.parametrize("name,key,ref", [
+    ("ep_accel", "branching_pattern_poly_taylor_moving_ref0", 0),
+    ("ep_jerk", "branching_pattern_poly_taylor_moving_ref64", 64),
+])
+def test_branching_pattern_matches_notebook(name, key, ref):
+    g = GOLD[name]
+    fu = g["from_upper"]
+    p = ParamLimits.from_upper(tuple(fu["freq"]), fu["true_d_params"],
+                               tuple(fu["d_upper_limits"]), g["nsamps"] * g["tsamp"])
+    cfg = PulsarSearchConfig(
+        nsamps=g["nsamps"], tsamp=g["tsamp"], nbins=g["nbins"], eta=g["eta"],
+        param_limits=p.limits, bseg_brute=g["bseg_brute"], bseg_ffa=g["bseg_ffa"],
+        prune_poly_order=g["prune_poly_order"], ducy_max=g["ducy_max"], wtsp=g["wtsp"],
+        branch_max=g["branch_max"],
+    )
+    bp = cfg.generate_branching_pattern("poly_taylor_moving", ref)
+    # numpy printed 8 decimals (9 significant digits)
+    np.testing.assert_allclose(bp, np.array(g[key]), rtol=6e-9, atol=5e-9)
