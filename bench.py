@@ -38,7 +38,25 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--nseries", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-series", type=int, default=2)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device-to-device 4 times (HBM counter calibration)")
     return ap.parse_args()
+
+
+def copy_calibration() -> int:
+    """A plain device copy of known size: 4 launches, each reading and writing 1 GiB.
+    Profiling it with the FETCH_SIZE / WRITE_SIZE counters gives the factor that turns
+    those counters into bytes on this device."""
+    import torch
+
+    n = (1 << 30) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(4):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_per_launch": n * 4, "launches": 4}))
+    return 0
 
 
 def make_config(nsamps: int):
@@ -88,6 +106,8 @@ def cpu_baseline(series, nsample: int) -> dict:
 
 def main() -> int:
     args = parse()
+    if args.copy_calibration:
+        return copy_calibration()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the FFA batch "
               "would shard by series across GPUs with no communication.", file=sys.stderr)

@@ -86,10 +86,12 @@ int pyl_ffa_shift_add(const float* fold_in_dev, int64_t nseg_in, int64_t ncoords
  * of every output row (unify_fold dyn_pffa.py:481-489 + get_fold_norm ffa.py:411 +
  * boxcar_snr_nd scoring.py:156-165), without storing the level's fold: snr_out f32
  * [nseg_in/2][ncoords_cur][nw]. Used for a series' last level (search.py:44-47).
+ * n_inner = grid count of the last (frequency) axis; it must divide ncoords_cur and sets
+ * the output tiling (rows of neighbouring outer indices share source rows).
  * nbins in {32, 64, 128}; widths as for pyl_score_boxcar. */
 int pyl_ffa_shift_add_score(const float* fold_in_dev, int64_t nseg_in, int64_t ncoords_prev,
-                            const int32_t* table_dev, int64_t ncoords_cur, int32_t nbins,
-                            const int64_t* widths, int32_t nw, double stdnoise,
+                            const int32_t* table_dev, int64_t ncoords_cur, int64_t n_inner,
+                            int32_t nbins, const int64_t* widths, int32_t nw, double stdnoise,
                             float* snr_out_dev, void* stream);
 /* BF16-storage / FP32-accumulate variants of the level kernels: rows are loaded as BF16,
  * added in FP32 and stored rounded to BF16 (RNE); the fused variant normalises and scores
@@ -99,8 +101,9 @@ int pyl_ffa_shift_add_bf16(const uint16_t* fold_in_dev, int64_t nseg_in, int64_t
                            uint16_t* fold_out_dev, void* stream);
 int pyl_ffa_shift_add_score_bf16(const uint16_t* fold_in_dev, int64_t nseg_in,
                                  int64_t ncoords_prev, const int32_t* table_dev,
-                                 int64_t ncoords_cur, int32_t nbins, const int64_t* widths,
-                                 int32_t nw, double stdnoise, float* snr_out_dev, void* stream);
+                                 int64_t ncoords_cur, int64_t n_inner, int32_t nbins,
+                                 const int64_t* widths, int32_t nw, double stdnoise,
+                                 float* snr_out_dev, void* stream);
 int pyl_ffa_level(int32_t nparams, const double* params_dev, const int64_t* counts_cur,
                   const int64_t* counts_prev, const double* param_limits, int32_t ffa_level,
                   double tseg_brute, int32_t nbins, const float* fold_in_dev, int64_t nseg_in,

@@ -42,14 +42,14 @@ SIGNATURES: list[tuple[str, object, list[object]]] = [
     ("pyl_ffa_shift_add_bf16", ctypes.c_int,
      [_vp, _c_i64, _c_i64, _vp, _c_i64, _c_i32, _vp, _vp]),
     ("pyl_ffa_shift_add_score_bf16", ctypes.c_int,
-     [_vp, _c_i64, _c_i64, _vp, _c_i64, _c_i32, _i64p, _c_i32, _c_dbl, _vp, _vp]),
+     [_vp, _c_i64, _c_i64, _vp, _c_i64, _c_i64, _c_i32, _i64p, _c_i32, _c_dbl, _vp, _vp]),
     ("pyl_ffa_resolve", ctypes.c_int,
      [_c_i32, _vp, _i64p, _i64p, _f64p, _c_i32, _c_dbl, _c_i32, _vp, _vp, _vp]),
     ("pyl_ffa_shift_add", ctypes.c_int, [_vp, _c_i64, _c_i64, _vp, _c_i64, _c_i32, _vp, _vp]),
     ("pyl_ffa_level", ctypes.c_int,
      [_c_i32, _vp, _i64p, _i64p, _f64p, _c_i32, _c_dbl, _c_i32, _vp, _c_i64, _vp, _vp, _vp]),
     ("pyl_ffa_shift_add_score", ctypes.c_int,
-     [_vp, _c_i64, _c_i64, _vp, _c_i64, _c_i32, _i64p, _c_i32, _c_dbl, _vp, _vp]),
+     [_vp, _c_i64, _c_i64, _vp, _c_i64, _c_i64, _c_i32, _i64p, _c_i32, _c_dbl, _vp, _vp]),
     ("pyl_score_boxcar", ctypes.c_int,
      [_vp, _c_i64, _c_i32, _c_i32, _i64p, _c_i32, _c_dbl, _c_i32, _vp, _vp]),
     ("pyl_prune_branch", ctypes.c_int, [_vp, _vp, _vp, _c_i64, _vp, _vp, _vp, _vp, _vp]),

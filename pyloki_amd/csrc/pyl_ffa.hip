@@ -260,6 +260,83 @@ ffa_shift_add_score_kernel(const T* __restrict__ in, int64_t ncoords_prev,
     pyl::score_column<true>(k, B, S, tid, out + (q * ncoords_cur + j0 + tid) * k.nw);
 }
 
+// Register-scoring variant of the fused last level (nbins 32 or 64). A block owns a tile of
+// 256 output rows of one segment pair: tile_o_rows (1, 2 or 4) "outer" x 256/tile_o_rows
+// "inner" rows, inner = index along the last (frequency) axis, outer = the product of the
+// other axes. Rows at neighbouring outer
+// indices (e.g. accelerations) resolve to the same source rows, so the tile reuses each
+// source row from L1/L2 several times instead of re-reading it from HBM.
+// Phase 1 forms the rows with coalesced, rotated float4 loads (16 lanes per row at B = 64)
+// and writes the normalised values transposed into LDS (row stride 257 words:
+// conflict-free both ways). Phase 2: every thread pulls its row into registers, scans it
+// sequentially and scores it with the unrolled max-difference loops of pyl_score.hpp.
+template <int B, typename T>
+__global__ void __launch_bounds__(256)
+ffa_shift_add_score_regs_kernel(const T* __restrict__ in, int64_t ncoords_prev,
+                                const int4* __restrict__ table, int64_t ncoords_cur,
+                                int64_t n_inner, int tile_o_rows, int64_t tiles_inner,
+                                int64_t tiles_per_pair, pyl::ScoreConst k,
+                                float* __restrict__ out) {
+  constexpr int kRows = 256;
+  constexpr int kLds = kRows + 1;
+  constexpr int CPR = B / 4;       // float4 chunks per component
+  constexpr int RPS = 64 / CPR;    // rows per wavefront step
+  __shared__ float S[B * kLds];
+  __shared__ int4 s_tab[kRows];
+  __shared__ int64_t s_j[kRows];
+  const int tid = threadIdx.x;
+  const int64_t q = blockIdx.x / tiles_per_pair;
+  const int64_t tile = blockIdx.x - q * tiles_per_pair;
+  const int64_t tile_o = tile / tiles_inner;
+  const int64_t tile_i = tile - tile_o * tiles_inner;
+  const int64_t n_outer = ncoords_cur / n_inner;
+  {
+    const int tile_i_rows = kRows / tile_o_rows;
+    const int64_t outer = tile_o * tile_o_rows + tid / tile_i_rows;
+    const int64_t inner = tile_i * tile_i_rows + tid % tile_i_rows;
+    const bool ok = outer < n_outer && inner < n_inner;
+    const int64_t j = ok ? outer * n_inner + inner : -1;
+    s_j[tid] = j;
+    if (ok) s_tab[tid] = table[j];
+  }
+  __syncthreads();
+  constexpr int64_t E = 2 * B;
+  const int wave = tid >> 6;
+  const int lane = tid & 63;
+  const int c4 = lane % CPR;
+  const int rsub = lane / CPR;
+  const T* base_t = in + (2 * q) * ncoords_prev * E;
+  const T* base_h = in + (2 * q + 1) * ncoords_prev * E;
+#pragma unroll 4
+  for (int step = 0; step < 64 / RPS; ++step) {
+    const int r = wave * 64 + step * RPS + rsub;
+    if (s_j[r] >= 0) {
+      const int4 tb = s_tab[r];
+      const T* tail = base_t + (int64_t)tb.x * E;
+      const T* head = base_h + (int64_t)tb.z * E;
+      const float4 te = rotated_chunk<B, T>(tail, c4, tb.y);
+      const float4 he = rotated_chunk<B, T>(head, c4, tb.w);
+      const float4 tv = rotated_chunk<B, T>(tail + B, c4, tb.y);
+      const float4 hv = rotated_chunk<B, T>(head + B, c4, tb.w);
+      auto st = [](float x) { return pyl::Store<T>::load(pyl::Store<T>::store(x)); };
+      const int b = 4 * c4;
+      S[(b + 0) * kLds + r] = st(te.x + he.x) / sqrtf(st(tv.x + hv.x));
+      S[(b + 1) * kLds + r] = st(te.y + he.y) / sqrtf(st(tv.y + hv.y));
+      S[(b + 2) * kLds + r] = st(te.z + he.z) / sqrtf(st(tv.z + hv.z));
+      S[(b + 3) * kLds + r] = st(te.w + he.w) / sqrtf(st(tv.w + hv.w));
+    }
+  }
+  __syncthreads();
+  const int64_t j = s_j[tid];
+  if (j >= 0) {
+    float x[B];
+#pragma unroll
+    for (int i = 0; i < B; ++i) x[i] = S[i * kLds + tid];
+    pyl::scan_regs<B>(x);
+    pyl::score_regs<B, true>(k, x, out + (q * ncoords_cur + j) * k.nw);
+  }
+}
+
 }  // namespace
 
 extern "C" int pyl_ffa_resolve(int32_t nparams, const double* params_dev,
@@ -354,8 +431,19 @@ extern "C" int pyl_ffa_shift_add(const float* fold_in, int64_t nseg_in, int64_t 
 
 template <int B, typename T>
 static void launch_shift_add_score(const T* in, int64_t nprev, const int4* tb, int64_t ncur,
-                                   int64_t nseg_out, const pyl::ScoreConst& k, float* out,
-                                   hipStream_t s) {
+                                   int64_t n_inner, int64_t nseg_out, const pyl::ScoreConst& k,
+                                   float* out, hipStream_t s) {
+  (void)n_inner;
+  if constexpr (B <= 64) {
+    const int64_t n_outer = ncur / n_inner;
+    const int to = n_outer >= 4 ? 4 : (n_outer >= 2 ? 2 : 1);
+    const int64_t tiles_inner = pyl::ceil_div(n_inner, 256 / to);
+    const int64_t tiles_outer = pyl::ceil_div(n_outer, to);
+    const int64_t per_pair = tiles_inner * tiles_outer;
+    ffa_shift_add_score_regs_kernel<B, T><<<(unsigned)(per_pair * nseg_out), 256, 0, s>>>(
+        in, nprev, tb, ncur, n_inner, to, tiles_inner, per_pair, k, out);
+    return;
+  }
   const int64_t nblk_j = pyl::ceil_div(ncur, pyl::kScoreBlock);
   const size_t lds = sizeof(float) * (size_t)(B + k.wmax) * pyl::kScoreStride;
   ffa_shift_add_score_kernel<B, T><<<(unsigned)(nblk_j * nseg_out), pyl::kScoreBlock, lds, s>>>(
@@ -364,15 +452,17 @@ static void launch_shift_add_score(const T* in, int64_t nprev, const int4* tb, i
 
 template <typename T>
 static int shift_add_score(const T* fold_in, int64_t nseg_in, int64_t ncoords_prev,
-                           const int32_t* table_dev, int64_t ncoords_cur, int32_t nbins,
-                           const int64_t* widths, int32_t nw, double stdnoise, float* snr_out,
-                           void* stream) {
+                           const int32_t* table_dev, int64_t ncoords_cur, int64_t n_inner,
+                           int32_t nbins, const int64_t* widths, int32_t nw, double stdnoise,
+                           float* snr_out, void* stream) {
   PYL_REQUIRE(fold_in && table_dev && snr_out, "null device pointer");
   PYL_REQUIRE(nseg_in >= 2 && nseg_in % 2 == 0, "nsegments must be even and >= 2, got %lld",
               (long long)nseg_in);
   PYL_REQUIRE(ncoords_prev > 0 && ncoords_cur > 0, "empty parameter grid");
   PYL_REQUIRE(nbins == 32 || nbins == 64 || nbins == 128,
               "fused level + score supports nbins in {32, 64, 128}, got %d", nbins);
+  PYL_REQUIRE(n_inner >= 1 && ncoords_cur % n_inner == 0,
+              "n_inner must divide ncoords_cur (last-axis grid count)");
   pyl::ScoreConst k;
   int rc = pyl::build_score_const(nbins, widths, nw, stdnoise, &k);
   if (rc != PYL_OK) return rc;
@@ -380,9 +470,9 @@ static int shift_add_score(const T* fold_in, int64_t nseg_in, int64_t ncoords_pr
   const int64_t nseg_out = nseg_in / 2;
   const int4* tb = reinterpret_cast<const int4*>(table_dev);
   switch (nbins) {
-    case 32: launch_shift_add_score<32, T>(fold_in, ncoords_prev, tb, ncoords_cur, nseg_out, k, snr_out, s); break;
-    case 64: launch_shift_add_score<64, T>(fold_in, ncoords_prev, tb, ncoords_cur, nseg_out, k, snr_out, s); break;
-    default: launch_shift_add_score<128, T>(fold_in, ncoords_prev, tb, ncoords_cur, nseg_out, k, snr_out, s); break;
+    case 32: launch_shift_add_score<32, T>(fold_in, ncoords_prev, tb, ncoords_cur, n_inner, nseg_out, k, snr_out, s); break;
+    case 64: launch_shift_add_score<64, T>(fold_in, ncoords_prev, tb, ncoords_cur, n_inner, nseg_out, k, snr_out, s); break;
+    default: launch_shift_add_score<128, T>(fold_in, ncoords_prev, tb, ncoords_cur, n_inner, nseg_out, k, snr_out, s); break;
   }
   PYL_CHECK_LAUNCH("ffa_shift_add_score_kernel");
   return PYL_OK;
@@ -390,20 +480,20 @@ static int shift_add_score(const T* fold_in, int64_t nseg_in, int64_t ncoords_pr
 
 extern "C" int pyl_ffa_shift_add_score(const float* fold_in, int64_t nseg_in,
                                        int64_t ncoords_prev, const int32_t* table_dev,
-                                       int64_t ncoords_cur, int32_t nbins, const int64_t* widths,
-                                       int32_t nw, double stdnoise, float* snr_out,
-                                       void* stream) {
-  return shift_add_score<float>(fold_in, nseg_in, ncoords_prev, table_dev, ncoords_cur, nbins,
-                                widths, nw, stdnoise, snr_out, stream);
+                                       int64_t ncoords_cur, int64_t n_inner, int32_t nbins,
+                                       const int64_t* widths, int32_t nw, double stdnoise,
+                                       float* snr_out, void* stream) {
+  return shift_add_score<float>(fold_in, nseg_in, ncoords_prev, table_dev, ncoords_cur, n_inner,
+                                nbins, widths, nw, stdnoise, snr_out, stream);
 }
 
 extern "C" int pyl_ffa_shift_add_score_bf16(const uint16_t* fold_in, int64_t nseg_in,
                                             int64_t ncoords_prev, const int32_t* table_dev,
-                                            int64_t ncoords_cur, int32_t nbins,
+                                            int64_t ncoords_cur, int64_t n_inner, int32_t nbins,
                                             const int64_t* widths, int32_t nw, double stdnoise,
                                             float* snr_out, void* stream) {
-  return shift_add_score<uint16_t>(fold_in, nseg_in, ncoords_prev, table_dev, ncoords_cur, nbins,
-                                   widths, nw, stdnoise, snr_out, stream);
+  return shift_add_score<uint16_t>(fold_in, nseg_in, ncoords_prev, table_dev, ncoords_cur,
+                                   n_inner, nbins, widths, nw, stdnoise, snr_out, stream);
 }
 
 extern "C" int pyl_ffa_shift_add_bf16(const uint16_t* fold_in, int64_t nseg_in,

@@ -63,4 +63,57 @@ __device__ inline double score_column(const ScoreConst& k, int nbins, float* S, 
   return best;
 }
 
+// ---- Register-resident scoring for small compile-time nbins (32, 64) ----------------
+// The profile's prefix sums live in B registers of the owning thread; the one-wrap
+// extension S[B + j] = S[j] + S[B-1] is formed on the fly (same FP32 addition as the
+// stored extension), and the max-difference loop is fully unrolled for each width, so
+// no LDS traffic is needed after the transpose.
+
+// In-place sequential inclusive scan (scoring.py:53-55).
+template <int B>
+__device__ __forceinline__ void scan_regs(float (&S)[B]) {
+#pragma unroll
+  for (int i = 1; i < B; ++i) S[i] = S[i - 1] + S[i];
+}
+
+template <int B, int W>
+__device__ __forceinline__ float maxdiff_regs(const float (&S)[B]) {
+  float dmax = -FLT_MAX;
+#pragma unroll
+  for (int i = 0; i < B; ++i) {
+    const float hi = (i + W < B) ? S[i + W] : (S[i + W - B] + S[B - 1]);
+    const float d = hi - S[i];
+    dmax = d > dmax ? d : dmax;
+  }
+  return dmax;
+}
+
+template <int B, int W = 1>
+__device__ __forceinline__ float maxdiff_dispatch(const float (&S)[B], int w) {
+  if constexpr (W >= B) {
+    return -FLT_MAX;  // unreachable: widths are validated to lie in [1, B-1]
+  } else {
+    if (w == W) return maxdiff_regs<B, W>(S);
+    return maxdiff_dispatch<B, W + 1>(S, w);
+  }
+}
+
+// S/N of every width from register prefix sums; per width to out[w] (kPerWidth) and the
+// FP64 maximum returned.
+template <int B, bool kPerWidth>
+__device__ __forceinline__ double score_regs(const ScoreConst& k, const float (&S)[B],
+                                             float* out) {
+  const float total = S[B - 1];
+  double best = -(double)FLT_MAX;
+  for (int w = 0; w < k.nw; ++w) {
+    const float dmax = maxdiff_dispatch<B>(S, k.widths[w]);
+    const double t1 = k.hpb[w] * (double)dmax;
+    const double t2 = k.bb[w] * (double)total;
+    const double snr = t1 - t2;
+    if (kPerWidth) out[w] = (float)(snr * k.inv_std);
+    best = snr > best ? snr : best;
+  }
+  return best;
+}
+
 }  // namespace pyl

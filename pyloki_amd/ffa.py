@@ -290,7 +290,8 @@ def ffa_batch(plan: FFAPlan, series: list[tuple[torch.Tensor, torch.Tensor]],
         if fin:
             part = fold[base:base + 2 * len(fin)]
             if fused:
-                snr = kernels.ffa_shift_add_score(part, lp.table, lp.ncoords, widths, stdnoise)
+                snr = kernels.ffa_shift_add_score(part, lp.table, lp.ncoords, widths, stdnoise,
+                                                  n_inner=int(lp.counts_cur[-1]))
                 for m, k in enumerate(fin):
                     results[order[k]] = snr[m]
             else:

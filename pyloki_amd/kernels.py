@@ -126,8 +126,12 @@ def ffa_shift_add(fold_in: torch.Tensor, table: torch.Tensor, ncoords_cur: int,
 
 def ffa_shift_add_score(fold_in: torch.Tensor, table: torch.Tensor, ncoords_cur: int,
                         widths: np.ndarray, stdnoise: float = 1.0,
-                        out: torch.Tensor | None = None) -> torch.Tensor:
-    """Last level fused with scoring: [nseg, nprev, 2, B] -> S/N [nseg/2, ncur, n_w]."""
+                        out: torch.Tensor | None = None, n_inner: int | None = None
+                        ) -> torch.Tensor:
+    """Last level fused with scoring: [nseg, nprev, 2, B] -> S/N [nseg/2, ncur, n_w].
+
+    ``n_inner`` is the last-axis (frequency) grid count (defaults to ``ncoords_cur``); it
+    shapes the output tiling only, not the result."""
     if fold_in.dim() != 4 or fold_in.shape[2] != 2 or fold_in.dtype not in (
             torch.float32, torch.bfloat16):
         msg = "fold_in must be float32/bfloat16 [nseg, ncoords, 2, nbins]"
@@ -148,7 +152,8 @@ def ffa_shift_add_score(fold_in: torch.Tensor, table: torch.Tensor, ncoords_cur:
         raise ValueError(msg)
     fn = (_lib.load().pyl_ffa_shift_add_score if fold_in.dtype == torch.float32
           else _lib.load().pyl_ffa_shift_add_score_bf16)
-    _lib.check(fn(ptr(fold_in), nseg, nprev, ptr(table), ncoords_cur, nbins,
+    n_inner = ncoords_cur if n_inner is None else int(n_inner)
+    _lib.check(fn(ptr(fold_in), nseg, nprev, ptr(table), ncoords_cur, n_inner, nbins,
                   _lib.i64_array(widths), len(widths), float(stdnoise), ptr(out), stream_ptr()))
     return out
 

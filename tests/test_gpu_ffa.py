@@ -143,9 +143,11 @@ def test_fused_last_level_score_bit_exact(gpu_device, nbins):
     folds = plan.run(_dev(ts_e), _dev(np.ones(cfg.nsamps, dtype=np.float32)), keep=True)
     lp = plan.levels[-1]
     widths = np.array([1, 2, 3, 5, 8, nbins // 4], dtype=np.int64)
-    got = kernels.ffa_shift_add_score(folds[-2], lp.table, lp.ncoords, widths).cpu().numpy()
     exp = orc.boxcar_snr(orc.fold_norm(folds[-1].cpu().numpy()), widths)
-    np.testing.assert_array_equal(got, exp)
+    for n_inner in (None, int(lp.counts_cur[-1])):   # 1-D rows and (outer x inner) tiles
+        got = kernels.ffa_shift_add_score(folds[-2], lp.table, lp.ncoords, widths,
+                                          n_inner=n_inner).cpu().numpy()
+        np.testing.assert_array_equal(got, exp)
 
 
 def test_score_empty_and_invalid(gpu_device):
