@@ -12,9 +12,10 @@ One "step" = the whole batch folded + scored (inputs resident in HBM before timi
 last level of each series is fused with its scoring, so final folds are never stored.
 value = sum over series of final-level trials / wall time per step.
 
-Also reported: the dominant kernel's roofline (FFA shift-add, HBM-bound), timed live with
-HIP events around each of its launches on the launch stream, and the CPU oracle timed on a
-bounded sample of the same workload.
+Also reported: the roofline of the dominant kernel (the one with the most time per step),
+timed live with HIP events around each of its launches on the launch stream, its HBM
+traffic and VALU issue fraction from the committed rocprofv3 counter profile, and the CPU
+oracle timed on a bounded sample of the same workload.
 """
 
 from __future__ import annotations
@@ -28,6 +29,28 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                               "ffa_batch_accel32_hbm_traffic.md")
+
+
+def committed_counters(kernel: str) -> dict | None:
+    """HBM traffic and VALU issue figures of ``kernel`` on this workload, measured with
+    rocprofv3 PMC passes (FETCH_SIZE, WRITE_SIZE scaled by a 1 GiB copy calibration; SQ
+    instruction counters) and committed as a JSON block in the traffic profile. None when
+    the profile does not cover the kernel."""
+    try:
+        text = open(TRAFFIC_PROFILE).read()
+    except OSError:
+        return None
+    start = text.find("```json")
+    if start < 0:
+        return None
+    end = text.find("```", start + 7)
+    try:
+        data = json.loads(text[start + 7:end])
+    except ValueError:
+        return None
+    return data.get("kernels", {}).get(kernel)
 
 
 def parse() -> argparse.Namespace:
@@ -137,9 +160,11 @@ def main() -> int:
     # the kernel is launched on (torch's current stream, which the wrappers pass down).
     nbins = cfg_max.nbins
     nw = len(widths)
+    # Algorithmic bytes per output row: every output row reads its 2 source rows and writes
+    # its row (shift-add) or its per-width S/N (fused last level + score).
     row_bytes = {
-        "ffa_shift_add_kernel<64>": 3 * 2 * nbins * 4,          # 2 source rows + 1 row out
-        "ffa_shift_add_score_kernel<64>": 2 * 2 * nbins * 4 + 4 * nw,  # 2 rows in, S/N out
+        "ffa_shift_add_kernel<64>": 3 * 2 * nbins * 4,
+        "ffa_shift_add_score_rounds_kernel<64>": 2 * 2 * nbins * 4 + 4 * nw,
     }
     events: dict[str, list] = {k: [] for k in row_bytes}
     record = {"on": False}
@@ -160,12 +185,16 @@ def main() -> int:
         return wrapper
 
     kernels.ffa_shift_add = _timed("ffa_shift_add_kernel<64>", orig_sa)
-    kernels.ffa_shift_add_score = _timed("ffa_shift_add_score_kernel<64>", orig_sas)
+    kernels.ffa_shift_add_score = _timed("ffa_shift_add_score_rounds_kernel<64>", orig_sas)
 
+    # Trials per step: the final grid of every series (known from the plan, no run needed).
+    trials = 0
+    for n in lens:
+        nlev = int(np.log2(n // cfg_max.bseg_brute))
+        trials += plan.levels[nlev - 1].ncoords if nlev > 0 else plan.bf.nfreq
     for _ in range(args.warmup):
-        res = ffa_batch(plan, dev_series, widths)
+        ffa_batch(plan, dev_series, widths)
     torch.cuda.synchronize()
-    trials = sum(int(r.shape[0]) for r in res)
 
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -198,6 +227,7 @@ def main() -> int:
         }
     dominant = max(kstats, key=lambda k: kstats[k]["total_ms_per_step"])
     achieved = kstats[dominant]["achieved_gbs"]
+    counters = committed_counters(dominant)
     ms_per_step = wall / args.steps * 1e3
     line = {
         "metric": "param trials folded+scored/sec & %HBM peak; EP recall vs exhaustive at equal S/N",
@@ -228,7 +258,10 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": counters["hbm_bytes_per_launch"] if counters else None,
+            "traffic_source": (os.path.relpath(TRAFFIC_PROFILE, os.path.dirname(
+                os.path.abspath(__file__))) if counters else None),
+            "valu_issue_frac": counters.get("valu_issue_frac") if counters else None,
             "launches_timed": kstats[dominant]["launches"],
             "avg_launch_ms": kstats[dominant]["avg_launch_ms"],
             "algorithmic_bytes_per_launch": kstats[dominant]["algorithmic_bytes_per_launch"],

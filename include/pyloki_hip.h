@@ -104,6 +104,26 @@ int pyl_ffa_shift_add_score_bf16(const uint16_t* fold_in_dev, int64_t nseg_in,
                                  int64_t ncoords_cur, int64_t n_inner, int32_t nbins,
                                  const int64_t* widths, int32_t nw, double stdnoise,
                                  float* snr_out_dev, void* stream);
+/* Fold row layouts of the FP32 level kernels (no reference counterpart: an internal
+ * storage choice between levels). PYL_FOLD_DENSE: [..][2][nbins], the reference's layout.
+ * PYL_FOLD_WRAP: [..][2][nbins + 4], each component followed by a copy of its first 4 bins,
+ * so that every 4 consecutive bins of a circular rotation are one contiguous 16-byte load;
+ * written by the level before a series' last level and read by the fused last level.
+ * The wrap layout needs nbins in {32, 64, 128, 256} (and FP32 storage). */
+#define PYL_FOLD_DENSE 0
+#define PYL_FOLD_WRAP 1
+/* pyl_ffa_shift_add / pyl_ffa_shift_add_score with explicit input and output row layouts
+ * (the plain entry points are the dense-to-dense case). */
+int pyl_ffa_shift_add_layout(const float* fold_in_dev, int32_t in_layout, int64_t nseg_in,
+                             int64_t ncoords_prev, const int32_t* table_dev,
+                             int64_t ncoords_cur, int32_t nbins, float* fold_out_dev,
+                             int32_t out_layout, void* stream);
+int pyl_ffa_shift_add_score_layout(const float* fold_in_dev, int32_t in_layout,
+                                   int64_t nseg_in, int64_t ncoords_prev,
+                                   const int32_t* table_dev, int64_t ncoords_cur,
+                                   int64_t n_inner, int32_t nbins, const int64_t* widths,
+                                   int32_t nw, double stdnoise, float* snr_out_dev,
+                                   void* stream);
 int pyl_ffa_level(int32_t nparams, const double* params_dev, const int64_t* counts_cur,
                   const int64_t* counts_prev, const double* param_limits, int32_t ffa_level,
                   double tseg_brute, int32_t nbins, const float* fold_in_dev, int64_t nseg_in,

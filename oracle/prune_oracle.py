@@ -516,3 +516,318 @@ def prune_run(dyp_fold: np.ndarray, param_arr: list[np.ndarray], dparams: np.nda
         tree = ascend(tree, dyp_fold, idx_seg, coord_seg, scheme.get_coord(final_level), cfg)
     return {"tree": tree, "report": report_batch(tree.leaves) if tree.size else tree.leaves,
             "stats": stats, "survivors": survivors, "trees": trees, "scheme": scheme}
+
+
+# ============================================================================ circular basis
+# Restatement of core/circular.py (branch :204-394, validate :82-124,397-440, resolve
+# :443-528, transform :669-705, ascend :605-666) and transforms.py:248-313,384-432.
+# Leaves have 7 rows: [crackle, snap, jerk, accel, vel, d0, (f0, flag)].
+
+def circ_mask(leaves: np.ndarray, sig: float):
+    """get_circ_taylor_mask / get_circ_mask (circular.py:13-79,127-166): boolean masks
+    (snap-circular, crackle-circular, taylor)."""
+    crackle = leaves[:, 0, 0]
+    snap, dsnap = leaves[:, 1, 0], leaves[:, 1, 1]
+    jerk, djerk = leaves[:, 2, 0], leaves[:, 2, 1]
+    accel, daccel = leaves[:, 3, 0], leaves[:, 3, 1]
+    is_sig_snap = np.abs(snap) > (sig * dsnap)
+    is_sig_accel = np.abs(accel) > (sig * daccel)
+    is_sig_jerk = np.abs(jerk) > (sig * djerk)
+    m_snap = is_sig_snap & is_sig_accel & ((-snap * accel) > 0)
+    m_crackle = ~m_snap & is_sig_jerk & ((-crackle * jerk) > 0)
+    m_taylor = ~(m_snap | m_crackle)
+    return m_snap, m_crackle, m_taylor
+
+
+def circ_branch_batch(leaves, coord_cur, nbins, eta, branch_max, sig):
+    """circ_taylor_branch_batch: crackle split deferred to hole leaves; within the batch the
+    kept leaves come first (stage-1 order), then every expanded leaf's crackle children."""
+    P = 5
+    n = leaves.shape[0]
+    scale = coord_cur[1]
+    pcur = leaves[:, :-2, 0]
+    dcur = leaves[:, :-2, 1]
+    f0 = leaves[:, -1, 0]
+    dpf = step_f(P, scale, nbins, eta)
+    dnew = dpf[None, :] * C_VAL / f0[:, None]
+    fac = shift_factors(P, scale, nbins)
+    shift_bins = np.abs(dcur - dnew) * (fac[None, :] * (f0 / C_VAL)[:, None])
+    counts = np.ones((n, P), dtype=np.int64)
+    dact = np.empty((n, P))
+    firsts = np.empty((n, P))
+    for i in range(n):
+        for j in range(1, P):
+            dc, dn = dcur[i, j], dnew[i, j]
+            if dc <= 0.0 or dn <= 0.0:
+                raise ValueError("Both dparam_cur and dparam_new must be positive.")
+            num = max(1, math.ceil(dc / dn - FLOAT_EPSILON))
+            if num > branch_max:
+                raise ValueError("Invalid input: increase branch_max.")
+            counts[i, j] = num
+            dact[i, j] = dc / num
+            firsts[i, j] = (pcur[i, j] - dc / 2.0) + dact[i, j] / 2.0
+    cr_num = np.empty(n, dtype=np.int64)
+    for i in range(n):  # branch_dparam_crackle (psr_utils.py:382-398)
+        dc, dn = dcur[i, 0], dnew[i, 0]
+        if dc <= 0.0 or dn <= 0.0:
+            raise ValueError("Both dparam_cur and dparam_new must be positive.")
+        num = max(1, math.ceil(dc / dn - FLOAT_EPSILON))
+        if num > branch_max:
+            raise ValueError("Invalid input: increase branch_max.")
+        cr_num[i] = num
+        dact[i, 0] = dc / num
+        firsts[i, 0] = pcur[i, 0]
+    needs = shift_bins >= (eta - FLOAT_EPSILON)
+    keep_par = ~needs
+    counts[keep_par] = 1
+    dact[keep_par] = dcur[keep_par]
+    # stage 1 (crackle held at the parent value)
+    rows, orgs = [], []
+    for i in range(n):
+        grids = [np.array([pcur[i, 0]])]
+        for j in range(1, P):
+            if keep_par[i, j]:
+                grids.append(np.array([pcur[i, j]]))
+            else:
+                grids.append(firsts[i, j] + np.arange(counts[i, j]) * dact[i, j])
+        mesh = np.meshgrid(*grids, indexing="ij")
+        cart = np.stack([m.reshape(-1) for m in mesh], axis=1)
+        rows.append(cart)
+        orgs.append(np.full(len(cart), i, dtype=np.int64))
+    cart = np.concatenate(rows)
+    org = np.concatenate(orgs)
+    dp = dact[org]
+    # get_circ_taylor_mask_branch (circular.py:169-201)
+    snap, jerk, accel = cart[:, 1], cart[:, 2], cart[:, 3]
+    dsnap, djerk, daccel = dp[:, 1], dp[:, 2], dp[:, 3]
+    m_snap = (np.abs(snap) > sig * dsnap) & (np.abs(accel) > sig * daccel) & ((-snap * accel) > 0)
+    hole = ~m_snap & (np.abs(jerk) > sig * djerk)
+    expand = hole & needs[org, 0]
+    out_rows, out_org = [], []
+    for s in np.where(~expand)[0]:
+        lv = np.zeros((P + 2, 2))
+        lv[:P, 0] = cart[s]
+        lv[:P, 1] = dp[s]
+        lv[P, 0] = leaves[org[s], -2, 0]
+        lv[P + 1] = leaves[org[s], -1]
+        out_rows.append(lv)
+        out_org.append(org[s])
+    for s in np.where(expand)[0]:
+        i = org[s]
+        dc, dn = dcur[i, 0], dnew[i, 0]
+        num = max(1, math.ceil(dc / dn - FLOAT_EPSILON))
+        if num > branch_max:
+            raise ValueError("Invalid input: increase branch_max.")
+        da = dc / num
+        first = (cart[s, 0] - dc / 2.0) + da / 2.0
+        for c in range(num):
+            lv = np.zeros((P + 2, 2))
+            lv[:P, 0] = cart[s]
+            lv[:P, 1] = dp[s]
+            lv[0, 0] = first + c * da
+            lv[0, 1] = da
+            lv[P, 0] = leaves[i, -2, 0]
+            lv[P + 1] = leaves[i, -1]
+            out_rows.append(lv)
+            out_org.append(i)
+    if not out_rows:
+        return np.zeros((0, P + 2, 2)), np.zeros(0, dtype=np.int64)
+    return np.stack(out_rows), np.array(out_org, dtype=np.int64)
+
+
+def circ_validate(leaves, origins, p_orb_min, x_mass_const, val_sig):
+    """circ_taylor_validate_batch / circ_validate_batch: indices of kept leaves."""
+    snap, dsnap = leaves[:, 1, 0], leaves[:, 1, 1]
+    accel, daccel = leaves[:, 3, 0], leaves[:, 3, 1]
+    omega_max_sq = (2 * np.pi / p_orb_min) ** 2
+    sig_s = np.abs(snap) > val_sig * dsnap
+    sig_a = np.abs(accel) > val_sig * daccel
+    phys = (-snap * accel) > 0.0
+    keep = np.ones(len(snap), dtype=bool)
+    keep[sig_s & sig_a & ~phys] = False
+    region = sig_s & sig_a & phys
+    if np.any(region):
+        idx = np.flatnonzero(region)
+        om = -snap[idx] / accel[idx]
+        lim = x_mass_const * (om ** (2.0 / 3.0))
+        keep[idx] = (om <= omega_max_sq) & (np.abs(accel[idx]) <= lim)
+    k = np.flatnonzero(keep)
+    return leaves[k], origins[k], k
+
+
+def shift_taylor_circular(vec: np.ndarray, dt: float, in_hole: bool) -> np.ndarray:
+    """transforms.py:248-313 (vec rows [d5, d4, d3, d2, d1, d0])."""
+    d5, d4, d3, d2, d1, d0 = (vec[:, k] for k in range(6))
+    osq = -d5 / d3 if in_hole else -d4 / d2
+    om = np.sqrt(osq)
+    odt = om * dt
+    c = np.cos(odt)
+    s = np.sin(odt)
+    d2j = d2 * c + (d3 / om) * s
+    d3j = d3 * c - (d2 * om) * s
+    d4j = -osq * d2j
+    d5j = -osq * d3j
+    d1ci = -d3 / osq
+    d1cj = -d3j / osq
+    d1j = d1cj + (d1 - d1ci)
+    d0cj = -d2j / osq
+    d0ci = -d2 / osq
+    d0j = d0cj + (d0 - d0ci) + (d1 - d1ci) * dt
+    return np.stack([d5j, d4j, d3j, d2j, d1j, d0j], axis=1)
+
+
+def circ_shift_all(vec, dt, m_snap, m_crackle, m_taylor):
+    out = np.empty_like(vec)
+    if m_snap.any():
+        out[m_snap] = shift_taylor_circular(vec[m_snap], dt, False)
+    if m_crackle.any():
+        out[m_crackle] = shift_taylor_circular(vec[m_crackle], dt, True)
+    if m_taylor.any():
+        out[m_taylor] = shift_taylor(vec[m_taylor], dt)
+    return out
+
+
+def circ_resolve_batch(leaves, coord_add, coord_cur, coord_init, counts_init, limits, nbins,
+                       sig):
+    """circ_taylor_resolve_batch (circular.py:443-528), including its in-place basis-flag
+    writes: snap -> 1, crackle -> 2, and crackle leaves reset to 0 when the batch also holds
+    a Taylor leaf (the reference indexes idx_circ_crackle in the Taylor branch, :510)."""
+    t0_cur, t0_init, t0_add = coord_cur[0], coord_init[0], coord_add[0]
+    vec = leaves[:, :-1, 0]
+    f0 = leaves[:, -1, 0]
+    ms, mc, mt = circ_mask(leaves, sig)
+    d_add = circ_shift_all(vec, t0_add - t0_cur, ms, mc, mt)
+    d_init = circ_shift_all(vec, t0_init - t0_cur, ms, mc, mt)
+    if ms.any():
+        leaves[ms, -1, 1] = 1
+    if mc.any():
+        leaves[mc, -1, 1] = 2
+    if mt.any():
+        leaves[mc, -1, 1] = 0
+    accel = d_add[:, -3]
+    vel = d_add[:, -2] - d_init[:, -2]
+    freq = f0 * (1 - vel / C_VAL)
+    delay = (d_add[:, -1] - d_init[:, -1]) / C_VAL
+    phase = phase_idx_vec(t0_add - t0_init, f0, nbins, delay)
+    return nearest_2d(accel, freq, counts_init, limits), phase
+
+
+def circ_transform_batch(leaves, coord_next, coord_cur, tiling, sig):
+    ms, mc, mt = circ_mask(leaves, sig)
+    dt = coord_next[0] - coord_cur[0]
+    out = leaves.copy()
+    vals = leaves[:, :-1, 0]
+    if ms.any():
+        out[ms, :-1, 0] = shift_taylor_circular(vals[ms], dt, False)
+    if mc.any():
+        out[mc, :-1, 0] = shift_taylor_circular(vals[mc], dt, True)
+    if mt.any():
+        out[mt] = transform_batch(leaves[mt], coord_next, coord_cur, tiling)
+    return out
+
+
+def circ_prune_level(tree: Tree, fold_segment, coords, threshold, cfg, max_sugg,
+                     batch_size=1024):
+    """pruning_iteration_batched with the circular plugin (prune.py:165-332)."""
+    P = 5
+    B = tree.folds.shape[-1]
+    buf = TreeBuffer(max_sugg, P, B)
+    n_leaves = n_phy = 0
+    smin, smax = math.inf, -math.inf
+    cur_thr = threshold
+    nb = tree.size
+    bs = max(1, min(batch_size, nb))
+    child_base = 0
+    for b0 in range(0, nb, bs):
+        b1 = min(b0 + bs, nb)
+        ch, org = circ_branch_batch(tree.leaves[b0:b1], coords["cur"], cfg["nbins"], cfg["eta"],
+                                    cfg["branch_max"], cfg["prop_sig"])
+        n_leaves += len(ch)
+        if len(ch) == 0:
+            continue
+        ch, org, _ = circ_validate(ch, org, cfg["p_orb_min"], cfg["x_mass_const"],
+                                   cfg["val_sig"])
+        n_phy += len(ch)
+        if len(ch) == 0:
+            continue
+        idx, phase = circ_resolve_batch(ch, coords["add"], coords["cur"], coords["init"],
+                                        cfg["counts_init"], cfg["param_limits"], cfg["nbins"],
+                                        cfg["prop_sig"])
+        seg = seg_rows(fold_segment, idx)
+        parents = np.arange(b0, b1)[org]
+        comb = shift_add_batch(seg, phase, tree.folds, parents)
+        scores = orc.snr_score_batch(comb, cfg["widths"])
+        smin = min(smin, float(scores.min()))
+        smax = max(smax, float(scores.max()))
+        fi = np.where(scores.astype(np.float64) >= max(threshold, cur_thr))[0]
+        if len(fi):
+            trans = circ_transform_batch(ch[fi], coords["next"], coords["cur"], cfg["tiling"],
+                                         cfg["prop_sig"])
+            bt = np.empty((len(fi), P + 2), dtype=np.int32)
+            bt[:, 0] = parents[fi]
+            bt[:, 1: idx.shape[1] + 1] = idx[fi]
+            bt[:, -1] = phase[fi].astype(np.int32)
+            cur_thr = buf.add_batch(trans, comb[fi], scores[fi], bt, child_base + fi, cur_thr)
+        child_base += len(ch)
+    v = buf.valid
+    new = Tree(buf.leaves[:v].copy(), buf.folds[:v].copy(), buf.scores[:v].copy(),
+               buf.backtracks[:v].copy())
+    stats = LevelStats(n_leaves, n_phy, smin if math.isfinite(smin) else 0.0,
+                       smax if math.isfinite(smax) else 0.0, v)
+    return new, stats, buf.origin[:v].copy()
+
+
+def circ_ascend(tree: Tree, dyp_fold, idx_segments, coord_segments, coord_cur, cfg) -> Tree:
+    n = tree.size
+    if n == 0:
+        return tree
+    vec = tree.leaves[:, :-1, 0]
+    f0 = tree.leaves[:, -1, 0]
+    ms, mc, mt = circ_mask(tree.leaves, cfg["prop_sig"])
+    B = tree.folds.shape[-1]
+    res = np.zeros((n, 2, B), dtype=np.float32)
+    for iseg, (t0_seg, _) in zip(idx_segments, coord_segments):
+        d = circ_shift_all(vec, t0_seg - coord_cur[0], ms, mc, mt)
+        accel = d[:, -3]
+        freq = f0 * (1 - d[:, -2] / C_VAL)
+        delay = d[:, -1] / C_VAL
+        phase = phase_idx_vec(t0_seg - coord_cur[0], f0, cfg["nbins"], delay)
+        idx = nearest_2d(accel, freq, cfg["counts_init"], cfg["param_limits"])
+        rows = seg_rows(dyp_fold[iseg], idx)
+        s = shift_int_vec(phase, B)
+        src = (np.arange(B)[None, :] - s[:, None]) % B
+        rolled = np.take_along_axis(rows, np.broadcast_to(src[:, None, :], rows.shape), axis=2)
+        res = (res + rolled).astype(np.float32)
+    return Tree(tree.leaves.copy(), res, orc.snr_score_batch(res, cfg["widths"]),
+                tree.backtracks.copy(), scores_ep=tree.scores.copy())
+
+
+def circ_prune_run(dyp_fold, param_arr, dparams, tseg, thresholds, ref_seg, cfg, max_sugg,
+                   batch_size=1024, nlevels=None):
+    """Pruning.execute with the circular plugin (moving grid, real folds)."""
+    nseg = dyp_fold.shape[0]
+    scheme = MiddleOut(nseg, ref_seg, tseg)
+    leaves = taylor_seed(param_arr, dparams, 5)
+    B = dyp_fold.shape[-1]
+    folds = dyp_fold[scheme.ref_idx].reshape(-1, 2, B).astype(np.float32)
+    tree = Tree(leaves, folds.copy(), orc.snr_score_batch(folds, cfg["widths"]),
+                np.zeros((len(leaves), 7), dtype=np.int32))
+    stats, survivors, trees = [], [], []
+    nl = nseg - 1 if nlevels is None else nlevels
+    level = 0
+    for level in range(1, nl + 1):
+        coords = {"init": scheme.get_coord(0), "cur": scheme.get_current_coord(level),
+                  "next": scheme.get_coord(level), "add": scheme.get_segment_coord(level)}
+        tree, st, surv = circ_prune_level(tree, dyp_fold[scheme.get_segment_idx(level)], coords,
+                                          float(thresholds[level - 1]), cfg, max_sugg,
+                                          batch_size)
+        stats.append(st)
+        survivors.append(surv)
+        trees.append(tree)
+        if tree.size == 0:
+            break
+    if tree.size > 0:
+        idx_seg, coord_seg = scheme.get_segment_coords_so_far(level)
+        tree = circ_ascend(tree, dyp_fold, idx_seg, coord_seg, scheme.get_coord(level), cfg)
+    return {"tree": tree, "report": report_batch(tree.leaves) if tree.size else tree.leaves,
+            "stats": stats, "survivors": survivors, "trees": trees, "scheme": scheme}

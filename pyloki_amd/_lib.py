@@ -20,6 +20,8 @@ PYL_ERR_ALLOC = -3
 PYL_LAYOUT_EV = 0
 PYL_LAYOUT_NORM = 1
 PYL_LAYOUT_EV_BF16 = 2
+PYL_FOLD_DENSE = 0
+PYL_FOLD_WRAP = 1
 PYL_SCORE_PER_WIDTH = 0
 PYL_SCORE_MAX = 1
 
@@ -50,6 +52,11 @@ SIGNATURES: list[tuple[str, object, list[object]]] = [
      [_c_i32, _vp, _i64p, _i64p, _f64p, _c_i32, _c_dbl, _c_i32, _vp, _c_i64, _vp, _vp, _vp]),
     ("pyl_ffa_shift_add_score", ctypes.c_int,
      [_vp, _c_i64, _c_i64, _vp, _c_i64, _c_i64, _c_i32, _i64p, _c_i32, _c_dbl, _vp, _vp]),
+    ("pyl_ffa_shift_add_layout", ctypes.c_int,
+     [_vp, _c_i32, _c_i64, _c_i64, _vp, _c_i64, _c_i32, _vp, _c_i32, _vp]),
+    ("pyl_ffa_shift_add_score_layout", ctypes.c_int,
+     [_vp, _c_i32, _c_i64, _c_i64, _vp, _c_i64, _c_i64, _c_i32, _i64p, _c_i32, _c_dbl, _vp,
+      _vp]),
     ("pyl_score_boxcar", ctypes.c_int,
      [_vp, _c_i64, _c_i32, _c_i32, _i64p, _c_i32, _c_dbl, _c_i32, _vp, _vp]),
     ("pyl_prune_branch", ctypes.c_int, [_vp, _vp, _vp, _c_i64, _vp, _vp, _vp, _vp, _vp]),

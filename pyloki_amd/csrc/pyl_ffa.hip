@@ -12,9 +12,9 @@
 // ffa_shift_add_kernel: the memory-bound part. Each output row (segment pair q, set j)
 //   is 2 x nbins floats = 512 B at nbins = 64; a row is read from the tail and head
 //   rows with a circular rotation. Rows are processed as float4 chunks: a lane loads the
-//   aligned 16 B chunk of the source row that holds its rotated elements (two chunks
-//   when the shift is not a multiple of 4) -- every load instruction of a wavefront
-//   covers whole 64 B segments, so the rotation costs no extra HBM traffic.
+//   two aligned 16 B chunks of the source row that hold its rotated elements and selects
+//   -- every load instruction of a wavefront covers whole 64 B segments, so the rotation
+//   costs no extra HBM traffic.
 //   Rows of consecutive j share their source rows (the previous level's grid is about
 //   half as fine), so each block takes a contiguous run of j and the runs are dealt to
 //   the 8 XCDs in contiguous slabs: neighbours' rows are reused from the same L2.
@@ -25,6 +25,7 @@
 namespace {
 
 constexpr int kMaxP = 5;
+constexpr int kRoundsMaxNW = 16;  // widths handled by the round-based fused kernel
 
 struct ResolveArgs {
   int32_t P;
@@ -119,31 +120,66 @@ __device__ __forceinline__ void store4(T* __restrict__ dst, int c4, const float4
 }
 
 // Rotated load of the 4-element chunk `c4` (elements 4*c4 .. 4*c4+3) of a row component of
-// B elements, widened to FP32: element e of the result is src[(4*c4 + e - s) mod B].
+// B elements (a power of two), widened to FP32: element e of the result is
+// src[(4*c4 + e - s) mod B]. Branch-free: the two aligned source chunks are always loaded
+// (the second one is usually in the same cache line) and the rotation is a select, so lanes
+// with different shifts never diverge.
 template <int B, typename T = float>
 __device__ __forceinline__ float4 rotated_chunk(const T* __restrict__ src, int c4, int s) {
+  static_assert((B & (B - 1)) == 0 && B >= 4, "B must be a power of two");
   constexpr int C = B / 4;
-  const int base = 4 * c4 - s;                 // may be negative
-  const int m = (base % B + B) % B;            // first source element
+  const int m = (4 * c4 - s) & (B - 1);       // first source element
   const int q = m >> 2;                        // source chunk
   const int o = m & 3;                         // offset inside the chunk
   const typename Vec4<T>::type* s4 = reinterpret_cast<const typename Vec4<T>::type*>(src);
   const float4 a = to_f32x4<T>(s4[q]);
-  if (o == 0) return a;
-  const float4 b = to_f32x4<T>(s4[(q + 1) % C]);
+  const float4 b = to_f32x4<T>(s4[(q + 1) & (C - 1)]);
   float4 r;
-  if (o == 1) {
-    r.x = a.y; r.y = a.z; r.z = a.w; r.w = b.x;
-  } else if (o == 2) {
-    r.x = a.z; r.y = a.w; r.z = b.x; r.w = b.y;
-  } else {
-    r.x = a.w; r.y = b.x; r.z = b.y; r.w = b.z;
-  }
+  r.x = o == 0 ? a.x : (o == 1 ? a.y : (o == 2 ? a.z : a.w));
+  r.y = o == 0 ? a.y : (o == 1 ? a.z : (o == 2 ? a.w : b.x));
+  r.z = o == 0 ? a.z : (o == 1 ? a.w : (o == 2 ? b.x : b.y));
+  r.w = o == 0 ? a.w : (o == 1 ? b.x : (o == 2 ? b.y : b.z));
   return r;
 }
 
+// Row layouts of the FP32 level kernels. Dense: [2][B] per row (the reference's layout).
+// Wrap: [2][B + 4] per row, each component followed by a copy of its first 4 bins, so that
+// the 4 consecutive bins (4*c4 - s) mod B .. +3 of any rotation are one contiguous,
+// dword-aligned 16-byte load (no select, no second load).
+template <int B, bool kWrap>
+struct RowLayout {
+  static constexpr int64_t kComp = kWrap ? B + 4 : B;  // elements per component
+  static constexpr int64_t kRow = 2 * kComp;            // elements per row
+};
+
+// Rotated chunk c4 of one component (element e = comp[(4*c4 + e - s) mod B]) in either
+// layout.
+template <int B, typename T, bool kWrap>
+__device__ __forceinline__ float4 row_chunk(const T* __restrict__ comp, int c4, int s) {
+  if constexpr (kWrap) {
+    static_assert(sizeof(T) == 4, "the wrap layout is FP32 only");
+    float4 v;
+    __builtin_memcpy(&v, comp + ((4 * c4 - s) & (B - 1)), sizeof(float4));
+    return v;
+  } else {
+    return rotated_chunk<B, T>(comp, c4, s);
+  }
+}
+
+// XCD-aware block order: hardware deals block i to XCD i % 8. Returns the virtual block
+// index such that the blocks one XCD receives form a contiguous slab of the grid, so
+// neighbouring blocks (which share source rows) hit the same L2.
+__device__ __forceinline__ int64_t xcd_slab_block(int64_t bid, int64_t nblocks) {
+  const int64_t per_xcd = (nblocks + 7) / 8;
+  const int64_t xcd = bid % 8;
+  const int64_t slot = bid / 8;
+  const int64_t full = nblocks % 8 == 0 ? 8 : nblocks % 8;  // XCDs with per_xcd blocks
+  return xcd < full ? xcd * per_xcd + slot
+                    : full * per_xcd + (xcd - full) * (per_xcd - 1) + slot;
+}
+
 // Rows per block = kThreads / (B/2) (each row = 2 components x B/4 chunks).
-template <int B, typename T>
+template <int B, typename T, bool kWrapIn, bool kWrapOut>
 __global__ void __launch_bounds__(256)
 ffa_shift_add_kernel(const T* __restrict__ in, int64_t ncoords_prev,
                      const int4* __restrict__ table, int64_t ncoords_cur, int64_t nseg_out,
@@ -151,17 +187,9 @@ ffa_shift_add_kernel(const T* __restrict__ in, int64_t ncoords_prev,
   constexpr int CPR = B / 4;          // chunks per component
   constexpr int TPR = 2 * CPR;        // threads per row
   constexpr int RPB = 256 / TPR;      // rows per block
-  // XCD-aware block order: hardware deals block i to XCD i % 8. Remap so that the
-  // blocks one XCD receives form a contiguous slab of (q, j) space.
-  const int64_t nblocks = (int64_t)gridDim.x;
-  const int64_t bid = blockIdx.x;
-  const int64_t per_xcd = (nblocks + 7) / 8;
-  const int64_t xcd = bid % 8;
-  const int64_t slot = bid / 8;
-  const int64_t full = nblocks % 8 == 0 ? 8 : nblocks % 8;  // XCDs with per_xcd blocks
-  int64_t vb;
-  if (xcd < full) vb = xcd * per_xcd + slot;
-  else vb = full * per_xcd + (xcd - full) * (per_xcd - 1) + slot;
+  using Li = RowLayout<B, kWrapIn>;
+  using Lo = RowLayout<B, kWrapOut>;
+  const int64_t vb = xcd_slab_block(blockIdx.x, gridDim.x);
   const int64_t q = vb / nblk_j;
   const int64_t jb = vb - q * nblk_j;
   if (q >= nseg_out) return;
@@ -172,17 +200,18 @@ ffa_shift_add_kernel(const T* __restrict__ in, int64_t ncoords_prev,
   const int comp = e4 / CPR;
   const int c4 = e4 % CPR;
   const int4 tb = table[j];
-  constexpr int64_t E = 2 * B;
-  const T* tail = in + ((2 * q) * ncoords_prev + tb.x) * E + comp * B;
-  const T* head = in + ((2 * q + 1) * ncoords_prev + tb.z) * E + comp * B;
-  const float4 a = rotated_chunk<B, T>(tail, c4, tb.y);
-  const float4 b = rotated_chunk<B, T>(head, c4, tb.w);
+  const T* tail = in + ((2 * q) * ncoords_prev + tb.x) * Li::kRow + comp * Li::kComp;
+  const T* head = in + ((2 * q + 1) * ncoords_prev + tb.z) * Li::kRow + comp * Li::kComp;
+  const float4 a = row_chunk<B, T, kWrapIn>(tail, c4, tb.y);
+  const float4 b = row_chunk<B, T, kWrapIn>(head, c4, tb.w);
   float4 r;
   r.x = a.x + b.x;
   r.y = a.y + b.y;
   r.z = a.z + b.z;
   r.w = a.w + b.w;
-  store4<T>(out + (q * ncoords_cur + j) * E + comp * B, c4, r);
+  T* o = out + (q * ncoords_cur + j) * Lo::kRow + comp * Lo::kComp;
+  store4<T>(o, c4, r);
+  if (kWrapOut && c4 == 0) store4<T>(o + B, 0, r);  // the wrap copy of bins 0..3
 }
 
 // Generic nbins: one thread per output element.
@@ -216,7 +245,7 @@ __global__ void ffa_shift_add_generic_kernel(const float* __restrict__ in, int64
 // (4 bins x 2 components per lane, float4 loads with the rotation of rotated_chunk), so a
 // wavefront forms 4 rows per step; the normalised rows are transposed into LDS and every
 // thread then scans and scores one row.
-template <int B, typename T>
+template <int B, typename T, bool kWrapIn>
 __global__ void __launch_bounds__(pyl::kScoreBlock)
 ffa_shift_add_score_kernel(const T* __restrict__ in, int64_t ncoords_prev,
                            const int4* __restrict__ table, int64_t ncoords_cur,
@@ -231,7 +260,7 @@ ffa_shift_add_score_kernel(const T* __restrict__ in, int64_t ncoords_prev,
   const int nvalid = (int)std::min<int64_t>(pyl::kScoreBlock, ncoords_cur - j0);
   if (tid < nvalid) s_tab[tid] = table[j0 + tid];
   __syncthreads();
-  constexpr int64_t E = 2 * B;
+  using Li = RowLayout<B, kWrapIn>;
   const int c4 = tid % CPR;
   const int rsub = tid / CPR;
 #pragma unroll 2
@@ -239,12 +268,12 @@ ffa_shift_add_score_kernel(const T* __restrict__ in, int64_t ncoords_prev,
     const int r = r0 + rsub;
     if (r < nvalid) {
       const int4 tb = s_tab[r];
-      const T* tail = in + ((2 * q) * ncoords_prev + tb.x) * E;
-      const T* head = in + ((2 * q + 1) * ncoords_prev + tb.z) * E;
-      const float4 te = rotated_chunk<B, T>(tail, c4, tb.y);
-      const float4 he = rotated_chunk<B, T>(head, c4, tb.w);
-      const float4 tv = rotated_chunk<B, T>(tail + B, c4, tb.y);
-      const float4 hv = rotated_chunk<B, T>(head + B, c4, tb.w);
+      const T* tail = in + ((2 * q) * ncoords_prev + tb.x) * Li::kRow;
+      const T* head = in + ((2 * q + 1) * ncoords_prev + tb.z) * Li::kRow;
+      const float4 te = row_chunk<B, T, kWrapIn>(tail, c4, tb.y);
+      const float4 he = row_chunk<B, T, kWrapIn>(head, c4, tb.w);
+      const float4 tv = row_chunk<B, T, kWrapIn>(tail + Li::kComp, c4, tb.y);
+      const float4 hv = row_chunk<B, T, kWrapIn>(head + Li::kComp, c4, tb.w);
       const int b = 4 * c4;
       // The level's fold element as it would be stored (identity for FP32, BF16 rounding
       // for BF16 storage), then e / sqrt(v).
@@ -260,80 +289,133 @@ ffa_shift_add_score_kernel(const T* __restrict__ in, int64_t ncoords_prev,
     pyl::score_column<true>(k, B, S, tid, out + (q * ncoords_cur + j0 + tid) * k.nw);
 }
 
-// Register-scoring variant of the fused last level (nbins 32 or 64). A block owns a tile of
-// 256 output rows of one segment pair: tile_o_rows (1, 2 or 4) "outer" x 256/tile_o_rows
-// "inner" rows, inner = index along the last (frequency) axis, outer = the product of the
-// other axes. Rows at neighbouring outer
+// Round-based variant of the fused last level (nbins 32 or 64, at most kRoundsMaxNW widths,
+// widths <= WC). A block owns a tile of 256 output rows of one segment pair: tile_o_rows
+// (up to 16) "outer" x 256/tile_o_rows "inner" rows, inner = index along the last
+// (frequency) axis, outer = the product of the other axes. Rows at neighbouring outer
 // indices (e.g. accelerations) resolve to the same source rows, so the tile reuses each
-// source row from L1/L2 several times instead of re-reading it from HBM.
-// Phase 1 forms the rows with coalesced, rotated float4 loads (16 lanes per row at B = 64)
-// and writes the normalised values transposed into LDS (row stride 257 words:
-// conflict-free both ways). Phase 2: every thread pulls its row into registers, scans it
-// sequentially and scores it with the unrolled max-difference loops of pyl_score.hpp.
-template <int B, typename T>
+// source row from L2 instead of re-reading it from HBM; blocks are dealt to the 8 XCDs in
+// contiguous slabs so that neighbouring tiles (which share boundary rows) meet in one L2.
+//
+// The row gather is latency-bound, so the kernel keeps its LDS footprint small (occupancy
+// is what hides the gather latency): rows are formed in B/32 rounds of 32 bins. In a round
+// 8 lanes cover one row's 32 bins (rotated float4 loads, 8 rows per wavefront step), write
+// the normalised values transposed into a 32 x 257-word LDS buffer, and after a barrier
+// every thread extends its own row's sequential FP32 prefix sum in registers. Scoring then
+// slides a window of 16 + WC prefix values per row through the same LDS buffer (only the
+// owning thread touches its column, so no further barriers): for window r the buffer holds
+// S[16r + j], j < 16 + WC (S[B + t] = S[t] + S[B-1], the one-wrap extension), and every
+// width w updates its running maximum of S[i + w] - S[i] over the window's 16 i, with S[i]
+// from registers and S[i + w] read at a runtime offset. The per-width maxima sit in a small
+// register array indexed by the (uniform) width counter. The i order, and so every FP32
+// operation, is that of the reference's sequential loop.
+template <int B, typename T, int WC, bool kWrapIn>
 __global__ void __launch_bounds__(256)
-ffa_shift_add_score_regs_kernel(const T* __restrict__ in, int64_t ncoords_prev,
-                                const int4* __restrict__ table, int64_t ncoords_cur,
-                                int64_t n_inner, int tile_o_rows, int64_t tiles_inner,
-                                int64_t tiles_per_pair, pyl::ScoreConst k,
-                                float* __restrict__ out) {
+ffa_shift_add_score_rounds_kernel(const T* __restrict__ in, int64_t ncoords_prev,
+                                  const int4* __restrict__ table, int64_t ncoords_cur,
+                                  int64_t n_inner, int tile_o_rows, int64_t tiles_inner,
+                                  int64_t tiles_per_pair, pyl::ScoreConst k,
+                                  float* __restrict__ out) {
   constexpr int kRows = 256;
   constexpr int kLds = kRows + 1;
-  constexpr int CPR = B / 4;       // float4 chunks per component
-  constexpr int RPS = 64 / CPR;    // rows per wavefront step
-  __shared__ float S[B * kLds];
+  constexpr int kSeg = 16;            // scoring window step
+  constexpr int kRounds = B / kSeg;    // scoring windows
+  constexpr int kWin = kSeg + WC;      // LDS rows of a scoring window
+  constexpr int kSeg1 = 32;            // bins formed per gather round
+  constexpr int kRounds1 = B / kSeg1;  // gather rounds
+  constexpr int kLpr = kSeg1 / 4;      // lanes per row in a gather round
+  constexpr int kRps = 64 / kLpr;      // rows per wavefront step
+  constexpr int kSteps = 64 / kRps;    // steps per wavefront and round
+  constexpr int kLdsRows = kWin > kSeg1 ? kWin : kSeg1;
+  static_assert(B % kSeg1 == 0 && WC < B, "unsupported round layout");
+  extern __shared__ float S[];  // [kLdsRows][kLds]
   __shared__ int4 s_tab[kRows];
-  __shared__ int64_t s_j[kRows];
   const int tid = threadIdx.x;
-  const int64_t q = blockIdx.x / tiles_per_pair;
-  const int64_t tile = blockIdx.x - q * tiles_per_pair;
+  const int64_t vb = xcd_slab_block(blockIdx.x, gridDim.x);
+  const int64_t q = vb / tiles_per_pair;
+  const int64_t tile = vb - q * tiles_per_pair;
   const int64_t tile_o = tile / tiles_inner;
   const int64_t tile_i = tile - tile_o * tiles_inner;
   const int64_t n_outer = ncoords_cur / n_inner;
-  {
-    const int tile_i_rows = kRows / tile_o_rows;
-    const int64_t outer = tile_o * tile_o_rows + tid / tile_i_rows;
-    const int64_t inner = tile_i * tile_i_rows + tid % tile_i_rows;
-    const bool ok = outer < n_outer && inner < n_inner;
-    const int64_t j = ok ? outer * n_inner + inner : -1;
-    s_j[tid] = j;
-    if (ok) s_tab[tid] = table[j];
-  }
-  __syncthreads();
-  constexpr int64_t E = 2 * B;
+  const int tile_i_rows = kRows / tile_o_rows;
+  const int64_t outer = tile_o * tile_o_rows + tid / tile_i_rows;
+  const int64_t inner = tile_i * tile_i_rows + tid % tile_i_rows;
+  const bool ok = outer < n_outer && inner < n_inner;
+  const int64_t j = outer * n_inner + inner;
+  s_tab[tid] = ok ? table[j] : make_int4(-1, 0, 0, 0);
+  using Li = RowLayout<B, kWrapIn>;
   const int wave = tid >> 6;
   const int lane = tid & 63;
-  const int c4 = lane % CPR;
-  const int rsub = lane / CPR;
-  const T* base_t = in + (2 * q) * ncoords_prev * E;
-  const T* base_h = in + (2 * q + 1) * ncoords_prev * E;
-#pragma unroll 4
-  for (int step = 0; step < 64 / RPS; ++step) {
-    const int r = wave * 64 + step * RPS + rsub;
-    if (s_j[r] >= 0) {
+  const int c4 = lane % kLpr;   // float4 chunk inside the round's bins
+  const int rsub = lane / kLpr;  // row inside the wavefront step
+  const T* base_t = in + (2 * q) * ncoords_prev * Li::kRow;
+  const T* base_h = in + (2 * q + 1) * ncoords_prev * Li::kRow;
+  float x[B];  // this thread's row: prefix sums S[0..B-1]
+#pragma unroll
+  for (int rd = 0; rd < kRounds1; ++rd) {
+    __syncthreads();  // s_tab ready (rd == 0) / previous round consumed
+#pragma unroll
+    for (int step = 0; step < kSteps; ++step) {
+      const int r = wave * 64 + step * kRps + rsub;
       const int4 tb = s_tab[r];
-      const T* tail = base_t + (int64_t)tb.x * E;
-      const T* head = base_h + (int64_t)tb.z * E;
-      const float4 te = rotated_chunk<B, T>(tail, c4, tb.y);
-      const float4 he = rotated_chunk<B, T>(head, c4, tb.w);
-      const float4 tv = rotated_chunk<B, T>(tail + B, c4, tb.y);
-      const float4 hv = rotated_chunk<B, T>(head + B, c4, tb.w);
-      auto st = [](float x) { return pyl::Store<T>::load(pyl::Store<T>::store(x)); };
-      const int b = 4 * c4;
-      S[(b + 0) * kLds + r] = st(te.x + he.x) / sqrtf(st(tv.x + hv.x));
-      S[(b + 1) * kLds + r] = st(te.y + he.y) / sqrtf(st(tv.y + hv.y));
-      S[(b + 2) * kLds + r] = st(te.z + he.z) / sqrtf(st(tv.z + hv.z));
-      S[(b + 3) * kLds + r] = st(te.w + he.w) / sqrtf(st(tv.w + hv.w));
+      if (tb.x >= 0) {
+        const T* tail = base_t + (int64_t)tb.x * Li::kRow;
+        const T* head = base_h + (int64_t)tb.z * Li::kRow;
+        const int cc = rd * kLpr + c4;
+        const float4 te = row_chunk<B, T, kWrapIn>(tail, cc, tb.y);
+        const float4 he = row_chunk<B, T, kWrapIn>(head, cc, tb.w);
+        const float4 tv = row_chunk<B, T, kWrapIn>(tail + Li::kComp, cc, tb.y);
+        const float4 hv = row_chunk<B, T, kWrapIn>(head + Li::kComp, cc, tb.w);
+        // The level's fold element as it would be stored (identity for FP32, BF16 rounding
+        // for BF16 storage), then e / sqrt(v).
+        auto st = [](float v) { return pyl::Store<T>::load(pyl::Store<T>::store(v)); };
+        const int b = 4 * c4;
+        S[(b + 0) * kLds + r] = st(te.x + he.x) / sqrtf(st(tv.x + hv.x));
+        S[(b + 1) * kLds + r] = st(te.y + he.y) / sqrtf(st(tv.y + hv.y));
+        S[(b + 2) * kLds + r] = st(te.z + he.z) / sqrtf(st(tv.z + hv.z));
+        S[(b + 3) * kLds + r] = st(te.w + he.w) / sqrtf(st(tv.w + hv.w));
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < kSeg1; ++b) {
+      const float v = S[b * kLds + tid];
+      const int i = rd * kSeg1 + b;
+      if (i == 0) x[0] = v;
+      else x[i] = x[i - 1] + v;  // scoring.py:53-55, sequential
     }
   }
-  __syncthreads();
-  const int64_t j = s_j[tid];
-  if (j >= 0) {
-    float x[B];
+  if (!ok) return;  // no barrier below
+  const float total = x[B - 1];
+  const int nw = k.nw;
+  float dmax[kRoundsMaxNW];
 #pragma unroll
-    for (int i = 0; i < B; ++i) x[i] = S[i * kLds + tid];
-    pyl::scan_regs<B>(x);
-    pyl::score_regs<B, true>(k, x, out + (q * ncoords_cur + j) * k.nw);
+  for (int m = 0; m < kRoundsMaxNW; ++m) dmax[m] = -FLT_MAX;
+#pragma unroll
+  for (int rd = 0; rd < kRounds; ++rd) {
+#pragma unroll
+    for (int jw = 1; jw < kWin; ++jw) {
+      const int g = rd * kSeg + jw;
+      S[jw * kLds + tid] = g < B ? x[g < B ? g : 0] : (x[g >= B ? g - B : 0] + total);
+    }
+#pragma unroll 1
+    for (int m = 0; m < nw; ++m) {
+      const float* Sw = S + k.widths[m] * kLds + tid;
+      float dm = dmax[m];
+#pragma unroll
+      for (int b = 0; b < kSeg; ++b) {
+        const float d = Sw[b * kLds] - x[rd * kSeg + b];
+        dm = d > dm ? d : dm;
+      }
+      dmax[m] = dm;
+    }
+  }
+  float* o = out + (q * ncoords_cur + j) * nw;
+#pragma unroll 1
+  for (int m = 0; m < nw; ++m) {
+    const double t1 = k.hpb[m] * (double)dmax[m];
+    const double t2 = k.bb[m] * (double)total;
+    o[m] = (float)((t1 - t2) * k.inv_std);
   }
 }
 
@@ -391,32 +473,54 @@ extern "C" int pyl_ffa_resolve(int32_t nparams, const double* params_dev,
   return PYL_OK;
 }
 
-template <int B, typename T>
+template <int B, typename T, bool kWrapIn, bool kWrapOut>
 static void launch_shift_add(const T* in, int64_t nprev, const int4* tb, int64_t ncur,
                              int64_t nseg_out, T* out, hipStream_t s) {
   constexpr int RPB = 256 / (B / 2);
   const int64_t nblk_j = pyl::ceil_div(ncur, RPB);
   const int64_t nblocks = nblk_j * nseg_out;
-  ffa_shift_add_kernel<B, T><<<(unsigned)nblocks, 256, 0, s>>>(in, nprev, tb, ncur, nseg_out,
-                                                               nblk_j, out);
+  ffa_shift_add_kernel<B, T, kWrapIn, kWrapOut><<<(unsigned)nblocks, 256, 0, s>>>(
+      in, nprev, tb, ncur, nseg_out, nblk_j, out);
 }
 
-extern "C" int pyl_ffa_shift_add(const float* fold_in, int64_t nseg_in, int64_t ncoords_prev,
-                                 const int32_t* table_dev, int64_t ncoords_cur, int32_t nbins,
-                                 float* fold_out, void* stream) {
+template <int B>
+static void launch_shift_add_f32(const float* in, bool wrap_in, int64_t nprev, const int4* tb,
+                                 int64_t ncur, int64_t nseg_out, float* out, bool wrap_out,
+                                 hipStream_t s) {
+  if (wrap_in) {
+    if (wrap_out) launch_shift_add<B, float, true, true>(in, nprev, tb, ncur, nseg_out, out, s);
+    else launch_shift_add<B, float, true, false>(in, nprev, tb, ncur, nseg_out, out, s);
+  } else {
+    if (wrap_out) launch_shift_add<B, float, false, true>(in, nprev, tb, ncur, nseg_out, out, s);
+    else launch_shift_add<B, float, false, false>(in, nprev, tb, ncur, nseg_out, out, s);
+  }
+}
+
+static bool valid_layout(int32_t layout) {
+  return layout == PYL_FOLD_DENSE || layout == PYL_FOLD_WRAP;
+}
+
+extern "C" int pyl_ffa_shift_add_layout(const float* fold_in, int32_t in_layout, int64_t nseg_in,
+                                        int64_t ncoords_prev, const int32_t* table_dev,
+                                        int64_t ncoords_cur, int32_t nbins, float* fold_out,
+                                        int32_t out_layout, void* stream) {
   PYL_REQUIRE(fold_in && table_dev && fold_out, "null device pointer");
   PYL_REQUIRE(nseg_in >= 2 && nseg_in % 2 == 0, "nsegments must be even and >= 2, got %lld",
               (long long)nseg_in);
   PYL_REQUIRE(ncoords_prev > 0 && ncoords_cur > 0, "empty parameter grid");
   PYL_REQUIRE(nbins > 0, "Number of bins must be positive.");
+  PYL_REQUIRE(valid_layout(in_layout) && valid_layout(out_layout), "invalid fold layout");
+  const bool wi = in_layout == PYL_FOLD_WRAP, wo = out_layout == PYL_FOLD_WRAP;
+  PYL_REQUIRE(!(wi || wo) || nbins == 32 || nbins == 64 || nbins == 128 || nbins == 256,
+              "the wrap layout supports nbins in {32, 64, 128, 256}, got %d", nbins);
   hipStream_t s = pyl::as_stream(stream);
   const int64_t nseg_out = nseg_in / 2;
   const int4* tb = reinterpret_cast<const int4*>(table_dev);
   switch (nbins) {
-    case 32: launch_shift_add<32, float>(fold_in, ncoords_prev, tb, ncoords_cur, nseg_out, fold_out, s); break;
-    case 64: launch_shift_add<64, float>(fold_in, ncoords_prev, tb, ncoords_cur, nseg_out, fold_out, s); break;
-    case 128: launch_shift_add<128, float>(fold_in, ncoords_prev, tb, ncoords_cur, nseg_out, fold_out, s); break;
-    case 256: launch_shift_add<256, float>(fold_in, ncoords_prev, tb, ncoords_cur, nseg_out, fold_out, s); break;
+    case 32: launch_shift_add_f32<32>(fold_in, wi, ncoords_prev, tb, ncoords_cur, nseg_out, fold_out, wo, s); break;
+    case 64: launch_shift_add_f32<64>(fold_in, wi, ncoords_prev, tb, ncoords_cur, nseg_out, fold_out, wo, s); break;
+    case 128: launch_shift_add_f32<128>(fold_in, wi, ncoords_prev, tb, ncoords_cur, nseg_out, fold_out, wo, s); break;
+    case 256: launch_shift_add_f32<256>(fold_in, wi, ncoords_prev, tb, ncoords_cur, nseg_out, fold_out, wo, s); break;
     default: {
       const int64_t total = nseg_out * ncoords_cur * 2 * (int64_t)nbins;
       const int64_t blocks = std::min<int64_t>(pyl::ceil_div(total, 256), 1 << 20);
@@ -428,33 +532,66 @@ extern "C" int pyl_ffa_shift_add(const float* fold_in, int64_t nseg_in, int64_t 
   return PYL_OK;
 }
 
+extern "C" int pyl_ffa_shift_add(const float* fold_in, int64_t nseg_in, int64_t ncoords_prev,
+                                 const int32_t* table_dev, int64_t ncoords_cur, int32_t nbins,
+                                 float* fold_out, void* stream) {
+  return pyl_ffa_shift_add_layout(fold_in, PYL_FOLD_DENSE, nseg_in, ncoords_prev, table_dev,
+                                  ncoords_cur, nbins, fold_out, PYL_FOLD_DENSE, stream);
+}
 
-template <int B, typename T>
+template <int B, typename T, int WC, bool kWrapIn>
+static void launch_rounds(const T* in, int64_t nprev, const int4* tb, int64_t ncur,
+                          int64_t n_inner, int64_t nseg_out, const pyl::ScoreConst& k,
+                          float* out, hipStream_t s) {
+  const int64_t n_outer = ncur / n_inner;
+  int to = 16;  // outer rows per tile: 16 x 16 tiles, fewer outer rows on short outer axes
+  while (to > 1 && n_outer < to) to /= 2;
+  const int64_t tiles_inner = pyl::ceil_div(n_inner, 256 / to);
+  const int64_t tiles_outer = pyl::ceil_div(n_outer, to);
+  const int64_t per_pair = tiles_inner * tiles_outer;
+  const size_t lds = sizeof(float) * (size_t)std::max(16 + WC, 32) * 257;
+  ffa_shift_add_score_rounds_kernel<B, T, WC, kWrapIn>
+      <<<(unsigned)(per_pair * nseg_out), 256, lds, s>>>(in, nprev, tb, ncur, n_inner, to,
+                                                          tiles_inner, per_pair, k, out);
+}
+
+template <int B, typename T, bool kWrapIn>
 static void launch_shift_add_score(const T* in, int64_t nprev, const int4* tb, int64_t ncur,
                                    int64_t n_inner, int64_t nseg_out, const pyl::ScoreConst& k,
                                    float* out, hipStream_t s) {
-  (void)n_inner;
   if constexpr (B <= 64) {
-    const int64_t n_outer = ncur / n_inner;
-    const int to = n_outer >= 4 ? 4 : (n_outer >= 2 ? 2 : 1);
-    const int64_t tiles_inner = pyl::ceil_div(n_inner, 256 / to);
-    const int64_t tiles_outer = pyl::ceil_div(n_outer, to);
-    const int64_t per_pair = tiles_inner * tiles_outer;
-    ffa_shift_add_score_regs_kernel<B, T><<<(unsigned)(per_pair * nseg_out), 256, 0, s>>>(
-        in, nprev, tb, ncur, n_inner, to, tiles_inner, per_pair, k, out);
-    return;
+    // Round-based kernel: the scoring window is 16 + WC prefix values, WC >= widest width.
+    if (k.nw <= kRoundsMaxNW) {
+      if (k.wmax <= 16)
+        return launch_rounds<B, T, 16, kWrapIn>(in, nprev, tb, ncur, n_inner, nseg_out, k, out, s);
+      if (k.wmax <= B / 2)
+        return launch_rounds<B, T, B / 2, kWrapIn>(in, nprev, tb, ncur, n_inner, nseg_out, k, out, s);
+      return launch_rounds<B, T, B - 1, kWrapIn>(in, nprev, tb, ncur, n_inner, nseg_out, k, out, s);
+    }
   }
   const int64_t nblk_j = pyl::ceil_div(ncur, pyl::kScoreBlock);
   const size_t lds = sizeof(float) * (size_t)(B + k.wmax) * pyl::kScoreStride;
-  ffa_shift_add_score_kernel<B, T><<<(unsigned)(nblk_j * nseg_out), pyl::kScoreBlock, lds, s>>>(
-      in, nprev, tb, ncur, nblk_j, k, out);
+  ffa_shift_add_score_kernel<B, T, kWrapIn>
+      <<<(unsigned)(nblk_j * nseg_out), pyl::kScoreBlock, lds, s>>>(in, nprev, tb, ncur,
+                                                                      nblk_j, k, out);
+}
+
+template <int B, typename T>
+static void launch_score_any(const T* in, bool wrap_in, int64_t nprev, const int4* tb,
+                             int64_t ncur, int64_t n_inner, int64_t nseg_out,
+                             const pyl::ScoreConst& k, float* out, hipStream_t s) {
+  if constexpr (sizeof(T) == 4) {
+    if (wrap_in)
+      return launch_shift_add_score<B, T, true>(in, nprev, tb, ncur, n_inner, nseg_out, k, out, s);
+  }
+  launch_shift_add_score<B, T, false>(in, nprev, tb, ncur, n_inner, nseg_out, k, out, s);
 }
 
 template <typename T>
-static int shift_add_score(const T* fold_in, int64_t nseg_in, int64_t ncoords_prev,
-                           const int32_t* table_dev, int64_t ncoords_cur, int64_t n_inner,
-                           int32_t nbins, const int64_t* widths, int32_t nw, double stdnoise,
-                           float* snr_out, void* stream) {
+static int shift_add_score(const T* fold_in, int32_t in_layout, int64_t nseg_in,
+                           int64_t ncoords_prev, const int32_t* table_dev, int64_t ncoords_cur,
+                           int64_t n_inner, int32_t nbins, const int64_t* widths, int32_t nw,
+                           double stdnoise, float* snr_out, void* stream) {
   PYL_REQUIRE(fold_in && table_dev && snr_out, "null device pointer");
   PYL_REQUIRE(nseg_in >= 2 && nseg_in % 2 == 0, "nsegments must be even and >= 2, got %lld",
               (long long)nseg_in);
@@ -463,19 +600,33 @@ static int shift_add_score(const T* fold_in, int64_t nseg_in, int64_t ncoords_pr
               "fused level + score supports nbins in {32, 64, 128}, got %d", nbins);
   PYL_REQUIRE(n_inner >= 1 && ncoords_cur % n_inner == 0,
               "n_inner must divide ncoords_cur (last-axis grid count)");
+  PYL_REQUIRE(valid_layout(in_layout), "invalid fold layout");
+  PYL_REQUIRE(in_layout == PYL_FOLD_DENSE || sizeof(T) == 4, "the wrap layout is FP32 only");
   pyl::ScoreConst k;
   int rc = pyl::build_score_const(nbins, widths, nw, stdnoise, &k);
   if (rc != PYL_OK) return rc;
   hipStream_t s = pyl::as_stream(stream);
   const int64_t nseg_out = nseg_in / 2;
   const int4* tb = reinterpret_cast<const int4*>(table_dev);
+  const bool wi = in_layout == PYL_FOLD_WRAP;
   switch (nbins) {
-    case 32: launch_shift_add_score<32, T>(fold_in, ncoords_prev, tb, ncoords_cur, n_inner, nseg_out, k, snr_out, s); break;
-    case 64: launch_shift_add_score<64, T>(fold_in, ncoords_prev, tb, ncoords_cur, n_inner, nseg_out, k, snr_out, s); break;
-    default: launch_shift_add_score<128, T>(fold_in, ncoords_prev, tb, ncoords_cur, n_inner, nseg_out, k, snr_out, s); break;
+    case 32: launch_score_any<32, T>(fold_in, wi, ncoords_prev, tb, ncoords_cur, n_inner, nseg_out, k, snr_out, s); break;
+    case 64: launch_score_any<64, T>(fold_in, wi, ncoords_prev, tb, ncoords_cur, n_inner, nseg_out, k, snr_out, s); break;
+    default: launch_score_any<128, T>(fold_in, wi, ncoords_prev, tb, ncoords_cur, n_inner, nseg_out, k, snr_out, s); break;
   }
   PYL_CHECK_LAUNCH("ffa_shift_add_score_kernel");
   return PYL_OK;
+}
+
+extern "C" int pyl_ffa_shift_add_score_layout(const float* fold_in, int32_t in_layout,
+                                              int64_t nseg_in, int64_t ncoords_prev,
+                                              const int32_t* table_dev, int64_t ncoords_cur,
+                                              int64_t n_inner, int32_t nbins,
+                                              const int64_t* widths, int32_t nw,
+                                              double stdnoise, float* snr_out, void* stream) {
+  return shift_add_score<float>(fold_in, in_layout, nseg_in, ncoords_prev, table_dev,
+                                ncoords_cur, n_inner, nbins, widths, nw, stdnoise, snr_out,
+                                stream);
 }
 
 extern "C" int pyl_ffa_shift_add_score(const float* fold_in, int64_t nseg_in,
@@ -483,8 +634,9 @@ extern "C" int pyl_ffa_shift_add_score(const float* fold_in, int64_t nseg_in,
                                        int64_t ncoords_cur, int64_t n_inner, int32_t nbins,
                                        const int64_t* widths, int32_t nw, double stdnoise,
                                        float* snr_out, void* stream) {
-  return shift_add_score<float>(fold_in, nseg_in, ncoords_prev, table_dev, ncoords_cur, n_inner,
-                                nbins, widths, nw, stdnoise, snr_out, stream);
+  return shift_add_score<float>(fold_in, PYL_FOLD_DENSE, nseg_in, ncoords_prev, table_dev,
+                                ncoords_cur, n_inner, nbins, widths, nw, stdnoise, snr_out,
+                                stream);
 }
 
 extern "C" int pyl_ffa_shift_add_score_bf16(const uint16_t* fold_in, int64_t nseg_in,
@@ -492,8 +644,9 @@ extern "C" int pyl_ffa_shift_add_score_bf16(const uint16_t* fold_in, int64_t nse
                                             int64_t ncoords_cur, int64_t n_inner, int32_t nbins,
                                             const int64_t* widths, int32_t nw, double stdnoise,
                                             float* snr_out, void* stream) {
-  return shift_add_score<uint16_t>(fold_in, nseg_in, ncoords_prev, table_dev, ncoords_cur,
-                                   n_inner, nbins, widths, nw, stdnoise, snr_out, stream);
+  return shift_add_score<uint16_t>(fold_in, PYL_FOLD_DENSE, nseg_in, ncoords_prev, table_dev,
+                                   ncoords_cur, n_inner, nbins, widths, nw, stdnoise, snr_out,
+                                   stream);
 }
 
 extern "C" int pyl_ffa_shift_add_bf16(const uint16_t* fold_in, int64_t nseg_in,
@@ -510,10 +663,10 @@ extern "C" int pyl_ffa_shift_add_bf16(const uint16_t* fold_in, int64_t nseg_in,
   const int64_t nseg_out = nseg_in / 2;
   const int4* tb = reinterpret_cast<const int4*>(table_dev);
   switch (nbins) {
-    case 32: launch_shift_add<32, uint16_t>(fold_in, ncoords_prev, tb, ncoords_cur, nseg_out, fold_out, s); break;
-    case 64: launch_shift_add<64, uint16_t>(fold_in, ncoords_prev, tb, ncoords_cur, nseg_out, fold_out, s); break;
-    case 128: launch_shift_add<128, uint16_t>(fold_in, ncoords_prev, tb, ncoords_cur, nseg_out, fold_out, s); break;
-    default: launch_shift_add<256, uint16_t>(fold_in, ncoords_prev, tb, ncoords_cur, nseg_out, fold_out, s); break;
+    case 32: launch_shift_add<32, uint16_t, false, false>(fold_in, ncoords_prev, tb, ncoords_cur, nseg_out, fold_out, s); break;
+    case 64: launch_shift_add<64, uint16_t, false, false>(fold_in, ncoords_prev, tb, ncoords_cur, nseg_out, fold_out, s); break;
+    case 128: launch_shift_add<128, uint16_t, false, false>(fold_in, ncoords_prev, tb, ncoords_cur, nseg_out, fold_out, s); break;
+    default: launch_shift_add<256, uint16_t, false, false>(fold_in, ncoords_prev, tb, ncoords_cur, nseg_out, fold_out, s); break;
   }
   PYL_CHECK_LAUNCH("ffa_shift_add_kernel<bf16>");
   return PYL_OK;

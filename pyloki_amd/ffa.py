@@ -247,7 +247,9 @@ def ffa_batch(plan: FFAPlan, series: list[tuple[torch.Tensor, torch.Tensor]],
     first and concatenated along the segment axis, so level L works on every series that
     still has more than one segment in one launch. A series whose last level this is gets
     that level fused with its scoring (``pyl_ffa_shift_add_score``), so its final fold is
-    never written; with ``fused=False`` the final fold is stored and scored separately.
+    never written; the level before it stores its output in the "wrap" row layout, which
+    the fused kernel reads without rotation selects. With ``fused=False`` the final fold is
+    stored and scored separately.
     Returns the per-width S/N grids ``[ncoords_final(series), n_w]`` in input order
     (``ffa_search``'s grid, ``search.py:46-47``).
     """
@@ -274,33 +276,49 @@ def ffa_batch(plan: FFAPlan, series: list[tuple[torch.Tensor, torch.Tensor]],
         off += n
     results: dict[int, torch.Tensor] = {}
     level = 0
+    nbins = plan.cfg.nbins
+    use_fused = fused and nbins in (32, 64, 128)
+    # The level before a series' last one writes the "wrap" row layout (FP32), which lets
+    # the fused last level load every rotated 4-bin chunk with one contiguous load.
+    pre_layout = "wrap" if use_fused and dtype == torch.float32 else "dense"
     # A series with a single segment from the start is scored at level 0.
     off = sum(n for n in nseg if n > 1)
     for k in range(nser):
         if nseg[k] == 1:
             results[order[k]] = kernels.score_boxcar(fold[off], widths, stdnoise)
             off += 1
+    fin_src: torch.Tensor | None = None  # output of the previous level's "pre" group
     while True:
-        cont = [k for k in range(nser) if nseg[k] > 2]
+        big = [k for k in range(nser) if nseg[k] > 4]
+        pre = [k for k in range(nser) if nseg[k] == 4]
         fin = [k for k in range(nser) if nseg[k] == 2]
-        if not cont and not fin:
+        if not big and not pre and not fin:
             break
         lp = plan.levels[level]
-        base = sum(nseg[k] for k in cont)
+        nb = sum(nseg[k] for k in big)
+        npre = 4 * len(pre)
         if fin:
-            part = fold[base:base + 2 * len(fin)]
-            if fused:
+            if fin_src is not None:
+                part, lay = fin_src, pre_layout
+            else:  # series that had 2 segments after the brute fold
+                part, lay = fold[nb + npre:nb + npre + 2 * len(fin)], "dense"
+            if use_fused:
                 snr = kernels.ffa_shift_add_score(part, lp.table, lp.ncoords, widths, stdnoise,
-                                                  n_inner=int(lp.counts_cur[-1]))
+                                                  n_inner=int(lp.counts_cur[-1]),
+                                                  in_layout=lay)
                 for m, k in enumerate(fin):
                     results[order[k]] = snr[m]
             else:
-                last = kernels.ffa_shift_add(part, lp.table, lp.ncoords)
+                last = kernels.ffa_shift_add(part, lp.table, lp.ncoords, in_layout=lay)
                 for m, k in enumerate(fin):
                     results[order[k]] = kernels.score_boxcar(last[m], widths, stdnoise)
-        if cont:
-            fold = kernels.ffa_shift_add(fold[:base], lp.table, lp.ncoords)
-        for k in cont + fin:
+        fin_src = None
+        if pre:
+            fin_src = kernels.ffa_shift_add(fold[nb:nb + npre], lp.table, lp.ncoords,
+                                            out_layout=pre_layout)
+        if big:
+            fold = kernels.ffa_shift_add(fold[:nb], lp.table, lp.ncoords)
+        for k in big + pre + fin:
             nseg[k] //= 2
         level += 1
     return [results[i] for i in range(nser)]

@@ -95,66 +95,98 @@ def ffa_resolve(params_dev: torch.Tensor, counts_cur, counts_prev, param_limits:
     return table
 
 
-def ffa_shift_add(fold_in: torch.Tensor, table: torch.Tensor, ncoords_cur: int,
-                  out: torch.Tensor | None = None) -> torch.Tensor:
-    """Segment-pair shift-add of one level: [nseg, nprev, 2, B] -> [nseg/2, ncur, 2, B].
+FOLD_LAYOUTS = {"dense": _lib.PYL_FOLD_DENSE, "wrap": _lib.PYL_FOLD_WRAP}
+WRAP_PAD = 4  # bins copied after each component in the "wrap" row layout
 
-    FP32 or BF16 storage (FP32 arithmetic); the output has the input's dtype."""
+
+def _layout(name: str) -> int:
+    if name not in FOLD_LAYOUTS:
+        msg = f"layout must be one of {sorted(FOLD_LAYOUTS)}, got {name!r}"
+        raise ValueError(msg)
+    return FOLD_LAYOUTS[name]
+
+
+def _check_level_input(fold_in: torch.Tensor, table: torch.Tensor, ncoords_cur: int,
+                       in_layout: str) -> tuple[int, int, int]:
     if fold_in.dim() != 4 or fold_in.shape[2] != 2 or fold_in.dtype not in (
             torch.float32, torch.bfloat16):
         msg = "fold_in must be float32/bfloat16 [nseg, ncoords, 2, nbins]"
         raise ValueError(msg)
-    nseg, nprev, _, nbins = fold_in.shape
+    if in_layout != "dense" and fold_in.dtype != torch.float32:
+        msg = "the wrap layout is FP32 only"
+        raise ValueError(msg)
+    nseg, nprev, _, width = fold_in.shape
+    nbins = width - (WRAP_PAD if _layout(in_layout) == _lib.PYL_FOLD_WRAP else 0)
     if nseg % 2:
         msg = f"nsegments must be even, got {nseg}"
         raise ValueError(msg)
     if tuple(table.shape) != (ncoords_cur, 4):
         msg = "table does not match ncoords_cur"
         raise ValueError(msg)
-    shape = (nseg // 2, ncoords_cur, 2, nbins)
+    if not fold_in.is_contiguous():
+        msg = "fold_in must be contiguous"
+        raise ValueError(msg)
+    return nseg, nprev, nbins
+
+
+def ffa_shift_add(fold_in: torch.Tensor, table: torch.Tensor, ncoords_cur: int,
+                  out: torch.Tensor | None = None, *, in_layout: str = "dense",
+                  out_layout: str = "dense") -> torch.Tensor:
+    """Segment-pair shift-add of one level: [nseg, nprev, 2, B] -> [nseg/2, ncur, 2, B].
+
+    FP32 or BF16 storage (FP32 arithmetic); the output has the input's dtype. FP32 folds
+    may use the "wrap" row layout ([.., 2, B + 4], each component followed by a copy of
+    its first 4 bins) on either side."""
+    nseg, nprev, nbins = _check_level_input(fold_in, table, ncoords_cur, in_layout)
+    lay_out = _layout(out_layout)
+    if out_layout != "dense" and fold_in.dtype != torch.float32:
+        msg = "the wrap layout is FP32 only"
+        raise ValueError(msg)
+    shape = (nseg // 2, ncoords_cur, 2, nbins + (WRAP_PAD if lay_out == _lib.PYL_FOLD_WRAP
+                                                  else 0))
     if out is None:
         out = torch.empty(shape, dtype=fold_in.dtype, device=fold_in.device)
-    elif tuple(out.shape) != shape or out.dtype != fold_in.dtype:
-        msg = f"out must have shape {shape} and dtype {fold_in.dtype}"
+    elif tuple(out.shape) != shape or out.dtype != fold_in.dtype or not out.is_contiguous():
+        msg = f"out must be contiguous with shape {shape} and dtype {fold_in.dtype}"
         raise ValueError(msg)
-    fn = (_lib.load().pyl_ffa_shift_add if fold_in.dtype == torch.float32
-          else _lib.load().pyl_ffa_shift_add_bf16)
-    _lib.check(fn(ptr(fold_in), nseg, nprev, ptr(table), ncoords_cur, nbins, ptr(out),
-                  stream_ptr()))
+    lib = _lib.load()
+    if fold_in.dtype == torch.float32:
+        _lib.check(lib.pyl_ffa_shift_add_layout(ptr(fold_in), _layout(in_layout), nseg, nprev,
+                                                ptr(table), ncoords_cur, nbins, ptr(out),
+                                                lay_out, stream_ptr()))
+    else:
+        _lib.check(lib.pyl_ffa_shift_add_bf16(ptr(fold_in), nseg, nprev, ptr(table),
+                                              ncoords_cur, nbins, ptr(out), stream_ptr()))
     return out
 
 
 def ffa_shift_add_score(fold_in: torch.Tensor, table: torch.Tensor, ncoords_cur: int,
                         widths: np.ndarray, stdnoise: float = 1.0,
-                        out: torch.Tensor | None = None, n_inner: int | None = None
-                        ) -> torch.Tensor:
+                        out: torch.Tensor | None = None, n_inner: int | None = None, *,
+                        in_layout: str = "dense") -> torch.Tensor:
     """Last level fused with scoring: [nseg, nprev, 2, B] -> S/N [nseg/2, ncur, n_w].
 
     ``n_inner`` is the last-axis (frequency) grid count (defaults to ``ncoords_cur``); it
-    shapes the output tiling only, not the result."""
-    if fold_in.dim() != 4 or fold_in.shape[2] != 2 or fold_in.dtype not in (
-            torch.float32, torch.bfloat16):
-        msg = "fold_in must be float32/bfloat16 [nseg, ncoords, 2, nbins]"
-        raise ValueError(msg)
-    nseg, nprev, _, nbins = fold_in.shape
-    if nseg % 2:
-        msg = f"nsegments must be even, got {nseg}"
-        raise ValueError(msg)
-    if tuple(table.shape) != (ncoords_cur, 4):
-        msg = "table does not match ncoords_cur"
-        raise ValueError(msg)
+    shapes the output tiling only, not the result. ``in_layout`` as for ffa_shift_add."""
+    nseg, nprev, nbins = _check_level_input(fold_in, table, ncoords_cur, in_layout)
     widths = np.ascontiguousarray(widths, dtype=np.int64)
     shape = (nseg // 2, ncoords_cur, len(widths))
     if out is None:
         out = torch.empty(shape, dtype=torch.float32, device=fold_in.device)
-    elif tuple(out.shape) != shape:
-        msg = f"out must have shape {shape}"
+    elif tuple(out.shape) != shape or out.dtype != torch.float32 or not out.is_contiguous():
+        msg = f"out must be contiguous float32 with shape {shape}"
         raise ValueError(msg)
-    fn = (_lib.load().pyl_ffa_shift_add_score if fold_in.dtype == torch.float32
-          else _lib.load().pyl_ffa_shift_add_score_bf16)
     n_inner = ncoords_cur if n_inner is None else int(n_inner)
-    _lib.check(fn(ptr(fold_in), nseg, nprev, ptr(table), ncoords_cur, n_inner, nbins,
-                  _lib.i64_array(widths), len(widths), float(stdnoise), ptr(out), stream_ptr()))
+    lib = _lib.load()
+    if fold_in.dtype == torch.float32:
+        _lib.check(lib.pyl_ffa_shift_add_score_layout(
+            ptr(fold_in), _layout(in_layout), nseg, nprev, ptr(table), ncoords_cur, n_inner,
+            nbins, _lib.i64_array(widths), len(widths), float(stdnoise), ptr(out),
+            stream_ptr()))
+    else:
+        _lib.check(lib.pyl_ffa_shift_add_score_bf16(
+            ptr(fold_in), nseg, nprev, ptr(table), ncoords_cur, n_inner, nbins,
+            _lib.i64_array(widths), len(widths), float(stdnoise), ptr(out), stream_ptr()))
     return out
 
 

@@ -133,20 +133,67 @@ def test_score_bit_exact(gpu_device, nbins, n):
     np.testing.assert_allclose(got, exp, rtol=1e-5)
 
 
-@pytest.mark.parametrize("nbins", [32, 64, 128])
-def test_fused_last_level_score_bit_exact(gpu_device, nbins):
-    _case(f"fused level+score nbins={nbins}")
+def _width_set(nbins: int, kind: str) -> np.ndarray:
+    if kind == "default":
+        return np.array([1, 2, 3, 5, 8, nbins // 4], dtype=np.int64)
+    if kind == "half":       # widest width in (16, nbins/2]
+        return np.array([1, 4, nbins // 2], dtype=np.int64)
+    if kind == "wide":       # widest width nbins - 1
+        return np.array([1, 7, nbins // 2 + 3, nbins - 1], dtype=np.int64)
+    return np.arange(1, 21, dtype=np.int64)  # "many": more than 16 widths
+
+
+@pytest.mark.parametrize("nbins,kind", [(32, "default"), (64, "default"), (128, "default"),
+                                        (32, "wide"), (64, "half"), (64, "wide"),
+                                        (64, "many")])
+def test_fused_last_level_score_bit_exact(gpu_device, nbins, kind):
+    _case(f"fused level+score nbins={nbins} widths={kind}")
     cfg = _cfg(2, 2**18, nbins, 4096)
     rng = np.random.default_rng(nbins)
     ts_e = rng.normal(size=cfg.nsamps).astype(np.float32)
     plan = FFAPlan(cfg)
     folds = plan.run(_dev(ts_e), _dev(np.ones(cfg.nsamps, dtype=np.float32)), keep=True)
     lp = plan.levels[-1]
-    widths = np.array([1, 2, 3, 5, 8, nbins // 4], dtype=np.int64)
+    widths = _width_set(nbins, kind)
     exp = orc.boxcar_snr(orc.fold_norm(folds[-1].cpu().numpy()), widths)
     for n_inner in (None, int(lp.counts_cur[-1])):   # 1-D rows and (outer x inner) tiles
         got = kernels.ffa_shift_add_score(folds[-2], lp.table, lp.ncoords, widths,
                                           n_inner=n_inner).cpu().numpy()
+        np.testing.assert_array_equal(got, exp)
+
+
+@pytest.mark.parametrize("nbins", [32, 64, 128])
+def test_wrap_layout_levels_bit_exact(gpu_device, nbins):
+    """The wrap row layout ([.., 2, B + 4], each component followed by its first 4 bins) on
+    either side of a level gives the dense level's values, and the fused last level reads
+    it to the same S/N (rounds kernel and, for nbins 128 or many widths, the LDS kernel)."""
+    _case(f"wrap layout nbins={nbins}")
+    cfg = _cfg(2, 2**18, nbins, 4096)
+    rng = np.random.default_rng(100 + nbins)
+    ts_e = rng.normal(size=cfg.nsamps).astype(np.float32)
+    plan = FFAPlan(cfg)
+    folds = plan.run(_dev(ts_e), _dev(np.ones(cfg.nsamps, dtype=np.float32)), keep=True)
+    lp_prev, lp = plan.levels[-2], plan.levels[-1]
+    src = folds[-3]
+    dense = kernels.ffa_shift_add(src, lp_prev.table, lp_prev.ncoords)
+    wrap = kernels.ffa_shift_add(src, lp_prev.table, lp_prev.ncoords, out_layout="wrap")
+    assert wrap.shape[-1] == nbins + 4
+    np.testing.assert_array_equal(wrap[..., :nbins].cpu().numpy(), dense.cpu().numpy())
+    np.testing.assert_array_equal(wrap[..., nbins:].cpu().numpy(), dense[..., :4].cpu().numpy())
+    src_wrap = torch.cat([src, src[..., :4]], dim=-1).contiguous()
+    for lay_out in ("dense", "wrap"):
+        got = kernels.ffa_shift_add(src_wrap, lp_prev.table, lp_prev.ncoords, in_layout="wrap",
+                                    out_layout=lay_out)
+        np.testing.assert_array_equal(got[..., :nbins].cpu().numpy(), dense.cpu().numpy())
+    for kind in ("default", "many"):
+        widths = _width_set(nbins, kind)
+        exp = kernels.ffa_shift_add_score(dense, lp.table, lp.ncoords, widths,
+                                          n_inner=int(lp.counts_cur[-1])).cpu().numpy()
+        np.testing.assert_array_equal(
+            exp, orc.boxcar_snr(orc.fold_norm(folds[-1].cpu().numpy()), widths))
+        got = kernels.ffa_shift_add_score(wrap, lp.table, lp.ncoords, widths,
+                                          n_inner=int(lp.counts_cur[-1]),
+                                          in_layout="wrap").cpu().numpy()
         np.testing.assert_array_equal(got, exp)
 
 
