@@ -17,7 +17,32 @@ reference's real-fold Taylor pruning with a moving grid:
 - ``ascend``: ``dyn_poly_taylor.py:587-628``, ``taylor.py:280-320``, ``common.py:432-459``.
 - ``report_batch``: ``taylor.py:342-360``.
 
-Pinning: tests/test_prune_oracle.py checks branching against the exact-tiling invariant
+Circular basis (``circ_*``, 5 parameters [crackle, snap, jerk, accel, vel]):
+
+- ``circ_mask``: ``circular.py:13-79,127-166``; ``circ_validate``: ``circular.py:82-124,
+  397-440``.
+- ``circ_branch_batch``: ``circular.py:204-394`` (crackle held at the parent value and
+  expanded only for children in the snap-accel "hole" whose crackle still needs a split;
+  kept children first, then each expanded child's crackle children).
+- ``shift_taylor_circular``: ``transforms.py:248-313`` (values; ``:384-432`` keeps the
+  errors unchanged); ``circ_resolve_batch``: ``circular.py:443-528`` with its in-place
+  basis-flag writes (snap 1, crackle 2, crackle reset to 0 when the batch holds a Taylor
+  cell); ``circ_transform_batch``: ``circular.py:669-705``; ``circ_ascend``:
+  ``dyn_circular_taylor.py:636-678`` with ``circular.py:605-666``.
+- One deliberate difference: the reference's circular branch leaves the d0 error of its
+  output uninitialised (``np.empty``, ``circular.py:339``); it is never read by the search,
+  and here it is 0.
+
+Parity unpinned here: the reference runs these functions under numba ``fastmath``, so
+its last-ulp choices (contraction, reassociation) are not reproduced; NumPy's cos/sin
+(glibc) stand for the reference's.
+
+Pinning: tests/test_circ_oracle.py checks the circular propagator against an analytic
+orbit, the region masks and validation rules on hand-made cells, the branch against the
+Taylor branch (no crackle split) and the crackle expansion's order and tiling, resolve
+against the Taylor resolve (Taylor cells) and the analytic orbit, the flag writes, and a
+level's survivors, flags, backtracks and transformed parameters against a child-by-child
+brute force. tests/test_prune_oracle.py checks branching against the exact-tiling invariant
 and closed-form child counts, resolve against exact rational arithmetic, the level loop's
 survivor set against a leaf-by-leaf brute force (every child scored independently, kept
 iff score >= threshold) when the buffer does not overflow, the overflow rule against a
@@ -687,27 +712,37 @@ def circ_shift_all(vec, dt, m_snap, m_crackle, m_taylor):
     return out
 
 
-def circ_resolve_batch(leaves, coord_add, coord_cur, coord_init, counts_init, limits, nbins,
-                       sig):
-    """circ_taylor_resolve_batch (circular.py:443-528), including its in-place basis-flag
-    writes: snap -> 1, crackle -> 2, and crackle leaves reset to 0 when the batch also holds
-    a Taylor leaf (the reference indexes idx_circ_crackle in the Taylor branch, :510)."""
+def circ_resolve_values(leaves, coord_add, coord_cur, coord_init, sig):
+    """The (accel, freq, delay) a leaf resolves to at the added segment, and the masks
+    (circ_taylor_resolve_batch, circular.py:443-505)."""
     t0_cur, t0_init, t0_add = coord_cur[0], coord_init[0], coord_add[0]
     vec = leaves[:, :-1, 0]
     f0 = leaves[:, -1, 0]
     ms, mc, mt = circ_mask(leaves, sig)
     d_add = circ_shift_all(vec, t0_add - t0_cur, ms, mc, mt)
     d_init = circ_shift_all(vec, t0_init - t0_cur, ms, mc, mt)
+    accel = d_add[:, -3]
+    vel = d_add[:, -2] - d_init[:, -2]
+    freq = f0 * (1 - vel / C_VAL)
+    delay = (d_add[:, -1] - d_init[:, -1]) / C_VAL
+    return accel, freq, delay, (ms, mc, mt)
+
+
+def circ_resolve_batch(leaves, coord_add, coord_cur, coord_init, counts_init, limits, nbins,
+                       sig):
+    """circ_taylor_resolve_batch (circular.py:443-528), including its in-place basis-flag
+    writes: snap -> 1, crackle -> 2, and crackle leaves reset to 0 when the batch also holds
+    a Taylor leaf (the reference indexes idx_circ_crackle in the Taylor branch, :510)."""
+    t0_init, t0_add = coord_init[0], coord_add[0]
+    f0 = leaves[:, -1, 0]
+    accel, freq, delay, (ms, mc, mt) = circ_resolve_values(leaves, coord_add, coord_cur,
+                                                           coord_init, sig)
     if ms.any():
         leaves[ms, -1, 1] = 1
     if mc.any():
         leaves[mc, -1, 1] = 2
     if mt.any():
         leaves[mc, -1, 1] = 0
-    accel = d_add[:, -3]
-    vel = d_add[:, -2] - d_init[:, -2]
-    freq = f0 * (1 - vel / C_VAL)
-    delay = (d_add[:, -1] - d_init[:, -1]) / C_VAL
     phase = phase_idx_vec(t0_add - t0_init, f0, nbins, delay)
     return nearest_2d(accel, freq, counts_init, limits), phase
 
