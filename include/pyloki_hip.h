@@ -238,6 +238,101 @@ int pyl_prune_ascend(const pyl_prune_params* p, const double* leaves_dev, int64_
                      const double* seg_t0, int32_t nseg_used, double t0_cur,
                      float* out_folds_dev, float* out_scores_dev, void* stream);
 
+/* ------------------------------------------------------------------------------------
+ * EP, circular basis (prune_poly_order = 5: crackle, snap, jerk, accel, vel; leaf rows 7).
+ * The entry points replace the circular plugin (dyn_circular_taylor.py:484-678) driven
+ * by pruning_iteration_batched (prune.py:165-332):
+ *  pyl_circ_branch     circ_taylor_branch_batch (circular.py:204-394) per parent: every
+ *                      parameter but crackle as in the Taylor branch; crackle is held at the
+ *                      parent value (its step already the split step when it needs a split)
+ *                      and its split (first centre, count) kept for the expansion;
+ *  pyl_circ_stage1     per held child: parent and whether it expands (snap-accel "hole",
+ *                      circular.py:169-201, with a crackle that needs a split);
+ *  (caller)            final child order per batch of batch_size parents: the kept
+ *                      children in held order, then every expanding child's crackle children
+ *                      (circular.py:359-394) -> s1_pos;
+ *  pyl_circ_expand     per final child: its held child and crackle index, validation
+ *                      (circ_taylor_validate_batch, circular.py:82-124,397-440) and region
+ *                      (get_circ_taylor_mask, circular.py:13-79,127-166);
+ *  pyl_circ_children   per validated child: resolve with the region's propagator
+ *                      (circ_taylor_resolve_batch, circular.py:443-528; transforms.py:248-313),
+ *                      shift-add and score;
+ *  pyl_circ_materialize per survivor: basis flag (snap 1, crackle 2, or 0 when its batch
+ *                      holds a Taylor cell, circular.py:478-511), transform
+ *                      (circ_taylor_transform_batch, circular.py:669-705), backtrack, fold;
+ *  pyl_circ_ascend     ascend_func (dyn_circular_taylor.py:636-678, circular.py:605-666).
+ * ------------------------------------------------------------------------------------ */
+typedef struct pyl_circ_params {
+  double prop_sig;        /* propagator_significance (region classification) */
+  double val_sig;         /* validation_significance */
+  double omega_max_sq;    /* (2 pi / p_orb_min)^2 */
+  double x_mass_const;    /* PulsarSearchConfig.x_mass_const */
+  int64_t batch_size;     /* parents per batch (the reference's batch_size) */
+} pyl_circ_params;
+
+/* Device buffers of one circular level (allocated by the caller). */
+typedef struct pyl_circ_level {
+  double* branch;         /* f64 [n_par][5][2] {first centre or held value, step} */
+  int32_t* counts;        /* i32 [n_par][5]: children per parameter, -1 = held (crackle) */
+  double* cr_first;       /* f64 [n_par] first crackle centre of an expansion */
+  int32_t* cr_num;        /* i32 [n_par] crackle children of an expansion (0: no split) */
+  int64_t* off1;          /* i64 [n_par + 1] exclusive prefix of held children per parent */
+  int32_t* s1_parent;     /* i32 [n_s1] */
+  uint8_t* s1_expand;     /* u8  [n_s1] */
+  int64_t* s1_pos;        /* i64 [n_s1] final position of the held child's first output */
+  int64_t* c_s1;          /* i64 [n_child] held child of every final child */
+  int32_t* c_sub;         /* i32 [n_child] crackle index, -1 = held crackle */
+  uint8_t* c_keep;        /* u8  [n_child] passes validation */
+  int8_t* c_region;       /* i8  [n_child] 0 snap-circular, 1 crackle-circular, 2 Taylor */
+  int64_t n_par, n_s1, n_child;
+} pyl_circ_level;
+
+#define PYL_REGION_SNAP 0
+#define PYL_REGION_CRACKLE 1
+#define PYL_REGION_TAYLOR 2
+
+/* Writes branch, counts, cr_first, cr_num and nchild1_dev (i64 [n_par], held children per
+ * parent). err_dev as for pyl_prune_branch; the key orders failures the way the reference
+ * raises them: per batch, the non-crackle parameters of every parent, then crackle. */
+int pyl_circ_branch(const pyl_prune_params* p, const pyl_circ_params* q,
+                    const pyl_prune_coords* c, const double* leaves_dev,
+                    const pyl_circ_level* L, int64_t* nchild1_dev, int64_t* err_dev,
+                    void* stream);
+/* Reads branch, counts, cr_num, off1; writes s1_parent, s1_expand. */
+int pyl_circ_stage1(const pyl_prune_params* p, const pyl_circ_params* q,
+                    const pyl_prune_coords* c, const double* leaves_dev,
+                    const pyl_circ_level* L, void* stream);
+/* Reads the stage-1 buffers and s1_pos; writes c_s1, c_sub, c_keep, c_region. */
+int pyl_circ_expand(const pyl_prune_params* p, const pyl_circ_params* q,
+                    const pyl_prune_coords* c, const double* leaves_dev,
+                    const pyl_circ_level* L, void* stream);
+/* valid_dev: i64 [n_valid] final-child ids that pass validation, increasing. Outputs per
+ * validated child as for pyl_prune_children. */
+int pyl_circ_children(const pyl_prune_params* p, const pyl_circ_params* q,
+                      const pyl_prune_coords* c, const double* leaves_dev,
+                      const float* folds_dev, const pyl_circ_level* L, const int64_t* valid_dev,
+                      int64_t n_valid, const float* seg_fold_dev, int32_t* child_parent_dev,
+                      int32_t* child_row_dev, int32_t* child_shift_dev, double* child_phase_dev,
+                      float* child_score_dev, float* minmax_dev, void* stream);
+/* surv_dev: i64 [n_surv] indices into valid_dev, increasing; has_taylor_dev: u8 [n_batches]
+ * whether the batch's validated children include a Taylor cell. Writes the new tree. */
+int pyl_circ_materialize(const pyl_prune_params* p, const pyl_circ_params* q,
+                         const pyl_prune_coords* c, const double* leaves_dev,
+                         const float* folds_dev, const pyl_circ_level* L,
+                         const int64_t* valid_dev, const float* seg_fold_dev,
+                         const int32_t* child_parent_dev, const int32_t* child_row_dev,
+                         const int32_t* child_shift_dev, const double* child_phase_dev,
+                         const float* child_score_dev, const uint8_t* has_taylor_dev,
+                         const int64_t* surv_dev, int64_t n_surv, double* out_leaves_dev,
+                         float* out_folds_dev, float* out_scores_dev,
+                         int32_t* out_backtracks_dev, void* stream);
+/* As pyl_prune_ascend, each survivor propagated with its region's propagator. */
+int pyl_circ_ascend(const pyl_prune_params* p, const pyl_circ_params* q,
+                    const double* leaves_dev, int64_t n, const float* dyp_fold_dev,
+                    int64_t ncoords_ffa, const int64_t* seg_idx, const double* seg_t0,
+                    int32_t nseg_used, double t0_cur, float* out_folds_dev,
+                    float* out_scores_dev, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

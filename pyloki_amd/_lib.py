@@ -59,6 +59,16 @@ SIGNATURES: list[tuple[str, object, list[object]]] = [
       _vp]),
     ("pyl_score_boxcar", ctypes.c_int,
      [_vp, _c_i64, _c_i32, _c_i32, _i64p, _c_i32, _c_dbl, _c_i32, _vp, _vp]),
+    ("pyl_circ_branch", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("pyl_circ_stage1", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    ("pyl_circ_expand", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    ("pyl_circ_children", ctypes.c_int,
+     [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("pyl_circ_materialize", ctypes.c_int,
+     [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_i64, _vp,
+      _vp, _vp, _vp, _vp]),
+    ("pyl_circ_ascend", ctypes.c_int,
+     [_vp, _vp, _vp, _c_i64, _vp, _c_i64, _i64p, _f64p, _c_i32, _c_dbl, _vp, _vp, _vp]),
     ("pyl_prune_branch", ctypes.c_int, [_vp, _vp, _vp, _c_i64, _vp, _vp, _vp, _vp, _vp]),
     ("pyl_prune_children", ctypes.c_int,
      [_vp, _vp, _vp, _vp, _c_i64, _vp, _vp, _vp, _c_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,

@@ -1,0 +1,566 @@
+// EP search in the circular basis (core/circular.py) on the GPU: one pruning level of
+// prune.py:165-332 with the circular plugin (dyn_circular_taylor.py:484-678), plus ascend.
+//
+// Leaves have 7 rows: [crackle, snap, jerk, accel, vel, d0, (f0, basis flag)].
+//
+// Child enumeration. The reference branches every parameter but crackle (odometer order,
+// crackle held at the parent value), then, inside each batch of parents, keeps the children
+// that are not in the snap-accel "hole" in that order and appends, for every hole child
+// whose crackle still needs a split, its crackle children. Here:
+//   circ_branch_kernel   thread per parent: counts / first centre / step of snap..vel as in
+//                        the Taylor branch; crackle held (step already the split step when
+//                        a split is needed) and its split kept (first centre, count);
+//   circ_stage1_kernel   thread per held child: parent and the expand decision;
+//   (caller)             per-batch prefix sums -> final position of every held child;
+//   circ_expand_kernel   thread per held child, looping over its outputs: held child and
+//                        crackle index of every final child, validation and region;
+//   circ_children_kernel thread per validated child: resolve with the region's propagator,
+//                        then shift-add + score as in the Taylor level;
+//   circ_materialize_kernel thread per survivor: basis flag, transform, backtrack, fold.
+//
+// The circular propagator (transforms.py:248-313) uses the device's FP64 cos/sin; the
+// other operations are the reference's, in its order, with no contraction.
+#include "pyl_prune_common.hpp"
+
+namespace {
+
+using namespace pyl_ep;
+
+constexpr int kP = 5;      // crackle, snap, jerk, accel, vel
+constexpr int kN = kP + 1;  // with d0
+constexpr int kRows = kP + 2;
+
+struct CircConst {
+  double prop_sig, val_sig, omega_max_sq, x_mass_const;
+  int64_t batch_size;
+  double dt_add, dt_init, dt_next;  // t0_add - t0_cur, t0_init - t0_cur, t0_next - t0_cur
+};
+
+int build_circ(const pyl_prune_params* p, const pyl_circ_params* q, const pyl_prune_coords* c,
+               LevelConst* k, CircConst* cc) {
+  PYL_REQUIRE(p != nullptr && q != nullptr, "null parameters");
+  PYL_REQUIRE(p->poly_order == kP, "the circular basis needs poly_order 5, got %d",
+              p->poly_order);
+  PYL_REQUIRE(p->nparams_ffa == kP, "the circular basis needs a 5-parameter FFA grid");
+  PYL_REQUIRE(q->batch_size >= 1, "batch_size must be >= 1");
+  PYL_REQUIRE(q->prop_sig > 0 && q->val_sig > 0, "significances must be positive");
+  int rc = build_const(p, c, k);
+  if (rc != PYL_OK) return rc;
+  cc->prop_sig = q->prop_sig;
+  cc->val_sig = q->val_sig;
+  cc->omega_max_sq = q->omega_max_sq;
+  cc->x_mass_const = q->x_mass_const;
+  cc->batch_size = q->batch_size;
+  cc->dt_add = c ? c->t0_add - c->t0_cur : 0.0;
+  cc->dt_init = c ? c->t0_init - c->t0_cur : 0.0;
+  cc->dt_next = c ? c->t0_next - c->t0_cur : 0.0;
+  return PYL_OK;
+}
+
+// circular.py:13-79: region of a cell (values v, errors e over crackle..accel).
+__device__ __forceinline__ int8_t circ_region(const double* v, const double* e, double sig) {
+  const bool sig_s = fabs(v[1]) > sig * e[1];
+  const bool sig_a = fabs(v[3]) > sig * e[3];
+  const bool sig_j = fabs(v[2]) > sig * e[2];
+  if (sig_s && sig_a && (-v[1]) * v[3] > 0.0) return PYL_REGION_SNAP;
+  if (sig_j && (-v[0]) * v[2] > 0.0) return PYL_REGION_CRACKLE;
+  return PYL_REGION_TAYLOR;
+}
+
+// circular.py:82-124: false when validation removes the cell.
+__device__ __forceinline__ bool circ_valid(const CircConst& q, const double* v,
+                                           const double* e) {
+  const bool sig_s = fabs(v[1]) > q.val_sig * e[1];
+  const bool sig_a = fabs(v[3]) > q.val_sig * e[3];
+  if (!(sig_s && sig_a)) return true;  // the hole and unresolved cells are never validated
+  if (!((-v[1]) * v[3] > 0.0)) return false;
+  const double osq = (-v[1]) / v[3];
+  const double lim = q.x_mass_const * pow(osq, 2.0 / 3.0);
+  return osq <= q.omega_max_sq && fabs(v[3]) <= lim;
+}
+
+// transforms.py:248-313: values [d5..d0] moved by dt along the circular orbit pinned by
+// -d4/d2 (snap region) or -d5/d3 (hole).
+__device__ __forceinline__ void circ_shift(const double* v, double dt, bool in_hole,
+                                           double* o) {
+  const double d5 = v[0], d4 = v[1], d3 = v[2], d2 = v[3], d1 = v[4], d0 = v[5];
+  const double osq = in_hole ? (-d5) / d3 : (-d4) / d2;
+  const double om = sqrt(osq);
+  const double odt = om * dt;
+  const double cs = cos(odt);
+  const double sn = sin(odt);
+  const double d2j = d2 * cs + (d3 / om) * sn;
+  const double d3j = d3 * cs - (d2 * om) * sn;
+  o[3] = d2j;
+  o[2] = d3j;
+  o[1] = (-osq) * d2j;
+  o[0] = (-osq) * d3j;
+  const double d1ci = (-d3) / osq;
+  const double d1cj = (-d3j) / osq;
+  o[4] = d1cj + (d1 - d1ci);
+  const double d0cj = (-d2j) / osq;
+  const double d0ci = (-d2) / osq;
+  o[5] = d0cj + (d0 - d0ci) + (d1 - d1ci) * dt;
+}
+
+// Values moved by dt with the region's propagator; T = Taylor translation by dt.
+__device__ __forceinline__ void region_shift(int8_t reg, const double* v, double dt,
+                                             const double* T, double* o) {
+  if (reg == PYL_REGION_TAYLOR) {
+    for (int i = 0; i < kN; ++i) o[i] = mat_row(T, v, kN, i);
+  } else {
+    circ_shift(v, dt, reg == PYL_REGION_CRACKLE, o);
+  }
+}
+
+// A final child: held child s1 (parent par) with crackle index sub (-1: held crackle).
+__device__ __forceinline__ void circ_decode(const LevelConst& k, const double* __restrict__ leaves,
+                                            const pyl_circ_level& L, int64_t par, int64_t s1,
+                                            int32_t sub, ChildLeaf& ch) {
+  const double* lf = leaves + par * kRows * 2;
+  const double* br = L.branch + par * kP * 2;
+  decode_child(k, lf, br, L.counts + par * kP, s1 - L.off1[par], ch);
+  if (sub >= 0) ch.val[0] = L.cr_first[par] + (double)sub * br[1];
+}
+
+__global__ void circ_branch_kernel(LevelConst k, CircConst q, const double* __restrict__ leaves,
+                                   pyl_circ_level L, int64_t* __restrict__ nchild,
+                                   unsigned long long* __restrict__ err) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= L.n_par) return;
+  const double* lf = leaves + i * kRows * 2;
+  const double f0 = lf[(kP + 1) * 2];
+  // Failures in the order the reference raises them: per batch, parameters 1..4 of every
+  // parent (circular.py:255-265), then crackle (:267-277).
+  const int64_t b = i / q.batch_size;
+  const int64_t ib = i - b * q.batch_size;
+  int64_t total = 1;
+  for (int j = 0; j < kP; ++j) {
+    const double pc = lf[2 * j];
+    const double dc = lf[2 * j + 1];
+    const double dn = k.dpf[j] * pyl::kCVal / f0;
+    const double shift = fabs(dc - dn) * (k.fac[j] * (f0 / pyl::kCVal));
+    const int64_t key = ((b * 2 + (j == 0 ? 1 : 0)) * q.batch_size + ib) * kP + j;
+    int32_t cnt = 1;
+    double first = pc, step = dc;
+    if (dc <= 0.0 || dn <= 0.0) {
+      atomicMin(err, (unsigned long long)(key * 4 + 2));
+    } else {
+      double nump = ceil(dc / dn - pyl::kFloatEps);
+      if (nump < 1.0) nump = 1.0;
+      if (nump > (double)k.branch_max) {
+        atomicMin(err, (unsigned long long)(key * 4 + 1));
+        nump = 1.0;
+      }
+      cnt = (int32_t)nump;
+      step = dc / (double)cnt;
+      first = (pc - dc / 2.0) + step / 2.0;
+    }
+    const bool split = !(shift < (k.eta - pyl::kFloatEps));
+    if (j == 0) {
+      // circular.py:267-285: held at the parent value; the step is the split step when
+      // crackle needs a split, else the parent's.
+      L.cr_first[i] = first;
+      L.cr_num[i] = split ? cnt : 0;
+      L.branch[(i * kP) * 2] = pc;
+      L.branch[(i * kP) * 2 + 1] = split ? step : dc;
+      L.counts[i * kP] = -1;
+      continue;
+    }
+    if (!split) {
+      cnt = -1;
+      first = pc;
+      step = dc;
+    }
+    L.branch[(i * kP + j) * 2] = first;
+    L.branch[(i * kP + j) * 2 + 1] = step;
+    L.counts[i * kP + j] = cnt;
+    total *= cnt < 0 ? 1 : cnt;
+  }
+  nchild[i] = total;
+}
+
+__global__ void circ_stage1_kernel(LevelConst k, CircConst q, const double* __restrict__ leaves,
+                                   pyl_circ_level L) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= L.n_s1) return;
+  const int64_t par = find_parent(L.off1, L.n_par, s);
+  ChildLeaf ch;
+  circ_decode(k, leaves, L, par, s, -1, ch);
+  // circular.py:169-201: the snap-accel system is not circular but jerk is significant.
+  const bool sig_s = fabs(ch.val[1]) > q.prop_sig * ch.err[1];
+  const bool sig_a = fabs(ch.val[3]) > q.prop_sig * ch.err[3];
+  const bool sig_j = fabs(ch.val[2]) > q.prop_sig * ch.err[2];
+  const bool snap = sig_s && sig_a && (-ch.val[1]) * ch.val[3] > 0.0;
+  L.s1_parent[s] = (int32_t)par;
+  L.s1_expand[s] = (!snap && sig_j && L.cr_num[par] > 0) ? 1 : 0;
+}
+
+__global__ void circ_expand_kernel(LevelConst k, CircConst q, const double* __restrict__ leaves,
+                                   pyl_circ_level L) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= L.n_s1) return;
+  const int64_t par = L.s1_parent[s];
+  const bool expand = L.s1_expand[s] != 0;
+  const int32_t nout = expand ? L.cr_num[par] : 1;
+  const int64_t pos = L.s1_pos[s];
+  for (int32_t sub = 0; sub < nout; ++sub) {
+    const int64_t c = pos + sub;
+    ChildLeaf ch;
+    circ_decode(k, leaves, L, par, s, expand ? sub : -1, ch);
+    L.c_s1[c] = s;
+    L.c_sub[c] = expand ? sub : -1;
+    L.c_keep[c] = circ_valid(q, ch.val, ch.err) ? 1 : 0;
+    L.c_region[c] = circ_region(ch.val, ch.err, q.prop_sig);
+  }
+}
+
+__global__ void __launch_bounds__(kBlk)
+circ_children_kernel(LevelConst k, CircConst q, const double* __restrict__ leaves,
+                     const float* __restrict__ folds, pyl_circ_level L,
+                     const int64_t* __restrict__ valid, int64_t n_valid,
+                     const float* __restrict__ seg, int32_t* __restrict__ c_parent,
+                     int32_t* __restrict__ c_row, int32_t* __restrict__ c_shift,
+                     double* __restrict__ c_phase, float* __restrict__ c_score,
+                     float* __restrict__ minmax) {
+  extern __shared__ float S[];  // [(B + wmax)][kStride]
+  __shared__ int32_t s_parent[kBlk], s_row[kBlk], s_shift[kBlk];
+  const int tid = threadIdx.x;
+  const int64_t v0 = (int64_t)blockIdx.x * kBlk;
+  const int nvalid = (int)std::min<int64_t>(kBlk, n_valid - v0);
+  const int B = k.nbins;
+  if (tid < nvalid) {
+    const int64_t v = v0 + tid;
+    const int64_t c = valid[v];
+    const int64_t s1 = L.c_s1[c];
+    const int64_t par = L.s1_parent[s1];
+    ChildLeaf ch;
+    circ_decode(k, leaves, L, par, s1, L.c_sub[c], ch);
+    const int8_t reg = L.c_region[c];
+    double da[kN], di[kN];
+    region_shift(reg, ch.val, q.dt_add, k.tadd, da);
+    region_shift(reg, ch.val, q.dt_init, k.tinit, di);
+    const double accel = da[kN - 3];
+    const double vel = da[kN - 2] - di[kN - 2];
+    const double freq = ch.f0 * (1.0 - vel / pyl::kCVal);
+    const double delay = (da[kN - 1] - di[kN - 1]) / pyl::kCVal;
+    const double phase = pyl::phase_idx(k.dt_add_init, ch.f0, B, delay);
+    const int64_t ia = trunc_idx(accel, k.amin, k.sa, k.n_accel);
+    const int64_t ifr = trunc_idx(freq, k.fmin, k.sf, k.n_freq);
+    const int32_t row = (int32_t)(ia * k.n_freq + ifr);
+    const int32_t sh = pyl::shift_int(phase, B);
+    c_parent[v] = (int32_t)par;
+    c_row[v] = row;
+    c_shift[v] = sh;
+    c_phase[v] = phase;
+    s_parent[tid] = (int32_t)par;
+    s_row[tid] = row;
+    s_shift[tid] = sh;
+  }
+  __syncthreads();
+  for (int r = 0; r < nvalid; ++r) {
+    const float* fr = folds + (int64_t)s_parent[r] * 2 * B;
+    const float* sr = seg + (int64_t)s_row[r] * 2 * B;
+    const int sh = s_shift[r];
+    for (int b = tid; b < B; b += kBlk) {
+      int src = b - sh;
+      if (src < 0) src += B;
+      const float e = fr[b] + sr[src];
+      const float vv = fr[B + b] + sr[B + src];
+      S[b * kStride + r] = e / sqrtf(vv);
+    }
+  }
+  __syncthreads();
+  float sc = 0.0f;
+  if (tid < nvalid) {
+    sc = score_column(k, S, tid);
+    c_score[v0 + tid] = sc;
+  }
+  block_minmax(sc, tid < nvalid, minmax);
+}
+
+__global__ void __launch_bounds__(kBlk)
+circ_materialize_kernel(LevelConst k, CircConst q, const double* __restrict__ leaves,
+                        const float* __restrict__ folds, pyl_circ_level L,
+                        const int64_t* __restrict__ valid, const float* __restrict__ seg,
+                        const int32_t* __restrict__ c_parent, const int32_t* __restrict__ c_row,
+                        const int32_t* __restrict__ c_shift, const double* __restrict__ c_phase,
+                        const float* __restrict__ c_score,
+                        const uint8_t* __restrict__ has_taylor, const int64_t* __restrict__ surv,
+                        int64_t n_surv, double* __restrict__ o_leaves,
+                        float* __restrict__ o_folds, float* __restrict__ o_scores,
+                        int32_t* __restrict__ o_bt) {
+  __shared__ int64_t s_v[kBlk];
+  const int tid = threadIdx.x;
+  const int64_t s0 = (int64_t)blockIdx.x * kBlk;
+  const int nvalid = (int)std::min<int64_t>(kBlk, n_surv - s0);
+  const int B = k.nbins;
+  if (tid < nvalid) {
+    const int64_t s = s0 + tid;
+    const int64_t v = surv[s];
+    s_v[tid] = v;
+    const int64_t c = valid[v];
+    const int64_t s1 = L.c_s1[c];
+    const int64_t par = c_parent[v];
+    ChildLeaf ch;
+    circ_decode(k, leaves, L, par, s1, L.c_sub[c], ch);
+    const int8_t reg = L.c_region[c];
+    double* ol = o_leaves + s * kRows * 2;
+    if (reg == PYL_REGION_TAYLOR) {
+      taylor_full(k, k.tnext, ch, ol);  // transforms.py:93-137
+    } else {
+      // transforms.py:384-432: values moved along the orbit, errors unchanged.
+      double o[kN];
+      circ_shift(ch.val, q.dt_next, reg == PYL_REGION_CRACKLE, o);
+      for (int i = 0; i < kN; ++i) {
+        ol[2 * i] = o[i];
+        ol[2 * i + 1] = ch.err[i];
+      }
+    }
+    // circular.py:478-511: snap -> 1, crackle -> 2, crackle reset to 0 when the batch
+    // holds a Taylor cell; Taylor cells keep the parent's flag.
+    double flag = ch.flag;
+    if (reg == PYL_REGION_SNAP) flag = 1.0;
+    else if (reg == PYL_REGION_CRACKLE) flag = has_taylor[par / q.batch_size] ? 0.0 : 2.0;
+    ol[2 * (kP + 1)] = ch.f0;
+    ol[2 * (kP + 1) + 1] = flag;
+    o_scores[s] = c_score[v];
+    int32_t* bt = o_bt + s * kRows;
+    bt[0] = (int32_t)par;
+    const int32_t row = c_row[v];
+    for (int qq = 1; qq <= kP; ++qq) bt[qq] = 0;
+    bt[kP - 1] = row / k.n_freq;
+    bt[kP] = row % k.n_freq;
+    bt[kP + 1] = (int32_t)c_phase[v];
+  }
+  __syncthreads();
+  for (int r = 0; r < nvalid; ++r) {
+    const int64_t v = s_v[r];
+    const float* fr = folds + (int64_t)c_parent[v] * 2 * B;
+    const float* sr = seg + (int64_t)c_row[v] * 2 * B;
+    const int sh = c_shift[v];
+    float* of = o_folds + (s0 + r) * 2 * B;
+    for (int b = tid; b < B; b += kBlk) {
+      int src = b - sh;
+      if (src < 0) src += B;
+      of[b] = fr[b] + sr[src];
+      of[B + b] = fr[B + b] + sr[B + src];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBlk)
+circ_ascend_kernel(LevelConst k, CircConst q, const double* __restrict__ leaves, int64_t n,
+                   const float* __restrict__ dyp, int64_t ncoords,
+                   const int64_t* __restrict__ segi, const double* __restrict__ segt,
+                   int32_t nseg, double t0_cur, int32_t* __restrict__ scratch_row,
+                   int32_t* __restrict__ scratch_shift, float* __restrict__ o_folds,
+                   float* __restrict__ o_scores) {
+  extern __shared__ float S[];
+  const int tid = threadIdx.x;
+  const int64_t s0 = (int64_t)blockIdx.x * kBlk;
+  const int nvalid = (int)std::min<int64_t>(kBlk, n - s0);
+  const int B = k.nbins;
+  // circular.py:605-666: every segment so far, with the survivor's region propagator.
+  if (tid < nvalid) {
+    const double* lf = leaves + (s0 + tid) * kRows * 2;
+    double v[kN], e[kN];
+    for (int i = 0; i < kN; ++i) {
+      v[i] = lf[2 * i];
+      e[i] = lf[2 * i + 1];
+    }
+    const double f0 = lf[(kP + 1) * 2];
+    const int8_t reg = circ_region(v, e, q.prop_sig);
+    for (int s = 0; s < nseg; ++s) {
+      const double dt = segt[s] - t0_cur;
+      double T[kN * kN];
+      for (int i = 0; i < kN; ++i)
+        for (int j = 0; j < kN; ++j)
+          T[i * kN + j] = j <= i ? pyl::powi(dt, i - j) / pyl::fact_d(i - j) : 0.0;
+      double d[kN];
+      region_shift(reg, v, dt, T, d);
+      const double freq = f0 * (1.0 - d[kN - 2] / pyl::kCVal);
+      const double delay = d[kN - 1] / pyl::kCVal;
+      const double phase = pyl::phase_idx(dt, f0, B, delay);
+      const int64_t ia = trunc_idx(d[kN - 3], k.amin, k.sa, k.n_accel);
+      const int64_t ifr = trunc_idx(freq, k.fmin, k.sf, k.n_freq);
+      scratch_row[(s0 + tid) * nseg + s] = (int32_t)(ia * k.n_freq + ifr);
+      scratch_shift[(s0 + tid) * nseg + s] = pyl::shift_int(phase, B);
+    }
+  }
+  __syncthreads();
+  // common.py:432-459: accumulate the rotated rows in segment order, from zero.
+  for (int r = 0; r < nvalid; ++r) {
+    const int64_t sv = s0 + r;
+    float* of = o_folds + sv * 2 * B;
+    for (int b = tid; b < B; b += kBlk) {
+      float ae = 0.0f, av = 0.0f;
+      for (int s = 0; s < nseg; ++s) {
+        const float* rowp =
+            dyp + ((int64_t)segi[s] * ncoords + scratch_row[sv * nseg + s]) * 2 * B;
+        int src = b - scratch_shift[sv * nseg + s];
+        if (src < 0) src += B;
+        ae = ae + rowp[src];
+        av = av + rowp[B + src];
+      }
+      of[b] = ae;
+      of[B + b] = av;
+      S[b * kStride + r] = ae / sqrtf(av);
+    }
+  }
+  __syncthreads();
+  if (tid < nvalid) o_scores[s0 + tid] = score_column(k, S, tid);
+}
+
+size_t scorer_lds(const LevelConst& k) {
+  return sizeof(float) * (size_t)(k.nbins + k.sc.wmax) * kStride;
+}
+
+}  // namespace
+
+extern "C" int pyl_circ_branch(const pyl_prune_params* p, const pyl_circ_params* q,
+                               const pyl_prune_coords* c, const double* leaves,
+                               const pyl_circ_level* L, int64_t* nchild1, int64_t* err,
+                               void* stream) {
+  PYL_REQUIRE(c != nullptr && L != nullptr, "null coords or level");
+  LevelConst k;
+  CircConst cc;
+  int rc = build_circ(p, q, c, &k, &cc);
+  if (rc != PYL_OK) return rc;
+  PYL_REQUIRE(leaves && L->branch && L->counts && L->cr_first && L->cr_num && nchild1 && err,
+              "null device pointer");
+  if (L->n_par == 0) return PYL_OK;
+  const int threads = 128;
+  circ_branch_kernel<<<(unsigned)pyl::ceil_div(L->n_par, threads), threads, 0,
+                       pyl::as_stream(stream)>>>(k, cc, leaves, *L, nchild1,
+                                                 reinterpret_cast<unsigned long long*>(err));
+  PYL_CHECK_LAUNCH("circ_branch_kernel");
+  return PYL_OK;
+}
+
+extern "C" int pyl_circ_stage1(const pyl_prune_params* p, const pyl_circ_params* q,
+                               const pyl_prune_coords* c, const double* leaves,
+                               const pyl_circ_level* L, void* stream) {
+  PYL_REQUIRE(c != nullptr && L != nullptr, "null coords or level");
+  LevelConst k;
+  CircConst cc;
+  int rc = build_circ(p, q, c, &k, &cc);
+  if (rc != PYL_OK) return rc;
+  if (L->n_s1 == 0) return PYL_OK;
+  PYL_REQUIRE(leaves && L->branch && L->counts && L->cr_num && L->off1 && L->s1_parent &&
+                  L->s1_expand,
+              "null device pointer");
+  const int threads = 128;
+  circ_stage1_kernel<<<(unsigned)pyl::ceil_div(L->n_s1, threads), threads, 0,
+                       pyl::as_stream(stream)>>>(k, cc, leaves, *L);
+  PYL_CHECK_LAUNCH("circ_stage1_kernel");
+  return PYL_OK;
+}
+
+extern "C" int pyl_circ_expand(const pyl_prune_params* p, const pyl_circ_params* q,
+                               const pyl_prune_coords* c, const double* leaves,
+                               const pyl_circ_level* L, void* stream) {
+  PYL_REQUIRE(c != nullptr && L != nullptr, "null coords or level");
+  LevelConst k;
+  CircConst cc;
+  int rc = build_circ(p, q, c, &k, &cc);
+  if (rc != PYL_OK) return rc;
+  if (L->n_s1 == 0) return PYL_OK;
+  PYL_REQUIRE(leaves && L->branch && L->counts && L->cr_first && L->cr_num && L->off1 &&
+                  L->s1_parent && L->s1_expand && L->s1_pos && L->c_s1 && L->c_sub &&
+                  L->c_keep && L->c_region,
+              "null device pointer");
+  const int threads = 128;
+  circ_expand_kernel<<<(unsigned)pyl::ceil_div(L->n_s1, threads), threads, 0,
+                       pyl::as_stream(stream)>>>(k, cc, leaves, *L);
+  PYL_CHECK_LAUNCH("circ_expand_kernel");
+  return PYL_OK;
+}
+
+extern "C" int pyl_circ_children(const pyl_prune_params* p, const pyl_circ_params* q,
+                                 const pyl_prune_coords* c, const double* leaves,
+                                 const float* folds, const pyl_circ_level* L,
+                                 const int64_t* valid, int64_t n_valid, const float* seg,
+                                 int32_t* c_parent, int32_t* c_row, int32_t* c_shift,
+                                 double* c_phase, float* c_score, float* minmax, void* stream) {
+  PYL_REQUIRE(c != nullptr && L != nullptr, "null coords or level");
+  LevelConst k;
+  CircConst cc;
+  int rc = build_circ(p, q, c, &k, &cc);
+  if (rc != PYL_OK) return rc;
+  PYL_REQUIRE((int64_t)p->n_accel * p->n_freq <= 0x7fffffff, "FFA grid too large");
+  if (n_valid == 0) return PYL_OK;
+  PYL_REQUIRE(leaves && folds && valid && seg && c_parent && c_row && c_shift && c_phase &&
+                  c_score && minmax && L->c_s1 && L->c_sub && L->c_region && L->s1_parent,
+              "null device pointer");
+  const size_t lds = scorer_lds(k);
+  PYL_REQUIRE(lds <= 64 * 1024, "nbins too large for the EP scorer (%d)", k.nbins);
+  circ_children_kernel<<<(unsigned)pyl::ceil_div(n_valid, kBlk), kBlk, lds,
+                         pyl::as_stream(stream)>>>(k, cc, leaves, folds, *L, valid, n_valid,
+                                                   seg, c_parent, c_row, c_shift, c_phase,
+                                                   c_score, minmax);
+  PYL_CHECK_LAUNCH("circ_children_kernel");
+  return PYL_OK;
+}
+
+extern "C" int pyl_circ_materialize(const pyl_prune_params* p, const pyl_circ_params* q,
+                                    const pyl_prune_coords* c, const double* leaves,
+                                    const float* folds, const pyl_circ_level* L,
+                                    const int64_t* valid, const float* seg,
+                                    const int32_t* c_parent, const int32_t* c_row,
+                                    const int32_t* c_shift, const double* c_phase,
+                                    const float* c_score, const uint8_t* has_taylor,
+                                    const int64_t* surv, int64_t n_surv, double* o_leaves,
+                                    float* o_folds, float* o_scores, int32_t* o_bt,
+                                    void* stream) {
+  PYL_REQUIRE(c != nullptr && L != nullptr, "null coords or level");
+  LevelConst k;
+  CircConst cc;
+  int rc = build_circ(p, q, c, &k, &cc);
+  if (rc != PYL_OK) return rc;
+  if (n_surv == 0) return PYL_OK;
+  PYL_REQUIRE(leaves && folds && valid && seg && c_parent && c_row && c_shift && c_phase &&
+                  c_score && has_taylor && surv && o_leaves && o_folds && o_scores && o_bt,
+              "null device pointer");
+  circ_materialize_kernel<<<(unsigned)pyl::ceil_div(n_surv, kBlk), kBlk, 0,
+                            pyl::as_stream(stream)>>>(k, cc, leaves, folds, *L, valid, seg,
+                                                      c_parent, c_row, c_shift, c_phase,
+                                                      c_score, has_taylor, surv, n_surv,
+                                                      o_leaves, o_folds, o_scores, o_bt);
+  PYL_CHECK_LAUNCH("circ_materialize_kernel");
+  return PYL_OK;
+}
+
+extern "C" int pyl_circ_ascend(const pyl_prune_params* p, const pyl_circ_params* q,
+                               const double* leaves, int64_t n, const float* dyp,
+                               int64_t ncoords, const int64_t* seg_idx, const double* seg_t0,
+                               int32_t nseg, double t0_cur, float* o_folds, float* o_scores,
+                               void* stream) {
+  LevelConst k;
+  CircConst cc;
+  int rc = build_circ(p, q, nullptr, &k, &cc);
+  if (rc != PYL_OK) return rc;
+  PYL_REQUIRE(nseg >= 1, "no segments to integrate");
+  PYL_REQUIRE(leaves && dyp && seg_idx && seg_t0 && o_folds && o_scores, "null pointer");
+  if (n == 0) return PYL_OK;
+  hipStream_t s = pyl::as_stream(stream);
+  int64_t* d_idx = nullptr;
+  double* d_t0 = nullptr;
+  int32_t* d_row = nullptr;
+  int32_t* d_sh = nullptr;
+  PYL_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&d_idx), sizeof(int64_t) * nseg, s));
+  PYL_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&d_t0), sizeof(double) * nseg, s));
+  PYL_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&d_row), sizeof(int32_t) * n * nseg, s));
+  PYL_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&d_sh), sizeof(int32_t) * n * nseg, s));
+  PYL_CHECK_HIP(hipMemcpyAsync(d_idx, seg_idx, sizeof(int64_t) * nseg, hipMemcpyHostToDevice, s));
+  PYL_CHECK_HIP(hipMemcpyAsync(d_t0, seg_t0, sizeof(double) * nseg, hipMemcpyHostToDevice, s));
+  circ_ascend_kernel<<<(unsigned)pyl::ceil_div(n, kBlk), kBlk, scorer_lds(k), s>>>(
+      k, cc, leaves, n, dyp, ncoords, d_idx, d_t0, nseg, t0_cur, d_row, d_sh, o_folds,
+      o_scores);
+  PYL_CHECK_LAUNCH("circ_ascend_kernel");
+  PYL_CHECK_HIP(hipFreeAsync(d_idx, s));
+  PYL_CHECK_HIP(hipFreeAsync(d_t0, s));
+  PYL_CHECK_HIP(hipFreeAsync(d_row, s));
+  PYL_CHECK_HIP(hipFreeAsync(d_sh, s));
+  return PYL_OK;
+}

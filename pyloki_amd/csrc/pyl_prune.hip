@@ -20,113 +20,11 @@
 // Taylor translation matrices T[i][j] = dt**(i-j)/(i-j)! are per-level constants; they are
 // built on the host with the binary-powering power (powi) the reference's numba code uses,
 // and the GPU only does the per-leaf FP64 matrix-vector products, in ascending column order.
-#include <algorithm>
-#include <cfloat>
-#include <climits>
-#include <cmath>
-
-#include "pyl_score.hpp"
+#include "pyl_prune_common.hpp"
 
 namespace {
 
-constexpr int kMaxP = 5;
-constexpr int kMaxN = kMaxP + 1;
-constexpr int kBlk = pyl::kScoreBlock;
-constexpr int kStride = pyl::kScoreStride;
-
-struct LevelConst {
-  int32_t P, nbins, branch_max, tiling;
-  int32_t nparams_ffa, n_accel, n_freq;
-  double eta;
-  double dpf[kMaxP];          // poly_taylor_step_f(P, scale_cur) row order d_P..d_1
-  double fac[kMaxP];          // shift factors, same order
-  double tadd[kMaxN * kMaxN];
-  double tinit[kMaxN * kMaxN];
-  double tnext[kMaxN * kMaxN];
-  double dt_add_init;         // t0_add - t0_init
-  double amin, fmin, sa, sf;  // accel/freq grid origin and inverse step
-  pyl::ScoreConst sc;         // boxcar widths and template constants
-};
-
-int build_const(const pyl_prune_params* p, const pyl_prune_coords* c, LevelConst* k) {
-  PYL_REQUIRE(p != nullptr, "null pyl_prune_params");
-  PYL_REQUIRE(p->poly_order >= 2 && p->poly_order <= kMaxP, "poly_order must be in [2, %d]",
-              kMaxP);
-  PYL_REQUIRE(p->nbins > 1, "nbins must be > 1");
-  PYL_REQUIRE(p->nparams_ffa >= 2 && p->nparams_ffa <= p->poly_order,
-              "FFA grid must have 2..poly_order parameters");
-  PYL_REQUIRE(p->n_accel >= 1 && p->n_freq >= 1, "empty FFA grid");
-  PYL_REQUIRE(p->tiling >= 0 && p->tiling <= 2, "invalid tiling strategy");
-  *k = LevelConst{};
-  const int P = p->poly_order;
-  k->P = P;
-  k->nbins = p->nbins;
-  k->branch_max = p->branch_max;
-  k->tiling = p->tiling;
-  k->nparams_ffa = p->nparams_ffa;
-  k->n_accel = p->n_accel;
-  k->n_freq = p->n_freq;
-  k->eta = p->eta;
-  if (c) {
-    // psr_utils.py:59-96 (t_ref = 0, Chebyshev coarsening), reversed to d_P..d_1.
-    const double dphi = p->eta / p->nbins;
-    for (int kk = 0; kk < P; ++kk) {
-      const double v = dphi * pyl::fact_d(kk + 1) / pyl::powi(c->scale_cur, kk + 1);
-      k->dpf[P - 1 - kk] = (double)(1LL << kk) * v;
-    }
-    // psr_utils.py:227-248: (tobs)^(k+1) * nbins / (k+1)! / 2^k, k = P-1..0.
-    for (int idx = 0; idx < P; ++idx) {
-      const int kk = P - 1 - idx;
-      const double f = pyl::powi(c->scale_cur, kk + 1) * p->nbins / pyl::fact_d(kk + 1);
-      k->fac[idx] = f / (double)(1LL << kk);
-    }
-    pyl::taylor_matrix(P + 1, c->t0_add - c->t0_cur, k->tadd);
-    pyl::taylor_matrix(P + 1, c->t0_init - c->t0_cur, k->tinit);
-    pyl::taylor_matrix(P + 1, c->t0_next - c->t0_cur, k->tnext);
-    k->dt_add_init = c->t0_add - c->t0_init;
-  }
-  k->amin = p->accel_lim[0];
-  k->fmin = p->freq_lim[0];
-  k->sa = p->accel_lim[1] > p->accel_lim[0] ? p->n_accel / (p->accel_lim[1] - p->accel_lim[0])
-                                             : 0.0;
-  k->sf = p->freq_lim[1] > p->freq_lim[0] ? p->n_freq / (p->freq_lim[1] - p->freq_lim[0]) : 0.0;
-  int64_t widths[PYL_MAX_WIDTHS];
-  PYL_REQUIRE(p->nw >= 1 && p->nw <= PYL_MAX_WIDTHS, "number of widths must be in [1, %d]",
-              PYL_MAX_WIDTHS);
-  for (int w = 0; w < p->nw; ++w) widths[w] = p->widths[w];
-  int rc = pyl::build_score_const(p->nbins, widths, p->nw, 1.0, &k->sc);
-  if (rc != PYL_OK) return rc;
-  return PYL_OK;
-}
-
-// psr_utils.py:517-554: (accel, freq) grid indices by truncation with a 1e-6 guard.
-__device__ __forceinline__ int64_t trunc_idx(double v, double vmin, double sinv, int n) {
-  if (!(sinv > 0.0)) return 0;
-  int64_t i = (int64_t)((v - vmin) * sinv + pyl::kFloatEps);
-  if (i < 0) i = 0;
-  else if (i >= n) i = n - 1;
-  return i;
-}
-
-__device__ void atomic_min_f32(float* addr, float v) {
-  int* ia = reinterpret_cast<int*>(addr);
-  int old = __atomic_load_n(ia, __ATOMIC_RELAXED);
-  while (__int_as_float(old) > v) {
-    int assumed = old;
-    old = atomicCAS(ia, assumed, __float_as_int(v));
-    if (old == assumed) break;
-  }
-}
-
-__device__ void atomic_max_f32(float* addr, float v) {
-  int* ia = reinterpret_cast<int*>(addr);
-  int old = __atomic_load_n(ia, __ATOMIC_RELAXED);
-  while (__int_as_float(old) < v) {
-    int assumed = old;
-    old = atomicCAS(ia, assumed, __float_as_int(v));
-    if (old == assumed) break;
-  }
-}
+using namespace pyl_ep;
 
 __global__ void prune_branch_kernel(LevelConst k, const double* __restrict__ leaves, int64_t n,
                                     double* __restrict__ branch, int32_t* __restrict__ counts,
@@ -170,56 +68,6 @@ __global__ void prune_branch_kernel(LevelConst k, const double* __restrict__ lea
     total *= cnt < 0 ? 1 : cnt;
   }
   nchild[i] = total;
-}
-
-struct ChildLeaf {
-  double val[kMaxN];  // d_P..d_1, d_0
-  double err[kMaxN];
-  double f0, flag;
-};
-
-// Parent of child c: the last p with off[p] <= c.
-__device__ __forceinline__ int64_t find_parent(const int64_t* __restrict__ off, int64_t n_par,
-                                               int64_t c) {
-  int64_t lo = 0, hi = n_par;  // off[lo] <= c < off[hi]
-  while (hi - lo > 1) {
-    const int64_t mid = (lo + hi) >> 1;
-    if (off[mid] <= c) lo = mid;
-    else hi = mid;
-  }
-  return lo;
-}
-
-__device__ __forceinline__ void decode_child(const LevelConst& k, const double* __restrict__ lf,
-                                             const double* __restrict__ br,
-                                             const int32_t* __restrict__ cn, int64_t local,
-                                             ChildLeaf& out) {
-  const int P = k.P;
-  for (int j = P - 1; j >= 0; --j) {
-    const int32_t cnt = cn[j];
-    const int64_t n = cnt < 0 ? 1 : cnt;
-    const int64_t ij = local % n;
-    local /= n;
-    out.val[j] = cnt < 0 ? br[2 * j] : br[2 * j] + (double)ij * br[2 * j + 1];
-    out.err[j] = br[2 * j + 1];
-  }
-  out.val[P] = lf[P * 2];  // d_0 copied from the parent; its error starts at 0
-  out.err[P] = 0.0;
-  out.f0 = lf[(P + 1) * 2];
-  out.flag = lf[(P + 1) * 2 + 1];
-}
-
-__device__ __forceinline__ double mat_row(const double* __restrict__ T, const double* v, int n,
-                                          int i) {
-  double acc = 0.0;
-  for (int j = 0; j < n; ++j) acc += v[j] * T[i * n + j];
-  return acc;
-}
-
-// Sequential scan + boxcar S/N (max over widths) of the profile in column `tid` of S
-// (scoring.py:249-282).
-__device__ __forceinline__ float score_column(const LevelConst& k, float* S, int tid) {
-  return (float)pyl::score_column<false>(k.sc, k.nbins, S, tid, nullptr);
 }
 
 __global__ void __launch_bounds__(kBlk)
@@ -287,17 +135,8 @@ prune_children_kernel(LevelConst k, const double* __restrict__ leaves,
     sc = score_column(k, S, tid);
     c_score[c0 + tid] = sc;
   }
-  // Block min/max of the scores, then one atomic per block.
-  float mn = tid < nvalid ? sc : FLT_MAX;
-  float mx = tid < nvalid ? sc : -FLT_MAX;
-  for (int o = 32; o > 0; o >>= 1) {
-    mn = fminf(mn, __shfl_xor(mn, o));
-    mx = fmaxf(mx, __shfl_xor(mx, o));
-  }
-  if (tid == 0 && nvalid > 0) {
-    atomic_min_f32(&minmax[0], mn);
-    atomic_max_f32(&minmax[1], mx);
-  }
+  // Block min/max of the scores, then one atomic per wavefront.
+  block_minmax(sc, tid < nvalid, minmax);
 }
 
 __global__ void __launch_bounds__(kBlk)
@@ -325,27 +164,9 @@ prune_materialize_kernel(LevelConst k, const double* __restrict__ leaves,
     const double* lf = leaves + par * (P + 2) * 2;
     ChildLeaf ch;
     decode_child(k, lf, branch + par * P * 2, counts + par * P, c - off[par], ch);
-    const int n = P + 1;
     double* ol = o_leaves + s * (P + 2) * 2;
     // transforms.py:93-137 shift_taylor_full to coord_next.
-    for (int i = 0; i < n; ++i) {
-      ol[2 * i] = mat_row(k.tnext, ch.val, n, i);
-      double e;
-      if (k.tiling == PYL_TILING_AGGRESSIVE) {
-        e = ch.err[i] * fabs(k.tnext[i * n + i]);
-      } else if (k.tiling == PYL_TILING_CONSERVATIVE) {
-        e = 0.0;
-        for (int j = 0; j < n; ++j) e += ch.err[j] * fabs(k.tnext[i * n + j]);
-      } else {
-        e = 0.0;
-        for (int j = 0; j < n; ++j) {
-          const double t = k.tnext[i * n + j];
-          e += (ch.err[j] * ch.err[j]) * (t * t);
-        }
-        e = sqrt(e);
-      }
-      ol[2 * i + 1] = e;
-    }
+    taylor_full(k, k.tnext, ch, ol);
     ol[2 * (P + 1)] = ch.f0;
     ol[2 * (P + 1) + 1] = ch.flag;
     o_scores[s] = c_score[c];

@@ -59,6 +59,26 @@ class PruneCoords(ctypes.Structure):
     ]
 
 
+class CircParams(ctypes.Structure):
+    _fields_ = [
+        ("prop_sig", ctypes.c_double),
+        ("val_sig", ctypes.c_double),
+        ("omega_max_sq", ctypes.c_double),
+        ("x_mass_const", ctypes.c_double),
+        ("batch_size", ctypes.c_int64),
+    ]
+
+
+class CircLevel(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_void_p) for name in (
+        "branch", "counts", "cr_first", "cr_num", "off1", "s1_parent", "s1_expand", "s1_pos",
+        "c_s1", "c_sub", "c_keep", "c_region")] + [
+        ("n_par", ctypes.c_int64), ("n_s1", ctypes.c_int64), ("n_child", ctypes.c_int64)]
+
+
+REGION_TAYLOR = 2
+
+
 @dataclass
 class PruneStats:
     """Per-level statistics (``io/cands.py:16-101``)."""
@@ -127,6 +147,37 @@ class DeviceTree:
         return float(self.scores.min()) if self.size else 0.0
 
 
+
+def circ_child_positions(off1: torch.Tensor, s1_parent: torch.Tensor, s1_expand: torch.Tensor,
+                         cr_num: torch.Tensor, batch_size: int) -> tuple[torch.Tensor, int, int]:
+    """Final position of every held child's first output (``circular.py:359-394``).
+
+    Per batch of ``batch_size`` parents the children are the held children that do not
+    expand, in held order, followed by the ``cr_num[parent]`` crackle children of every
+    expanding held child, in held order. Returns (positions, total children, batches)."""
+    n = int(off1.shape[0]) - 1
+    dev = off1.device
+    bs = max(1, min(batch_size, n))
+    expand = s1_expand.bool()
+    par = s1_parent.long()
+    n_s1 = int(par.shape[0])
+    nout = torch.where(expand, cr_num.long()[par], torch.ones_like(par))
+    ck0 = torch.zeros(n_s1 + 1, dtype=torch.int64, device=dev)
+    torch.cumsum((~expand).long(), 0, out=ck0[1:])
+    ce0 = torch.zeros(n_s1 + 1, dtype=torch.int64, device=dev)
+    torch.cumsum(torch.where(expand, nout, torch.zeros_like(nout)), 0, out=ce0[1:])
+    bfirst = torch.arange(0, n, bs, device=dev)
+    bstart = off1[bfirst]
+    bend = off1[torch.clamp(bfirst + bs, max=n)]
+    nk_b = ck0[bend] - ck0[bstart]
+    tot_b = nk_b + (ce0[bend] - ce0[bstart])
+    base_b = torch.zeros_like(tot_b)
+    base_b[1:] = torch.cumsum(tot_b, 0)[:-1]
+    b_of = par // bs
+    pos = base_b[b_of] + torch.where(expand, nk_b[b_of] + ce0[:-1] - ce0[bstart][b_of],
+                                     ck0[:-1] - ck0[bstart][b_of])
+    return pos, int(tot_b.sum().item()), int(bfirst.shape[0])
+
 def overflow_survivors(scores: np.ndarray, child_off: np.ndarray, batch_size: int,
                        threshold: float, max_sugg: int) -> np.ndarray:
     """Children kept by the reference's batch-sequential loop when the buffer overflows.
@@ -183,8 +234,9 @@ class Pruning:
                  batch_size: int = 1024, poly_basis: str = "taylor", intermediate_k: int = 1,
                  *, use_moving_grid: bool = True, logger=None) -> None:  # noqa: ARG002
         cfg = dyp.cfg
-        if poly_basis != "taylor" or cfg.prune_poly_order == 5:
-            msg = f"Invalid pruning configuration: {(cfg.prune_poly_order == 5, poly_basis, cfg.use_fourier)}"
+        self._circular = cfg.prune_poly_order == 5  # prune.py:369-392: circular plugin
+        if poly_basis != "taylor":
+            msg = f"Invalid pruning configuration: {(self._circular, poly_basis, cfg.use_fourier)}"
             raise ValueError(msg)
         if cfg.use_fourier:
             msg = "complex (use_fourier=True) pruning is not supported yet"
@@ -219,6 +271,15 @@ class Pruning:
         for i, w in enumerate(widths.tolist()):
             pp.widths[i] = w
         self._pp = pp
+        self._qq = None
+        if self._circular:
+            qq = CircParams()
+            qq.prop_sig = float(cfg.propagator_significance)
+            qq.val_sig = float(cfg.validation_significance)
+            qq.omega_max_sq = float((2 * np.pi / cfg.p_orb_min) ** 2)
+            qq.x_mass_const = float(cfg.x_mass_const)
+            qq.batch_size = self._batch_size
+            self._qq = qq
         self._widths = widths
         self._ncoords = int(np.prod(counts))
 
@@ -333,8 +394,22 @@ class Pruning:
             return
         self._backtrack_hist.append(new_tree.backtracks.clone())
 
+    def _survivors(self, c_score: torch.Tensor, child_off_fn, threshold: float) -> torch.Tensor:
+        """Children kept: all passing ones when they fit the buffer, else the reference's
+        batch-sequential overflow rule replayed on the scores (``child_off_fn()`` gives the
+        per-parent child offsets that mark the batch boundaries)."""
+        passing = c_score.double() >= threshold
+        npass = int(passing.sum().item())
+        if npass <= self.max_sugg:
+            return torch.nonzero(passing).reshape(-1)
+        surv_np = overflow_survivors(c_score.cpu().numpy(), child_off_fn(), self.batch_size,
+                                     threshold, self.max_sugg)
+        return torch.from_numpy(surv_np).to(c_score.device)
+
     def _level(self, tree: DeviceTree, seg: torch.Tensor, coords: PruneCoords,
                threshold: float) -> tuple[DeviceTree, dict]:
+        if self._circular:
+            return self._level_circ(tree, seg, coords, threshold)
         lib = _lib.load()
         dev = tree.folds.device
         P = self._P
@@ -367,14 +442,7 @@ class Pruning:
             pp, cc, ptr(tree.leaves), ptr(tree.folds), n, ptr(branch), ptr(counts), ptr(off),
             total, ptr(segc), ptr(c_parent), ptr(c_row), ptr(c_shift), ptr(c_phase),
             ptr(c_score), ptr(minmax), stream_ptr()))
-        passing = c_score.double() >= threshold
-        npass = int(passing.sum().item())
-        if npass <= self.max_sugg:
-            surv = torch.nonzero(passing).reshape(-1)
-        else:
-            surv_np = overflow_survivors(c_score.cpu().numpy(), off.cpu().numpy(),
-                                         self.batch_size, threshold, self.max_sugg)
-            surv = torch.from_numpy(surv_np).to(dev)
+        surv = self._survivors(c_score, lambda: off.cpu().numpy(), threshold)
         m = int(surv.shape[0])
         o_leaves = torch.empty((m, P + 2, 2), dtype=torch.float64, device=dev)
         o_folds = torch.empty((m, 2, B), dtype=torch.float32, device=dev)
@@ -391,6 +459,97 @@ class Pruning:
                  "score_max": float(mm[1]) if total else 0.0}
         return DeviceTree(o_leaves, o_folds, o_scores, o_bt), stats
 
+    def _level_circ(self, tree: DeviceTree, seg: torch.Tensor, coords: PruneCoords,
+                    threshold: float) -> tuple[DeviceTree, dict]:
+        """One circular-basis level (``circular.py`` via ``prune.py:165-332``)."""
+        lib = _lib.load()
+        dev = tree.folds.device
+        P = self._P
+        n = tree.size
+        B = self.dyp.cfg.nbins
+        bs = max(1, min(self.batch_size, n))
+        pp, qq, cc = ctypes.byref(self._pp), ctypes.byref(self._qq), ctypes.byref(coords)
+        st = stream_ptr()
+        i64, i32 = torch.int64, torch.int32
+        buf = {
+            "branch": torch.empty((n, P, 2), dtype=torch.float64, device=dev),
+            "counts": torch.empty((n, P), dtype=i32, device=dev),
+            "cr_first": torch.empty(n, dtype=torch.float64, device=dev),
+            "cr_num": torch.empty(n, dtype=i32, device=dev),
+        }
+        L = CircLevel()
+        for name, t in buf.items():
+            setattr(L, name, ptr(t))
+        L.n_par = n
+        nchild1 = torch.empty(n, dtype=i64, device=dev)
+        err = torch.full((1,), INT64_MAX, dtype=i64, device=dev)
+        _lib.check(lib.pyl_circ_branch(pp, qq, cc, ptr(tree.leaves), ctypes.byref(L),
+                                       ptr(nchild1), ptr(err), st))
+        off1 = torch.zeros(n + 1, dtype=i64, device=dev)
+        torch.cumsum(nchild1, 0, out=off1[1:])
+        errv = int(err.item())
+        if errv != INT64_MAX:
+            if errv % 4 == 1:
+                raise ValueError("Invalid input: increase branch_max.")
+            raise ValueError("Both dparam_cur and dparam_new must be positive.")
+        n_s1 = int(off1[-1].item())
+        buf["off1"] = off1
+        buf["s1_parent"] = torch.empty(n_s1, dtype=i32, device=dev)
+        buf["s1_expand"] = torch.empty(n_s1, dtype=torch.uint8, device=dev)
+        for name in ("off1", "s1_parent", "s1_expand"):
+            setattr(L, name, ptr(buf[name]))
+        L.n_s1 = n_s1
+        _lib.check(lib.pyl_circ_stage1(pp, qq, cc, ptr(tree.leaves), ctypes.byref(L), st))
+        pos, total, nb = circ_child_positions(off1, buf["s1_parent"], buf["s1_expand"],
+                                              buf["cr_num"], bs)
+        buf["s1_pos"] = pos.contiguous()
+        buf["c_s1"] = torch.empty(total, dtype=i64, device=dev)
+        buf["c_sub"] = torch.empty(total, dtype=i32, device=dev)
+        buf["c_keep"] = torch.empty(total, dtype=torch.uint8, device=dev)
+        buf["c_region"] = torch.empty(total, dtype=torch.int8, device=dev)
+        for name in ("s1_pos", "c_s1", "c_sub", "c_keep", "c_region"):
+            setattr(L, name, ptr(buf[name]))
+        L.n_child = total
+        _lib.check(lib.pyl_circ_expand(pp, qq, cc, ptr(tree.leaves), ctypes.byref(L), st))
+        valid = torch.nonzero(buf["c_keep"]).reshape(-1).contiguous()
+        nv = int(valid.shape[0])
+        c_parent = torch.empty(nv, dtype=i32, device=dev)
+        c_row = torch.empty(nv, dtype=i32, device=dev)
+        c_shift = torch.empty(nv, dtype=i32, device=dev)
+        c_phase = torch.empty(nv, dtype=torch.float64, device=dev)
+        c_score = torch.empty(nv, dtype=torch.float32, device=dev)
+        minmax = torch.tensor([np.inf, -np.inf], dtype=torch.float32, device=dev)
+        segc = seg.contiguous()
+        _lib.check(lib.pyl_circ_children(
+            pp, qq, cc, ptr(tree.leaves), ptr(tree.folds), ctypes.byref(L), ptr(valid), nv,
+            ptr(segc), ptr(c_parent), ptr(c_row), ptr(c_shift), ptr(c_phase), ptr(c_score),
+            ptr(minmax), st))
+
+        def valid_off() -> np.ndarray:
+            cnt = torch.bincount(c_parent.long(), minlength=n)
+            return np.concatenate([[0], np.cumsum(cnt.cpu().numpy())])
+
+        surv = self._survivors(c_score, valid_off, threshold).contiguous()
+        m = int(surv.shape[0])
+        # Batches whose validated children include a Taylor cell (circular.py:506-511).
+        has_taylor = torch.zeros(nb, dtype=torch.uint8, device=dev)
+        tay = buf["c_region"][valid] == REGION_TAYLOR
+        has_taylor[c_parent.long()[tay] // bs] = 1
+        o_leaves = torch.empty((m, P + 2, 2), dtype=torch.float64, device=dev)
+        o_folds = torch.empty((m, 2, B), dtype=torch.float32, device=dev)
+        o_scores = torch.empty(m, dtype=torch.float32, device=dev)
+        o_bt = torch.empty((m, P + 2), dtype=i32, device=dev)
+        _lib.check(lib.pyl_circ_materialize(
+            pp, qq, cc, ptr(tree.leaves), ptr(tree.folds), ctypes.byref(L), ptr(valid),
+            ptr(segc), ptr(c_parent), ptr(c_row), ptr(c_shift), ptr(c_phase), ptr(c_score),
+            ptr(has_taylor), ptr(surv), m, ptr(o_leaves), ptr(o_folds), ptr(o_scores),
+            ptr(o_bt), st))
+        mm = minmax.cpu().numpy()
+        stats = {"n_leaves": total, "n_leaves_phy": nv,
+                 "score_min": float(mm[0]) if nv else 0.0,
+                 "score_max": float(mm[1]) if nv else 0.0}
+        return DeviceTree(o_leaves, o_folds, o_scores, o_bt), stats
+
     def _do_ascend(self) -> None:
         """Re-integrate the survivors over all segments so far (``prune.py:671-693``)."""
         if self._prune_complete or self._world_tree.size == 0:
@@ -404,11 +563,16 @@ class Pruning:
         scores = torch.empty(n, dtype=torch.float32, device=tree.folds.device)
         idx = np.ascontiguousarray(seg_idx, dtype=np.int64)
         t0 = np.ascontiguousarray(seg_coords[:, 0], dtype=np.float64)
-        _lib.check(_lib.load().pyl_prune_ascend(
-            ctypes.byref(self._pp), ptr(tree.leaves), n, ptr(self.dyp.fold_device.contiguous()),
-            self._ncoords, idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
-            t0.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(idx), t0_cur, ptr(folds),
-            ptr(scores), stream_ptr()))
+        args = (ptr(tree.leaves), n, ptr(self.dyp.fold_device.contiguous()), self._ncoords,
+                idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                t0.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(idx), t0_cur,
+                ptr(folds), ptr(scores), stream_ptr())
+        lib = _lib.load()
+        if self._circular:
+            _lib.check(lib.pyl_circ_ascend(ctypes.byref(self._pp), ctypes.byref(self._qq),
+                                           *args))
+        else:
+            _lib.check(lib.pyl_prune_ascend(ctypes.byref(self._pp), *args))
         tree.scores_ep = tree.scores.clone()
         tree.folds = folds
         tree.scores = scores

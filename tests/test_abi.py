@@ -37,8 +37,48 @@ def test_bindings_cover_declarations():
 
 def test_abi_version_and_error_path_without_gpu():
     lib = _lib.load()
-    assert lib.pyl_abi_version() == 100
+    assert lib.pyl_abi_version() == 102
     # An invalid argument is rejected on the host before anything touches the GPU.
     rc = lib.pyl_ffa_resolve(0, None, None, None, None, 1, 1.0, 64, None, None, None)
     assert rc == _lib.PYL_ERR_INVALID
     assert b"nparams" in lib.pyl_last_error()
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """The ctypes mirrors of the ABI structs have the C compiler's size and field offsets."""
+    import shutil
+    import subprocess
+
+    from pyloki_amd import prune
+
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        import pytest
+
+        pytest.skip("gcc not available")
+    structs = {"pyl_prune_params": prune.PruneParams, "pyl_prune_coords": prune.PruneCoords,
+               "pyl_circ_params": prune.CircParams, "pyl_circ_level": prune.CircLevel}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "pyloki_hip.h"',
+             "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    inc = Path(__file__).resolve().parent.parent / "include"
+    subprocess.run([gcc, "-I", str(inc), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    for line in out.splitlines():
+        cname, field, val = line.split()
+        py = structs[cname]
+        got = ctypes_size(py) if field == "size" else getattr(py, field).offset
+        assert got == int(val), (cname, field, got, val)
+
+
+def ctypes_size(t) -> int:
+    import ctypes
+
+    return ctypes.sizeof(t)

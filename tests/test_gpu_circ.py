@@ -1,0 +1,128 @@
+"""GPU parity of the circular-basis EP search against the oracle, level by level: survivor
+sets, scores, folds, backtracks and basis flags bit-exactly; leaf parameters bit-exactly for
+Taylor cells and within 1e-12 (relative to the column's scale) for cells moved along the
+circular orbit, whose propagator uses the device's FP64 cos/sin. Then ascend and report."""
+
+from __future__ import annotations
+
+import functools
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import prune_oracle as po
+from pyloki_amd.config import ParamLimits, PulsarSearchConfig
+from pyloki_amd.ffa import DynamicProgramming
+from pyloki_amd.prune import Pruning
+from pyloki_amd.simulation.pulse import PulseSignalConfig
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(msg: str) -> None:
+    print(f"[case] {msg}", flush=True)
+
+
+def _setup(nsamps=2**18, nseg=32, p_orb_min=20.0):
+    tsamp = 64e-6
+    lim = ParamLimits.from_circular((199.0, 201.0), p_orb_min=p_orb_min, m_c_max=0.2,
+                                    m_p_min=1.4).limits
+    cfg = PulsarSearchConfig(nsamps=nsamps, tsamp=tsamp, nbins=32, eta=1.0, param_limits=lim,
+                             bseg_brute=nsamps // 256, bseg_ffa=nsamps // nseg,
+                             prune_poly_order=5, p_orb_min=p_orb_min, m_c_max=0.2, m_p_min=1.4,
+                             ducy_max=0.3, wtsp=1.5, use_fourier=False, branch_max=32)
+    sim = PulseSignalConfig(period=1 / 200.17, dt=tsamp, nsamps=nsamps, snr=30.0, ducy=0.1,
+                            mod_kwargs={"p_orb": 30.0, "x_orb": 0.002, "phi": 0.7})
+    dyp = DynamicProgramming(sim.generate(seed=3), cfg)
+    dyp.initialize()
+    dyp.execute()
+    pcfg = {"poly_order": 5, "nbins": cfg.nbins, "eta": cfg.eta,
+            "branch_max": cfg.branch_max,
+            "counts_init": np.array([len(a) for a in dyp.param_arr]),
+            "param_limits": cfg.param_limits, "widths": cfg.score_widths,
+            "tiling": cfg.tiling_strategy, "prop_sig": cfg.propagator_significance,
+            "val_sig": cfg.validation_significance, "p_orb_min": cfg.p_orb_min,
+            "x_mass_const": cfg.x_mass_const}
+    return cfg, dyp, pcfg
+
+
+@functools.lru_cache(maxsize=4)
+def _oracle_run(batch_size: int, max_sugg: int):
+    """The oracle's run of the workload of _setup (cached: it takes tens of seconds)."""
+    cfg, dyp, pcfg = _setup()
+    nseg = dyp.nsegments
+    thresholds = np.linspace(2.0, 8.0, nseg - 1)
+    return po.circ_prune_run(dyp.fold, dyp.param_arr, dyp.dparams_actual, dyp.tseg,
+                             thresholds, nseg // 2, pcfg, max_sugg, batch_size=batch_size)
+
+
+def _compare(gtree, otree, what: str) -> int:
+    np.testing.assert_array_equal(gtree.scores.cpu().numpy(), otree.scores, err_msg=what)
+    np.testing.assert_array_equal(gtree.backtracks.cpu().numpy(), otree.backtracks, err_msg=what)
+    np.testing.assert_array_equal(gtree.folds.cpu().numpy(), otree.folds, err_msg=what)
+    gl = gtree.leaves.cpu().numpy()
+    ol = otree.leaves
+    np.testing.assert_array_equal(gl[:, -1], ol[:, -1], err_msg=what)  # f0 and basis flag
+    if len(ol):
+        scale = np.maximum(np.abs(ol[:, :-1]).max(axis=0, keepdims=True), 1e-300)
+        assert np.all(np.abs(gl[:, :-1] - ol[:, :-1]) <= 1e-12 * scale), what
+    return int(np.sum(gl != ol))
+
+
+@pytest.mark.parametrize("batch_size,max_sugg", [(1024, 2**16), (256, 2**16)])
+def test_circ_prune_levels_match_oracle(gpu_device, batch_size, max_sugg):
+    _case(f"circular prune batch_size={batch_size} max_sugg={max_sugg}")
+    cfg, dyp, pcfg = _setup()
+    nseg = dyp.nsegments
+    thresholds = np.linspace(2.0, 8.0, nseg - 1)
+    ref = nseg // 2
+    exp = _oracle_run(batch_size, max_sugg)
+    pr = Pruning(dyp, thresholds, max_sugg=max_sugg, batch_size=batch_size)
+    pr.initialize(ref)
+    ulps = 0
+    flags_seen = set()
+    for level in range(1, nseg):
+        pr.execute_iter()
+        ot = exp["trees"][level - 1]
+        ulps += _compare(pr.world_tree, ot, f"level {level}")
+        flags_seen |= set(np.unique(ot.leaves[:, -1, 1]).tolist())
+        st, ost = pr.pstats[level], exp["stats"][level - 1]
+        assert (st.n_leaves, st.n_leaves_phy) == (ost.n_leaves, ost.n_leaves_phy), level
+        assert np.float32(st.score_min) == np.float32(ost.score_min)
+        assert np.float32(st.score_max) == np.float32(ost.score_max)
+        if pr.prune_complete:
+            break
+    print(f"[result] leaf values differing in the last bits: {ulps}; flags {sorted(flags_seen)}",
+          flush=True)
+    assert 1.0 in flags_seen
+    assert any(s.n_leaves_phy < s.n_leaves for s in exp["stats"])
+    pr._do_ascend()
+    t = exp["tree"]
+    np.testing.assert_array_equal(pr.world_tree.folds.cpu().numpy(), t.folds)
+    np.testing.assert_array_equal(pr.world_tree.scores.cpu().numpy(), t.scores)
+    np.testing.assert_array_equal(pr.world_tree.scores_ep.cpu().numpy(), t.scores_ep)
+
+
+def test_circ_execute_report(gpu_device):
+    _case("circular execute + report")
+    cfg, dyp, pcfg = _setup()
+    nseg = dyp.nsegments
+    thresholds = np.linspace(2.0, 8.0, nseg - 1)
+    res = Pruning(dyp, thresholds, max_sugg=2**16).execute(nseg // 2)
+    exp = _oracle_run(1024, 2**16)
+    np.testing.assert_array_equal(res["scores"], exp["tree"].scores)
+    np.testing.assert_allclose(res["param_sets"], exp["report"], rtol=1e-12, atol=1e-9)
+    best = int(np.argmax(res["scores"]))
+    assert abs(res["param_sets"][best, -3, 0] - 200.17) < 0.05
+
+
+def test_circ_branch_max_error(gpu_device):
+    _case("circular branch_max error")
+    cfg, dyp, pcfg = _setup()
+    pr = Pruning(dyp, np.full(dyp.nsegments - 1, 2.0), max_sugg=2**16)
+    pr.initialize(dyp.nsegments // 2)
+    with torch.no_grad():
+        pr.world_tree.leaves[:, 2, 1] = 1e12  # absurd jerk cell -> too many children
+    with pytest.raises(ValueError, match="increase branch_max"):
+        pr.execute_iter()
