@@ -60,6 +60,37 @@ int pyl_brutefold_execute_bf16(const pyl_bf_plan* plan, const float* ts_e_dev,
                                void* stream);
 
 /* ------------------------------------------------------------------------------------
+ * Complex (Fourier-domain) FFA, use_fourier=True.
+ * pyl_cfold_*: core/fold.py:395-481 brutefold_start_complex_opt (via
+ *   ffa_taylor_init_complex fold.py:572-614, FFATaylorComplexDPFuncts.init
+ *   dyn_pffa.py:82-140): fold_dev complex64 (float pairs) [nseg][nfreq][2][nbins/2 + 1],
+ *   nsamps a multiple of seg_len. The plan holds the FP32 basis of the comb of
+ *   harmonics; an execute is one FP32 GEMM per component (rocBLAS) plus the t_ref
+ *   correction. Summation order is the GEMM's (the reference's is its BLAS library's).
+ * pyl_ffa_shift_add_complex: the segment-pair loop of unify_fold with
+ *   common.py:344-372 shift_add_complex: out[k] = tail[k] phi_t^k + head[k] phi_h^k, the
+ *   phasors by the reference's complex64 recurrence from the FP64 phases of
+ *   pyl_ffa_resolve (phases_dev f64 [ncoords_cur][2]); nbins in {32, 64, 128}.
+ * pyl_score_boxcar_complex: ffa.py:400-411 / scoring.py:285-292: irfft of each complex
+ *   row pair to nbins real bins (FP32), e / sqrt(v), boxcar S/N (modes as for
+ *   pyl_score_boxcar); folds_dev complex64 [n][2][nbins/2 + 1].
+ * ------------------------------------------------------------------------------------ */
+typedef struct pyl_cfold_plan pyl_cfold_plan;
+
+int pyl_cfold_plan_create(const double* freqs, int64_t nfreq, int64_t seg_len, int32_t nbins,
+                          double tsamp, double t_ref, void* stream, pyl_cfold_plan** plan_out);
+int pyl_cfold_plan_destroy(pyl_cfold_plan* plan);
+int pyl_cfold_execute(pyl_cfold_plan* plan, const float* ts_e_dev, const float* ts_v_dev,
+                      int64_t nsamps, float* fold_dev, void* stream);
+int pyl_ffa_shift_add_complex(const float* fold_in_dev, int64_t nseg_in, int64_t ncoords_prev,
+                              const int32_t* table_dev, const double* phases_dev,
+                              int64_t ncoords_cur, int32_t nbins, float* fold_out_dev,
+                              void* stream);
+int pyl_score_boxcar_complex(const float* folds_dev, int64_t n, int32_t nbins,
+                             const int64_t* widths, int32_t nw, double stdnoise, int32_t mode,
+                             float* out_dev, void* stream);
+
+/* ------------------------------------------------------------------------------------
  * FFA level: resolve + shift-add.
  * pyl_ffa_resolve replaces core/fold.py:617-671 ffa_taylor_resolve (with
  * transforms.py:140-170 and psr_utils.py:12-56,466-514), evaluated in FP64 on the GPU

@@ -41,6 +41,14 @@ SIGNATURES: list[tuple[str, object, list[object]]] = [
     ("pyl_brutefold_plan_destroy", ctypes.c_int, [_vp]),
     ("pyl_brutefold_execute", ctypes.c_int, [_vp, _vp, _vp, _c_i64, _vp, _vp]),
     ("pyl_brutefold_execute_bf16", ctypes.c_int, [_vp, _vp, _vp, _c_i64, _vp, _vp]),
+    ("pyl_cfold_plan_create", ctypes.c_int,
+     [_f64p, _c_i64, _c_i64, _c_i32, _c_dbl, _c_dbl, _vp, ctypes.POINTER(_vp)]),
+    ("pyl_cfold_plan_destroy", ctypes.c_int, [_vp]),
+    ("pyl_cfold_execute", ctypes.c_int, [_vp, _vp, _vp, _c_i64, _vp, _vp]),
+    ("pyl_ffa_shift_add_complex", ctypes.c_int,
+     [_vp, _c_i64, _c_i64, _vp, _vp, _c_i64, _c_i32, _vp, _vp]),
+    ("pyl_score_boxcar_complex", ctypes.c_int,
+     [_vp, _c_i64, _c_i32, _i64p, _c_i32, _c_dbl, _c_i32, _vp, _vp]),
     ("pyl_ffa_shift_add_bf16", ctypes.c_int,
      [_vp, _c_i64, _c_i64, _vp, _c_i64, _c_i32, _vp, _vp]),
     ("pyl_ffa_shift_add_score_bf16", ctypes.c_int,

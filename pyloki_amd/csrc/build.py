@@ -35,6 +35,10 @@ HIPCC_FLAGS = [
     "-Wno-unused-result",
 ]
 
+# rocBLAS for the one plain GEMM of the path (the complex leaf fold). It resolves to the
+# rocBLAS already loaded by PyTorch-ROCm when that is present (same soname).
+LINK_FLAGS = ["-L/opt/rocm/lib", "-lrocblas"]
+
 
 def hipcc() -> str:
     cand = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
@@ -48,7 +52,7 @@ def build(verbose: bool = False) -> Path:
     """Compile all HIP sources into the shared library (always recompiles)."""
     sources = sorted(str(p) for p in CSRC.glob("*.hip"))
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), *HIPCC_FLAGS, "-o", str(tmp), *sources]
+    cmd = [hipcc(), *HIPCC_FLAGS, "-o", str(tmp), *sources, *LINK_FLAGS]
     if verbose:
         print(" ".join(cmd), flush=True)
     env = dict(os.environ)

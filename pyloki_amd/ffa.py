@@ -36,16 +36,15 @@ class LevelPlan:
     params_dev: torch.Tensor
     table: torch.Tensor
     ncoords: int
+    phases: torch.Tensor | None = None  # FP64 resolved phases (complex folds shift by them)
 
 
 class FFAPlan:
     """Data-independent plan of an FFA search (grids, resolve tables, brute-fold plan)."""
 
     def __init__(self, cfg: PulsarSearchConfig, nlevels: int | None = None) -> None:
-        if cfg.use_fourier:
-            msg = "use_fourier=True (complex folds) is not supported by this FFA plan yet"
-            raise NotImplementedError(msg)
         require_gpu()
+        self.complex = bool(cfg.use_fourier)
         self.cfg = cfg
         self.nlevels = cfg.niters_ffa if nlevels is None else int(nlevels)
         dparams0 = cfg.get_dparams(0)
@@ -57,8 +56,12 @@ class FFAPlan:
                     raise ValueError(msg)
         self.counts0 = cfg.get_param_grid_count(0)
         self.t_ref0 = 0.0 if cfg.nparams == 1 else cfg.bseg_brute * cfg.tsamp / 2
-        self.bf = kernels.BruteFoldPlan(self.param_arr0[-1], cfg.bseg_brute, cfg.nbins,
-                                        cfg.tsamp, self.t_ref0)
+        if self.complex:
+            self.bf = kernels.ComplexFoldPlan(self.param_arr0[-1], cfg.bseg_brute, cfg.nbins,
+                                              cfg.tsamp, self.t_ref0)
+        else:
+            self.bf = kernels.BruteFoldPlan(self.param_arr0[-1], cfg.bseg_brute, cfg.nbins,
+                                            cfg.tsamp, self.t_ref0)
         self.levels: list[LevelPlan] = []
         counts_prev = self.counts0
         for level in range(1, self.nlevels + 1):
@@ -66,11 +69,20 @@ class FFAPlan:
             counts_cur = np.array([len(a) for a in pa], dtype=np.int64)
             ncur = int(np.prod(counts_cur))
             params_dev = to_device(np.concatenate(pa).astype(np.float64))
+            phases = (torch.empty((ncur, 2), dtype=torch.float64, device=params_dev.device)
+                      if self.complex else None)
             table = kernels.ffa_resolve(params_dev, counts_cur, counts_prev, cfg.param_limits,
-                                        level, cfg.tseg_brute, cfg.nbins)
+                                        level, cfg.tseg_brute, cfg.nbins, phases=phases)
             self.levels.append(LevelPlan(level, counts_prev, counts_cur, pa, params_dev, table,
-                                         ncur))
+                                         ncur, phases))
             counts_prev = counts_cur
+
+    def level(self, fold: torch.Tensor, lp: LevelPlan) -> torch.Tensor:
+        """One level's shift-add (real, or complex with fractional phase shifts)."""
+        if self.complex:
+            return kernels.ffa_shift_add_complex(fold, lp.table, lp.phases, lp.ncoords,
+                                                 self.cfg.nbins)
+        return kernels.ffa_shift_add(fold, lp.table, lp.ncoords)
 
     def final_param_arr(self) -> list[np.ndarray]:
         return self.levels[-1].param_arr if self.levels else self.param_arr0
@@ -85,13 +97,16 @@ class FFAPlan:
 
         ``dtype`` selects the fold storage: FP32, or BF16 with FP32 accumulation."""
         nl = self.nlevels if nlevels is None else nlevels
-        fold = self.bf.fold(ts_e, ts_v, dtype=dtype)
+        if self.complex:
+            fold = self.bf.fold(ts_e, ts_v)
+        else:
+            fold = self.bf.fold(ts_e, ts_v, dtype=dtype)
         folds = [fold] if keep else None
         for lp in self.levels[:nl]:
             if fold.shape[0] < 2:
                 msg = "not enough segments for another FFA level"
                 raise ValueError(msg)
-            fold = kernels.ffa_shift_add(fold, lp.table, lp.ncoords)
+            fold = self.level(fold, lp)
             if keep:
                 folds.append(fold)
         return folds if keep else fold
@@ -103,10 +118,7 @@ class DynamicProgramming:
     def __init__(self, ts_data: TimeSeries, cfg: PulsarSearchConfig) -> None:
         self._ts_data = ts_data
         self._cfg = cfg
-        if cfg.use_fourier:
-            msg = "use_fourier=True (complex folds) is not supported yet; set use_fourier=False"
-            raise NotImplementedError(msg)
-        self._data_type = np.float32
+        self._data_type = np.complex64 if cfg.use_fourier else np.float32
         self._plan: FFAPlan | None = None
         self._fold_dev: torch.Tensor | None = None
         self._fold_host: np.ndarray | None = None
@@ -125,7 +137,8 @@ class DynamicProgramming:
 
     @property
     def fold_device(self) -> torch.Tensor:
-        """Current fold in HBM, flattened coords: ``[nseg, ncoords, 2, nbins]``."""
+        """Current fold in HBM, flattened coords: ``[nseg, ncoords, 2, nbins]`` (complex:
+        ``nbins // 2 + 1`` harmonics)."""
         return self._fold_dev
 
     @property
@@ -208,7 +221,7 @@ class DynamicProgramming:
         """One level; ``ffa.py:349-380``."""
         lp = self._plan.levels[self._ffa_level]
         self._ffa_level += 1
-        self._set_fold(kernels.ffa_shift_add(self._fold_dev, lp.table, lp.ncoords))
+        self._set_fold(self._plan.level(self._fold_dev, lp))
         self._param_arr = lp.param_arr
         self._param_grid_count = self.cfg.get_param_grid_count(self._ffa_level)
         self._dparams = self.cfg.get_dparams(self._ffa_level)
@@ -216,8 +229,11 @@ class DynamicProgramming:
         self._tseg *= 2
 
     def get_fold_norm(self, iseg: int = 0, param_idx: tuple | None = None) -> np.ndarray:
-        """``e / sqrt(v)`` of one segment (host); ``ffa.py:382-411``."""
+        """``e / sqrt(v)`` of one segment (host); ``ffa.py:382-411`` (complex folds are
+        inverse-FFTed first)."""
         fold = self.fold
+        if self.cfg.use_fourier:
+            fold = np.fft.irfft(fold).astype(np.float32)
         sel = fold[iseg] if param_idx is None else fold[(iseg, *param_idx)]
         return sel[..., 0, :] / np.sqrt(sel[..., 1, :])
 
@@ -253,6 +269,9 @@ def ffa_batch(plan: FFAPlan, series: list[tuple[torch.Tensor, torch.Tensor]],
     Returns the per-width S/N grids ``[ncoords_final(series), n_w]`` in input order
     (``ffa_search``'s grid, ``search.py:46-47``).
     """
+    if plan.complex:
+        msg = "ffa_batch runs real folds; use DynamicProgramming for use_fourier=True"
+        raise NotImplementedError(msg)
     nser = len(series)
     lens = [int(e.shape[0]) for e, _ in series]
     order = sorted(range(nser), key=lambda i: -lens[i])

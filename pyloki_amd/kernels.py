@@ -71,6 +71,104 @@ class BruteFoldPlan:
             self._handle = None
 
 
+class ComplexFoldPlan:
+    """Device plan of the complex (Fourier-domain) leaf fold (``fold.py:395-481``): the FP32
+    basis of the harmonic comb of ``freqs``, built once; ``fold`` is one GEMM per component.
+    """
+
+    def __init__(self, freqs: np.ndarray, seg_len: int, nbins: int, tsamp: float,
+                 t_ref: float) -> None:
+        require_gpu()
+        lib = _lib.load()
+        freqs = np.ascontiguousarray(freqs, dtype=np.float64)
+        self.freqs = freqs
+        self.nfreq = len(freqs)
+        self.seg_len = int(seg_len)
+        self.nbins = int(nbins)
+        self.nb_f = self.nbins // 2 + 1
+        handle = ctypes.c_void_p()
+        _lib.check(lib.pyl_cfold_plan_create(
+            freqs.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), self.nfreq, self.seg_len,
+            self.nbins, float(tsamp), float(t_ref), stream_ptr(), ctypes.byref(handle)))
+        self._handle = handle
+
+    def fold(self, ts_e: torch.Tensor, ts_v: torch.Tensor,
+             out: torch.Tensor | None = None) -> torch.Tensor:
+        """complex64 [nseg, nfreq, 2, nbins // 2 + 1]."""
+        if ts_e.shape != ts_v.shape or ts_e.dim() != 1:
+            msg = "ts_e, ts_v must be 1-D and of the same length."
+            raise ValueError(msg)
+        if ts_e.dtype != torch.float32 or ts_v.dtype != torch.float32:
+            msg = "ts_e, ts_v must be float32"
+            raise ValueError(msg)
+        nsamps = ts_e.shape[0]
+        if nsamps % self.seg_len:
+            msg = "the complex fold needs nsamps to be a multiple of segment_len"
+            raise ValueError(msg)
+        shape = (nsamps // self.seg_len, self.nfreq, 2, self.nb_f)
+        if out is None:
+            out = torch.empty(shape, dtype=torch.complex64, device=ts_e.device)
+        elif tuple(out.shape) != shape or out.dtype != torch.complex64:
+            msg = f"out must be complex64 of shape {shape}"
+            raise ValueError(msg)
+        _lib.check(_lib.load().pyl_cfold_execute(self._handle, ptr(ts_e.contiguous()),
+                                                 ptr(ts_v.contiguous()), nsamps, ptr(out),
+                                                 stream_ptr()))
+        return out
+
+    def __del__(self) -> None:
+        h = getattr(self, "_handle", None)
+        if h is not None and h.value:
+            try:
+                _lib.load().pyl_cfold_plan_destroy(h)
+            except Exception:  # noqa: BLE001 - interpreter shutdown
+                pass
+            self._handle = None
+
+
+def ffa_shift_add_complex(fold_in: torch.Tensor, table: torch.Tensor, phases: torch.Tensor,
+                          ncoords_cur: int, nbins: int,
+                          out: torch.Tensor | None = None) -> torch.Tensor:
+    """Complex level: [nseg, nprev, 2, nb_f] -> [nseg/2, ncur, 2, nb_f] (complex64)."""
+    if fold_in.dim() != 4 or fold_in.shape[2] != 2 or fold_in.dtype != torch.complex64:
+        msg = "fold_in must be complex64 [nseg, ncoords, 2, nbins // 2 + 1]"
+        raise ValueError(msg)
+    nseg, nprev, _, nb_f = fold_in.shape
+    if nb_f != nbins // 2 + 1 or nseg % 2:
+        msg = "fold_in does not match nbins or has an odd number of segments"
+        raise ValueError(msg)
+    if tuple(table.shape) != (ncoords_cur, 4) or tuple(phases.shape) != (ncoords_cur, 2):
+        msg = "table / phases do not match ncoords_cur"
+        raise ValueError(msg)
+    shape = (nseg // 2, ncoords_cur, 2, nb_f)
+    if out is None:
+        out = torch.empty(shape, dtype=torch.complex64, device=fold_in.device)
+    _lib.check(_lib.load().pyl_ffa_shift_add_complex(
+        ptr(fold_in.contiguous()), nseg, nprev, ptr(table), ptr(phases), ncoords_cur, nbins,
+        ptr(out), stream_ptr()))
+    return out
+
+
+def score_boxcar_complex(folds: torch.Tensor, widths: np.ndarray, nbins: int,
+                         stdnoise: float = 1.0, mode: int = _lib.PYL_SCORE_PER_WIDTH,
+                         out: torch.Tensor | None = None) -> torch.Tensor:
+    """Boxcar S/N of complex rows [..., 2, nbins // 2 + 1] (irfft first)."""
+    widths = np.ascontiguousarray(widths, dtype=np.int64)
+    if folds.dtype != torch.complex64 or folds.shape[-2] != 2 or (
+            folds.shape[-1] != nbins // 2 + 1):
+        msg = "folds must be complex64 [..., 2, nbins // 2 + 1]"
+        raise ValueError(msg)
+    lead = tuple(folds.shape[:-2])
+    n = int(np.prod(lead)) if lead else 1
+    shape = (*lead, len(widths)) if mode == _lib.PYL_SCORE_PER_WIDTH else lead
+    if out is None:
+        out = torch.empty(shape, dtype=torch.float32, device=folds.device)
+    _lib.check(_lib.load().pyl_score_boxcar_complex(
+        ptr(folds.contiguous()), n, nbins, _lib.i64_array(widths), len(widths), float(stdnoise),
+        mode, ptr(out), stream_ptr()))
+    return out
+
+
 def ffa_resolve(params_dev: torch.Tensor, counts_cur, counts_prev, param_limits: np.ndarray,
                 level: int, tseg_brute: float, nbins: int, table: torch.Tensor | None = None,
                 phases: torch.Tensor | None = None) -> torch.Tensor:

@@ -25,7 +25,10 @@ def ffa_search(
     dyp.initialize()
     dyp.execute(show_progress=show_progress)
     widths = search_cfg.score_widths
-    snr_dev = kernels.score_boxcar(dyp.fold_device[0], widths, 1.0)
+    if search_cfg.use_fourier:
+        snr_dev = kernels.score_boxcar_complex(dyp.fold_device[0], widths, search_cfg.nbins, 1.0)
+    else:
+        snr_dev = kernels.score_boxcar(dyp.fold_device[0], widths, 1.0)
     counts = [len(p) for p in dyp.param_arr]
     snrs = snr_dev.cpu().numpy().reshape(*counts, len(widths))
     pgram = Periodogram(params={"width": widths, **dyp.param_arr_dict}, snrs=snrs,
