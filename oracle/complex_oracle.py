@@ -148,3 +148,56 @@ def ffa_search_snr_complex(ts_e, ts_v, cfg) -> np.ndarray:
     fold, _ = ffa_fold_complex(ts_e, ts_v, cfg)
     norm = fold_norm_complex(fold[0], cfg.nbins)
     return orc.boxcar_snr(norm, cfg.score_widths)
+
+
+# ---------------------------------------------------------------- EP with complex folds
+# common.py:403-429 shift_add_complex_batch (the segment row rotated, the parent row not),
+# scoring.py:285-292 snr_score_batch_func_complex, common.py:463-490
+# shift_add_ascend_complex_batch.
+
+def ep_phasors(shift: np.ndarray, nb_f: int) -> np.ndarray:
+    """[n, nb_f] complex64 phasors of the EP complex shift-add: the angle is
+    f32(-2 * f32(pi) * f32(shift) / B) (common.py:419-421), then the recurrence."""
+    nbins = (nb_f - 1) * 2
+    s32 = np.asarray(shift, dtype=np.float64).astype(np.float32)
+    ang = ((-2.0 * float(np.float32(np.pi))) * s32.astype(np.float64) / nbins).astype(
+        np.float32)
+    dre, dim = np.cos(ang), np.sin(ang)
+    out = np.empty((len(s32), nb_f), dtype=np.complex64)
+    pre = np.ones(len(s32), dtype=np.float32)
+    pim = np.zeros(len(s32), dtype=np.float32)
+    for k in range(nb_f):
+        out[:, k] = pre + 1j * pim
+        nre = pre * dre - pim * dim
+        nim = pre * dim + pim * dre
+        pre, pim = nre.astype(np.float32), nim.astype(np.float32)
+    return out
+
+
+def cshift_add_batch(seg: np.ndarray, shifts: np.ndarray, folds: np.ndarray,
+                     parents: np.ndarray) -> np.ndarray:
+    """res = seg * phi^k + parent fold (complex64, [n, 2, nb_f])."""
+    ph = ep_phasors(shifts, seg.shape[-1])[:, None, :]
+    sr, si = _cmul(seg, ph)
+    par = folds[parents]
+    re = (sr + par.real.astype(np.float32)).astype(np.float32)
+    im = (si + par.imag.astype(np.float32)).astype(np.float32)
+    return (re + 1j * im).astype(np.complex64)
+
+
+def cscore_batch(folds: np.ndarray, widths: np.ndarray) -> np.ndarray:
+    """irfft (FP32) then the real scorer's max over widths."""
+    return orc.snr_score_batch(np.fft.irfft(folds).astype(np.float32), widths)
+
+
+def cascend_accumulate(rows: list[np.ndarray], shifts: list[np.ndarray]) -> np.ndarray:
+    """sum over segments (in order, from zero) of row * phi^k, complex64."""
+    n, _, nb_f = rows[0].shape
+    re = np.zeros((n, 2, nb_f), dtype=np.float32)
+    im = np.zeros((n, 2, nb_f), dtype=np.float32)
+    for r, s in zip(rows, shifts):
+        ph = ep_phasors(s, nb_f)[:, None, :]
+        pr, pi = _cmul(r, ph)
+        re = (re + pr).astype(np.float32)
+        im = (im + pi).astype(np.float32)
+    return (re + 1j * im).astype(np.complex64)

@@ -346,11 +346,11 @@ def transform_batch(leaves, coord_next, coord_cur, tiling: str = "aggressive"):
 class TreeBuffer:
     """``WorldTree`` of capacity ``size`` with ``add_batch`` / ``prune_on_overload``."""
 
-    def __init__(self, size: int, P: int, B: int) -> None:
+    def __init__(self, size: int, P: int, B: int, fold_dtype=np.float32) -> None:
         self.size = size
         self.valid = 0
         self.leaves = np.empty((size, P + 2, 2))
-        self.folds = np.empty((size, 2, B), dtype=np.float32)
+        self.folds = np.empty((size, 2, B), dtype=fold_dtype)
         self.scores = np.empty(size, dtype=np.float32)
         self.backtracks = np.empty((size, P + 2), dtype=np.int32)
         self.origin = np.empty(size, dtype=np.int64)  # global child index (for tests)
@@ -400,6 +400,43 @@ class TreeBuffer:
         return eff
 
 
+def _fold_and_score(seg, phase, folds, parents, widths):
+    """Children folds and scores: real rows (integer shift), or complex64 rows
+    (fractional phase rotation, irfft before scoring) when the tree's folds are complex."""
+    if np.iscomplexobj(folds):
+        from oracle import complex_oracle as co
+
+        comb = co.cshift_add_batch(seg, phase, folds, parents)
+        return comb, co.cscore_batch(comb, widths)
+    comb = shift_add_batch(seg, phase, folds, parents)
+    return comb, orc.snr_score_batch(comb, widths)
+
+
+def _ascend_sum(parts, like: np.ndarray) -> np.ndarray:
+    """``shift_add_ascend_batch`` (common.py:432-459) or its complex form (:463-490): the
+    segments' rows, each rotated by its phase, summed in order from zero."""
+    if np.iscomplexobj(like):
+        from oracle import complex_oracle as co
+
+        return co.cascend_accumulate([r for r, _ in parts], [p for _, p in parts])
+    B = like.shape[-1]
+    res = np.zeros((len(like), 2, B), dtype=np.float32)
+    for rows, phase in parts:
+        s = shift_int_vec(phase, B)
+        src = (np.arange(B)[None, :] - s[:, None]) % B
+        rolled = np.take_along_axis(rows, np.broadcast_to(src[:, None, :], rows.shape), axis=2)
+        res = (res + rolled).astype(np.float32)
+    return res
+
+
+def _score_folds(folds, widths):
+    if np.iscomplexobj(folds):
+        from oracle import complex_oracle as co
+
+        return co.cscore_batch(folds, widths)
+    return orc.snr_score_batch(folds, widths)
+
+
 @dataclass
 class LevelStats:
     n_leaves: int
@@ -414,7 +451,7 @@ def prune_level(tree: Tree, fold_segment: np.ndarray, coords: dict, threshold: f
     """One ``pruning_iteration_batched``. Returns (new tree, stats, survivor child ids)."""
     P = cfg["poly_order"]
     B = tree.folds.shape[-1]
-    buf = TreeBuffer(max_sugg, P, B)
+    buf = TreeBuffer(max_sugg, P, B, tree.folds.dtype)
     n_leaves = 0
     smin, smax = math.inf, -math.inf
     cur_thr = threshold
@@ -433,8 +470,7 @@ def prune_level(tree: Tree, fold_segment: np.ndarray, coords: dict, threshold: f
                                    cfg["counts_init"], cfg["param_limits"], cfg["nbins"])
         seg = seg_rows(fold_segment, idx)
         parents = np.arange(b0, b1)[org]
-        comb = shift_add_batch(seg, phase, tree.folds, parents)
-        scores = orc.snr_score_batch(comb, cfg["widths"])
+        comb, scores = _fold_and_score(seg, phase, tree.folds, parents, cfg["widths"])
         smin = min(smin, float(scores.min()))
         smax = max(smax, float(scores.max()))
         # numba compares the f32 scores with the f64 threshold in f64
@@ -466,8 +502,7 @@ def ascend(tree: Tree, dyp_fold: np.ndarray, idx_segments, coord_segments, coord
         return tree
     vec = tree.leaves[:, :-1, 0]
     f0 = tree.leaves[:, -1, 0]
-    B = tree.folds.shape[-1]
-    res = np.zeros((n, 2, B), dtype=np.float32)
+    parts = []
     for iseg, (t0_seg, _) in zip(idx_segments, coord_segments):
         d = shift_taylor(vec, t0_seg - coord_cur[0])
         accel = d[:, -3]
@@ -475,12 +510,9 @@ def ascend(tree: Tree, dyp_fold: np.ndarray, idx_segments, coord_segments, coord
         delay = d[:, -1] / C_VAL
         phase = phase_idx_vec(t0_seg - coord_cur[0], f0, cfg["nbins"], delay)
         idx = nearest_2d(accel, freq, cfg["counts_init"], cfg["param_limits"])
-        rows = seg_rows(dyp_fold[iseg], idx)
-        s = shift_int_vec(phase, B)
-        src = (np.arange(B)[None, :] - s[:, None]) % B
-        rolled = np.take_along_axis(rows, np.broadcast_to(src[:, None, :], rows.shape), axis=2)
-        res = (res + rolled).astype(np.float32)
-    return Tree(tree.leaves.copy(), res, orc.snr_score_batch(res, cfg["widths"]),
+        parts.append((seg_rows(dyp_fold[iseg], idx), phase))
+    res = _ascend_sum(parts, tree.folds)
+    return Tree(tree.leaves.copy(), res, _score_folds(res, cfg["widths"]),
                 tree.backtracks.copy(), scores_ep=tree.scores.copy())
 
 
@@ -512,8 +544,9 @@ def prune_run(dyp_fold: np.ndarray, param_arr: list[np.ndarray], dparams: np.nda
     scheme = MiddleOut(nseg, ref_seg, tseg)
     leaves = taylor_seed(param_arr, dparams, cfg["poly_order"])
     B = dyp_fold.shape[-1]
-    folds = dyp_fold[scheme.ref_idx].reshape(-1, 2, B).astype(np.float32)
-    scores = orc.snr_score_batch(folds, cfg["widths"])
+    folds = dyp_fold[scheme.ref_idx].reshape(-1, 2, B)
+    folds = folds.astype(np.complex64 if np.iscomplexobj(folds) else np.float32)
+    scores = _score_folds(folds, cfg["widths"])
     tree = Tree(leaves, folds.copy(), scores, np.zeros((len(leaves), cfg["poly_order"] + 2),
                                                        dtype=np.int32))
     stats, survivors, trees = [], [], []
@@ -766,7 +799,7 @@ def circ_prune_level(tree: Tree, fold_segment, coords, threshold, cfg, max_sugg,
     """pruning_iteration_batched with the circular plugin (prune.py:165-332)."""
     P = 5
     B = tree.folds.shape[-1]
-    buf = TreeBuffer(max_sugg, P, B)
+    buf = TreeBuffer(max_sugg, P, B, tree.folds.dtype)
     n_leaves = n_phy = 0
     smin, smax = math.inf, -math.inf
     cur_thr = threshold
@@ -790,8 +823,7 @@ def circ_prune_level(tree: Tree, fold_segment, coords, threshold, cfg, max_sugg,
                                         cfg["prop_sig"])
         seg = seg_rows(fold_segment, idx)
         parents = np.arange(b0, b1)[org]
-        comb = shift_add_batch(seg, phase, tree.folds, parents)
-        scores = orc.snr_score_batch(comb, cfg["widths"])
+        comb, scores = _fold_and_score(seg, phase, tree.folds, parents, cfg["widths"])
         smin = min(smin, float(scores.min()))
         smax = max(smax, float(scores.max()))
         fi = np.where(scores.astype(np.float64) >= max(threshold, cur_thr))[0]
@@ -819,8 +851,7 @@ def circ_ascend(tree: Tree, dyp_fold, idx_segments, coord_segments, coord_cur, c
     vec = tree.leaves[:, :-1, 0]
     f0 = tree.leaves[:, -1, 0]
     ms, mc, mt = circ_mask(tree.leaves, cfg["prop_sig"])
-    B = tree.folds.shape[-1]
-    res = np.zeros((n, 2, B), dtype=np.float32)
+    parts = []
     for iseg, (t0_seg, _) in zip(idx_segments, coord_segments):
         d = circ_shift_all(vec, t0_seg - coord_cur[0], ms, mc, mt)
         accel = d[:, -3]
@@ -828,12 +859,9 @@ def circ_ascend(tree: Tree, dyp_fold, idx_segments, coord_segments, coord_cur, c
         delay = d[:, -1] / C_VAL
         phase = phase_idx_vec(t0_seg - coord_cur[0], f0, cfg["nbins"], delay)
         idx = nearest_2d(accel, freq, cfg["counts_init"], cfg["param_limits"])
-        rows = seg_rows(dyp_fold[iseg], idx)
-        s = shift_int_vec(phase, B)
-        src = (np.arange(B)[None, :] - s[:, None]) % B
-        rolled = np.take_along_axis(rows, np.broadcast_to(src[:, None, :], rows.shape), axis=2)
-        res = (res + rolled).astype(np.float32)
-    return Tree(tree.leaves.copy(), res, orc.snr_score_batch(res, cfg["widths"]),
+        parts.append((seg_rows(dyp_fold[iseg], idx), phase))
+    res = _ascend_sum(parts, tree.folds)
+    return Tree(tree.leaves.copy(), res, _score_folds(res, cfg["widths"]),
                 tree.backtracks.copy(), scores_ep=tree.scores.copy())
 
 
@@ -844,8 +872,9 @@ def circ_prune_run(dyp_fold, param_arr, dparams, tseg, thresholds, ref_seg, cfg,
     scheme = MiddleOut(nseg, ref_seg, tseg)
     leaves = taylor_seed(param_arr, dparams, 5)
     B = dyp_fold.shape[-1]
-    folds = dyp_fold[scheme.ref_idx].reshape(-1, 2, B).astype(np.float32)
-    tree = Tree(leaves, folds.copy(), orc.snr_score_batch(folds, cfg["widths"]),
+    folds = dyp_fold[scheme.ref_idx].reshape(-1, 2, B)
+    folds = folds.astype(np.complex64 if np.iscomplexobj(folds) else np.float32)
+    tree = Tree(leaves, folds.copy(), _score_folds(folds, cfg["widths"]),
                 np.zeros((len(leaves), 7), dtype=np.int32))
     stats, survivors, trees = [], [], []
     nl = nseg - 1 if nlevels is None else nlevels

@@ -83,3 +83,69 @@ def test_complex_search_agrees_with_real_search():
     ir = np.unravel_index(np.argmax(snr_r), snr_r.shape)
     assert abs(ic[0] - ir[0]) <= 1 and abs(ic[1] - ir[1]) <= 1
     assert abs(snr_c.max() - snr_r.max()) / snr_r.max() < 0.1
+
+
+def test_ep_complex_shift_add_and_ascend_match_real_at_integer_shifts():
+    from oracle import prune_oracle as po
+
+    nbins, n, npar = 64, 40, 7
+    seg = RNG.normal(size=(n, 2, nbins)).astype(np.float32)
+    par = RNG.normal(size=(npar, 2, nbins)).astype(np.float32)
+    parents = RNG.integers(0, npar, n)
+    shifts = RNG.integers(0, nbins, n).astype(np.float64)
+    got = co.cshift_add_batch(np.fft.rfft(seg).astype(np.complex64), shifts,
+                              np.fft.rfft(par).astype(np.complex64), parents)
+    exp = np.fft.rfft(po.shift_add_batch(seg, shifts, par, parents))
+    assert np.max(np.abs(got - exp)) < 1e-4 * np.max(np.abs(exp))
+    # phasors: the EP angle uses f32(pi) (common.py:419-421)
+    ph = co.ep_phasors(shifts, nbins // 2 + 1)
+    k = np.arange(nbins // 2 + 1)
+    closed = np.exp(-2j * float(np.float32(np.pi)) * k[None, :] * shifts[:, None] / nbins)
+    assert np.max(np.abs(ph - closed)) < 2e-5
+    # ascend: sum of rotated rows from zero
+    rows = [RNG.normal(size=(n, 2, nbins)).astype(np.float32) for _ in range(3)]
+    shs = [RNG.integers(0, nbins, n).astype(np.float64) for _ in range(3)]
+    got = co.cascend_accumulate([np.fft.rfft(r).astype(np.complex64) for r in rows], shs)
+    real = sum(np.stack([np.roll(r[i], int(s[i]), axis=-1) for i in range(n)])
+               for r, s in zip(rows, shs))
+    exp = np.fft.rfft(real)
+    assert np.max(np.abs(got - exp)) < 1e-4 * np.max(np.abs(exp))
+    # scoring = the real scorer on the inverse transform
+    folds = np.fft.rfft(np.abs(RNG.normal(size=(n, 2, nbins))) + 1.0).astype(np.complex64)
+    np.testing.assert_array_equal(co.cscore_batch(folds, np.array([1, 2, 4])),
+                                  orc.snr_score_batch(np.fft.irfft(folds).astype(np.float32),
+                                                      np.array([1, 2, 4])))
+
+
+def test_complex_ep_search_finds_the_real_search_candidate():
+    from oracle import prune_oracle as po
+
+    nsamps, tsamp = 2**17, 64e-6
+    lim = ParamLimits.from_taylor((195.0, 205.0), accel=(-50.0, 50.0)).limits
+    common = dict(nsamps=nsamps, tsamp=tsamp, nbins=32, eta=1.0, param_limits=lim,
+                  bseg_brute=nsamps // 256, bseg_ffa=nsamps // 8, prune_poly_order=2,
+                  ducy_max=0.3, wtsp=1.5, branch_max=32)
+    sim = PulseSignalConfig(period=1 / 200.37, dt=tsamp, nsamps=nsamps, snr=30.0, ducy=0.1,
+                            mod_kwargs={"acc": 12.0})
+    ts = sim.generate(seed=4)
+    out = {}
+    for cplx in (False, True):
+        cfg = PulsarSearchConfig(**common, use_fourier=cplx)
+        if cplx:
+            fold, pa = co.ffa_fold_complex(ts.ts_e, ts.ts_v, cfg)
+        else:
+            fold, pa = orc.ffa_fold(ts.ts_e, ts.ts_v, cfg)
+        counts = [len(a) for a in pa]
+        dyp_fold = fold.reshape(fold.shape[0], *counts, 2, fold.shape[-1])
+        pcfg = {"poly_order": 2, "nbins": 32, "eta": 1.0, "branch_max": 32,
+                "counts_init": np.array(counts), "param_limits": cfg.param_limits,
+                "widths": cfg.score_widths, "tiling": "aggressive"}
+        res = po.prune_run(dyp_fold, pa, cfg.get_dparams_actual(cfg.niters_ffa), cfg.tseg_ffa,
+                           np.linspace(2.0, 5.0, 7), ref_seg=3, cfg=pcfg, max_sugg=2**16)
+        assert np.iscomplexobj(res["tree"].folds) == cplx
+        best = int(np.argmax(res["tree"].scores))
+        out[cplx] = (res["report"][best], float(res["tree"].scores[best]))
+    (rep_r, s_r), (rep_c, s_c) = out[False], out[True]
+    assert abs(rep_c[-3, 0] - rep_r[-3, 0]) < 0.01          # frequency
+    assert abs(rep_c[0, 0] - rep_r[0, 0]) < 2.0             # acceleration
+    assert abs(s_c - s_r) / s_r < 0.15
