@@ -261,6 +261,36 @@ int pyl_prune_materialize(const pyl_prune_params* p, const pyl_prune_coords* c,
                           double* out_leaves_dev, float* out_folds_dev, float* out_scores_dev,
                           int32_t* out_backtracks_dev, void* stream);
 
+/* Complex-fold variants (use_fourier=True; PrunePolyTaylorComplexDPFuncts,
+ * dyn_poly_taylor.py): folds, segment rows and outputs are complex64 [..][2][nbins/2 + 1]
+ * (float pairs). A child's fold is common.py:403-429 shift_add_complex_batch (segment row
+ * times the phasor powers of its FP64 phase, plus the parent row) and its score
+ * scoring.py:285-292 (irfft, then the boxcar scorer); the ascend sums the rotated complex
+ * rows (common.py:463-490). Arguments as for the real entry points. */
+int pyl_prune_children_complex(const pyl_prune_params* p, const pyl_prune_coords* c,
+                               const double* leaves_dev, const float* folds_dev,
+                               int64_t n_parents, const double* branch_dev,
+                               const int32_t* counts_dev, const int64_t* child_off_dev,
+                               int64_t n_children, const float* seg_fold_dev,
+                               int32_t* child_parent_dev, int32_t* child_row_dev,
+                               int32_t* child_shift_dev, double* child_phase_dev,
+                               float* child_score_dev, float* minmax_dev, void* stream);
+int pyl_prune_materialize_complex(const pyl_prune_params* p, const pyl_prune_coords* c,
+                                  const double* leaves_dev, const float* folds_dev,
+                                  const double* branch_dev, const int32_t* counts_dev,
+                                  const int64_t* child_off_dev, const float* seg_fold_dev,
+                                  const int32_t* child_parent_dev, const int32_t* child_row_dev,
+                                  const int32_t* child_shift_dev, const double* child_phase_dev,
+                                  const float* child_score_dev, const int64_t* surv_dev,
+                                  int64_t n_surv, double* out_leaves_dev, float* out_folds_dev,
+                                  float* out_scores_dev, int32_t* out_backtracks_dev,
+                                  void* stream);
+int pyl_prune_ascend_complex(const pyl_prune_params* p, const double* leaves_dev, int64_t n,
+                             const float* dyp_fold_dev, int64_t ncoords_ffa,
+                             const int64_t* seg_idx, const double* seg_t0, int32_t nseg_used,
+                             double t0_cur, float* out_folds_dev, float* out_scores_dev,
+                             void* stream);
+
 /* dyp_fold_dev: f32 [nseg][ncoords][2][nbins] (the FFA fold handed to the EP).
  * seg_idx / seg_t0: host [nseg_used] segment indices and reference times (segments so
  * far, in scheme order). Writes the re-integrated folds and their scores. */
@@ -357,6 +387,31 @@ int pyl_circ_materialize(const pyl_prune_params* p, const pyl_circ_params* q,
                          const int64_t* surv_dev, int64_t n_surv, double* out_leaves_dev,
                          float* out_folds_dev, float* out_scores_dev,
                          int32_t* out_backtracks_dev, void* stream);
+/* Complex-fold variants of the circular entry points (as for pyl_prune_*_complex). */
+int pyl_circ_children_complex(const pyl_prune_params* p, const pyl_circ_params* q,
+                              const pyl_prune_coords* c, const double* leaves_dev,
+                              const float* folds_dev, const pyl_circ_level* L,
+                              const int64_t* valid_dev, int64_t n_valid,
+                              const float* seg_fold_dev, int32_t* child_parent_dev,
+                              int32_t* child_row_dev, int32_t* child_shift_dev,
+                              double* child_phase_dev, float* child_score_dev,
+                              float* minmax_dev, void* stream);
+int pyl_circ_materialize_complex(const pyl_prune_params* p, const pyl_circ_params* q,
+                                 const pyl_prune_coords* c, const double* leaves_dev,
+                                 const float* folds_dev, const pyl_circ_level* L,
+                                 const int64_t* valid_dev, const float* seg_fold_dev,
+                                 const int32_t* child_parent_dev, const int32_t* child_row_dev,
+                                 const int32_t* child_shift_dev, const double* child_phase_dev,
+                                 const float* child_score_dev, const uint8_t* has_taylor_dev,
+                                 const int64_t* surv_dev, int64_t n_surv,
+                                 double* out_leaves_dev, float* out_folds_dev,
+                                 float* out_scores_dev, int32_t* out_backtracks_dev,
+                                 void* stream);
+int pyl_circ_ascend_complex(const pyl_prune_params* p, const pyl_circ_params* q,
+                            const double* leaves_dev, int64_t n, const float* dyp_fold_dev,
+                            int64_t ncoords_ffa, const int64_t* seg_idx, const double* seg_t0,
+                            int32_t nseg_used, double t0_cur, float* out_folds_dev,
+                            float* out_scores_dev, void* stream);
 /* As pyl_prune_ascend, each survivor propagated with its region's propagator. */
 int pyl_circ_ascend(const pyl_prune_params* p, const pyl_circ_params* q,
                     const double* leaves_dev, int64_t n, const float* dyp_fold_dev,

@@ -77,6 +77,21 @@ SIGNATURES: list[tuple[str, object, list[object]]] = [
       _vp, _vp, _vp, _vp]),
     ("pyl_circ_ascend", ctypes.c_int,
      [_vp, _vp, _vp, _c_i64, _vp, _c_i64, _i64p, _f64p, _c_i32, _c_dbl, _vp, _vp, _vp]),
+    ("pyl_prune_children_complex", ctypes.c_int,
+     [_vp, _vp, _vp, _vp, _c_i64, _vp, _vp, _vp, _c_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+      _vp]),
+    ("pyl_prune_materialize_complex", ctypes.c_int,
+     [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_i64, _vp, _vp,
+      _vp, _vp, _vp]),
+    ("pyl_prune_ascend_complex", ctypes.c_int,
+     [_vp, _vp, _c_i64, _vp, _c_i64, _i64p, _f64p, _c_i32, _c_dbl, _vp, _vp, _vp]),
+    ("pyl_circ_children_complex", ctypes.c_int,
+     [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("pyl_circ_materialize_complex", ctypes.c_int,
+     [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_i64, _vp,
+      _vp, _vp, _vp, _vp]),
+    ("pyl_circ_ascend_complex", ctypes.c_int,
+     [_vp, _vp, _vp, _c_i64, _vp, _c_i64, _i64p, _f64p, _c_i32, _c_dbl, _vp, _vp, _vp]),
     ("pyl_prune_branch", ctypes.c_int, [_vp, _vp, _vp, _c_i64, _vp, _vp, _vp, _vp, _vp]),
     ("pyl_prune_children", ctypes.c_int,
      [_vp, _vp, _vp, _vp, _c_i64, _vp, _vp, _vp, _c_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,

@@ -53,9 +53,7 @@ __global__ void cfold_basis_kernel(const float* __restrict__ target, int64_t n_c
   basis[n_comb * seg_len + gid] = (float)sin(ang);
 }
 
-__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
-  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
+using pyl::cmul;
 
 // C_comp[s][i] (real part, i < n_comb) and C_comp[s][n_comb + i] (imaginary part) times the
 // correction phasor, into fold[s][f][comp][m].
@@ -130,29 +128,14 @@ cscore_kernel(const float2* __restrict__ folds, int64_t n, int nbins, pyl::Score
   const int64_t p0 = (int64_t)blockIdx.x * pyl::kScoreBlock;
   const int nprof = (int)std::min<int64_t>(pyl::kScoreBlock, n - p0);
   float* tw = S + (nbins + k.wmax) * pyl::kScoreStride;  // cos, then sin of 2 pi j / B
-  for (int j = tid; j < nbins; j += pyl::kScoreBlock) {
-    const double a = 2.0 * M_PI * (double)j / (double)nbins;
-    tw[j] = (float)cos(a);
-    tw[nbins + j] = (float)sin(a);
-  }
+  pyl::fill_twiddles(tw, nbins, tid, pyl::kScoreBlock);
   __syncthreads();
-  const float inv_n = 1.0f / (float)nbins;
   for (int r = 0; r < nprof; ++r) {
     const float2* row = folds + (p0 + r) * 2 * nb_f;
     for (int b = tid; b < nbins; b += pyl::kScoreBlock) {
-      float x[2];
-      for (int c = 0; c < 2; ++c) {
-        const float2* X = row + c * nb_f;
-        float acc = X[0].x;
-        const float ny = X[nb_f - 1].x;
-        acc += (b & 1) ? -ny : ny;
-        for (int kk = 1; kk < nb_f - 1; ++kk) {
-          const int idx = (kk * b) & (nbins - 1);
-          acc += 2.0f * (X[kk].x * tw[idx] - X[kk].y * tw[nbins + idx]);
-        }
-        x[c] = acc * inv_n;
-      }
-      S[b * pyl::kScoreStride + r] = x[0] / sqrtf(x[1]);
+      const float e = pyl::irfft_bin(row, nbins, b, tw);
+      const float v = pyl::irfft_bin(row + nb_f, nbins, b, tw);
+      S[b * pyl::kScoreStride + r] = e / sqrtf(v);
     }
   }
   __syncthreads();

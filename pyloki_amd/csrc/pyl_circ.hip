@@ -215,6 +215,7 @@ __global__ void circ_expand_kernel(LevelConst k, CircConst q, const double* __re
   }
 }
 
+template <bool kComplex>
 __global__ void __launch_bounds__(kBlk)
 circ_children_kernel(LevelConst k, CircConst q, const double* __restrict__ leaves,
                      const float* __restrict__ folds, pyl_circ_level L,
@@ -256,29 +257,41 @@ circ_children_kernel(LevelConst k, CircConst q, const double* __restrict__ leave
     s_parent[tid] = (int32_t)par;
     s_row[tid] = row;
     s_shift[tid] = sh;
-  }
-  __syncthreads();
-  for (int r = 0; r < nvalid; ++r) {
-    const float* fr = folds + (int64_t)s_parent[r] * 2 * B;
-    const float* sr = seg + (int64_t)s_row[r] * 2 * B;
-    const int sh = s_shift[r];
-    for (int b = tid; b < B; b += kBlk) {
-      int src = b - sh;
-      if (src < 0) src += B;
-      const float e = fr[b] + sr[src];
-      const float vv = fr[B + b] + sr[B + src];
-      S[b * kStride + r] = e / sqrtf(vv);
+    if constexpr (kComplex) {
+      const int nb_f = B / 2 + 1;
+      cfold_child(reinterpret_cast<const float2*>(seg) + (int64_t)row * 2 * nb_f,
+                  reinterpret_cast<const float2*>(folds) + par * 2 * nb_f, phase, B,
+                  reinterpret_cast<float2*>(S) + (size_t)tid * 2 * nb_f);
     }
   }
   __syncthreads();
+  float* Sc = S;
+  if constexpr (kComplex) {
+    Sc = complex_block_columns(k, reinterpret_cast<char*>(S), nvalid, tid);
+  } else {
+    for (int r = 0; r < nvalid; ++r) {
+      const float* fr = folds + (int64_t)s_parent[r] * 2 * B;
+      const float* sr = seg + (int64_t)s_row[r] * 2 * B;
+      const int sh = s_shift[r];
+      for (int b = tid; b < B; b += kBlk) {
+        int src = b - sh;
+        if (src < 0) src += B;
+        const float e = fr[b] + sr[src];
+        const float vv = fr[B + b] + sr[B + src];
+        S[b * kStride + r] = e / sqrtf(vv);
+      }
+    }
+    __syncthreads();
+  }
   float sc = 0.0f;
   if (tid < nvalid) {
-    sc = score_column(k, S, tid);
+    sc = score_column(k, Sc, tid);
     c_score[v0 + tid] = sc;
   }
   block_minmax(sc, tid < nvalid, minmax);
 }
 
+template <bool kComplex>
 __global__ void __launch_bounds__(kBlk)
 circ_materialize_kernel(LevelConst k, CircConst q, const double* __restrict__ leaves,
                         const float* __restrict__ folds, pyl_circ_level L,
@@ -332,29 +345,38 @@ circ_materialize_kernel(LevelConst k, CircConst q, const double* __restrict__ le
     bt[kP - 1] = row / k.n_freq;
     bt[kP] = row % k.n_freq;
     bt[kP + 1] = (int32_t)c_phase[v];
+    if constexpr (kComplex) {
+      const int nb_f = B / 2 + 1;
+      cfold_child(reinterpret_cast<const float2*>(seg) + (int64_t)row * 2 * nb_f,
+                  reinterpret_cast<const float2*>(folds) + par * 2 * nb_f, c_phase[v], B,
+                  reinterpret_cast<float2*>(o_folds) + s * 2 * nb_f);
+    }
   }
-  __syncthreads();
-  for (int r = 0; r < nvalid; ++r) {
-    const int64_t v = s_v[r];
-    const float* fr = folds + (int64_t)c_parent[v] * 2 * B;
-    const float* sr = seg + (int64_t)c_row[v] * 2 * B;
-    const int sh = c_shift[v];
-    float* of = o_folds + (s0 + r) * 2 * B;
-    for (int b = tid; b < B; b += kBlk) {
-      int src = b - sh;
-      if (src < 0) src += B;
-      of[b] = fr[b] + sr[src];
-      of[B + b] = fr[B + b] + sr[B + src];
+  if constexpr (!kComplex) {
+    __syncthreads();
+    for (int r = 0; r < nvalid; ++r) {
+      const int64_t v = s_v[r];
+      const float* fr = folds + (int64_t)c_parent[v] * 2 * B;
+      const float* sr = seg + (int64_t)c_row[v] * 2 * B;
+      const int sh = c_shift[v];
+      float* of = o_folds + (s0 + r) * 2 * B;
+      for (int b = tid; b < B; b += kBlk) {
+        int src = b - sh;
+        if (src < 0) src += B;
+        of[b] = fr[b] + sr[src];
+        of[B + b] = fr[B + b] + sr[B + src];
+      }
     }
   }
 }
 
+template <bool kComplex>
 __global__ void __launch_bounds__(kBlk)
 circ_ascend_kernel(LevelConst k, CircConst q, const double* __restrict__ leaves, int64_t n,
                    const float* __restrict__ dyp, int64_t ncoords,
                    const int64_t* __restrict__ segi, const double* __restrict__ segt,
                    int32_t nseg, double t0_cur, int32_t* __restrict__ scratch_row,
-                   int32_t* __restrict__ scratch_shift, float* __restrict__ o_folds,
+                   double* __restrict__ scratch_phase, float* __restrict__ o_folds,
                    float* __restrict__ o_scores) {
   extern __shared__ float S[];
   const int tid = threadIdx.x;
@@ -385,36 +407,51 @@ circ_ascend_kernel(LevelConst k, CircConst q, const double* __restrict__ leaves,
       const int64_t ia = trunc_idx(d[kN - 3], k.amin, k.sa, k.n_accel);
       const int64_t ifr = trunc_idx(freq, k.fmin, k.sf, k.n_freq);
       scratch_row[(s0 + tid) * nseg + s] = (int32_t)(ia * k.n_freq + ifr);
-      scratch_shift[(s0 + tid) * nseg + s] = pyl::shift_int(phase, B);
+      scratch_phase[(s0 + tid) * nseg + s] = phase;
     }
-  }
-  __syncthreads();
-  // common.py:432-459: accumulate the rotated rows in segment order, from zero.
-  for (int r = 0; r < nvalid; ++r) {
-    const int64_t sv = s0 + r;
-    float* of = o_folds + sv * 2 * B;
-    for (int b = tid; b < B; b += kBlk) {
-      float ae = 0.0f, av = 0.0f;
+    if constexpr (kComplex) {
+      // common.py:463-490: sum of the rotated complex rows in segment order, from zero.
+      const int nb_f = B / 2 + 1;
+      float2* X = reinterpret_cast<float2*>(S) + (size_t)tid * 2 * nb_f;
+      for (int i = 0; i < 2 * nb_f; ++i) X[i] = make_float2(0.0f, 0.0f);
+      const float2* dc = reinterpret_cast<const float2*>(dyp);
       for (int s = 0; s < nseg; ++s) {
-        const float* rowp =
-            dyp + ((int64_t)segi[s] * ncoords + scratch_row[sv * nseg + s]) * 2 * B;
-        int src = b - scratch_shift[sv * nseg + s];
-        if (src < 0) src += B;
-        ae = ae + rowp[src];
-        av = av + rowp[B + src];
+        const int64_t rr = (int64_t)segi[s] * ncoords + scratch_row[(s0 + tid) * nseg + s];
+        cascend_add(dc + rr * 2 * nb_f, scratch_phase[(s0 + tid) * nseg + s], B, X);
       }
-      of[b] = ae;
-      of[B + b] = av;
-      S[b * kStride + r] = ae / sqrtf(av);
+      float2* of = reinterpret_cast<float2*>(o_folds) + (s0 + tid) * 2 * nb_f;
+      for (int i = 0; i < 2 * nb_f; ++i) of[i] = X[i];
     }
   }
   __syncthreads();
-  if (tid < nvalid) o_scores[s0 + tid] = score_column(k, S, tid);
+  float* Sc = S;
+  if constexpr (kComplex) {
+    Sc = complex_block_columns(k, reinterpret_cast<char*>(S), nvalid, tid);
+  } else {
+    // common.py:432-459: accumulate the rotated rows in segment order, from zero.
+    for (int r = 0; r < nvalid; ++r) {
+      const int64_t sv = s0 + r;
+      float* of = o_folds + sv * 2 * B;
+      for (int b = tid; b < B; b += kBlk) {
+        float ae = 0.0f, av = 0.0f;
+        for (int s = 0; s < nseg; ++s) {
+          const float* rowp =
+              dyp + ((int64_t)segi[s] * ncoords + scratch_row[sv * nseg + s]) * 2 * B;
+          int src = b - pyl::shift_int(scratch_phase[sv * nseg + s], B);
+          if (src < 0) src += B;
+          ae = ae + rowp[src];
+          av = av + rowp[B + src];
+        }
+        of[b] = ae;
+        of[B + b] = av;
+        S[b * kStride + r] = ae / sqrtf(av);
+      }
+    }
+    __syncthreads();
+  }
+  if (tid < nvalid) o_scores[s0 + tid] = score_column(k, Sc, tid);
 }
 
-size_t scorer_lds(const LevelConst& k) {
-  return sizeof(float) * (size_t)(k.nbins + k.sc.wmax) * kStride;
-}
 
 }  // namespace
 
@@ -477,12 +514,12 @@ extern "C" int pyl_circ_expand(const pyl_prune_params* p, const pyl_circ_params*
   return PYL_OK;
 }
 
-extern "C" int pyl_circ_children(const pyl_prune_params* p, const pyl_circ_params* q,
-                                 const pyl_prune_coords* c, const double* leaves,
-                                 const float* folds, const pyl_circ_level* L,
-                                 const int64_t* valid, int64_t n_valid, const float* seg,
-                                 int32_t* c_parent, int32_t* c_row, int32_t* c_shift,
-                                 double* c_phase, float* c_score, float* minmax, void* stream) {
+template <bool kComplex>
+static int circ_children(const pyl_prune_params* p, const pyl_circ_params* q,
+                         const pyl_prune_coords* c, const double* leaves, const float* folds,
+                         const pyl_circ_level* L, const int64_t* valid, int64_t n_valid,
+                         const float* seg, int32_t* c_parent, int32_t* c_row, int32_t* c_shift,
+                         double* c_phase, float* c_score, float* minmax, void* stream) {
   PYL_REQUIRE(c != nullptr && L != nullptr, "null coords or level");
   LevelConst k;
   CircConst cc;
@@ -493,13 +530,63 @@ extern "C" int pyl_circ_children(const pyl_prune_params* p, const pyl_circ_param
   PYL_REQUIRE(leaves && folds && valid && seg && c_parent && c_row && c_shift && c_phase &&
                   c_score && minmax && L->c_s1 && L->c_sub && L->c_region && L->s1_parent,
               "null device pointer");
-  const size_t lds = scorer_lds(k);
-  PYL_REQUIRE(lds <= 64 * 1024, "nbins too large for the EP scorer (%d)", k.nbins);
-  circ_children_kernel<<<(unsigned)pyl::ceil_div(n_valid, kBlk), kBlk, lds,
-                         pyl::as_stream(stream)>>>(k, cc, leaves, folds, *L, valid, n_valid,
-                                                   seg, c_parent, c_row, c_shift, c_phase,
-                                                   c_score, minmax);
+  const size_t lds = ep_block_lds<kComplex>(k);
+  PYL_REQUIRE(lds <= kMaxEpLds, "nbins too large for the EP scorer (%d)", k.nbins);
+  if constexpr (kComplex)
+    PYL_REQUIRE((k.nbins & (k.nbins - 1)) == 0,
+                "complex folds need a power-of-two nbins (%d)", k.nbins);
+  circ_children_kernel<kComplex><<<(unsigned)pyl::ceil_div(n_valid, kBlk), kBlk, lds,
+                                   pyl::as_stream(stream)>>>(
+      k, cc, leaves, folds, *L, valid, n_valid, seg, c_parent, c_row, c_shift, c_phase, c_score,
+      minmax);
   PYL_CHECK_LAUNCH("circ_children_kernel");
+  return PYL_OK;
+}
+
+extern "C" int pyl_circ_children(const pyl_prune_params* p, const pyl_circ_params* q,
+                                 const pyl_prune_coords* c, const double* leaves,
+                                 const float* folds, const pyl_circ_level* L,
+                                 const int64_t* valid, int64_t n_valid, const float* seg,
+                                 int32_t* c_parent, int32_t* c_row, int32_t* c_shift,
+                                 double* c_phase, float* c_score, float* minmax, void* stream) {
+  return circ_children<false>(p, q, c, leaves, folds, L, valid, n_valid, seg, c_parent, c_row,
+                              c_shift, c_phase, c_score, minmax, stream);
+}
+
+extern "C" int pyl_circ_children_complex(const pyl_prune_params* p, const pyl_circ_params* q,
+                                         const pyl_prune_coords* c, const double* leaves,
+                                         const float* folds, const pyl_circ_level* L,
+                                         const int64_t* valid, int64_t n_valid,
+                                         const float* seg, int32_t* c_parent, int32_t* c_row,
+                                         int32_t* c_shift, double* c_phase, float* c_score,
+                                         float* minmax, void* stream) {
+  return circ_children<true>(p, q, c, leaves, folds, L, valid, n_valid, seg, c_parent, c_row,
+                             c_shift, c_phase, c_score, minmax, stream);
+}
+
+template <bool kComplex>
+static int circ_materialize(const pyl_prune_params* p, const pyl_circ_params* q,
+                            const pyl_prune_coords* c, const double* leaves, const float* folds,
+                            const pyl_circ_level* L, const int64_t* valid, const float* seg,
+                            const int32_t* c_parent, const int32_t* c_row,
+                            const int32_t* c_shift, const double* c_phase,
+                            const float* c_score, const uint8_t* has_taylor,
+                            const int64_t* surv, int64_t n_surv, double* o_leaves,
+                            float* o_folds, float* o_scores, int32_t* o_bt, void* stream) {
+  PYL_REQUIRE(c != nullptr && L != nullptr, "null coords or level");
+  LevelConst k;
+  CircConst cc;
+  int rc = build_circ(p, q, c, &k, &cc);
+  if (rc != PYL_OK) return rc;
+  if (n_surv == 0) return PYL_OK;
+  PYL_REQUIRE(leaves && folds && valid && seg && c_parent && c_row && c_shift && c_phase &&
+                  c_score && has_taylor && surv && o_leaves && o_folds && o_scores && o_bt,
+              "null device pointer");
+  circ_materialize_kernel<kComplex><<<(unsigned)pyl::ceil_div(n_surv, kBlk), kBlk, 0,
+                                      pyl::as_stream(stream)>>>(
+      k, cc, leaves, folds, *L, valid, seg, c_parent, c_row, c_shift, c_phase, c_score,
+      has_taylor, surv, n_surv, o_leaves, o_folds, o_scores, o_bt);
+  PYL_CHECK_LAUNCH("circ_materialize_kernel");
   return PYL_OK;
 }
 
@@ -513,21 +600,59 @@ extern "C" int pyl_circ_materialize(const pyl_prune_params* p, const pyl_circ_pa
                                     const int64_t* surv, int64_t n_surv, double* o_leaves,
                                     float* o_folds, float* o_scores, int32_t* o_bt,
                                     void* stream) {
-  PYL_REQUIRE(c != nullptr && L != nullptr, "null coords or level");
+  return circ_materialize<false>(p, q, c, leaves, folds, L, valid, seg, c_parent, c_row,
+                                 c_shift, c_phase, c_score, has_taylor, surv, n_surv, o_leaves,
+                                 o_folds, o_scores, o_bt, stream);
+}
+
+extern "C" int pyl_circ_materialize_complex(
+    const pyl_prune_params* p, const pyl_circ_params* q, const pyl_prune_coords* c,
+    const double* leaves, const float* folds, const pyl_circ_level* L, const int64_t* valid,
+    const float* seg, const int32_t* c_parent, const int32_t* c_row, const int32_t* c_shift,
+    const double* c_phase, const float* c_score, const uint8_t* has_taylor, const int64_t* surv,
+    int64_t n_surv, double* o_leaves, float* o_folds, float* o_scores, int32_t* o_bt,
+    void* stream) {
+  return circ_materialize<true>(p, q, c, leaves, folds, L, valid, seg, c_parent, c_row,
+                                c_shift, c_phase, c_score, has_taylor, surv, n_surv, o_leaves,
+                                o_folds, o_scores, o_bt, stream);
+}
+
+template <bool kComplex>
+static int circ_ascend(const pyl_prune_params* p, const pyl_circ_params* q,
+                       const double* leaves, int64_t n, const float* dyp, int64_t ncoords,
+                       const int64_t* seg_idx, const double* seg_t0, int32_t nseg,
+                       double t0_cur, float* o_folds, float* o_scores, void* stream) {
   LevelConst k;
   CircConst cc;
-  int rc = build_circ(p, q, c, &k, &cc);
+  int rc = build_circ(p, q, nullptr, &k, &cc);
   if (rc != PYL_OK) return rc;
-  if (n_surv == 0) return PYL_OK;
-  PYL_REQUIRE(leaves && folds && valid && seg && c_parent && c_row && c_shift && c_phase &&
-                  c_score && has_taylor && surv && o_leaves && o_folds && o_scores && o_bt,
-              "null device pointer");
-  circ_materialize_kernel<<<(unsigned)pyl::ceil_div(n_surv, kBlk), kBlk, 0,
-                            pyl::as_stream(stream)>>>(k, cc, leaves, folds, *L, valid, seg,
-                                                      c_parent, c_row, c_shift, c_phase,
-                                                      c_score, has_taylor, surv, n_surv,
-                                                      o_leaves, o_folds, o_scores, o_bt);
-  PYL_CHECK_LAUNCH("circ_materialize_kernel");
+  PYL_REQUIRE(nseg >= 1, "no segments to integrate");
+  PYL_REQUIRE(leaves && dyp && seg_idx && seg_t0 && o_folds && o_scores, "null pointer");
+  if (n == 0) return PYL_OK;
+  const size_t lds = ep_block_lds<kComplex>(k);
+  PYL_REQUIRE(lds <= kMaxEpLds, "nbins too large for the EP scorer (%d)", k.nbins);
+  if constexpr (kComplex)
+    PYL_REQUIRE((k.nbins & (k.nbins - 1)) == 0,
+                "complex folds need a power-of-two nbins (%d)", k.nbins);
+  hipStream_t s = pyl::as_stream(stream);
+  int64_t* d_idx = nullptr;
+  double* d_t0 = nullptr;
+  int32_t* d_row = nullptr;
+  double* d_ph = nullptr;
+  PYL_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&d_idx), sizeof(int64_t) * nseg, s));
+  PYL_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&d_t0), sizeof(double) * nseg, s));
+  PYL_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&d_row), sizeof(int32_t) * n * nseg, s));
+  PYL_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&d_ph), sizeof(double) * n * nseg, s));
+  PYL_CHECK_HIP(hipMemcpyAsync(d_idx, seg_idx, sizeof(int64_t) * nseg, hipMemcpyHostToDevice, s));
+  PYL_CHECK_HIP(hipMemcpyAsync(d_t0, seg_t0, sizeof(double) * nseg, hipMemcpyHostToDevice, s));
+  circ_ascend_kernel<kComplex><<<(unsigned)pyl::ceil_div(n, kBlk), kBlk, lds, s>>>(
+      k, cc, leaves, n, dyp, ncoords, d_idx, d_t0, nseg, t0_cur, d_row, d_ph, o_folds,
+      o_scores);
+  PYL_CHECK_LAUNCH("circ_ascend_kernel");
+  PYL_CHECK_HIP(hipFreeAsync(d_idx, s));
+  PYL_CHECK_HIP(hipFreeAsync(d_t0, s));
+  PYL_CHECK_HIP(hipFreeAsync(d_row, s));
+  PYL_CHECK_HIP(hipFreeAsync(d_ph, s));
   return PYL_OK;
 }
 
@@ -536,31 +661,15 @@ extern "C" int pyl_circ_ascend(const pyl_prune_params* p, const pyl_circ_params*
                                int64_t ncoords, const int64_t* seg_idx, const double* seg_t0,
                                int32_t nseg, double t0_cur, float* o_folds, float* o_scores,
                                void* stream) {
-  LevelConst k;
-  CircConst cc;
-  int rc = build_circ(p, q, nullptr, &k, &cc);
-  if (rc != PYL_OK) return rc;
-  PYL_REQUIRE(nseg >= 1, "no segments to integrate");
-  PYL_REQUIRE(leaves && dyp && seg_idx && seg_t0 && o_folds && o_scores, "null pointer");
-  if (n == 0) return PYL_OK;
-  hipStream_t s = pyl::as_stream(stream);
-  int64_t* d_idx = nullptr;
-  double* d_t0 = nullptr;
-  int32_t* d_row = nullptr;
-  int32_t* d_sh = nullptr;
-  PYL_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&d_idx), sizeof(int64_t) * nseg, s));
-  PYL_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&d_t0), sizeof(double) * nseg, s));
-  PYL_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&d_row), sizeof(int32_t) * n * nseg, s));
-  PYL_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&d_sh), sizeof(int32_t) * n * nseg, s));
-  PYL_CHECK_HIP(hipMemcpyAsync(d_idx, seg_idx, sizeof(int64_t) * nseg, hipMemcpyHostToDevice, s));
-  PYL_CHECK_HIP(hipMemcpyAsync(d_t0, seg_t0, sizeof(double) * nseg, hipMemcpyHostToDevice, s));
-  circ_ascend_kernel<<<(unsigned)pyl::ceil_div(n, kBlk), kBlk, scorer_lds(k), s>>>(
-      k, cc, leaves, n, dyp, ncoords, d_idx, d_t0, nseg, t0_cur, d_row, d_sh, o_folds,
-      o_scores);
-  PYL_CHECK_LAUNCH("circ_ascend_kernel");
-  PYL_CHECK_HIP(hipFreeAsync(d_idx, s));
-  PYL_CHECK_HIP(hipFreeAsync(d_t0, s));
-  PYL_CHECK_HIP(hipFreeAsync(d_row, s));
-  PYL_CHECK_HIP(hipFreeAsync(d_sh, s));
-  return PYL_OK;
+  return circ_ascend<false>(p, q, leaves, n, dyp, ncoords, seg_idx, seg_t0, nseg, t0_cur,
+                            o_folds, o_scores, stream);
+}
+
+extern "C" int pyl_circ_ascend_complex(const pyl_prune_params* p, const pyl_circ_params* q,
+                                       const double* leaves, int64_t n, const float* dyp,
+                                       int64_t ncoords, const int64_t* seg_idx,
+                                       const double* seg_t0, int32_t nseg, double t0_cur,
+                                       float* o_folds, float* o_scores, void* stream) {
+  return circ_ascend<true>(p, q, leaves, n, dyp, ncoords, seg_idx, seg_t0, nseg, t0_cur,
+                           o_folds, o_scores, stream);
 }

@@ -70,6 +70,8 @@ __global__ void prune_branch_kernel(LevelConst k, const double* __restrict__ lea
   nchild[i] = total;
 }
 
+// Folds are real [2][B] rows or, with kComplex, complex64 [2][B/2 + 1] rows (as float2).
+template <bool kComplex>
 __global__ void __launch_bounds__(kBlk)
 prune_children_kernel(LevelConst k, const double* __restrict__ leaves,
                       const float* __restrict__ folds, int64_t n_par,
@@ -79,13 +81,14 @@ prune_children_kernel(LevelConst k, const double* __restrict__ leaves,
                       int32_t* __restrict__ c_row, int32_t* __restrict__ c_shift,
                       double* __restrict__ c_phase, float* __restrict__ c_score,
                       float* __restrict__ minmax) {
-  extern __shared__ float S[];  // [(B + wmax)][kStride]
+  extern __shared__ float S[];  // real: [(B + wmax)][kStride]; complex: complex_block_lds
   __shared__ int32_t s_parent[kBlk], s_row[kBlk], s_shift[kBlk];
   const int tid = threadIdx.x;
   const int64_t c0 = (int64_t)blockIdx.x * kBlk;
   const int nvalid = (int)std::min<int64_t>(kBlk, n_child - c0);
   const int P = k.P;
   const int B = k.nbins;
+  const int nb_f = B / 2 + 1;
   if (tid < nvalid) {
     const int64_t c = c0 + tid;
     const int64_t par = find_parent(off, n_par, c);
@@ -113,32 +116,44 @@ prune_children_kernel(LevelConst k, const double* __restrict__ leaves,
     s_parent[tid] = (int32_t)par;
     s_row[tid] = row;
     s_shift[tid] = sh;
-  }
-  __syncthreads();
-  // Fold (common.py:389-399: parent row + segment row rotated by the shift) and normalise,
-  // lane = bin, transposed into S.
-  for (int r = 0; r < nvalid; ++r) {
-    const float* fr = folds + (int64_t)s_parent[r] * 2 * B;
-    const float* sr = seg + (int64_t)s_row[r] * 2 * B;
-    const int sh = s_shift[r];
-    for (int b = tid; b < B; b += kBlk) {
-      int src = b - sh;
-      if (src < 0) src += B;
-      const float e = fr[b] + sr[src];
-      const float v = fr[B + b] + sr[B + src];
-      S[b * kStride + r] = e / sqrtf(v);
+    if constexpr (kComplex) {
+      const float2* fc = reinterpret_cast<const float2*>(folds);
+      const float2* sc = reinterpret_cast<const float2*>(seg);
+      cfold_child(sc + (int64_t)row * 2 * nb_f, fc + par * 2 * nb_f, phase, B,
+                  reinterpret_cast<float2*>(S) + (size_t)tid * 2 * nb_f);
     }
   }
   __syncthreads();
+  float* Sc = S;
+  if constexpr (kComplex) {
+    Sc = complex_block_columns(k, reinterpret_cast<char*>(S), nvalid, tid);
+  } else {
+    // Fold (common.py:389-399: parent row + segment row rotated by the shift) and
+    // normalise, lane = bin, transposed into S.
+    for (int r = 0; r < nvalid; ++r) {
+      const float* fr = folds + (int64_t)s_parent[r] * 2 * B;
+      const float* sr = seg + (int64_t)s_row[r] * 2 * B;
+      const int sh = s_shift[r];
+      for (int b = tid; b < B; b += kBlk) {
+        int src = b - sh;
+        if (src < 0) src += B;
+        const float e = fr[b] + sr[src];
+        const float v = fr[B + b] + sr[B + src];
+        S[b * kStride + r] = e / sqrtf(v);
+      }
+    }
+    __syncthreads();
+  }
   float sc = 0.0f;
   if (tid < nvalid) {
-    sc = score_column(k, S, tid);
+    sc = score_column(k, Sc, tid);
     c_score[c0 + tid] = sc;
   }
   // Block min/max of the scores, then one atomic per wavefront.
   block_minmax(sc, tid < nvalid, minmax);
 }
 
+template <bool kComplex>
 __global__ void __launch_bounds__(kBlk)
 prune_materialize_kernel(LevelConst k, const double* __restrict__ leaves,
                          const float* __restrict__ folds, int64_t n_par,
@@ -179,28 +194,37 @@ prune_materialize_kernel(LevelConst k, const double* __restrict__ leaves,
     bt[k.nparams_ffa] = row % k.n_freq;
     for (int q = k.nparams_ffa + 1; q < P + 1; ++q) bt[q] = 0;
     bt[P + 1] = (int32_t)c_phase[c];
+    if constexpr (kComplex) {
+      const int nb_f = B / 2 + 1;
+      cfold_child(reinterpret_cast<const float2*>(seg) + (int64_t)row * 2 * nb_f,
+                  reinterpret_cast<const float2*>(folds) + par * 2 * nb_f, c_phase[c], B,
+                  reinterpret_cast<float2*>(o_folds) + s * 2 * nb_f);
+    }
   }
-  __syncthreads();
-  for (int r = 0; r < nvalid; ++r) {
-    const int64_t c = s_child[r];
-    const float* fr = folds + (int64_t)c_parent[c] * 2 * B;
-    const float* sr = seg + (int64_t)c_row[c] * 2 * B;
-    const int sh = c_shift[c];
-    float* of = o_folds + (s0 + r) * 2 * B;
-    for (int b = tid; b < B; b += kBlk) {
-      int src = b - sh;
-      if (src < 0) src += B;
-      of[b] = fr[b] + sr[src];
-      of[B + b] = fr[B + b] + sr[B + src];
+  if constexpr (!kComplex) {
+    __syncthreads();
+    for (int r = 0; r < nvalid; ++r) {
+      const int64_t c = s_child[r];
+      const float* fr = folds + (int64_t)c_parent[c] * 2 * B;
+      const float* sr = seg + (int64_t)c_row[c] * 2 * B;
+      const int sh = c_shift[c];
+      float* of = o_folds + (s0 + r) * 2 * B;
+      for (int b = tid; b < B; b += kBlk) {
+        int src = b - sh;
+        if (src < 0) src += B;
+        of[b] = fr[b] + sr[src];
+        of[B + b] = fr[B + b] + sr[B + src];
+      }
     }
   }
 }
 
+template <bool kComplex>
 __global__ void __launch_bounds__(kBlk)
 prune_ascend_kernel(LevelConst k, const double* __restrict__ leaves, int64_t n,
                     const float* __restrict__ dyp, int64_t ncoords, const int64_t* __restrict__ segi,
                     const double* __restrict__ segt, int32_t nseg, double t0_cur,
-                    int32_t* __restrict__ scratch_row, int32_t* __restrict__ scratch_shift,
+                    int32_t* __restrict__ scratch_row, double* __restrict__ scratch_phase,
                     float* __restrict__ o_folds, float* __restrict__ o_scores) {
   extern __shared__ float S[];
   const int tid = threadIdx.x;
@@ -208,6 +232,7 @@ prune_ascend_kernel(LevelConst k, const double* __restrict__ leaves, int64_t n,
   const int nvalid = (int)std::min<int64_t>(kBlk, n - s0);
   const int P = k.P;
   const int B = k.nbins;
+  const int nb_f = B / 2 + 1;
   const int nn = P + 1;
   // taylor.py:280-320: resolve every segment so far for this survivor.
   if (tid < nvalid) {
@@ -230,31 +255,48 @@ prune_ascend_kernel(LevelConst k, const double* __restrict__ leaves, int64_t n,
       const int64_t ia = trunc_idx(accel, k.amin, k.sa, k.n_accel);
       const int64_t ifr = trunc_idx(freq, k.fmin, k.sf, k.n_freq);
       scratch_row[(s0 + tid) * nseg + s] = (int32_t)(ia * k.n_freq + ifr);
-      scratch_shift[(s0 + tid) * nseg + s] = pyl::shift_int(phase, B);
+      scratch_phase[(s0 + tid) * nseg + s] = phase;
     }
-  }
-  __syncthreads();
-  // common.py:432-459: accumulate the rotated rows in segment order, from zero.
-  for (int r = 0; r < nvalid; ++r) {
-    const int64_t sv = s0 + r;
-    float* of = o_folds + sv * 2 * B;
-    for (int b = tid; b < B; b += kBlk) {
-      float ae = 0.0f, av = 0.0f;
+    if constexpr (kComplex) {
+      // common.py:463-490: sum of the rotated complex rows in segment order, from zero.
+      float2* X = reinterpret_cast<float2*>(S) + (size_t)tid * 2 * nb_f;
+      for (int i = 0; i < 2 * nb_f; ++i) X[i] = make_float2(0.0f, 0.0f);
+      const float2* dc = reinterpret_cast<const float2*>(dyp);
       for (int s = 0; s < nseg; ++s) {
-        const float* rowp =
-            dyp + ((int64_t)segi[s] * ncoords + scratch_row[sv * nseg + s]) * 2 * B;
-        int src = b - scratch_shift[sv * nseg + s];
-        if (src < 0) src += B;
-        ae = ae + rowp[src];
-        av = av + rowp[B + src];
+        const int64_t rr = (int64_t)segi[s] * ncoords + scratch_row[(s0 + tid) * nseg + s];
+        cascend_add(dc + rr * 2 * nb_f, scratch_phase[(s0 + tid) * nseg + s], B, X);
       }
-      of[b] = ae;
-      of[B + b] = av;
-      S[b * kStride + r] = ae / sqrtf(av);
+      float2* of = reinterpret_cast<float2*>(o_folds) + (s0 + tid) * 2 * nb_f;
+      for (int i = 0; i < 2 * nb_f; ++i) of[i] = X[i];
     }
   }
   __syncthreads();
-  if (tid < nvalid) o_scores[s0 + tid] = score_column(k, S, tid);
+  float* Sc = S;
+  if constexpr (kComplex) {
+    Sc = complex_block_columns(k, reinterpret_cast<char*>(S), nvalid, tid);
+  } else {
+    // common.py:432-459: accumulate the rotated rows in segment order, from zero.
+    for (int r = 0; r < nvalid; ++r) {
+      const int64_t sv = s0 + r;
+      float* of = o_folds + sv * 2 * B;
+      for (int b = tid; b < B; b += kBlk) {
+        float ae = 0.0f, av = 0.0f;
+        for (int s = 0; s < nseg; ++s) {
+          const float* rowp =
+              dyp + ((int64_t)segi[s] * ncoords + scratch_row[sv * nseg + s]) * 2 * B;
+          int src = b - pyl::shift_int(scratch_phase[sv * nseg + s], B);
+          if (src < 0) src += B;
+          ae = ae + rowp[src];
+          av = av + rowp[B + src];
+        }
+        of[b] = ae;
+        of[B + b] = av;
+        S[b * kStride + r] = ae / sqrtf(av);
+      }
+    }
+    __syncthreads();
+  }
+  if (tid < nvalid) o_scores[s0 + tid] = score_column(k, Sc, tid);
 }
 
 }  // namespace
@@ -276,13 +318,13 @@ extern "C" int pyl_prune_branch(const pyl_prune_params* p, const pyl_prune_coord
   return PYL_OK;
 }
 
-extern "C" int pyl_prune_children(const pyl_prune_params* p, const pyl_prune_coords* c,
-                                  const double* leaves, const float* folds, int64_t n_par,
-                                  const double* branch, const int32_t* counts,
-                                  const int64_t* off, int64_t n_child, const float* seg,
-                                  int32_t* c_parent, int32_t* c_row, int32_t* c_shift,
-                                  double* c_phase, float* c_score, float* minmax,
-                                  void* stream) {
+template <bool kComplex>
+static int prune_children(const pyl_prune_params* p, const pyl_prune_coords* c,
+                          const double* leaves, const float* folds, int64_t n_par,
+                          const double* branch, const int32_t* counts, const int64_t* off,
+                          int64_t n_child, const float* seg, int32_t* c_parent, int32_t* c_row,
+                          int32_t* c_shift, double* c_phase, float* c_score, float* minmax,
+                          void* stream) {
   LevelConst k;
   int rc = build_const(p, c, &k);
   if (rc != PYL_OK) return rc;
@@ -292,13 +334,64 @@ extern "C" int pyl_prune_children(const pyl_prune_params* p, const pyl_prune_coo
               "null device pointer");
   PYL_REQUIRE((int64_t)p->n_accel * p->n_freq <= 0x7fffffff, "FFA grid too large");
   if (n_child == 0) return PYL_OK;
-  const size_t lds = sizeof(float) * (size_t)(k.nbins + k.sc.wmax) * kStride;
-  PYL_REQUIRE(lds <= 64 * 1024, "nbins too large for the EP scorer (%d)", k.nbins);
-  prune_children_kernel<<<(unsigned)pyl::ceil_div(n_child, kBlk), kBlk, lds,
-                          pyl::as_stream(stream)>>>(k, leaves, folds, n_par, branch, counts,
-                                                    off, n_child, seg, c_parent, c_row, c_shift,
-                                                    c_phase, c_score, minmax);
+  const size_t lds = ep_block_lds<kComplex>(k);
+  PYL_REQUIRE(lds <= kMaxEpLds, "nbins too large for the EP scorer (%d)", k.nbins);
+  if constexpr (kComplex)
+    PYL_REQUIRE((k.nbins & (k.nbins - 1)) == 0,
+                "complex folds need a power-of-two nbins (%d)", k.nbins);
+  prune_children_kernel<kComplex><<<(unsigned)pyl::ceil_div(n_child, kBlk), kBlk, lds,
+                                    pyl::as_stream(stream)>>>(
+      k, leaves, folds, n_par, branch, counts, off, n_child, seg, c_parent, c_row, c_shift,
+      c_phase, c_score, minmax);
   PYL_CHECK_LAUNCH("prune_children_kernel");
+  return PYL_OK;
+}
+
+extern "C" int pyl_prune_children(const pyl_prune_params* p, const pyl_prune_coords* c,
+                                  const double* leaves, const float* folds, int64_t n_par,
+                                  const double* branch, const int32_t* counts,
+                                  const int64_t* off, int64_t n_child, const float* seg,
+                                  int32_t* c_parent, int32_t* c_row, int32_t* c_shift,
+                                  double* c_phase, float* c_score, float* minmax,
+                                  void* stream) {
+  return prune_children<false>(p, c, leaves, folds, n_par, branch, counts, off, n_child, seg,
+                               c_parent, c_row, c_shift, c_phase, c_score, minmax, stream);
+}
+
+extern "C" int pyl_prune_children_complex(const pyl_prune_params* p, const pyl_prune_coords* c,
+                                          const double* leaves, const float* folds,
+                                          int64_t n_par, const double* branch,
+                                          const int32_t* counts, const int64_t* off,
+                                          int64_t n_child, const float* seg, int32_t* c_parent,
+                                          int32_t* c_row, int32_t* c_shift, double* c_phase,
+                                          float* c_score, float* minmax, void* stream) {
+  return prune_children<true>(p, c, leaves, folds, n_par, branch, counts, off, n_child, seg,
+                              c_parent, c_row, c_shift, c_phase, c_score, minmax, stream);
+}
+
+template <bool kComplex>
+static int prune_materialize(const pyl_prune_params* p, const pyl_prune_coords* c,
+                             const double* leaves, const float* folds, const double* branch,
+                             const int32_t* counts, const int64_t* off, const float* seg,
+                             const int32_t* c_parent, const int32_t* c_row,
+                             const int32_t* c_shift, const double* c_phase,
+                             const float* c_score, const int64_t* surv, int64_t n_surv,
+                             double* o_leaves, float* o_folds, float* o_scores, int32_t* o_bt,
+                             void* stream) {
+  LevelConst k;
+  int rc = build_const(p, c, &k);
+  if (rc != PYL_OK) return rc;
+  PYL_REQUIRE(c != nullptr, "null pyl_prune_coords");
+  if (n_surv == 0) return PYL_OK;
+  PYL_REQUIRE(leaves && folds && branch && counts && off && seg && c_parent && c_row &&
+                  c_shift && c_phase && c_score && surv && o_leaves && o_folds && o_scores &&
+                  o_bt,
+              "null device pointer");
+  prune_materialize_kernel<kComplex><<<(unsigned)pyl::ceil_div(n_surv, kBlk), kBlk, 0,
+                                       pyl::as_stream(stream)>>>(
+      k, leaves, folds, 0, branch, counts, off, seg, c_parent, c_row, c_shift, c_phase, c_score,
+      surv, n_surv, o_leaves, o_folds, o_scores, o_bt);
+  PYL_CHECK_LAUNCH("prune_materialize_kernel");
   return PYL_OK;
 }
 
@@ -311,20 +404,57 @@ extern "C" int pyl_prune_materialize(const pyl_prune_params* p, const pyl_prune_
                                      const float* c_score, const int64_t* surv, int64_t n_surv,
                                      double* o_leaves, float* o_folds, float* o_scores,
                                      int32_t* o_bt, void* stream) {
+  return prune_materialize<false>(p, c, leaves, folds, branch, counts, off, seg, c_parent,
+                                  c_row, c_shift, c_phase, c_score, surv, n_surv, o_leaves,
+                                  o_folds, o_scores, o_bt, stream);
+}
+
+extern "C" int pyl_prune_materialize_complex(
+    const pyl_prune_params* p, const pyl_prune_coords* c, const double* leaves,
+    const float* folds, const double* branch, const int32_t* counts, const int64_t* off,
+    const float* seg, const int32_t* c_parent, const int32_t* c_row, const int32_t* c_shift,
+    const double* c_phase, const float* c_score, const int64_t* surv, int64_t n_surv,
+    double* o_leaves, float* o_folds, float* o_scores, int32_t* o_bt, void* stream) {
+  return prune_materialize<true>(p, c, leaves, folds, branch, counts, off, seg, c_parent,
+                                 c_row, c_shift, c_phase, c_score, surv, n_surv, o_leaves,
+                                 o_folds, o_scores, o_bt, stream);
+}
+
+template <bool kComplex>
+static int prune_ascend(const pyl_prune_params* p, const double* leaves, int64_t n,
+                        const float* dyp, int64_t ncoords, const int64_t* seg_idx,
+                        const double* seg_t0, int32_t nseg, double t0_cur, float* o_folds,
+                        float* o_scores, void* stream) {
   LevelConst k;
-  int rc = build_const(p, c, &k);
+  int rc = build_const(p, nullptr, &k);
   if (rc != PYL_OK) return rc;
-  PYL_REQUIRE(c != nullptr, "null pyl_prune_coords");
-  if (n_surv == 0) return PYL_OK;
-  PYL_REQUIRE(leaves && folds && branch && counts && off && seg && c_parent && c_row &&
-                  c_shift && c_phase && c_score && surv && o_leaves && o_folds && o_scores &&
-                  o_bt,
-              "null device pointer");
-  prune_materialize_kernel<<<(unsigned)pyl::ceil_div(n_surv, kBlk), kBlk, 0,
-                             pyl::as_stream(stream)>>>(
-      k, leaves, folds, 0, branch, counts, off, seg, c_parent, c_row, c_shift, c_phase, c_score,
-      surv, n_surv, o_leaves, o_folds, o_scores, o_bt);
-  PYL_CHECK_LAUNCH("prune_materialize_kernel");
+  PYL_REQUIRE(nseg >= 1, "no segments to integrate");
+  PYL_REQUIRE(leaves && dyp && seg_idx && seg_t0 && o_folds && o_scores, "null pointer");
+  if (n == 0) return PYL_OK;
+  const size_t lds = ep_block_lds<kComplex>(k);
+  PYL_REQUIRE(lds <= kMaxEpLds, "nbins too large for the EP scorer (%d)", k.nbins);
+  if constexpr (kComplex)
+    PYL_REQUIRE((k.nbins & (k.nbins - 1)) == 0,
+                "complex folds need a power-of-two nbins (%d)", k.nbins);
+  hipStream_t s = pyl::as_stream(stream);
+  // Small per-call tables: segment indices/times and per-(survivor, segment) row/phase.
+  int64_t* d_idx = nullptr;
+  double* d_t0 = nullptr;
+  int32_t* d_row = nullptr;
+  double* d_ph = nullptr;
+  PYL_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&d_idx), sizeof(int64_t) * nseg, s));
+  PYL_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&d_t0), sizeof(double) * nseg, s));
+  PYL_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&d_row), sizeof(int32_t) * n * nseg, s));
+  PYL_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&d_ph), sizeof(double) * n * nseg, s));
+  PYL_CHECK_HIP(hipMemcpyAsync(d_idx, seg_idx, sizeof(int64_t) * nseg, hipMemcpyHostToDevice, s));
+  PYL_CHECK_HIP(hipMemcpyAsync(d_t0, seg_t0, sizeof(double) * nseg, hipMemcpyHostToDevice, s));
+  prune_ascend_kernel<kComplex><<<(unsigned)pyl::ceil_div(n, kBlk), kBlk, lds, s>>>(
+      k, leaves, n, dyp, ncoords, d_idx, d_t0, nseg, t0_cur, d_row, d_ph, o_folds, o_scores);
+  PYL_CHECK_LAUNCH("prune_ascend_kernel");
+  PYL_CHECK_HIP(hipFreeAsync(d_idx, s));
+  PYL_CHECK_HIP(hipFreeAsync(d_t0, s));
+  PYL_CHECK_HIP(hipFreeAsync(d_row, s));
+  PYL_CHECK_HIP(hipFreeAsync(d_ph, s));
   return PYL_OK;
 }
 
@@ -332,31 +462,15 @@ extern "C" int pyl_prune_ascend(const pyl_prune_params* p, const double* leaves,
                                 const float* dyp, int64_t ncoords, const int64_t* seg_idx,
                                 const double* seg_t0, int32_t nseg, double t0_cur,
                                 float* o_folds, float* o_scores, void* stream) {
-  LevelConst k;
-  int rc = build_const(p, nullptr, &k);
-  if (rc != PYL_OK) return rc;
-  PYL_REQUIRE(nseg >= 1, "no segments to integrate");
-  PYL_REQUIRE(leaves && dyp && seg_idx && seg_t0 && o_folds && o_scores, "null pointer");
-  if (n == 0) return PYL_OK;
-  hipStream_t s = pyl::as_stream(stream);
-  // Small per-call tables: segment indices/times and per-(survivor, segment) row/shift.
-  int64_t* d_idx = nullptr;
-  double* d_t0 = nullptr;
-  int32_t* d_row = nullptr;
-  int32_t* d_sh = nullptr;
-  PYL_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&d_idx), sizeof(int64_t) * nseg, s));
-  PYL_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&d_t0), sizeof(double) * nseg, s));
-  PYL_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&d_row), sizeof(int32_t) * n * nseg, s));
-  PYL_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&d_sh), sizeof(int32_t) * n * nseg, s));
-  PYL_CHECK_HIP(hipMemcpyAsync(d_idx, seg_idx, sizeof(int64_t) * nseg, hipMemcpyHostToDevice, s));
-  PYL_CHECK_HIP(hipMemcpyAsync(d_t0, seg_t0, sizeof(double) * nseg, hipMemcpyHostToDevice, s));
-  const size_t lds = sizeof(float) * (size_t)(k.nbins + k.sc.wmax) * kStride;
-  prune_ascend_kernel<<<(unsigned)pyl::ceil_div(n, kBlk), kBlk, lds, s>>>(
-      k, leaves, n, dyp, ncoords, d_idx, d_t0, nseg, t0_cur, d_row, d_sh, o_folds, o_scores);
-  PYL_CHECK_LAUNCH("prune_ascend_kernel");
-  PYL_CHECK_HIP(hipFreeAsync(d_idx, s));
-  PYL_CHECK_HIP(hipFreeAsync(d_t0, s));
-  PYL_CHECK_HIP(hipFreeAsync(d_row, s));
-  PYL_CHECK_HIP(hipFreeAsync(d_sh, s));
-  return PYL_OK;
+  return prune_ascend<false>(p, leaves, n, dyp, ncoords, seg_idx, seg_t0, nseg, t0_cur,
+                             o_folds, o_scores, stream);
+}
+
+extern "C" int pyl_prune_ascend_complex(const pyl_prune_params* p, const double* leaves,
+                                        int64_t n, const float* dyp, int64_t ncoords,
+                                        const int64_t* seg_idx, const double* seg_t0,
+                                        int32_t nseg, double t0_cur, float* o_folds,
+                                        float* o_scores, void* stream) {
+  return prune_ascend<true>(p, leaves, n, dyp, ncoords, seg_idx, seg_t0, nseg, t0_cur,
+                            o_folds, o_scores, stream);
 }

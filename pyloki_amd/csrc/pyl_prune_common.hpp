@@ -201,5 +201,84 @@ __device__ __forceinline__ void block_minmax(float sc, bool valid, float* minmax
   }
 }
 
+// ---- Complex folds (use_fourier=True) ------------------------------------------------
+// common.py:403-429 shift_add_complex_batch for one child: X[c][k] = seg[c][k] phi^k +
+// parent[c][k], phi^k by the complex64 recurrence of delta = (cos, sin)(f32(-2 f32(pi)
+// f32(phase) / B)). X is [2][B/2 + 1].
+__device__ inline void cfold_child(const float2* __restrict__ seg, const float2* __restrict__ par,
+                                   double phase, int nbins, float2* X) {
+  const int nb_f = nbins / 2 + 1;
+  const float s32 = (float)phase;
+  const float ang = (float)((-2.0 * (double)(float)M_PI * (double)s32) / (double)nbins);
+  const float2 d = make_float2(cosf(ang), sinf(ang));
+  float2 p = make_float2(1.0f, 0.0f);
+  for (int kk = 0; kk < nb_f; ++kk) {
+    for (int c = 0; c < 2; ++c) {
+      const float2 a = pyl::cmul(seg[c * nb_f + kk], p);
+      const float2 f = par[c * nb_f + kk];
+      X[c * nb_f + kk] = make_float2(a.x + f.x, a.y + f.y);
+    }
+    p = pyl::cmul(p, d);
+  }
+}
+
+// common.py:463-490: X += row[c][k] phi^k for one segment of the ascend.
+__device__ inline void cascend_add(const float2* __restrict__ row, double phase, int nbins,
+                                   float2* X) {
+  const int nb_f = nbins / 2 + 1;
+  const float s32 = (float)phase;
+  const float ang = (float)((-2.0 * (double)(float)M_PI * (double)s32) / (double)nbins);
+  const float2 d = make_float2(cosf(ang), sinf(ang));
+  float2 p = make_float2(1.0f, 0.0f);
+  for (int kk = 0; kk < nb_f; ++kk) {
+    for (int c = 0; c < 2; ++c) {
+      const float2 a = pyl::cmul(row[c * nb_f + kk], p);
+      const float2 x = X[c * nb_f + kk];
+      X[c * nb_f + kk] = make_float2(x.x + a.x, x.y + a.y);
+    }
+    p = pyl::cmul(p, d);
+  }
+}
+
+// LDS of the complex block scorer: rows Xs [kBlk][2][B/2 + 1] complex, the scorer's columns
+// and the irfft twiddles.
+__host__ __device__ inline size_t complex_block_lds(int nbins, int wmax) {
+  return sizeof(float2) * (size_t)kBlk * 2 * (nbins / 2 + 1) +
+         sizeof(float) * ((size_t)(nbins + wmax) * kStride + 2 * nbins);
+}
+
+// irfft + e/sqrt(v) of the block's rows (lane = bin) into the scorer's columns; the caller
+// has filled Xs and synchronised. Returns after a barrier; thread tid < nvalid then scores
+// column tid.
+__device__ inline float* complex_block_columns(const LevelConst& k, char* lds, int nvalid,
+                                               int tid) {
+  const int B = k.nbins;
+  const int nb_f = B / 2 + 1;
+  const float2* Xs = reinterpret_cast<const float2*>(lds);
+  float* S = reinterpret_cast<float*>(lds + sizeof(float2) * (size_t)kBlk * 2 * nb_f);
+  float* tw = S + (B + k.sc.wmax) * kStride;
+  pyl::fill_twiddles(tw, B, tid, kBlk);
+  __syncthreads();
+  for (int r = 0; r < nvalid; ++r) {
+    const float2* X = Xs + (size_t)r * 2 * nb_f;
+    for (int b = tid; b < B; b += kBlk) {
+      const float e = pyl::irfft_bin(X, B, b, tw);
+      const float v = pyl::irfft_bin(X + nb_f, B, b, tw);
+      S[b * kStride + r] = e / sqrtf(v);
+    }
+  }
+  __syncthreads();
+  return S;
+}
+
+// Dynamic LDS of the EP scoring blocks: the real scorer's columns, or the complex rows, the
+// columns and the twiddles. gfx950 gives a workgroup up to 160 KB.
+constexpr size_t kMaxEpLds = 160 * 1024;
+template <bool kComplex>
+inline size_t ep_block_lds(const LevelConst& k) {
+  if constexpr (kComplex) return complex_block_lds(k.nbins, k.sc.wmax);
+  return sizeof(float) * (size_t)(k.nbins + k.sc.wmax) * kStride;
+}
+
 }  // namespace
 }  // namespace pyl_ep

@@ -116,4 +116,33 @@ __device__ __forceinline__ double score_regs(const ScoreConst& k, const float (&
   return best;
 }
 
+// ---- Complex (Fourier-domain) rows --------------------------------------------------
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// Twiddles for irfft_bin: tw[j] = cos(2 pi j / B), tw[B + j] = sin(2 pi j / B), FP32 of FP64.
+__device__ inline void fill_twiddles(float* tw, int nbins, int tid, int nthreads) {
+  for (int j = tid; j < nbins; j += nthreads) {
+    const double a = 2.0 * M_PI * (double)j / (double)nbins;
+    tw[j] = (float)cos(a);
+    tw[nbins + j] = (float)sin(a);
+  }
+}
+
+// Real bin b of the inverse real FFT of one row of B/2 + 1 harmonics (power-of-two B):
+// x[b] = (X_0 + (-1)^b X_{B/2} + 2 sum_{k=1}^{B/2-1} Re(X_k e^{2 pi i k b / B})) / B, FP32,
+// the imaginary parts of X_0 and X_{B/2} ignored (as a real inverse transform does).
+__device__ inline float irfft_bin(const float2* X, int nbins, int b, const float* tw) {
+  const int nb_f = nbins / 2 + 1;
+  float acc = X[0].x;
+  const float ny = X[nb_f - 1].x;
+  acc += (b & 1) ? -ny : ny;
+  for (int kk = 1; kk < nb_f - 1; ++kk) {
+    const int idx = (kk * b) & (nbins - 1);
+    acc += 2.0f * (X[kk].x * tw[idx] - X[kk].y * tw[nbins + idx]);
+  }
+  return acc * (1.0f / (float)nbins);
+}
+
 }  // namespace pyl

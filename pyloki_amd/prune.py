@@ -1,7 +1,8 @@
 """EP (enumerate-and-prune) search on the GPU: ``Pruning`` and ``prune_dyp_tree``.
 
-Mirrors ``pyloki.prune`` (``src/pyloki/prune.py``) for the Taylor basis with real folds
-and a moving grid (``use_fourier=False``, ``poly_basis="taylor"``, ``use_moving_grid=True``):
+Mirrors ``pyloki.prune`` (``src/pyloki/prune.py``) for the Taylor and circular bases
+(``poly_basis="taylor"``; circular when ``prune_poly_order == 5``), with real or complex
+(``use_fourier=True``) folds and a moving grid (``use_moving_grid=True``):
 seed from the FFA fold of the reference segment, one pruning level per remaining segment
 (middle-out order), the final ascend (re-integration + rescoring) and the report transform.
 
@@ -238,9 +239,7 @@ class Pruning:
         if poly_basis != "taylor":
             msg = f"Invalid pruning configuration: {(self._circular, poly_basis, cfg.use_fourier)}"
             raise ValueError(msg)
-        if cfg.use_fourier:
-            msg = "complex (use_fourier=True) pruning is not supported yet"
-            raise NotImplementedError(msg)
+        self._complex = bool(cfg.use_fourier)  # complex64 folds (Fourier domain)
         if not use_moving_grid:
             msg = "only the moving grid is supported"
             raise NotImplementedError(msg)
@@ -349,7 +348,11 @@ class Pruning:
         folds = self._segment_rows(self.scheme.ref_idx).reshape(n, 2, -1).contiguous()
         from pyloki_amd import kernels
 
-        scores = kernels.score_boxcar(folds, self._widths, mode=_lib.PYL_SCORE_MAX)
+        if self._complex:
+            scores = kernels.score_boxcar_complex(folds, self._widths, self.dyp.cfg.nbins,
+                                                  mode=_lib.PYL_SCORE_MAX)
+        else:
+            scores = kernels.score_boxcar(folds, self._widths, mode=_lib.PYL_SCORE_MAX)
         bt = torch.zeros((n, P + 2), dtype=torch.int32, device=folds.device)
         self._world_tree = DeviceTree(to_device(leaves), folds.clone(), scores, bt)
         self._backtrack_hist = []
@@ -393,6 +396,16 @@ class Pruning:
             self._prune_complete = True
             return
         self._backtrack_hist.append(new_tree.backtracks.clone())
+
+    def _fn(self, lib, name: str):
+        """The real or complex-fold variant of an EP entry point."""
+        return getattr(lib, f"{name}_complex" if self._complex else name)
+
+    def _new_folds(self, n: int, dev) -> torch.Tensor:
+        B = self.dyp.cfg.nbins
+        if self._complex:
+            return torch.empty((n, 2, B // 2 + 1), dtype=torch.complex64, device=dev)
+        return torch.empty((n, 2, B), dtype=torch.float32, device=dev)
 
     def _survivors(self, c_score: torch.Tensor, child_off_fn, threshold: float) -> torch.Tensor:
         """Children kept: all passing ones when they fit the buffer, else the reference's
@@ -438,17 +451,17 @@ class Pruning:
         c_score = torch.empty(total, dtype=torch.float32, device=dev)
         minmax = torch.tensor([np.inf, -np.inf], dtype=torch.float32, device=dev)
         segc = seg.contiguous()
-        _lib.check(lib.pyl_prune_children(
+        _lib.check(self._fn(lib, "pyl_prune_children")(
             pp, cc, ptr(tree.leaves), ptr(tree.folds), n, ptr(branch), ptr(counts), ptr(off),
             total, ptr(segc), ptr(c_parent), ptr(c_row), ptr(c_shift), ptr(c_phase),
             ptr(c_score), ptr(minmax), stream_ptr()))
         surv = self._survivors(c_score, lambda: off.cpu().numpy(), threshold)
         m = int(surv.shape[0])
         o_leaves = torch.empty((m, P + 2, 2), dtype=torch.float64, device=dev)
-        o_folds = torch.empty((m, 2, B), dtype=torch.float32, device=dev)
+        o_folds = self._new_folds(m, dev)
         o_scores = torch.empty(m, dtype=torch.float32, device=dev)
         o_bt = torch.empty((m, P + 2), dtype=torch.int32, device=dev)
-        _lib.check(lib.pyl_prune_materialize(
+        _lib.check(self._fn(lib, "pyl_prune_materialize")(
             pp, cc, ptr(tree.leaves), ptr(tree.folds), ptr(branch), ptr(counts), ptr(off),
             ptr(segc), ptr(c_parent), ptr(c_row), ptr(c_shift), ptr(c_phase), ptr(c_score),
             ptr(surv.contiguous()), m, ptr(o_leaves), ptr(o_folds), ptr(o_scores), ptr(o_bt),
@@ -520,7 +533,7 @@ class Pruning:
         c_score = torch.empty(nv, dtype=torch.float32, device=dev)
         minmax = torch.tensor([np.inf, -np.inf], dtype=torch.float32, device=dev)
         segc = seg.contiguous()
-        _lib.check(lib.pyl_circ_children(
+        _lib.check(self._fn(lib, "pyl_circ_children")(
             pp, qq, cc, ptr(tree.leaves), ptr(tree.folds), ctypes.byref(L), ptr(valid), nv,
             ptr(segc), ptr(c_parent), ptr(c_row), ptr(c_shift), ptr(c_phase), ptr(c_score),
             ptr(minmax), st))
@@ -536,10 +549,10 @@ class Pruning:
         tay = buf["c_region"][valid] == REGION_TAYLOR
         has_taylor[c_parent.long()[tay] // bs] = 1
         o_leaves = torch.empty((m, P + 2, 2), dtype=torch.float64, device=dev)
-        o_folds = torch.empty((m, 2, B), dtype=torch.float32, device=dev)
+        o_folds = self._new_folds(m, dev)
         o_scores = torch.empty(m, dtype=torch.float32, device=dev)
         o_bt = torch.empty((m, P + 2), dtype=i32, device=dev)
-        _lib.check(lib.pyl_circ_materialize(
+        _lib.check(self._fn(lib, "pyl_circ_materialize")(
             pp, qq, cc, ptr(tree.leaves), ptr(tree.folds), ctypes.byref(L), ptr(valid),
             ptr(segc), ptr(c_parent), ptr(c_row), ptr(c_shift), ptr(c_phase), ptr(c_score),
             ptr(has_taylor), ptr(surv), m, ptr(o_leaves), ptr(o_folds), ptr(o_scores),
@@ -559,7 +572,7 @@ class Pruning:
         t0_cur = self.scheme.get_coord(self._prune_level)[0]
         n = tree.size
         B = self.dyp.cfg.nbins
-        folds = torch.empty((n, 2, B), dtype=torch.float32, device=tree.folds.device)
+        folds = self._new_folds(n, tree.folds.device)
         scores = torch.empty(n, dtype=torch.float32, device=tree.folds.device)
         idx = np.ascontiguousarray(seg_idx, dtype=np.int64)
         t0 = np.ascontiguousarray(seg_coords[:, 0], dtype=np.float64)
@@ -569,10 +582,10 @@ class Pruning:
                 ptr(folds), ptr(scores), stream_ptr())
         lib = _lib.load()
         if self._circular:
-            _lib.check(lib.pyl_circ_ascend(ctypes.byref(self._pp), ctypes.byref(self._qq),
-                                           *args))
+            _lib.check(self._fn(lib, "pyl_circ_ascend")(ctypes.byref(self._pp),
+                                                        ctypes.byref(self._qq), *args))
         else:
-            _lib.check(lib.pyl_prune_ascend(ctypes.byref(self._pp), *args))
+            _lib.check(self._fn(lib, "pyl_prune_ascend")(ctypes.byref(self._pp), *args))
         tree.scores_ep = tree.scores.clone()
         tree.folds = folds
         tree.scores = scores
