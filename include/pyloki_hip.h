@@ -227,6 +227,9 @@ typedef struct pyl_prune_coords {
   double scale_cur;       /* coord_cur[1] */
   double t0_next;         /* coord_next[0] */
   double t0_add;          /* coord_add[0] */
+  double t0_prev;         /* coord_prev[0]  (Chebyshev basis only) */
+  double scale_prev;      /* coord_prev[1]  (Chebyshev basis only) */
+  double scale_next;      /* coord_next[1]  (Chebyshev basis only) */
 } pyl_prune_coords;
 
 /* branch_dev: f64 [n_parents][P][2] = {first child centre, child step}; counts_dev int32
@@ -298,6 +301,74 @@ int pyl_prune_ascend(const pyl_prune_params* p, const double* leaves_dev, int64_
                      const float* dyp_fold_dev, int64_t ncoords_ffa, const int64_t* seg_idx,
                      const double* seg_t0, int32_t nseg_used, double t0_cur,
                      float* out_folds_dev, float* out_scores_dev, void* stream);
+
+/* ------------------------------------------------------------------------------------
+ * EP, Chebyshev basis (PrunePolyChebyshevDPFuncts, dyn_poly_cheby.py:587-842; moving grid).
+ * Leaves hold Chebyshev coefficients [alpha_P .. alpha_1, alpha_0, (f0, flag)] with their
+ * cell widths, on the domain (t0, scale) of the current coordinate. One level is the same
+ * three calls as the Taylor basis:
+ *  pyl_cheb_branch      chebyshev.py:72-177 poly_chebyshev_branch_batch: every parent moved
+ *                       from coord_prev to coord_cur (transforms.py:650-700 shift_cheby_full,
+ *                       values and widths by the tiling strategy) into leaves_cur_dev
+ *                       (f64 [n][P+2][2]), then the Taylor split rule with the step
+ *                       eta/nbins * C / f0 for every coefficient (psr_utils.py:258-278);
+ *  pyl_cheb_children    as pyl_prune_children on leaves_cur_dev, resolved by
+ *                       chebyshev.py:180-258 (the series moved to t_add and t_init, then
+ *                       converted to Taylor coefficients, transforms.py:532-560,703-737);
+ *  pyl_cheb_materialize as pyl_prune_materialize on leaves_cur_dev, the survivors moved to
+ *                       coord_next by chebyshev.py:352-368.
+ * pyl_cheb_ascend replaces dyn_poly_cheby.py:733-776 (chebyshev.py:261-349) and needs the
+ * scale of the survivors' domain. The domain-transfer matrices (maths.py:399-557) are built
+ * per call on the host. Arguments otherwise as for the Taylor entry points.
+ * ------------------------------------------------------------------------------------ */
+int pyl_cheb_branch(const pyl_prune_params* p, const pyl_prune_coords* c,
+                    const double* leaves_dev, int64_t n_parents, double* leaves_cur_dev,
+                    double* branch_dev, int32_t* counts_dev, int64_t* nchild_dev,
+                    int64_t* err_dev, void* stream);
+int pyl_cheb_children(const pyl_prune_params* p, const pyl_prune_coords* c,
+                      const double* leaves_cur_dev, const float* folds_dev, int64_t n_parents,
+                      const double* branch_dev, const int32_t* counts_dev,
+                      const int64_t* child_off_dev, int64_t n_children,
+                      const float* seg_fold_dev, int32_t* child_parent_dev,
+                      int32_t* child_row_dev, int32_t* child_shift_dev, double* child_phase_dev,
+                      float* child_score_dev, float* minmax_dev, void* stream);
+int pyl_cheb_materialize(const pyl_prune_params* p, const pyl_prune_coords* c,
+                         const double* leaves_cur_dev, const float* folds_dev,
+                         const double* branch_dev, const int32_t* counts_dev,
+                         const int64_t* child_off_dev, const float* seg_fold_dev,
+                         const int32_t* child_parent_dev, const int32_t* child_row_dev,
+                         const int32_t* child_shift_dev, const double* child_phase_dev,
+                         const float* child_score_dev, const int64_t* surv_dev, int64_t n_surv,
+                         double* out_leaves_dev, float* out_folds_dev, float* out_scores_dev,
+                         int32_t* out_backtracks_dev, void* stream);
+int pyl_cheb_ascend(const pyl_prune_params* p, const double* leaves_dev, int64_t n,
+                    const float* dyp_fold_dev, int64_t ncoords_ffa, const int64_t* seg_idx,
+                    const double* seg_t0, int32_t nseg_used, double t0_cur, double scale_cur,
+                    float* out_folds_dev, float* out_scores_dev, void* stream);
+/* Complex-fold variants (PrunePolyChebyshevComplexDPFuncts, dyn_poly_cheby.py:304-469). */
+int pyl_cheb_children_complex(const pyl_prune_params* p, const pyl_prune_coords* c,
+                              const double* leaves_cur_dev, const float* folds_dev,
+                              int64_t n_parents, const double* branch_dev,
+                              const int32_t* counts_dev, const int64_t* child_off_dev,
+                              int64_t n_children, const float* seg_fold_dev,
+                              int32_t* child_parent_dev, int32_t* child_row_dev,
+                              int32_t* child_shift_dev, double* child_phase_dev,
+                              float* child_score_dev, float* minmax_dev, void* stream);
+int pyl_cheb_materialize_complex(const pyl_prune_params* p, const pyl_prune_coords* c,
+                                 const double* leaves_cur_dev, const float* folds_dev,
+                                 const double* branch_dev, const int32_t* counts_dev,
+                                 const int64_t* child_off_dev, const float* seg_fold_dev,
+                                 const int32_t* child_parent_dev, const int32_t* child_row_dev,
+                                 const int32_t* child_shift_dev, const double* child_phase_dev,
+                                 const float* child_score_dev, const int64_t* surv_dev,
+                                 int64_t n_surv, double* out_leaves_dev, float* out_folds_dev,
+                                 float* out_scores_dev, int32_t* out_backtracks_dev,
+                                 void* stream);
+int pyl_cheb_ascend_complex(const pyl_prune_params* p, const double* leaves_dev, int64_t n,
+                            const float* dyp_fold_dev, int64_t ncoords_ffa,
+                            const int64_t* seg_idx, const double* seg_t0, int32_t nseg_used,
+                            double t0_cur, double scale_cur, float* out_folds_dev,
+                            float* out_scores_dev, void* stream);
 
 /* ------------------------------------------------------------------------------------
  * EP, circular basis (prune_poly_order = 5: crackle, snap, jerk, accel, vel; leaf rows 7).

@@ -171,7 +171,6 @@ def shift_factors(n_params: int, tobs: float, nbins: int) -> np.ndarray:
 
 def branch_batch(leaves: np.ndarray, coord_cur, nbins: int, eta: float, poly_order: int,
                  branch_max: int) -> tuple[np.ndarray, np.ndarray]:
-    n_batch = leaves.shape[0]
     scale = coord_cur[1]
     P = poly_order
     pcur = leaves[:, :-2, 0]
@@ -181,6 +180,17 @@ def branch_batch(leaves: np.ndarray, coord_cur, nbins: int, eta: float, poly_ord
     dnew = dpf[None, :] * C_VAL / f0[:, None]
     fac = shift_factors(P, scale, nbins)
     shift_bins = np.abs(dcur - dnew) * (fac[None, :] * (f0 / C_VAL)[:, None])
+    return odometer_children(pcur, dcur, dnew, shift_bins, eta, branch_max, leaves[:, -2, 0],
+                             f0, leaves[:, -1, 1])
+
+
+def odometer_children(pcur, dcur, dnew, shift_bins, eta: float, branch_max: int, d0, f0,
+                      flag) -> tuple[np.ndarray, np.ndarray]:
+    """``branch_param_padded`` per parameter (``psr_utils.py:347-379``), the keep rule for
+    cells whose shift is below eta, and the odometer cartesian product per parent
+    (``np_utils.py:174-238``). Children rows: branched parameters with their steps, then
+    (d0, 0) and (f0, flag)."""
+    n_batch, P = pcur.shape
     counts = np.ones((n_batch, P), dtype=np.int64)
     dact = np.empty((n_batch, P))
     firsts = np.empty((n_batch, P))
@@ -218,9 +228,9 @@ def branch_batch(leaves: np.ndarray, coord_cur, nbins: int, eta: float, poly_ord
         m = cart.shape[0]
         out[row:row + m, :-2, 0] = cart
         out[row:row + m, :-2, 1] = dact[i]
-        out[row:row + m, -2, 0] = leaves[i, -2, 0]
-        out[row:row + m, -1, 0] = leaves[i, -1, 0]
-        out[row:row + m, -1, 1] = leaves[i, -1, 1]
+        out[row:row + m, -2, 0] = d0[i]
+        out[row:row + m, -1, 0] = f0[i]
+        out[row:row + m, -1, 1] = flag[i]
         row += m
     return out, origins
 
@@ -446,9 +456,12 @@ class LevelStats:
     n_surv: int
 
 
-def prune_level(tree: Tree, fold_segment: np.ndarray, coords: dict, threshold: float,
-                cfg: dict, max_sugg: int, batch_size: int = 1024):
-    """One ``pruning_iteration_batched``. Returns (new tree, stats, survivor child ids)."""
+def level_loop(tree: Tree, fold_segment: np.ndarray, threshold: float, cfg: dict,
+               max_sugg: int, batch_size: int, branch, resolve, transform):
+    """The batch loop of ``pruning_iteration_batched`` (``prune.py:165-332``) for a basis
+    given by ``branch(leaves) -> (children, origins)``, ``resolve(children) -> (idx,
+    phase)`` and ``transform(children) -> leaves at coord_next`` (validation is the
+    identity for the polynomial bases). Returns (new tree, stats, survivor child ids)."""
     P = cfg["poly_order"]
     B = tree.folds.shape[-1]
     buf = TreeBuffer(max_sugg, P, B, tree.folds.dtype)
@@ -460,14 +473,12 @@ def prune_level(tree: Tree, fold_segment: np.ndarray, coords: dict, threshold: f
     child_base = 0
     for b0 in range(0, nb, bs):
         b1 = min(b0 + bs, nb)
-        ch, org = branch_batch(tree.leaves[b0:b1], coords["cur"], cfg["nbins"], cfg["eta"], P,
-                               cfg["branch_max"])
+        ch, org = branch(tree.leaves[b0:b1])
         nl = len(ch)
         n_leaves += nl
         if nl == 0:
             continue
-        idx, phase = resolve_batch(ch, coords["add"], coords["cur"], coords["init"],
-                                   cfg["counts_init"], cfg["param_limits"], cfg["nbins"])
+        idx, phase = resolve(ch)
         seg = seg_rows(fold_segment, idx)
         parents = np.arange(b0, b1)[org]
         comb, scores = _fold_and_score(seg, phase, tree.folds, parents, cfg["widths"])
@@ -479,7 +490,7 @@ def prune_level(tree: Tree, fold_segment: np.ndarray, coords: dict, threshold: f
         if len(fi) == 0:
             child_base += nl
             continue
-        trans = transform_batch(ch[fi], coords["next"], coords["cur"], cfg["tiling"])
+        trans = transform(ch[fi])
         bt = np.empty((len(fi), P + 2), dtype=np.int32)
         bt[:, 0] = parents[fi]
         bt[:, 1: idx.shape[1] + 1] = idx[fi]
@@ -492,6 +503,19 @@ def prune_level(tree: Tree, fold_segment: np.ndarray, coords: dict, threshold: f
     stats = LevelStats(n_leaves, n_leaves, smin if math.isfinite(smin) else 0.0,
                        smax if math.isfinite(smax) else 0.0, v)
     return new, stats, buf.origin[:v].copy()
+
+
+def prune_level(tree: Tree, fold_segment: np.ndarray, coords: dict, threshold: float,
+                cfg: dict, max_sugg: int, batch_size: int = 1024):
+    """One ``pruning_iteration_batched`` with the Taylor plugin."""
+    P = cfg["poly_order"]
+    return level_loop(
+        tree, fold_segment, threshold, cfg, max_sugg, batch_size,
+        lambda lv: branch_batch(lv, coords["cur"], cfg["nbins"], cfg["eta"], P,
+                                cfg["branch_max"]),
+        lambda ch: resolve_batch(ch, coords["add"], coords["cur"], coords["init"],
+                                 cfg["counts_init"], cfg["param_limits"], cfg["nbins"]),
+        lambda ch: transform_batch(ch, coords["next"], coords["cur"], cfg["tiling"]))
 
 
 def ascend(tree: Tree, dyp_fold: np.ndarray, idx_segments, coord_segments, coord_cur,

@@ -101,7 +101,17 @@ SIGNATURES: list[tuple[str, object, list[object]]] = [
       _vp, _vp, _vp]),
     ("pyl_prune_ascend", ctypes.c_int,
      [_vp, _vp, _c_i64, _vp, _c_i64, _i64p, _f64p, _c_i32, _c_dbl, _vp, _vp, _vp]),
+    ("pyl_cheb_branch", ctypes.c_int, [_vp, _vp, _vp, _c_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
 ]
+# Chebyshev children / materialize take the Taylor arguments (the parents moved to the
+# current domain in place of the leaves); the ascend adds the domain's scale.
+_BY_NAME = {name: (rt, args) for name, rt, args in SIGNATURES}
+for _suffix in ("", "_complex"):
+    for _step in ("children", "materialize"):
+        _rt, _args = _BY_NAME[f"pyl_prune_{_step}{_suffix}"]
+        SIGNATURES.append((f"pyl_cheb_{_step}{_suffix}", _rt, list(_args)))
+    _rt, _args = _BY_NAME[f"pyl_prune_ascend{_suffix}"]
+    SIGNATURES.append((f"pyl_cheb_ascend{_suffix}", _rt, [*_args[:9], _c_dbl, *_args[9:]]))
 
 _lib: ctypes.CDLL | None = None
 

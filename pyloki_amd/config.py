@@ -314,20 +314,26 @@ class PulsarSearchConfig:
     ) -> np.ndarray:
         """Expected children per parent at every pruning level (``config.py:758-819``).
 
-        Taylor kinds restate ``generate_bp_poly_taylor`` (``core/taylor.py:414-499``):
-        per final-grid frequency, the children per parent of each level, averaged with
-        weights equal to the running product of earlier branchings.
+        Taylor kinds restate ``generate_bp_poly_taylor`` (``core/taylor.py:414-499``) and
+        Chebyshev kinds ``generate_bp_poly_chebyshev`` (``core/chebyshev.py:458-546``): per
+        final-grid frequency, the children per parent of each level, averaged with weights
+        equal to the running product of earlier branchings.
         """
-        funcs = {"poly_taylor_moving": True, "poly_taylor_fixed": False}
+        funcs = {"poly_taylor_moving": (generate_bp_poly_taylor, True),
+                 "poly_taylor_fixed": (generate_bp_poly_taylor, False),
+                 "poly_chebyshev_moving": (generate_bp_poly_chebyshev, True),
+                 "poly_chebyshev_fixed": (generate_bp_poly_chebyshev, False)}
         if kind not in funcs:
             msg = f"Invalid kind: {kind}"
             raise ValueError(msg)
-        use_moving_grid = funcs[kind]
+        if kind.startswith("poly_chebyshev"):
+            use_cheby_coarsening = True
+        func, use_moving_grid = funcs[kind]
         nsegments = int(np.ceil(self.nsamps / self.bseg_ffa))
         dparams = self.get_dparams(self.niters_ffa, use_cheby_coarsening)
         param_arr = self.get_param_arr(dparams)
         dparams_act = self.get_dparams_actual(self.niters_ffa, use_cheby_coarsening)
-        return generate_bp_poly_taylor(
+        return func(
             param_arr, dparams_act, self.tseg_ffa, nsegments, self.nbins, self.eta, ref_seg,
             use_moving_grid=use_moving_grid, tiling_strategy=self.tiling_strategy,
             use_cheby_coarsening=use_cheby_coarsening,
@@ -408,3 +414,65 @@ def ffa_level_counts(cfg: PulsarSearchConfig) -> list[np.ndarray]:
 
 def ncoords(counts: np.ndarray) -> int:
     return int(math.prod(int(c) for c in counts))
+
+
+def generate_bp_poly_chebyshev(
+    param_arr: list[np.ndarray],
+    dparams_act: np.ndarray,
+    tseg_ffa: float,
+    nsegments: int,
+    nbins: int,
+    eta: float,
+    ref_seg: int,
+    *,
+    use_moving_grid: bool,
+    tiling_strategy: str,
+    use_cheby_coarsening: bool = True,  # noqa: ARG001
+) -> np.ndarray:
+    """Exact Chebyshev branching pattern (``core/chebyshev.py:458-546``), host FP64.
+
+    The cell widths of every Chebyshev coefficient (alpha_0 included) are moved from the
+    previous to the current domain, split where the step ``eta/nbins * C / f0`` is exceeded
+    by a bin, then moved on to the next domain. The reference leaves the alpha_0 width of
+    the split result uninitialised; it is never read and no other width depends on it, so
+    it is kept at 0 here.
+    """
+    from pyloki_amd.utils.misc import FLOAT_EPSILON
+    from pyloki_amd.utils.snail import MiddleOutScheme
+
+    n_params = len(dparams_act)
+    f0 = np.asarray(param_arr[-1], dtype=np.float64)
+    nf = len(f0)
+    scheme = MiddleOutScheme(nsegments, ref_seg, tseg_ffa)
+    weights = np.ones(nf)
+    bp = np.empty(nsegments - 1)
+    dfull = np.zeros((nf, n_params + 1))
+    dfull[:, :n_params] = np.asarray(dparams_act, dtype=np.float64)[None, :]
+    dfull[:, n_params - 1] *= C_VAL / f0
+    dcur = transforms.taylor_to_cheby_errors(dfull, scheme.get_coord(0)[1])
+    dphi = eta / nbins
+    for level in range(1, nsegments):
+        coord_next = scheme.get_coord(level)
+        coord_cur = scheme.get_current_coord(level, use_moving_grid)
+        coord_prev = scheme.get_previous_coord(level, use_moving_grid)
+        dcur = transforms.shift_cheby_errors(dcur, coord_cur, coord_prev, tiling_strategy)
+        dnew = np.empty_like(dcur)
+        for j in range(n_params + 1):
+            dnew[:, j] = dphi * C_VAL / f0
+        shift = np.abs(dcur - dnew) * (nbins * (f0 / C_VAL))[:, None]
+        nbr = np.ones(nf, dtype=np.int64)
+        dnext = np.zeros_like(dcur)
+        for i in range(nf):
+            for j in range(n_params):
+                if shift[i, j] < (eta - FLOAT_EPSILON):
+                    dnext[i, j] = dcur[i, j]
+                    continue
+                num = max(1, math.ceil(dcur[i, j] / dnew[i, j] - FLOAT_EPSILON))
+                nbr[i] *= num
+                dnext[i, j] = dcur[i, j] / num
+        bp[level - 1] = np.sum(weights * nbr) / np.sum(weights)
+        weights = weights * nbr
+        if use_moving_grid:
+            dnext = transforms.shift_cheby_errors(dnext, coord_next, coord_cur, tiling_strategy)
+        dcur = dnext
+    return bp

@@ -20,6 +20,8 @@
 // Taylor translation matrices T[i][j] = dt**(i-j)/(i-j)! are per-level constants; they are
 // built on the host with the binary-powering power (powi) the reference's numba code uses,
 // and the GPU only does the per-leaf FP64 matrix-vector products, in ascending column order.
+#include <vector>
+
 #include "pyl_prune_common.hpp"
 
 namespace {
@@ -70,8 +72,28 @@ __global__ void prune_branch_kernel(LevelConst k, const double* __restrict__ lea
   nchild[i] = total;
 }
 
+// chebyshev.py:72-177, first step: every parent moved from coord_prev to coord_cur
+// (shift_cheby_full, values and widths), thread per parent; f0 and flag copied.
+__global__ void cheb_parents_kernel(LevelConst k, const double* __restrict__ leaves, int64_t n,
+                                    double* __restrict__ leaves_cur) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int P = k.P;
+  const double* lf = leaves + i * (P + 2) * 2;
+  ChildLeaf ch;
+  for (int r = 0; r <= P; ++r) {
+    ch.val[r] = lf[2 * r];
+    ch.err[r] = lf[2 * r + 1];
+  }
+  double* ol = leaves_cur + i * (P + 2) * 2;
+  taylor_full(k, k.tprev, ch, ol);
+  ol[2 * (P + 1)] = lf[2 * (P + 1)];
+  ol[2 * (P + 1) + 1] = lf[2 * (P + 1) + 1];
+}
+
 // Folds are real [2][B] rows or, with kComplex, complex64 [2][B/2 + 1] rows (as float2).
-template <bool kComplex>
+// kCheby: the leaves are Chebyshev coefficients on the current domain (chebyshev.py:180-258).
+template <bool kComplex, bool kCheby>
 __global__ void __launch_bounds__(kBlk)
 prune_children_kernel(LevelConst k, const double* __restrict__ leaves,
                       const float* __restrict__ folds, int64_t n_par,
@@ -96,11 +118,26 @@ prune_children_kernel(LevelConst k, const double* __restrict__ leaves,
     ChildLeaf ch;
     decode_child(k, lf, branch + par * P * 2, counts + par * P, c - off[par], ch);
     const int n = P + 1;
-    const double accel = mat_row(k.tadd, ch.val, n, P - 2);
-    const double v_add = mat_row(k.tadd, ch.val, n, P - 1);
-    const double d_add = mat_row(k.tadd, ch.val, n, P);
-    const double v_init = mat_row(k.tinit, ch.val, n, P - 1);
-    const double d_init = mat_row(k.tinit, ch.val, n, P);
+    double accel, v_add, d_add, v_init, d_init;
+    if constexpr (kCheby) {
+      // cheby_to_taylor_param_shift to t_add and t_init (transforms.py:703-737).
+      double a[kMaxN], da[kMaxN], di[kMaxN];
+      for (int j = 0; j < n; ++j) a[j] = mat_row(k.tadd, ch.val, n, j);
+      cheb_to_taylor(k, a, da);
+      for (int j = 0; j < n; ++j) a[j] = mat_row(k.tinit, ch.val, n, j);
+      cheb_to_taylor(k, a, di);
+      accel = da[P - 2];
+      v_add = da[P - 1];
+      d_add = da[P];
+      v_init = di[P - 1];
+      d_init = di[P];
+    } else {
+      accel = mat_row(k.tadd, ch.val, n, P - 2);
+      v_add = mat_row(k.tadd, ch.val, n, P - 1);
+      d_add = mat_row(k.tadd, ch.val, n, P);
+      v_init = mat_row(k.tinit, ch.val, n, P - 1);
+      d_init = mat_row(k.tinit, ch.val, n, P);
+    }
     const double vel = v_add - v_init;
     const double freq = ch.f0 * (1.0 - vel / pyl::kCVal);
     const double delay = (d_add - d_init) / pyl::kCVal;
@@ -219,11 +256,14 @@ prune_materialize_kernel(LevelConst k, const double* __restrict__ leaves,
   }
 }
 
-template <bool kComplex>
+// kCheby: cseg holds per segment the transposed transfer matrix from the survivors' domain
+// to the segment's reference time (chebyshev.py:261-349); unused otherwise.
+template <bool kComplex, bool kCheby>
 __global__ void __launch_bounds__(kBlk)
 prune_ascend_kernel(LevelConst k, const double* __restrict__ leaves, int64_t n,
                     const float* __restrict__ dyp, int64_t ncoords, const int64_t* __restrict__ segi,
-                    const double* __restrict__ segt, int32_t nseg, double t0_cur,
+                    const double* __restrict__ segt, const double* __restrict__ cseg,
+                    int32_t nseg, double t0_cur,
                     int32_t* __restrict__ scratch_row, double* __restrict__ scratch_phase,
                     float* __restrict__ o_folds, float* __restrict__ o_scores) {
   extern __shared__ float S[];
@@ -242,13 +282,24 @@ prune_ascend_kernel(LevelConst k, const double* __restrict__ leaves, int64_t n,
     const double f0 = lf[(P + 1) * 2];
     for (int s = 0; s < nseg; ++s) {
       const double dt = segt[s] - t0_cur;
-      double T[kMaxN * kMaxN];
-      for (int i = 0; i < nn; ++i)
-        for (int j = 0; j < nn; ++j)
-          T[i * nn + j] = j <= i ? pyl::powi(dt, i - j) / pyl::fact_d(i - j) : 0.0;
-      const double accel = mat_row(T, v, nn, P - 2);
-      const double vel = mat_row(T, v, nn, P - 1);
-      const double d0 = mat_row(T, v, nn, P);
+      double accel, vel, d0;
+      if constexpr (kCheby) {
+        const double* Ct = cseg + (size_t)s * nn * nn;
+        double a[kMaxN], d[kMaxN];
+        for (int j = 0; j < nn; ++j) a[j] = mat_row(Ct, v, nn, j);
+        cheb_to_taylor(k, a, d);
+        accel = d[P - 2];
+        vel = d[P - 1];
+        d0 = d[P];
+      } else {
+        double T[kMaxN * kMaxN];
+        for (int i = 0; i < nn; ++i)
+          for (int j = 0; j < nn; ++j)
+            T[i * nn + j] = j <= i ? pyl::powi(dt, i - j) / pyl::fact_d(i - j) : 0.0;
+        accel = mat_row(T, v, nn, P - 2);
+        vel = mat_row(T, v, nn, P - 1);
+        d0 = mat_row(T, v, nn, P);
+      }
       const double freq = f0 * (1.0 - vel / pyl::kCVal);
       const double delay = d0 / pyl::kCVal;
       const double phase = pyl::phase_idx(dt, f0, B, delay);
@@ -318,7 +369,7 @@ extern "C" int pyl_prune_branch(const pyl_prune_params* p, const pyl_prune_coord
   return PYL_OK;
 }
 
-template <bool kComplex>
+template <bool kComplex, bool kCheby = false>
 static int prune_children(const pyl_prune_params* p, const pyl_prune_coords* c,
                           const double* leaves, const float* folds, int64_t n_par,
                           const double* branch, const int32_t* counts, const int64_t* off,
@@ -326,7 +377,7 @@ static int prune_children(const pyl_prune_params* p, const pyl_prune_coords* c,
                           int32_t* c_shift, double* c_phase, float* c_score, float* minmax,
                           void* stream) {
   LevelConst k;
-  int rc = build_const(p, c, &k);
+  int rc = kCheby ? build_cheb_const(p, c, &k) : build_const(p, c, &k);
   if (rc != PYL_OK) return rc;
   PYL_REQUIRE(c != nullptr, "null pyl_prune_coords");
   PYL_REQUIRE(leaves && folds && branch && counts && off && seg && c_parent && c_row &&
@@ -339,8 +390,8 @@ static int prune_children(const pyl_prune_params* p, const pyl_prune_coords* c,
   if constexpr (kComplex)
     PYL_REQUIRE((k.nbins & (k.nbins - 1)) == 0,
                 "complex folds need a power-of-two nbins (%d)", k.nbins);
-  prune_children_kernel<kComplex><<<(unsigned)pyl::ceil_div(n_child, kBlk), kBlk, lds,
-                                    pyl::as_stream(stream)>>>(
+  prune_children_kernel<kComplex, kCheby><<<(unsigned)pyl::ceil_div(n_child, kBlk), kBlk, lds,
+                                            pyl::as_stream(stream)>>>(
       k, leaves, folds, n_par, branch, counts, off, n_child, seg, c_parent, c_row, c_shift,
       c_phase, c_score, minmax);
   PYL_CHECK_LAUNCH("prune_children_kernel");
@@ -369,7 +420,8 @@ extern "C" int pyl_prune_children_complex(const pyl_prune_params* p, const pyl_p
                               c_parent, c_row, c_shift, c_phase, c_score, minmax, stream);
 }
 
-template <bool kComplex>
+// kCheby only changes the level constants: tnext is then the Chebyshev transfer matrix.
+template <bool kComplex, bool kCheby = false>
 static int prune_materialize(const pyl_prune_params* p, const pyl_prune_coords* c,
                              const double* leaves, const float* folds, const double* branch,
                              const int32_t* counts, const int64_t* off, const float* seg,
@@ -379,7 +431,7 @@ static int prune_materialize(const pyl_prune_params* p, const pyl_prune_coords* 
                              double* o_leaves, float* o_folds, float* o_scores, int32_t* o_bt,
                              void* stream) {
   LevelConst k;
-  int rc = build_const(p, c, &k);
+  int rc = kCheby ? build_cheb_const(p, c, &k) : build_const(p, c, &k);
   if (rc != PYL_OK) return rc;
   PYL_REQUIRE(c != nullptr, "null pyl_prune_coords");
   if (n_surv == 0) return PYL_OK;
@@ -420,16 +472,25 @@ extern "C" int pyl_prune_materialize_complex(
                                  o_folds, o_scores, o_bt, stream);
 }
 
-template <bool kComplex>
+template <bool kComplex, bool kCheby = false>
 static int prune_ascend(const pyl_prune_params* p, const double* leaves, int64_t n,
                         const float* dyp, int64_t ncoords, const int64_t* seg_idx,
                         const double* seg_t0, int32_t nseg, double t0_cur, float* o_folds,
-                        float* o_scores, void* stream) {
+                        float* o_scores, void* stream, double scale_cur = 1.0) {
   LevelConst k;
-  int rc = build_const(p, nullptr, &k);
+  int rc = kCheby ? build_cheb_const(p, nullptr, &k) : build_const(p, nullptr, &k);
   if (rc != PYL_OK) return rc;
   PYL_REQUIRE(nseg >= 1, "no segments to integrate");
   PYL_REQUIRE(leaves && dyp && seg_idx && seg_t0 && o_folds && o_scores, "null pointer");
+  const int nn = k.P + 1;
+  std::vector<double> cseg;
+  if constexpr (kCheby) {
+    PYL_REQUIRE(scale_cur > 0, "ts1 and ts2 must be positive.");
+    for (int m = 0; m < nn; ++m) k.tspow[m] = pyl::powi(scale_cur, m);
+    cseg.resize((size_t)nseg * nn * nn);
+    for (int q = 0; q < nseg; ++q)
+      cheb_matrix_t(k.P, t0_cur, scale_cur, seg_t0[q], scale_cur, cseg.data() + (size_t)q * nn * nn);
+  }
   if (n == 0) return PYL_OK;
   const size_t lds = ep_block_lds<kComplex>(k);
   PYL_REQUIRE(lds <= kMaxEpLds, "nbins too large for the EP scorer (%d)", k.nbins);
@@ -448,9 +509,17 @@ static int prune_ascend(const pyl_prune_params* p, const double* leaves, int64_t
   PYL_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&d_ph), sizeof(double) * n * nseg, s));
   PYL_CHECK_HIP(hipMemcpyAsync(d_idx, seg_idx, sizeof(int64_t) * nseg, hipMemcpyHostToDevice, s));
   PYL_CHECK_HIP(hipMemcpyAsync(d_t0, seg_t0, sizeof(double) * nseg, hipMemcpyHostToDevice, s));
-  prune_ascend_kernel<kComplex><<<(unsigned)pyl::ceil_div(n, kBlk), kBlk, lds, s>>>(
-      k, leaves, n, dyp, ncoords, d_idx, d_t0, nseg, t0_cur, d_row, d_ph, o_folds, o_scores);
+  double* d_c = nullptr;
+  if constexpr (kCheby) {
+    PYL_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&d_c), sizeof(double) * cseg.size(), s));
+    PYL_CHECK_HIP(hipMemcpyAsync(d_c, cseg.data(), sizeof(double) * cseg.size(),
+                                 hipMemcpyHostToDevice, s));
+  }
+  prune_ascend_kernel<kComplex, kCheby><<<(unsigned)pyl::ceil_div(n, kBlk), kBlk, lds, s>>>(
+      k, leaves, n, dyp, ncoords, d_idx, d_t0, d_c, nseg, t0_cur, d_row, d_ph, o_folds,
+      o_scores);
   PYL_CHECK_LAUNCH("prune_ascend_kernel");
+  if (d_c) PYL_CHECK_HIP(hipFreeAsync(d_c, s));
   PYL_CHECK_HIP(hipFreeAsync(d_idx, s));
   PYL_CHECK_HIP(hipFreeAsync(d_t0, s));
   PYL_CHECK_HIP(hipFreeAsync(d_row, s));
@@ -473,4 +542,93 @@ extern "C" int pyl_prune_ascend_complex(const pyl_prune_params* p, const double*
                                         float* o_scores, void* stream) {
   return prune_ascend<true>(p, leaves, n, dyp, ncoords, seg_idx, seg_t0, nseg, t0_cur,
                             o_folds, o_scores, stream);
+}
+
+// ---- Chebyshev basis (dyn_poly_cheby.py): the Taylor level kernels with the Chebyshev
+// level constants, on the parents moved to coord_cur.
+extern "C" int pyl_cheb_branch(const pyl_prune_params* p, const pyl_prune_coords* c,
+                               const double* leaves, int64_t n_par, double* leaves_cur,
+                               double* branch, int32_t* counts, int64_t* nchild, int64_t* err,
+                               void* stream) {
+  LevelConst k;
+  PYL_REQUIRE(c != nullptr, "null pyl_prune_coords");
+  int rc = build_cheb_const(p, c, &k);
+  if (rc != PYL_OK) return rc;
+  PYL_REQUIRE(leaves && leaves_cur && branch && counts && nchild && err, "null device pointer");
+  if (n_par == 0) return PYL_OK;
+  const int threads = 128;
+  const unsigned grid = (unsigned)pyl::ceil_div(n_par, threads);
+  hipStream_t s = pyl::as_stream(stream);
+  cheb_parents_kernel<<<grid, threads, 0, s>>>(k, leaves, n_par, leaves_cur);
+  PYL_CHECK_LAUNCH("cheb_parents_kernel");
+  prune_branch_kernel<<<grid, threads, 0, s>>>(k, leaves_cur, n_par, branch, counts, nchild,
+                                               reinterpret_cast<unsigned long long*>(err));
+  PYL_CHECK_LAUNCH("prune_branch_kernel");
+  return PYL_OK;
+}
+
+extern "C" int pyl_cheb_children(const pyl_prune_params* p, const pyl_prune_coords* c,
+                                 const double* leaves_cur, const float* folds, int64_t n_par,
+                                 const double* branch, const int32_t* counts,
+                                 const int64_t* off, int64_t n_child, const float* seg,
+                                 int32_t* c_parent, int32_t* c_row, int32_t* c_shift,
+                                 double* c_phase, float* c_score, float* minmax, void* stream) {
+  return prune_children<false, true>(p, c, leaves_cur, folds, n_par, branch, counts, off,
+                                     n_child, seg, c_parent, c_row, c_shift, c_phase, c_score,
+                                     minmax, stream);
+}
+
+extern "C" int pyl_cheb_children_complex(const pyl_prune_params* p, const pyl_prune_coords* c,
+                                         const double* leaves_cur, const float* folds,
+                                         int64_t n_par, const double* branch,
+                                         const int32_t* counts, const int64_t* off,
+                                         int64_t n_child, const float* seg, int32_t* c_parent,
+                                         int32_t* c_row, int32_t* c_shift, double* c_phase,
+                                         float* c_score, float* minmax, void* stream) {
+  return prune_children<true, true>(p, c, leaves_cur, folds, n_par, branch, counts, off,
+                                    n_child, seg, c_parent, c_row, c_shift, c_phase, c_score,
+                                    minmax, stream);
+}
+
+extern "C" int pyl_cheb_materialize(const pyl_prune_params* p, const pyl_prune_coords* c,
+                                    const double* leaves_cur, const float* folds,
+                                    const double* branch, const int32_t* counts,
+                                    const int64_t* off, const float* seg,
+                                    const int32_t* c_parent, const int32_t* c_row,
+                                    const int32_t* c_shift, const double* c_phase,
+                                    const float* c_score, const int64_t* surv, int64_t n_surv,
+                                    double* o_leaves, float* o_folds, float* o_scores,
+                                    int32_t* o_bt, void* stream) {
+  return prune_materialize<false, true>(p, c, leaves_cur, folds, branch, counts, off, seg,
+                                        c_parent, c_row, c_shift, c_phase, c_score, surv,
+                                        n_surv, o_leaves, o_folds, o_scores, o_bt, stream);
+}
+
+extern "C" int pyl_cheb_materialize_complex(
+    const pyl_prune_params* p, const pyl_prune_coords* c, const double* leaves_cur,
+    const float* folds, const double* branch, const int32_t* counts, const int64_t* off,
+    const float* seg, const int32_t* c_parent, const int32_t* c_row, const int32_t* c_shift,
+    const double* c_phase, const float* c_score, const int64_t* surv, int64_t n_surv,
+    double* o_leaves, float* o_folds, float* o_scores, int32_t* o_bt, void* stream) {
+  return prune_materialize<true, true>(p, c, leaves_cur, folds, branch, counts, off, seg,
+                                       c_parent, c_row, c_shift, c_phase, c_score, surv, n_surv,
+                                       o_leaves, o_folds, o_scores, o_bt, stream);
+}
+
+extern "C" int pyl_cheb_ascend(const pyl_prune_params* p, const double* leaves, int64_t n,
+                               const float* dyp, int64_t ncoords, const int64_t* seg_idx,
+                               const double* seg_t0, int32_t nseg, double t0_cur,
+                               double scale_cur, float* o_folds, float* o_scores,
+                               void* stream) {
+  return prune_ascend<false, true>(p, leaves, n, dyp, ncoords, seg_idx, seg_t0, nseg, t0_cur,
+                                   o_folds, o_scores, stream, scale_cur);
+}
+
+extern "C" int pyl_cheb_ascend_complex(const pyl_prune_params* p, const double* leaves,
+                                       int64_t n, const float* dyp, int64_t ncoords,
+                                       const int64_t* seg_idx, const double* seg_t0,
+                                       int32_t nseg, double t0_cur, double scale_cur,
+                                       float* o_folds, float* o_scores, void* stream) {
+  return prune_ascend<true, true>(p, leaves, n, dyp, ncoords, seg_idx, seg_t0, nseg, t0_cur,
+                                  o_folds, o_scores, stream, scale_cur);
 }

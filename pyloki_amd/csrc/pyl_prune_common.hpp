@@ -30,6 +30,13 @@ struct LevelConst {
   double dt_add_init;         // t0_add - t0_init
   double amin, fmin, sa, sf;  // accel/freq grid origin and inverse step
   pyl::ScoreConst sc;         // boxcar widths and template constants
+  // Chebyshev basis: tadd / tinit / tnext hold transposed domain-transfer matrices
+  // (cur -> add, cur -> init, cur -> next), tprev the one of prev -> cur; rmat, cfact and
+  // tspow convert coefficients on the current domain to Taylor coefficients.
+  double tprev[kMaxN * kMaxN];
+  double rmat[kMaxN * kMaxN];  // R[k][m]: T_k(x) = sum_m R[k][m] x^m
+  double cfact[kMaxN];         // m!
+  double tspow[kMaxN];         // scale_cur^m
 };
 
 int build_const(const pyl_prune_params* p, const pyl_prune_coords* c, LevelConst* k) {
@@ -81,6 +88,95 @@ int build_const(const pyl_prune_params* p, const pyl_prune_coords* c, LevelConst
   int rc = pyl::build_score_const(p->nbins, widths, p->nw, 1.0, &k->sc);
   if (rc != PYL_OK) return rc;
   return PYL_OK;
+}
+
+// ---- Chebyshev tables (maths.py:399-557), host FP64 in the reference's evaluation order;
+// float-to-integer powers by binary powering.
+inline double nbinom_d(int n, int k) {
+  if (k < 0 || k > n) return 0.0;
+  int64_t r = 1;
+  for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;  // exact: each prefix is binomial
+  return (double)r;
+}
+
+inline double cheb_conn_s(int k, int m) {  // x^k = sum_m S[k][m] T_m(x)
+  if (k < 0 || m < 0 || m > k || (k - m) % 2 != 0) return 0.0;
+  const int n = (k - m) / 2;
+  const int d = m == 0 ? 1 : 0;
+  return std::ldexp(1.0, 1 - k - d) * nbinom_d(k, n);
+}
+
+inline double cheb_conn_r(int k, int m) {  // T_k(x) = sum_m R[k][m] x^m
+  if (k < 0 || m < 0 || m > k || (k - m) % 2 != 0) return 0.0;
+  if (m == 0 && k % 2 == 0) return (k / 2) % 2 == 0 ? 1.0 : -1.0;
+  const int n = (k - m) / 2;
+  const int r = (k + m) / 2;
+  const double sign = n % 2 == 0 ? 1.0 : -1.0;
+  return sign * pyl::powi(2.0, m - 1) * (2.0 * k / (k + m)) * nbinom_d(r, n);
+}
+
+inline double cheb_trans_c(int n, int k, double p, double q) {  // T_n(px+q) = sum_k C T_k
+  if (k > n) return 0.0;
+  double res = 0.0;
+  for (int m = k; m <= n; ++m) {
+    const double r_nm = cheb_conn_r(n, m);
+    double inner = 0.0;
+    for (int i = k; i <= m; ++i)
+      inner += nbinom_d(m, i) * pyl::powi(p, i) * pyl::powi(q, m - i) * cheb_conn_s(i, k);
+    res += r_nm * inner;
+  }
+  return res;
+}
+
+// Transposed descending domain-transfer matrix (poly_chebyshev_transform_matrix with
+// coeff_order 1): out[j * n + i] = C[i][j], so that mat_row(out, alpha, n, j) is column j
+// of alpha @ C summed over i in ascending order.
+inline void cheb_matrix_t(int K, double tc1, double ts1, double tc2, double ts2, double* out) {
+  const int n = K + 1;
+  const double p = ts2 / ts1;
+  const double q = (tc2 - tc1) / ts1;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) out[j * n + i] = cheb_trans_c(K - i, K - j, p, q);
+}
+
+// The Chebyshev tables of one level on top of the Taylor ones: step eta/nbins * C / f0 and
+// shift factor nbins for every coefficient (psr_utils.py:258-278), transfer matrices and
+// the conversion to Taylor coefficients on the current domain.
+int build_cheb_const(const pyl_prune_params* p, const pyl_prune_coords* c, LevelConst* k) {
+  int rc = build_const(p, c, k);
+  if (rc != PYL_OK) return rc;
+  const int P = k->P;
+  const int n = P + 1;
+  if (c) {
+    PYL_REQUIRE(c->scale_cur > 0 && c->scale_prev > 0 && c->scale_next > 0,
+                "ts1 and ts2 must be positive.");
+    const double dphi = p->eta / p->nbins;
+    for (int j = 0; j < P; ++j) {
+      k->dpf[j] = dphi;
+      k->fac[j] = (double)p->nbins;
+    }
+    cheb_matrix_t(P, c->t0_cur, c->scale_cur, c->t0_add, c->scale_cur, k->tadd);
+    cheb_matrix_t(P, c->t0_cur, c->scale_cur, c->t0_init, c->scale_cur, k->tinit);
+    cheb_matrix_t(P, c->t0_cur, c->scale_cur, c->t0_next, c->scale_next, k->tnext);
+    cheb_matrix_t(P, c->t0_prev, c->scale_prev, c->t0_cur, c->scale_cur, k->tprev);
+    for (int m = 0; m < n; ++m) k->tspow[m] = pyl::powi(c->scale_cur, m);
+  }
+  for (int kk = 0; kk < n; ++kk)
+    for (int m = 0; m < n; ++m) k->rmat[kk * n + m] = cheb_conn_r(kk, m);
+  for (int m = 0; m < n; ++m) k->cfact[m] = pyl::fact_d(m);
+  return PYL_OK;
+}
+
+// transforms.py:532-560 cheby_to_taylor of descending coefficients a on the current domain:
+// d_m = ((sum_k a_k R[k][m]) * m!) / scale^m, k ascending, written descending.
+__device__ __forceinline__ void cheb_to_taylor(const LevelConst& k, const double* a,
+                                               double* d) {
+  const int n = k.P + 1;
+  for (int m = 0; m < n; ++m) {
+    double acc = 0.0;
+    for (int kk = 0; kk < n; ++kk) acc += a[n - 1 - kk] * k.rmat[kk * n + m];
+    d[n - 1 - m] = (acc * k.cfact[m]) / k.tspow[m];
+  }
 }
 
 // psr_utils.py:517-554: (accel, freq) grid indices by truncation with a 1e-6 guard.

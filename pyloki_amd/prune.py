@@ -26,6 +26,7 @@ import torch
 
 from pyloki_amd import _lib
 from pyloki_amd.device import ptr, stream_ptr, to_device
+from pyloki_amd.utils import transforms
 from pyloki_amd.utils.misc import C_VAL
 from pyloki_amd.utils.snail import MiddleOutScheme
 
@@ -57,6 +58,9 @@ class PruneCoords(ctypes.Structure):
         ("scale_cur", ctypes.c_double),
         ("t0_next", ctypes.c_double),
         ("t0_add", ctypes.c_double),
+        ("t0_prev", ctypes.c_double),
+        ("scale_prev", ctypes.c_double),
+        ("scale_next", ctypes.c_double),
     ]
 
 
@@ -236,9 +240,13 @@ class Pruning:
                  *, use_moving_grid: bool = True, logger=None) -> None:  # noqa: ARG002
         cfg = dyp.cfg
         self._circular = cfg.prune_poly_order == 5  # prune.py:369-392: circular plugin
-        if poly_basis != "taylor":
+        # (is_circular, poly_basis) pairs with a plugin: circular Taylor, polynomial Taylor,
+        # polynomial Chebyshev (prune.py:373-381).
+        if poly_basis not in ("taylor", "chebyshev") or (self._circular
+                                                         and poly_basis == "chebyshev"):
             msg = f"Invalid pruning configuration: {(self._circular, poly_basis, cfg.use_fourier)}"
             raise ValueError(msg)
+        self._cheby = poly_basis == "chebyshev"
         self._complex = bool(cfg.use_fourier)  # complex64 folds (Fourier domain)
         if not use_moving_grid:
             msg = "only the moving grid is supported"
@@ -333,18 +341,12 @@ class Pruning:
         self._prune_complete = False
         self._prune_level = 0
         P = self._P
-        param_arr = dyp.param_arr
-        dparams = dyp.dparams_actual
-        mesh = np.meshgrid(*param_arr, indexing="ij")
-        cart = np.stack([m.reshape(-1) for m in mesh], axis=1)
-        n = cart.shape[0]
-        leaves = np.zeros((n, P + 2, 2), dtype=np.float64)
-        for k in range(P - 1):
-            leaves[:, k, 0] = cart[:, k]
-            leaves[:, k, 1] = dparams[k]
-        f0 = cart[:, -1]
-        leaves[:, -3, 1] = np.full(n, dparams[-1]) * (C_VAL / f0)
-        leaves[:, -1, 0] = f0
+        if self._cheby:
+            leaves = poly_chebyshev_seed(dyp.param_arr, dyp.dparams_actual, P,
+                                         self.scheme.get_coord(0))
+        else:
+            leaves = poly_taylor_seed(dyp.param_arr, dyp.dparams_actual, P)
+        n = leaves.shape[0]
         folds = self._segment_rows(self.scheme.ref_idx).reshape(n, 2, -1).contiguous()
         from pyloki_amd import kernels
 
@@ -370,8 +372,9 @@ class Pruning:
         c.t0_init = s.get_coord(0)[0]
         cur = s.get_current_coord(level, True)
         c.t0_cur, c.scale_cur = cur
-        c.t0_next = s.get_coord(level)[0]
+        c.t0_next, c.scale_next = s.get_coord(level)
         c.t0_add = s.get_segment_coord(level)[0]
+        c.t0_prev, c.scale_prev = s.get_previous_coord(level, True)
         return c
 
     def execute_iter(self, log_file: Path | None = None) -> None:
@@ -434,8 +437,17 @@ class Pruning:
         counts = torch.empty((n, P), dtype=torch.int32, device=dev)
         nchild = torch.empty(n, dtype=torch.int64, device=dev)
         err = torch.full((1,), INT64_MAX, dtype=torch.int64, device=dev)
-        _lib.check(lib.pyl_prune_branch(pp, cc, ptr(tree.leaves), n, ptr(branch), ptr(counts),
-                                        ptr(nchild), ptr(err), stream_ptr()))
+        if self._cheby:
+            # the parents moved to coord_cur feed the children (chebyshev.py:112-119)
+            src = torch.empty_like(tree.leaves)
+            _lib.check(lib.pyl_cheb_branch(pp, cc, ptr(tree.leaves), n, ptr(src), ptr(branch),
+                                           ptr(counts), ptr(nchild), ptr(err), stream_ptr()))
+            pre = "pyl_cheb"
+        else:
+            src = tree.leaves
+            _lib.check(lib.pyl_prune_branch(pp, cc, ptr(tree.leaves), n, ptr(branch),
+                                            ptr(counts), ptr(nchild), ptr(err), stream_ptr()))
+            pre = "pyl_prune"
         off = torch.zeros(n + 1, dtype=torch.int64, device=dev)
         torch.cumsum(nchild, 0, out=off[1:])
         errv = int(err.item())
@@ -451,8 +463,8 @@ class Pruning:
         c_score = torch.empty(total, dtype=torch.float32, device=dev)
         minmax = torch.tensor([np.inf, -np.inf], dtype=torch.float32, device=dev)
         segc = seg.contiguous()
-        _lib.check(self._fn(lib, "pyl_prune_children")(
-            pp, cc, ptr(tree.leaves), ptr(tree.folds), n, ptr(branch), ptr(counts), ptr(off),
+        _lib.check(self._fn(lib, f"{pre}_children")(
+            pp, cc, ptr(src), ptr(tree.folds), n, ptr(branch), ptr(counts), ptr(off),
             total, ptr(segc), ptr(c_parent), ptr(c_row), ptr(c_shift), ptr(c_phase),
             ptr(c_score), ptr(minmax), stream_ptr()))
         surv = self._survivors(c_score, lambda: off.cpu().numpy(), threshold)
@@ -461,8 +473,8 @@ class Pruning:
         o_folds = self._new_folds(m, dev)
         o_scores = torch.empty(m, dtype=torch.float32, device=dev)
         o_bt = torch.empty((m, P + 2), dtype=torch.int32, device=dev)
-        _lib.check(self._fn(lib, "pyl_prune_materialize")(
-            pp, cc, ptr(tree.leaves), ptr(tree.folds), ptr(branch), ptr(counts), ptr(off),
+        _lib.check(self._fn(lib, f"{pre}_materialize")(
+            pp, cc, ptr(src), ptr(tree.folds), ptr(branch), ptr(counts), ptr(off),
             ptr(segc), ptr(c_parent), ptr(c_row), ptr(c_shift), ptr(c_phase), ptr(c_score),
             ptr(surv.contiguous()), m, ptr(o_leaves), ptr(o_folds), ptr(o_scores), ptr(o_bt),
             stream_ptr()))
@@ -569,7 +581,7 @@ class Pruning:
             return
         tree = self._world_tree
         seg_idx, seg_coords = self.scheme.get_segment_coords_so_far(self._prune_level)
-        t0_cur = self.scheme.get_coord(self._prune_level)[0]
+        t0_cur, scale_cur = self.scheme.get_coord(self._prune_level)
         n = tree.size
         B = self.dyp.cfg.nbins
         folds = self._new_folds(n, tree.folds.device)
@@ -584,6 +596,9 @@ class Pruning:
         if self._circular:
             _lib.check(self._fn(lib, "pyl_circ_ascend")(ctypes.byref(self._pp),
                                                         ctypes.byref(self._qq), *args))
+        elif self._cheby:
+            _lib.check(self._fn(lib, "pyl_cheb_ascend")(ctypes.byref(self._pp), *args[:8],
+                                                        scale_cur, *args[8:]))
         else:
             _lib.check(self._fn(lib, "pyl_prune_ascend")(ctypes.byref(self._pp), *args))
         tree.scores_ep = tree.scores.clone()
@@ -603,7 +618,12 @@ class Pruning:
         if asc is None or self._prune_level not in asc:
             self._do_ascend()
         leaves = self._world_tree.leaves.cpu().numpy()
-        report = poly_taylor_report_batch(leaves) if len(leaves) else leaves
+        if not len(leaves):
+            report = leaves
+        elif self._cheby:
+            report = poly_chebyshev_report_batch(leaves, self.scheme.get_report_coord(True))
+        else:
+            report = poly_taylor_report_batch(leaves)
         res = {
             "run_name": f"{ref_seg:03d}",
             "scheme": self.scheme.data.copy(),
@@ -617,6 +637,34 @@ class Pruning:
         return res
 
 
+def poly_taylor_seed(param_arr: list[np.ndarray], dparams: np.ndarray,
+                     poly_order: int) -> np.ndarray:
+    """Seed leaves from the last FFA grid (``taylor.py:14-65``, ``common.py:14-33``):
+    cartesian product (first parameter slowest), ``v = 0`` with width ``df C / f0``,
+    ``d0 = 0``, then (f0, flag 0)."""
+    mesh = np.meshgrid(*param_arr, indexing="ij")
+    cart = np.stack([m.reshape(-1) for m in mesh], axis=1)
+    n = cart.shape[0]
+    P = poly_order
+    leaves = np.zeros((n, P + 2, 2), dtype=np.float64)
+    for k in range(P - 1):
+        leaves[:, k, 0] = cart[:, k]
+        leaves[:, k, 1] = dparams[k]
+    f0 = cart[:, -1]
+    leaves[:, -3, 1] = np.full(n, dparams[-1]) * (C_VAL / f0)
+    leaves[:, -1, 0] = f0
+    return leaves
+
+
+def poly_chebyshev_seed(param_arr: list[np.ndarray], dparams: np.ndarray, poly_order: int,
+                        coord_init: tuple[float, float]) -> np.ndarray:
+    """``chebyshev.py:14-69``: the Taylor seed cells as Chebyshev coefficients on the
+    reference segment's domain (values and widths)."""
+    leaves = poly_taylor_seed(param_arr, dparams, poly_order)
+    leaves[:, :-1] = transforms.taylor_to_cheby_full(leaves[:, :-1], coord_init[1])
+    return leaves
+
+
 def poly_taylor_report_batch(leaves: np.ndarray) -> np.ndarray:
     """Gauge transform to physical units with error propagation (``taylor.py:342-360``)."""
     out = leaves.copy()
@@ -624,6 +672,28 @@ def poly_taylor_report_batch(leaves: np.ndarray) -> np.ndarray:
     sigs = leaves[:, :-3, 1]
     v = leaves[:, -3, 0]
     dv = leaves[:, -3, 1]
+    f0 = leaves[:, -1, 0]
+    s = 1 - v / C_VAL
+    out[:, :-3, 0] = vals / s[:, None]
+    out[:, :-3, 1] = np.sqrt((sigs / s[:, None]) ** 2
+                             + ((vals / (C_VAL * s[:, None] ** 2)) ** 2) * (dv[:, None] ** 2))
+    out[:, -3, 0] = f0 * s
+    out[:, -3, 1] = f0 * dv / C_VAL
+    return out
+
+
+def poly_chebyshev_report_batch(leaves: np.ndarray,
+                                coord_report: tuple[float, float]) -> np.ndarray:
+    """Chebyshev coefficients to physical parameters (``chebyshev.py:371-398``): Taylor
+    coefficients on the report domain, d0 into its row, then the Taylor gauge transform."""
+    out = leaves.copy()
+    d = transforms.cheby_to_taylor_full(leaves[:, :-1, :], coord_report[1])
+    vals = d[:, :-2, 0]
+    sigs = d[:, :-2, 1]
+    out[:, -2, 0] = d[:, -1, 0]
+    out[:, -2, 1] = d[:, -1, 1]
+    v = d[:, -2, 0]
+    dv = d[:, -2, 1]
     f0 = leaves[:, -1, 0]
     s = 1 - v / C_VAL
     out[:, :-3, 0] = vals / s[:, None]
