@@ -490,6 +490,37 @@ int pyl_circ_ascend(const pyl_prune_params* p, const pyl_circ_params* q,
                     int32_t nseg_used, double t0_cur, float* out_folds_dev,
                     float* out_scores_dev, void* stream);
 
+/* ------------------------------------------------------------------------------------
+ * Learned branch ranking (no counterpart in the reference). A small 1-D CNN scores EP
+ * survivors from their folds and 8 parameter features so that a budgeted, best-first
+ * search keeps the most promising ones; the exhaustive EP above stays the judge.
+ * Network: profile e/sqrt(v) (irfft first for complex folds) standardised over the bins;
+ * three circular Conv1d layers (32 channels, kernel 5, ReLU); max over bins; with the
+ * features, a 32-unit ReLU layer and one logit. Parameters are one flat FP32 vector of
+ * pyl_cnn_num_params() values: W1[32][1][5], b1[32], W2[32][32][5], b2[32], W3[32][32][5],
+ * b3[32], Wf1[32][40], bf1[32], Wf2[32], bf2[1]. nbins must be 32, 64 or 128.
+ * Loss: sum_n w_pos y softplus(-logit) + w_neg (1 - y) softplus(logit).
+ * ------------------------------------------------------------------------------------ */
+int pyl_cnn_num_params(void);
+/* Floats of training cache per sample (-1 for an unsupported nbins). */
+int64_t pyl_cnn_cache_floats(int nbins);
+/* folds_dev: f32 [n][2][nbins], or complex64 [n][2][nbins/2 + 1] with complex_folds = 1;
+ * feats_dev f32 [n][8]; logits_dev f32 [n]. cache_dev (nullable): f32
+ * [n][pyl_cnn_cache_floats(nbins)], written for pyl_cnn_backward. */
+int pyl_cnn_forward(const float* params_dev, const float* folds_dev, int64_t n, int nbins,
+                    int complex_folds, const float* feats_dev, float* logits_dev,
+                    float* cache_dev, void* stream);
+/* Gradient of the loss (summed over the n samples) into grad_dev [num_params], from the
+ * cache and logits of pyl_cnn_forward; labels_dev f32 [n] in {0, 1}. work_dev holds the
+ * per-block partial sums (at least num_params floats; more blocks with more room). */
+int pyl_cnn_backward(const float* params_dev, const float* cache_dev, const float* logits_dev,
+                     const float* labels_dev, int64_t n, int nbins, float w_pos, float w_neg,
+                     float* work_dev, int64_t work_floats, float* grad_dev, void* stream);
+/* One Adam step (bias-corrected; step counts from 1). */
+int pyl_cnn_adam(float* params_dev, const float* grad_dev, float* m_dev, float* v_dev,
+                 int64_t nparams, float lr, float beta1, float beta2, float eps, int step,
+                 void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -102,6 +102,16 @@ SIGNATURES: list[tuple[str, object, list[object]]] = [
     ("pyl_prune_ascend", ctypes.c_int,
      [_vp, _vp, _c_i64, _vp, _c_i64, _i64p, _f64p, _c_i32, _c_dbl, _vp, _vp, _vp]),
     ("pyl_cheb_branch", ctypes.c_int, [_vp, _vp, _vp, _c_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("pyl_cnn_num_params", ctypes.c_int, []),
+    ("pyl_cnn_cache_floats", ctypes.c_int64, [_c_i32]),
+    ("pyl_cnn_forward", ctypes.c_int,
+     [_vp, _vp, _c_i64, _c_i32, _c_i32, _vp, _vp, _vp, _vp]),
+    ("pyl_cnn_backward", ctypes.c_int,
+     [_vp, _vp, _vp, _vp, _c_i64, _c_i32, ctypes.c_float, ctypes.c_float, _vp, _c_i64, _vp,
+      _vp]),
+    ("pyl_cnn_adam", ctypes.c_int,
+     [_vp, _vp, _vp, _vp, _c_i64, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+      ctypes.c_float, _c_i32, _vp]),
 ]
 # Chebyshev children / materialize take the Taylor arguments (the parents moved to the
 # current domain in place of the leaves); the ascend adds the domain's scale.

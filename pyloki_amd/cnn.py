@@ -1,0 +1,208 @@
+"""Learned branch ranking for a budgeted EP search.
+
+This part has no counterpart in the reference. A small 1-D CNN (three circular Conv1d
+layers, 32 channels, kernel 5, on a survivor's 32/64/128-bin profile, plus 8 parameter
+features) predicts whether an EP survivor is an ancestor of a final candidate. The
+budgeted search keeps, at every level, the ``budget`` survivors it ranks highest, so the
+levels that follow branch fewer parents; the exhaustive search is the judge (recall =
+fraction of its final candidates that the budgeted search also returns).
+
+The network runs and trains only through the HIP library (``pyl_cnn_forward``,
+``pyl_cnn_backward``, ``pyl_cnn_adam``); torch provides device memory.
+"""
+
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from pyloki_amd import _lib
+from pyloki_amd.device import ptr, stream_ptr
+
+NFEAT = 8
+_SHAPES = [("W1", (32, 1, 5)), ("b1", (32,)), ("W2", (32, 32, 5)), ("b2", (32,)),
+           ("W3", (32, 32, 5)), ("b3", (32,)), ("Wf1", (32, 40)), ("bf1", (32,)),
+           ("Wf2", (32,)), ("bf2", (1,))]
+
+
+def init_params(seed: int = 0) -> np.ndarray:
+    """He-normal weights, zero biases, in the library's flat order (FP32)."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for name, shape in _SHAPES:
+        if name.startswith("b"):
+            parts.append(np.zeros(shape))
+            continue
+        fan_in = int(np.prod(shape[1:])) if len(shape) > 1 else 32
+        parts.append(rng.normal(0.0, np.sqrt(2.0 / fan_in), shape))
+    return np.concatenate([p.reshape(-1) for p in parts]).astype(np.float32)
+
+
+def leaf_features(leaves: torch.Tensor, scores: torch.Tensor, level: int,
+                  nlevels: int) -> torch.Tensor:
+    """[n][8]: S/N / 10, level / nlevels, then log10(1 + cell width) / 10 of the branched
+    parameters (highest order first, at most 6)."""
+    n = leaves.shape[0]
+    P = leaves.shape[1] - 2
+    f = torch.zeros((n, NFEAT), dtype=torch.float32, device=leaves.device)
+    f[:, 0] = scores.float() / 10.0
+    f[:, 1] = float(level) / float(max(nlevels, 1))
+    k = min(P, NFEAT - 2)
+    f[:, 2:2 + k] = (torch.log10(1.0 + leaves[:, :k, 1].abs()) / 10.0).float()
+    return f.contiguous()
+
+
+class BranchRanker:
+    """Parameters and Adam state of the ranking network on the current GPU."""
+
+    def __init__(self, nbins: int, params: np.ndarray | None = None, seed: int = 0) -> None:
+        lib = _lib.load()
+        self.nparams = int(lib.pyl_cnn_num_params())
+        if lib.pyl_cnn_cache_floats(nbins) < 0:
+            msg = f"the ranking network takes 32, 64 or 128 bins (got {nbins})"
+            raise ValueError(msg)
+        self.nbins = int(nbins)
+        p = init_params(seed) if params is None else np.asarray(params, dtype=np.float32)
+        if p.shape != (self.nparams,):
+            msg = f"expected {self.nparams} parameters, got {p.shape}"
+            raise ValueError(msg)
+        dev = torch.device("cuda")
+        self.params = torch.from_numpy(p.copy()).to(dev)
+        self.m = torch.zeros_like(self.params)
+        self.v = torch.zeros_like(self.params)
+        self.step = 0
+        self._work = torch.empty(self.nparams * 2048, dtype=torch.float32, device=dev)
+        self.grad = torch.zeros_like(self.params)
+
+    def logits(self, folds: torch.Tensor, feats: torch.Tensor,
+               cache: torch.Tensor | None = None) -> torch.Tensor:
+        lib = _lib.load()
+        n = int(folds.shape[0])
+        cplx = bool(folds.is_complex())
+        out = torch.empty(n, dtype=torch.float32, device=folds.device)
+        f = folds.contiguous()
+        ft = feats.contiguous().float()
+        _lib.check(lib.pyl_cnn_forward(ptr(self.params), ptr(f), n, self.nbins, int(cplx),
+                                       ptr(ft), ptr(out), ptr(cache), stream_ptr()))
+        return out
+
+    def train_step(self, folds: torch.Tensor, feats: torch.Tensor, labels: torch.Tensor,
+                   lr: float = 1e-3, w_pos: float = 1.0, w_neg: float = 1.0,
+                   beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-8) -> float:
+        """One Adam step on the mean weighted BCE of the batch; returns that loss."""
+        lib = _lib.load()
+        n = int(folds.shape[0])
+        stride = int(lib.pyl_cnn_cache_floats(self.nbins))
+        cache = torch.empty(n * stride, dtype=torch.float32, device=folds.device)
+        lg = self.logits(folds, feats, cache)
+        y = labels.contiguous().float()
+        _lib.check(lib.pyl_cnn_backward(ptr(self.params), ptr(cache), ptr(lg), ptr(y), n,
+                                        self.nbins, w_pos / n, w_neg / n, ptr(self._work),
+                                        int(self._work.numel()), ptr(self.grad),
+                                        stream_ptr()))
+        self.step += 1
+        _lib.check(lib.pyl_cnn_adam(ptr(self.params), ptr(self.grad), ptr(self.m),
+                                    ptr(self.v), self.nparams, lr, beta1, beta2, eps,
+                                    self.step, stream_ptr()))
+        loss = (w_pos * y * torch.nn.functional.softplus(-lg)
+                + w_neg * (1 - y) * torch.nn.functional.softplus(lg)).mean()
+        return float(loss.item())
+
+    def state(self) -> dict:
+        return {"params": self.params.cpu().numpy(), "nbins": self.nbins}
+
+    @classmethod
+    def from_state(cls, state: dict) -> BranchRanker:
+        return cls(int(state["nbins"]), params=np.asarray(state["params"]))
+
+
+# ----------------------------------------------------------------------- data from EP runs
+@dataclass
+class LevelRecord:
+    level: int
+    folds: torch.Tensor
+    feats: torch.Tensor
+    backtracks: torch.Tensor
+
+
+def record_levels(pr, ref_seg: int) -> list[LevelRecord]:
+    """Run an exhaustive search level by level, keeping every level's survivors."""
+    nlev = pr.dyp.nsegments - 1
+    pr.initialize(ref_seg)
+    recs = []
+    for _ in range(nlev):
+        pr.execute_iter()
+        t = pr.world_tree
+        if t.size == 0:
+            break
+        recs.append(LevelRecord(pr.prune_level, t.folds.clone(),
+                                leaf_features(t.leaves, t.scores, pr.prune_level, nlev),
+                                t.backtracks.clone()))
+    return recs
+
+
+def ancestor_labels(recs: list[LevelRecord]) -> list[torch.Tensor]:
+    """1 for the survivors of each level whose subtree reaches the last level's tree."""
+    labels = [None] * len(recs)
+    if not recs:
+        return labels
+    dev = recs[-1].folds.device
+    cur = torch.ones(recs[-1].folds.shape[0], dtype=torch.float32, device=dev)
+    labels[-1] = cur
+    for k in range(len(recs) - 1, 0, -1):
+        prev = torch.zeros(recs[k - 1].folds.shape[0], dtype=torch.float32, device=dev)
+        parents = recs[k].backtracks[:, 0].long()
+        prev[parents[cur > 0]] = 1.0
+        labels[k - 1] = prev
+        cur = prev
+    return labels
+
+
+def budget_hook(ranker: BranchRanker, budget: int):
+    """A ``Pruning`` level hook that keeps the ``budget`` best-ranked survivors."""
+
+    def hook(pr, tree):
+        if tree.size <= budget:
+            return tree
+        nlev = pr.dyp.nsegments - 1
+        lg = ranker.logits(tree.folds, leaf_features(tree.leaves, tree.scores, pr.prune_level,
+                                                     nlev))
+        keep = torch.topk(lg, budget, sorted=False).indices.sort().values
+        return type(tree)(tree.leaves[keep].contiguous(), tree.folds[keep].contiguous(),
+                          tree.scores[keep].contiguous(), tree.backtracks[keep].contiguous())
+
+    return hook
+
+
+def _final_keys(leaves: np.ndarray) -> set[bytes]:
+    return {r.tobytes() for r in np.ascontiguousarray(leaves)}
+
+
+def compare_searches(pr_factory, ref_seg: int, ranker: BranchRanker, budget: int) -> dict:
+    """Exhaustive vs budgeted run of the same search: recall of the exhaustive final
+    candidates, leaves scored (trials) and wall time of each."""
+    out = {}
+    for name, hook in (("exhaustive", None), ("budgeted", budget_hook(ranker, budget))):
+        pr = pr_factory()
+        pr.level_hook = hook
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pr.initialize(ref_seg)
+        for _ in range(pr.dyp.nsegments - 1):
+            pr.execute_iter()
+            if pr.prune_complete:
+                break
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        out[name] = {"time_s": dt, "trials": int(sum(s.n_leaves for s in pr.pstats[1:])),
+                     "final": pr.world_tree.leaves.cpu().numpy()}
+    ex, bu = _final_keys(out["exhaustive"]["final"]), _final_keys(out["budgeted"]["final"])
+    recall = len(ex & bu) / len(ex) if ex else 1.0
+    return {"recall": recall, "n_final_exhaustive": len(ex), "n_final_budgeted": len(bu),
+            "trials_exhaustive": out["exhaustive"]["trials"],
+            "trials_budgeted": out["budgeted"]["trials"],
+            "time_exhaustive_s": out["exhaustive"]["time_s"],
+            "time_budgeted_s": out["budgeted"]["time_s"]}

@@ -1,0 +1,473 @@
+// Learned branch ranking for the EP search: a small 1-D CNN over a survivor's folded
+// profile plus 8 parameter features, trained on the GPU with these kernels. It has no
+// counterpart in the reference; the budgeted search keeps the survivors it ranks highest.
+//
+// Network (one sample; B bins, B in {32, 64, 128}):
+//   x  = (p - mean p) / (std p + 1e-6), p = e / sqrt(v) (irfft of complex rows first);
+//   a1 = relu(conv(x;  W1[32][1][5], b1)), a2 = relu(conv(a1; W2[32][32][5], b2)),
+//   a3 = relu(conv(a2; W3[32][32][5], b3)), conv circular: y[c][p] = b[c] +
+//        sum_{i,t} W[c][i][t] x[i][(p + t - 2) mod B];
+//   h[c] = max_p a3[c][p] (first maximum); z = (h, features[8]);
+//   u = relu(Wf1[32][40] z + bf1); logit = Wf2[32] . u + bf2.
+// Loss: sum_n w_pos y softplus(-logit) + w_neg (1 - y) softplus(logit).
+//
+// Work decomposition: a block of 256 threads holds 256 / B samples, one thread per
+// (sample, bin). The conv layers keep the block's activations in LDS (32 KB) and each
+// thread accumulates its 32 output channels in registers; weights are read with
+// block-uniform indices (scalar loads, no LDS). The backward kernel walks tiles of samples
+// grid-stride; every thread owns a fixed set of weight-gradient outputs, accumulated in
+// registers over the block's tiles, and writes them once as the block's partial sum; a
+// second kernel adds the partials in block order, so the gradient is deterministic.
+#include <algorithm>
+#include <cmath>
+
+#include "pyl_common.hpp"
+#include "pyl_score.hpp"
+
+namespace {
+
+constexpr int kC = 32, kK = 5, kF = 8, kH = 32, kZ = kC + kF;
+constexpr int kThreads = 256;
+constexpr int oW1 = 0, oB1 = oW1 + kC * kK, oW2 = oB1 + kC, oB2 = oW2 + kC * kC * kK;
+constexpr int oW3 = oB2 + kC, oB3 = oW3 + kC * kC * kK, oF1 = oB3 + kC, oBF1 = oF1 + kH * kZ;
+constexpr int oF2 = oBF1 + kH, oBF2 = oF2 + kH, kNParams = oBF2 + 1;
+constexpr int kConvOut = kC * kC * kK;            // 5120 weights per 32 -> 32 layer
+constexpr int kPerThread = kConvOut / kThreads;   // 20
+constexpr int kF1PerThread = (kH * kZ + kThreads - 1) / kThreads;  // 5
+static_assert(kConvOut % kThreads == 0, "conv weight split");
+
+// Per-sample training cache (floats): a1 [kC][B], a2 [kC][B], x0 [B], argmax [kC] (int32
+// bits), z [kZ], q [kH].
+__host__ __device__ constexpr int64_t cache_stride(int B) {
+  return 2 * (int64_t)kC * B + B + kC + kZ + kH;
+}
+__host__ __device__ constexpr int64_t off_a2(int B) { return (int64_t)kC * B; }
+__host__ __device__ constexpr int64_t off_x0(int B) { return 2 * (int64_t)kC * B; }
+__host__ __device__ constexpr int64_t off_arg(int B) { return off_x0(B) + B; }
+__host__ __device__ constexpr int64_t off_z(int B) { return off_arg(B) + kC; }
+__host__ __device__ constexpr int64_t off_q(int B) { return off_z(B) + kZ; }
+
+// One circular conv layer for thread (s, p): acc[co] = b[co] + sum_ci sum_t
+// W[co][ci][t] in[s][ci][(p + t - 2) & (B - 1)], ci and t ascending.
+template <int B, int CIN>
+__device__ __forceinline__ void conv_layer(const float* __restrict__ W,
+                                           const float* __restrict__ bias,
+                                           const float* in, int p, float* acc) {
+#pragma unroll
+  for (int co = 0; co < kC; ++co) acc[co] = bias[co];
+#pragma unroll 1
+  for (int ci = 0; ci < CIN; ++ci) {
+    float v[kK];
+#pragma unroll
+    for (int t = 0; t < kK; ++t) v[t] = in[ci * B + ((p + t - 2) & (B - 1))];
+#pragma unroll
+    for (int co = 0; co < kC; ++co) {
+      const float* w = W + (co * CIN + ci) * kK;
+      float a = acc[co];
+#pragma unroll
+      for (int t = 0; t < kK; ++t) a += w[t] * v[t];
+      acc[co] = a;
+    }
+  }
+}
+
+template <int B, bool kComplex, bool kTrain>
+__global__ void __launch_bounds__(kThreads)
+cnn_forward_kernel(const float* __restrict__ th, const float* __restrict__ folds, int64_t n,
+                   const float* __restrict__ feats, float* __restrict__ logits,
+                   float* __restrict__ cache) {
+  constexpr int S = kThreads / B;
+  __shared__ float A[S][kC][B];
+  __shared__ float X[S][B];
+  __shared__ float stat[S][2];
+  __shared__ float Z[S][kZ];
+  __shared__ float U[S][kH];
+  __shared__ float tw[kComplex ? 2 * B : 1];
+  const int tid = threadIdx.x;
+  const int s = tid / B, p = tid % B;
+  const int64_t i = (int64_t)blockIdx.x * S + s;
+  const bool valid = i < n;
+  if constexpr (kComplex) {
+    pyl::fill_twiddles(tw, B, tid, kThreads);
+    __syncthreads();
+  }
+  float pv = 0.0f;
+  if (valid) {
+    float e, v;
+    if constexpr (kComplex) {
+      const float2* row = reinterpret_cast<const float2*>(folds) + i * 2 * (B / 2 + 1);
+      e = pyl::irfft_bin(row, B, p, tw);
+      v = pyl::irfft_bin(row + (B / 2 + 1), B, p, tw);
+    } else {
+      e = folds[i * 2 * B + p];
+      v = folds[i * 2 * B + B + p];
+    }
+    pv = e / sqrtf(v);
+  }
+  X[s][p] = pv;
+  __syncthreads();
+  if (p == 0) {
+    float sum = 0.0f;
+#pragma unroll 1
+    for (int q = 0; q < B; ++q) sum += X[s][q];
+    const float mu = sum / (float)B;
+    float ss = 0.0f;
+    for (int q = 0; q < B; ++q) {
+      const float d = X[s][q] - mu;
+      ss += d * d;
+    }
+    stat[s][0] = mu;
+    stat[s][1] = sqrtf(ss / (float)B);
+  }
+  __syncthreads();
+  const float x0 = valid ? (pv - stat[s][0]) / (stat[s][1] + 1e-6f) : 0.0f;
+  __syncthreads();
+  X[s][p] = x0;
+  float* cs = cache + (kTrain && valid ? i * cache_stride(B) : 0);
+  if constexpr (kTrain) {
+    if (valid) cs[off_x0(B) + p] = x0;
+  }
+  __syncthreads();
+  float acc[kC];
+  conv_layer<B, 1>(th + oW1, th + oB1, &X[s][0], p, acc);
+#pragma unroll
+  for (int co = 0; co < kC; ++co) {
+    const float a = fmaxf(acc[co], 0.0f);
+    A[s][co][p] = a;
+    if constexpr (kTrain) {
+      if (valid) cs[co * B + p] = a;
+    }
+  }
+  __syncthreads();
+  conv_layer<B, kC>(th + oW2, th + oB2, &A[s][0][0], p, acc);
+  __syncthreads();
+#pragma unroll
+  for (int co = 0; co < kC; ++co) {
+    const float a = fmaxf(acc[co], 0.0f);
+    A[s][co][p] = a;
+    if constexpr (kTrain) {
+      if (valid) cs[off_a2(B) + co * B + p] = a;
+    }
+  }
+  __syncthreads();
+  conv_layer<B, kC>(th + oW3, th + oB3, &A[s][0][0], p, acc);
+  __syncthreads();
+#pragma unroll
+  for (int co = 0; co < kC; ++co) A[s][co][p] = fmaxf(acc[co], 0.0f);
+  __syncthreads();
+  // Max-pool over bins (first maximum), then the features.
+  if (p < kC) {
+    float best = A[s][p][0];
+    int arg = 0;
+#pragma unroll 1
+    for (int q = 1; q < B; ++q) {
+      const float a = A[s][p][q];
+      if (a > best) {
+        best = a;
+        arg = q;
+      }
+    }
+    Z[s][p] = best;
+    if constexpr (kTrain) {
+      if (valid) reinterpret_cast<int32_t*>(cs + off_arg(B))[p] = arg;
+    }
+  }
+  if (p < kF) Z[s][kC + p] = valid ? feats[i * kF + p] : 0.0f;
+  __syncthreads();
+  if (p < kH) {
+    float q = th[oBF1 + p];
+    for (int z = 0; z < kZ; ++z) q += th[oF1 + p * kZ + z] * Z[s][z];
+    U[s][p] = fmaxf(q, 0.0f);
+    if constexpr (kTrain) {
+      if (valid) cs[off_q(B) + p] = q;
+    }
+  }
+  if constexpr (kTrain) {
+    for (int z = p; z < kZ; z += B)
+      if (valid) cs[off_z(B) + z] = Z[s][z];
+  }
+  __syncthreads();
+  if (p == 0 && valid) {
+    float lg = th[oBF2];
+    for (int j = 0; j < kH; ++j) lg += th[oF2 + j] * U[s][j];
+    logits[i] = lg;
+  }
+}
+
+template <int B>
+__global__ void __launch_bounds__(kThreads)
+cnn_backward_kernel(const float* __restrict__ th, const float* __restrict__ cache,
+                    const float* __restrict__ logits, const float* __restrict__ labels,
+                    int64_t n, float w_pos, float w_neg, float* __restrict__ partial) {
+  constexpr int S = kThreads / B;
+  __shared__ float Aa[S][kC][B];  // a2, then a1
+  __shared__ float D[S][kC][B];   // dz3, dz2, dz1
+  __shared__ float X[S][B];
+  __shared__ float DQ[S][kH], ZZ[S][kZ], UU[S][kH], DH[S][kC], DL[S];
+  __shared__ int ARG[S][kC];
+  const int tid = threadIdx.x;
+  const int s = tid / B, p = tid % B;
+  float g3[kPerThread], g2[kPerThread], gf1[kF1PerThread];
+#pragma unroll
+  for (int r = 0; r < kPerThread; ++r) g3[r] = g2[r] = 0.0f;
+#pragma unroll
+  for (int r = 0; r < kF1PerThread; ++r) gf1[r] = 0.0f;
+  float g1 = 0.0f, gb1 = 0.0f, gb2 = 0.0f, gb3 = 0.0f, gbf1 = 0.0f, gf2 = 0.0f, gbf2 = 0.0f;
+  const int64_t ntiles = (n + S - 1) / S;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t i = tile * S + s;
+    const bool valid = i < n;
+    const float* cs = cache + (valid ? i * cache_stride(B) : 0);
+    if (p < kH) UU[s][p] = valid ? fmaxf(cs[off_q(B) + p], 0.0f) : 0.0f;
+    for (int z = p; z < kZ; z += B) ZZ[s][z] = valid ? cs[off_z(B) + z] : 0.0f;
+    if (p < kC) ARG[s][p] = valid ? reinterpret_cast<const int32_t*>(cs + off_arg(B))[p] : 0;
+    if (p == 0) {
+      float dl = 0.0f;
+      if (valid) {
+        const float lg = logits[i];
+        const float y = labels[i];
+        const float sig = 1.0f / (1.0f + expf(-lg));
+        dl = w_pos * y * (sig - 1.0f) + w_neg * (1.0f - y) * sig;
+      }
+      DL[s] = dl;
+    }
+    __syncthreads();
+    if (p < kH) {
+      const float q = valid ? cs[off_q(B) + p] : 0.0f;
+      DQ[s][p] = q > 0.0f ? DL[s] * th[oF2 + p] : 0.0f;
+    }
+    __syncthreads();
+    if (p < kC) {
+      float dh = 0.0f;
+      for (int j = 0; j < kH; ++j) dh += DQ[s][j] * th[oF1 + j * kZ + p];
+      DH[s][p] = ZZ[s][p] > 0.0f ? dh : 0.0f;  // relu mask of a3 at its maximum
+    }
+    __syncthreads();
+    for (int c = 0; c < kC; ++c) {
+      D[s][c][p] = (p == ARG[s][c]) ? DH[s][c] : 0.0f;
+      Aa[s][c][p] = valid ? cs[off_a2(B) + c * B + p] : 0.0f;
+    }
+    __syncthreads();
+    // Head gradients.
+    if (tid < kH) {
+      for (int ss = 0; ss < S; ++ss) {
+        gf2 += DL[ss] * UU[ss][tid];
+        gbf1 += DQ[ss][tid];
+      }
+    }
+    if (tid == 0)
+      for (int ss = 0; ss < S; ++ss) gbf2 += DL[ss];
+#pragma unroll
+    for (int r = 0; r < kF1PerThread; ++r) {
+      const int o = tid + kThreads * r;
+      if (o < kH * kZ) {
+        const int j = o / kZ, z = o % kZ;
+        for (int ss = 0; ss < S; ++ss) gf1[r] += DQ[ss][j] * ZZ[ss][z];
+      }
+    }
+    // conv3: weight gradients from dz3 and a2, then da2 -> dz2 (kept in registers).
+#pragma unroll
+    for (int r = 0; r < kPerThread; ++r) {
+      const int o = tid + kThreads * r;
+      const int co = o / (kC * kK), rem = o % (kC * kK), ci = rem / kK, t = rem % kK;
+      float a = g3[r];
+#pragma unroll 1
+      for (int ss = 0; ss < S; ++ss)
+#pragma unroll 4
+        for (int q = 0; q < B; ++q) a += D[ss][co][q] * Aa[ss][ci][(q + t - 2) & (B - 1)];
+      g3[r] = a;
+    }
+    if (tid < kC)
+      for (int ss = 0; ss < S; ++ss)
+#pragma unroll 1
+        for (int q = 0; q < B; ++q) gb3 += D[ss][tid][q];
+    float dz[kC];
+#pragma unroll
+    for (int ci = 0; ci < kC; ++ci) dz[ci] = 0.0f;
+#pragma unroll 1
+    for (int co = 0; co < kC; ++co) {
+#pragma unroll
+      for (int t = 0; t < kK; ++t) {
+        const float d = D[s][co][(p - t + 2) & (B - 1)];
+#pragma unroll
+        for (int ci = 0; ci < kC; ++ci) dz[ci] += th[oW3 + (co * kC + ci) * kK + t] * d;
+      }
+    }
+#pragma unroll
+    for (int ci = 0; ci < kC; ++ci) dz[ci] = Aa[s][ci][p] > 0.0f ? dz[ci] : 0.0f;
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < kC; ++c) {
+      D[s][c][p] = dz[c];
+      Aa[s][c][p] = valid ? cs[c * B + p] : 0.0f;  // a1
+    }
+    __syncthreads();
+    // conv2: weight gradients from dz2 and a1, then da1 -> dz1.
+#pragma unroll
+    for (int r = 0; r < kPerThread; ++r) {
+      const int o = tid + kThreads * r;
+      const int co = o / (kC * kK), rem = o % (kC * kK), ci = rem / kK, t = rem % kK;
+      float a = g2[r];
+#pragma unroll 1
+      for (int ss = 0; ss < S; ++ss)
+#pragma unroll 4
+        for (int q = 0; q < B; ++q) a += D[ss][co][q] * Aa[ss][ci][(q + t - 2) & (B - 1)];
+      g2[r] = a;
+    }
+    if (tid < kC)
+      for (int ss = 0; ss < S; ++ss)
+#pragma unroll 1
+        for (int q = 0; q < B; ++q) gb2 += D[ss][tid][q];
+#pragma unroll
+    for (int ci = 0; ci < kC; ++ci) dz[ci] = 0.0f;
+#pragma unroll 1
+    for (int co = 0; co < kC; ++co) {
+#pragma unroll
+      for (int t = 0; t < kK; ++t) {
+        const float d = D[s][co][(p - t + 2) & (B - 1)];
+#pragma unroll
+        for (int ci = 0; ci < kC; ++ci) dz[ci] += th[oW2 + (co * kC + ci) * kK + t] * d;
+      }
+    }
+#pragma unroll
+    for (int ci = 0; ci < kC; ++ci) dz[ci] = Aa[s][ci][p] > 0.0f ? dz[ci] : 0.0f;
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < kC; ++c) D[s][c][p] = dz[c];
+    X[s][p] = valid ? cs[off_x0(B) + p] : 0.0f;
+    __syncthreads();
+    // conv1: weight gradients from dz1 and x0.
+    if (tid < kC * kK) {
+      const int co = tid / kK, t = tid % kK;
+      for (int ss = 0; ss < S; ++ss)
+#pragma unroll 1
+        for (int q = 0; q < B; ++q) g1 += D[ss][co][q] * X[ss][(q + t - 2) & (B - 1)];
+    }
+    if (tid < kC)
+      for (int ss = 0; ss < S; ++ss)
+#pragma unroll 1
+        for (int q = 0; q < B; ++q) gb1 += D[ss][tid][q];
+    __syncthreads();
+  }
+  float* out = partial + (int64_t)blockIdx.x * kNParams;
+#pragma unroll
+  for (int r = 0; r < kPerThread; ++r) {
+    out[oW3 + tid + kThreads * r] = g3[r];
+    out[oW2 + tid + kThreads * r] = g2[r];
+  }
+#pragma unroll
+  for (int r = 0; r < kF1PerThread; ++r) {
+    const int o = tid + kThreads * r;
+    if (o < kH * kZ) out[oF1 + o] = gf1[r];
+  }
+  if (tid < kC * kK) out[oW1 + tid] = g1;
+  if (tid < kC) {
+    out[oB1 + tid] = gb1;
+    out[oB2 + tid] = gb2;
+    out[oB3 + tid] = gb3;
+  }
+  if (tid < kH) {
+    out[oBF1 + tid] = gbf1;
+    out[oF2 + tid] = gf2;
+  }
+  if (tid == 0) out[oBF2] = gbf2;
+}
+
+__global__ void cnn_reduce_kernel(const float* __restrict__ partial, int nblk,
+                                  float* __restrict__ grad) {
+  const int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= kNParams) return;
+  float a = 0.0f;
+  for (int b = 0; b < nblk; ++b) a += partial[(int64_t)b * kNParams + o];
+  grad[o] = a;
+}
+
+__global__ void cnn_adam_kernel(float* __restrict__ th, const float* __restrict__ g,
+                                float* __restrict__ m, float* __restrict__ v, int64_t np,
+                                float lr, float b1, float b2, float eps, float c1, float c2) {
+  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= np) return;
+  const float gi = g[o];
+  const float mi = b1 * m[o] + (1.0f - b1) * gi;
+  const float vi = b2 * v[o] + (1.0f - b2) * gi * gi;
+  m[o] = mi;
+  v[o] = vi;
+  th[o] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+template <int B>
+int launch_forward(const float* th, const float* folds, int64_t n, const float* feats,
+                   float* logits, float* cache, int complex_folds, hipStream_t s) {
+  constexpr int S = kThreads / B;
+  const unsigned grid = (unsigned)pyl::ceil_div(n, S);
+  if (complex_folds) {
+    if (cache) cnn_forward_kernel<B, true, true><<<grid, kThreads, 0, s>>>(th, folds, n, feats, logits, cache);
+    else cnn_forward_kernel<B, true, false><<<grid, kThreads, 0, s>>>(th, folds, n, feats, logits, cache);
+  } else {
+    if (cache) cnn_forward_kernel<B, false, true><<<grid, kThreads, 0, s>>>(th, folds, n, feats, logits, cache);
+    else cnn_forward_kernel<B, false, false><<<grid, kThreads, 0, s>>>(th, folds, n, feats, logits, cache);
+  }
+  PYL_CHECK_LAUNCH("cnn_forward_kernel");
+  return PYL_OK;
+}
+
+bool valid_bins(int nbins) { return nbins == 32 || nbins == 64 || nbins == 128; }
+
+}  // namespace
+
+extern "C" int pyl_cnn_num_params(void) { return kNParams; }
+
+extern "C" int64_t pyl_cnn_cache_floats(int nbins) {
+  return valid_bins(nbins) ? cache_stride(nbins) : -1;
+}
+
+extern "C" int pyl_cnn_forward(const float* params, const float* folds, int64_t n, int nbins,
+                               int complex_folds, const float* feats, float* logits,
+                               float* cache, void* stream) {
+  PYL_REQUIRE(valid_bins(nbins), "the ranking network takes 32, 64 or 128 bins (got %d)", nbins);
+  PYL_REQUIRE(n >= 0, "negative sample count");
+  if (n == 0) return PYL_OK;
+  PYL_REQUIRE(params && folds && feats && logits, "null device pointer");
+  hipStream_t s = pyl::as_stream(stream);
+  switch (nbins) {
+    case 32: return launch_forward<32>(params, folds, n, feats, logits, cache, complex_folds, s);
+    case 64: return launch_forward<64>(params, folds, n, feats, logits, cache, complex_folds, s);
+    default: return launch_forward<128>(params, folds, n, feats, logits, cache, complex_folds, s);
+  }
+}
+
+extern "C" int pyl_cnn_backward(const float* params, const float* cache, const float* logits,
+                                const float* labels, int64_t n, int nbins, float w_pos,
+                                float w_neg, float* work, int64_t work_floats, float* grad,
+                                void* stream) {
+  PYL_REQUIRE(valid_bins(nbins), "the ranking network takes 32, 64 or 128 bins (got %d)", nbins);
+  PYL_REQUIRE(params && cache && logits && labels && work && grad, "null device pointer");
+  PYL_REQUIRE(work_floats >= kNParams, "workspace must hold at least %d floats", kNParams);
+  hipStream_t s = pyl::as_stream(stream);
+  const int S = kThreads / nbins;
+  const int64_t tiles = std::max<int64_t>(1, (n + S - 1) / S);
+  const int nblk = (int)std::min<int64_t>({tiles, (int64_t)2048, work_floats / kNParams});
+  switch (nbins) {
+    case 32: cnn_backward_kernel<32><<<nblk, kThreads, 0, s>>>(params, cache, logits, labels, n, w_pos, w_neg, work); break;
+    case 64: cnn_backward_kernel<64><<<nblk, kThreads, 0, s>>>(params, cache, logits, labels, n, w_pos, w_neg, work); break;
+    default: cnn_backward_kernel<128><<<nblk, kThreads, 0, s>>>(params, cache, logits, labels, n, w_pos, w_neg, work); break;
+  }
+  PYL_CHECK_LAUNCH("cnn_backward_kernel");
+  cnn_reduce_kernel<<<(unsigned)pyl::ceil_div(kNParams, 256), 256, 0, s>>>(work, nblk, grad);
+  PYL_CHECK_LAUNCH("cnn_reduce_kernel");
+  return PYL_OK;
+}
+
+extern "C" int pyl_cnn_adam(float* params, const float* grad, float* m, float* v,
+                            int64_t nparams, float lr, float beta1, float beta2, float eps,
+                            int step, void* stream) {
+  PYL_REQUIRE(params && grad && m && v, "null device pointer");
+  PYL_REQUIRE(step >= 1, "Adam step counts from 1");
+  if (nparams <= 0) return PYL_OK;
+  const float c1 = 1.0f - (float)std::pow((double)beta1, step);
+  const float c2 = 1.0f - (float)std::pow((double)beta2, step);
+  cnn_adam_kernel<<<(unsigned)pyl::ceil_div(nparams, 256), 256, 0, pyl::as_stream(stream)>>>(
+      params, grad, m, v, nparams, lr, beta1, beta2, eps, c1, c2);
+  PYL_CHECK_LAUNCH("cnn_adam_kernel");
+  return PYL_OK;
+}

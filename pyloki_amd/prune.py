@@ -289,6 +289,9 @@ class Pruning:
             self._qq = qq
         self._widths = widths
         self._ncoords = int(np.prod(counts))
+        # Called as level_hook(self, tree) -> tree after every level (no reference
+        # counterpart): lets a budgeted search thin the survivors before the next level.
+        self.level_hook = None
 
     @property
     def dyp(self):
@@ -388,6 +391,8 @@ class Pruning:
         tree = self._world_tree
         new_tree, stats = self._level(tree, self._segment_rows(seg_idx), self._coords(level),
                                       threshold)
+        if self.level_hook is not None and new_tree.size:
+            new_tree = self.level_hook(self, new_tree)  # e.g. a learned budget (cnn.py)
         st = PruneStats(level=level, seg_idx=seg_idx, threshold=threshold,
                         n_branches=tree.size, n_leaves_surv=new_tree.size, **stats)
         self._pstats.append(st)
