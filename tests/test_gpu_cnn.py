@@ -141,7 +141,8 @@ def test_cnn_adam_step(gpu_device):
 
 
 def test_cnn_training_learns_pulses(gpu_device):
-    """Pulse (label 1) against noise (label 0) profiles; features carry nothing."""
+    """Pulse (label 1, matched-filter S/N 6) against noise (label 0) profiles; the
+    features carry nothing."""
     _case("cnn training")
     B = 64
     rng = np.random.default_rng(9)
@@ -150,14 +151,14 @@ def test_cnn_training_learns_pulses(gpu_device):
     y = (rng.random(n) < 0.5).astype(np.float32)
     ph = rng.integers(0, B, n)
     for k in np.where(y > 0)[0]:
-        e[k, (ph[k] + np.arange(4)) % B] += 2.0
+        e[k, (ph[k] + np.arange(4)) % B] += 3.0
     folds = _dev(np.stack([e, np.ones((n, B))], axis=1).astype(np.float32))
     feats = torch.zeros((n, 8), dtype=torch.float32, device="cuda")
     yd = _dev(y)
     r = cnn.BranchRanker(B, seed=1)
     first = r.train_step(folds, feats, yd, lr=2e-3)
-    for _ in range(150):
+    for _ in range(200):
         last = r.train_step(folds, feats, yd, lr=2e-3)
     acc = float(((r.logits(folds, feats) > 0).float() == yd).float().mean().item())
     print(f"[result] training loss {first:.3f} -> {last:.3f}, accuracy {acc:.3f}", flush=True)
-    assert last < 0.5 * first and acc > 0.95
+    assert last < 0.3 * first and acc > 0.97
