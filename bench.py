@@ -61,6 +61,9 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--nseries", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-series", type=int, default=2)
+    ap.add_argument("--workload", choices=["ffa", "ep-circular-cnn"], default="ffa",
+                    help="ffa: the headline FFA batch (default); ep-circular-cnn: the "
+                         "circular-orbit EP with a learned survivor budget vs exhaustive")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device 4 times (HBM counter calibration)")
     return ap.parse_args()
@@ -127,10 +130,82 @@ def cpu_baseline(series, nsample: int) -> dict:
     }
 
 
+# Circular-orbit EP with the learned survivor budget: N = 2^24 samples at 64 us, 128 FFA
+# segments (127 EP levels), f in [249, 251] Hz, p_orb_min = 6 h, m_c_max = 0.7 Msun
+# (x up to 1.5 lt-s), nbins 64; an injected 250 Hz pulsar (5 % duty, S/N 15) in a 6 h
+# orbit with x = 1.5 lt-s and a seeded random orbital phase per seed.
+EP_CNN = {"nsamps": 2**24, "tsamp": 64e-6, "nseg": 128, "nbins": 64, "freq": (249.0, 251.0),
+          "p_orb": 6 * 3600.0, "x_orb": 1.5, "m_c_max": 0.7, "m_p_min": 1.4, "snr": 15.0,
+          "thresholds": (0.5, 8.0), "max_sugg": 2**21, "budgets": (2**16, 2**14),
+          "train_seeds": (1, 2, 3, 4), "test_seed": 9, "epochs": 16}
+
+
+def ep_circular_dyp(seed: int, w: dict = EP_CNN):
+    from pyloki_amd.config import ParamLimits, PulsarSearchConfig
+    from pyloki_amd.ffa import DynamicProgramming
+    from pyloki_amd.simulation.pulse import PulseSignalConfig
+
+    n = w["nsamps"]
+    lim = ParamLimits.from_circular(w["freq"], p_orb_min=w["p_orb"], m_c_max=w["m_c_max"],
+                                    m_p_min=w["m_p_min"]).limits
+    cfg = PulsarSearchConfig(nsamps=n, tsamp=w["tsamp"], nbins=w["nbins"], eta=1.0,
+                             param_limits=lim, bseg_brute=n // 1024, bseg_ffa=n // w["nseg"],
+                             prune_poly_order=5, p_orb_min=w["p_orb"], m_c_max=w["m_c_max"],
+                             m_p_min=w["m_p_min"], ducy_max=0.3, wtsp=1.5, branch_max=256)
+    rng = np.random.default_rng(seed)
+    f0 = 0.5 * (w["freq"][0] + w["freq"][1]) * (1 + 1e-4 * rng.uniform(-1, 1))
+    ts = PulseSignalConfig(period=1 / f0, dt=w["tsamp"], nsamps=n, snr=w["snr"], ducy=0.05,
+                           mod_kwargs={"p_orb": w["p_orb"], "x_orb": w["x_orb"],
+                                       "phi": float(rng.uniform(0, 2 * np.pi))}
+                           ).generate(seed=seed)
+    dyp = DynamicProgramming(ts, cfg)
+    dyp.initialize()
+    dyp.execute()
+    return dyp
+
+
+def ep_circular_cnn() -> int:
+    import torch
+
+    from pyloki_amd import cnn
+
+    if not torch.cuda.is_available():
+        print("no ROCm device visible", file=sys.stderr)
+        return 1
+    w = EP_CNN
+    thr = np.linspace(w["thresholds"][0], w["thresholds"][1], w["nseg"] - 1)
+    res = cnn.budget_experiment(ep_circular_dyp, w["train_seeds"], w["test_seed"], thr,
+                                w["max_sugg"], w["budgets"], epochs=w["epochs"])
+    b = min(w["budgets"])
+    r = res[f"cnn_{b}"]
+    ex = res["exhaustive"]
+    line = {
+        "metric": "EP recall of the exhaustive candidates under a learned survivor budget",
+        "value": r["recall"], "unit": "fraction", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": r["time_s"] * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "ep_circular_cnn", "nsamps": w["nsamps"], "nseg": w["nseg"],
+                   "nbins": w["nbins"], "prune_poly_order": 5, "budget": b,
+                   "max_sugg": w["max_sugg"], "thresholds": list(w["thresholds"]),
+                   "train_seeds": list(w["train_seeds"]), "test_seed": w["test_seed"]},
+        "trial_ratio": r["trial_ratio"], "speedup": r["speedup"],
+        "exhaustive": ex, "exhaustive_trials_per_s": ex["trials"] / ex["time_s"],
+        "runs": {k: v for k, v in res.items() if k not in ("exhaustive", "train")},
+        "train": res["train"],
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     args = parse()
     if args.copy_calibration:
         return copy_calibration()
+    if args.workload == "ep-circular-cnn":
+        if args.gpus != 1:
+            print("bench.py measures one GPU only", file=sys.stderr)
+            return 2
+        return ep_circular_cnn()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the FFA batch "
               "would shard by series across GPUs with no communication.", file=sys.stderr)

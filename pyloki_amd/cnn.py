@@ -43,15 +43,23 @@ def init_params(seed: int = 0) -> np.ndarray:
 
 def leaf_features(leaves: torch.Tensor, scores: torch.Tensor, level: int,
                   nlevels: int) -> torch.Tensor:
-    """[n][8]: S/N / 10, level / nlevels, then log10(1 + cell width) / 10 of the branched
-    parameters (highest order first, at most 6)."""
+    """[n][8] features of one level's survivors: S/N / 10, level / nlevels, the S/N's
+    z-score and rank fraction within the level, then log10(1 + cell width) / 10 of the
+    four lowest-order branched parameters."""
     n = leaves.shape[0]
     P = leaves.shape[1] - 2
     f = torch.zeros((n, NFEAT), dtype=torch.float32, device=leaves.device)
-    f[:, 0] = scores.float() / 10.0
+    sc = scores.float()
+    f[:, 0] = sc / 10.0
     f[:, 1] = float(level) / float(max(nlevels, 1))
-    k = min(P, NFEAT - 2)
-    f[:, 2:2 + k] = (torch.log10(1.0 + leaves[:, :k, 1].abs()) / 10.0).float()
+    if n > 1:
+        f[:, 2] = (sc - sc.mean()) / (sc.std() + 1e-6)
+        rank = torch.empty(n, dtype=torch.float32, device=leaves.device)
+        rank[torch.argsort(sc)] = torch.arange(n, dtype=torch.float32,
+                                               device=leaves.device) / (n - 1)
+        f[:, 3] = rank
+    k = min(P, NFEAT - 4)
+    f[:, 4:4 + k] = (torch.log10(1.0 + leaves[:, P - k:P, 1].abs()) / 10.0).float()
     return f.contiguous()
 
 
@@ -161,8 +169,56 @@ def ancestor_labels(recs: list[LevelRecord]) -> list[torch.Tensor]:
     return labels
 
 
+def training_set(recs: list[LevelRecord], labels: list[torch.Tensor],
+                 neg_per_level: int = 2048, seed: int = 0):
+    """Every positive survivor and up to ``neg_per_level`` random negatives per level."""
+    dev = recs[0].folds.device
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    fo, fe, la = [], [], []
+    for r, lab in zip(recs, labels):
+        pos = torch.nonzero(lab > 0).reshape(-1)
+        neg = torch.nonzero(lab == 0).reshape(-1)
+        if len(neg) > neg_per_level:
+            neg = neg[torch.randperm(len(neg), device=dev, generator=g)[:neg_per_level]]
+        idx = torch.cat([pos, neg])
+        fo.append(r.folds[idx])
+        fe.append(r.feats[idx])
+        la.append(lab[idx])
+    return torch.cat(fo), torch.cat(fe), torch.cat(la)
+
+
+def train(ranker: BranchRanker, folds: torch.Tensor, feats: torch.Tensor,
+          labels: torch.Tensor, epochs: int = 16, batch: int = 4096, lr: float = 1e-3,
+          seed: int = 0) -> list[float]:
+    """Mini-batch Adam on the class-balanced loss (positives weighted by the ratio of
+    negatives to positives); returns the mean loss of every epoch."""
+    n = int(folds.shape[0])
+    npos = int((labels > 0).sum().item())
+    w_pos = float(n - npos) / max(npos, 1)
+    g = torch.Generator(device=folds.device)
+    g.manual_seed(seed)
+    hist = []
+    for _ in range(epochs):
+        perm = torch.randperm(n, device=folds.device, generator=g)
+        tot, nb = 0.0, 0
+        for b0 in range(0, n, batch):
+            idx = perm[b0:b0 + batch]
+            tot += ranker.train_step(folds[idx], feats[idx], labels[idx], lr=lr, w_pos=w_pos)
+            nb += 1
+        hist.append(tot / max(nb, 1))
+    return hist
+
+
+def _keep_top(tree, key: torch.Tensor, budget: int):
+    keep = torch.topk(key, budget, sorted=False).indices.sort().values
+    return type(tree)(tree.leaves[keep].contiguous(), tree.folds[keep].contiguous(),
+                      tree.scores[keep].contiguous(), tree.backtracks[keep].contiguous())
+
+
 def budget_hook(ranker: BranchRanker, budget: int):
-    """A ``Pruning`` level hook that keeps the ``budget`` best-ranked survivors."""
+    """A ``Pruning`` level hook that keeps the ``budget`` survivors the network ranks
+    highest (in their original order)."""
 
     def hook(pr, tree):
         if tree.size <= budget:
@@ -170,9 +226,16 @@ def budget_hook(ranker: BranchRanker, budget: int):
         nlev = pr.dyp.nsegments - 1
         lg = ranker.logits(tree.folds, leaf_features(tree.leaves, tree.scores, pr.prune_level,
                                                      nlev))
-        keep = torch.topk(lg, budget, sorted=False).indices.sort().values
-        return type(tree)(tree.leaves[keep].contiguous(), tree.folds[keep].contiguous(),
-                          tree.scores[keep].contiguous(), tree.backtracks[keep].contiguous())
+        return _keep_top(tree, lg, budget)
+
+    return hook
+
+
+def score_budget_hook(budget: int):
+    """The baseline: keep the ``budget`` survivors of highest S/N."""
+
+    def hook(pr, tree):
+        return tree if tree.size <= budget else _keep_top(tree, tree.scores, budget)
 
     return hook
 
@@ -181,28 +244,75 @@ def _final_keys(leaves: np.ndarray) -> set[bytes]:
     return {r.tobytes() for r in np.ascontiguousarray(leaves)}
 
 
-def compare_searches(pr_factory, ref_seg: int, ranker: BranchRanker, budget: int) -> dict:
-    """Exhaustive vs budgeted run of the same search: recall of the exhaustive final
-    candidates, leaves scored (trials) and wall time of each."""
-    out = {}
-    for name, hook in (("exhaustive", None), ("budgeted", budget_hook(ranker, budget))):
-        pr = pr_factory()
-        pr.level_hook = hook
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        pr.initialize(ref_seg)
-        for _ in range(pr.dyp.nsegments - 1):
-            pr.execute_iter()
-            if pr.prune_complete:
-                break
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        out[name] = {"time_s": dt, "trials": int(sum(s.n_leaves for s in pr.pstats[1:])),
-                     "final": pr.world_tree.leaves.cpu().numpy()}
-    ex, bu = _final_keys(out["exhaustive"]["final"]), _final_keys(out["budgeted"]["final"])
-    recall = len(ex & bu) / len(ex) if ex else 1.0
-    return {"recall": recall, "n_final_exhaustive": len(ex), "n_final_budgeted": len(bu),
-            "trials_exhaustive": out["exhaustive"]["trials"],
-            "trials_budgeted": out["budgeted"]["trials"],
-            "time_exhaustive_s": out["exhaustive"]["time_s"],
-            "time_budgeted_s": out["budgeted"]["time_s"]}
+def _timed_run(pr, ref_seg: int, hook) -> dict:
+    pr.level_hook = hook
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pr.initialize(ref_seg)
+    for _ in range(pr.dyp.nsegments - 1):
+        pr.execute_iter()
+        if pr.prune_complete:
+            break
+    torch.cuda.synchronize()
+    return {"time_s": time.perf_counter() - t0,
+            "trials": int(sum(s.n_leaves for s in pr.pstats[1:])),
+            "final": _final_keys(pr.world_tree.leaves.cpu().numpy())}
+
+
+def compare_searches(pr_factory, ref_seg: int, hooks: dict) -> dict:
+    """The exhaustive run and one run per named level hook of the same search. For each
+    hook: recall of the exhaustive final candidates, leaves scored (trials), wall time,
+    and the ratios to the exhaustive run."""
+    ex = _timed_run(pr_factory(), ref_seg, None)
+    out = {"exhaustive": {"trials": ex["trials"], "time_s": ex["time_s"],
+                          "n_final": len(ex["final"])}}
+    for name, hook in hooks.items():
+        r = _timed_run(pr_factory(), ref_seg, hook)
+        common = len(ex["final"] & r["final"])
+        recall = common / len(ex["final"]) if ex["final"] else 1.0
+        precision = common / len(r["final"]) if r["final"] else 1.0
+        out[name] = {"recall": recall, "precision": precision, "trials": r["trials"],
+                     "time_s": r["time_s"],
+                     "n_final": len(r["final"]),
+                     "trial_ratio": ex["trials"] / max(r["trials"], 1),
+                     "speedup": ex["time_s"] / r["time_s"]}
+    return out
+
+
+def budget_experiment(make_dyp, train_seeds, test_seed: int, thresholds: np.ndarray,
+                      max_sugg: int, budgets, epochs: int = 16, batch_size: int = 1024,
+                      nbins: int = 64, seed: int = 0) -> dict:
+    """Train the network on exhaustive runs of ``make_dyp(s)`` for the training seeds,
+    then compare the exhaustive run of the test seed with the learned and the S/N budget
+    at every budget. The reference segment is the middle one."""
+    from pyloki_amd.prune import Pruning
+
+    t0 = time.perf_counter()
+    parts = []
+    for s in train_seeds:
+        dyp = make_dyp(s)
+        pr = Pruning(dyp, thresholds, max_sugg=max_sugg, batch_size=batch_size)
+        recs = record_levels(pr, dyp.nsegments // 2)
+        if recs:
+            parts.append(training_set(recs, ancestor_labels(recs), seed=seed + s))
+    folds = torch.cat([p[0] for p in parts])
+    feats = torch.cat([p[1] for p in parts])
+    labels = torch.cat([p[2] for p in parts])
+    t_data = time.perf_counter() - t0
+    ranker = BranchRanker(nbins, seed=seed)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist = train(ranker, folds, feats, labels, epochs=epochs, seed=seed)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    dyp = make_dyp(test_seed)
+    hooks = {}
+    for b in budgets:
+        hooks[f"cnn_{b}"] = budget_hook(ranker, b)
+        hooks[f"snr_{b}"] = score_budget_hook(b)
+    res = compare_searches(lambda: Pruning(dyp, thresholds, max_sugg=max_sugg,
+                                           batch_size=batch_size), dyp.nsegments // 2, hooks)
+    res["train"] = {"samples": int(folds.shape[0]), "positives": int(labels.sum().item()),
+                    "epochs": epochs, "loss_first": hist[0], "loss_last": hist[-1],
+                    "data_s": t_data, "train_s": t_train}
+    return res

@@ -162,3 +162,23 @@ def test_cnn_training_learns_pulses(gpu_device):
     acc = float(((r.logits(folds, feats) > 0).float() == yd).float().mean().item())
     print(f"[result] training loss {first:.3f} -> {last:.3f}, accuracy {acc:.3f}", flush=True)
     assert last < 0.3 * first and acc > 0.97
+
+
+def test_budgeted_search_pipeline(gpu_device):
+    """Exhaustive vs budgeted circular EP on a smaller orbit workload: the budgeted
+    search's candidates are exhaustive candidates (no overflow: the same thresholds), it
+    scores fewer leaves, and the learned budget keeps most of the exhaustive candidates."""
+    import bench
+
+    _case("budgeted search pipeline")
+    w = dict(bench.EP_CNN, nsamps=2**22, nseg=64, max_sugg=2**21)
+    thr = np.linspace(0.5, 8.0, w["nseg"] - 1)
+    res = cnn.budget_experiment(lambda s: bench.ep_circular_dyp(s, w), (1, 2), 9, thr,
+                                w["max_sugg"], (2**13,), epochs=6)
+    ex = res["exhaustive"]
+    c, s = res["cnn_8192"], res["snr_8192"]
+    print(f"[result] exhaustive {ex}; learned {c}; S/N {s}; train {res['train']}", flush=True)
+    assert ex["n_final"] > 0
+    assert c["precision"] == 1.0 and s["precision"] == 1.0
+    assert c["trials"] < ex["trials"] and s["trials"] < ex["trials"]
+    assert c["recall"] >= 0.8
