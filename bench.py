@@ -61,8 +61,10 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--nseries", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-series", type=int, default=2)
-    ap.add_argument("--workload", choices=["ffa", "ep-circular-cnn"], default="ffa",
-                    help="ffa: the headline FFA batch (default); ep-circular-cnn: the "
+    ap.add_argument("--workload", choices=["ffa", "ep-taylor-jerk", "ep-circular-cnn"],
+                    default="ffa",
+                    help="ffa: the headline FFA batch (default); ep-taylor-jerk: the EP "
+                         "search of a jerk pulsar (trials/s); ep-circular-cnn: the "
                          "circular-orbit EP with a learned survivor budget vs exhaustive")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device 4 times (HBM counter calibration)")
@@ -197,10 +199,144 @@ def ep_circular_cnn() -> int:
     return 0
 
 
+# EP search of a jerk pulsar (Taylor order 3): N = 2^23 samples at 64 us, 64 FFA segments
+# (63 EP levels), bseg_brute 1024, nbins 64, eta 1, f in [495, 505] Hz, accel +-30 m/s^2,
+# jerk +-2 m/s^3; injected P = 2 ms pulsar (a = 5 m/s^2, j = 0.05 m/s^3, so that its
+# acceleration stays inside the grid over the whole observation; 5 % duty, S/N 12); thresholds
+# bound_scheme(63, 5); max_sugg 2^22, batches of 1024 parents; reference segment 32.
+EP_JERK = {"nsamps": 2**23, "tsamp": 64e-6, "nseg": 64, "nbins": 64, "snr": 12.0,
+           "snr_bound": 5.0, "max_sugg": 2**22, "ref_seg": 32, "seed": 7}
+
+
+def ep_jerk_dyp(w: dict = EP_JERK):
+    from pyloki_amd.config import ParamLimits, PulsarSearchConfig
+    from pyloki_amd.ffa import DynamicProgramming
+    from pyloki_amd.simulation.pulse import PulseSignalConfig
+
+    n = w["nsamps"]
+    lim = ParamLimits.from_taylor((495.0, 505.0), accel=(-30.0, 30.0), jerk=(-2.0, 2.0)).limits
+    cfg = PulsarSearchConfig(nsamps=n, tsamp=w["tsamp"], nbins=w["nbins"], eta=1.0,
+                             param_limits=lim, bseg_brute=1024, bseg_ffa=n // w["nseg"],
+                             prune_poly_order=3, ducy_max=0.3, wtsp=1.5, branch_max=64,
+                             use_fourier=False)
+    ts = PulseSignalConfig(period=0.002, dt=w["tsamp"], nsamps=n, snr=w["snr"], ducy=0.05,
+                           mod_kwargs={"acc": 5.0, "jerk": 0.05}).generate(seed=w["seed"])
+    dyp = DynamicProgramming(ts, cfg)
+    dyp.initialize()
+    dyp.execute()
+    return dyp
+
+
+def _ep_run(pr, ref_seg: int) -> None:
+    pr.initialize(ref_seg)
+    for _ in range(pr.dyp.nsegments - 1):
+        pr.execute_iter()
+        if pr.prune_complete:
+            break
+    pr._do_ascend()
+
+
+def ep_taylor_jerk(args) -> int:
+    import torch
+
+    from pyloki_amd.detection.schemes import bound_scheme
+    from pyloki_amd.prune import Pruning
+
+    if not torch.cuda.is_available():
+        print("no ROCm device visible", file=sys.stderr)
+        return 1
+    w = EP_JERK
+    dyp = ep_jerk_dyp()
+    thr = bound_scheme(dyp.nsegments - 1, w["snr_bound"]).astype(np.float64)
+
+    def new():
+        return Pruning(dyp, thr, max_sugg=w["max_sugg"], batch_size=1024)
+
+    for _ in range(args.warmup):
+        _ep_run(new(), w["ref_seg"])
+    times, trials = [], 0
+    for _ in range(args.steps):
+        pr = new()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _ep_run(pr, w["ref_seg"])
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        trials = int(sum(st.n_leaves for st in pr.pstats[1:]))
+    # The dominant kernel timed per launch on the launch stream (a separate run).
+    pr = new()
+    pr.timing = True
+    _ep_run(pr, w["ref_seg"])
+    kt = pr.kernel_times()
+    B, P = w["nbins"], 3
+    stats = pr.pstats[1:]
+    ch_bytes = 0.0
+    for st in stats:
+        nb = max(st.n_branches, 1)
+        ch_bytes += st.n_leaves * 2 * B * 4 + nb * (2 * B * 4 + 16 * (P + 2)) + st.n_leaves * 24
+    ch_ms = float(sum(kt.get("children", [])))
+    achieved = ch_bytes / (ch_ms * 1e-3) / 1e9 if ch_ms > 0 else 0.0
+    ms = float(np.median(times)) * 1e3
+    res = {"n_final": pr.world_tree.size,
+           "best_snr": float(pr.world_tree.scores.max()) if pr.world_tree.size else None}
+    line = {
+        "metric": "EP trials (scored leaves) per second", "value": trials / (ms * 1e-3),
+        "unit": "trials/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "ep_taylor_jerk", "nsamps": w["nsamps"], "nseg": w["nseg"],
+                   "nbins": B, "prune_poly_order": P, "max_sugg": w["max_sugg"],
+                   "thresholds": f"bound_scheme(63, {w['snr_bound']})",
+                   "levels": len(stats), "trials_per_step": trials,
+                   "max_level_children": max(st.n_leaves for st in stats)},
+        "roofline": {"bound": "hbm", "kernel": "prune_children_kernel",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "launches": len(kt.get("children", [])), "kernel_ms_per_step": ch_ms},
+        "kernel_ms_per_step": {k: float(sum(v)) for k, v in kt.items()},
+        "result": res,
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = ep_cpu_baseline(dyp, thr, w)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def ep_cpu_baseline(dyp, thr, w: dict) -> dict:
+    """The oracle (NumPy restatement, one thread) on the first levels of the same search,
+    as many as fit about 20 s."""
+    from oracle import prune_oracle as po
+
+    cfg = dyp.cfg
+    pcfg = {"poly_order": 3, "nbins": cfg.nbins, "eta": cfg.eta, "branch_max": cfg.branch_max,
+            "counts_init": np.array([len(a) for a in dyp.param_arr]),
+            "param_limits": cfg.param_limits, "widths": cfg.score_widths,
+            "tiling": cfg.tiling_strategy}
+    t0 = time.perf_counter()
+    levels, trials = 0, 0
+    for nl in range(1, dyp.nsegments):
+        res = po.prune_run(dyp.fold, dyp.param_arr, dyp.dparams_actual, dyp.tseg, thr,
+                           w["ref_seg"], pcfg, w["max_sugg"], nlevels=nl)
+        levels, trials = nl, int(sum(s.n_leaves for s in res["stats"]))
+        if time.perf_counter() - t0 > 10.0:
+            break
+    t0 = time.perf_counter()
+    res = po.prune_run(dyp.fold, dyp.param_arr, dyp.dparams_actual, dyp.tseg, thr,
+                       w["ref_seg"], pcfg, w["max_sugg"], nlevels=levels)
+    dt = time.perf_counter() - t0
+    return {"value": trials / dt, "unit": "trials/s", "cores": 1, "kind": "port",
+            "sample": f"the oracle on the first {levels} of 63 levels ({trials} trials)"}
+
+
 def main() -> int:
     args = parse()
     if args.copy_calibration:
         return copy_calibration()
+    if args.workload == "ep-taylor-jerk":
+        if args.gpus != 1:
+            print("bench.py measures one GPU only", file=sys.stderr)
+            return 2
+        return ep_taylor_jerk(args)
     if args.workload == "ep-circular-cnn":
         if args.gpus != 1:
             print("bench.py measures one GPU only", file=sys.stderr)

@@ -183,6 +183,26 @@ def circ_child_positions(off1: torch.Tensor, s1_parent: torch.Tensor, s1_expand:
                                      ck0[:-1] - ck0[bstart][b_of])
     return pos, int(tot_b.sum().item()), int(bfirst.shape[0])
 
+def _raise_branch_error(errv: int) -> None:
+    """The branch kernels' error word (smallest failing (parent * P + param) * 4 + kind)."""
+    if errv != INT64_MAX:
+        if errv % 4 == 1:
+            raise ValueError("Invalid input: increase branch_max.")
+        raise ValueError("Both dparam_cur and dparam_new must be positive.")
+
+
+def compact(mask: torch.Tensor, count: int) -> torch.Tensor:
+    """Indices of the true entries of ``mask`` in increasing order, ``count`` of them,
+    without a host synchronisation: each true entry goes to its prefix-sum slot, every
+    false entry to a discarded extra slot."""
+    n = int(mask.shape[0])
+    pos = torch.cumsum(mask, 0) - 1
+    out = torch.empty(count + 1, dtype=torch.int64, device=mask.device)
+    out.index_copy_(0, torch.where(mask, pos, torch.full_like(pos, count)),
+                    torch.arange(n, dtype=torch.int64, device=mask.device))
+    return out[:count]
+
+
 def overflow_survivors(scores: np.ndarray, child_off: np.ndarray, batch_size: int,
                        threshold: float, max_sugg: int) -> np.ndarray:
     """Children kept by the reference's batch-sequential loop when the buffer overflows.
@@ -292,6 +312,29 @@ class Pruning:
         # Called as level_hook(self, tree) -> tree after every level (no reference
         # counterpart): lets a budgeted search thin the survivors before the next level.
         self.level_hook = None
+        # With timing on, HIP events on the launch stream bracket every library call
+        # (the GPU side of the reference's per-level timers, prune.py:217-322).
+        self.timing = False
+        self._events: list[tuple[str, torch.cuda.Event, torch.cuda.Event]] = []
+
+    def _timed(self, phase: str, fn, *args):
+        if not self.timing:
+            return fn(*args)
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        rc = fn(*args)
+        b.record()
+        self._events.append((phase, a, b))
+        return rc
+
+    def kernel_times(self) -> dict[str, list[float]]:
+        """Milliseconds of every timed library call, per phase, in call order."""
+        torch.cuda.synchronize()
+        out: dict[str, list[float]] = {}
+        for phase, a, b in self._events:
+            out.setdefault(phase, []).append(a.elapsed_time(b))
+        return out
 
     @property
     def dyp(self):
@@ -415,17 +458,21 @@ class Pruning:
             return torch.empty((n, 2, B // 2 + 1), dtype=torch.complex64, device=dev)
         return torch.empty((n, 2, B), dtype=torch.float32, device=dev)
 
-    def _survivors(self, c_score: torch.Tensor, child_off_fn, threshold: float) -> torch.Tensor:
-        """Children kept: all passing ones when they fit the buffer, else the reference's
-        batch-sequential overflow rule replayed on the scores (``child_off_fn()`` gives the
-        per-parent child offsets that mark the batch boundaries)."""
+    def _select(self, c_score: torch.Tensor, minmax: torch.Tensor, child_off_fn,
+                threshold: float) -> tuple[torch.Tensor, float, float]:
+        """Children kept and the level's score range, with one host transfer: all passing
+        children when they fit the buffer, else the reference's batch-sequential overflow
+        rule replayed on the scores (``child_off_fn()`` gives the per-parent child offsets
+        that mark the batch boundaries)."""
         passing = c_score.double() >= threshold
-        npass = int(passing.sum().item())
+        info = torch.cat([passing.sum().reshape(1).double(), minmax.double()]).cpu()
+        npass = int(info[0])
+        smin, smax = float(info[1]), float(info[2])
         if npass <= self.max_sugg:
-            return torch.nonzero(passing).reshape(-1)
+            return compact(passing, npass), smin, smax
         surv_np = overflow_survivors(c_score.cpu().numpy(), child_off_fn(), self.batch_size,
                                      threshold, self.max_sugg)
-        return torch.from_numpy(surv_np).to(c_score.device)
+        return torch.from_numpy(surv_np).to(c_score.device), smin, smax
 
     def _level(self, tree: DeviceTree, seg: torch.Tensor, coords: PruneCoords,
                threshold: float) -> tuple[DeviceTree, dict]:
@@ -435,9 +482,7 @@ class Pruning:
         dev = tree.folds.device
         P = self._P
         n = tree.size
-        B = self.dyp.cfg.nbins
-        pp = ctypes.byref(self._pp)
-        cc = ctypes.byref(coords)
+        pp, cc, st = ctypes.byref(self._pp), ctypes.byref(coords), stream_ptr()
         branch = torch.empty((n, P, 2), dtype=torch.float64, device=dev)
         counts = torch.empty((n, P), dtype=torch.int32, device=dev)
         nchild = torch.empty(n, dtype=torch.int64, device=dev)
@@ -445,22 +490,19 @@ class Pruning:
         if self._cheby:
             # the parents moved to coord_cur feed the children (chebyshev.py:112-119)
             src = torch.empty_like(tree.leaves)
-            _lib.check(lib.pyl_cheb_branch(pp, cc, ptr(tree.leaves), n, ptr(src), ptr(branch),
-                                           ptr(counts), ptr(nchild), ptr(err), stream_ptr()))
+            _lib.check(self._timed("branch", lib.pyl_cheb_branch, pp, cc, ptr(tree.leaves), n,
+                                   ptr(src), ptr(branch), ptr(counts), ptr(nchild), ptr(err),
+                                   st))
             pre = "pyl_cheb"
         else:
             src = tree.leaves
-            _lib.check(lib.pyl_prune_branch(pp, cc, ptr(tree.leaves), n, ptr(branch),
-                                            ptr(counts), ptr(nchild), ptr(err), stream_ptr()))
+            _lib.check(self._timed("branch", lib.pyl_prune_branch, pp, cc, ptr(tree.leaves), n,
+                                   ptr(branch), ptr(counts), ptr(nchild), ptr(err), st))
             pre = "pyl_prune"
         off = torch.zeros(n + 1, dtype=torch.int64, device=dev)
         torch.cumsum(nchild, 0, out=off[1:])
-        errv = int(err.item())
-        if errv != INT64_MAX:
-            if errv % 4 == 1:
-                raise ValueError("Invalid input: increase branch_max.")
-            raise ValueError("Both dparam_cur and dparam_new must be positive.")
-        total = int(off[-1].item())
+        errv, total = (int(x) for x in torch.stack([err[0], off[-1]]).cpu())
+        _raise_branch_error(errv)
         c_parent = torch.empty(total, dtype=torch.int32, device=dev)
         c_row = torch.empty(total, dtype=torch.int32, device=dev)
         c_shift = torch.empty(total, dtype=torch.int32, device=dev)
@@ -468,35 +510,34 @@ class Pruning:
         c_score = torch.empty(total, dtype=torch.float32, device=dev)
         minmax = torch.tensor([np.inf, -np.inf], dtype=torch.float32, device=dev)
         segc = seg.contiguous()
-        _lib.check(self._fn(lib, f"{pre}_children")(
-            pp, cc, ptr(src), ptr(tree.folds), n, ptr(branch), ptr(counts), ptr(off),
-            total, ptr(segc), ptr(c_parent), ptr(c_row), ptr(c_shift), ptr(c_phase),
-            ptr(c_score), ptr(minmax), stream_ptr()))
-        surv = self._survivors(c_score, lambda: off.cpu().numpy(), threshold)
+        _lib.check(self._timed(
+            "children", self._fn(lib, f"{pre}_children"), pp, cc, ptr(src), ptr(tree.folds), n,
+            ptr(branch), ptr(counts), ptr(off), total, ptr(segc), ptr(c_parent), ptr(c_row),
+            ptr(c_shift), ptr(c_phase), ptr(c_score), ptr(minmax), st))
+        surv, smin, smax = self._select(c_score, minmax, lambda: off.cpu().numpy(), threshold)
         m = int(surv.shape[0])
         o_leaves = torch.empty((m, P + 2, 2), dtype=torch.float64, device=dev)
         o_folds = self._new_folds(m, dev)
         o_scores = torch.empty(m, dtype=torch.float32, device=dev)
         o_bt = torch.empty((m, P + 2), dtype=torch.int32, device=dev)
-        _lib.check(self._fn(lib, f"{pre}_materialize")(
-            pp, cc, ptr(src), ptr(tree.folds), ptr(branch), ptr(counts), ptr(off),
-            ptr(segc), ptr(c_parent), ptr(c_row), ptr(c_shift), ptr(c_phase), ptr(c_score),
-            ptr(surv.contiguous()), m, ptr(o_leaves), ptr(o_folds), ptr(o_scores), ptr(o_bt),
-            stream_ptr()))
-        mm = minmax.cpu().numpy()
+        _lib.check(self._timed(
+            "materialize", self._fn(lib, f"{pre}_materialize"), pp, cc, ptr(src),
+            ptr(tree.folds), ptr(branch), ptr(counts), ptr(off), ptr(segc), ptr(c_parent),
+            ptr(c_row), ptr(c_shift), ptr(c_phase), ptr(c_score), ptr(surv.contiguous()), m,
+            ptr(o_leaves), ptr(o_folds), ptr(o_scores), ptr(o_bt), st))
         stats = {"n_leaves": total, "n_leaves_phy": total,
-                 "score_min": float(mm[0]) if total else 0.0,
-                 "score_max": float(mm[1]) if total else 0.0}
+                 "score_min": smin if total else 0.0, "score_max": smax if total else 0.0}
         return DeviceTree(o_leaves, o_folds, o_scores, o_bt), stats
 
     def _level_circ(self, tree: DeviceTree, seg: torch.Tensor, coords: PruneCoords,
                     threshold: float) -> tuple[DeviceTree, dict]:
-        """One circular-basis level (``circular.py`` via ``prune.py:165-332``)."""
+        """One circular-basis level (``circular.py`` via ``prune.py:165-332``). Four host
+        transfers size the buffers: held children, final children, validated children,
+        survivors."""
         lib = _lib.load()
         dev = tree.folds.device
         P = self._P
         n = tree.size
-        B = self.dyp.cfg.nbins
         bs = max(1, min(self.batch_size, n))
         pp, qq, cc = ctypes.byref(self._pp), ctypes.byref(self._qq), ctypes.byref(coords)
         st = stream_ptr()
@@ -513,23 +554,20 @@ class Pruning:
         L.n_par = n
         nchild1 = torch.empty(n, dtype=i64, device=dev)
         err = torch.full((1,), INT64_MAX, dtype=i64, device=dev)
-        _lib.check(lib.pyl_circ_branch(pp, qq, cc, ptr(tree.leaves), ctypes.byref(L),
-                                       ptr(nchild1), ptr(err), st))
+        _lib.check(self._timed("branch", lib.pyl_circ_branch, pp, qq, cc, ptr(tree.leaves),
+                               ctypes.byref(L), ptr(nchild1), ptr(err), st))
         off1 = torch.zeros(n + 1, dtype=i64, device=dev)
         torch.cumsum(nchild1, 0, out=off1[1:])
-        errv = int(err.item())
-        if errv != INT64_MAX:
-            if errv % 4 == 1:
-                raise ValueError("Invalid input: increase branch_max.")
-            raise ValueError("Both dparam_cur and dparam_new must be positive.")
-        n_s1 = int(off1[-1].item())
+        errv, n_s1 = (int(x) for x in torch.stack([err[0], off1[-1]]).cpu())
+        _raise_branch_error(errv)
         buf["off1"] = off1
         buf["s1_parent"] = torch.empty(n_s1, dtype=i32, device=dev)
         buf["s1_expand"] = torch.empty(n_s1, dtype=torch.uint8, device=dev)
         for name in ("off1", "s1_parent", "s1_expand"):
             setattr(L, name, ptr(buf[name]))
         L.n_s1 = n_s1
-        _lib.check(lib.pyl_circ_stage1(pp, qq, cc, ptr(tree.leaves), ctypes.byref(L), st))
+        _lib.check(self._timed("branch", lib.pyl_circ_stage1, pp, qq, cc, ptr(tree.leaves),
+                               ctypes.byref(L), st))
         pos, total, nb = circ_child_positions(off1, buf["s1_parent"], buf["s1_expand"],
                                               buf["cr_num"], bs)
         buf["s1_pos"] = pos.contiguous()
@@ -540,9 +578,11 @@ class Pruning:
         for name in ("s1_pos", "c_s1", "c_sub", "c_keep", "c_region"):
             setattr(L, name, ptr(buf[name]))
         L.n_child = total
-        _lib.check(lib.pyl_circ_expand(pp, qq, cc, ptr(tree.leaves), ctypes.byref(L), st))
-        valid = torch.nonzero(buf["c_keep"]).reshape(-1).contiguous()
-        nv = int(valid.shape[0])
+        _lib.check(self._timed("branch", lib.pyl_circ_expand, pp, qq, cc, ptr(tree.leaves),
+                               ctypes.byref(L), st))
+        keep = buf["c_keep"].bool()
+        nv = int(keep.sum().item())
+        valid = compact(keep, nv)
         c_parent = torch.empty(nv, dtype=i32, device=dev)
         c_row = torch.empty(nv, dtype=i32, device=dev)
         c_shift = torch.empty(nv, dtype=i32, device=dev)
@@ -550,34 +590,35 @@ class Pruning:
         c_score = torch.empty(nv, dtype=torch.float32, device=dev)
         minmax = torch.tensor([np.inf, -np.inf], dtype=torch.float32, device=dev)
         segc = seg.contiguous()
-        _lib.check(self._fn(lib, "pyl_circ_children")(
-            pp, qq, cc, ptr(tree.leaves), ptr(tree.folds), ctypes.byref(L), ptr(valid), nv,
-            ptr(segc), ptr(c_parent), ptr(c_row), ptr(c_shift), ptr(c_phase), ptr(c_score),
-            ptr(minmax), st))
+        _lib.check(self._timed(
+            "children", self._fn(lib, "pyl_circ_children"), pp, qq, cc, ptr(tree.leaves),
+            ptr(tree.folds), ctypes.byref(L), ptr(valid), nv, ptr(segc), ptr(c_parent),
+            ptr(c_row), ptr(c_shift), ptr(c_phase), ptr(c_score), ptr(minmax), st))
 
         def valid_off() -> np.ndarray:
             cnt = torch.bincount(c_parent.long(), minlength=n)
             return np.concatenate([[0], np.cumsum(cnt.cpu().numpy())])
 
-        surv = self._survivors(c_score, valid_off, threshold).contiguous()
+        surv, smin, smax = self._select(c_score, minmax, valid_off, threshold)
+        surv = surv.contiguous()
         m = int(surv.shape[0])
         # Batches whose validated children include a Taylor cell (circular.py:506-511).
-        has_taylor = torch.zeros(nb, dtype=torch.uint8, device=dev)
-        tay = buf["c_region"][valid] == REGION_TAYLOR
-        has_taylor[c_parent.long()[tay] // bs] = 1
+        tay = (buf["c_region"][valid] == REGION_TAYLOR).to(i32)
+        ht = torch.zeros(nb, dtype=i32, device=dev)
+        ht.scatter_reduce_(0, c_parent.long() // bs, tay, reduce="amax")
+        has_taylor = ht.to(torch.uint8)
         o_leaves = torch.empty((m, P + 2, 2), dtype=torch.float64, device=dev)
         o_folds = self._new_folds(m, dev)
         o_scores = torch.empty(m, dtype=torch.float32, device=dev)
         o_bt = torch.empty((m, P + 2), dtype=i32, device=dev)
-        _lib.check(self._fn(lib, "pyl_circ_materialize")(
-            pp, qq, cc, ptr(tree.leaves), ptr(tree.folds), ctypes.byref(L), ptr(valid),
-            ptr(segc), ptr(c_parent), ptr(c_row), ptr(c_shift), ptr(c_phase), ptr(c_score),
+        _lib.check(self._timed(
+            "materialize", self._fn(lib, "pyl_circ_materialize"), pp, qq, cc,
+            ptr(tree.leaves), ptr(tree.folds), ctypes.byref(L), ptr(valid), ptr(segc),
+            ptr(c_parent), ptr(c_row), ptr(c_shift), ptr(c_phase), ptr(c_score),
             ptr(has_taylor), ptr(surv), m, ptr(o_leaves), ptr(o_folds), ptr(o_scores),
             ptr(o_bt), st))
-        mm = minmax.cpu().numpy()
         stats = {"n_leaves": total, "n_leaves_phy": nv,
-                 "score_min": float(mm[0]) if nv else 0.0,
-                 "score_max": float(mm[1]) if nv else 0.0}
+                 "score_min": smin if nv else 0.0, "score_max": smax if nv else 0.0}
         return DeviceTree(o_leaves, o_folds, o_scores, o_bt), stats
 
     def _do_ascend(self) -> None:
@@ -588,7 +629,6 @@ class Pruning:
         seg_idx, seg_coords = self.scheme.get_segment_coords_so_far(self._prune_level)
         t0_cur, scale_cur = self.scheme.get_coord(self._prune_level)
         n = tree.size
-        B = self.dyp.cfg.nbins
         folds = self._new_folds(n, tree.folds.device)
         scores = torch.empty(n, dtype=torch.float32, device=tree.folds.device)
         idx = np.ascontiguousarray(seg_idx, dtype=np.int64)
@@ -599,13 +639,14 @@ class Pruning:
                 ptr(folds), ptr(scores), stream_ptr())
         lib = _lib.load()
         if self._circular:
-            _lib.check(self._fn(lib, "pyl_circ_ascend")(ctypes.byref(self._pp),
-                                                        ctypes.byref(self._qq), *args))
+            _lib.check(self._timed("ascend", self._fn(lib, "pyl_circ_ascend"),
+                                   ctypes.byref(self._pp), ctypes.byref(self._qq), *args))
         elif self._cheby:
-            _lib.check(self._fn(lib, "pyl_cheb_ascend")(ctypes.byref(self._pp), *args[:8],
-                                                        scale_cur, *args[8:]))
+            _lib.check(self._timed("ascend", self._fn(lib, "pyl_cheb_ascend"),
+                                   ctypes.byref(self._pp), *args[:8], scale_cur, *args[8:]))
         else:
-            _lib.check(self._fn(lib, "pyl_prune_ascend")(ctypes.byref(self._pp), *args))
+            _lib.check(self._timed("ascend", self._fn(lib, "pyl_prune_ascend"),
+                                   ctypes.byref(self._pp), *args))
         tree.scores_ep = tree.scores.clone()
         tree.folds = folds
         tree.scores = scores
