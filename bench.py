@@ -373,9 +373,13 @@ def main() -> int:
     nw = len(widths)
     # Algorithmic bytes per output row: every output row reads its 2 source rows and writes
     # its row (shift-add) or its per-width S/N (fused last level + score).
+    # The last two levels fused with scoring read the 4 rows of the level before both and
+    # write the S/N; the intermediate level's rows are never stored.
+    two = "ffa_shift_add_score_rounds_kernel<64, two levels>"
     row_bytes = {
         "ffa_shift_add_kernel<64>": 3 * 2 * nbins * 4,
         "ffa_shift_add_score_rounds_kernel<64>": 2 * 2 * nbins * 4 + 4 * nw,
+        two: 4 * 2 * nbins * 4 + 4 * nw,
     }
     events: dict[str, list] = {k: [] for k in row_bytes}
     record = {"on": False}
@@ -399,6 +403,22 @@ def main() -> int:
 
     kernels.ffa_shift_add = _timed("ffa_shift_add_kernel<64>", orig_sa)
     kernels.ffa_shift_add_score = _timed("ffa_shift_add_score_rounds_kernel<64>", orig_sas)
+    orig_s2 = kernels.ffa_shift_add2_score
+
+    def _timed2(fold_in, table_prev, table, ncoords_prev, ncoords_cur, *a, **kw):
+        if not record["on"]:
+            return orig_s2(fold_in, table_prev, table, ncoords_prev, ncoords_cur, *a, **kw)
+        s = torch.cuda.current_stream()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        res = orig_s2(fold_in, table_prev, table, ncoords_prev, ncoords_cur, *a, **kw)
+        e1.record(s)
+        compulsory = fold_in.numel() * fold_in.element_size() + res.numel() * res.element_size()
+        events[two].append((e0, e1, (fold_in.shape[0] // 4) * ncoords_cur, compulsory))
+        return res
+
+    kernels.ffa_shift_add2_score = _timed2
 
     # Trials per step: the final grid of every series (known from the plan, no run needed).
     trials = 0
@@ -423,6 +443,7 @@ def main() -> int:
     torch.cuda.synchronize()
     record["on"] = False
     kernels.ffa_shift_add, kernels.ffa_shift_add_score = orig_sa, orig_sas
+    kernels.ffa_shift_add2_score = orig_s2
     kstats = {}
     for name, evs in events.items():
         if not evs:

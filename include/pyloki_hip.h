@@ -155,6 +155,21 @@ int pyl_ffa_shift_add_score_layout(const float* fold_in_dev, int32_t in_layout,
                                    int64_t n_inner, int32_t nbins, const int64_t* widths,
                                    int32_t nw, double stdnoise, float* snr_out_dev,
                                    void* stream);
+/* A series' last two levels fused with scoring: fold_in_dev holds the fold two levels
+ * before the last ([nseg_in][ncoords_prev2][2][nbins (+4 in the wrap layout)], nseg_in a
+ * multiple of 4), table_prev the resolve table of the level before the last (ncoords_prev
+ * rows, indices into ncoords_prev2) and table_dev that of the last level. The result equals
+ * pyl_ffa_shift_add followed by pyl_ffa_shift_add_score_layout bit for bit (the sums are
+ * formed in the same order and rounded to FP32 at the same points); the intermediate fold
+ * is never written. nbins 32 or 64, at most 16 widths. No reference counterpart as an
+ * interface: two unify_fold calls (dyn_pffa.py:435-489) and the final scoring. */
+int pyl_ffa_shift_add2_score_layout(const float* fold_in_dev, int32_t in_layout,
+                                    int64_t nseg_in, int64_t ncoords_prev2,
+                                    const int32_t* table_prev_dev, int64_t ncoords_prev,
+                                    const int32_t* table_dev, int64_t ncoords_cur,
+                                    int64_t n_inner, int32_t nbins, const int64_t* widths,
+                                    int32_t nw, double stdnoise, float* snr_out_dev,
+                                    void* stream);
 int pyl_ffa_level(int32_t nparams, const double* params_dev, const int64_t* counts_cur,
                   const int64_t* counts_prev, const double* param_limits, int32_t ffa_level,
                   double tseg_brute, int32_t nbins, const float* fold_in_dev, int64_t nseg_in,

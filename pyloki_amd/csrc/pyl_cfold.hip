@@ -135,7 +135,7 @@ cscore_kernel(const float2* __restrict__ folds, int64_t n, int nbins, pyl::Score
     for (int b = tid; b < nbins; b += pyl::kScoreBlock) {
       const float e = pyl::irfft_bin(row, nbins, b, tw);
       const float v = pyl::irfft_bin(row + nb_f, nbins, b, tw);
-      S[b * pyl::kScoreStride + r] = e / sqrtf(v);
+      S[b * pyl::kScoreStride + r] = pyl::norm_ev(e, v);
     }
   }
   __syncthreads();

@@ -278,7 +278,7 @@ circ_children_kernel(LevelConst k, CircConst q, const double* __restrict__ leave
         if (src < 0) src += B;
         const float e = fr[b] + sr[src];
         const float vv = fr[B + b] + sr[B + src];
-        S[b * kStride + r] = e / sqrtf(vv);
+        S[b * kStride + r] = pyl::norm_ev(e, vv);
       }
     }
     __syncthreads();
@@ -444,7 +444,7 @@ circ_ascend_kernel(LevelConst k, CircConst q, const double* __restrict__ leaves,
         }
         of[b] = ae;
         of[B + b] = av;
-        S[b * kStride + r] = ae / sqrtf(av);
+        S[b * kStride + r] = pyl::norm_ev(ae, av);
       }
     }
     __syncthreads();

@@ -102,7 +102,7 @@ cnn_forward_kernel(const float* __restrict__ th, const float* __restrict__ folds
       e = folds[i * 2 * B + p];
       v = folds[i * 2 * B + B + p];
     }
-    pv = e / sqrtf(v);
+    pv = pyl::norm_ev(e, v);
   }
   X[s][p] = pv;
   __syncthreads();

@@ -75,6 +75,47 @@ __device__ __host__ inline double fact_d(int n) {
   return r;
 }
 
+// e / sqrtf(v), both operations IEEE round-to-nearest. For operands well inside the normal
+// range (|e| in [2^-40, 2^40), v in [2^-50, 2^50); norm_in_range) norm_fast runs the
+// arithmetic of the compiler's correctly rounded square root (v_sqrt_f32, then the +-1 ulp
+// correction by FMA residuals) and divide (reciprocal, one Newton step, two residual
+// corrections) without their operand scaling and special-value steps, which are identities
+// there, so the result is the same to the bit; other operands take the library expansion.
+__device__ __forceinline__ bool norm_in_range(float e, float v) {
+  const uint32_t eb = __float_as_uint(e) & 0x7fffffffu;
+  const uint32_t vb = __float_as_uint(v);
+  return (eb - 0x2b800000u < 0x28000000u) & (vb - 0x26800000u < 0x32000000u);
+}
+
+__device__ __forceinline__ float norm_fast(float e, float v) {
+  float s = __builtin_amdgcn_sqrtf(v);
+  const float sd = __uint_as_float(__float_as_uint(s) - 1u);
+  const float su = __uint_as_float(__float_as_uint(s) + 1u);
+  const float rd = __builtin_fmaf(-sd, s, v);
+  const float ru = __builtin_fmaf(-su, s, v);
+  s = (rd <= 0.0f) ? sd : s;
+  s = (ru > 0.0f) ? su : s;
+  float y = __builtin_amdgcn_rcpf(s);
+  y = __builtin_fmaf(__builtin_fmaf(-s, y, 1.0f), y, y);
+  float q = e * y;
+  q = __builtin_fmaf(__builtin_fmaf(-s, q, e), y, q);
+  return __builtin_fmaf(__builtin_fmaf(-s, q, e), y, q);
+}
+
+__device__ __forceinline__ float norm_ev(float e, float v) {
+  if (norm_in_range(e, v)) return norm_fast(e, v);
+  return e / sqrtf(v);
+}
+
+// Four bins at once: one range test, so the four fast sequences interleave.
+__device__ __forceinline__ float4 norm_ev4(float4 e, float4 v) {
+  if (norm_in_range(e.x, v.x) & norm_in_range(e.y, v.y) & norm_in_range(e.z, v.z) &
+      norm_in_range(e.w, v.w))
+    return make_float4(norm_fast(e.x, v.x), norm_fast(e.y, v.y), norm_fast(e.z, v.z),
+                       norm_fast(e.w, v.w));
+  return make_float4(e.x / sqrtf(v.x), e.y / sqrtf(v.y), e.z / sqrtf(v.z), e.w / sqrtf(v.w));
+}
+
 // x**n for an integer n by binary powering. The reference raises floats to integer
 // arrays inside numba fastmath code (`delta_t**powers`, transforms.py:43;
 // `(tobs - t_ref)**(k+1)`, psr_utils.py:89); LLVM's fast-math simplification turns

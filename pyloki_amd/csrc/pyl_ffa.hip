@@ -278,10 +278,13 @@ ffa_shift_add_score_kernel(const T* __restrict__ in, int64_t ncoords_prev,
       // The level's fold element as it would be stored (identity for FP32, BF16 rounding
       // for BF16 storage), then e / sqrt(v).
       auto st = [](float x) { return pyl::Store<T>::load(pyl::Store<T>::store(x)); };
-      S[(b + 0) * pyl::kScoreStride + r] = st(te.x + he.x) / sqrtf(st(tv.x + hv.x));
-      S[(b + 1) * pyl::kScoreStride + r] = st(te.y + he.y) / sqrtf(st(tv.y + hv.y));
-      S[(b + 2) * pyl::kScoreStride + r] = st(te.z + he.z) / sqrtf(st(tv.z + hv.z));
-      S[(b + 3) * pyl::kScoreStride + r] = st(te.w + he.w) / sqrtf(st(tv.w + hv.w));
+      const float4 x = pyl::norm_ev4(
+          make_float4(st(te.x + he.x), st(te.y + he.y), st(te.z + he.z), st(te.w + he.w)),
+          make_float4(st(tv.x + hv.x), st(tv.y + hv.y), st(tv.z + hv.z), st(tv.w + hv.w)));
+      S[(b + 0) * pyl::kScoreStride + r] = x.x;
+      S[(b + 1) * pyl::kScoreStride + r] = x.y;
+      S[(b + 2) * pyl::kScoreStride + r] = x.z;
+      S[(b + 3) * pyl::kScoreStride + r] = x.w;
     }
   }
   __syncthreads();
@@ -309,13 +312,22 @@ ffa_shift_add_score_kernel(const T* __restrict__ in, int64_t ncoords_prev,
 // from registers and S[i + w] read at a runtime offset. The per-width maxima sit in a small
 // register array indexed by the (uniform) width counter. The i order, and so every FP32
 // operation, is that of the reference's sequential loop.
-template <int B, typename T, int WC, bool kWrapIn>
+//
+// kTwo: the last two levels in one pass. ``in`` then holds the fold two levels before the
+// last (segments 4q .. 4q+3 feed output segment q) and ``table2`` is the resolve table of
+// the level before the last: an output row's tail row is the sum of two such rows (its
+// table2 entry), rotated by the tail shift, and likewise its head row. Each pair is added
+// and stored exactly as the intermediate level would store it, so the result is that of
+// the two levels run one after the other; the intermediate fold is never written.
+template <int B, typename T, int WC, bool kWrapIn, bool kTwo = false>
 __global__ void __launch_bounds__(256)
 ffa_shift_add_score_rounds_kernel(const T* __restrict__ in, int64_t ncoords_prev,
                                   const int4* __restrict__ table, int64_t ncoords_cur,
                                   int64_t n_inner, int tile_o_rows, int64_t tiles_inner,
                                   int64_t tiles_per_pair, pyl::ScoreConst k,
-                                  float* __restrict__ out) {
+                                  float* __restrict__ out,
+                                  const int4* __restrict__ table2 = nullptr,
+                                  int64_t ncoords_prev2 = 0) {
   constexpr int kRows = 256;
   constexpr int kLds = kRows + 1;
   constexpr int kSeg = 16;            // scoring window step
@@ -342,14 +354,23 @@ ffa_shift_add_score_rounds_kernel(const T* __restrict__ in, int64_t ncoords_prev
   const int64_t inner = tile_i * tile_i_rows + tid % tile_i_rows;
   const bool ok = outer < n_outer && inner < n_inner;
   const int64_t j = outer * n_inner + inner;
-  s_tab[tid] = ok ? table[j] : make_int4(-1, 0, 0, 0);
+  const int4 t0 = ok ? table[j] : make_int4(-1, 0, 0, 0);
+  s_tab[tid] = t0;
+  __shared__ int4 s_tab2[kTwo ? 2 * kRows : 1];
+  if constexpr (kTwo) {
+    if (ok) {
+      s_tab2[2 * tid] = table2[t0.x];
+      s_tab2[2 * tid + 1] = table2[t0.z];
+    }
+  }
   using Li = RowLayout<B, kWrapIn>;
   const int wave = tid >> 6;
   const int lane = tid & 63;
   const int c4 = lane % kLpr;   // float4 chunk inside the round's bins
   const int rsub = lane / kLpr;  // row inside the wavefront step
-  const T* base_t = in + (2 * q) * ncoords_prev * Li::kRow;
-  const T* base_h = in + (2 * q + 1) * ncoords_prev * Li::kRow;
+  const int64_t seg_rows = (kTwo ? ncoords_prev2 : ncoords_prev) * Li::kRow;
+  const T* base_t = in + (kTwo ? 4 * q : 2 * q) * seg_rows;
+  const T* base_h = in + (kTwo ? 4 * q + 2 : 2 * q + 1) * seg_rows;
   float x[B];  // this thread's row: prefix sums S[0..B-1]
 #pragma unroll
   for (int rd = 0; rd < kRounds1; ++rd) {
@@ -359,21 +380,45 @@ ffa_shift_add_score_rounds_kernel(const T* __restrict__ in, int64_t ncoords_prev
       const int r = wave * 64 + step * kRps + rsub;
       const int4 tb = s_tab[r];
       if (tb.x >= 0) {
-        const T* tail = base_t + (int64_t)tb.x * Li::kRow;
-        const T* head = base_h + (int64_t)tb.z * Li::kRow;
         const int cc = rd * kLpr + c4;
-        const float4 te = row_chunk<B, T, kWrapIn>(tail, cc, tb.y);
-        const float4 he = row_chunk<B, T, kWrapIn>(head, cc, tb.w);
-        const float4 tv = row_chunk<B, T, kWrapIn>(tail + Li::kComp, cc, tb.y);
-        const float4 hv = row_chunk<B, T, kWrapIn>(head + Li::kComp, cc, tb.w);
         // The level's fold element as it would be stored (identity for FP32, BF16 rounding
         // for BF16 storage), then e / sqrt(v).
         auto st = [](float v) { return pyl::Store<T>::load(pyl::Store<T>::store(v)); };
+        float4 te, he, tv, hv;
+        if constexpr (kTwo) {
+          const int4 ta = s_tab2[2 * r];
+          const int4 ha = s_tab2[2 * r + 1];
+          const T* t1 = base_t + (int64_t)ta.x * Li::kRow;
+          const T* t2 = base_t + seg_rows + (int64_t)ta.z * Li::kRow;
+          const T* h1 = base_h + (int64_t)ha.x * Li::kRow;
+          const T* h2 = base_h + seg_rows + (int64_t)ha.z * Li::kRow;
+          const int st1 = (tb.y + ta.y) & (B - 1), st2 = (tb.y + ta.w) & (B - 1);
+          const int sh1 = (tb.w + ha.y) & (B - 1), sh2 = (tb.w + ha.w) & (B - 1);
+          auto add = [&](float4 a, float4 b) {
+            return make_float4(st(a.x + b.x), st(a.y + b.y), st(a.z + b.z), st(a.w + b.w));
+          };
+          te = add(row_chunk<B, T, kWrapIn>(t1, cc, st1), row_chunk<B, T, kWrapIn>(t2, cc, st2));
+          he = add(row_chunk<B, T, kWrapIn>(h1, cc, sh1), row_chunk<B, T, kWrapIn>(h2, cc, sh2));
+          tv = add(row_chunk<B, T, kWrapIn>(t1 + Li::kComp, cc, st1),
+                   row_chunk<B, T, kWrapIn>(t2 + Li::kComp, cc, st2));
+          hv = add(row_chunk<B, T, kWrapIn>(h1 + Li::kComp, cc, sh1),
+                   row_chunk<B, T, kWrapIn>(h2 + Li::kComp, cc, sh2));
+        } else {
+          const T* tail = base_t + (int64_t)tb.x * Li::kRow;
+          const T* head = base_h + (int64_t)tb.z * Li::kRow;
+          te = row_chunk<B, T, kWrapIn>(tail, cc, tb.y);
+          he = row_chunk<B, T, kWrapIn>(head, cc, tb.w);
+          tv = row_chunk<B, T, kWrapIn>(tail + Li::kComp, cc, tb.y);
+          hv = row_chunk<B, T, kWrapIn>(head + Li::kComp, cc, tb.w);
+        }
         const int b = 4 * c4;
-        S[(b + 0) * kLds + r] = st(te.x + he.x) / sqrtf(st(tv.x + hv.x));
-        S[(b + 1) * kLds + r] = st(te.y + he.y) / sqrtf(st(tv.y + hv.y));
-        S[(b + 2) * kLds + r] = st(te.z + he.z) / sqrtf(st(tv.z + hv.z));
-        S[(b + 3) * kLds + r] = st(te.w + he.w) / sqrtf(st(tv.w + hv.w));
+        const float4 x = pyl::norm_ev4(
+            make_float4(st(te.x + he.x), st(te.y + he.y), st(te.z + he.z), st(te.w + he.w)),
+            make_float4(st(tv.x + hv.x), st(tv.y + hv.y), st(tv.z + hv.z), st(tv.w + hv.w)));
+        S[(b + 0) * kLds + r] = x.x;
+        S[(b + 1) * kLds + r] = x.y;
+        S[(b + 2) * kLds + r] = x.z;
+        S[(b + 3) * kLds + r] = x.w;
       }
     }
     __syncthreads();
@@ -555,6 +600,34 @@ static void launch_rounds(const T* in, int64_t nprev, const int4* tb, int64_t nc
                                                           tiles_inner, per_pair, k, out);
 }
 
+template <int B, int WC, bool kWrapIn>
+static void launch_rounds2(const float* in, int64_t nprev2, const int4* tb2, int64_t nprev,
+                           const int4* tb, int64_t ncur, int64_t n_inner, int64_t nseg_out,
+                           const pyl::ScoreConst& k, float* out, hipStream_t s) {
+  const int64_t n_outer = ncur / n_inner;
+  int to = 16;
+  while (to > 1 && n_outer < to) to /= 2;
+  const int64_t tiles_inner = pyl::ceil_div(n_inner, 256 / to);
+  const int64_t tiles_outer = pyl::ceil_div(n_outer, to);
+  const int64_t per_pair = tiles_inner * tiles_outer;
+  const size_t lds = sizeof(float) * (size_t)std::max(16 + WC, 32) * 257;
+  ffa_shift_add_score_rounds_kernel<B, float, WC, kWrapIn, true>
+      <<<(unsigned)(per_pair * nseg_out), 256, lds, s>>>(in, nprev, tb, ncur, n_inner, to,
+                                                          tiles_inner, per_pair, k, out, tb2,
+                                                          nprev2);
+}
+
+template <int B, bool kWrapIn>
+static void launch_two(const float* in, int64_t nprev2, const int4* tb2, int64_t nprev,
+                       const int4* tb, int64_t ncur, int64_t n_inner, int64_t nseg_out,
+                       const pyl::ScoreConst& k, float* out, hipStream_t s) {
+  if (k.wmax <= 16)
+    return launch_rounds2<B, 16, kWrapIn>(in, nprev2, tb2, nprev, tb, ncur, n_inner, nseg_out, k, out, s);
+  if (k.wmax <= B / 2)
+    return launch_rounds2<B, B / 2, kWrapIn>(in, nprev2, tb2, nprev, tb, ncur, n_inner, nseg_out, k, out, s);
+  launch_rounds2<B, B - 1, kWrapIn>(in, nprev2, tb2, nprev, tb, ncur, n_inner, nseg_out, k, out, s);
+}
+
 template <int B, typename T, bool kWrapIn>
 static void launch_shift_add_score(const T* in, int64_t nprev, const int4* tb, int64_t ncur,
                                    int64_t n_inner, int64_t nseg_out, const pyl::ScoreConst& k,
@@ -627,6 +700,41 @@ extern "C" int pyl_ffa_shift_add_score_layout(const float* fold_in, int32_t in_l
   return shift_add_score<float>(fold_in, in_layout, nseg_in, ncoords_prev, table_dev,
                                 ncoords_cur, n_inner, nbins, widths, nw, stdnoise, snr_out,
                                 stream);
+}
+
+extern "C" int pyl_ffa_shift_add2_score_layout(
+    const float* fold_in, int32_t in_layout, int64_t nseg_in, int64_t ncoords_prev2,
+    const int32_t* table_prev, int64_t ncoords_prev, const int32_t* table_dev,
+    int64_t ncoords_cur, int64_t n_inner, int32_t nbins, const int64_t* widths, int32_t nw,
+    double stdnoise, float* snr_out, void* stream) {
+  PYL_REQUIRE(fold_in && table_prev && table_dev && snr_out, "null device pointer");
+  PYL_REQUIRE(nseg_in >= 4 && nseg_in % 4 == 0,
+              "two fused levels need a multiple of 4 segments, got %lld", (long long)nseg_in);
+  PYL_REQUIRE(ncoords_prev2 > 0 && ncoords_prev > 0 && ncoords_cur > 0, "empty parameter grid");
+  PYL_REQUIRE(nbins == 32 || nbins == 64, "two fused levels support nbins 32 or 64, got %d",
+              nbins);
+  PYL_REQUIRE(n_inner >= 1 && ncoords_cur % n_inner == 0,
+              "n_inner must divide ncoords_cur (last-axis grid count)");
+  PYL_REQUIRE(valid_layout(in_layout), "invalid fold layout");
+  pyl::ScoreConst k;
+  int rc = pyl::build_score_const(nbins, widths, nw, stdnoise, &k);
+  if (rc != PYL_OK) return rc;
+  PYL_REQUIRE(k.nw <= kRoundsMaxNW, "two fused levels support at most %d widths",
+              kRoundsMaxNW);
+  hipStream_t s = pyl::as_stream(stream);
+  const int64_t nseg_out = nseg_in / 4;
+  const int4* tb = reinterpret_cast<const int4*>(table_dev);
+  const int4* tb2 = reinterpret_cast<const int4*>(table_prev);
+  const bool wi = in_layout == PYL_FOLD_WRAP;
+  if (nbins == 32) {
+    if (wi) launch_two<32, true>(fold_in, ncoords_prev2, tb2, ncoords_prev, tb, ncoords_cur, n_inner, nseg_out, k, snr_out, s);
+    else launch_two<32, false>(fold_in, ncoords_prev2, tb2, ncoords_prev, tb, ncoords_cur, n_inner, nseg_out, k, snr_out, s);
+  } else {
+    if (wi) launch_two<64, true>(fold_in, ncoords_prev2, tb2, ncoords_prev, tb, ncoords_cur, n_inner, nseg_out, k, snr_out, s);
+    else launch_two<64, false>(fold_in, ncoords_prev2, tb2, ncoords_prev, tb, ncoords_cur, n_inner, nseg_out, k, snr_out, s);
+  }
+  PYL_CHECK_LAUNCH("ffa_shift_add_score_rounds_kernel<two levels>");
+  return PYL_OK;
 }
 
 extern "C" int pyl_ffa_shift_add_score(const float* fold_in, int64_t nseg_in,

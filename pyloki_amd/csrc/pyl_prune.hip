@@ -176,7 +176,7 @@ prune_children_kernel(LevelConst k, const double* __restrict__ leaves,
         if (src < 0) src += B;
         const float e = fr[b] + sr[src];
         const float v = fr[B + b] + sr[B + src];
-        S[b * kStride + r] = e / sqrtf(v);
+        S[b * kStride + r] = pyl::norm_ev(e, v);
       }
     }
     __syncthreads();
@@ -342,7 +342,7 @@ prune_ascend_kernel(LevelConst k, const double* __restrict__ leaves, int64_t n,
         }
         of[b] = ae;
         of[B + b] = av;
-        S[b * kStride + r] = ae / sqrtf(av);
+        S[b * kStride + r] = pyl::norm_ev(ae, av);
       }
     }
     __syncthreads();

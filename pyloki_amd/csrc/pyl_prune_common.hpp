@@ -360,7 +360,7 @@ __device__ inline float* complex_block_columns(const LevelConst& k, char* lds, i
     for (int b = tid; b < B; b += kBlk) {
       const float e = pyl::irfft_bin(X, B, b, tw);
       const float v = pyl::irfft_bin(X + nb_f, B, b, tw);
-      S[b * kStride + r] = e / sqrtf(v);
+      S[b * kStride + r] = pyl::norm_ev(e, v);
     }
   }
   __syncthreads();

@@ -56,10 +56,10 @@ score_boxcar_kernel(const float* __restrict__ folds, int64_t n, int nbins, int l
       float x;
       if (layout == PYL_LAYOUT_EV) {
         const float* row = folds + (p0 + r) * rowlen;
-        x = row[b] / sqrtf(row[nbins + b]);
+        x = pyl::norm_ev(row[b], row[nbins + b]);
       } else if (layout == PYL_LAYOUT_EV_BF16) {
         const uint16_t* row = reinterpret_cast<const uint16_t*>(folds) + (p0 + r) * rowlen;
-        x = pyl::bf16_to_f32(row[b]) / sqrtf(pyl::bf16_to_f32(row[nbins + b]));
+        x = pyl::norm_ev(pyl::bf16_to_f32(row[b]), pyl::bf16_to_f32(row[nbins + b]));
       } else {
         x = folds[(p0 + r) * rowlen + b];
       }

@@ -306,6 +306,9 @@ def ffa_batch(plan: FFAPlan, series: list[tuple[torch.Tensor, torch.Tensor]],
         if nseg[k] == 1:
             results[order[k]] = kernels.score_boxcar(fold[off], widths, stdnoise)
             off += 1
+    if use_fused and nbins in (32, 64) and len(widths) <= 16 and dtype == torch.float32:
+        _batch_two_levels(plan, fold, nseg, order, widths, stdnoise, results)
+        return [results[i] for i in range(nser)]
     fin_src: torch.Tensor | None = None  # output of the previous level's "pre" group
     while True:
         big = [k for k in range(nser) if nseg[k] > 4]
@@ -341,3 +344,56 @@ def ffa_batch(plan: FFAPlan, series: list[tuple[torch.Tensor, torch.Tensor]],
             nseg[k] //= 2
         level += 1
     return [results[i] for i in range(nser)]
+
+
+def _batch_two_levels(plan: FFAPlan, fold: torch.Tensor, nseg: list[int], order: list[int],
+                      widths: np.ndarray, stdnoise: float, results: dict) -> None:
+    """The level loop of ``ffa_batch`` with every series' last two levels fused with its
+    scoring (``pyl_ffa_shift_add2_score_layout``): the level before them (a series with 8
+    segments) writes the "wrap" row layout, a series with 4 segments runs its last two
+    levels and scoring in one launch, and only a series with 2 segments after the brute
+    fold uses the one-level fused kernel. ``fold`` holds the brute folds, longest series
+    first; the series with one segment have been scored already."""
+    nser = len(nseg)
+    level = 0
+    fin2_src: torch.Tensor | None = None  # the previous level's output for the 8-segment group
+    while True:
+        big = [k for k in range(nser) if nseg[k] > 8]
+        pre2 = [k for k in range(nser) if nseg[k] == 8]
+        fin2 = [k for k in range(nser) if nseg[k] == 4]
+        fin = [k for k in range(nser) if nseg[k] == 2]
+        if not big and not pre2 and not fin2 and not fin:
+            break
+        lp = plan.levels[level]
+        nb = sum(nseg[k] for k in big)
+        npre2 = 8 * len(pre2)
+        off = nb + npre2
+        if fin2:
+            lp1 = plan.levels[level + 1]
+            if fin2_src is not None:
+                part, lay = fin2_src, "wrap"
+            else:  # series that had 4 segments after the brute fold
+                part, lay = fold[off:off + 4 * len(fin2)], "dense"
+                off += 4 * len(fin2)
+            snr = kernels.ffa_shift_add2_score(part, lp.table, lp1.table, lp.ncoords,
+                                               lp1.ncoords, widths, stdnoise,
+                                               n_inner=int(lp1.counts_cur[-1]), in_layout=lay)
+            for m, k in enumerate(fin2):
+                results[order[k]] = snr[m]
+        if fin:  # series that had 2 segments after the brute fold
+            snr = kernels.ffa_shift_add_score(fold[off:off + 2 * len(fin)], lp.table,
+                                              lp.ncoords, widths, stdnoise,
+                                              n_inner=int(lp.counts_cur[-1]))
+            for m, k in enumerate(fin):
+                results[order[k]] = snr[m]
+        fin2_src = None
+        if pre2:
+            fin2_src = kernels.ffa_shift_add(fold[nb:nb + npre2], lp.table, lp.ncoords,
+                                             out_layout="wrap")
+        if big:
+            fold = kernels.ffa_shift_add(fold[:nb], lp.table, lp.ncoords)
+        for k in big + pre2:
+            nseg[k] //= 2
+        for k in fin2 + fin:
+            nseg[k] = 1
+        level += 1

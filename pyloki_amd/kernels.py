@@ -288,6 +288,40 @@ def ffa_shift_add_score(fold_in: torch.Tensor, table: torch.Tensor, ncoords_cur:
     return out
 
 
+def ffa_shift_add2_score(fold_in: torch.Tensor, table_prev: torch.Tensor, table: torch.Tensor,
+                         ncoords_prev: int, ncoords_cur: int, widths: np.ndarray,
+                         stdnoise: float = 1.0, out: torch.Tensor | None = None,
+                         n_inner: int | None = None, *, in_layout: str = "dense") -> torch.Tensor:
+    """The last two levels fused with scoring: [nseg, nprev2, 2, B] -> S/N
+    [nseg/4, ncur, n_w], equal bit for bit to ``ffa_shift_add`` with ``table_prev`` then
+    ``ffa_shift_add_score`` with ``table``; the intermediate fold is never stored. FP32,
+    nbins 32 or 64, at most 16 widths."""
+    if fold_in.dtype != torch.float32:
+        msg = "two fused levels need FP32 folds"
+        raise ValueError(msg)
+    nseg, nprev2, nbins = _check_level_input(fold_in, table_prev, ncoords_prev, in_layout)
+    if nseg % 4:
+        msg = f"two fused levels need a multiple of 4 segments, got {nseg}"
+        raise ValueError(msg)
+    if tuple(table.shape) != (ncoords_cur, 4):
+        msg = "table does not match ncoords_cur"
+        raise ValueError(msg)
+    widths = np.ascontiguousarray(widths, dtype=np.int64)
+    shape = (nseg // 4, ncoords_cur, len(widths))
+    if out is None:
+        out = torch.empty(shape, dtype=torch.float32, device=fold_in.device)
+    elif tuple(out.shape) != shape or out.dtype != torch.float32 or not out.is_contiguous():
+        msg = f"out must be contiguous float32 with shape {shape}"
+        raise ValueError(msg)
+    n_inner = ncoords_cur if n_inner is None else int(n_inner)
+    lib = _lib.load()
+    _lib.check(lib.pyl_ffa_shift_add2_score_layout(
+        ptr(fold_in), _layout(in_layout), nseg, nprev2, ptr(table_prev), ncoords_prev,
+        ptr(table), ncoords_cur, n_inner, nbins, _lib.i64_array(widths), len(widths),
+        float(stdnoise), ptr(out), stream_ptr()))
+    return out
+
+
 def score_boxcar(folds: torch.Tensor, widths: np.ndarray, stdnoise: float = 1.0,
                  layout: int = _lib.PYL_LAYOUT_EV, mode: int = _lib.PYL_SCORE_PER_WIDTH,
                  out: torch.Tensor | None = None) -> torch.Tensor:

@@ -197,6 +197,38 @@ def test_wrap_layout_levels_bit_exact(gpu_device, nbins):
         np.testing.assert_array_equal(got, exp)
 
 
+@pytest.mark.parametrize("nbins", [32, 64])
+def test_two_fused_levels_bit_exact(gpu_device, nbins):
+    """The last two levels fused with scoring, from the fold two levels before the last
+    (dense or wrap rows), equal the two levels run one after the other and the oracle."""
+    _case(f"two fused levels nbins={nbins}")
+    cfg = _cfg(2, 2**18, nbins, 4096)
+    rng = np.random.default_rng(200 + nbins)
+    ts_e = rng.normal(size=cfg.nsamps).astype(np.float32)
+    ts_v = rng.uniform(0.5, 1.5, cfg.nsamps).astype(np.float32)
+    plan = FFAPlan(cfg)
+    folds = plan.run(_dev(ts_e), _dev(ts_v), keep=True)
+    lp_prev, lp = plan.levels[-2], plan.levels[-1]
+    src = folds[-3]
+    assert src.shape[0] == 4
+    src_wrap = torch.cat([src, src[..., :4]], dim=-1).contiguous()
+    for kind in ("default", "half", "wide"):
+        widths = _width_set(nbins, kind)
+        exp = kernels.ffa_shift_add_score(folds[-2], lp.table, lp.ncoords, widths,
+                                          n_inner=int(lp.counts_cur[-1])).cpu().numpy()
+        if kind == "default":
+            np.testing.assert_array_equal(
+                exp, orc.boxcar_snr(orc.fold_norm(folds[-1].cpu().numpy()), widths))
+        for part, lay in ((src, "dense"), (src_wrap, "wrap")):
+            got = kernels.ffa_shift_add2_score(part, lp_prev.table, lp.table, lp_prev.ncoords,
+                                               lp.ncoords, widths,
+                                               n_inner=int(lp.counts_cur[-1]), in_layout=lay)
+            np.testing.assert_array_equal(got.cpu().numpy(), exp)
+    with pytest.raises(ValueError, match="at most 16 widths"):
+        kernels.ffa_shift_add2_score(src, lp_prev.table, lp.table, lp_prev.ncoords, lp.ncoords,
+                                     _width_set(nbins, "many"))
+
+
 def test_score_empty_and_invalid(gpu_device):
     _case("score empty / invalid")
     empty = torch.empty((0, 2, 64), dtype=torch.float32, device="cuda")
@@ -241,12 +273,15 @@ def test_dynamic_programming_api(gpu_device):
     assert norm.shape == (*counts, 64)
 
 
-def test_ffa_batch_matches_single_series(gpu_device):
-    _case("ffa_batch mixed lengths")
+@pytest.mark.parametrize("lens", [[2**17, 2**19, 2**18, 2**17],
+                                  [2**14, 2**13, 2**12, 2**15, 2**16]])
+def test_ffa_batch_matches_single_series(gpu_device, lens):
+    """Mixed lengths, including series with 8, 4, 2 and 1 segments after the brute fold
+    (the two-level, one-level and level-0 scoring paths)."""
+    _case(f"ffa_batch mixed lengths {lens}")
     cfg = _cfg(2, 2**19, 64, 4096)
     plan = FFAPlan(cfg)
     rng = np.random.default_rng(9)
-    lens = [2**17, 2**19, 2**18, 2**17]
     series = []
     for n in lens:
         e = rng.normal(size=n).astype(np.float32)
@@ -257,6 +292,8 @@ def test_ffa_batch_matches_single_series(gpu_device):
     for a, b in zip(out, out_unfused):
         np.testing.assert_array_equal(a.cpu().numpy(), b.cpu().numpy())
     for (e, v), snr, n in zip(series, out, lens):
+        if n == 4096:  # one segment: scored at level 0 (no FFA config exists for it)
+            continue
         sub = PulsarSearchConfig(nsamps=n, tsamp=cfg.tsamp, nbins=64, eta=1.0,
                                  param_limits=cfg.param_limits, bseg_brute=4096,
                                  use_fourier=False)
