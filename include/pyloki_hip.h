@@ -386,6 +386,80 @@ int pyl_cheb_ascend_complex(const pyl_prune_params* p, const double* leaves_dev,
                             float* out_scores_dev, void* stream);
 
 /* ------------------------------------------------------------------------------------
+ * EP level loop without host synchronisation (Taylor and Chebyshev bases, real or complex
+ * folds). Replaces the reference's per-level driver loop prune.py:615-669 execute_iter
+ * (branch, resolve, shift-add, score, threshold, add to the WorldTree) for `nlev`
+ * consecutive levels in one call: every count (parents, children, survivors) stays in
+ * device memory, kernels stride over capacity-sized grids and read their counts there,
+ * and the call only enqueues work. Results equal those of the per-level entry points above
+ * bit for bit.
+ *
+ * A level stops the run, and leaves its input tree untouched, when it cannot be finished
+ * without the host: the branch step failed (the reference's ValueError), the children
+ * exceed cap_child, the passing children exceed max_sugg (the batch-sequential overflow
+ * rule, world_tree.py:482-551, is then replayed on the host) or the backtrack history is
+ * full. Its record then holds a PYL_EP_ST_* status and every later level has no parents;
+ * the caller runs that level through the per-level entry points and calls again.
+ *
+ * Level l reads the tree of level l - 1 (the input tree for l = 0) and writes tree buffer
+ * (first_out + l) % 2 of L. Its survivors' backtracks go to rows rec[l].HIST.. of L->hist
+ * (hist_row0 for l = 0).
+ * coords / thresholds / seg_idx: host arrays [nlev] (the level's coordinates, S/N threshold
+ * and segment index); the segment rows of level l start at dyp_fold_dev + seg_idx[l] *
+ * seg_stride floats.
+ * flags: PYL_EP_COMPLEX (complex64 folds) | PYL_EP_CHEBYSHEV.
+ * ------------------------------------------------------------------------------------ */
+#define PYL_EP_COMPLEX 1
+#define PYL_EP_CHEBYSHEV 2
+#define PYL_EP_SCAN_BLOCKS 256
+
+#define PYL_EP_REC_NPAR 0   /* parents of the level */
+#define PYL_EP_REC_NCHILD 1 /* children (circular: before validation) */
+#define PYL_EP_REC_NSURV 2  /* survivors (the next level's parents) */
+#define PYL_EP_REC_STATUS 3 /* PYL_EP_ST_* */
+#define PYL_EP_REC_ERR 4    /* branch error word, as err_dev of pyl_prune_branch */
+#define PYL_EP_REC_HIST 5   /* first history row of the level's survivors */
+#define PYL_EP_REC_NVALID 6 /* children scored (circular: after validation) */
+#define PYL_EP_REC_WORDS 8
+
+#define PYL_EP_ST_OK 0
+#define PYL_EP_ST_BRANCH 1    /* branch error: rerun the level to raise it */
+#define PYL_EP_ST_CHILD_CAP 2 /* children exceed cap_child */
+#define PYL_EP_ST_OVERFLOW 3  /* passing children exceed max_sugg */
+#define PYL_EP_ST_HIST_FULL 4 /* backtrack history full */
+
+typedef struct pyl_ep_loop {
+  int64_t cap_par;      /* rows of each tree buffer (>= max_sugg and the input tree) */
+  int64_t cap_child;    /* children of one level */
+  int64_t cap_hist;     /* rows of the backtrack history */
+  int64_t max_sugg;     /* survivor buffer of the reference's WorldTree */
+  double* leaves[2];    /* tree buffers: f64 [cap_par][P + 2][2] */
+  float* folds[2];      /*   f32 [cap_par][2][nbins] or complex64 [cap_par][2][nbins/2 + 1] */
+  float* scores[2];     /*   f32 [cap_par] */
+  int32_t* hist;        /* int32 [cap_hist][P + 2]: every level's backtracks, level by level */
+  double* leaves_cur;   /* Chebyshev: f64 [cap_par][P + 2][2] (parents on coord_cur) */
+  double* branch;       /* f64 [cap_par][P][2] */
+  int32_t* counts;      /* int32 [cap_par][P] */
+  int64_t* nchild;      /* int64 [cap_par] */
+  int64_t* off;         /* int64 [cap_par + 1] */
+  int32_t* c_parent;    /* int32 [cap_child] */
+  int32_t* c_row;       /* int32 [cap_child] */
+  int32_t* c_shift;     /* int32 [cap_child] */
+  double* c_phase;      /* f64 [cap_child] */
+  float* c_score;       /* f32 [cap_child] */
+  int64_t* surv;        /* int64 [max_sugg] */
+  int64_t* part;        /* int64 [PYL_EP_SCAN_BLOCKS] */
+  int64_t* rec;         /* int64 [nlev + 1][PYL_EP_REC_WORDS]: level records (written) */
+  float* minmax;        /* f32 [nlev][2]: children's score min/max per level (written) */
+} pyl_ep_loop;
+
+int pyl_ep_levels(const pyl_prune_params* p, int32_t flags, const pyl_prune_coords* coords,
+                  const double* thresholds, const int64_t* seg_idx, int32_t nlev,
+                  const float* dyp_fold_dev, int64_t seg_stride, const double* in_leaves_dev,
+                  const float* in_folds_dev, int64_t n_in, int64_t hist_row0,
+                  int32_t first_out, const pyl_ep_loop* L, void* stream);
+
+/* ------------------------------------------------------------------------------------
  * EP, circular basis (prune_poly_order = 5: crackle, snap, jerk, accel, vel; leaf rows 7).
  * The entry points replace the circular plugin (dyn_circular_taylor.py:484-678) driven
  * by pruning_iteration_batched (prune.py:165-332):

@@ -105,6 +105,9 @@ SIGNATURES: list[tuple[str, object, list[object]]] = [
     ("pyl_prune_ascend", ctypes.c_int,
      [_vp, _vp, _c_i64, _vp, _c_i64, _i64p, _f64p, _c_i32, _c_dbl, _vp, _vp, _vp]),
     ("pyl_cheb_branch", ctypes.c_int, [_vp, _vp, _vp, _c_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("pyl_ep_levels", ctypes.c_int,
+     [_vp, _c_i32, _vp, _f64p, _i64p, _c_i32, _vp, _c_i64, _vp, _vp, _c_i64, _c_i64, _c_i32,
+      _vp, _vp]),
     ("pyl_cnn_num_params", ctypes.c_int, []),
     ("pyl_cnn_cache_floats", ctypes.c_int64, [_c_i32]),
     ("pyl_cnn_forward", ctypes.c_int,

@@ -109,8 +109,8 @@ __device__ __forceinline__ float norm_ev(float e, float v) {
 
 // Four bins at once: one range test, so the four fast sequences interleave.
 __device__ __forceinline__ float4 norm_ev4(float4 e, float4 v) {
-  if (norm_in_range(e.x, v.x) & norm_in_range(e.y, v.y) & norm_in_range(e.z, v.z) &
-      norm_in_range(e.w, v.w))
+  if ((int)norm_in_range(e.x, v.x) & (int)norm_in_range(e.y, v.y) &
+      (int)norm_in_range(e.z, v.z) & (int)norm_in_range(e.w, v.w))
     return make_float4(norm_fast(e.x, v.x), norm_fast(e.y, v.y), norm_fast(e.z, v.z),
                        norm_fast(e.w, v.w));
   return make_float4(e.x / sqrtf(v.x), e.y / sqrtf(v.y), e.z / sqrtf(v.z), e.w / sqrtf(v.w));

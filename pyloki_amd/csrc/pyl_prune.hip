@@ -22,18 +22,26 @@
 // and the GPU only does the per-leaf FP64 matrix-vector products, in ascending column order.
 #include <vector>
 
+#include "pyl_loop.hpp"
 #include "pyl_prune_common.hpp"
 
 namespace {
 
 using namespace pyl_ep;
 
-__global__ void prune_branch_kernel(LevelConst k, const double* __restrict__ leaves, int64_t n,
-                                    double* __restrict__ branch, int32_t* __restrict__ counts,
-                                    int64_t* __restrict__ nchild,
-                                    unsigned long long* __restrict__ err) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// The kernels below take their item count either by value or, when the pointer is not null,
+// from device memory (the sync-free level loop); they stride over items by the grid, so a
+// launch sized to the count does one pass and a capped launch loops.
+__device__ __forceinline__ int64_t dev_count(int64_t n, const int64_t* n_dev) {
+  return n_dev ? *n_dev : n;
+}
+
+// Branching of parent i: per-parameter child count, first centre and step (FP64).
+__device__ __forceinline__ void branch_one(const LevelConst& k, const double* __restrict__ leaves,
+                                           int64_t i, double* __restrict__ branch,
+                                           int32_t* __restrict__ counts,
+                                           int64_t* __restrict__ nchild,
+                                           unsigned long long* __restrict__ err) {
   const int P = k.P;
   const double* lf = leaves + i * (P + 2) * 2;
   const double f0 = lf[(P + 1) * 2];
@@ -72,12 +80,11 @@ __global__ void prune_branch_kernel(LevelConst k, const double* __restrict__ lea
   nchild[i] = total;
 }
 
-// chebyshev.py:72-177, first step: every parent moved from coord_prev to coord_cur
-// (shift_cheby_full, values and widths), thread per parent; f0 and flag copied.
-__global__ void cheb_parents_kernel(LevelConst k, const double* __restrict__ leaves, int64_t n,
-                                    double* __restrict__ leaves_cur) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// chebyshev.py:72-177, first step: parent i moved from coord_prev to coord_cur
+// (shift_cheby_full, values and widths); f0 and flag copied.
+__device__ __forceinline__ void cheb_parent_one(const LevelConst& k,
+                                                const double* __restrict__ leaves, int64_t i,
+                                                double* __restrict__ leaves_cur) {
   const int P = k.P;
   const double* lf = leaves + i * (P + 2) * 2;
   ChildLeaf ch;
@@ -91,14 +98,37 @@ __global__ void cheb_parents_kernel(LevelConst k, const double* __restrict__ lea
   ol[2 * (P + 1) + 1] = lf[2 * (P + 1) + 1];
 }
 
+// Thread per parent.
+__global__ void prune_branch_kernel(LevelConst k, const double* __restrict__ leaves,
+                                    int64_t n_arg, const int64_t* __restrict__ n_dev,
+                                    double* __restrict__ branch, int32_t* __restrict__ counts,
+                                    int64_t* __restrict__ nchild,
+                                    unsigned long long* __restrict__ err) {
+  const int64_t n = dev_count(n_arg, n_dev);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    branch_one(k, leaves, i, branch, counts, nchild, err);
+}
+
+__global__ void cheb_parents_kernel(LevelConst k, const double* __restrict__ leaves,
+                                    int64_t n_arg, const int64_t* __restrict__ n_dev,
+                                    double* __restrict__ leaves_cur) {
+  const int64_t n = dev_count(n_arg, n_dev);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    cheb_parent_one(k, leaves, i, leaves_cur);
+}
+
 // Folds are real [2][B] rows or, with kComplex, complex64 [2][B/2 + 1] rows (as float2).
 // kCheby: the leaves are Chebyshev coefficients on the current domain (chebyshev.py:180-258).
 template <bool kComplex, bool kCheby>
 __global__ void __launch_bounds__(kBlk)
 prune_children_kernel(LevelConst k, const double* __restrict__ leaves,
-                      const float* __restrict__ folds, int64_t n_par,
+                      const float* __restrict__ folds, int64_t n_par_arg,
+                      const int64_t* __restrict__ n_par_dev,
                       const double* __restrict__ branch, const int32_t* __restrict__ counts,
-                      const int64_t* __restrict__ off, int64_t n_child,
+                      const int64_t* __restrict__ off, int64_t n_child_arg,
+                      const int64_t* __restrict__ n_child_dev,
                       const float* __restrict__ seg, int32_t* __restrict__ c_parent,
                       int32_t* __restrict__ c_row, int32_t* __restrict__ c_shift,
                       double* __restrict__ c_phase, float* __restrict__ c_score,
@@ -106,11 +136,13 @@ prune_children_kernel(LevelConst k, const double* __restrict__ leaves,
   extern __shared__ float S[];  // real: [(B + wmax)][kStride]; complex: complex_block_lds
   __shared__ int32_t s_parent[kBlk], s_row[kBlk], s_shift[kBlk];
   const int tid = threadIdx.x;
-  const int64_t c0 = (int64_t)blockIdx.x * kBlk;
-  const int nvalid = (int)std::min<int64_t>(kBlk, n_child - c0);
+  const int64_t n_par = dev_count(n_par_arg, n_par_dev);
+  const int64_t n_child = dev_count(n_child_arg, n_child_dev);
   const int P = k.P;
   const int B = k.nbins;
   const int nb_f = B / 2 + 1;
+  for (int64_t c0 = (int64_t)blockIdx.x * kBlk; c0 < n_child; c0 += (int64_t)gridDim.x * kBlk) {
+  const int nvalid = (int)std::min<int64_t>(kBlk, n_child - c0);
   if (tid < nvalid) {
     const int64_t c = c0 + tid;
     const int64_t par = find_parent(off, n_par, c);
@@ -188,26 +220,34 @@ prune_children_kernel(LevelConst k, const double* __restrict__ leaves,
   }
   // Block min/max of the scores, then one atomic per wavefront.
   block_minmax(sc, tid < nvalid, minmax);
+  __syncthreads();  // LDS is refilled by the next block of children
+  }
 }
 
 template <bool kComplex>
 __global__ void __launch_bounds__(kBlk)
 prune_materialize_kernel(LevelConst k, const double* __restrict__ leaves,
-                         const float* __restrict__ folds, int64_t n_par,
+                         const float* __restrict__ folds,
                          const double* __restrict__ branch, const int32_t* __restrict__ counts,
                          const int64_t* __restrict__ off, const float* __restrict__ seg,
                          const int32_t* __restrict__ c_parent, const int32_t* __restrict__ c_row,
                          const int32_t* __restrict__ c_shift,
                          const double* __restrict__ c_phase, const float* __restrict__ c_score,
-                         const int64_t* __restrict__ surv, int64_t n_surv,
+                         const int64_t* __restrict__ surv, int64_t n_surv_arg,
+                         const int64_t* __restrict__ n_surv_dev,
                          double* __restrict__ o_leaves, float* __restrict__ o_folds,
-                         float* __restrict__ o_scores, int32_t* __restrict__ o_bt) {
+                         float* __restrict__ o_scores, int32_t* __restrict__ o_bt_base,
+                         const int64_t* __restrict__ bt_row0) {
   __shared__ int64_t s_child[kBlk];
   const int tid = threadIdx.x;
-  const int64_t s0 = (int64_t)blockIdx.x * kBlk;
-  const int nvalid = (int)std::min<int64_t>(kBlk, n_surv - s0);
+  const int64_t n_surv = dev_count(n_surv_arg, n_surv_dev);
   const int P = k.P;
   const int B = k.nbins;
+  // Backtracks go to rows bt_row0[0].. of o_bt_base when bt_row0 is given (the level
+  // loop's history), else to rows 0..
+  int32_t* o_bt = o_bt_base + (bt_row0 ? *bt_row0 : 0) * (P + 2);
+  for (int64_t s0 = (int64_t)blockIdx.x * kBlk; s0 < n_surv; s0 += (int64_t)gridDim.x * kBlk) {
+  const int nvalid = (int)std::min<int64_t>(kBlk, n_surv - s0);
   if (tid < nvalid) {
     const int64_t s = s0 + tid;
     const int64_t c = surv[s];
@@ -253,6 +293,8 @@ prune_materialize_kernel(LevelConst k, const double* __restrict__ leaves,
         of[B + b] = fr[B + b] + sr[B + src];
       }
     }
+    __syncthreads();  // s_child is refilled by the next block of survivors
+  }
   }
 }
 
@@ -363,7 +405,8 @@ extern "C" int pyl_prune_branch(const pyl_prune_params* p, const pyl_prune_coord
   if (n_par == 0) return PYL_OK;
   const int threads = 128;
   prune_branch_kernel<<<(unsigned)pyl::ceil_div(n_par, threads), threads, 0,
-                        pyl::as_stream(stream)>>>(k, leaves, n_par, branch, counts, nchild,
+                        pyl::as_stream(stream)>>>(k, leaves, n_par, nullptr, branch, counts,
+                                                  nchild,
                                                   reinterpret_cast<unsigned long long*>(err));
   PYL_CHECK_LAUNCH("prune_branch_kernel");
   return PYL_OK;
@@ -392,8 +435,8 @@ static int prune_children(const pyl_prune_params* p, const pyl_prune_coords* c,
                 "complex folds need a power-of-two nbins (%d)", k.nbins);
   prune_children_kernel<kComplex, kCheby><<<(unsigned)pyl::ceil_div(n_child, kBlk), kBlk, lds,
                                             pyl::as_stream(stream)>>>(
-      k, leaves, folds, n_par, branch, counts, off, n_child, seg, c_parent, c_row, c_shift,
-      c_phase, c_score, minmax);
+      k, leaves, folds, n_par, nullptr, branch, counts, off, n_child, nullptr, seg, c_parent,
+      c_row, c_shift, c_phase, c_score, minmax);
   PYL_CHECK_LAUNCH("prune_children_kernel");
   return PYL_OK;
 }
@@ -441,8 +484,8 @@ static int prune_materialize(const pyl_prune_params* p, const pyl_prune_coords* 
               "null device pointer");
   prune_materialize_kernel<kComplex><<<(unsigned)pyl::ceil_div(n_surv, kBlk), kBlk, 0,
                                        pyl::as_stream(stream)>>>(
-      k, leaves, folds, 0, branch, counts, off, seg, c_parent, c_row, c_shift, c_phase, c_score,
-      surv, n_surv, o_leaves, o_folds, o_scores, o_bt);
+      k, leaves, folds, branch, counts, off, seg, c_parent, c_row, c_shift, c_phase, c_score,
+      surv, n_surv, nullptr, o_leaves, o_folds, o_scores, o_bt, nullptr);
   PYL_CHECK_LAUNCH("prune_materialize_kernel");
   return PYL_OK;
 }
@@ -559,9 +602,10 @@ extern "C" int pyl_cheb_branch(const pyl_prune_params* p, const pyl_prune_coords
   const int threads = 128;
   const unsigned grid = (unsigned)pyl::ceil_div(n_par, threads);
   hipStream_t s = pyl::as_stream(stream);
-  cheb_parents_kernel<<<grid, threads, 0, s>>>(k, leaves, n_par, leaves_cur);
+  cheb_parents_kernel<<<grid, threads, 0, s>>>(k, leaves, n_par, nullptr, leaves_cur);
   PYL_CHECK_LAUNCH("cheb_parents_kernel");
-  prune_branch_kernel<<<grid, threads, 0, s>>>(k, leaves_cur, n_par, branch, counts, nchild,
+  prune_branch_kernel<<<grid, threads, 0, s>>>(k, leaves_cur, n_par, nullptr, branch, counts,
+                                               nchild,
                                                reinterpret_cast<unsigned long long*>(err));
   PYL_CHECK_LAUNCH("prune_branch_kernel");
   return PYL_OK;
@@ -631,4 +675,101 @@ extern "C" int pyl_cheb_ascend_complex(const pyl_prune_params* p, const double* 
                                        float* o_folds, float* o_scores, void* stream) {
   return prune_ascend<true, true>(p, leaves, n, dyp, ncoords, seg_idx, seg_t0, nseg, t0_cur,
                                   o_folds, o_scores, stream, scale_cur);
+}
+
+// ---- The sync-free level loop ------------------------------------------------------
+namespace {
+
+template <bool kComplex, bool kCheby>
+int ep_levels(const pyl_prune_params* p, const pyl_prune_coords* coords, const double* thr,
+              const int64_t* seg_idx, int32_t nlev, const float* fold, int64_t seg_stride,
+              const double* in_leaves, const float* in_folds, int64_t n_in, int64_t hist_row0,
+              int32_t first_out, const pyl_ep_loop* L, hipStream_t s) {
+  PYL_REQUIRE(L != nullptr && coords != nullptr && thr != nullptr && seg_idx != nullptr,
+              "null host argument");
+  PYL_REQUIRE(nlev >= 1, "nlev must be >= 1");
+  PYL_REQUIRE(first_out == 0 || first_out == 1, "first_out must be 0 or 1");
+  PYL_REQUIRE(L->cap_par >= 1 && L->cap_child >= 1 && L->cap_hist >= 0 && L->max_sugg >= 1,
+              "invalid capacities");
+  PYL_REQUIRE(L->max_sugg <= L->cap_par, "max_sugg exceeds the tree buffers");
+  PYL_REQUIRE(n_in >= 0 && n_in <= L->cap_par, "input tree exceeds the tree buffers");
+  PYL_REQUIRE(hist_row0 >= 0 && hist_row0 <= L->cap_hist, "hist_row0 out of range");
+  PYL_REQUIRE(fold && in_leaves && in_folds && L->leaves[0] && L->leaves[1] && L->folds[0] &&
+                  L->folds[1] && L->scores[0] && L->scores[1] && L->hist && L->branch &&
+                  L->counts && L->nchild && L->off && L->c_parent && L->c_row &&
+                  L->c_shift && L->c_phase && L->c_score && L->surv && L->part && L->rec &&
+                  L->minmax && (!kCheby || L->leaves_cur),
+              "null device pointer");
+  PYL_REQUIRE(in_leaves != L->leaves[first_out] && in_folds != L->folds[first_out],
+              "the input tree is the first output buffer");
+  PYL_REQUIRE((int64_t)p->n_accel * p->n_freq <= 0x7fffffff, "FFA grid too large");
+  ep_records_init_kernel<<<(unsigned)pyl::ceil_div(nlev + 1, 64), 64, 0, s>>>(
+      L->rec, L->minmax, nlev, n_in, hist_row0);
+  PYL_CHECK_LAUNCH("ep_records_init_kernel");
+  const int tpar = 128;
+  const unsigned g_par = capped_grid(L->cap_par, tpar, 2048);
+  const unsigned g_child = capped_grid(L->cap_child, kBlk, 4096);
+  const unsigned g_surv = capped_grid(L->max_sugg, kBlk, 4096);
+  for (int l = 0; l < nlev; ++l) {
+    LevelConst k;
+    int rc = kCheby ? build_cheb_const(p, &coords[l], &k) : build_const(p, &coords[l], &k);
+    if (rc != PYL_OK) return rc;
+    const size_t lds = ep_block_lds<kComplex>(k);
+    PYL_REQUIRE(lds <= kMaxEpLds, "nbins too large for the EP scorer (%d)", k.nbins);
+    if constexpr (kComplex)
+      PYL_REQUIRE((k.nbins & (k.nbins - 1)) == 0,
+                  "complex folds need a power-of-two nbins (%d)", k.nbins);
+    PYL_REQUIRE(seg_idx[l] >= 0, "negative segment index");
+    int64_t* rec = L->rec + (int64_t)l * kRecWords;
+    const double* leaves = l == 0 ? in_leaves : L->leaves[(first_out + l - 1) & 1];
+    const float* folds = l == 0 ? in_folds : L->folds[(first_out + l - 1) & 1];
+    const int o = (first_out + l) & 1;
+    const float* seg = fold + seg_idx[l] * seg_stride;
+    const double* src = leaves;
+    if constexpr (kCheby) {
+      cheb_parents_kernel<<<g_par, tpar, 0, s>>>(k, leaves, 0, rec + kRecNpar, L->leaves_cur);
+      PYL_CHECK_LAUNCH("cheb_parents_kernel");
+      src = L->leaves_cur;
+    }
+    prune_branch_kernel<<<g_par, tpar, 0, s>>>(
+        k, src, 0, rec + kRecNpar, L->branch, L->counts, L->nchild,
+        reinterpret_cast<unsigned long long*>(rec + kRecErr));
+    PYL_CHECK_LAUNCH("prune_branch_kernel");
+    chunked_scan(CountSrc{L->nchild}, ChildOffSink{L->off, rec, L->cap_child}, rec + kRecNpar,
+                 L->part, s);
+    PYL_CHECK_LAUNCH("chunk_scan_kernel");
+    prune_children_kernel<kComplex, kCheby><<<g_child, kBlk, lds, s>>>(
+        k, src, folds, 0, rec + kRecNpar, L->branch, L->counts, L->off, 0, rec + kRecNchild,
+        seg, L->c_parent, L->c_row, L->c_shift, L->c_phase, L->c_score, L->minmax + 2 * l);
+    PYL_CHECK_LAUNCH("prune_children_kernel");
+    chunked_scan(PassSrc{L->c_score, thr[l]},
+                 SurvSink{L->surv, rec, L->max_sugg, L->cap_hist}, rec + kRecNchild, L->part,
+                 s);
+    PYL_CHECK_LAUNCH("chunk_scan_kernel");
+    prune_materialize_kernel<kComplex><<<g_surv, kBlk, 0, s>>>(
+        k, src, folds, L->branch, L->counts, L->off, seg, L->c_parent, L->c_row, L->c_shift,
+        L->c_phase, L->c_score, L->surv, 0, rec + kRecWords + kRecNpar, L->leaves[o],
+        L->folds[o], L->scores[o], L->hist, rec + kRecHist);
+    PYL_CHECK_LAUNCH("prune_materialize_kernel");
+  }
+  return PYL_OK;
+}
+
+}  // namespace
+
+extern "C" int pyl_ep_levels(const pyl_prune_params* p, int32_t flags,
+                             const pyl_prune_coords* coords, const double* thresholds,
+                             const int64_t* seg_idx, int32_t nlev, const float* dyp_fold,
+                             int64_t seg_stride, const double* in_leaves, const float* in_folds,
+                             int64_t n_in, int64_t hist_row0, int32_t first_out,
+                             const pyl_ep_loop* L, void* stream) {
+  PYL_REQUIRE(p != nullptr, "null pyl_prune_params");
+  PYL_REQUIRE((flags & ~(PYL_EP_COMPLEX | PYL_EP_CHEBYSHEV)) == 0, "unknown flags %d", flags);
+  const hipStream_t s = pyl::as_stream(stream);
+  const bool cplx = flags & PYL_EP_COMPLEX;
+  const bool cheb = flags & PYL_EP_CHEBYSHEV;
+  auto* f = cplx ? (cheb ? ep_levels<true, true> : ep_levels<true, false>)
+                 : (cheb ? ep_levels<false, true> : ep_levels<false, false>);
+  return f(p, coords, thresholds, seg_idx, nlev, dyp_fold, seg_stride, in_leaves, in_folds,
+           n_in, hist_row0, first_out, L, s);
 }

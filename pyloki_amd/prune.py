@@ -83,6 +83,96 @@ class CircLevel(ctypes.Structure):
 
 REGION_TAYLOR = 2
 
+# Sync-free level loop (pyl_ep_levels): flags, level record words and statuses.
+EP_COMPLEX, EP_CHEBYSHEV = 1, 2
+EP_SCAN_BLOCKS = 256
+REC_NPAR, REC_NCHILD, REC_NSURV, REC_STATUS, REC_ERR, REC_HIST, REC_NVALID = range(7)
+REC_WORDS = 8
+ST_OK, ST_BRANCH, ST_CHILD_CAP, ST_OVERFLOW, ST_HIST_FULL = range(5)
+
+
+class EpLoop(ctypes.Structure):
+    _fields_ = [("cap_par", ctypes.c_int64), ("cap_child", ctypes.c_int64),
+                ("cap_hist", ctypes.c_int64), ("max_sugg", ctypes.c_int64),
+                ("leaves", ctypes.c_void_p * 2), ("folds", ctypes.c_void_p * 2),
+                ("scores", ctypes.c_void_p * 2)] + [(name, ctypes.c_void_p) for name in (
+        "hist", "leaves_cur", "branch", "counts", "nchild", "off", "c_parent", "c_row",
+        "c_shift", "c_phase", "c_score", "surv", "part", "rec", "minmax")]
+
+
+class LoopBuffers:
+    """Device buffers of the sync-free level loop, sized by capacities: two tree buffers
+    of ``cap_par`` rows, the per-child arrays of one level (``cap_child``), the backtrack
+    history (``cap_hist`` rows, appended level after level) and the level records."""
+
+    _CHILD = (("c_parent", torch.int32), ("c_row", torch.int32), ("c_shift", torch.int32),
+              ("c_phase", torch.float64), ("c_score", torch.float32))
+
+    def __init__(self, P: int, new_folds, cap_par: int, cap_child: int, cap_hist: int,
+                 max_sugg: int, nlev: int, cheby: bool) -> None:
+        dev = torch.device("cuda")
+        f64, i32, i64 = torch.float64, torch.int32, torch.int64
+        self.P = P
+        self.leaves = [torch.empty((cap_par, P + 2, 2), dtype=f64, device=dev) for _ in (0, 1)]
+        self.folds = [new_folds(cap_par, dev) for _ in (0, 1)]
+        self.scores = [torch.empty(cap_par, dtype=torch.float32, device=dev) for _ in (0, 1)]
+        self.leaves_cur = (torch.empty((cap_par, P + 2, 2), dtype=f64, device=dev) if cheby
+                           else None)
+        self.branch = torch.empty((cap_par, P, 2), dtype=f64, device=dev)
+        self.counts = torch.empty((cap_par, P), dtype=i32, device=dev)
+        self.nchild = torch.empty(cap_par, dtype=i64, device=dev)
+        self.off = torch.empty(cap_par + 1, dtype=i64, device=dev)
+        self.surv = torch.empty(max_sugg, dtype=i64, device=dev)
+        self.part = torch.empty(EP_SCAN_BLOCKS, dtype=i64, device=dev)
+        self.rec = torch.empty((nlev + 1, REC_WORDS), dtype=i64, device=dev)
+        self.minmax = torch.empty((nlev, 2), dtype=torch.float32, device=dev)
+        self.s = EpLoop()
+        self.s.cap_par, self.s.max_sugg = cap_par, max_sugg
+        for j in (0, 1):
+            self.s.leaves[j] = ptr(self.leaves[j])
+            self.s.folds[j] = ptr(self.folds[j])
+            self.s.scores[j] = ptr(self.scores[j])
+        for name in ("leaves_cur", "branch", "counts", "nchild", "off", "surv", "part", "rec",
+                     "minmax"):
+            setattr(self.s, name, ptr(getattr(self, name)))
+        self.set_children(cap_child)
+        self.new_hist(cap_hist)
+
+    @property
+    def cap_par(self) -> int:
+        return int(self.s.cap_par)
+
+    @property
+    def cap_child(self) -> int:
+        return int(self.s.cap_child)
+
+    @property
+    def nlev(self) -> int:
+        return int(self.minmax.shape[0])
+
+    def set_children(self, cap_child: int) -> None:
+        dev = self.nchild.device
+        for name, dt in self._CHILD:
+            t = torch.empty(cap_child, dtype=dt, device=dev)
+            setattr(self, name, t)
+            setattr(self.s, name, ptr(t))
+        self.s.cap_child = cap_child
+
+    def new_hist(self, cap_hist: int) -> None:
+        """A fresh history (views of the old one stay valid)."""
+        self.hist = torch.empty((cap_hist, self.P + 2), dtype=torch.int32,
+                                device=self.nchild.device)
+        self.s.hist = ptr(self.hist)
+        self.s.cap_hist = cap_hist
+        self.hist_next = 0
+
+    def holds(self, tree) -> int:
+        """Index of the tree buffer that ``tree`` is a view of, else -1."""
+        for j in (0, 1):
+            if tree.leaves.numel() and tree.leaves.data_ptr() == self.leaves[j].data_ptr():
+                return j
+        return -1
+
 
 @dataclass
 class PruneStats:
@@ -257,7 +347,8 @@ class Pruning:
 
     def __init__(self, dyp, threshold_scheme: np.ndarray, max_sugg: int = 2**17,
                  batch_size: int = 1024, poly_basis: str = "taylor", intermediate_k: int = 1,
-                 *, use_moving_grid: bool = True, logger=None) -> None:  # noqa: ARG002
+                 *, use_moving_grid: bool = True, logger=None,  # noqa: ARG002
+                 device_loop: bool = True) -> None:
         cfg = dyp.cfg
         self._circular = cfg.prune_poly_order == 5  # prune.py:369-392: circular plugin
         # (is_circular, poly_basis) pairs with a plugin: circular Taylor, polynomial Taylor,
@@ -316,6 +407,12 @@ class Pruning:
         # (the GPU side of the reference's per-level timers, prune.py:217-322).
         self.timing = False
         self._events: list[tuple[str, torch.cuda.Event, torch.cuda.Event]] = []
+        # execute() runs the levels through the sync-free loop (pyl_ep_levels) when this is
+        # set and no Python level hook is installed; execute_iter() is always per level.
+        self.device_loop = bool(device_loop)
+        # Capacities of the loop's buffers (None: defaults from max_sugg, see _loop_buffers).
+        self.loop_caps: dict[str, int] | None = None
+        self._loop: LoopBuffers | None = None
 
     def _timed(self, phase: str, fn, *args):
         if not self.timing:
@@ -438,6 +535,11 @@ class Pruning:
             new_tree = self.level_hook(self, new_tree)  # e.g. a learned budget (cnn.py)
         st = PruneStats(level=level, seg_idx=seg_idx, threshold=threshold,
                         n_branches=tree.size, n_leaves_surv=new_tree.size, **stats)
+        self._end_level(st, new_tree, log_file, new_tree.backtracks.clone())
+
+    def _end_level(self, st: PruneStats, new_tree: DeviceTree, log_file: Path | None,
+                   backtracks: torch.Tensor) -> None:
+        self._prune_level = st.level
         self._pstats.append(st)
         if log_file is not None:
             with Path(log_file).open("a") as f:
@@ -446,7 +548,83 @@ class Pruning:
         if new_tree.size == 0:
             self._prune_complete = True
             return
-        self._backtrack_hist.append(new_tree.backtracks.clone())
+        self._backtrack_hist.append(backtracks)
+
+    def _use_device_loop(self) -> bool:
+        return self.device_loop and not self._circular and self.level_hook is None
+
+    def _loop_buffers(self, n_in: int, nlev: int) -> LoopBuffers:
+        """The loop's buffers, (re)allocated when too small. Defaults: tree buffers of
+        max(max_sugg, input) rows, 4 * that many children per level, a history of
+        4 * max_sugg rows (at least one level's survivors)."""
+        caps = self.loop_caps or {}
+        cap_par = max(self.max_sugg, n_in)
+        buf = self._loop
+        if buf is None or buf.cap_par < cap_par or buf.nlev < nlev:
+            cap_child = int(caps.get("cap_child", max(2**16, 4 * cap_par)))
+            cap_hist = max(self.max_sugg, int(caps.get("cap_hist", 4 * self.max_sugg)))
+            buf = LoopBuffers(self._P, self._new_folds, cap_par, cap_child, cap_hist,
+                              self.max_sugg, max(nlev, self.dyp.nsegments), self._cheby)
+            self._loop = buf
+        return buf
+
+    def _run_levels(self, last: int, log_file: Path | None = None) -> None:
+        """Levels prune_level + 1 .. last through the sync-free loop (one host transfer
+        per call, for the level records). A level the loop stops at (branch error, child
+        buffer, survivor overflow) runs through execute_iter; a full history is replaced."""
+        lib = _lib.load()
+        fold = self.dyp.fold_device
+        stride = int(fold[0].numel()) * (2 if fold.is_complex() else 1)  # floats per segment
+        flags = (EP_COMPLEX if self._complex else 0) | (EP_CHEBYSHEV if self._cheby else 0)
+        while not self._prune_complete and self._prune_level < last:
+            l0 = self._prune_level + 1
+            levels = range(l0, last + 1)
+            nlev = len(levels)
+            tree = self._world_tree
+            buf = self._loop_buffers(tree.size, nlev)
+            coords = (PruneCoords * nlev)(*[self._coords(lv) for lv in levels])
+            thr = np.ascontiguousarray(self.threshold_scheme[l0 - 1:last], dtype=np.float64)
+            segs = np.array([self.scheme.get_segment_idx(lv) for lv in levels], dtype=np.int64)
+            first_out = 1 if buf.holds(tree) == 0 else 0
+            _lib.check(self._timed(
+                "levels", lib.pyl_ep_levels, ctypes.byref(self._pp), flags, coords,
+                thr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                segs.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), nlev, ptr(fold), stride,
+                ptr(tree.leaves), ptr(tree.folds), tree.size, buf.hist_next, first_out,
+                ctypes.byref(buf.s), stream_ptr()))
+            # one transfer: the records, then the min/max pairs' bits as int64 words
+            info = torch.cat([buf.rec[:nlev + 1].reshape(-1),
+                              buf.minmax[:nlev].view(torch.int64).reshape(-1)]).cpu().numpy()
+            rec = info[:(nlev + 1) * REC_WORDS].reshape(nlev + 1, REC_WORDS)
+            mm = info[(nlev + 1) * REC_WORDS:].view(np.float32).reshape(nlev, 2)
+            stop = ST_OK
+            for i, level in enumerate(levels):
+                r = rec[i]
+                stop = int(r[REC_STATUS])
+                if stop != ST_OK:
+                    break
+                o = (first_out + i) & 1
+                m, h, nc = int(r[REC_NSURV]), int(r[REC_HIST]), int(r[REC_NCHILD])
+                sc = buf.scores[o][:m]
+                new_tree = DeviceTree(buf.leaves[o][:m], buf.folds[o][:m], sc,
+                                      buf.hist[h:h + m], scores_ep=sc)
+                st = PruneStats(level=level, seg_idx=int(segs[i]), threshold=float(thr[i]),
+                                score_min=float(mm[i, 0]) if nc else 0.0,
+                                score_max=float(mm[i, 1]) if nc else 0.0,
+                                n_branches=int(r[REC_NPAR]), n_leaves=nc,
+                                n_leaves_phy=int(r[REC_NVALID]), n_leaves_surv=m)
+                buf.hist_next = h + m
+                self._end_level(st, new_tree, log_file, new_tree.backtracks)
+                if self._prune_complete:
+                    break
+            if stop == ST_OK or self._prune_complete:
+                continue
+            if stop == ST_HIST_FULL:
+                buf.new_hist(2 * int(buf.s.cap_hist))
+                continue
+            self.execute_iter(log_file)  # raises on a branch error; replays an overflow
+            if stop == ST_CHILD_CAP:
+                buf.set_children(2 * max(buf.cap_child, self._pstats[-1].n_leaves))
 
     def _fn(self, lib, name: str):
         """The real or complex-fold variant of an EP entry point."""
@@ -657,10 +835,18 @@ class Pruning:
         """Seed, prune every level, ascend, report (``prune.py:526-613``)."""
         asc = frozenset(ascend_levels) if ascend_levels is not None else None
         self.initialize(ref_seg, log_file)
-        for _i in range(self.dyp.nsegments - 1):
-            self.execute_iter(log_file)
-            if asc is not None and self._prune_level in asc:
-                self._do_ascend()
+        nlast = self.dyp.nsegments - 1
+        if self._use_device_loop():
+            stops = sorted(lv for lv in (asc or ()) if 1 <= lv < nlast)
+            for stop in [*stops, nlast]:
+                self._run_levels(stop, log_file)
+                if asc is not None and self._prune_level == stop and stop in asc:
+                    self._do_ascend()
+        else:
+            for _i in range(nlast):
+                self.execute_iter(log_file)
+                if asc is not None and self._prune_level in asc:
+                    self._do_ascend()
         if asc is None or self._prune_level not in asc:
             self._do_ascend()
         leaves = self._world_tree.leaves.cpu().numpy()

@@ -1,0 +1,94 @@
+"""The sync-free EP level loop (``pyl_ep_levels``, ``Pruning(device_loop=True)``) against
+the per-level path, which the oracle pins bit for bit (test_gpu_prune.py,
+test_gpu_cheby.py, test_gpu_complex.py): reports, scores, per-level statistics and the
+backtrack history must be identical, also when the loop stops at a level and hands it to
+the host (survivor overflow, child buffer, history buffer, branch error)."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import prune_oracle as po
+from pyloki_amd.prune import Pruning
+from tests.test_gpu_cheby import _setup
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(msg: str) -> None:
+    print(f"[case] {msg}", flush=True)
+
+
+def _run(dyp, thresholds, ref, device_loop, caps=None, ascend_levels=None, **kw):
+    pr = Pruning(dyp, thresholds, device_loop=device_loop, **kw)
+    pr.loop_caps = caps
+    res = pr.execute(ref, ascend_levels=ascend_levels)
+    return pr, res
+
+
+def _same(a, b) -> None:
+    (pa, ra), (pb, rb) = a, b
+    np.testing.assert_array_equal(ra["param_sets"], rb["param_sets"])
+    np.testing.assert_array_equal(ra["scores"], rb["scores"])
+    np.testing.assert_array_equal(ra["scores_ep"], rb["scores_ep"])
+    assert ra["level_stats"] == rb["level_stats"]
+    assert pa.prune_level == pb.prune_level and pa.prune_complete == pb.prune_complete
+    assert len(pa.backtrack_hist) == len(pb.backtrack_hist)
+    for lv, (x, y) in enumerate(zip(pa.backtrack_hist, pb.backtrack_hist)):
+        np.testing.assert_array_equal(x.cpu().numpy(), y.cpu().numpy(), err_msg=f"level {lv}")
+    for name in ("leaves", "folds", "scores", "backtracks"):
+        np.testing.assert_array_equal(getattr(pa.world_tree, name).cpu().numpy(),
+                                      getattr(pb.world_tree, name).cpu().numpy(), err_msg=name)
+
+
+@pytest.mark.parametrize("P,nseg,thr,max_sugg,ref,fourier,basis,caps", [
+    (2, 8, 2.5, 2**20, 3, False, "taylor", None),
+    (2, 16, 2.0, 300, 7, False, "taylor", None),             # overflow: host replays levels
+    (3, 8, 3.0, 2**20, 0, False, "taylor", None),            # jerk, reference at the edge
+    (2, 16, 2.0, 2**12, 7, False, "taylor", {"cap_child": 64, "cap_hist": 1}),  # small buffers
+    (2, 16, 2.0, 2**20, 7, True, "taylor", None),            # complex folds
+    (2, 8, 2.5, 2**20, 3, False, "chebyshev", None),
+    (2, 16, 2.0, 300, 7, False, "chebyshev", None),
+    (2, 16, 2.0, 2**20, 7, True, "chebyshev", None),
+])
+def test_loop_equals_per_level_path(gpu_device, P, nseg, thr, max_sugg, ref, fourier, basis,
+                                    caps):
+    _case(f"ep loop P={P} nseg={nseg} max_sugg={max_sugg} ref={ref} fourier={fourier} "
+          f"basis={basis} caps={caps}")
+    cfg, dyp, pcfg = _setup(P, 2**17 if nseg <= 8 else 2**18, nseg, use_fourier=fourier)
+    thresholds = np.linspace(thr, thr + 3.0, dyp.nsegments - 1)
+    kw = {"max_sugg": max_sugg, "batch_size": 64, "poly_basis": basis}
+    slow = _run(dyp, thresholds, ref, False, **kw)
+    fast = _run(dyp, thresholds, ref, True, caps=caps, **kw)
+    _same(fast, slow)
+    st = fast[0].pstats
+    print(f"[result] levels {fast[0].prune_level}, leaves {sum(s.n_leaves for s in st)}, "
+          f"final {len(fast[1]['scores'])}", flush=True)
+    assert len(fast[1]["scores"]) > 0
+
+
+def test_loop_matches_oracle_with_ascend_levels(gpu_device):
+    _case("ep loop with intermediate ascends vs the oracle")
+    cfg, dyp, pcfg = _setup(2, 2**17, 8)
+    thresholds = np.linspace(2.0, 5.0, 7)
+    fast = _run(dyp, thresholds, 3, True, max_sugg=2**18, ascend_levels=[3, 5])
+    slow = _run(dyp, thresholds, 3, False, max_sugg=2**18, ascend_levels=[3, 5])
+    _same(fast, slow)
+    res = _run(dyp, thresholds, 3, True, max_sugg=2**18)[1]
+    exp = po.prune_run(dyp.fold, dyp.param_arr, dyp.dparams_actual, dyp.tseg, thresholds, 3,
+                       pcfg, 2**18)
+    np.testing.assert_array_equal(res["param_sets"], exp["report"])
+    np.testing.assert_array_equal(res["scores"], exp["tree"].scores)
+
+
+def test_loop_branch_error(gpu_device):
+    _case("ep loop branch_max error")
+    cfg, dyp, pcfg = _setup(2, 2**17, 8)
+    pr = Pruning(dyp, np.full(7, 2.0), max_sugg=2**16)
+    pr.initialize(3)
+    with torch.no_grad():
+        pr.world_tree.leaves[:, 0, 1] = 1e9  # absurd accel cell -> too many children
+    with pytest.raises(ValueError, match="increase branch_max"):
+        pr._run_levels(7)
