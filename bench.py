@@ -229,10 +229,7 @@ def ep_jerk_dyp(w: dict = EP_JERK):
 
 def _ep_run(pr, ref_seg: int) -> None:
     pr.initialize(ref_seg)
-    for _ in range(pr.dyp.nsegments - 1):
-        pr.execute_iter()
-        if pr.prune_complete:
-            break
+    pr.execute_levels()
     pr._do_ascend()
 
 

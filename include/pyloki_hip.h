@@ -428,6 +428,19 @@ int pyl_cheb_ascend_complex(const pyl_prune_params* p, const double* leaves_dev,
 #define PYL_EP_ST_OVERFLOW 3  /* passing children exceed max_sugg */
 #define PYL_EP_ST_HIST_FULL 4 /* backtrack history full */
 
+/* Optional per-phase timing of pyl_ep_levels: HIP events recorded on the stream around
+ * each level's phases (PYL_EP_PH_*), read back by pyl_ep_timer_read (which waits for them
+ * and clears the timer). Records beyond the capacity are dropped. */
+typedef struct pyl_ep_timer pyl_ep_timer;
+#define PYL_EP_PH_BRANCH 0      /* branch + child offsets */
+#define PYL_EP_PH_CHILDREN 1    /* resolve + shift-add + score of every child */
+#define PYL_EP_PH_SELECT 2      /* survivor indices */
+#define PYL_EP_PH_MATERIALIZE 3 /* survivors' leaves, folds, backtracks */
+int pyl_ep_timer_create(int32_t capacity, pyl_ep_timer** out);
+int pyl_ep_timer_destroy(pyl_ep_timer* t);
+int pyl_ep_timer_read(pyl_ep_timer* t, int32_t* phase_out, float* ms_out, int32_t cap,
+                      int32_t* n_out);
+
 typedef struct pyl_ep_loop {
   int64_t cap_par;      /* rows of each tree buffer (>= max_sugg and the input tree) */
   int64_t cap_child;    /* children of one level */
@@ -451,6 +464,7 @@ typedef struct pyl_ep_loop {
   int64_t* part;        /* int64 [PYL_EP_SCAN_BLOCKS] */
   int64_t* rec;         /* int64 [nlev + 1][PYL_EP_REC_WORDS]: level records (written) */
   float* minmax;        /* f32 [nlev][2]: children's score min/max per level (written) */
+  pyl_ep_timer* timer;  /* NULL, or the phases are timed */
 } pyl_ep_loop;
 
 int pyl_ep_levels(const pyl_prune_params* p, int32_t flags, const pyl_prune_coords* coords,

@@ -105,6 +105,11 @@ SIGNATURES: list[tuple[str, object, list[object]]] = [
     ("pyl_prune_ascend", ctypes.c_int,
      [_vp, _vp, _c_i64, _vp, _c_i64, _i64p, _f64p, _c_i32, _c_dbl, _vp, _vp, _vp]),
     ("pyl_cheb_branch", ctypes.c_int, [_vp, _vp, _vp, _c_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("pyl_ep_timer_create", ctypes.c_int, [_c_i32, ctypes.POINTER(_vp)]),
+    ("pyl_ep_timer_destroy", ctypes.c_int, [_vp]),
+    ("pyl_ep_timer_read", ctypes.c_int,
+     [_vp, ctypes.POINTER(_c_i32), ctypes.POINTER(ctypes.c_float), _c_i32,
+      ctypes.POINTER(_c_i32)]),
     ("pyl_ep_levels", ctypes.c_int,
      [_vp, _c_i32, _vp, _f64p, _i64p, _c_i32, _vp, _c_i64, _vp, _vp, _c_i64, _c_i64, _c_i32,
       _vp, _vp]),

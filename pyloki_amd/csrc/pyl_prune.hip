@@ -678,6 +678,70 @@ extern "C" int pyl_cheb_ascend_complex(const pyl_prune_params* p, const double* 
 }
 
 // ---- The sync-free level loop ------------------------------------------------------
+struct pyl_ep_timer {
+  int32_t capacity = 0;
+  std::vector<hipEvent_t> ev;  // 2 per record
+  std::vector<int32_t> phase;
+  int32_t n = 0;
+  bool open = false;
+  void begin(int32_t ph, hipStream_t s) {
+    if (n >= capacity) return;
+    phase[n] = ph;
+    hipEventRecord(ev[2 * n], s);
+    open = true;
+  }
+  void end(hipStream_t s) {
+    if (!open) return;
+    hipEventRecord(ev[2 * n + 1], s);
+    ++n;
+    open = false;
+  }
+};
+
+extern "C" int pyl_ep_timer_create(int32_t capacity, pyl_ep_timer** out) {
+  PYL_REQUIRE(out != nullptr && capacity >= 1, "invalid timer arguments");
+  auto* t = new pyl_ep_timer();
+  t->capacity = capacity;
+  t->ev.resize(2 * (size_t)capacity);
+  t->phase.resize(capacity);
+  for (auto& e : t->ev) {
+    if (hipEventCreate(&e) != hipSuccess) {
+      e = nullptr;
+      pyl_ep_timer_destroy(t);
+      return pyl::fail(PYL_ERR_HIP, "hipEventCreate failed");
+    }
+  }
+  *out = t;
+  return PYL_OK;
+}
+
+extern "C" int pyl_ep_timer_destroy(pyl_ep_timer* t) {
+  if (t == nullptr) return PYL_OK;
+  for (auto e : t->ev)
+    if (e) hipEventDestroy(e);
+  delete t;
+  return PYL_OK;
+}
+
+extern "C" int pyl_ep_timer_read(pyl_ep_timer* t, int32_t* phase_out, float* ms_out,
+                                 int32_t cap, int32_t* n_out) {
+  PYL_REQUIRE(t != nullptr && phase_out != nullptr && ms_out != nullptr && n_out != nullptr,
+              "null timer argument");
+  const int32_t n = std::min(t->n, cap);
+  for (int32_t i = 0; i < n; ++i) {
+    if (hipEventSynchronize(t->ev[2 * i + 1]) != hipSuccess)
+      return pyl::fail(PYL_ERR_HIP, "hipEventSynchronize failed");
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, t->ev[2 * i], t->ev[2 * i + 1]) != hipSuccess)
+      return pyl::fail(PYL_ERR_HIP, "hipEventElapsedTime failed");
+    phase_out[i] = t->phase[i];
+    ms_out[i] = ms;
+  }
+  *n_out = n;
+  t->n = 0;
+  return PYL_OK;
+}
+
 namespace {
 
 template <bool kComplex, bool kCheby>
@@ -726,6 +790,8 @@ int ep_levels(const pyl_prune_params* p, const pyl_prune_coords* coords, const d
     const int o = (first_out + l) & 1;
     const float* seg = fold + seg_idx[l] * seg_stride;
     const double* src = leaves;
+    pyl_ep_timer* tm = L->timer;
+    if (tm) tm->begin(PYL_EP_PH_BRANCH, s);
     if constexpr (kCheby) {
       cheb_parents_kernel<<<g_par, tpar, 0, s>>>(k, leaves, 0, rec + kRecNpar, L->leaves_cur);
       PYL_CHECK_LAUNCH("cheb_parents_kernel");
@@ -738,19 +804,26 @@ int ep_levels(const pyl_prune_params* p, const pyl_prune_coords* coords, const d
     chunked_scan(CountSrc{L->nchild}, ChildOffSink{L->off, rec, L->cap_child}, rec + kRecNpar,
                  L->part, s);
     PYL_CHECK_LAUNCH("chunk_scan_kernel");
+    if (tm) tm->end(s);
+    if (tm) tm->begin(PYL_EP_PH_CHILDREN, s);
     prune_children_kernel<kComplex, kCheby><<<g_child, kBlk, lds, s>>>(
         k, src, folds, 0, rec + kRecNpar, L->branch, L->counts, L->off, 0, rec + kRecNchild,
         seg, L->c_parent, L->c_row, L->c_shift, L->c_phase, L->c_score, L->minmax + 2 * l);
     PYL_CHECK_LAUNCH("prune_children_kernel");
+    if (tm) tm->end(s);
+    if (tm) tm->begin(PYL_EP_PH_SELECT, s);
     chunked_scan(PassSrc{L->c_score, thr[l]},
                  SurvSink{L->surv, rec, L->max_sugg, L->cap_hist}, rec + kRecNchild, L->part,
                  s);
     PYL_CHECK_LAUNCH("chunk_scan_kernel");
+    if (tm) tm->end(s);
+    if (tm) tm->begin(PYL_EP_PH_MATERIALIZE, s);
     prune_materialize_kernel<kComplex><<<g_surv, kBlk, 0, s>>>(
         k, src, folds, L->branch, L->counts, L->off, seg, L->c_parent, L->c_row, L->c_shift,
         L->c_phase, L->c_score, L->surv, 0, rec + kRecWords + kRecNpar, L->leaves[o],
         L->folds[o], L->scores[o], L->hist, rec + kRecHist);
     PYL_CHECK_LAUNCH("prune_materialize_kernel");
+    if (tm) tm->end(s);
   }
   return PYL_OK;
 }

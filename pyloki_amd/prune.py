@@ -97,7 +97,8 @@ class EpLoop(ctypes.Structure):
                 ("leaves", ctypes.c_void_p * 2), ("folds", ctypes.c_void_p * 2),
                 ("scores", ctypes.c_void_p * 2)] + [(name, ctypes.c_void_p) for name in (
         "hist", "leaves_cur", "branch", "counts", "nchild", "off", "c_parent", "c_row",
-        "c_shift", "c_phase", "c_score", "surv", "part", "rec", "minmax")]
+        "c_shift", "c_phase", "c_score", "surv", "part", "rec", "minmax", "timer")]
+EP_PHASES = ("branch", "children", "select", "materialize")
 
 
 class LoopBuffers:
@@ -413,6 +414,10 @@ class Pruning:
         # Capacities of the loop's buffers (None: defaults from max_sugg, see _loop_buffers).
         self.loop_caps: dict[str, int] | None = None
         self._loop: LoopBuffers | None = None
+        self._loop_ms: list[tuple[str, float]] = []
+        self._timer = None
+        # Levels the loop handed to the host, by status (ST_*).
+        self.loop_handoffs: dict[int, int] = {}
 
     def _timed(self, phase: str, fn, *args):
         if not self.timing:
@@ -426,12 +431,46 @@ class Pruning:
         return rc
 
     def kernel_times(self) -> dict[str, list[float]]:
-        """Milliseconds of every timed library call, per phase, in call order."""
+        """Milliseconds of every timed library call, per phase, in call order (the level
+        loop's phases are timed inside the loop: branch, children, select, materialize)."""
         torch.cuda.synchronize()
         out: dict[str, list[float]] = {}
         for phase, a, b in self._events:
             out.setdefault(phase, []).append(a.elapsed_time(b))
+        for phase, ms in self._loop_ms:
+            out.setdefault(phase, []).append(ms)
         return out
+
+    def _loop_timer(self, lib, buf: LoopBuffers, nlev: int) -> None:
+        """Attach (timing on) or detach the loop's phase timer."""
+        if not self.timing:
+            buf.s.timer = None
+            return
+        if self._timer is None or self._timer[1] < 4 * nlev:
+            if self._timer is not None:
+                lib.pyl_ep_timer_destroy(self._timer[0])
+            h = ctypes.c_void_p()
+            _lib.check(lib.pyl_ep_timer_create(4 * nlev, ctypes.byref(h)))
+            self._timer = (h, 4 * nlev)
+        buf.s.timer = self._timer[0]
+
+    def _read_loop_timer(self, lib) -> None:
+        if not self.timing or self._timer is None:
+            return
+        cap = self._timer[1]
+        ph = (ctypes.c_int32 * cap)()
+        ms = (ctypes.c_float * cap)()
+        n = ctypes.c_int32()
+        _lib.check(lib.pyl_ep_timer_read(self._timer[0], ph, ms, cap, ctypes.byref(n)))
+        self._loop_ms.extend((EP_PHASES[ph[i]], float(ms[i])) for i in range(n.value))
+
+    def __del__(self) -> None:
+        t = getattr(self, "_timer", None)
+        if t is not None:
+            try:
+                _lib.load().pyl_ep_timer_destroy(t[0])
+            except Exception:  # noqa: BLE001 - interpreter shutdown
+                pass
 
     @property
     def dyp(self):
@@ -568,6 +607,19 @@ class Pruning:
             self._loop = buf
         return buf
 
+    def execute_levels(self, last_level: int | None = None,
+                       log_file: Path | None = None) -> None:
+        """Prune levels prune_level + 1 .. last_level (default: every level left): through
+        the sync-free loop when ``device_loop`` is set and no level hook is installed,
+        else level by level."""
+        nlast = self.dyp.nsegments - 1
+        last = nlast if last_level is None else min(int(last_level), nlast)
+        if self._use_device_loop():
+            self._run_levels(last, log_file)
+            return
+        while not self._prune_complete and self._prune_level < last:
+            self.execute_iter(log_file)
+
     def _run_levels(self, last: int, log_file: Path | None = None) -> None:
         """Levels prune_level + 1 .. last through the sync-free loop (one host transfer
         per call, for the level records). A level the loop stops at (branch error, child
@@ -586,12 +638,14 @@ class Pruning:
             thr = np.ascontiguousarray(self.threshold_scheme[l0 - 1:last], dtype=np.float64)
             segs = np.array([self.scheme.get_segment_idx(lv) for lv in levels], dtype=np.int64)
             first_out = 1 if buf.holds(tree) == 0 else 0
+            self._loop_timer(lib, buf, nlev)
             _lib.check(self._timed(
                 "levels", lib.pyl_ep_levels, ctypes.byref(self._pp), flags, coords,
                 thr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                 segs.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), nlev, ptr(fold), stride,
                 ptr(tree.leaves), ptr(tree.folds), tree.size, buf.hist_next, first_out,
                 ctypes.byref(buf.s), stream_ptr()))
+            self._read_loop_timer(lib)
             # one transfer: the records, then the min/max pairs' bits as int64 words
             info = torch.cat([buf.rec[:nlev + 1].reshape(-1),
                               buf.minmax[:nlev].view(torch.int64).reshape(-1)]).cpu().numpy()
@@ -619,6 +673,7 @@ class Pruning:
                     break
             if stop == ST_OK or self._prune_complete:
                 continue
+            self.loop_handoffs[stop] = self.loop_handoffs.get(stop, 0) + 1
             if stop == ST_HIST_FULL:
                 buf.new_hist(2 * int(buf.s.cap_hist))
                 continue

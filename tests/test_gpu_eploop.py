@@ -11,7 +11,7 @@ import pytest
 import torch
 
 from oracle import prune_oracle as po
-from pyloki_amd.prune import Pruning
+from pyloki_amd.prune import ST_CHILD_CAP, ST_HIST_FULL, ST_OVERFLOW, Pruning
 from tests.test_gpu_cheby import _setup
 
 pytestmark = pytest.mark.gpu
@@ -63,7 +63,15 @@ def test_loop_equals_per_level_path(gpu_device, P, nseg, thr, max_sugg, ref, fou
     slow = _run(dyp, thresholds, ref, False, **kw)
     fast = _run(dyp, thresholds, ref, True, caps=caps, **kw)
     _same(fast, slow)
+    hand = fast[0].loop_handoffs
+    if max_sugg == 300:
+        assert hand.get(ST_OVERFLOW, 0) >= 1
     st = fast[0].pstats
+    if caps is not None:
+        assert hand.get(ST_CHILD_CAP, 0) >= 1
+        if sum(s.n_leaves_surv for s in st[1:]) > 4096:  # more than the first history holds
+            assert hand.get(ST_HIST_FULL, 0) >= 1
+            assert fast[0]._loop.hist.shape[0] > 4096
     print(f"[result] levels {fast[0].prune_level}, leaves {sum(s.n_leaves for s in st)}, "
           f"final {len(fast[1]['scores'])}", flush=True)
     assert len(fast[1]["scores"]) > 0
