@@ -130,17 +130,21 @@ void chunked_scan(Src src, Sink sink, const int64_t* n_dev, int64_t* part, hipSt
 }
 
 // Children of a level: off[i + 1] = inclusive count; the total becomes the level's child
-// count unless the branch step failed or the children exceed the buffers.
+// count unless the branch step failed, the children exceed the buffers or the history
+// could not take a full survivor buffer (checked here, before any child is scored).
 struct ChildOffSink {
   int64_t* off;
   int64_t* rec;  // this level's record
   int64_t cap_child;
+  int64_t max_sugg;
+  int64_t cap_hist;
   __device__ void operator()(int64_t i, int64_t excl, int64_t x) const { off[i + 1] = excl + x; }
   __device__ void finish(int64_t total) const {
     off[0] = 0;
     int64_t st = rec[kRecStatus];
     if (st == 0 && rec[kRecErr] != INT64_MAX) st = PYL_EP_ST_BRANCH;
     if (st == 0 && total > cap_child) st = PYL_EP_ST_CHILD_CAP;
+    if (st == 0 && rec[kRecHist] + max_sugg > cap_hist) st = PYL_EP_ST_HIST_FULL;
     rec[kRecStatus] = st;
     rec[kRecNchild] = st ? 0 : total;
     rec[kRecNvalid] = st ? 0 : total;
@@ -188,10 +192,20 @@ __global__ void ep_records_init_kernel(int64_t* __restrict__ rec, float* __restr
   }
 }
 
-// Grid of a launch that strides over at most `cap` items of `per_block` each.
-inline unsigned capped_grid(int64_t cap, int per_block, int64_t max_blocks) {
+// Grid of a launch that strides over at most `cap` items of `per_block` each: one wave of
+// resident blocks (`resident` per CU on every CU), so that no block starts after others
+// have finished their share (a second, partial wave would run its whole share alone).
+template <class K>
+unsigned persistent_grid(K kernel, int threads, size_t lds, int64_t cap, int per_block) {
+  int dev = 0, ncu = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds) !=
+          hipSuccess)
+    ncu = per_cu = 0;
+  const int64_t resident = std::max<int64_t>(1, (int64_t)std::max(ncu, 1) * std::max(per_cu, 1));
   const int64_t g = (cap + per_block - 1) / per_block;
-  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, max_blocks));
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, resident));
 }
 
 }  // namespace

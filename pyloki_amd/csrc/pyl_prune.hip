@@ -771,14 +771,18 @@ int ep_levels(const pyl_prune_params* p, const pyl_prune_coords* coords, const d
       L->rec, L->minmax, nlev, n_in, hist_row0);
   PYL_CHECK_LAUNCH("ep_records_init_kernel");
   const int tpar = 128;
-  const unsigned g_par = capped_grid(L->cap_par, tpar, 2048);
-  const unsigned g_child = capped_grid(L->cap_child, kBlk, 4096);
-  const unsigned g_surv = capped_grid(L->max_sugg, kBlk, 4096);
+  unsigned g_par = 0, g_child = 0, g_surv = 0;
   for (int l = 0; l < nlev; ++l) {
     LevelConst k;
     int rc = kCheby ? build_cheb_const(p, &coords[l], &k) : build_const(p, &coords[l], &k);
     if (rc != PYL_OK) return rc;
     const size_t lds = ep_block_lds<kComplex>(k);
+    if (l == 0) {  // the LDS size is the same at every level
+      g_par = persistent_grid(prune_branch_kernel, tpar, 0, L->cap_par, tpar);
+      g_child = persistent_grid(prune_children_kernel<kComplex, kCheby>, kBlk, lds, L->cap_child,
+                                kBlk);
+      g_surv = persistent_grid(prune_materialize_kernel<kComplex>, kBlk, 0, L->max_sugg, kBlk);
+    }
     PYL_REQUIRE(lds <= kMaxEpLds, "nbins too large for the EP scorer (%d)", k.nbins);
     if constexpr (kComplex)
       PYL_REQUIRE((k.nbins & (k.nbins - 1)) == 0,
@@ -801,8 +805,9 @@ int ep_levels(const pyl_prune_params* p, const pyl_prune_coords* coords, const d
         k, src, 0, rec + kRecNpar, L->branch, L->counts, L->nchild,
         reinterpret_cast<unsigned long long*>(rec + kRecErr));
     PYL_CHECK_LAUNCH("prune_branch_kernel");
-    chunked_scan(CountSrc{L->nchild}, ChildOffSink{L->off, rec, L->cap_child}, rec + kRecNpar,
-                 L->part, s);
+    chunked_scan(CountSrc{L->nchild},
+                 ChildOffSink{L->off, rec, L->cap_child, L->max_sugg, L->cap_hist},
+                 rec + kRecNpar, L->part, s);
     PYL_CHECK_LAUNCH("chunk_scan_kernel");
     if (tm) tm->end(s);
     if (tm) tm->begin(PYL_EP_PH_CHILDREN, s);
