@@ -192,20 +192,10 @@ __global__ void ep_records_init_kernel(int64_t* __restrict__ rec, float* __restr
   }
 }
 
-// Grid of a launch that strides over at most `cap` items of `per_block` each: one wave of
-// resident blocks (`resident` per CU on every CU), so that no block starts after others
-// have finished their share (a second, partial wave would run its whole share alone).
-template <class K>
-unsigned persistent_grid(K kernel, int threads, size_t lds, int64_t cap, int per_block) {
-  int dev = 0, ncu = 0, per_cu = 0;
-  if (hipGetDevice(&dev) != hipSuccess ||
-      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds) !=
-          hipSuccess)
-    ncu = per_cu = 0;
-  const int64_t resident = std::max<int64_t>(1, (int64_t)std::max(ncu, 1) * std::max(per_cu, 1));
+// Blocks of a launch that covers `cap` items, `per_block` per block (at most 2^31 - 1).
+inline unsigned capacity_grid(int64_t cap, int per_block) {
   const int64_t g = (cap + per_block - 1) / per_block;
-  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, resident));
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, 0x7fffffff));
 }
 
 }  // namespace

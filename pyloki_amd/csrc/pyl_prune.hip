@@ -30,8 +30,10 @@ namespace {
 using namespace pyl_ep;
 
 // The kernels below take their item count either by value or, when the pointer is not null,
-// from device memory (the sync-free level loop); they stride over items by the grid, so a
-// launch sized to the count does one pass and a capped launch loops.
+// from device memory (the sync-free level loop, whose launches are sized for the buffers'
+// capacity: blocks past the count exit at once). They have no grid-stride loop: a loop
+// makes the compiler keep the level constants live in scalar registers across iterations,
+// which spills them and costs the children kernel a fifth of its speed.
 __device__ __forceinline__ int64_t dev_count(int64_t n, const int64_t* n_dev) {
   return n_dev ? *n_dev : n;
 }
@@ -104,19 +106,15 @@ __global__ void prune_branch_kernel(LevelConst k, const double* __restrict__ lea
                                     double* __restrict__ branch, int32_t* __restrict__ counts,
                                     int64_t* __restrict__ nchild,
                                     unsigned long long* __restrict__ err) {
-  const int64_t n = dev_count(n_arg, n_dev);
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    branch_one(k, leaves, i, branch, counts, nchild, err);
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < dev_count(n_arg, n_dev)) branch_one(k, leaves, i, branch, counts, nchild, err);
 }
 
 __global__ void cheb_parents_kernel(LevelConst k, const double* __restrict__ leaves,
                                     int64_t n_arg, const int64_t* __restrict__ n_dev,
                                     double* __restrict__ leaves_cur) {
-  const int64_t n = dev_count(n_arg, n_dev);
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    cheb_parent_one(k, leaves, i, leaves_cur);
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < dev_count(n_arg, n_dev)) cheb_parent_one(k, leaves, i, leaves_cur);
 }
 
 // Folds are real [2][B] rows or, with kComplex, complex64 [2][B/2 + 1] rows (as float2).
@@ -136,12 +134,13 @@ prune_children_kernel(LevelConst k, const double* __restrict__ leaves,
   extern __shared__ float S[];  // real: [(B + wmax)][kStride]; complex: complex_block_lds
   __shared__ int32_t s_parent[kBlk], s_row[kBlk], s_shift[kBlk];
   const int tid = threadIdx.x;
-  const int64_t n_par = dev_count(n_par_arg, n_par_dev);
   const int64_t n_child = dev_count(n_child_arg, n_child_dev);
+  const int64_t c0 = (int64_t)blockIdx.x * kBlk;
+  if (c0 >= n_child) return;  // the whole block
+  const int64_t n_par = dev_count(n_par_arg, n_par_dev);
   const int P = k.P;
   const int B = k.nbins;
   const int nb_f = B / 2 + 1;
-  for (int64_t c0 = (int64_t)blockIdx.x * kBlk; c0 < n_child; c0 += (int64_t)gridDim.x * kBlk) {
   const int nvalid = (int)std::min<int64_t>(kBlk, n_child - c0);
   if (tid < nvalid) {
     const int64_t c = c0 + tid;
@@ -220,8 +219,6 @@ prune_children_kernel(LevelConst k, const double* __restrict__ leaves,
   }
   // Block min/max of the scores, then one atomic per wavefront.
   block_minmax(sc, tid < nvalid, minmax);
-  __syncthreads();  // LDS is refilled by the next block of children
-  }
 }
 
 template <bool kComplex>
@@ -245,8 +242,9 @@ prune_materialize_kernel(LevelConst k, const double* __restrict__ leaves,
   const int B = k.nbins;
   // Backtracks go to rows bt_row0[0].. of o_bt_base when bt_row0 is given (the level
   // loop's history), else to rows 0..
+  const int64_t s0 = (int64_t)blockIdx.x * kBlk;
+  if (s0 >= n_surv) return;  // the whole block
   int32_t* o_bt = o_bt_base + (bt_row0 ? *bt_row0 : 0) * (P + 2);
-  for (int64_t s0 = (int64_t)blockIdx.x * kBlk; s0 < n_surv; s0 += (int64_t)gridDim.x * kBlk) {
   const int nvalid = (int)std::min<int64_t>(kBlk, n_surv - s0);
   if (tid < nvalid) {
     const int64_t s = s0 + tid;
@@ -293,8 +291,6 @@ prune_materialize_kernel(LevelConst k, const double* __restrict__ leaves,
         of[B + b] = fr[B + b] + sr[B + src];
       }
     }
-    __syncthreads();  // s_child is refilled by the next block of survivors
-  }
   }
 }
 
@@ -777,11 +773,10 @@ int ep_levels(const pyl_prune_params* p, const pyl_prune_coords* coords, const d
     int rc = kCheby ? build_cheb_const(p, &coords[l], &k) : build_const(p, &coords[l], &k);
     if (rc != PYL_OK) return rc;
     const size_t lds = ep_block_lds<kComplex>(k);
-    if (l == 0) {  // the LDS size is the same at every level
-      g_par = persistent_grid(prune_branch_kernel, tpar, 0, L->cap_par, tpar);
-      g_child = persistent_grid(prune_children_kernel<kComplex, kCheby>, kBlk, lds, L->cap_child,
-                                kBlk);
-      g_surv = persistent_grid(prune_materialize_kernel<kComplex>, kBlk, 0, L->max_sugg, kBlk);
+    if (l == 0) {  // grids for the capacities: blocks past a level's count exit at once
+      g_par = capacity_grid(L->cap_par, tpar);
+      g_child = capacity_grid(L->cap_child, kBlk);
+      g_surv = capacity_grid(L->max_sugg, kBlk);
     }
     PYL_REQUIRE(lds <= kMaxEpLds, "nbins too large for the EP scorer (%d)", k.nbins);
     if constexpr (kComplex)
