@@ -90,9 +90,12 @@ __device__ __forceinline__ void cheb_parent_one(const LevelConst& k,
   const int P = k.P;
   const double* lf = leaves + i * (P + 2) * 2;
   ChildLeaf ch;
-  for (int r = 0; r <= P; ++r) {
-    ch.val[r] = lf[2 * r];
-    ch.err[r] = lf[2 * r + 1];
+#pragma unroll
+  for (int r = 0; r < kMaxN; ++r) {
+    if (r <= P) {
+      ch.val[r] = lf[2 * r];
+      ch.err[r] = lf[2 * r + 1];
+    }
   }
   double* ol = leaves_cur + i * (P + 2) * 2;
   taylor_full(k, k.tprev, ch, ol);
@@ -153,15 +156,17 @@ prune_children_kernel(LevelConst k, const double* __restrict__ leaves,
     if constexpr (kCheby) {
       // cheby_to_taylor_param_shift to t_add and t_init (transforms.py:703-737).
       double a[kMaxN], da[kMaxN], di[kMaxN];
-      for (int j = 0; j < n; ++j) a[j] = mat_row(k.tadd, ch.val, n, j);
+#pragma unroll
+      for (int j = 0; j < kMaxN; ++j) a[j] = j < n ? mat_row(k.tadd, ch.val, n, j) : 0.0;
       cheb_to_taylor(k, a, da);
-      for (int j = 0; j < n; ++j) a[j] = mat_row(k.tinit, ch.val, n, j);
+#pragma unroll
+      for (int j = 0; j < kMaxN; ++j) a[j] = j < n ? mat_row(k.tinit, ch.val, n, j) : 0.0;
       cheb_to_taylor(k, a, di);
-      accel = da[P - 2];
-      v_add = da[P - 1];
-      d_add = da[P];
-      v_init = di[P - 1];
-      d_init = di[P];
+      accel = pick(da, P - 2);
+      v_add = pick(da, P - 1);
+      d_add = pick(da, P);
+      v_init = pick(di, P - 1);
+      d_init = pick(di, P);
     } else {
       accel = mat_row(k.tadd, ch.val, n, P - 2);
       v_add = mat_row(k.tadd, ch.val, n, P - 1);

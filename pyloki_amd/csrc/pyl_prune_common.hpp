@@ -167,15 +167,39 @@ int build_cheb_const(const pyl_prune_params* p, const pyl_prune_coords* c, Level
   return PYL_OK;
 }
 
+// Per-leaf arrays hold at most kMaxN values. Their loops run over all kMaxN slots with a
+// guard (j < n), fully unrolled, so that every index is a compile-time constant and the
+// arrays stay in registers instead of per-lane scratch memory; the arithmetic and its order
+// are those of the plain loops. pick / put read and write slot i (run-time) by selects.
+__device__ __forceinline__ double pick(const double* a, int i) {
+  double r = a[0];
+#pragma unroll
+  for (int t = 1; t < kMaxN; ++t) r = i == t ? a[t] : r;
+  return r;
+}
+__device__ __forceinline__ void put(double* a, int i, double v) {
+#pragma unroll
+  for (int t = 0; t < kMaxN; ++t)
+    if (i == t) a[t] = v;
+}
+
 // transforms.py:532-560 cheby_to_taylor of descending coefficients a on the current domain:
 // d_m = ((sum_k a_k R[k][m]) * m!) / scale^m, k ascending, written descending.
 __device__ __forceinline__ void cheb_to_taylor(const LevelConst& k, const double* a,
                                                double* d) {
   const int n = k.P + 1;
-  for (int m = 0; m < n; ++m) {
-    double acc = 0.0;
-    for (int kk = 0; kk < n; ++kk) acc += a[n - 1 - kk] * k.rmat[kk * n + m];
-    d[n - 1 - m] = (acc * k.cfact[m]) / k.tspow[m];
+  double asc[kMaxN];  // a in ascending order: asc[kk] = a[n - 1 - kk]
+#pragma unroll
+  for (int kk = 0; kk < kMaxN; ++kk) asc[kk] = pick(a, n - 1 - kk);
+#pragma unroll
+  for (int m = 0; m < kMaxN; ++m) {
+    if (m < n) {
+      double acc = 0.0;
+#pragma unroll
+      for (int kk = 0; kk < kMaxN; ++kk)
+        if (kk < n) acc += asc[kk] * k.rmat[kk * n + m];
+      put(d, n - 1 - m, (acc * k.cfact[m]) / k.tspow[m]);
+    }
   }
 }
 
@@ -231,16 +255,19 @@ __device__ __forceinline__ void decode_child(const LevelConst& k, const double* 
                                              const int32_t* __restrict__ cn, int64_t local,
                                              ChildLeaf& out) {
   const int P = k.P;
-  for (int j = P - 1; j >= 0; --j) {
-    const int32_t cnt = cn[j];
-    const int64_t n = cnt < 0 ? 1 : cnt;
-    const int64_t ij = local % n;
-    local /= n;
-    out.val[j] = cnt < 0 ? br[2 * j] : br[2 * j] + (double)ij * br[2 * j + 1];
-    out.err[j] = br[2 * j + 1];
+#pragma unroll
+  for (int j = kMaxP - 1; j >= 0; --j) {
+    if (j < P) {
+      const int32_t cnt = cn[j];
+      const int64_t n = cnt < 0 ? 1 : cnt;
+      const int64_t ij = local % n;
+      local /= n;
+      out.val[j] = cnt < 0 ? br[2 * j] : br[2 * j] + (double)ij * br[2 * j + 1];
+      out.err[j] = br[2 * j + 1];
+    }
   }
-  out.val[P] = lf[P * 2];  // d_0 copied from the parent; its error starts at 0
-  out.err[P] = 0.0;
+  put(out.val, P, lf[P * 2]);  // d_0 copied from the parent; its error starts at 0
+  put(out.err, P, 0.0);
   out.f0 = lf[(P + 1) * 2];
   out.flag = lf[(P + 1) * 2 + 1];
 }
@@ -248,7 +275,9 @@ __device__ __forceinline__ void decode_child(const LevelConst& k, const double* 
 __device__ __forceinline__ double mat_row(const double* __restrict__ T, const double* v, int n,
                                           int i) {
   double acc = 0.0;
-  for (int j = 0; j < n; ++j) acc += v[j] * T[i * n + j];
+#pragma unroll
+  for (int j = 0; j < kMaxN; ++j)
+    if (j < n) acc += v[j] * T[i * n + j];
   return acc;
 }
 
@@ -263,23 +292,31 @@ __device__ __forceinline__ float score_column(const LevelConst& k, float* S, int
 __device__ __forceinline__ void taylor_full(const LevelConst& k, const double* T,
                                             const ChildLeaf& ch, double* ol) {
   const int n = k.P + 1;
-  for (int i = 0; i < n; ++i) {
-    ol[2 * i] = mat_row(T, ch.val, n, i);
-    double e;
-    if (k.tiling == PYL_TILING_AGGRESSIVE) {
-      e = ch.err[i] * fabs(T[i * n + i]);
-    } else if (k.tiling == PYL_TILING_CONSERVATIVE) {
-      e = 0.0;
-      for (int j = 0; j < n; ++j) e += ch.err[j] * fabs(T[i * n + j]);
-    } else {
-      e = 0.0;
-      for (int j = 0; j < n; ++j) {
-        const double t = T[i * n + j];
-        e += (ch.err[j] * ch.err[j]) * (t * t);
+#pragma unroll
+  for (int i = 0; i < kMaxN; ++i) {
+    if (i < n) {
+      ol[2 * i] = mat_row(T, ch.val, n, i);
+      double e;
+      if (k.tiling == PYL_TILING_AGGRESSIVE) {
+        e = ch.err[i] * fabs(T[i * n + i]);
+      } else if (k.tiling == PYL_TILING_CONSERVATIVE) {
+        e = 0.0;
+#pragma unroll
+        for (int j = 0; j < kMaxN; ++j)
+          if (j < n) e += ch.err[j] * fabs(T[i * n + j]);
+      } else {
+        e = 0.0;
+#pragma unroll
+        for (int j = 0; j < kMaxN; ++j) {
+          if (j < n) {
+            const double t = T[i * n + j];
+            e += (ch.err[j] * ch.err[j]) * (t * t);
+          }
+        }
+        e = sqrt(e);
       }
-      e = sqrt(e);
+      ol[2 * i + 1] = e;
     }
-    ol[2 * i + 1] = e;
   }
 }
 

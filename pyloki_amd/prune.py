@@ -595,14 +595,14 @@ class Pruning:
     def _loop_buffers(self, n_in: int, nlev: int) -> LoopBuffers:
         """The loop's buffers, (re)allocated when too small. Defaults: tree buffers of
         max(max_sugg, input) rows, 4 * that many children per level, a history of
-        8 * max_sugg rows (at least one level's survivors; a level starts only while a
+        16 * max_sugg rows (at least one level's survivors; a level starts only while a
         full survivor buffer still fits)."""
         caps = self.loop_caps or {}
         cap_par = max(self.max_sugg, n_in)
         buf = self._loop
         if buf is None or buf.cap_par < cap_par or buf.nlev < nlev:
             cap_child = int(caps.get("cap_child", max(2**16, 4 * cap_par)))
-            cap_hist = max(self.max_sugg, int(caps.get("cap_hist", 8 * self.max_sugg)))
+            cap_hist = max(self.max_sugg, int(caps.get("cap_hist", 16 * self.max_sugg)))
             buf = LoopBuffers(self._P, self._new_folds, cap_par, cap_child, cap_hist,
                               self.max_sugg, max(nlev, self.dyp.nsegments), self._cheby)
             self._loop = buf
