@@ -420,7 +420,10 @@ int pyl_cheb_ascend_complex(const pyl_prune_params* p, const double* leaves_dev,
 #define PYL_EP_REC_ERR 4    /* branch error word, as err_dev of pyl_prune_branch */
 #define PYL_EP_REC_HIST 5   /* first history row of the level's survivors */
 #define PYL_EP_REC_NVALID 6 /* children scored (circular: after validation) */
-#define PYL_EP_REC_WORDS 8
+#define PYL_EP_REC_NHELD 7  /* circular: held children (before the crackle expansion) */
+#define PYL_EP_REC_NBATCH 8 /* circular: batches of parents */
+#define PYL_EP_REC_NEXPAND 9 /* circular: held children to expand (0 if the level stops) */
+#define PYL_EP_REC_WORDS 12
 
 #define PYL_EP_ST_OK 0
 #define PYL_EP_ST_BRANCH 1    /* branch error: rerun the level to raise it */
@@ -592,6 +595,37 @@ int pyl_circ_ascend(const pyl_prune_params* p, const pyl_circ_params* q,
                     int64_t ncoords_ffa, const int64_t* seg_idx, const double* seg_t0,
                     int32_t nseg_used, double t0_cur, float* out_folds_dev,
                     float* out_scores_dev, void* stream);
+
+/* Sync-free level loop in the circular basis: pyl_ep_levels for prune_poly_order 5 (same
+ * records, statuses, tree buffers and history; flags: PYL_EP_COMPLEX). The per-batch
+ * child positions, the validation compaction and the per-batch Taylor flags that the
+ * per-level path computes between its calls are computed on the device; C holds their
+ * buffers (n_child below = L->cap_child, n_par = L->cap_par). */
+typedef struct pyl_circ_loop {
+  double* cr_first;    /* f64 [n_par] */
+  int32_t* cr_num;     /* i32 [n_par] */
+  int64_t* off1;       /* i64 [n_par + 1] */
+  int32_t* s1_parent;  /* i32 [n_child] */
+  uint8_t* s1_expand;  /* u8  [n_child] */
+  int64_t* s1_pos;     /* i64 [n_child] */
+  int64_t* held_off;   /* i64 [n_child + 1]: packed prefix sums over held children */
+  int64_t* c_s1;       /* i64 [n_child] */
+  int32_t* c_sub;      /* i32 [n_child] */
+  uint8_t* c_keep;     /* u8  [n_child] */
+  int8_t* c_region;    /* i8  [n_child] */
+  int64_t* valid;      /* i64 [n_child] */
+  int64_t* batch_tot;  /* i64 [n_par]: children per batch of parents */
+  int64_t* batch_nk;   /* i64 [n_par]: held children per batch that do not expand */
+  int64_t* batch_off;  /* i64 [n_par]: first child of every batch */
+  uint8_t* has_taylor; /* u8  [n_par] */
+} pyl_circ_loop;
+
+int pyl_circ_levels(const pyl_prune_params* p, const pyl_circ_params* q, int32_t flags,
+                    const pyl_prune_coords* coords, const double* thresholds,
+                    const int64_t* seg_idx, int32_t nlev, const float* dyp_fold_dev,
+                    int64_t seg_stride, const double* in_leaves_dev, const float* in_folds_dev,
+                    int64_t n_in, int64_t hist_row0, int32_t first_out, const pyl_ep_loop* L,
+                    const pyl_circ_loop* C, void* stream);
 
 /* ------------------------------------------------------------------------------------
  * Learned branch ranking (no counterpart in the reference). A small 1-D CNN scores EP

@@ -20,6 +20,7 @@
 //
 // The circular propagator (transforms.py:248-313) uses the device's FP64 cos/sin; the
 // other operations are the reference's, in its order, with no contraction.
+#include "pyl_loop.hpp"
 #include "pyl_prune_common.hpp"
 
 namespace {
@@ -123,11 +124,18 @@ __device__ __forceinline__ void circ_decode(const LevelConst& k, const double* _
   if (sub >= 0) ch.val[0] = L.cr_first[par] + (double)sub * br[1];
 }
 
+// Item counts come from the level struct or, when the pointer is not null, from device
+// memory (the sync-free level loop, whose launches are sized for the buffers' capacity).
+__device__ __forceinline__ int64_t dev_count(int64_t n, const int64_t* n_dev) {
+  return n_dev ? *n_dev : n;
+}
+
 __global__ void circ_branch_kernel(LevelConst k, CircConst q, const double* __restrict__ leaves,
-                                   pyl_circ_level L, int64_t* __restrict__ nchild,
+                                   pyl_circ_level L, const int64_t* __restrict__ n_dev,
+                                   int64_t* __restrict__ nchild,
                                    unsigned long long* __restrict__ err) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= L.n_par) return;
+  if (i >= dev_count(L.n_par, n_dev)) return;
   const double* lf = leaves + i * kRows * 2;
   const double f0 = lf[(kP + 1) * 2];
   // Failures in the order the reference raises them: per batch, parameters 1..4 of every
@@ -181,10 +189,11 @@ __global__ void circ_branch_kernel(LevelConst k, CircConst q, const double* __re
 }
 
 __global__ void circ_stage1_kernel(LevelConst k, CircConst q, const double* __restrict__ leaves,
-                                   pyl_circ_level L) {
+                                   pyl_circ_level L, const int64_t* __restrict__ npar_dev,
+                                   const int64_t* __restrict__ n_dev) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= L.n_s1) return;
-  const int64_t par = find_parent(L.off1, L.n_par, s);
+  if (s >= dev_count(L.n_s1, n_dev)) return;
+  const int64_t par = find_parent(L.off1, dev_count(L.n_par, npar_dev), s);
   ChildLeaf ch;
   circ_decode(k, leaves, L, par, s, -1, ch);
   // circular.py:169-201: the snap-accel system is not circular but jerk is significant.
@@ -197,9 +206,9 @@ __global__ void circ_stage1_kernel(LevelConst k, CircConst q, const double* __re
 }
 
 __global__ void circ_expand_kernel(LevelConst k, CircConst q, const double* __restrict__ leaves,
-                                   pyl_circ_level L) {
+                                   pyl_circ_level L, const int64_t* __restrict__ n_dev) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= L.n_s1) return;
+  if (s >= dev_count(L.n_s1, n_dev)) return;
   const int64_t par = L.s1_parent[s];
   const bool expand = L.s1_expand[s] != 0;
   const int32_t nout = expand ? L.cr_num[par] : 1;
@@ -219,7 +228,8 @@ template <bool kComplex>
 __global__ void __launch_bounds__(kBlk)
 circ_children_kernel(LevelConst k, CircConst q, const double* __restrict__ leaves,
                      const float* __restrict__ folds, pyl_circ_level L,
-                     const int64_t* __restrict__ valid, int64_t n_valid,
+                     const int64_t* __restrict__ valid, int64_t n_valid_arg,
+                     const int64_t* __restrict__ n_dev,
                      const float* __restrict__ seg, int32_t* __restrict__ c_parent,
                      int32_t* __restrict__ c_row, int32_t* __restrict__ c_shift,
                      double* __restrict__ c_phase, float* __restrict__ c_score,
@@ -227,7 +237,9 @@ circ_children_kernel(LevelConst k, CircConst q, const double* __restrict__ leave
   extern __shared__ float S[];  // [(B + wmax)][kStride]
   __shared__ int32_t s_parent[kBlk], s_row[kBlk], s_shift[kBlk];
   const int tid = threadIdx.x;
+  const int64_t n_valid = dev_count(n_valid_arg, n_dev);
   const int64_t v0 = (int64_t)blockIdx.x * kBlk;
+  if (v0 >= n_valid) return;  // the whole block
   const int nvalid = (int)std::min<int64_t>(kBlk, n_valid - v0);
   const int B = k.nbins;
   if (tid < nvalid) {
@@ -300,12 +312,16 @@ circ_materialize_kernel(LevelConst k, CircConst q, const double* __restrict__ le
                         const int32_t* __restrict__ c_shift, const double* __restrict__ c_phase,
                         const float* __restrict__ c_score,
                         const uint8_t* __restrict__ has_taylor, const int64_t* __restrict__ surv,
-                        int64_t n_surv, double* __restrict__ o_leaves,
+                        int64_t n_surv_arg, const int64_t* __restrict__ n_dev,
+                        double* __restrict__ o_leaves,
                         float* __restrict__ o_folds, float* __restrict__ o_scores,
-                        int32_t* __restrict__ o_bt) {
+                        int32_t* __restrict__ o_bt_base, const int64_t* __restrict__ bt_row0) {
   __shared__ int64_t s_v[kBlk];
   const int tid = threadIdx.x;
+  const int64_t n_surv = dev_count(n_surv_arg, n_dev);
   const int64_t s0 = (int64_t)blockIdx.x * kBlk;
+  if (s0 >= n_surv) return;  // the whole block
+  int32_t* o_bt = o_bt_base + (bt_row0 ? *bt_row0 : 0) * kRows;
   const int nvalid = (int)std::min<int64_t>(kBlk, n_surv - s0);
   const int B = k.nbins;
   if (tid < nvalid) {
@@ -469,7 +485,7 @@ extern "C" int pyl_circ_branch(const pyl_prune_params* p, const pyl_circ_params*
   if (L->n_par == 0) return PYL_OK;
   const int threads = 128;
   circ_branch_kernel<<<(unsigned)pyl::ceil_div(L->n_par, threads), threads, 0,
-                       pyl::as_stream(stream)>>>(k, cc, leaves, *L, nchild1,
+                       pyl::as_stream(stream)>>>(k, cc, leaves, *L, nullptr, nchild1,
                                                  reinterpret_cast<unsigned long long*>(err));
   PYL_CHECK_LAUNCH("circ_branch_kernel");
   return PYL_OK;
@@ -489,7 +505,7 @@ extern "C" int pyl_circ_stage1(const pyl_prune_params* p, const pyl_circ_params*
               "null device pointer");
   const int threads = 128;
   circ_stage1_kernel<<<(unsigned)pyl::ceil_div(L->n_s1, threads), threads, 0,
-                       pyl::as_stream(stream)>>>(k, cc, leaves, *L);
+                       pyl::as_stream(stream)>>>(k, cc, leaves, *L, nullptr, nullptr);
   PYL_CHECK_LAUNCH("circ_stage1_kernel");
   return PYL_OK;
 }
@@ -509,7 +525,7 @@ extern "C" int pyl_circ_expand(const pyl_prune_params* p, const pyl_circ_params*
               "null device pointer");
   const int threads = 128;
   circ_expand_kernel<<<(unsigned)pyl::ceil_div(L->n_s1, threads), threads, 0,
-                       pyl::as_stream(stream)>>>(k, cc, leaves, *L);
+                       pyl::as_stream(stream)>>>(k, cc, leaves, *L, nullptr);
   PYL_CHECK_LAUNCH("circ_expand_kernel");
   return PYL_OK;
 }
@@ -537,8 +553,8 @@ static int circ_children(const pyl_prune_params* p, const pyl_circ_params* q,
                 "complex folds need a power-of-two nbins (%d)", k.nbins);
   circ_children_kernel<kComplex><<<(unsigned)pyl::ceil_div(n_valid, kBlk), kBlk, lds,
                                    pyl::as_stream(stream)>>>(
-      k, cc, leaves, folds, *L, valid, n_valid, seg, c_parent, c_row, c_shift, c_phase, c_score,
-      minmax);
+      k, cc, leaves, folds, *L, valid, n_valid, nullptr, seg, c_parent, c_row, c_shift, c_phase,
+      c_score, minmax);
   PYL_CHECK_LAUNCH("circ_children_kernel");
   return PYL_OK;
 }
@@ -585,7 +601,7 @@ static int circ_materialize(const pyl_prune_params* p, const pyl_circ_params* q,
   circ_materialize_kernel<kComplex><<<(unsigned)pyl::ceil_div(n_surv, kBlk), kBlk, 0,
                                       pyl::as_stream(stream)>>>(
       k, cc, leaves, folds, *L, valid, seg, c_parent, c_row, c_shift, c_phase, c_score,
-      has_taylor, surv, n_surv, o_leaves, o_folds, o_scores, o_bt);
+      has_taylor, surv, n_surv, nullptr, o_leaves, o_folds, o_scores, o_bt, nullptr);
   PYL_CHECK_LAUNCH("circ_materialize_kernel");
   return PYL_OK;
 }
@@ -672,4 +688,249 @@ extern "C" int pyl_circ_ascend_complex(const pyl_prune_params* p, const pyl_circ
                                        float* o_folds, float* o_scores, void* stream) {
   return circ_ascend<true>(p, q, leaves, n, dyp, ncoords, seg_idx, seg_t0, nseg, t0_cur,
                            o_folds, o_scores, stream);
+}
+
+// ---- The sync-free level loop (circular basis) ---------------------------------------
+namespace {
+
+// Held children: the level goes on only if they fit the buffers and every crackle
+// expansion's outputs stay below 2^31 in the packed prefix sums below.
+struct HeldOffSink {
+  int64_t* off1;
+  int64_t* rec;
+  int64_t cap_child, max_sugg, cap_hist, batch_size, branch_max;
+  __device__ void operator()(int64_t i, int64_t excl, int64_t x) const { off1[i + 1] = excl + x; }
+  __device__ void finish(int64_t n_s1) const {
+    off1[0] = 0;
+    int64_t st = rec[kRecStatus];
+    if (st == 0 && rec[kRecErr] != INT64_MAX) st = PYL_EP_ST_BRANCH;
+    if (st == 0 && (n_s1 > cap_child || n_s1 * branch_max > 0x7fffffffLL))
+      st = PYL_EP_ST_CHILD_CAP;
+    if (st == 0 && rec[kRecHist] + max_sugg > cap_hist) st = PYL_EP_ST_HIST_FULL;
+    rec[kRecStatus] = st;
+    rec[PYL_EP_REC_NHELD] = st ? 0 : n_s1;
+    rec[PYL_EP_REC_NBATCH] = st ? 0 : (rec[kRecNpar] + batch_size - 1) / batch_size;
+  }
+};
+
+// Per held child, packed: 1 in the low half when it does not expand, its crackle outputs
+// in the high half when it does (both halves stay below 2^31).
+struct HeldSrc {
+  const uint8_t* expand;
+  const int32_t* s1_parent;
+  const int32_t* cr_num;
+  __device__ int64_t operator()(int64_t s) const {
+    return expand[s] ? ((int64_t)cr_num[s1_parent[s]] << 32) : 1;
+  }
+};
+struct PackedOffSink {
+  int64_t* off;
+  __device__ void operator()(int64_t i, int64_t excl, int64_t x) const { off[i + 1] = excl + x; }
+  __device__ void finish(int64_t) const { off[0] = 0; }
+};
+
+// Children of every batch of parents (circular.py:359-394): its held children that do not
+// expand, then the crackle children of those that do.
+__global__ void circ_batch_totals_kernel(const int64_t* __restrict__ off1,
+                                         const int64_t* __restrict__ held_off,
+                                         const int64_t* __restrict__ rec, int64_t batch_size,
+                                         int64_t* __restrict__ batch_tot,
+                                         int64_t* __restrict__ batch_nk) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= rec[PYL_EP_REC_NBATCH]) return;
+  const int64_t n_par = rec[kRecNpar];
+  const int64_t p0 = b * batch_size;
+  const int64_t p1 = std::min<int64_t>(p0 + batch_size, n_par);
+  const int64_t d = held_off[off1[p1]] - held_off[off1[p0]];
+  const int64_t nk = d & 0xffffffffLL;
+  batch_nk[b] = nk;
+  batch_tot[b] = nk + (d >> 32);
+}
+
+// First child of every batch; the level's children must fit the buffers.
+struct BatchOffSink {
+  int64_t* batch_off;
+  int64_t* rec;
+  int64_t cap_child;
+  __device__ void operator()(int64_t i, int64_t excl, int64_t) const { batch_off[i] = excl; }
+  __device__ void finish(int64_t total) const {
+    int64_t st = rec[kRecStatus];
+    if (st == 0 && total > cap_child) st = PYL_EP_ST_CHILD_CAP;
+    rec[kRecStatus] = st;
+    rec[kRecNchild] = st ? 0 : total;
+    rec[PYL_EP_REC_NEXPAND] = st ? 0 : rec[PYL_EP_REC_NHELD];
+  }
+};
+
+__global__ void circ_positions_kernel(const int64_t* __restrict__ off1,
+                                      const int64_t* __restrict__ held_off,
+                                      const int32_t* __restrict__ s1_parent,
+                                      const uint8_t* __restrict__ s1_expand,
+                                      const int64_t* __restrict__ batch_off,
+                                      const int64_t* __restrict__ batch_nk,
+                                      const int64_t* __restrict__ n_dev, int64_t batch_size,
+                                      int64_t* __restrict__ s1_pos) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= *n_dev) return;
+  const int64_t b = s1_parent[s] / batch_size;
+  const int64_t d = held_off[s] - held_off[off1[b * batch_size]];
+  s1_pos[s] = batch_off[b] + (s1_expand[s] ? batch_nk[b] + (d >> 32) : (d & 0xffffffffLL));
+}
+
+struct ValidSink {
+  int64_t* valid;
+  int64_t* rec;
+  __device__ void operator()(int64_t i, int64_t excl, int64_t x) const {
+    if (x) valid[excl] = i;
+  }
+  __device__ void finish(int64_t nv) const { rec[kRecNvalid] = nv; }
+};
+
+__global__ void circ_taylor_flags_clear_kernel(uint8_t* __restrict__ has_taylor,
+                                               const int64_t* __restrict__ rec) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < rec[PYL_EP_REC_NBATCH]) has_taylor[b] = 0;
+}
+
+// circular.py:506-511: batches whose validated children include a Taylor cell.
+__global__ void circ_taylor_flags_kernel(const int64_t* __restrict__ valid,
+                                         const int8_t* __restrict__ c_region,
+                                         const int32_t* __restrict__ c_parent,
+                                         const int64_t* __restrict__ rec, int64_t batch_size,
+                                         uint8_t* __restrict__ has_taylor) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= rec[kRecNvalid]) return;
+  if (c_region[valid[v]] == PYL_REGION_TAYLOR) has_taylor[c_parent[v] / batch_size] = 1;
+}
+
+template <bool kComplex>
+int circ_levels(const pyl_prune_params* p, const pyl_circ_params* q,
+                const pyl_prune_coords* coords, const double* thr, const int64_t* seg_idx,
+                int32_t nlev, const float* fold, int64_t seg_stride, const double* in_leaves,
+                const float* in_folds, int64_t n_in, int64_t hist_row0, int32_t first_out,
+                const pyl_ep_loop* L, const pyl_circ_loop* C, hipStream_t s) {
+  PYL_REQUIRE(L != nullptr && C != nullptr && coords != nullptr && thr != nullptr &&
+                  seg_idx != nullptr,
+              "null host argument");
+  PYL_REQUIRE(nlev >= 1, "nlev must be >= 1");
+  PYL_REQUIRE(first_out == 0 || first_out == 1, "first_out must be 0 or 1");
+  PYL_REQUIRE(L->cap_par >= 1 && L->cap_child >= 1 && L->cap_child < 0x7fffffffLL &&
+                  L->max_sugg >= 1 && L->max_sugg <= L->cap_par,
+              "invalid capacities");
+  PYL_REQUIRE(n_in >= 0 && n_in <= L->cap_par, "input tree exceeds the tree buffers");
+  PYL_REQUIRE(hist_row0 >= 0 && hist_row0 <= L->cap_hist, "hist_row0 out of range");
+  PYL_REQUIRE(fold && in_leaves && in_folds && L->leaves[0] && L->leaves[1] && L->folds[0] &&
+                  L->folds[1] && L->scores[0] && L->scores[1] && L->hist && L->branch &&
+                  L->counts && L->nchild && L->c_parent && L->c_row && L->c_shift &&
+                  L->c_phase && L->c_score && L->surv && L->part && L->rec && L->minmax &&
+                  C->cr_first && C->cr_num && C->off1 && C->s1_parent && C->s1_expand &&
+                  C->s1_pos && C->held_off && C->c_s1 && C->c_sub && C->c_keep &&
+                  C->c_region && C->valid && C->batch_tot && C->batch_nk && C->batch_off &&
+                  C->has_taylor,
+              "null device pointer");
+  PYL_REQUIRE(in_leaves != L->leaves[first_out] && in_folds != L->folds[first_out],
+              "the input tree is the first output buffer");
+  PYL_REQUIRE((int64_t)p->n_accel * p->n_freq <= 0x7fffffff, "FFA grid too large");
+  ep_records_init_kernel<<<(unsigned)pyl::ceil_div(nlev + 1, 64), 64, 0, s>>>(
+      L->rec, L->minmax, nlev, n_in, hist_row0);
+  PYL_CHECK_LAUNCH("ep_records_init_kernel");
+  pyl_circ_level cl{};
+  cl.branch = L->branch;
+  cl.counts = L->counts;
+  cl.cr_first = C->cr_first;
+  cl.cr_num = C->cr_num;
+  cl.off1 = C->off1;
+  cl.s1_parent = C->s1_parent;
+  cl.s1_expand = C->s1_expand;
+  cl.s1_pos = C->s1_pos;
+  cl.c_s1 = C->c_s1;
+  cl.c_sub = C->c_sub;
+  cl.c_keep = C->c_keep;
+  cl.c_region = C->c_region;
+  const int tpar = 128;
+  const unsigned g_par = capacity_grid(L->cap_par, tpar);
+  const unsigned g_held = capacity_grid(L->cap_child, tpar);
+  const unsigned g_child = capacity_grid(L->cap_child, kBlk);
+  const unsigned g_surv = capacity_grid(L->max_sugg, kBlk);
+  for (int l = 0; l < nlev; ++l) {
+    LevelConst k;
+    CircConst cc;
+    int rc = build_circ(p, q, &coords[l], &k, &cc);
+    if (rc != PYL_OK) return rc;
+    const size_t lds = ep_block_lds<kComplex>(k);
+    PYL_REQUIRE(lds <= kMaxEpLds, "nbins too large for the EP scorer (%d)", k.nbins);
+    if constexpr (kComplex)
+      PYL_REQUIRE((k.nbins & (k.nbins - 1)) == 0,
+                  "complex folds need a power-of-two nbins (%d)", k.nbins);
+    PYL_REQUIRE(seg_idx[l] >= 0, "negative segment index");
+    int64_t* rec = L->rec + (int64_t)l * kRecWords;
+    const double* leaves = l == 0 ? in_leaves : L->leaves[(first_out + l - 1) & 1];
+    const float* folds = l == 0 ? in_folds : L->folds[(first_out + l - 1) & 1];
+    const int o = (first_out + l) & 1;
+    const float* seg = fold + seg_idx[l] * seg_stride;
+    const int64_t bs = cc.batch_size;
+    pyl_ep_timer* tm = L->timer;
+    if (tm) tm->begin(PYL_EP_PH_BRANCH, s);
+    circ_branch_kernel<<<g_par, tpar, 0, s>>>(
+        k, cc, leaves, cl, rec + kRecNpar, L->nchild,
+        reinterpret_cast<unsigned long long*>(rec + kRecErr));
+    chunked_scan(CountSrc{L->nchild},
+                 HeldOffSink{C->off1, rec, L->cap_child, L->max_sugg, L->cap_hist, bs,
+                             (int64_t)std::max(p->branch_max, 1)},
+                 rec + kRecNpar, L->part, s);
+    circ_stage1_kernel<<<g_held, tpar, 0, s>>>(k, cc, leaves, cl, rec + kRecNpar,
+                                              rec + PYL_EP_REC_NHELD);
+    chunked_scan(HeldSrc{C->s1_expand, C->s1_parent, C->cr_num}, PackedOffSink{C->held_off},
+                 rec + PYL_EP_REC_NHELD, L->part, s);
+    circ_batch_totals_kernel<<<g_par, tpar, 0, s>>>(C->off1, C->held_off, rec, bs, C->batch_tot,
+                                                   C->batch_nk);
+    chunked_scan(CountSrc{C->batch_tot}, BatchOffSink{C->batch_off, rec, L->cap_child},
+                 rec + PYL_EP_REC_NBATCH, L->part, s);
+    circ_positions_kernel<<<g_held, tpar, 0, s>>>(C->off1, C->held_off, C->s1_parent,
+                                                 C->s1_expand, C->batch_off, C->batch_nk,
+                                                 rec + PYL_EP_REC_NEXPAND, bs, C->s1_pos);
+    circ_expand_kernel<<<g_held, tpar, 0, s>>>(k, cc, leaves, cl, rec + PYL_EP_REC_NEXPAND);
+    chunked_scan(FlagSrc{C->c_keep}, ValidSink{C->valid, rec}, rec + kRecNchild, L->part, s);
+    PYL_CHECK_LAUNCH("circ_expand_kernel");
+    if (tm) tm->end(s);
+    if (tm) tm->begin(PYL_EP_PH_CHILDREN, s);
+    circ_children_kernel<kComplex><<<g_child, kBlk, lds, s>>>(
+        k, cc, leaves, folds, cl, C->valid, 0, rec + kRecNvalid, seg, L->c_parent, L->c_row,
+        L->c_shift, L->c_phase, L->c_score, L->minmax + 2 * l);
+    PYL_CHECK_LAUNCH("circ_children_kernel");
+    if (tm) tm->end(s);
+    if (tm) tm->begin(PYL_EP_PH_SELECT, s);
+    circ_taylor_flags_clear_kernel<<<g_par, tpar, 0, s>>>(C->has_taylor, rec);
+    circ_taylor_flags_kernel<<<g_held, tpar, 0, s>>>(C->valid, C->c_region, L->c_parent, rec,
+                                                    bs, C->has_taylor);
+    chunked_scan(PassSrc{L->c_score, thr[l]},
+                 SurvSink{L->surv, rec, L->max_sugg, L->cap_hist}, rec + kRecNvalid, L->part,
+                 s);
+    PYL_CHECK_LAUNCH("chunk_scan_kernel");
+    if (tm) tm->end(s);
+    if (tm) tm->begin(PYL_EP_PH_MATERIALIZE, s);
+    circ_materialize_kernel<kComplex><<<g_surv, kBlk, 0, s>>>(
+        k, cc, leaves, folds, cl, C->valid, seg, L->c_parent, L->c_row, L->c_shift,
+        L->c_phase, L->c_score, C->has_taylor, L->surv, 0, rec + kRecWords + kRecNpar,
+        L->leaves[o], L->folds[o], L->scores[o], L->hist, rec + kRecHist);
+    PYL_CHECK_LAUNCH("circ_materialize_kernel");
+    if (tm) tm->end(s);
+  }
+  return PYL_OK;
+}
+
+}  // namespace
+
+extern "C" int pyl_circ_levels(const pyl_prune_params* p, const pyl_circ_params* q,
+                               int32_t flags, const pyl_prune_coords* coords,
+                               const double* thresholds, const int64_t* seg_idx, int32_t nlev,
+                               const float* dyp_fold, int64_t seg_stride,
+                               const double* in_leaves, const float* in_folds, int64_t n_in,
+                               int64_t hist_row0, int32_t first_out, const pyl_ep_loop* L,
+                               const pyl_circ_loop* C, void* stream) {
+  PYL_REQUIRE(p != nullptr && q != nullptr, "null parameters");
+  PYL_REQUIRE((flags & ~PYL_EP_COMPLEX) == 0, "unknown flags %d", flags);
+  auto* f = (flags & PYL_EP_COMPLEX) ? circ_levels<true> : circ_levels<false>;
+  return f(p, q, coords, thresholds, seg_idx, nlev, dyp_fold, seg_stride, in_leaves, in_folds,
+           n_in, hist_row0, first_out, L, C, pyl::as_stream(stream));
 }

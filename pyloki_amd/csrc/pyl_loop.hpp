@@ -10,8 +10,30 @@
 #pragma once
 
 #include <cstdint>
+#include <vector>
 
 #include "pyl_common.hpp"
+
+// Phase timer of the level loops (opaque in the C ABI): event pairs on the loop's stream.
+struct pyl_ep_timer {
+  int32_t capacity = 0;
+  std::vector<hipEvent_t> ev;  // 2 per record
+  std::vector<int32_t> phase;
+  int32_t n = 0;
+  bool open = false;
+  void begin(int32_t ph, hipStream_t s) {
+    if (n >= capacity) return;
+    phase[n] = ph;
+    (void)hipEventRecord(ev[2 * n], s);
+    open = true;
+  }
+  void end(hipStream_t s) {
+    if (!open) return;
+    (void)hipEventRecord(ev[2 * n + 1], s);
+    ++n;
+    open = false;
+  }
+};
 
 namespace pyl_ep {
 namespace {  // internal linkage: every translation unit gets its own copy

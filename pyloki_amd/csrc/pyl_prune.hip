@@ -679,26 +679,6 @@ extern "C" int pyl_cheb_ascend_complex(const pyl_prune_params* p, const double* 
 }
 
 // ---- The sync-free level loop ------------------------------------------------------
-struct pyl_ep_timer {
-  int32_t capacity = 0;
-  std::vector<hipEvent_t> ev;  // 2 per record
-  std::vector<int32_t> phase;
-  int32_t n = 0;
-  bool open = false;
-  void begin(int32_t ph, hipStream_t s) {
-    if (n >= capacity) return;
-    phase[n] = ph;
-    hipEventRecord(ev[2 * n], s);
-    open = true;
-  }
-  void end(hipStream_t s) {
-    if (!open) return;
-    hipEventRecord(ev[2 * n + 1], s);
-    ++n;
-    open = false;
-  }
-};
-
 extern "C" int pyl_ep_timer_create(int32_t capacity, pyl_ep_timer** out) {
   PYL_REQUIRE(out != nullptr && capacity >= 1, "invalid timer arguments");
   auto* t = new pyl_ep_timer();

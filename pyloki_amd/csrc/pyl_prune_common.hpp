@@ -142,7 +142,7 @@ inline void cheb_matrix_t(int K, double tc1, double ts1, double tc2, double ts2,
 // The Chebyshev tables of one level on top of the Taylor ones: step eta/nbins * C / f0 and
 // shift factor nbins for every coefficient (psr_utils.py:258-278), transfer matrices and
 // the conversion to Taylor coefficients on the current domain.
-int build_cheb_const(const pyl_prune_params* p, const pyl_prune_coords* c, LevelConst* k) {
+[[maybe_unused]] int build_cheb_const(const pyl_prune_params* p, const pyl_prune_coords* c, LevelConst* k) {
   int rc = build_const(p, c, k);
   if (rc != PYL_OK) return rc;
   const int P = k->P;

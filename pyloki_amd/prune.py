@@ -87,7 +87,7 @@ REGION_TAYLOR = 2
 EP_COMPLEX, EP_CHEBYSHEV = 1, 2
 EP_SCAN_BLOCKS = 256
 REC_NPAR, REC_NCHILD, REC_NSURV, REC_STATUS, REC_ERR, REC_HIST, REC_NVALID = range(7)
-REC_WORDS = 8
+REC_WORDS = 12
 ST_OK, ST_BRANCH, ST_CHILD_CAP, ST_OVERFLOW, ST_HIST_FULL = range(5)
 
 
@@ -101,6 +101,13 @@ class EpLoop(ctypes.Structure):
 EP_PHASES = ("branch", "children", "select", "materialize")
 
 
+class CircLoop(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_void_p) for name in (
+        "cr_first", "cr_num", "off1", "s1_parent", "s1_expand", "s1_pos", "held_off", "c_s1",
+        "c_sub", "c_keep", "c_region", "valid", "batch_tot", "batch_nk", "batch_off",
+        "has_taylor")]
+
+
 class LoopBuffers:
     """Device buffers of the sync-free level loop, sized by capacities: two tree buffers
     of ``cap_par`` rows, the per-child arrays of one level (``cap_child``), the backtrack
@@ -109,8 +116,18 @@ class LoopBuffers:
     _CHILD = (("c_parent", torch.int32), ("c_row", torch.int32), ("c_shift", torch.int32),
               ("c_phase", torch.float64), ("c_score", torch.float32))
 
+    _CIRC_PAR = (("cr_first", torch.float64, 0), ("cr_num", torch.int32, 0),
+                 ("off1", torch.int64, 1), ("batch_tot", torch.int64, 0),
+                 ("batch_nk", torch.int64, 0), ("batch_off", torch.int64, 0),
+                 ("has_taylor", torch.uint8, 0))
+    _CIRC_CHILD = (("s1_parent", torch.int32, 0), ("s1_expand", torch.uint8, 0),
+                   ("s1_pos", torch.int64, 0), ("held_off", torch.int64, 1),
+                   ("c_s1", torch.int64, 0), ("c_sub", torch.int32, 0),
+                   ("c_keep", torch.uint8, 0), ("c_region", torch.int8, 0),
+                   ("valid", torch.int64, 0))
+
     def __init__(self, P: int, new_folds, cap_par: int, cap_child: int, cap_hist: int,
-                 max_sugg: int, nlev: int, cheby: bool) -> None:
+                 max_sugg: int, nlev: int, cheby: bool, circular: bool = False) -> None:
         dev = torch.device("cuda")
         f64, i32, i64 = torch.float64, torch.int32, torch.int64
         self.P = P
@@ -129,6 +146,12 @@ class LoopBuffers:
         self.minmax = torch.empty((nlev, 2), dtype=torch.float32, device=dev)
         self.s = EpLoop()
         self.s.cap_par, self.s.max_sugg = cap_par, max_sugg
+        self.c = CircLoop() if circular else None
+        if circular:
+            for name, dt, extra in self._CIRC_PAR:
+                t = torch.empty(cap_par + extra, dtype=dt, device=dev)
+                setattr(self, name, t)
+                setattr(self.c, name, ptr(t))
         for j in (0, 1):
             self.s.leaves[j] = ptr(self.leaves[j])
             self.s.folds[j] = ptr(self.folds[j])
@@ -157,6 +180,11 @@ class LoopBuffers:
             t = torch.empty(cap_child, dtype=dt, device=dev)
             setattr(self, name, t)
             setattr(self.s, name, ptr(t))
+        if self.c is not None:
+            for name, dt, extra in self._CIRC_CHILD:
+                t = torch.empty(cap_child + extra, dtype=dt, device=dev)
+                setattr(self, name, t)
+                setattr(self.c, name, ptr(t))
         self.s.cap_child = cap_child
 
     def new_hist(self, cap_hist: int) -> None:
@@ -590,7 +618,7 @@ class Pruning:
         self._backtrack_hist.append(backtracks)
 
     def _use_device_loop(self) -> bool:
-        return self.device_loop and not self._circular and self.level_hook is None
+        return self.device_loop and self.level_hook is None
 
     def _loop_buffers(self, n_in: int, nlev: int) -> LoopBuffers:
         """The loop's buffers, (re)allocated when too small. Defaults: tree buffers of
@@ -604,7 +632,8 @@ class Pruning:
             cap_child = int(caps.get("cap_child", max(2**16, 4 * cap_par)))
             cap_hist = max(self.max_sugg, int(caps.get("cap_hist", 16 * self.max_sugg)))
             buf = LoopBuffers(self._P, self._new_folds, cap_par, cap_child, cap_hist,
-                              self.max_sugg, max(nlev, self.dyp.nsegments), self._cheby)
+                              self.max_sugg, max(nlev, self.dyp.nsegments), self._cheby,
+                              self._circular)
             self._loop = buf
         return buf
 
@@ -640,12 +669,17 @@ class Pruning:
             segs = np.array([self.scheme.get_segment_idx(lv) for lv in levels], dtype=np.int64)
             first_out = 1 if buf.holds(tree) == 0 else 0
             self._loop_timer(lib, buf, nlev)
-            _lib.check(self._timed(
-                "levels", lib.pyl_ep_levels, ctypes.byref(self._pp), flags, coords,
-                thr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
-                segs.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), nlev, ptr(fold), stride,
-                ptr(tree.leaves), ptr(tree.folds), tree.size, buf.hist_next, first_out,
-                ctypes.byref(buf.s), stream_ptr()))
+            args = (coords, thr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                    segs.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), nlev, ptr(fold),
+                    stride, ptr(tree.leaves), ptr(tree.folds), tree.size, buf.hist_next,
+                    first_out, ctypes.byref(buf.s))
+            if self._circular:
+                _lib.check(self._timed(
+                    "levels", lib.pyl_circ_levels, ctypes.byref(self._pp),
+                    ctypes.byref(self._qq), flags, *args, ctypes.byref(buf.c), stream_ptr()))
+            else:
+                _lib.check(self._timed("levels", lib.pyl_ep_levels, ctypes.byref(self._pp),
+                                       flags, *args, stream_ptr()))
             self._read_loop_timer(lib)
             # one transfer: the records, then the min/max pairs' bits as int64 words
             info = torch.cat([buf.rec[:nlev + 1].reshape(-1),
@@ -659,15 +693,15 @@ class Pruning:
                 if stop != ST_OK:
                     break
                 o = (first_out + i) & 1
-                m, h, nc = int(r[REC_NSURV]), int(r[REC_HIST]), int(r[REC_NCHILD])
+                m, h, nv = int(r[REC_NSURV]), int(r[REC_HIST]), int(r[REC_NVALID])
                 sc = buf.scores[o][:m]
                 new_tree = DeviceTree(buf.leaves[o][:m], buf.folds[o][:m], sc,
                                       buf.hist[h:h + m], scores_ep=sc)
                 st = PruneStats(level=level, seg_idx=int(segs[i]), threshold=float(thr[i]),
-                                score_min=float(mm[i, 0]) if nc else 0.0,
-                                score_max=float(mm[i, 1]) if nc else 0.0,
-                                n_branches=int(r[REC_NPAR]), n_leaves=nc,
-                                n_leaves_phy=int(r[REC_NVALID]), n_leaves_surv=m)
+                                score_min=float(mm[i, 0]) if nv else 0.0,
+                                score_max=float(mm[i, 1]) if nv else 0.0,
+                                n_branches=int(r[REC_NPAR]), n_leaves=int(r[REC_NCHILD]),
+                                n_leaves_phy=nv, n_leaves_surv=m)
                 buf.hist_next = h + m
                 self._end_level(st, new_tree, log_file, new_tree.backtracks)
                 if self._prune_complete:

@@ -24,14 +24,14 @@ def _case(msg: str) -> None:
     print(f"[case] {msg}", flush=True)
 
 
-def _setup(nsamps=2**18, nseg=32, p_orb_min=20.0):
+def _setup(nsamps=2**18, nseg=32, p_orb_min=20.0, use_fourier=False):
     tsamp = 64e-6
     lim = ParamLimits.from_circular((199.0, 201.0), p_orb_min=p_orb_min, m_c_max=0.2,
                                     m_p_min=1.4).limits
     cfg = PulsarSearchConfig(nsamps=nsamps, tsamp=tsamp, nbins=32, eta=1.0, param_limits=lim,
                              bseg_brute=nsamps // 256, bseg_ffa=nsamps // nseg,
                              prune_poly_order=5, p_orb_min=p_orb_min, m_c_max=0.2, m_p_min=1.4,
-                             ducy_max=0.3, wtsp=1.5, use_fourier=False, branch_max=32)
+                             ducy_max=0.3, wtsp=1.5, use_fourier=use_fourier, branch_max=32)
     sim = PulseSignalConfig(period=1 / 200.17, dt=tsamp, nsamps=nsamps, snr=30.0, ducy=0.1,
                             mod_kwargs={"p_orb": 30.0, "x_orb": 0.002, "phi": 0.7})
     dyp = DynamicProgramming(sim.generate(seed=3), cfg)

@@ -13,6 +13,7 @@ import torch
 from oracle import prune_oracle as po
 from pyloki_amd.prune import ST_CHILD_CAP, ST_HIST_FULL, ST_OVERFLOW, Pruning
 from tests.test_gpu_cheby import _setup
+from tests.test_gpu_circ import _setup as _circ_setup
 
 pytestmark = pytest.mark.gpu
 
@@ -100,3 +101,42 @@ def test_loop_branch_error(gpu_device):
         pr.world_tree.leaves[:, 0, 1] = 1e9  # absurd accel cell -> too many children
     with pytest.raises(ValueError, match="increase branch_max"):
         pr._run_levels(7)
+
+
+@pytest.mark.parametrize("batch_size,max_sugg,fourier,caps", [
+    (1024, 2**16, False, None),
+    (256, 2**16, False, None),
+    (256, 300, False, None),                                  # overflow
+    (64, 2**12, False, {"cap_child": 256, "cap_hist": 1}),    # small buffers
+    (256, 2**16, True, None),                                 # complex folds
+])
+def test_circ_loop_equals_per_level_path(gpu_device, batch_size, max_sugg, fourier, caps):
+    _case(f"circular ep loop batch_size={batch_size} max_sugg={max_sugg} fourier={fourier} "
+          f"caps={caps}")
+    cfg, dyp, pcfg = _circ_setup(use_fourier=fourier)
+    nseg = dyp.nsegments
+    thresholds = np.linspace(2.0, 8.0, nseg - 1)
+    kw = {"max_sugg": max_sugg, "batch_size": batch_size}
+    slow = _run(dyp, thresholds, nseg // 2, False, **kw)
+    fast = _run(dyp, thresholds, nseg // 2, True, caps=caps, **kw)
+    _same(fast, slow)
+    hand = fast[0].loop_handoffs
+    if max_sugg == 300:
+        assert hand.get(ST_OVERFLOW, 0) >= 1
+    if caps is not None:
+        assert hand.get(ST_CHILD_CAP, 0) >= 1
+    st = fast[0].pstats
+    print(f"[result] levels {fast[0].prune_level}, leaves {sum(s.n_leaves for s in st)}, "
+          f"validated {sum(s.n_leaves_phy for s in st)}, final {len(fast[1]['scores'])}, "
+          f"hand-offs {hand}", flush=True)
+
+
+def test_circ_loop_branch_error(gpu_device):
+    _case("circular ep loop branch_max error")
+    cfg, dyp, pcfg = _circ_setup()
+    pr = Pruning(dyp, np.full(dyp.nsegments - 1, 2.0), max_sugg=2**16)
+    pr.initialize(dyp.nsegments // 2)
+    with torch.no_grad():
+        pr.world_tree.leaves[:, 2, 1] = 1e12  # absurd jerk cell -> too many children
+    with pytest.raises(ValueError, match="increase branch_max"):
+        pr.execute_levels()
