@@ -423,6 +423,7 @@ int pyl_cheb_ascend_complex(const pyl_prune_params* p, const double* leaves_dev,
 #define PYL_EP_REC_NHELD 7  /* circular: held children (before the crackle expansion) */
 #define PYL_EP_REC_NBATCH 8 /* circular: batches of parents */
 #define PYL_EP_REC_NEXPAND 9 /* circular: held children to expand (0 if the level stops) */
+#define PYL_EP_REC_NKEPT 10 /* survivors kept by the budget (= the next level's parents) */
 #define PYL_EP_REC_WORDS 12
 
 #define PYL_EP_ST_OK 0
@@ -439,10 +440,59 @@ typedef struct pyl_ep_timer pyl_ep_timer;
 #define PYL_EP_PH_CHILDREN 1    /* resolve + shift-add + score of every child */
 #define PYL_EP_PH_SELECT 2      /* survivor indices */
 #define PYL_EP_PH_MATERIALIZE 3 /* survivors' leaves, folds, backtracks */
+#define PYL_EP_PH_BUDGET 4      /* the survivor budget (pyl_ep_budget) */
 int pyl_ep_timer_create(int32_t capacity, pyl_ep_timer** out);
 int pyl_ep_timer_destroy(pyl_ep_timer* t);
 int pyl_ep_timer_read(pyl_ep_timer* t, int32_t* phase_out, float* ms_out, int32_t cap,
                       int32_t* n_out);
+
+/* Survivor budget of a level (no counterpart in the reference: a learned, best-first
+ * search, for callers that trade recall for speed). After a level's survivors are
+ * materialised, the `budget` of highest key are kept, in their order: the key is the S/N
+ * (PYL_BUDGET_SCORE) or the ranking network's logit (PYL_BUDGET_CNN, pyl_cnn_forward on
+ * the survivors' folds and pyl_cnn_features). Ties at the cut keep the lowest rows. The
+ * exhaustive search (no budget) stays the reference: every kept survivor is still scored
+ * exactly. Scratch arrays hold `cap` rows (the tree buffers' rows). */
+#define PYL_BUDGET_SCORE 1
+#define PYL_BUDGET_CNN 2
+#define PYL_BUDGET_SEL_WORDS 272
+#define PYL_BUDGET_HIST_BINS 1024
+#define PYL_BUDGET_STATS 1032 /* f64 words of `stats` */
+typedef struct pyl_ep_budget {
+  int32_t kind;          /* PYL_BUDGET_* */
+  int32_t nbins;         /* fold bins (the network's input) */
+  int32_t complex_folds; /* folds are complex64 [2][nbins/2 + 1] */
+  int32_t nlevels;       /* levels of the search (the feature level / nlevels) */
+  int32_t level0;        /* level loop: the search level of the call's first level */
+  int64_t budget;        /* survivors kept per level (>= 1) */
+  const float* params;   /* the network's parameters (PYL_BUDGET_CNN) */
+  float* keys;           /* f32 [cap]: ranking key of every survivor */
+  float* feats;          /* f32 [cap][8]: network features (PYL_BUDGET_CNN) */
+  int64_t* kept;         /* i64 [cap]: kept rows */
+  int32_t* bt_tmp;       /* i32 [cap][P + 2]: the level loop's backtracks before the cut */
+  uint32_t* sel;         /* u32 [PYL_BUDGET_SEL_WORDS]: radix-select state and histogram */
+  double* stats;         /* f64 [PYL_BUDGET_STATS]: per-level score statistics */
+  uint32_t* shist;       /* u32 [2 * PYL_BUDGET_HIST_BINS]: score histogram and its prefix */
+  int64_t* part;         /* i64 [PYL_EP_SCAN_BLOCKS] */
+} pyl_ep_budget;
+
+/* The budget on one tree of n rows (the per-level path): writes the kept rows to out_*
+ * (out_bt: int32 [budget][P + 2]) and their count to rec_dev[PYL_EP_REC_NKEPT]; rec_dev is
+ * an int64 [2][PYL_EP_REC_WORDS] workspace. `level` feeds the network's features. */
+int pyl_ep_budget_apply(const pyl_ep_budget* b, int32_t P, int32_t level,
+                        const double* leaves_dev, const float* folds_dev,
+                        const float* scores_dev, const int32_t* bt_dev, int64_t n,
+                        int64_t* rec_dev, double* out_leaves_dev, float* out_folds_dev,
+                        float* out_scores_dev, int32_t* out_bt_dev, void* stream);
+
+/* The ranking network's 8 features of n survivors of one level (f32 [n][8]): S/N / 10,
+ * level / nlevels, the S/N's z-score over the level, its rank fraction over the level
+ * (from a 1024-bin histogram between the level's smallest and largest S/N: rows below its
+ * bin plus half of its bin's other rows, over n - 1), then log10(1 + cell width) / 10 of
+ * the (up to) four lowest-order branched parameters. stats / shist as in pyl_ep_budget. */
+int pyl_cnn_features(const double* leaves_dev, const float* scores_dev, int64_t n, int32_t P,
+                     int32_t level, int32_t nlevels, double* stats_dev, uint32_t* shist_dev,
+                     float* feats_dev, void* stream);
 
 typedef struct pyl_ep_loop {
   int64_t cap_par;      /* rows of each tree buffer (>= max_sugg and the input tree) */
@@ -468,6 +518,10 @@ typedef struct pyl_ep_loop {
   int64_t* rec;         /* int64 [nlev + 1][PYL_EP_REC_WORDS]: level records (written) */
   float* minmax;        /* f32 [nlev][2]: children's score min/max per level (written) */
   pyl_ep_timer* timer;  /* NULL, or the phases are timed */
+  const pyl_ep_budget* budget; /* NULL, or every level keeps at most budget->budget
+                                  survivors: they are materialised into tree buffer 0 (and
+                                  budget->bt_tmp) and the kept rows gathered into tree
+                                  buffer 1, which the next level reads (first_out unused) */
 } pyl_ep_loop;
 
 int pyl_ep_levels(const pyl_prune_params* p, int32_t flags, const pyl_prune_coords* coords,

@@ -13,6 +13,7 @@ The network runs and trains only through the HIP library (``pyl_cnn_forward``,
 
 from __future__ import annotations
 
+import ctypes
 import time
 from dataclasses import dataclass
 
@@ -23,6 +24,10 @@ from pyloki_amd import _lib
 from pyloki_amd.device import ptr, stream_ptr
 
 NFEAT = 8
+_BUDGET_SCORE, _BUDGET_CNN = 1, 2
+_BUDGET_SEL_WORDS, _BUDGET_HIST_BINS, _BUDGET_STATS = 272, 1024, 1032
+_EP_SCAN_BLOCKS = 256
+_REC_WORDS, _REC_NKEPT = 12, 10
 _SHAPES = [("W1", (32, 1, 5)), ("b1", (32,)), ("W2", (32, 32, 5)), ("b2", (32,)),
            ("W3", (32, 32, 5)), ("b3", (32,)), ("Wf1", (32, 40)), ("bf1", (32,)),
            ("Wf2", (32,)), ("bf2", (1,))]
@@ -43,24 +48,23 @@ def init_params(seed: int = 0) -> np.ndarray:
 
 def leaf_features(leaves: torch.Tensor, scores: torch.Tensor, level: int,
                   nlevels: int) -> torch.Tensor:
-    """[n][8] features of one level's survivors: S/N / 10, level / nlevels, the S/N's
-    z-score and rank fraction within the level, then log10(1 + cell width) / 10 of the
-    four lowest-order branched parameters."""
-    n = leaves.shape[0]
-    P = leaves.shape[1] - 2
-    f = torch.zeros((n, NFEAT), dtype=torch.float32, device=leaves.device)
-    sc = scores.float()
-    f[:, 0] = sc / 10.0
-    f[:, 1] = float(level) / float(max(nlevels, 1))
-    if n > 1:
-        f[:, 2] = (sc - sc.mean()) / (sc.std() + 1e-6)
-        rank = torch.empty(n, dtype=torch.float32, device=leaves.device)
-        rank[torch.argsort(sc)] = torch.arange(n, dtype=torch.float32,
-                                               device=leaves.device) / (n - 1)
-        f[:, 3] = rank
-    k = min(P, NFEAT - 4)
-    f[:, 4:4 + k] = (torch.log10(1.0 + leaves[:, P - k:P, 1].abs()) / 10.0).float()
-    return f.contiguous()
+    """[n][8] features of one level's survivors (``pyl_cnn_features``): S/N / 10,
+    level / nlevels, the S/N's z-score and rank fraction within the level (the rank from a
+    1024-bin histogram of the level's S/N), then log10(1 + cell width) / 10 of the four
+    lowest-order branched parameters."""
+    lib = _lib.load()
+    n = int(leaves.shape[0])
+    P = int(leaves.shape[1]) - 2
+    dev = leaves.device
+    f = torch.zeros((n, NFEAT), dtype=torch.float32, device=dev)
+    if n == 0:
+        return f
+    stats = torch.empty(_BUDGET_STATS, dtype=torch.float64, device=dev)
+    shist = torch.empty(2 * _BUDGET_HIST_BINS, dtype=torch.int32, device=dev)
+    _lib.check(lib.pyl_cnn_features(ptr(leaves.contiguous()), ptr(scores.contiguous().float()),
+                                    n, P, int(level), int(nlevels), ptr(stats), ptr(shist),
+                                    ptr(f), stream_ptr()))
+    return f
 
 
 class BranchRanker:
@@ -210,34 +214,87 @@ def train(ranker: BranchRanker, folds: torch.Tensor, feats: torch.Tensor,
     return hist
 
 
-def _keep_top(tree, key: torch.Tensor, budget: int):
-    keep = torch.topk(key, budget, sorted=False).indices.sort().values
-    return type(tree)(tree.leaves[keep].contiguous(), tree.folds[keep].contiguous(),
-                      tree.scores[keep].contiguous(), tree.backtracks[keep].contiguous())
+class SurvivorBudget:
+    """Keep, at every level, the ``budget`` survivors of highest key: the ranking
+    network's logit when ``ranker`` is given, else the S/N (``pyl_ep_budget_apply``; ties
+    at the cut keep the lowest rows). Installed as ``Pruning.level_hook``: the sync-free
+    level loop applies it on the device (``device_budget``), the per-level path calls it."""
 
+    def __init__(self, budget: int, ranker: BranchRanker | None = None) -> None:
+        if budget < 1:
+            msg = "budget must be >= 1"
+            raise ValueError(msg)
+        self.budget = int(budget)
+        self.ranker = ranker
+        self._cap = -1
+        self._s = None
 
-def budget_hook(ranker: BranchRanker, budget: int):
-    """A ``Pruning`` level hook that keeps the ``budget`` survivors the network ranks
-    highest (in their original order)."""
+    def _buffers(self, pr, cap: int):
+        """The C struct with scratch for ``cap`` rows (kept alive here)."""
+        from pyloki_amd.prune import EpBudget
 
-    def hook(pr, tree):
-        if tree.size <= budget:
+        cfg = pr.dyp.cfg
+        P = cfg.prune_poly_order
+        if self._s is None or self._cap < cap or self._s.nbins != cfg.nbins:
+            dev = torch.device("cuda")
+            t = {"keys": torch.empty(cap, dtype=torch.float32, device=dev),
+                 "feats": torch.empty((cap, NFEAT), dtype=torch.float32, device=dev),
+                 "kept": torch.empty(cap, dtype=torch.int64, device=dev),
+                 "bt_tmp": torch.empty((cap, P + 2), dtype=torch.int32, device=dev),
+                 "sel": torch.empty(_BUDGET_SEL_WORDS, dtype=torch.int32, device=dev),
+                 "stats": torch.empty(_BUDGET_STATS, dtype=torch.float64, device=dev),
+                 "shist": torch.empty(2 * _BUDGET_HIST_BINS, dtype=torch.int32, device=dev),
+                 "part": torch.empty(_EP_SCAN_BLOCKS, dtype=torch.int64, device=dev)}
+            s = EpBudget()
+            s.kind = _BUDGET_CNN if self.ranker is not None else _BUDGET_SCORE
+            s.nbins = cfg.nbins
+            s.complex_folds = int(bool(cfg.use_fourier))
+            s.budget = self.budget
+            s.params = ptr(self.ranker.params) if self.ranker is not None else None
+            for name, ten in t.items():
+                setattr(s, name, ptr(ten))
+            self._t, self._s, self._cap = t, s, cap
+        self._s.nlevels = pr.dyp.nsegments - 1
+        return self._s
+
+    def device_budget(self, pr, cap: int, level0: int):
+        """The budget for the level loop, whose first level is ``level0``."""
+        s = self._buffers(pr, cap)
+        s.level0 = int(level0)
+        return s
+
+    def __call__(self, pr, tree):
+        from pyloki_amd.prune import DeviceTree
+
+        n = tree.size
+        if n == 0:
             return tree
-        nlev = pr.dyp.nsegments - 1
-        lg = ranker.logits(tree.folds, leaf_features(tree.leaves, tree.scores, pr.prune_level,
-                                                     nlev))
-        return _keep_top(tree, lg, budget)
+        lib = _lib.load()
+        s = self._buffers(pr, n)
+        P = int(tree.leaves.shape[1]) - 2
+        k = min(n, self.budget)
+        dev = tree.leaves.device
+        rec = torch.empty((2, _REC_WORDS), dtype=torch.int64, device=dev)
+        out = (torch.empty((k, P + 2, 2), dtype=torch.float64, device=dev),
+               torch.empty((k, *tree.folds.shape[1:]), dtype=tree.folds.dtype, device=dev),
+               torch.empty(k, dtype=torch.float32, device=dev),
+               torch.empty((k, P + 2), dtype=torch.int32, device=dev))
+        _lib.check(lib.pyl_ep_budget_apply(
+            ctypes.byref(s), P, int(pr.prune_level), ptr(tree.leaves), ptr(tree.folds),
+            ptr(tree.scores), ptr(tree.backtracks.contiguous()), n, ptr(rec), *map(ptr, out),
+            stream_ptr()))
+        kept = int(rec[0, _REC_NKEPT].item())
+        return DeviceTree(*(o[:kept] for o in out))
 
-    return hook
+
+def budget_hook(ranker: BranchRanker, budget: int) -> SurvivorBudget:
+    """Keep the ``budget`` survivors the network ranks highest (in their original order)."""
+    return SurvivorBudget(budget, ranker)
 
 
-def score_budget_hook(budget: int):
+def score_budget_hook(budget: int) -> SurvivorBudget:
     """The baseline: keep the ``budget`` survivors of highest S/N."""
-
-    def hook(pr, tree):
-        return tree if tree.size <= budget else _keep_top(tree, tree.scores, budget)
-
-    return hook
+    return SurvivorBudget(budget)
 
 
 def _final_keys(leaves: np.ndarray) -> set[bytes]:
@@ -249,10 +306,7 @@ def _timed_run(pr, ref_seg: int, hook) -> dict:
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     pr.initialize(ref_seg)
-    for _ in range(pr.dyp.nsegments - 1):
-        pr.execute_iter()
-        if pr.prune_complete:
-            break
+    pr.execute_levels()
     torch.cuda.synchronize()
     return {"time_s": time.perf_counter() - t0,
             "trials": int(sum(s.n_leaves for s in pr.pstats[1:])),

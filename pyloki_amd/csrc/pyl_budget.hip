@@ -1,0 +1,383 @@
+// Survivor budget of an EP level (no counterpart in the reference): keep the `budget`
+// survivors of highest key (S/N, or the ranking network's logit), in their order.
+//
+// The cut is an exact radix select on the keys' order-preserving 32-bit images: four
+// passes of an 8-bit digit histogram over the candidates that share the prefix found so
+// far, each followed by a one-block pick of the digit where the running count from the
+// top reaches the budget. The kept rows are then the keys above the cut value plus the
+// first rows equal to it (in row order) that fill the budget; one packed chunked scan
+// (count above in the low half, count equal in the high half) gives every kept row its
+// position. Counts come from device memory, so the level loop enqueues the cut without
+// the host.
+//
+// The network's features need the level's S/N mean, standard deviation and rank: the
+// statistics are reduced in a fixed order (per-chunk partials, then one thread over the
+// chunks), the rank comes from a 1024-bin histogram between the level's smallest and
+// largest S/N (integer counts), so the features do not depend on scheduling.
+#include <cfloat>
+
+#include "pyl_budget.hpp"
+#include "pyl_loop.hpp"
+
+namespace {
+
+using namespace pyl_ep;
+
+constexpr int kT = 256;
+constexpr int kBins = PYL_BUDGET_HIST_BINS;
+constexpr int kSelPrefix = 0, kSelMask = 1, kSelRemain = 2, kSelAll = 3, kSelHist = 16;
+constexpr int kFeat = 8;
+static_assert(kSelHist + 256 == PYL_BUDGET_SEL_WORDS, "select state layout");
+static_assert(4 * kScanBlocks + 8 == PYL_BUDGET_STATS, "statistics layout");
+
+__device__ __forceinline__ int64_t count_of(int64_t n, const int64_t* n_dev) {
+  return n_dev ? *n_dev : n;
+}
+
+// Order-preserving image of a float: larger key, larger unsigned value.
+__device__ __forceinline__ uint32_t key_bits(float x) {
+  const uint32_t b = __float_as_uint(x);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+// ---- Features --------------------------------------------------------------------------
+
+// Per chunk of the rows (kScanBlocks chunks): sum, sum of squares (FP64), min, max.
+__global__ void __launch_bounds__(kT)
+feat_stats_part_kernel(const float* __restrict__ scores, int64_t n_arg,
+                       const int64_t* __restrict__ n_dev, double* __restrict__ stats) {
+  __shared__ double s_sum[kT], s_sq[kT];
+  __shared__ float s_mn[kT], s_mx[kT];
+  const int64_t n = count_of(n_arg, n_dev);
+  int64_t b0, b1;
+  chunk_bounds(n, &b0, &b1);
+  double su = 0.0, sq = 0.0;
+  float mn = FLT_MAX, mx = -FLT_MAX;
+  for (int64_t i = b0 + threadIdx.x; i < b1; i += kT) {
+    const float x = scores[i];
+    su += (double)x;
+    sq += (double)x * (double)x;
+    mn = fminf(mn, x);
+    mx = fmaxf(mx, x);
+  }
+  const int t = threadIdx.x;
+  s_sum[t] = su;
+  s_sq[t] = sq;
+  s_mn[t] = mn;
+  s_mx[t] = mx;
+  __syncthreads();
+  for (int o = kT / 2; o > 0; o >>= 1) {
+    if (t < o) {
+      s_sum[t] += s_sum[t + o];
+      s_sq[t] += s_sq[t + o];
+      s_mn[t] = fminf(s_mn[t], s_mn[t + o]);
+      s_mx[t] = fmaxf(s_mx[t], s_mx[t + o]);
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    double* o = stats + 4 * (int64_t)blockIdx.x;
+    o[0] = s_sum[0];
+    o[1] = s_sq[0];
+    o[2] = (double)s_mn[0];
+    o[3] = (double)s_mx[0];
+  }
+}
+
+// One block: the chunks in order -> mean, unbiased standard deviation, min, max; clears
+// the score histogram.
+__global__ void __launch_bounds__(kT)
+feat_stats_final_kernel(double* __restrict__ stats, int64_t n_arg,
+                        const int64_t* __restrict__ n_dev, uint32_t* __restrict__ shist) {
+  const int64_t n = count_of(n_arg, n_dev);
+  for (int j = threadIdx.x; j < 2 * kBins; j += kT) shist[j] = 0;
+  if (threadIdx.x != 0) return;
+  double su = 0.0, sq = 0.0, mn = DBL_MAX, mx = -DBL_MAX;
+  for (int b = 0; b < kScanBlocks; ++b) {
+    const double* o = stats + 4 * b;
+    su += o[0];
+    sq += o[1];
+    mn = fmin(mn, o[2]);
+    mx = fmax(mx, o[3]);
+  }
+  const double mean = n > 0 ? su / (double)n : 0.0;
+  const double var = n > 1 ? (sq - su * mean) / (double)(n - 1) : 0.0;
+  double* f = stats + 4 * kScanBlocks;
+  f[0] = mean;
+  f[1] = sqrt(fmax(var, 0.0));
+  f[2] = mn;
+  f[3] = mx;
+}
+
+__device__ __forceinline__ int score_bin(float x, const double* f) {
+  const double lo = f[2], hi = f[3];
+  if (!(hi > lo)) return 0;
+  int b = (int)(((double)x - lo) / (hi - lo) * (double)kBins);
+  return b < 0 ? 0 : (b >= kBins ? kBins - 1 : b);
+}
+
+__global__ void __launch_bounds__(kT)
+feat_hist_kernel(const float* __restrict__ scores, int64_t n_arg,
+                 const int64_t* __restrict__ n_dev, const double* __restrict__ stats,
+                 uint32_t* __restrict__ shist) {
+  const int64_t n = count_of(n_arg, n_dev);
+  const double* f = stats + 4 * kScanBlocks;
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT)
+    atomicAdd(&shist[score_bin(scores[i], f)], 1u);
+}
+
+// One block: exclusive prefix of the histogram into shist[kBins ..].
+__global__ void __launch_bounds__(kT) feat_cdf_kernel(uint32_t* __restrict__ shist) {
+  __shared__ uint32_t s_part[kT];
+  constexpr int kPer = kBins / kT;
+  const int t = threadIdx.x;
+  uint32_t acc = 0;
+  for (int j = 0; j < kPer; ++j) acc += shist[t * kPer + j];
+  s_part[t] = acc;
+  __syncthreads();
+  uint32_t pre = 0;
+  for (int j = 0; j < t; ++j) pre += s_part[j];
+  for (int j = 0; j < kPer; ++j) {
+    shist[kBins + t * kPer + j] = pre;
+    pre += shist[t * kPer + j];
+  }
+}
+
+__global__ void __launch_bounds__(kT)
+feat_write_kernel(const double* __restrict__ leaves, const float* __restrict__ scores,
+                  int64_t n_arg, const int64_t* __restrict__ n_dev, int P, int level,
+                  int nlevels, const double* __restrict__ stats,
+                  const uint32_t* __restrict__ shist, float* __restrict__ feats) {
+  const int64_t n = count_of(n_arg, n_dev);
+  const double* f = stats + 4 * kScanBlocks;
+  const int kw = P < kFeat - 4 ? P : kFeat - 4;
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kT) {
+    const float x = scores[i];
+    float* o = feats + i * kFeat;
+    o[0] = x / 10.0f;
+    o[1] = (float)level / (float)(nlevels > 1 ? nlevels : 1);
+    if (n > 1) {
+      o[2] = (float)(((double)x - f[0]) / (f[1] + 1e-6));
+      const int b = score_bin(x, f);
+      o[3] = (float)(((double)shist[kBins + b] + 0.5 * ((double)shist[b] - 1.0)) /
+                     (double)(n - 1));
+    } else {
+      o[2] = 0.0f;
+      o[3] = 0.0f;
+    }
+    const double* lf = leaves + i * (P + 2) * 2;
+    for (int j = 0; j < kFeat - 4; ++j)
+      o[4 + j] = j < kw ? (float)(log10(1.0 + fabs(lf[2 * (P - kw + j) + 1])) / 10.0) : 0.0f;
+  }
+}
+
+inline unsigned stride_grid(int64_t cap) {
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>((cap + kT - 1) / kT, 2048));
+}
+
+int features_enqueue(const double* leaves, const float* scores, int64_t n_cap,
+                     const int64_t* n_dev, int P, int level, int nlevels, double* stats,
+                     uint32_t* shist, float* feats, hipStream_t s) {
+  feat_stats_part_kernel<<<kScanBlocks, kT, 0, s>>>(scores, n_cap, n_dev, stats);
+  feat_stats_final_kernel<<<1, kT, 0, s>>>(stats, n_cap, n_dev, shist);
+  feat_hist_kernel<<<stride_grid(n_cap), kT, 0, s>>>(scores, n_cap, n_dev, stats, shist);
+  feat_cdf_kernel<<<1, kT, 0, s>>>(shist);
+  feat_write_kernel<<<stride_grid(n_cap), kT, 0, s>>>(leaves, scores, n_cap, n_dev, P, level,
+                                                      nlevels, stats, shist, feats);
+  PYL_CHECK_LAUNCH("feat_write_kernel");
+  return PYL_OK;
+}
+
+// ---- The cut -------------------------------------------------------------------------
+
+__global__ void sel_init_kernel(const int64_t* __restrict__ m_dev, int64_t budget,
+                                uint32_t* __restrict__ sel) {
+  const int64_t m = *m_dev;
+  if (threadIdx.x == 0) {
+    sel[kSelPrefix] = 0;
+    sel[kSelMask] = 0;
+    sel[kSelRemain] = (uint32_t)std::min<int64_t>(budget, m);
+    sel[kSelAll] = m <= budget ? 1u : 0u;
+  }
+  sel[kSelHist + threadIdx.x] = 0;
+}
+
+__global__ void __launch_bounds__(kT)
+sel_hist_kernel(const float* __restrict__ keys, const int64_t* __restrict__ m_dev,
+                uint32_t* __restrict__ sel, int shift) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  if (sel[kSelAll] == 0) {
+    const int64_t m = *m_dev;
+    const uint32_t prefix = sel[kSelPrefix], mask = sel[kSelMask];
+    for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < m;
+         i += (int64_t)gridDim.x * kT) {
+      const uint32_t u = key_bits(keys[i]);
+      if ((u & mask) == prefix) atomicAdd(&h[(u >> shift) & 255u], 1u);
+    }
+  }
+  __syncthreads();
+  if (h[threadIdx.x]) atomicAdd(&sel[kSelHist + threadIdx.x], h[threadIdx.x]);
+}
+
+// One block: the digit where the count of candidates from the top reaches the remaining
+// budget; clears the histogram for the next pass.
+__global__ void __launch_bounds__(kT) sel_pick_kernel(uint32_t* __restrict__ sel, int shift) {
+  __shared__ uint32_t suf[257];
+  const int t = threadIdx.x;
+  if (sel[kSelAll] != 0) return;  // the whole block
+  if (t == 0) {
+    uint32_t acc = 0;
+    suf[256] = 0;
+    for (int d = 255; d >= 0; --d) {
+      acc += sel[kSelHist + d];
+      suf[d] = acc;
+    }
+  }
+  __syncthreads();
+  const uint32_t rem = sel[kSelRemain];
+  __syncthreads();  // every thread has read the state before it changes
+  if (suf[t + 1] < rem && rem <= suf[t]) {
+    sel[kSelPrefix] |= (uint32_t)t << shift;
+    sel[kSelMask] |= 0xffu << shift;
+    sel[kSelRemain] = rem - suf[t + 1];
+  }
+  sel[kSelHist + t] = 0;
+}
+
+struct KeepSrc {  // packed: 1 = above the cut, 1 << 32 = at the cut
+  const float* keys;
+  const uint32_t* sel;
+  __device__ int64_t operator()(int64_t i) const {
+    if (sel[kSelAll]) return 1;
+    const uint32_t u = key_bits(keys[i]);
+    const uint32_t t = sel[kSelPrefix];
+    return u > t ? 1 : (u == t ? ((int64_t)1 << 32) : 0);
+  }
+};
+
+struct KeepSink {
+  int64_t* kept;
+  const uint32_t* sel;
+  int64_t* rec;
+  __device__ void operator()(int64_t i, int64_t excl, int64_t x) const {
+    if (!x) return;
+    const int64_t above = excl & 0xffffffffLL, at = excl >> 32;
+    if (sel[kSelAll]) {
+      kept[above] = i;
+      return;
+    }
+    const int64_t need = sel[kSelRemain];
+    if (x & 0xffffffffLL) kept[above + (at < need ? at : need)] = i;
+    else if (at < need) kept[above + at] = i;
+  }
+  __device__ void finish(int64_t tot) const {
+    const int64_t m = rec[kRecNsurv];
+    const int64_t need = sel[kSelRemain];
+    const int64_t k = sel[kSelAll] ? m : (tot & 0xffffffffLL) + std::min<int64_t>(tot >> 32, need);
+    rec[PYL_EP_REC_NKEPT] = k;
+    rec[kRecWords + kRecNpar] = k;
+    rec[kRecWords + kRecHist] = rec[kRecHist] + k;
+  }
+};
+
+// One wavefront per kept row: leaves, fold row, score, and the backtrack row into the
+// history at the level's rows.
+__global__ void __launch_bounds__(kT)
+budget_gather_kernel(const int64_t* __restrict__ kept, const int64_t* __restrict__ rec,
+                     int P, int row_floats, const double* __restrict__ leaves,
+                     const float* __restrict__ folds, const float* __restrict__ scores,
+                     const int32_t* __restrict__ bt, double* __restrict__ o_leaves,
+                     float* __restrict__ o_folds, float* __restrict__ o_scores,
+                     int32_t* __restrict__ o_hist) {
+  const int64_t k = rec[kRecWords + kRecNpar];
+  const int lane = threadIdx.x & 63;
+  const int64_t r = (int64_t)blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
+  if (r >= k) return;
+  const int64_t i = kept[r];
+  const int nl = (P + 2) * 2;
+  const int nb = P + 2;
+  if (lane < nl) o_leaves[r * nl + lane] = leaves[i * nl + lane];
+  for (int j = lane; j < row_floats; j += 64)
+    o_folds[r * row_floats + j] = folds[i * row_floats + j];
+  if (lane == 0) o_scores[r] = scores[i];
+  if (lane < nb) o_hist[(rec[kRecHist] + r) * nb + lane] = bt[i * nb + lane];
+}
+
+__global__ void budget_record_kernel(int64_t* __restrict__ rec, int64_t n) {
+  const int w = threadIdx.x;
+  if (w < 2 * kRecWords) rec[w] = 0;
+  if (w == kRecNsurv) rec[w] = n;
+}
+
+}  // namespace
+
+int pyl::ep_budget_enqueue(const pyl_ep_budget* b, int P, int level, const double* leaves,
+                           const float* folds, const float* scores, const int32_t* bt,
+                           int64_t n_cap, int64_t* rec, double* o_leaves, float* o_folds,
+                           float* o_scores, int32_t* o_hist, hipStream_t s) {
+  PYL_REQUIRE(b != nullptr, "null budget");
+  PYL_REQUIRE(b->kind == PYL_BUDGET_SCORE || b->kind == PYL_BUDGET_CNN, "unknown budget kind %d",
+              b->kind);
+  PYL_REQUIRE(b->budget >= 1, "budget must be >= 1");
+  PYL_REQUIRE(n_cap >= 0 && n_cap < 0x7fffffffLL, "too many survivors for the budget");
+  PYL_REQUIRE(b->kept && b->sel && b->part && (b->kind != PYL_BUDGET_CNN ||
+                                               (b->params && b->keys && b->feats &&
+                                                b->stats && b->shist)),
+              "null budget buffer");
+  const int64_t* m_dev = rec + kRecNsurv;
+  const float* keys = scores;
+  if (b->kind == PYL_BUDGET_CNN) {
+    int rc = features_enqueue(leaves, scores, n_cap, m_dev, P, level, b->nlevels, b->stats,
+                              b->shist, b->feats, s);
+    if (rc != PYL_OK) return rc;
+    rc = pyl::cnn_forward_count(b->params, folds, n_cap, m_dev, b->nbins, b->complex_folds,
+                                b->feats, b->keys, s);
+    if (rc != PYL_OK) return rc;
+    keys = b->keys;
+  }
+  sel_init_kernel<<<1, 256, 0, s>>>(m_dev, b->budget, b->sel);
+  for (int shift = 24; shift >= 0; shift -= 8) {
+    sel_hist_kernel<<<stride_grid(n_cap), kT, 0, s>>>(keys, m_dev, b->sel, shift);
+    sel_pick_kernel<<<1, kT, 0, s>>>(b->sel, shift);
+  }
+  chunked_scan(KeepSrc{keys, b->sel}, KeepSink{b->kept, b->sel, rec}, m_dev, b->part, s);
+  const int row_floats = b->complex_folds ? 4 * (b->nbins / 2 + 1) : 2 * b->nbins;
+  const int64_t rows = std::min<int64_t>(n_cap, b->budget);
+  budget_gather_kernel<<<(unsigned)std::max<int64_t>(1, pyl::ceil_div(rows, kT / 64)), kT, 0,
+                         s>>>(b->kept, rec, P, row_floats, leaves, folds, scores, bt,
+                              o_leaves, o_folds, o_scores, o_hist);
+  PYL_CHECK_LAUNCH("budget_gather_kernel");
+  return PYL_OK;
+}
+
+extern "C" int pyl_ep_budget_apply(const pyl_ep_budget* b, int32_t P, int32_t level,
+                                   const double* leaves, const float* folds,
+                                   const float* scores, const int32_t* bt, int64_t n,
+                                   int64_t* rec, double* out_leaves, float* out_folds,
+                                   float* out_scores, int32_t* out_bt, void* stream) {
+  PYL_REQUIRE(b != nullptr, "null budget");
+  PYL_REQUIRE(P >= 2 && P <= 5, "P must be in [2, 5]");
+  PYL_REQUIRE(n >= 0, "negative row count");
+  PYL_REQUIRE(leaves && folds && scores && bt && rec && out_leaves && out_folds && out_scores &&
+                  out_bt,
+              "null device pointer");
+  const hipStream_t s = pyl::as_stream(stream);
+  budget_record_kernel<<<1, 64, 0, s>>>(rec, n);
+  PYL_CHECK_LAUNCH("budget_record_kernel");
+  return pyl::ep_budget_enqueue(b, P, level, leaves, folds, scores, bt, n, rec, out_leaves,
+                                out_folds, out_scores, out_bt, s);
+}
+
+extern "C" int pyl_cnn_features(const double* leaves, const float* scores, int64_t n, int32_t P,
+                                int32_t level, int32_t nlevels, double* stats, uint32_t* shist,
+                                float* feats, void* stream) {
+  PYL_REQUIRE(P >= 2 && P <= 5, "P must be in [2, 5]");
+  PYL_REQUIRE(n >= 0, "negative row count");
+  if (n == 0) return PYL_OK;
+  PYL_REQUIRE(leaves && scores && stats && shist && feats, "null device pointer");
+  return features_enqueue(leaves, scores, n, nullptr, P, level, nlevels, stats, shist, feats,
+                          pyl::as_stream(stream));
+}

@@ -1,0 +1,23 @@
+// Internal (not C-ABI) entry points shared by the EP level loops, the survivor budget and
+// the ranking network's translation units.
+#pragma once
+
+#include "pyl_common.hpp"
+
+namespace pyl {
+
+// The network's forward pass over at most n_cap samples, the count read from n_dev.
+int cnn_forward_count(const float* params, const float* folds, int64_t n_cap,
+                      const int64_t* n_dev, int nbins, int complex_folds, const float* feats,
+                      float* logits, hipStream_t s);
+
+// The survivor budget of one level. The level's record `rec` (PYL_EP_REC_WORDS words; the
+// next record follows) gives the survivors (NSURV) and the first history row (HIST); the
+// kept rows of leaves/folds/scores go to o_*, their backtracks to rows HIST.. of o_hist,
+// and NKEPT, the next record's NPAR and HIST are written. n_cap bounds the survivors.
+int ep_budget_enqueue(const pyl_ep_budget* b, int P, int level, const double* leaves,
+                      const float* folds, const float* scores, const int32_t* bt,
+                      int64_t n_cap, int64_t* rec, double* o_leaves, float* o_folds,
+                      float* o_scores, int32_t* o_hist, hipStream_t s);
+
+}  // namespace pyl

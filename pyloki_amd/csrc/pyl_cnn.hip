@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <cmath>
 
+#include "pyl_budget.hpp"
 #include "pyl_common.hpp"
 #include "pyl_score.hpp"
 
@@ -73,10 +74,12 @@ __device__ __forceinline__ void conv_layer(const float* __restrict__ W,
 
 template <int B, bool kComplex, bool kTrain>
 __global__ void __launch_bounds__(kThreads)
-cnn_forward_kernel(const float* __restrict__ th, const float* __restrict__ folds, int64_t n,
-                   const float* __restrict__ feats, float* __restrict__ logits,
-                   float* __restrict__ cache) {
+cnn_forward_kernel(const float* __restrict__ th, const float* __restrict__ folds, int64_t n_arg,
+                   const int64_t* __restrict__ n_dev, const float* __restrict__ feats,
+                   float* __restrict__ logits, float* __restrict__ cache) {
   constexpr int S = kThreads / B;
+  const int64_t n = n_dev ? *n_dev : n_arg;  // device count: the EP level loop
+  if ((int64_t)blockIdx.x * S >= n) return;  // the whole block
   __shared__ float A[S][kC][B];
   __shared__ float X[S][B];
   __shared__ float stat[S][2];
@@ -395,17 +398,19 @@ __global__ void cnn_adam_kernel(float* __restrict__ th, const float* __restrict_
   th[o] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
+// n samples, or at most n with the count read from n_dev.
 template <int B>
-int launch_forward(const float* th, const float* folds, int64_t n, const float* feats,
-                   float* logits, float* cache, int complex_folds, hipStream_t s) {
+int launch_forward(const float* th, const float* folds, int64_t n, const int64_t* n_dev,
+                   const float* feats, float* logits, float* cache, int complex_folds,
+                   hipStream_t s) {
   constexpr int S = kThreads / B;
   const unsigned grid = (unsigned)pyl::ceil_div(n, S);
   if (complex_folds) {
-    if (cache) cnn_forward_kernel<B, true, true><<<grid, kThreads, 0, s>>>(th, folds, n, feats, logits, cache);
-    else cnn_forward_kernel<B, true, false><<<grid, kThreads, 0, s>>>(th, folds, n, feats, logits, cache);
+    if (cache) cnn_forward_kernel<B, true, true><<<grid, kThreads, 0, s>>>(th, folds, n, n_dev, feats, logits, cache);
+    else cnn_forward_kernel<B, true, false><<<grid, kThreads, 0, s>>>(th, folds, n, n_dev, feats, logits, cache);
   } else {
-    if (cache) cnn_forward_kernel<B, false, true><<<grid, kThreads, 0, s>>>(th, folds, n, feats, logits, cache);
-    else cnn_forward_kernel<B, false, false><<<grid, kThreads, 0, s>>>(th, folds, n, feats, logits, cache);
+    if (cache) cnn_forward_kernel<B, false, true><<<grid, kThreads, 0, s>>>(th, folds, n, n_dev, feats, logits, cache);
+    else cnn_forward_kernel<B, false, false><<<grid, kThreads, 0, s>>>(th, folds, n, n_dev, feats, logits, cache);
   }
   PYL_CHECK_LAUNCH("cnn_forward_kernel");
   return PYL_OK;
@@ -430,9 +435,22 @@ extern "C" int pyl_cnn_forward(const float* params, const float* folds, int64_t 
   PYL_REQUIRE(params && folds && feats && logits, "null device pointer");
   hipStream_t s = pyl::as_stream(stream);
   switch (nbins) {
-    case 32: return launch_forward<32>(params, folds, n, feats, logits, cache, complex_folds, s);
-    case 64: return launch_forward<64>(params, folds, n, feats, logits, cache, complex_folds, s);
-    default: return launch_forward<128>(params, folds, n, feats, logits, cache, complex_folds, s);
+    case 32: return launch_forward<32>(params, folds, n, nullptr, feats, logits, cache, complex_folds, s);
+    case 64: return launch_forward<64>(params, folds, n, nullptr, feats, logits, cache, complex_folds, s);
+    default: return launch_forward<128>(params, folds, n, nullptr, feats, logits, cache, complex_folds, s);
+  }
+}
+
+// The forward pass over at most n_cap samples, the count read from n_dev (no cache).
+int pyl::cnn_forward_count(const float* params, const float* folds, int64_t n_cap,
+                           const int64_t* n_dev, int nbins, int complex_folds,
+                           const float* feats, float* logits, hipStream_t s) {
+  PYL_REQUIRE(valid_bins(nbins), "the ranking network takes 32, 64 or 128 bins (got %d)", nbins);
+  if (n_cap <= 0) return PYL_OK;
+  switch (nbins) {
+    case 32: return launch_forward<32>(params, folds, n_cap, n_dev, feats, logits, nullptr, complex_folds, s);
+    case 64: return launch_forward<64>(params, folds, n_cap, n_dev, feats, logits, nullptr, complex_folds, s);
+    default: return launch_forward<128>(params, folds, n_cap, n_dev, feats, logits, nullptr, complex_folds, s);
   }
 }
 

@@ -22,6 +22,7 @@
 // and the GPU only does the per-leaf FP64 matrix-vector products, in ascending column order.
 #include <vector>
 
+#include "pyl_budget.hpp"
 #include "pyl_loop.hpp"
 #include "pyl_prune_common.hpp"
 
@@ -745,8 +746,11 @@ int ep_levels(const pyl_prune_params* p, const pyl_prune_coords* coords, const d
                   L->c_shift && L->c_phase && L->c_score && L->surv && L->part && L->rec &&
                   L->minmax && (!kCheby || L->leaves_cur),
               "null device pointer");
-  PYL_REQUIRE(in_leaves != L->leaves[first_out] && in_folds != L->folds[first_out],
+  const pyl_ep_budget* bud = L->budget;
+  const int out0 = bud ? 0 : first_out;  // with a budget: materialise into 0, keep into 1
+  PYL_REQUIRE(in_leaves != L->leaves[out0] && in_folds != L->folds[out0],
               "the input tree is the first output buffer");
+  PYL_REQUIRE(!bud || bud->bt_tmp, "null budget buffer");
   PYL_REQUIRE((int64_t)p->n_accel * p->n_freq <= 0x7fffffff, "FFA grid too large");
   ep_records_init_kernel<<<(unsigned)pyl::ceil_div(nlev + 1, 64), 64, 0, s>>>(
       L->rec, L->minmax, nlev, n_in, hist_row0);
@@ -769,9 +773,10 @@ int ep_levels(const pyl_prune_params* p, const pyl_prune_coords* coords, const d
                   "complex folds need a power-of-two nbins (%d)", k.nbins);
     PYL_REQUIRE(seg_idx[l] >= 0, "negative segment index");
     int64_t* rec = L->rec + (int64_t)l * kRecWords;
-    const double* leaves = l == 0 ? in_leaves : L->leaves[(first_out + l - 1) & 1];
-    const float* folds = l == 0 ? in_folds : L->folds[(first_out + l - 1) & 1];
-    const int o = (first_out + l) & 1;
+    const int in_buf = bud ? 1 : (first_out + l - 1) & 1;
+    const double* leaves = l == 0 ? in_leaves : L->leaves[in_buf];
+    const float* folds = l == 0 ? in_folds : L->folds[in_buf];
+    const int o = bud ? 0 : (first_out + l) & 1;
     const float* seg = fold + seg_idx[l] * seg_stride;
     const double* src = leaves;
     pyl_ep_timer* tm = L->timer;
@@ -806,9 +811,17 @@ int ep_levels(const pyl_prune_params* p, const pyl_prune_coords* coords, const d
     prune_materialize_kernel<kComplex><<<g_surv, kBlk, 0, s>>>(
         k, src, folds, L->branch, L->counts, L->off, seg, L->c_parent, L->c_row, L->c_shift,
         L->c_phase, L->c_score, L->surv, 0, rec + kRecWords + kRecNpar, L->leaves[o],
-        L->folds[o], L->scores[o], L->hist, rec + kRecHist);
+        L->folds[o], L->scores[o], bud ? bud->bt_tmp : L->hist, bud ? nullptr : rec + kRecHist);
     PYL_CHECK_LAUNCH("prune_materialize_kernel");
     if (tm) tm->end(s);
+    if (bud) {
+      if (tm) tm->begin(PYL_EP_PH_BUDGET, s);
+      rc = pyl::ep_budget_enqueue(bud, k.P, bud->level0 + l, L->leaves[0], L->folds[0],
+                                  L->scores[0], bud->bt_tmp, L->max_sugg, rec, L->leaves[1],
+                                  L->folds[1], L->scores[1], L->hist, s);
+      if (rc != PYL_OK) return rc;
+      if (tm) tm->end(s);
+    }
   }
   return PYL_OK;
 }

@@ -97,8 +97,19 @@ class EpLoop(ctypes.Structure):
                 ("leaves", ctypes.c_void_p * 2), ("folds", ctypes.c_void_p * 2),
                 ("scores", ctypes.c_void_p * 2)] + [(name, ctypes.c_void_p) for name in (
         "hist", "leaves_cur", "branch", "counts", "nchild", "off", "c_parent", "c_row",
-        "c_shift", "c_phase", "c_score", "surv", "part", "rec", "minmax", "timer")]
-EP_PHASES = ("branch", "children", "select", "materialize")
+        "c_shift", "c_phase", "c_score", "surv", "part", "rec", "minmax", "timer", "budget")]
+EP_PHASES = ("branch", "children", "select", "materialize", "budget")
+REC_NKEPT = 10
+BUDGET_SCORE, BUDGET_CNN = 1, 2
+BUDGET_SEL_WORDS, BUDGET_HIST_BINS, BUDGET_STATS = 272, 1024, 1032
+
+
+class EpBudget(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("nbins", ctypes.c_int32),
+                ("complex_folds", ctypes.c_int32), ("nlevels", ctypes.c_int32),
+                ("level0", ctypes.c_int32), ("budget", ctypes.c_int64)] + [
+        (name, ctypes.c_void_p) for name in (
+            "params", "keys", "feats", "kept", "bt_tmp", "sel", "stats", "shist", "part")]
 
 
 class CircLoop(ctypes.Structure):
@@ -618,7 +629,8 @@ class Pruning:
         self._backtrack_hist.append(backtracks)
 
     def _use_device_loop(self) -> bool:
-        return self.device_loop and self.level_hook is None
+        hook = self.level_hook
+        return self.device_loop and (hook is None or hasattr(hook, "device_budget"))
 
     def _loop_buffers(self, n_in: int, nlev: int) -> LoopBuffers:
         """The loop's buffers, (re)allocated when too small. Defaults: tree buffers of
@@ -667,6 +679,13 @@ class Pruning:
             coords = (PruneCoords * nlev)(*[self._coords(lv) for lv in levels])
             thr = np.ascontiguousarray(self.threshold_scheme[l0 - 1:last], dtype=np.float64)
             segs = np.array([self.scheme.get_segment_idx(lv) for lv in levels], dtype=np.int64)
+            budget = None
+            if self.level_hook is not None:  # a survivor budget the loop applies itself
+                if buf.holds(tree) == 0:
+                    tree = DeviceTree(tree.leaves.clone(), tree.folds.clone(),
+                                      tree.scores.clone(), tree.backtracks)
+                budget = self.level_hook.device_budget(self, buf.cap_par, l0)
+            buf.s.budget = ctypes.addressof(budget) if budget is not None else None
             first_out = 1 if buf.holds(tree) == 0 else 0
             self._loop_timer(lib, buf, nlev)
             args = (coords, thr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
@@ -692,8 +711,10 @@ class Pruning:
                 stop = int(r[REC_STATUS])
                 if stop != ST_OK:
                     break
-                o = (first_out + i) & 1
+                o = 1 if budget is not None else (first_out + i) & 1
                 m, h, nv = int(r[REC_NSURV]), int(r[REC_HIST]), int(r[REC_NVALID])
+                if budget is not None:
+                    m = int(r[REC_NKEPT])
                 sc = buf.scores[o][:m]
                 new_tree = DeviceTree(buf.leaves[o][:m], buf.folds[o][:m], sc,
                                       buf.hist[h:h + m], scores_ep=sc)

@@ -140,3 +140,100 @@ def test_circ_loop_branch_error(gpu_device):
         pr.world_tree.leaves[:, 2, 1] = 1e12  # absurd jerk cell -> too many children
     with pytest.raises(ValueError, match="increase branch_max"):
         pr.execute_levels()
+
+
+# ---- survivor budget -------------------------------------------------------------------
+
+def _tree(n: int, P: int, B: int, seed: int, ties: bool):
+    from pyloki_amd.prune import DeviceTree
+
+    rng = np.random.default_rng(seed)
+    sc = rng.integers(0, 7, n).astype(np.float32) if ties else rng.normal(size=n).astype(np.float32)
+    lv = rng.normal(size=(n, P + 2, 2))
+    fo = rng.normal(size=(n, 2, B)).astype(np.float32)
+    bt = rng.integers(0, 1000, (n, P + 2)).astype(np.int32)
+    t = DeviceTree(*(torch.from_numpy(a).cuda() for a in (lv, fo, sc, bt)))
+    return t, sc
+
+
+class _Pr:  # what a budget reads from its Pruning
+    def __init__(self, P, B, nlev=10, level=3):
+        class _C:
+            prune_poly_order = P
+            nbins = B
+            use_fourier = False
+
+        class _D:
+            cfg = _C()
+            nsegments = nlev + 1
+
+        self.dyp = _D()
+        self.prune_level = level
+
+
+@pytest.mark.parametrize("n,budget,ties", [(5000, 700, True), (5000, 700, False),
+                                           (300, 700, False), (1, 1, False), (70000, 4096, True)])
+def test_score_budget_cut(gpu_device, n, budget, ties):
+    from pyloki_amd.cnn import SurvivorBudget
+
+    _case(f"budget cut n={n} budget={budget} ties={ties}")
+    P, B = 3, 32
+    tree, sc = _tree(n, P, B, n + budget, ties)
+    got = SurvivorBudget(budget)(_Pr(P, B), tree)
+    order = np.lexsort((np.arange(n), -sc.astype(np.float64)))  # key desc, then row asc
+    keep = np.sort(order[:min(n, budget)])
+    np.testing.assert_array_equal(got.scores.cpu().numpy(), sc[keep])
+    np.testing.assert_array_equal(got.leaves.cpu().numpy(), tree.leaves.cpu().numpy()[keep])
+    np.testing.assert_array_equal(got.folds.cpu().numpy(), tree.folds.cpu().numpy()[keep])
+    np.testing.assert_array_equal(got.backtracks.cpu().numpy(),
+                                  tree.backtracks.cpu().numpy()[keep])
+
+
+def test_cnn_features_definition(gpu_device):
+    from pyloki_amd.cnn import leaf_features
+
+    _case("cnn features")
+    P, B, n = 5, 32, 3001
+    tree, sc = _tree(n, P, B, 5, False)
+    f = leaf_features(tree.leaves, tree.scores, 7, 20).cpu().numpy()
+    x = sc.astype(np.float64)
+    lo, hi = x.min(), x.max()
+    b = np.clip(((x - lo) / (hi - lo) * 1024).astype(np.int64), 0, 1023)
+    cnt = np.bincount(b, minlength=1024)
+    below = np.concatenate([[0], np.cumsum(cnt)[:-1]])
+    rank = (below[b] + 0.5 * (cnt[b] - 1)) / (n - 1)
+    z = (x - x.mean()) / (x.std(ddof=1) + 1e-6)
+    w = np.log10(1 + np.abs(tree.leaves.cpu().numpy()[:, 1:5, 1])) / 10
+    np.testing.assert_allclose(f[:, 0], sc / 10, rtol=1e-6)
+    np.testing.assert_allclose(f[:, 1], 7 / 20, rtol=1e-6)
+    np.testing.assert_allclose(f[:, 2], z, rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(f[:, 3], rank, rtol=1e-6, atol=1e-7)
+    np.testing.assert_allclose(f[:, 4:8], w, rtol=1e-6, atol=1e-7)
+
+
+@pytest.mark.parametrize("kind,circular,budget", [("snr", False, 64), ("cnn", False, 64),
+                                                  ("snr", True, 512), ("cnn", True, 512)])
+def test_loop_budget_equals_per_level_path(gpu_device, kind, circular, budget):
+    from pyloki_amd.cnn import BranchRanker, SurvivorBudget
+
+    _case(f"ep loop budget kind={kind} circular={circular} budget={budget}")
+    if circular:
+        cfg, dyp, pcfg = _circ_setup(use_fourier=True)
+        ref = dyp.nsegments // 2
+        thresholds = np.linspace(2.0, 8.0, dyp.nsegments - 1)
+    else:
+        cfg, dyp, pcfg = _setup(2, 2**18, 16)
+        ref = 7
+        thresholds = np.linspace(2.0, 5.0, dyp.nsegments - 1)
+    ranker = BranchRanker(cfg.nbins, seed=3) if kind == "cnn" else None
+    runs = []
+    for dl in (False, True):
+        pr = Pruning(dyp, thresholds, max_sugg=2**16, batch_size=256, device_loop=dl)
+        pr.level_hook = SurvivorBudget(budget, ranker)
+        runs.append((pr, pr.execute(ref)))
+    _same(runs[1], runs[0])
+    st = runs[1][0].pstats
+    assert max(s.n_leaves_surv for s in st[1:]) <= budget
+    assert any(s.n_leaves_surv == budget for s in st[1:])  # the cut was applied
+    print(f"[result] levels {runs[1][0].prune_level}, leaves {sum(s.n_leaves for s in st)}, "
+          f"final {len(runs[1][1]['scores'])}", flush=True)
