@@ -455,6 +455,8 @@ int pyl_ep_timer_read(pyl_ep_timer* t, int32_t* phase_out, float* ms_out, int32_
  * exactly. Scratch arrays hold `cap` rows (the tree buffers' rows). */
 #define PYL_BUDGET_SCORE 1
 #define PYL_BUDGET_CNN 2
+#define PYL_BUDGET_FP32 0
+#define PYL_BUDGET_BF16 1
 #define PYL_BUDGET_SEL_WORDS 272
 #define PYL_BUDGET_HIST_BINS 1024
 #define PYL_BUDGET_STATS 1032 /* f64 words of `stats` */
@@ -464,6 +466,7 @@ typedef struct pyl_ep_budget {
   int32_t complex_folds; /* folds are complex64 [2][nbins/2 + 1] */
   int32_t nlevels;       /* levels of the search (the feature level / nlevels) */
   int32_t level0;        /* level loop: the search level of the call's first level */
+  int32_t precision;     /* PYL_BUDGET_FP32, or PYL_BUDGET_BF16: pyl_cnn_forward_bf16 */
   int64_t budget;        /* survivors kept per level (>= 1) */
   const float* params;   /* the network's parameters (PYL_BUDGET_CNN) */
   float* keys;           /* f32 [cap]: ranking key of every survivor */
@@ -708,6 +711,11 @@ int pyl_cnn_backward(const float* params_dev, const float* cache_dev, const floa
                      const float* labels_dev, int64_t n, int nbins, float w_pos, float w_neg,
                      float* work_dev, int64_t work_floats, float* grad_dev, void* stream);
 /* One Adam step (bias-corrected; step counts from 1). */
+/* Inference only: the forward pass with the two 32 -> 32 convolutions on MFMA (BF16
+ * operands, FP32 accumulation; the profile, the first convolution and the head in FP32).
+ * Arguments as pyl_cnn_forward without the training cache. */
+int pyl_cnn_forward_bf16(const float* params, const float* folds, int64_t n, int nbins,
+                         int complex_folds, const float* feats, float* logits, void* stream);
 int pyl_cnn_adam(float* params_dev, const float* grad_dev, float* m_dev, float* v_dev,
                  int64_t nparams, float lr, float beta1, float beta2, float eps, int step,
                  void* stream);

@@ -110,6 +110,7 @@ SIGNATURES: list[tuple[str, object, list[object]]] = [
     ("pyl_ep_timer_read", ctypes.c_int,
      [_vp, ctypes.POINTER(_c_i32), ctypes.POINTER(ctypes.c_float), _c_i32,
       ctypes.POINTER(_c_i32)]),
+    ("pyl_cnn_forward_bf16", ctypes.c_int, [_vp, _vp, _c_i64, _c_i32, _c_i32, _vp, _vp, _vp]),
     ("pyl_ep_budget_apply", ctypes.c_int,
      [_vp, _c_i32, _c_i32, _vp, _vp, _vp, _vp, _c_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("pyl_cnn_features", ctypes.c_int,

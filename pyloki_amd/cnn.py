@@ -90,15 +90,27 @@ class BranchRanker:
         self.grad = torch.zeros_like(self.params)
 
     def logits(self, folds: torch.Tensor, feats: torch.Tensor,
-               cache: torch.Tensor | None = None) -> torch.Tensor:
+               cache: torch.Tensor | None = None, precision: str = "fp32") -> torch.Tensor:
+        """The network's logits; ``precision="bf16"`` runs the convolutions on MFMA
+        (inference only, no training cache)."""
         lib = _lib.load()
         n = int(folds.shape[0])
         cplx = bool(folds.is_complex())
         out = torch.empty(n, dtype=torch.float32, device=folds.device)
         f = folds.contiguous()
         ft = feats.contiguous().float()
-        _lib.check(lib.pyl_cnn_forward(ptr(self.params), ptr(f), n, self.nbins, int(cplx),
-                                       ptr(ft), ptr(out), ptr(cache), stream_ptr()))
+        if precision == "bf16":
+            if cache is not None:
+                msg = "the BF16 forward pass keeps no training cache"
+                raise ValueError(msg)
+            _lib.check(lib.pyl_cnn_forward_bf16(ptr(self.params), ptr(f), n, self.nbins,
+                                                int(cplx), ptr(ft), ptr(out), stream_ptr()))
+        elif precision == "fp32":
+            _lib.check(lib.pyl_cnn_forward(ptr(self.params), ptr(f), n, self.nbins, int(cplx),
+                                           ptr(ft), ptr(out), ptr(cache), stream_ptr()))
+        else:
+            msg = f"unknown precision {precision!r}"
+            raise ValueError(msg)
         return out
 
     def train_step(self, folds: torch.Tensor, feats: torch.Tensor, labels: torch.Tensor,
@@ -220,12 +232,17 @@ class SurvivorBudget:
     at the cut keep the lowest rows). Installed as ``Pruning.level_hook``: the sync-free
     level loop applies it on the device (``device_budget``), the per-level path calls it."""
 
-    def __init__(self, budget: int, ranker: BranchRanker | None = None) -> None:
+    def __init__(self, budget: int, ranker: BranchRanker | None = None,
+                 precision: str = "bf16") -> None:
         if budget < 1:
             msg = "budget must be >= 1"
             raise ValueError(msg)
+        if precision not in ("fp32", "bf16"):
+            msg = f"unknown precision {precision!r}"
+            raise ValueError(msg)
         self.budget = int(budget)
         self.ranker = ranker
+        self.precision = precision  # of the network's forward pass (bf16: MFMA)
         self._cap = -1
         self._s = None
 
@@ -250,6 +267,7 @@ class SurvivorBudget:
             s.nbins = cfg.nbins
             s.complex_folds = int(bool(cfg.use_fourier))
             s.budget = self.budget
+            s.precision = 1 if self.precision == "bf16" else 0
             s.params = ptr(self.ranker.params) if self.ranker is not None else None
             for name, ten in t.items():
                 setattr(s, name, ptr(ten))
@@ -287,9 +305,9 @@ class SurvivorBudget:
         return DeviceTree(*(o[:kept] for o in out))
 
 
-def budget_hook(ranker: BranchRanker, budget: int) -> SurvivorBudget:
+def budget_hook(ranker: BranchRanker, budget: int, precision: str = "bf16") -> SurvivorBudget:
     """Keep the ``budget`` survivors the network ranks highest (in their original order)."""
-    return SurvivorBudget(budget, ranker)
+    return SurvivorBudget(budget, ranker, precision)
 
 
 def score_budget_hook(budget: int) -> SurvivorBudget:

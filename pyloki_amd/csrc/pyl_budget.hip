@@ -334,7 +334,7 @@ int pyl::ep_budget_enqueue(const pyl_ep_budget* b, int P, int level, const doubl
                               b->shist, b->feats, s);
     if (rc != PYL_OK) return rc;
     rc = pyl::cnn_forward_count(b->params, folds, n_cap, m_dev, b->nbins, b->complex_folds,
-                                b->feats, b->keys, s);
+                                b->feats, b->keys, b->precision == PYL_BUDGET_BF16, s);
     if (rc != PYL_OK) return rc;
     keys = b->keys;
   }

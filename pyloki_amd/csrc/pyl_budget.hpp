@@ -6,10 +6,11 @@
 
 namespace pyl {
 
-// The network's forward pass over at most n_cap samples, the count read from n_dev.
+// The network's forward pass over at most n_cap samples, the count read from n_dev
+// when not null; bf16 selects the MFMA inference kernel.
 int cnn_forward_count(const float* params, const float* folds, int64_t n_cap,
                       const int64_t* n_dev, int nbins, int complex_folds, const float* feats,
-                      float* logits, hipStream_t s);
+                      float* logits, int bf16, hipStream_t s);
 
 // The survivor budget of one level. The level's record `rec` (PYL_EP_REC_WORDS words; the
 // next record follows) gives the survivors (NSURV) and the first history row (HIST); the

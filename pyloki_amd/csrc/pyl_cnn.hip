@@ -19,6 +19,7 @@
 // registers over the block's tiles, and writes them once as the block's partial sum; a
 // second kernel adds the partials in block order, so the gradient is deterministic.
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 
 #include "pyl_budget.hpp"
@@ -398,6 +399,179 @@ __global__ void cnn_adam_kernel(float* __restrict__ th, const float* __restrict_
   th[o] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
+// ---- Inference on MFMA (BF16 operands, FP32 accumulation) -----------------------------
+// The two 32 -> 32 convolutions are dense contractions, out[co][p] = sum over (t, ci) of
+// W[co][t][ci] act[(p + t - 2) mod B][ci]: a 32 x B x 160 product per sample, done with
+// v_mfma_f32_32x32x8_bf16 (K ordered tap-major, so that a lane's 4 K values are 4
+// consecutive channels at one position: one 8-byte LDS read per operand). One wavefront
+// owns one sample at a time; its activations live in wave-private LDS as bf16 [p][ci];
+// the weights are staged once per block as bf16 [co][t][ci]. The profile, the first
+// convolution (1 -> 32), the max-pool and the head stay FP32. Used for ranking only: the
+// logits differ from the FP32 path by the bf16 rounding of weights and activations.
+typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ uint16_t to_bf16(float x) {  // round to nearest even
+  uint32_t u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// One 32 -> 32 convolution of wave-private act_in into the 16 accumulators per N-tile.
+// Lane l: column p = 32 nt + l % 32; acc[r] is row co = 8 (r / 4) + 4 (l / 32) + r % 4.
+template <int B>
+__device__ __forceinline__ void conv_mfma(const uint16_t* __restrict__ sw,
+                                          const uint16_t* __restrict__ act, int nt, int lane,
+                                          f32x16_t& acc) {
+  const int row = lane & 31, kg = lane >> 5;
+  for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+#pragma unroll
+  for (int t = 0; t < kK; ++t) {
+    const int p = (32 * nt + row + t - 2) & (B - 1);
+#pragma unroll
+    for (int cs = 0; cs < kC / 8; ++cs) {
+      const bf16x4_t a =
+          *reinterpret_cast<const bf16x4_t*>(sw + (row * kK + t) * kC + 8 * cs + 4 * kg);
+      const bf16x4_t b = *reinterpret_cast<const bf16x4_t*>(act + p * kC + 8 * cs + 4 * kg);
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    }
+  }
+}
+
+template <int B, bool kComplex>
+__global__ void __launch_bounds__(kThreads)
+cnn_forward_mfma_kernel(const float* __restrict__ th, const float* __restrict__ folds,
+                        int64_t n_arg, const int64_t* __restrict__ n_dev,
+                        const float* __restrict__ feats, float* __restrict__ logits) {
+  constexpr int kW = kThreads / 64;
+  __shared__ __align__(16) uint16_t sW2[kC * kK * kC];
+  __shared__ __align__(16) uint16_t sW3[kC * kK * kC];
+  __shared__ __align__(16) uint16_t sAct[kW][2][B * kC];
+  __shared__ float sX[kW][B];
+  __shared__ float sH[kW][kC];
+  __shared__ float tw[kComplex ? 2 * B : 1];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int64_t n = n_dev ? *n_dev : n_arg;
+  if ((int64_t)blockIdx.x * kW >= n) return;  // the whole block
+  for (int j = tid; j < kC * kC * kK; j += kThreads) {  // W[co][ci][t] -> [co][t][ci]
+    const int co = j / (kC * kK), rem = j % (kC * kK), ci = rem / kK, t = rem % kK;
+    sW2[(co * kK + t) * kC + ci] = to_bf16(th[oW2 + j]);
+    sW3[(co * kK + t) * kC + ci] = to_bf16(th[oW3 + j]);
+  }
+  if constexpr (kComplex) pyl::fill_twiddles(tw, B, tid, kThreads);
+  __syncthreads();
+  float* x = sX[wave];
+  uint16_t* a1 = sAct[wave][0];
+  uint16_t* a2 = sAct[wave][1];
+  for (int64_t i = (int64_t)blockIdx.x * kW + wave; i < n; i += (int64_t)gridDim.x * kW) {
+    // Profile e / sqrt(v), standardised over the bins (FP32).
+    float sum = 0.0f;
+    for (int p = lane; p < B; p += 64) {
+      float e, v;
+      if constexpr (kComplex) {
+        const float2* rowp = reinterpret_cast<const float2*>(folds) + i * 2 * (B / 2 + 1);
+        e = pyl::irfft_bin(rowp, B, p, tw);
+        v = pyl::irfft_bin(rowp + (B / 2 + 1), B, p, tw);
+      } else {
+        e = folds[i * 2 * B + p];
+        v = folds[i * 2 * B + B + p];
+      }
+      const float pv = pyl::norm_ev(e, v);
+      x[p] = pv;
+      sum += pv;
+    }
+    for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+    const float mu = sum / (float)B;
+    float ss = 0.0f;
+    for (int p = lane; p < B; p += 64) {
+      const float d = x[p] - mu;
+      ss += d * d;
+    }
+    for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o);
+    const float sd = sqrtf(ss / (float)B);
+    for (int p = lane; p < B; p += 64) x[p] = (x[p] - mu) / (sd + 1e-6f);
+    wave_sync_lds();
+    // conv1 (1 -> 32), FP32, relu, stored bf16 [p][co].
+    for (int p = lane; p < B; p += 64) {
+      float xv[kK];
+#pragma unroll
+      for (int t = 0; t < kK; ++t) xv[t] = x[(p + t - 2) & (B - 1)];
+#pragma unroll
+      for (int co = 0; co < kC; ++co) {
+        float acc = th[oB1 + co];
+#pragma unroll
+        for (int t = 0; t < kK; ++t) acc += th[oW1 + co * kK + t] * xv[t];
+        a1[p * kC + co] = to_bf16(fmaxf(acc, 0.0f));
+      }
+    }
+    wave_sync_lds();
+    // conv2 on MFMA, bias + relu, stored bf16.
+    const int kg = lane >> 5;
+    for (int nt = 0; nt < B / 32; ++nt) {
+      f32x16_t acc;
+      conv_mfma<B>(sW2, a1, nt, lane, acc);
+      const int p = 32 * nt + (lane & 31);
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int co = 8 * (r >> 2) + 4 * kg + (r & 3);
+        a2[p * kC + co] = to_bf16(fmaxf(acc[r] + th[oB2 + co], 0.0f));
+      }
+    }
+    wave_sync_lds();
+    // conv3 on MFMA, max over positions (relu and bias commute with it).
+    float mx[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) mx[r] = -FLT_MAX;
+    for (int nt = 0; nt < B / 32; ++nt) {
+      f32x16_t acc;
+      conv_mfma<B>(sW3, a2, nt, lane, acc);
+#pragma unroll
+      for (int r = 0; r < 16; ++r) mx[r] = fmaxf(mx[r], acc[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r)
+      for (int o = 16; o > 0; o >>= 1) mx[r] = fmaxf(mx[r], __shfl_xor(mx[r], o));
+    if ((lane & 31) == 0) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int co = 8 * (r >> 2) + 4 * kg + (r & 3);
+        sH[wave][co] = fmaxf(mx[r] + th[oB3 + co], 0.0f);
+      }
+    }
+    wave_sync_lds();
+    // Head (FP32): z = (h, features); u = relu(Wf1 z + bf1); logit = Wf2 . u + bf2.
+    float part = 0.0f;
+    if (lane < kH) {
+      float q = th[oBF1 + lane];
+      for (int z = 0; z < kC; ++z) q += th[oF1 + lane * kZ + z] * sH[wave][z];
+      for (int f = 0; f < kF; ++f) q += th[oF1 + lane * kZ + kC + f] * feats[i * kF + f];
+      part = th[oF2 + lane] * fmaxf(q, 0.0f);
+    }
+    for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
+    if (lane == 0) logits[i] = th[oBF2] + part;
+    wave_sync_lds();
+  }
+}
+
+template <int B>
+int launch_forward_mfma(const float* th, const float* folds, int64_t n, const int64_t* n_dev,
+                        const float* feats, float* logits, int complex_folds, hipStream_t s) {
+  constexpr int kW = kThreads / 64;
+  const unsigned grid = (unsigned)std::max<int64_t>(
+      1, std::min<int64_t>(pyl::ceil_div(n, kW), 2048));
+  if (complex_folds)
+    cnn_forward_mfma_kernel<B, true><<<grid, kThreads, 0, s>>>(th, folds, n, n_dev, feats, logits);
+  else
+    cnn_forward_mfma_kernel<B, false><<<grid, kThreads, 0, s>>>(th, folds, n, n_dev, feats, logits);
+  PYL_CHECK_LAUNCH("cnn_forward_mfma_kernel");
+  return PYL_OK;
+}
+
 // n samples, or at most n with the count read from n_dev.
 template <int B>
 int launch_forward(const float* th, const float* folds, int64_t n, const int64_t* n_dev,
@@ -441,12 +615,31 @@ extern "C" int pyl_cnn_forward(const float* params, const float* folds, int64_t 
   }
 }
 
-// The forward pass over at most n_cap samples, the count read from n_dev (no cache).
+extern "C" int pyl_cnn_forward_bf16(const float* params, const float* folds, int64_t n,
+                                    int nbins, int complex_folds, const float* feats,
+                                    float* logits, void* stream) {
+  PYL_REQUIRE(valid_bins(nbins), "the ranking network takes 32, 64 or 128 bins (got %d)", nbins);
+  PYL_REQUIRE(n >= 0, "negative sample count");
+  if (n == 0) return PYL_OK;
+  PYL_REQUIRE(params && folds && feats && logits, "null device pointer");
+  return pyl::cnn_forward_count(params, folds, n, nullptr, nbins, complex_folds, feats, logits,
+                                1, pyl::as_stream(stream));
+}
+
+// The forward pass over at most n_cap samples, the count read from n_dev when not null
+// (no cache); bf16: the MFMA inference kernel.
 int pyl::cnn_forward_count(const float* params, const float* folds, int64_t n_cap,
                            const int64_t* n_dev, int nbins, int complex_folds,
-                           const float* feats, float* logits, hipStream_t s) {
+                           const float* feats, float* logits, int bf16, hipStream_t s) {
   PYL_REQUIRE(valid_bins(nbins), "the ranking network takes 32, 64 or 128 bins (got %d)", nbins);
   if (n_cap <= 0) return PYL_OK;
+  if (bf16) {
+    switch (nbins) {
+      case 32: return launch_forward_mfma<32>(params, folds, n_cap, n_dev, feats, logits, complex_folds, s);
+      case 64: return launch_forward_mfma<64>(params, folds, n_cap, n_dev, feats, logits, complex_folds, s);
+      default: return launch_forward_mfma<128>(params, folds, n_cap, n_dev, feats, logits, complex_folds, s);
+    }
+  }
   switch (nbins) {
     case 32: return launch_forward<32>(params, folds, n_cap, n_dev, feats, logits, nullptr, complex_folds, s);
     case 64: return launch_forward<64>(params, folds, n_cap, n_dev, feats, logits, nullptr, complex_folds, s);

@@ -107,7 +107,8 @@ BUDGET_SEL_WORDS, BUDGET_HIST_BINS, BUDGET_STATS = 272, 1024, 1032
 class EpBudget(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("nbins", ctypes.c_int32),
                 ("complex_folds", ctypes.c_int32), ("nlevels", ctypes.c_int32),
-                ("level0", ctypes.c_int32), ("budget", ctypes.c_int64)] + [
+                ("level0", ctypes.c_int32), ("precision", ctypes.c_int32),
+                ("budget", ctypes.c_int64)] + [
         (name, ctypes.c_void_p) for name in (
             "params", "keys", "feats", "kept", "bt_tmp", "sel", "stats", "shist", "part")]
 

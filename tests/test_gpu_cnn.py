@@ -182,3 +182,28 @@ def test_budgeted_search_pipeline(gpu_device):
     assert c["precision"] == 1.0 and s["precision"] == 1.0
     assert c["trials"] < ex["trials"] and s["trials"] < ex["trials"]
     assert c["recall"] >= 0.8
+
+
+@pytest.mark.parametrize("B,cplx", [(32, False), (64, False), (128, False), (64, True)])
+def test_cnn_forward_bf16_ranks_like_fp32(gpu_device, B, cplx):
+    """The MFMA (BF16 operands) inference pass against the FP64 oracle: logits within
+    0.05 of max(1, |logit|) (bf16 weights and activations), and the ranking of the
+    samples nearly unchanged (Spearman correlation with the FP32 logits >= 0.995)."""
+    _case(f"cnn forward bf16 B={B} complex={cplx}")
+    folds, feats, _ = _batch(777, B, 31 + B)
+    th = cn.init_params(5) + np.random.default_rng(8).normal(0, 0.02, cn.NPARAMS).astype(
+        np.float32)
+    r = cnn.BranchRanker(B, params=th)
+    fd = folds
+    if cplx:
+        fd = np.fft.rfft(folds.astype(np.float64), axis=-1).astype(np.complex64)
+        folds = np.fft.irfft(fd.astype(np.complex128), n=B, axis=-1)
+    got = r.logits(_dev(fd), _dev(feats), precision="bf16").cpu().numpy()
+    f32 = r.logits(_dev(fd), _dev(feats)).cpu().numpy()
+    exp = cn.forward(th, folds, feats)
+    err = np.abs(got - exp) / np.maximum(1.0, np.abs(exp))
+    rank = lambda x: np.argsort(np.argsort(x))  # noqa: E731
+    rho = np.corrcoef(rank(got), rank(f32))[0, 1]
+    print(f"[result] bf16 logits max error {err.max():.2e}, spearman {rho:.5f}", flush=True)
+    assert err.max() <= 0.05
+    assert rho >= 0.995
