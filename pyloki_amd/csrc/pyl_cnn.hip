@@ -422,6 +422,22 @@ __device__ __forceinline__ void wave_sync_lds() {
   __builtin_amdgcn_wave_barrier();
 }
 
+// pyl::irfft_bin's arithmetic, in its order, over a row staged in LDS (partly unrolled:
+// full unrolling keeps every harmonic in registers).
+template <int B>
+__device__ __forceinline__ float irfft_bin_lds(const float2* X, int b, const float* tw) {
+  constexpr int nb_f = B / 2 + 1;
+  float acc = X[0].x;
+  const float ny = X[nb_f - 1].x;
+  acc += (b & 1) ? -ny : ny;
+#pragma unroll 4
+  for (int kk = 1; kk < nb_f - 1; ++kk) {
+    const int idx = (kk * b) & (B - 1);
+    acc += 2.0f * (X[kk].x * tw[idx] - X[kk].y * tw[B + idx]);
+  }
+  return acc / (float)B;
+}
+
 // One 32 -> 32 convolution of wave-private act_in into the 16 accumulators per N-tile.
 // Lane l: column p = 32 nt + l % 32; acc[r] is row co = 8 (r / 4) + 4 (l / 32) + r % 4.
 template <int B>
@@ -469,14 +485,20 @@ cnn_forward_mfma_kernel(const float* __restrict__ th, const float* __restrict__ 
   uint16_t* a1 = sAct[wave][0];
   uint16_t* a2 = sAct[wave][1];
   for (int64_t i = (int64_t)blockIdx.x * kW + wave; i < n; i += (int64_t)gridDim.x * kW) {
-    // Profile e / sqrt(v), standardised over the bins (FP32).
+    // Profile e / sqrt(v), standardised over the bins (FP32). A complex row is staged in
+    // the wave's activation buffer first (one coalesced load), then inverse-transformed.
     float sum = 0.0f;
+    float2* xs = reinterpret_cast<float2*>(a1);
+    if constexpr (kComplex) {
+      const float2* rowp = reinterpret_cast<const float2*>(folds) + i * 2 * (B / 2 + 1);
+      for (int j = lane; j < 2 * (B / 2 + 1); j += 64) xs[j] = rowp[j];
+      wave_sync_lds();
+    }
     for (int p = lane; p < B; p += 64) {
       float e, v;
       if constexpr (kComplex) {
-        const float2* rowp = reinterpret_cast<const float2*>(folds) + i * 2 * (B / 2 + 1);
-        e = pyl::irfft_bin(rowp, B, p, tw);
-        v = pyl::irfft_bin(rowp + (B / 2 + 1), B, p, tw);
+        e = irfft_bin_lds<B>(xs, p, tw);
+        v = irfft_bin_lds<B>(xs + (B / 2 + 1), p, tw);
       } else {
         e = folds[i * 2 * B + p];
         v = folds[i * 2 * B + B + p];
@@ -501,7 +523,7 @@ cnn_forward_mfma_kernel(const float* __restrict__ th, const float* __restrict__ 
       float xv[kK];
 #pragma unroll
       for (int t = 0; t < kK; ++t) xv[t] = x[(p + t - 2) & (B - 1)];
-#pragma unroll
+#pragma unroll 4
       for (int co = 0; co < kC; ++co) {
         float acc = th[oB1 + co];
 #pragma unroll
