@@ -270,17 +270,22 @@ circ_children_kernel(LevelConst k, CircConst q, const double* __restrict__ leave
     s_parent[tid] = (int32_t)par;
     s_row[tid] = row;
     s_shift[tid] = sh;
-    if constexpr (kComplex) {
-      const int nb_f = B / 2 + 1;
-      cfold_child(reinterpret_cast<const float2*>(seg) + (int64_t)row * 2 * nb_f,
-                  reinterpret_cast<const float2*>(folds) + par * 2 * nb_f, phase, B,
-                  reinterpret_cast<float2*>(S) + (size_t)tid * 2 * nb_f);
-    }
+    if constexpr (kComplex)
+      cphasors(phase, B,
+               reinterpret_cast<float2*>(reinterpret_cast<char*>(S) +
+                                         complex_ph_offset(B, k.sc.wmax)) +
+                   (size_t)tid * (B / 2 + 1));
   }
   __syncthreads();
   float* Sc = S;
   if constexpr (kComplex) {
-    Sc = complex_block_columns(k, reinterpret_cast<char*>(S), nvalid, tid);
+    char* lds = reinterpret_cast<char*>(S);
+    cfold_rows(reinterpret_cast<const float2*>(seg), reinterpret_cast<const float2*>(folds),
+               s_row, s_parent,
+               reinterpret_cast<const float2*>(lds + complex_ph_offset(B, k.sc.wmax)), nvalid,
+               B, reinterpret_cast<float2*>(lds), tid, kBlk);
+    __syncthreads();
+    Sc = complex_block_columns(k, lds, nvalid, tid);
   } else {
     for (int r = 0; r < nvalid; ++r) {
       const float* fr = folds + (int64_t)s_parent[r] * 2 * B;
@@ -318,6 +323,7 @@ circ_materialize_kernel(LevelConst k, CircConst q, const double* __restrict__ le
                         float* __restrict__ o_folds, float* __restrict__ o_scores,
                         int32_t* __restrict__ o_bt_base, const int64_t* __restrict__ bt_row0) {
   __shared__ int64_t s_v[kBlk];
+  extern __shared__ float2 s_ph[];  // complex: [kBlk][B/2 + 1] phasors, then rows, parents
   const int tid = threadIdx.x;
   const int64_t n_surv = dev_count(n_surv_arg, n_dev);
   const int64_t s0 = (int64_t)blockIdx.x * kBlk;
@@ -362,14 +368,21 @@ circ_materialize_kernel(LevelConst k, CircConst q, const double* __restrict__ le
     bt[kP - 1] = row / k.n_freq;
     bt[kP] = row % k.n_freq;
     bt[kP + 1] = (int32_t)c_phase[v];
-    if constexpr (kComplex) {
-      const int nb_f = B / 2 + 1;
-      cfold_child(reinterpret_cast<const float2*>(seg) + (int64_t)row * 2 * nb_f,
-                  reinterpret_cast<const float2*>(folds) + par * 2 * nb_f, c_phase[v], B,
-                  reinterpret_cast<float2*>(o_folds) + s * 2 * nb_f);
-    }
+    if constexpr (kComplex) cphasors(c_phase[v], B, s_ph + (size_t)tid * (B / 2 + 1));
   }
-  if constexpr (!kComplex) {
+  if constexpr (kComplex) {
+    __syncthreads();
+    int32_t* s_rw = reinterpret_cast<int32_t*>(s_ph + (size_t)kBlk * (B / 2 + 1));
+    int32_t* s_pa = s_rw + kBlk;
+    if (tid < nvalid) {
+      s_rw[tid] = c_row[s_v[tid]];
+      s_pa[tid] = c_parent[s_v[tid]];
+    }
+    __syncthreads();
+    cfold_rows(reinterpret_cast<const float2*>(seg), reinterpret_cast<const float2*>(folds),
+               s_rw, s_pa, s_ph, nvalid, B,
+               reinterpret_cast<float2*>(o_folds) + s0 * 2 * (B / 2 + 1), tid, kBlk);
+  } else {
     __syncthreads();
     for (int r = 0; r < nvalid; ++r) {
       const int64_t v = s_v[r];
@@ -599,7 +612,8 @@ static int circ_materialize(const pyl_prune_params* p, const pyl_circ_params* q,
   PYL_REQUIRE(leaves && folds && valid && seg && c_parent && c_row && c_shift && c_phase &&
                   c_score && has_taylor && surv && o_leaves && o_folds && o_scores && o_bt,
               "null device pointer");
-  circ_materialize_kernel<kComplex><<<(unsigned)pyl::ceil_div(n_surv, kBlk), kBlk, 0,
+  circ_materialize_kernel<kComplex><<<(unsigned)pyl::ceil_div(n_surv, kBlk), kBlk,
+                                      materialize_lds(kComplex, k.nbins),
                                       pyl::as_stream(stream)>>>(
       k, cc, leaves, folds, *L, valid, seg, c_parent, c_row, c_shift, c_phase, c_score,
       has_taylor, surv, n_surv, nullptr, o_leaves, o_folds, o_scores, o_bt, nullptr);
@@ -914,7 +928,7 @@ int circ_levels(const pyl_prune_params* p, const pyl_circ_params* q,
     PYL_CHECK_LAUNCH("chunk_scan_kernel");
     if (tm) tm->end(s);
     if (tm) tm->begin(PYL_EP_PH_MATERIALIZE, s);
-    circ_materialize_kernel<kComplex><<<g_surv, kBlk, 0, s>>>(
+    circ_materialize_kernel<kComplex><<<g_surv, kBlk, materialize_lds(kComplex, k.nbins), s>>>(
         k, cc, leaves, folds, cl, C->valid, seg, L->c_parent, L->c_row, L->c_shift,
         L->c_phase, L->c_score, C->has_taylor, L->surv, 0, rec + kRecWords + kRecNpar,
         L->leaves[o], L->folds[o], L->scores[o], bud ? bud->bt_tmp : L->hist,

@@ -190,17 +190,21 @@ prune_children_kernel(LevelConst k, const double* __restrict__ leaves,
     s_parent[tid] = (int32_t)par;
     s_row[tid] = row;
     s_shift[tid] = sh;
-    if constexpr (kComplex) {
-      const float2* fc = reinterpret_cast<const float2*>(folds);
-      const float2* sc = reinterpret_cast<const float2*>(seg);
-      cfold_child(sc + (int64_t)row * 2 * nb_f, fc + par * 2 * nb_f, phase, B,
-                  reinterpret_cast<float2*>(S) + (size_t)tid * 2 * nb_f);
-    }
+    if constexpr (kComplex)
+      cphasors(phase, B,
+               reinterpret_cast<float2*>(reinterpret_cast<char*>(S) +
+                                         complex_ph_offset(B, k.sc.wmax)) + (size_t)tid * nb_f);
   }
   __syncthreads();
   float* Sc = S;
   if constexpr (kComplex) {
-    Sc = complex_block_columns(k, reinterpret_cast<char*>(S), nvalid, tid);
+    char* lds = reinterpret_cast<char*>(S);
+    cfold_rows(reinterpret_cast<const float2*>(seg), reinterpret_cast<const float2*>(folds),
+               s_row, s_parent,
+               reinterpret_cast<const float2*>(lds + complex_ph_offset(B, k.sc.wmax)), nvalid,
+               B, reinterpret_cast<float2*>(lds), tid, kBlk);
+    __syncthreads();
+    Sc = complex_block_columns(k, lds, nvalid, tid);
   } else {
     // Fold (common.py:389-399: parent row + segment row rotated by the shift) and
     // normalise, lane = bin, transposed into S.
@@ -242,6 +246,7 @@ prune_materialize_kernel(LevelConst k, const double* __restrict__ leaves,
                          float* __restrict__ o_scores, int32_t* __restrict__ o_bt_base,
                          const int64_t* __restrict__ bt_row0) {
   __shared__ int64_t s_child[kBlk];
+  extern __shared__ float2 s_ph[];  // complex: [kBlk][B/2 + 1] phasors, then rows, parents
   const int tid = threadIdx.x;
   const int64_t n_surv = dev_count(n_surv_arg, n_surv_dev);
   const int P = k.P;
@@ -275,14 +280,21 @@ prune_materialize_kernel(LevelConst k, const double* __restrict__ leaves,
     bt[k.nparams_ffa] = row % k.n_freq;
     for (int q = k.nparams_ffa + 1; q < P + 1; ++q) bt[q] = 0;
     bt[P + 1] = (int32_t)c_phase[c];
-    if constexpr (kComplex) {
-      const int nb_f = B / 2 + 1;
-      cfold_child(reinterpret_cast<const float2*>(seg) + (int64_t)row * 2 * nb_f,
-                  reinterpret_cast<const float2*>(folds) + par * 2 * nb_f, c_phase[c], B,
-                  reinterpret_cast<float2*>(o_folds) + s * 2 * nb_f);
-    }
+    if constexpr (kComplex) cphasors(c_phase[c], B, s_ph + (size_t)tid * (B / 2 + 1));
   }
-  if constexpr (!kComplex) {
+  if constexpr (kComplex) {
+    __syncthreads();
+    int32_t* s_rw = reinterpret_cast<int32_t*>(s_ph + (size_t)kBlk * (B / 2 + 1));
+    int32_t* s_pa = s_rw + kBlk;
+    if (tid < nvalid) {
+      s_rw[tid] = c_row[s_child[tid]];
+      s_pa[tid] = c_parent[s_child[tid]];
+    }
+    __syncthreads();
+    cfold_rows(reinterpret_cast<const float2*>(seg), reinterpret_cast<const float2*>(folds),
+               s_rw, s_pa, s_ph, nvalid, B,
+               reinterpret_cast<float2*>(o_folds) + s0 * 2 * (B / 2 + 1), tid, kBlk);
+  } else {
     __syncthreads();
     for (int r = 0; r < nvalid; ++r) {
       const int64_t c = s_child[r];
@@ -484,7 +496,8 @@ static int prune_materialize(const pyl_prune_params* p, const pyl_prune_coords* 
                   c_shift && c_phase && c_score && surv && o_leaves && o_folds && o_scores &&
                   o_bt,
               "null device pointer");
-  prune_materialize_kernel<kComplex><<<(unsigned)pyl::ceil_div(n_surv, kBlk), kBlk, 0,
+  prune_materialize_kernel<kComplex><<<(unsigned)pyl::ceil_div(n_surv, kBlk), kBlk,
+                                       materialize_lds(kComplex, k.nbins),
                                        pyl::as_stream(stream)>>>(
       k, leaves, folds, branch, counts, off, seg, c_parent, c_row, c_shift, c_phase, c_score,
       surv, n_surv, nullptr, o_leaves, o_folds, o_scores, o_bt, nullptr);
@@ -700,7 +713,7 @@ extern "C" int pyl_ep_timer_create(int32_t capacity, pyl_ep_timer** out) {
 extern "C" int pyl_ep_timer_destroy(pyl_ep_timer* t) {
   if (t == nullptr) return PYL_OK;
   for (auto e : t->ev)
-    if (e) hipEventDestroy(e);
+    if (e) (void)hipEventDestroy(e);
   delete t;
   return PYL_OK;
 }
@@ -808,7 +821,7 @@ int ep_levels(const pyl_prune_params* p, const pyl_prune_coords* coords, const d
     PYL_CHECK_LAUNCH("chunk_scan_kernel");
     if (tm) tm->end(s);
     if (tm) tm->begin(PYL_EP_PH_MATERIALIZE, s);
-    prune_materialize_kernel<kComplex><<<g_surv, kBlk, 0, s>>>(
+    prune_materialize_kernel<kComplex><<<g_surv, kBlk, materialize_lds(kComplex, k.nbins), s>>>(
         k, src, folds, L->branch, L->counts, L->off, seg, L->c_parent, L->c_row, L->c_shift,
         L->c_phase, L->c_score, L->surv, 0, rec + kRecWords + kRecNpar, L->leaves[o],
         L->folds[o], L->scores[o], bud ? bud->bt_tmp : L->hist, bud ? nullptr : rec + kRecHist);

@@ -355,6 +355,41 @@ __device__ inline void cfold_child(const float2* __restrict__ seg, const float2*
   }
 }
 
+// The same child fold for a block of children, in two steps whose arithmetic is
+// cfold_child's: every child's phasor powers phi^0..phi^(B/2) (its thread, the same
+// recurrence) into ph[child][k], then the rows formed with lane = (component, harmonic),
+// one child after another, so that segment and parent rows are read coalesced. Xs (or an
+// output) row r = child r of the block.
+__device__ inline void cphasors(double phase, int nbins, float2* ph) {
+  const int nb_f = nbins / 2 + 1;
+  const float s32 = (float)phase;
+  const float ang = (float)((-2.0 * (double)(float)M_PI * (double)s32) / (double)nbins);
+  const float2 d = make_float2(cosf(ang), sinf(ang));
+  float2 p = make_float2(1.0f, 0.0f);
+  for (int kk = 0; kk < nb_f; ++kk) {
+    ph[kk] = p;
+    p = pyl::cmul(p, d);
+  }
+}
+
+__device__ inline void cfold_rows(const float2* __restrict__ seg,
+                                  const float2* __restrict__ folds,
+                                  const int32_t* s_row, const int32_t* s_parent,
+                                  const float2* ph, int nvalid, int nbins, float2* X,
+                                  int tid, int nthreads) {
+  const int nb_f = nbins / 2 + 1;
+  for (int r = 0; r < nvalid; ++r) {
+    const float2* sr = seg + (int64_t)s_row[r] * 2 * nb_f;
+    const float2* fr = folds + (int64_t)s_parent[r] * 2 * nb_f;
+    for (int j = tid; j < 2 * nb_f; j += nthreads) {
+      const int kk = j < nb_f ? j : j - nb_f;
+      const float2 a = pyl::cmul(sr[j], ph[r * nb_f + kk]);
+      const float2 f = fr[j];
+      X[(int64_t)r * 2 * nb_f + j] = make_float2(a.x + f.x, a.y + f.y);
+    }
+  }
+}
+
 // common.py:463-490: X += row[c][k] phi^k for one segment of the ascend.
 __device__ inline void cascend_add(const float2* __restrict__ row, double phase, int nbins,
                                    float2* X) {
@@ -374,10 +409,22 @@ __device__ inline void cascend_add(const float2* __restrict__ row, double phase,
 }
 
 // LDS of the complex block scorer: rows Xs [kBlk][2][B/2 + 1] complex, the scorer's columns
-// and the irfft twiddles.
+// and the irfft twiddles. The children's phasor powers [kBlk][B/2 + 1] are needed only
+// before the columns are written, so they share the columns' space when it is large
+// enough, else they follow the twiddles (16-byte aligned).
+__host__ __device__ inline size_t complex_ph_offset(int nbins, int wmax) {
+  const size_t xs = sizeof(float2) * (size_t)kBlk * 2 * (nbins / 2 + 1);
+  const size_t cols = sizeof(float) * (size_t)(nbins + wmax) * kStride;
+  const size_t ph = sizeof(float2) * (size_t)kBlk * (nbins / 2 + 1);
+  if (ph <= cols) return xs;
+  return (xs + cols + sizeof(float) * 2 * (size_t)nbins + 15) & ~(size_t)15;
+}
 __host__ __device__ inline size_t complex_block_lds(int nbins, int wmax) {
-  return sizeof(float2) * (size_t)kBlk * 2 * (nbins / 2 + 1) +
-         sizeof(float) * ((size_t)(nbins + wmax) * kStride + 2 * nbins);
+  const size_t base = sizeof(float2) * (size_t)kBlk * 2 * (nbins / 2 + 1) +
+                      sizeof(float) * ((size_t)(nbins + wmax) * kStride + 2 * nbins);
+  const size_t ph_end =
+      complex_ph_offset(nbins, wmax) + sizeof(float2) * (size_t)kBlk * (nbins / 2 + 1);
+  return base > ph_end ? base : ph_end;
 }
 
 // irfft + e/sqrt(v) of the block's rows (lane = bin) into the scorer's columns; the caller
@@ -402,6 +449,12 @@ __device__ inline float* complex_block_columns(const LevelConst& k, char* lds, i
   }
   __syncthreads();
   return S;
+}
+
+// Dynamic LDS of the complex materialize blocks: phasors [kBlk][B/2 + 1], rows and parents.
+inline size_t materialize_lds(bool complex_folds, int nbins) {
+  return complex_folds ? sizeof(float2) * (size_t)kBlk * (nbins / 2 + 1) + 2 * sizeof(int32_t) * kBlk
+                       : 0;
 }
 
 // Dynamic LDS of the EP scoring blocks: the real scorer's columns, or the complex rows, the
