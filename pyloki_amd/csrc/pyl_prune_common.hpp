@@ -439,7 +439,36 @@ __device__ inline float* complex_block_columns(const LevelConst& k, char* lds, i
   float* tw = S + (B + k.sc.wmax) * kStride;
   pyl::fill_twiddles(tw, B, tid, kBlk);
   __syncthreads();
-  for (int r = 0; r < nvalid; ++r) {
+  // Four children at a time: eight independent accumulations per lane (each one
+  // pyl::irfft_bin's, in its order) instead of one long dependent chain per row.
+  int r = 0;
+  for (; r + 4 <= nvalid; r += 4) {
+    const float2* X = Xs + (size_t)r * 2 * nb_f;
+    for (int b = tid; b < B; b += kBlk) {
+      float acc[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const float2* row = X + (size_t)q * nb_f;  // child r + q / 2, component q % 2
+        const float ny = row[nb_f - 1].x;
+        acc[q] = row[0].x;
+        acc[q] += (b & 1) ? -ny : ny;
+      }
+      for (int kk = 1; kk < nb_f - 1; ++kk) {
+        const int idx = (kk * b) & (B - 1);
+        const float c = tw[idx], sn = tw[B + idx];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const float2 xk = X[(size_t)q * nb_f + kk];
+          acc[q] += 2.0f * (xk.x * c - xk.y * sn);
+        }
+      }
+      const float inv = 1.0f / (float)B;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        S[b * kStride + r + q] = pyl::norm_ev(acc[2 * q] * inv, acc[2 * q + 1] * inv);
+    }
+  }
+  for (; r < nvalid; ++r) {
     const float2* X = Xs + (size_t)r * 2 * nb_f;
     for (int b = tid; b < B; b += kBlk) {
       const float e = pyl::irfft_bin(X, B, b, tw);
