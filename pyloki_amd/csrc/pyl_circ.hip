@@ -131,12 +131,25 @@ __device__ __forceinline__ int64_t dev_count(int64_t n, const int64_t* n_dev) {
   return n_dev ? *n_dev : n;
 }
 
-__global__ void circ_branch_kernel(LevelConst k, CircConst q, const double* __restrict__ leaves,
-                                   pyl_circ_level L, const int64_t* __restrict__ n_dev,
-                                   int64_t* __restrict__ nchild,
-                                   unsigned long long* __restrict__ err) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= dev_count(L.n_par, n_dev)) return;
+// Per-item bodies of the thread-per-item kernels (defined below the kernels).
+__device__ __forceinline__ void circ_branch_one(const LevelConst& k, const CircConst& q,
+                                                const double* __restrict__ leaves,
+                                                const pyl_circ_level& L, int64_t i,
+                                                int64_t* __restrict__ nchild,
+                                                unsigned long long* __restrict__ err);
+__device__ __forceinline__ void circ_stage1_one(const LevelConst& k, const CircConst& q,
+                                                const double* __restrict__ leaves,
+                                                const pyl_circ_level& L, int64_t n_par,
+                                                int64_t s);
+__device__ __forceinline__ void circ_expand_one(const LevelConst& k, const CircConst& q,
+                                                const double* __restrict__ leaves,
+                                                const pyl_circ_level& L, int64_t s);
+
+__device__ __forceinline__ void circ_branch_one(const LevelConst& k, const CircConst& q,
+                                                const double* __restrict__ leaves,
+                                                const pyl_circ_level& L, int64_t i,
+                                                int64_t* __restrict__ nchild,
+                                                unsigned long long* __restrict__ err) {
   const double* lf = leaves + i * kRows * 2;
   const double f0 = lf[(kP + 1) * 2];
   // Failures in the order the reference raises them: per batch, parameters 1..4 of every
@@ -189,12 +202,31 @@ __global__ void circ_branch_kernel(LevelConst k, CircConst q, const double* __re
   nchild[i] = total;
 }
 
+__global__ void circ_branch_kernel(LevelConst k, CircConst q, const double* __restrict__ leaves,
+                                   pyl_circ_level L, const int64_t* __restrict__ n_dev,
+                                   int64_t* __restrict__ nchild,
+                                   unsigned long long* __restrict__ err) {
+  const int64_t n = dev_count(L.n_par, n_dev);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    circ_branch_one(k, q, leaves, L, i, nchild, err);
+}
+
 __global__ void circ_stage1_kernel(LevelConst k, CircConst q, const double* __restrict__ leaves,
                                    pyl_circ_level L, const int64_t* __restrict__ npar_dev,
                                    const int64_t* __restrict__ n_dev) {
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= dev_count(L.n_s1, n_dev)) return;
-  const int64_t par = find_parent(L.off1, dev_count(L.n_par, npar_dev), s);
+  const int64_t n = dev_count(L.n_s1, n_dev);
+  const int64_t n_par = dev_count(L.n_par, npar_dev);
+  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < n;
+       s += (int64_t)gridDim.x * blockDim.x)
+    circ_stage1_one(k, q, leaves, L, n_par, s);
+}
+
+__device__ __forceinline__ void circ_stage1_one(const LevelConst& k, const CircConst& q,
+                                                const double* __restrict__ leaves,
+                                                const pyl_circ_level& L, int64_t n_par,
+                                                int64_t s) {
+  const int64_t par = find_parent(L.off1, n_par, s);
   ChildLeaf ch;
   circ_decode(k, leaves, L, par, s, -1, ch);
   // circular.py:169-201: the snap-accel system is not circular but jerk is significant.
@@ -208,8 +240,15 @@ __global__ void circ_stage1_kernel(LevelConst k, CircConst q, const double* __re
 
 __global__ void circ_expand_kernel(LevelConst k, CircConst q, const double* __restrict__ leaves,
                                    pyl_circ_level L, const int64_t* __restrict__ n_dev) {
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= dev_count(L.n_s1, n_dev)) return;
+  const int64_t n = dev_count(L.n_s1, n_dev);
+  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < n;
+       s += (int64_t)gridDim.x * blockDim.x)
+    circ_expand_one(k, q, leaves, L, s);
+}
+
+__device__ __forceinline__ void circ_expand_one(const LevelConst& k, const CircConst& q,
+                                                const double* __restrict__ leaves,
+                                                const pyl_circ_level& L, int64_t s) {
   const int64_t par = L.s1_parent[s];
   const bool expand = L.s1_expand[s] != 0;
   const int32_t nout = expand ? L.cr_num[par] : 1;
@@ -287,6 +326,7 @@ circ_children_kernel(LevelConst k, CircConst q, const double* __restrict__ leave
     __syncthreads();
     Sc = complex_block_columns(k, lds, nvalid, tid);
   } else {
+#pragma unroll 4
     for (int r = 0; r < nvalid; ++r) {
       const float* fr = folds + (int64_t)s_parent[r] * 2 * B;
       const float* sr = seg + (int64_t)s_row[r] * 2 * B;
@@ -751,15 +791,17 @@ __global__ void circ_batch_totals_kernel(const int64_t* __restrict__ off1,
                                          const int64_t* __restrict__ rec, int64_t batch_size,
                                          int64_t* __restrict__ batch_tot,
                                          int64_t* __restrict__ batch_nk) {
-  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= rec[PYL_EP_REC_NBATCH]) return;
+  const int64_t nb = rec[PYL_EP_REC_NBATCH];
   const int64_t n_par = rec[kRecNpar];
-  const int64_t p0 = b * batch_size;
-  const int64_t p1 = std::min<int64_t>(p0 + batch_size, n_par);
-  const int64_t d = held_off[off1[p1]] - held_off[off1[p0]];
-  const int64_t nk = d & 0xffffffffLL;
-  batch_nk[b] = nk;
-  batch_tot[b] = nk + (d >> 32);
+  for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < nb;
+       b += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p0 = b * batch_size;
+    const int64_t p1 = std::min<int64_t>(p0 + batch_size, n_par);
+    const int64_t d = held_off[off1[p1]] - held_off[off1[p0]];
+    const int64_t nk = d & 0xffffffffLL;
+    batch_nk[b] = nk;
+    batch_tot[b] = nk + (d >> 32);
+  }
 }
 
 // First child of every batch; the level's children must fit the buffers.
@@ -785,11 +827,13 @@ __global__ void circ_positions_kernel(const int64_t* __restrict__ off1,
                                       const int64_t* __restrict__ batch_nk,
                                       const int64_t* __restrict__ n_dev, int64_t batch_size,
                                       int64_t* __restrict__ s1_pos) {
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= *n_dev) return;
-  const int64_t b = s1_parent[s] / batch_size;
-  const int64_t d = held_off[s] - held_off[off1[b * batch_size]];
-  s1_pos[s] = batch_off[b] + (s1_expand[s] ? batch_nk[b] + (d >> 32) : (d & 0xffffffffLL));
+  const int64_t n = *n_dev;
+  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < n;
+       s += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = s1_parent[s] / batch_size;
+    const int64_t d = held_off[s] - held_off[off1[b * batch_size]];
+    s1_pos[s] = batch_off[b] + (s1_expand[s] ? batch_nk[b] + (d >> 32) : (d & 0xffffffffLL));
+  }
 }
 
 struct ValidSink {
@@ -803,8 +847,10 @@ struct ValidSink {
 
 __global__ void circ_taylor_flags_clear_kernel(uint8_t* __restrict__ has_taylor,
                                                const int64_t* __restrict__ rec) {
-  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b < rec[PYL_EP_REC_NBATCH]) has_taylor[b] = 0;
+  const int64_t nb = rec[PYL_EP_REC_NBATCH];
+  for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < nb;
+       b += (int64_t)gridDim.x * blockDim.x)
+    has_taylor[b] = 0;
 }
 
 // circular.py:506-511: batches whose validated children include a Taylor cell.
@@ -813,9 +859,10 @@ __global__ void circ_taylor_flags_kernel(const int64_t* __restrict__ valid,
                                          const int32_t* __restrict__ c_parent,
                                          const int64_t* __restrict__ rec, int64_t batch_size,
                                          uint8_t* __restrict__ has_taylor) {
-  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= rec[kRecNvalid]) return;
-  if (c_region[valid[v]] == PYL_REGION_TAYLOR) has_taylor[c_parent[v] / batch_size] = 1;
+  const int64_t nv = rec[kRecNvalid];
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
+       v += (int64_t)gridDim.x * blockDim.x)
+    if (c_region[valid[v]] == PYL_REGION_TAYLOR) has_taylor[c_parent[v] / batch_size] = 1;
 }
 
 template <bool kComplex>
@@ -866,8 +913,8 @@ int circ_levels(const pyl_prune_params* p, const pyl_circ_params* q,
   cl.c_keep = C->c_keep;
   cl.c_region = C->c_region;
   const int tpar = 128;
-  const unsigned g_par = capacity_grid(L->cap_par, tpar);
-  const unsigned g_held = capacity_grid(L->cap_child, tpar);
+  const unsigned g_par = strided_grid(L->cap_par, tpar);
+  const unsigned g_held = strided_grid(L->cap_child, tpar);
   const unsigned g_child = capacity_grid(L->cap_child, kBlk);
   const unsigned g_surv = capacity_grid(L->max_sugg, kBlk);
   for (int l = 0; l < nlev; ++l) {

@@ -214,6 +214,13 @@ __global__ void ep_records_init_kernel(int64_t* __restrict__ rec, float* __restr
   }
 }
 
+// Blocks of a launch whose threads stride over at most `cap` items (capped: blocks that
+// find no item cost a dispatch each, and a level's count is unknown at launch).
+inline unsigned strided_grid(int64_t cap, int per_block) {
+  const int64_t g = (cap + per_block - 1) / per_block;
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, 2048));
+}
+
 // Blocks of a launch that covers `cap` items, `per_block` per block (at most 2^31 - 1).
 inline unsigned capacity_grid(int64_t cap, int per_block) {
   const int64_t g = (cap + per_block - 1) / per_block;

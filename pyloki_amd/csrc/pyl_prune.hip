@@ -31,10 +31,12 @@ namespace {
 using namespace pyl_ep;
 
 // The kernels below take their item count either by value or, when the pointer is not null,
-// from device memory (the sync-free level loop, whose launches are sized for the buffers'
-// capacity: blocks past the count exit at once). They have no grid-stride loop: a loop
-// makes the compiler keep the level constants live in scalar registers across iterations,
-// which spills them and costs the children kernel a fifth of its speed.
+// from device memory (the sync-free level loop). The light thread-per-parent kernels
+// stride over their items, so the loop launches them with a capped grid. The children and
+// materialize kernels have no such loop (a loop makes the compiler keep the level
+// constants live in scalar registers across iterations, which spills them and costs the
+// children kernel a fifth of its speed): the loop sizes their launches for the buffers'
+// capacity and blocks past the count exit at once.
 __device__ __forceinline__ int64_t dev_count(int64_t n, const int64_t* n_dev) {
   return n_dev ? *n_dev : n;
 }
@@ -110,15 +112,19 @@ __global__ void prune_branch_kernel(LevelConst k, const double* __restrict__ lea
                                     double* __restrict__ branch, int32_t* __restrict__ counts,
                                     int64_t* __restrict__ nchild,
                                     unsigned long long* __restrict__ err) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < dev_count(n_arg, n_dev)) branch_one(k, leaves, i, branch, counts, nchild, err);
+  const int64_t n = dev_count(n_arg, n_dev);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    branch_one(k, leaves, i, branch, counts, nchild, err);
 }
 
 __global__ void cheb_parents_kernel(LevelConst k, const double* __restrict__ leaves,
                                     int64_t n_arg, const int64_t* __restrict__ n_dev,
                                     double* __restrict__ leaves_cur) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < dev_count(n_arg, n_dev)) cheb_parent_one(k, leaves, i, leaves_cur);
+  const int64_t n = dev_count(n_arg, n_dev);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    cheb_parent_one(k, leaves, i, leaves_cur);
 }
 
 // Folds are real [2][B] rows or, with kComplex, complex64 [2][B/2 + 1] rows (as float2).
@@ -208,6 +214,7 @@ prune_children_kernel(LevelConst k, const double* __restrict__ leaves,
   } else {
     // Fold (common.py:389-399: parent row + segment row rotated by the shift) and
     // normalise, lane = bin, transposed into S.
+#pragma unroll 4
     for (int r = 0; r < nvalid; ++r) {
       const float* fr = folds + (int64_t)s_parent[r] * 2 * B;
       const float* sr = seg + (int64_t)s_row[r] * 2 * B;
@@ -776,7 +783,7 @@ int ep_levels(const pyl_prune_params* p, const pyl_prune_coords* coords, const d
     if (rc != PYL_OK) return rc;
     const size_t lds = ep_block_lds<kComplex>(k);
     if (l == 0) {  // grids for the capacities: blocks past a level's count exit at once
-      g_par = capacity_grid(L->cap_par, tpar);
+      g_par = strided_grid(L->cap_par, tpar);
       g_child = capacity_grid(L->cap_child, kBlk);
       g_surv = capacity_grid(L->max_sugg, kBlk);
     }

@@ -378,6 +378,7 @@ __device__ inline void cfold_rows(const float2* __restrict__ seg,
                                   const float2* ph, int nvalid, int nbins, float2* X,
                                   int tid, int nthreads) {
   const int nb_f = nbins / 2 + 1;
+#pragma unroll 4
   for (int r = 0; r < nvalid; ++r) {
     const float2* sr = seg + (int64_t)s_row[r] * 2 * nb_f;
     const float2* fr = folds + (int64_t)s_parent[r] * 2 * nb_f;
