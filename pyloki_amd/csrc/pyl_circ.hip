@@ -278,9 +278,10 @@ circ_children_kernel(LevelConst k, CircConst q, const double* __restrict__ leave
   __shared__ int32_t s_parent[kBlk], s_row[kBlk], s_shift[kBlk];
   const int tid = threadIdx.x;
   const int64_t n_valid = dev_count(n_valid_arg, n_dev);
-  const int64_t v0 = (int64_t)blockIdx.x * kBlk;
+  const int per = per_block(n_valid);
+  const int64_t v0 = (int64_t)blockIdx.x * per;
   if (v0 >= n_valid) return;  // the whole block
-  const int nvalid = (int)std::min<int64_t>(kBlk, n_valid - v0);
+  const int nvalid = (int)std::min<int64_t>(per, n_valid - v0);
   const int B = k.nbins;
   if (tid < nvalid) {
     const int64_t v = v0 + tid;
@@ -366,10 +367,11 @@ circ_materialize_kernel(LevelConst k, CircConst q, const double* __restrict__ le
   extern __shared__ float2 s_ph[];  // complex: [kBlk][B/2 + 1] phasors, then rows, parents
   const int tid = threadIdx.x;
   const int64_t n_surv = dev_count(n_surv_arg, n_dev);
-  const int64_t s0 = (int64_t)blockIdx.x * kBlk;
+  const int per = per_block(n_surv);
+  const int64_t s0 = (int64_t)blockIdx.x * per;
   if (s0 >= n_surv) return;  // the whole block
   int32_t* o_bt = o_bt_base + (bt_row0 ? *bt_row0 : 0) * kRows;
-  const int nvalid = (int)std::min<int64_t>(kBlk, n_surv - s0);
+  const int nvalid = (int)std::min<int64_t>(per, n_surv - s0);
   const int B = k.nbins;
   if (tid < nvalid) {
     const int64_t s = s0 + tid;
@@ -605,7 +607,7 @@ static int circ_children(const pyl_prune_params* p, const pyl_circ_params* q,
   if constexpr (kComplex)
     PYL_REQUIRE((k.nbins & (k.nbins - 1)) == 0,
                 "complex folds need a power-of-two nbins (%d)", k.nbins);
-  circ_children_kernel<kComplex><<<(unsigned)pyl::ceil_div(n_valid, kBlk), kBlk, lds,
+  circ_children_kernel<kComplex><<<(unsigned)items_grid(n_valid), kBlk, lds,
                                    pyl::as_stream(stream)>>>(
       k, cc, leaves, folds, *L, valid, n_valid, nullptr, seg, c_parent, c_row, c_shift, c_phase,
       c_score, minmax);
@@ -652,7 +654,7 @@ static int circ_materialize(const pyl_prune_params* p, const pyl_circ_params* q,
   PYL_REQUIRE(leaves && folds && valid && seg && c_parent && c_row && c_shift && c_phase &&
                   c_score && has_taylor && surv && o_leaves && o_folds && o_scores && o_bt,
               "null device pointer");
-  circ_materialize_kernel<kComplex><<<(unsigned)pyl::ceil_div(n_surv, kBlk), kBlk,
+  circ_materialize_kernel<kComplex><<<(unsigned)items_grid(n_surv), kBlk,
                                       materialize_lds(kComplex, k.nbins),
                                       pyl::as_stream(stream)>>>(
       k, cc, leaves, folds, *L, valid, seg, c_parent, c_row, c_shift, c_phase, c_score,
@@ -915,8 +917,8 @@ int circ_levels(const pyl_prune_params* p, const pyl_circ_params* q,
   const int tpar = 128;
   const unsigned g_par = strided_grid(L->cap_par, tpar);
   const unsigned g_held = strided_grid(L->cap_child, tpar);
-  const unsigned g_child = capacity_grid(L->cap_child, kBlk);
-  const unsigned g_surv = capacity_grid(L->max_sugg, kBlk);
+  const unsigned g_child = items_grid_cap(L->cap_child);
+  const unsigned g_surv = items_grid_cap(L->max_sugg);
   for (int l = 0; l < nlev; ++l) {
     LevelConst k;
     CircConst cc;

@@ -145,13 +145,14 @@ prune_children_kernel(LevelConst k, const double* __restrict__ leaves,
   __shared__ int32_t s_parent[kBlk], s_row[kBlk], s_shift[kBlk];
   const int tid = threadIdx.x;
   const int64_t n_child = dev_count(n_child_arg, n_child_dev);
-  const int64_t c0 = (int64_t)blockIdx.x * kBlk;
+  const int per = per_block(n_child);
+  const int64_t c0 = (int64_t)blockIdx.x * per;
   if (c0 >= n_child) return;  // the whole block
   const int64_t n_par = dev_count(n_par_arg, n_par_dev);
   const int P = k.P;
   const int B = k.nbins;
   const int nb_f = B / 2 + 1;
-  const int nvalid = (int)std::min<int64_t>(kBlk, n_child - c0);
+  const int nvalid = (int)std::min<int64_t>(per, n_child - c0);
   if (tid < nvalid) {
     const int64_t c = c0 + tid;
     const int64_t par = find_parent(off, n_par, c);
@@ -260,10 +261,11 @@ prune_materialize_kernel(LevelConst k, const double* __restrict__ leaves,
   const int B = k.nbins;
   // Backtracks go to rows bt_row0[0].. of o_bt_base when bt_row0 is given (the level
   // loop's history), else to rows 0..
-  const int64_t s0 = (int64_t)blockIdx.x * kBlk;
+  const int per = per_block(n_surv);
+  const int64_t s0 = (int64_t)blockIdx.x * per;
   if (s0 >= n_surv) return;  // the whole block
   int32_t* o_bt = o_bt_base + (bt_row0 ? *bt_row0 : 0) * (P + 2);
-  const int nvalid = (int)std::min<int64_t>(kBlk, n_surv - s0);
+  const int nvalid = (int)std::min<int64_t>(per, n_surv - s0);
   if (tid < nvalid) {
     const int64_t s = s0 + tid;
     const int64_t c = surv[s];
@@ -454,7 +456,7 @@ static int prune_children(const pyl_prune_params* p, const pyl_prune_coords* c,
   if constexpr (kComplex)
     PYL_REQUIRE((k.nbins & (k.nbins - 1)) == 0,
                 "complex folds need a power-of-two nbins (%d)", k.nbins);
-  prune_children_kernel<kComplex, kCheby><<<(unsigned)pyl::ceil_div(n_child, kBlk), kBlk, lds,
+  prune_children_kernel<kComplex, kCheby><<<(unsigned)items_grid(n_child), kBlk, lds,
                                             pyl::as_stream(stream)>>>(
       k, leaves, folds, n_par, nullptr, branch, counts, off, n_child, nullptr, seg, c_parent,
       c_row, c_shift, c_phase, c_score, minmax);
@@ -503,7 +505,7 @@ static int prune_materialize(const pyl_prune_params* p, const pyl_prune_coords* 
                   c_shift && c_phase && c_score && surv && o_leaves && o_folds && o_scores &&
                   o_bt,
               "null device pointer");
-  prune_materialize_kernel<kComplex><<<(unsigned)pyl::ceil_div(n_surv, kBlk), kBlk,
+  prune_materialize_kernel<kComplex><<<(unsigned)items_grid(n_surv), kBlk,
                                        materialize_lds(kComplex, k.nbins),
                                        pyl::as_stream(stream)>>>(
       k, leaves, folds, branch, counts, off, seg, c_parent, c_row, c_shift, c_phase, c_score,
@@ -784,8 +786,8 @@ int ep_levels(const pyl_prune_params* p, const pyl_prune_coords* coords, const d
     const size_t lds = ep_block_lds<kComplex>(k);
     if (l == 0) {  // grids for the capacities: blocks past a level's count exit at once
       g_par = strided_grid(L->cap_par, tpar);
-      g_child = capacity_grid(L->cap_child, kBlk);
-      g_surv = capacity_grid(L->max_sugg, kBlk);
+      g_child = items_grid_cap(L->cap_child);
+      g_surv = items_grid_cap(L->max_sugg);
     }
     PYL_REQUIRE(lds <= kMaxEpLds, "nbins too large for the EP scorer (%d)", k.nbins);
     if constexpr (kComplex)

@@ -232,6 +232,25 @@ __device__ void atomic_max_f32(float* addr, float v) {
   }
 }
 
+// Children (or survivors) per block of the children and materialize kernels: fewer on
+// small levels, so that more blocks share a level whose blocks would otherwise run one
+// long dependent sequence each on a few CUs. A launch for n items needs
+// items_grid(n) blocks; per_block depends only on n, so a capacity-sized launch and an
+// exact one partition a level alike, and a block never sees the other's rows.
+__host__ __device__ inline int per_block(int64_t n) {
+  return n < (1 << 15) ? 16 : (n < (1 << 17) ? 32 : kBlk);
+}
+__host__ __device__ inline int64_t items_grid(int64_t n) {
+  return (n + per_block(n) - 1) / per_block(n);
+}
+// Blocks that cover any count up to cap.
+inline unsigned items_grid_cap(int64_t cap) {
+  const int64_t g = std::max<int64_t>(items_grid(cap), std::max<int64_t>(
+      items_grid(std::min<int64_t>(cap, (1 << 15) - 1)),
+      items_grid(std::min<int64_t>(cap, (1 << 17) - 1))));
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, 0x7fffffff));
+}
+
 struct ChildLeaf {
   double val[kMaxN];  // d_P..d_1, d_0
   double err[kMaxN];
