@@ -1,0 +1,113 @@
+"""An EP search captured once as a HIP graph and replayed for many series.
+
+A survey searches many series of one configuration: the same grids, segments, reference
+segment and thresholds, so every level's constants and launches are the same and only the
+FFA fold differs. ``CapturedSearch`` records the seed scoring and every level of the
+sync-free level loop (``Pruning._enqueue_levels``) once into a ``torch.cuda.CUDAGraph``;
+``run(dyp)`` copies the series' fold into the graph's input buffer, replays the graph
+(about 20 kernels per level, enqueued with one call) and reads the level records once.
+
+A level the loop hands back to the host (survivor overflow, buffers too small, a branch
+error) is finished eagerly, exactly as ``Pruning.execute`` would, and the graph is
+captured again before the next run when the buffers changed. Results equal
+``Pruning(dyp, ...).execute(ref_seg)`` bit for bit. No memset node is recorded: the loop
+clears its records with a kernel.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from pyloki_amd import _lib
+from pyloki_amd.prune import DeviceTree, Pruning, PruneStats
+
+
+class _FoldView:
+    """A DynamicProgramming whose fold is the graph's input buffer."""
+
+    def __init__(self, dyp, fold: torch.Tensor) -> None:
+        self._dyp = dyp
+        self.fold_device = fold
+
+    def __getattr__(self, name):
+        return getattr(self._dyp, name)
+
+
+class CapturedSearch:
+    """``Pruning(dyp, threshold_scheme, **kw).execute(ref_seg)`` for many series of the
+    configuration of ``dyp``, replayed from one captured graph."""
+
+    def __init__(self, dyp, threshold_scheme: np.ndarray, ref_seg: int, level_hook=None,
+                 loop_caps: dict | None = None, **kw) -> None:
+        self.fold = dyp.fold_device.clone()
+        self.pr = Pruning(_FoldView(dyp, self.fold), threshold_scheme, device_loop=True, **kw)
+        self.pr.level_hook = level_hook
+        self.pr.loop_caps = loop_caps
+        self.ref_seg = int(ref_seg)
+        self.graph = None
+        self.captures = 0
+        # An eager run sizes the loop's buffers (and the budget's scratch) for this data.
+        self.pr.execute(self.ref_seg)
+
+    def _buffer_ptrs(self):
+        buf = self.pr._loop
+        return (int(buf.s.hist or 0), int(buf.s.cap_child), int(buf.s.c_score or 0))
+
+    def _capture(self) -> None:
+        pr = self.pr
+        pr.initialize(self.ref_seg)
+        seed = pr.world_tree
+        self.seed = DeviceTree(seed.leaves.clone(), seed.folds.clone(), seed.scores.clone(),
+                               seed.backtracks.clone(), scores_ep=seed.scores)
+        n = seed.size
+        nlast = pr.dyp.nsegments - 1
+        buf = pr._loop_buffers(n, nlast)
+        buf.hist_next = 0
+        pr.timing = False
+        from pyloki_amd import kernels
+
+        rows = self.fold[pr.scheme.ref_idx].reshape(n, 2, -1)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.seed.folds.copy_(rows)
+            if pr._complex:
+                kernels.score_boxcar_complex(self.seed.folds, pr._widths, pr.dyp.cfg.nbins,
+                                             mode=_lib.PYL_SCORE_MAX, out=self.seed.scores)
+            else:
+                kernels.score_boxcar(self.seed.folds, pr._widths, mode=_lib.PYL_SCORE_MAX,
+                                     out=self.seed.scores)
+            self.run_ctx = pr._enqueue_levels(1, nlast, self.seed)
+        torch.cuda.synchronize()
+        self.graph = g
+        self.captures += 1
+        self.ptrs = self._buffer_ptrs()
+
+    def run(self, dyp) -> dict:
+        """The search of ``dyp`` (same configuration): the result dict of
+        ``Pruning.execute``. Device views in it (the world tree) live until the next run."""
+        src = dyp.fold_device
+        if src.shape != self.fold.shape or src.dtype != self.fold.dtype:
+            msg = "the series' fold does not match the captured configuration"
+            raise ValueError(msg)
+        if self.graph is None or self._buffer_ptrs() != self.ptrs:
+            self._capture()
+        pr = self.pr
+        self.fold.copy_(src)
+        self.graph.replay()
+        # The host side of initialize(): the same scheme, the replayed seed.
+        pr._prune_complete = False
+        pr._prune_level = 0
+        pr._backtrack_hist = []
+        pr._world_tree = self.seed
+        pr._loop.hist_next = 0
+        n = self.seed.size
+        pr._pstats = [PruneStats(level=0, seg_idx=pr.scheme.ref_idx, threshold=0,
+                                 score_min=self.seed.score_min,
+                                 score_max=self.seed.score_max, n_branches=n, n_leaves=n,
+                                 n_leaves_phy=n, n_leaves_surv=n)]
+        pr._finish_levels(self.run_ctx)
+        pr.execute_levels()  # levels after a hand-off, if any
+        pr._do_ascend()
+        return pr.report(self.ref_seg)

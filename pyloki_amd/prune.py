@@ -667,76 +667,92 @@ class Pruning:
         """Levels prune_level + 1 .. last through the sync-free loop (one host transfer
         per call, for the level records). A level the loop stops at (branch error, child
         buffer, survivor overflow) runs through execute_iter; a full history is replaced."""
+        while not self._prune_complete and self._prune_level < last:
+            run = self._enqueue_levels(self._prune_level + 1, last, self._world_tree)
+            self._finish_levels(run, log_file)
+
+    def _enqueue_levels(self, l0: int, last: int, tree: DeviceTree) -> dict:
+        """Enqueue levels l0 .. last on the current stream, reading ``tree``; no host
+        synchronisation (a graph may capture it). Returns what ``_finish_levels`` needs."""
         lib = _lib.load()
         fold = self.dyp.fold_device
         stride = int(fold[0].numel()) * (2 if fold.is_complex() else 1)  # floats per segment
         flags = (EP_COMPLEX if self._complex else 0) | (EP_CHEBYSHEV if self._cheby else 0)
-        while not self._prune_complete and self._prune_level < last:
-            l0 = self._prune_level + 1
-            levels = range(l0, last + 1)
-            nlev = len(levels)
-            tree = self._world_tree
-            buf = self._loop_buffers(tree.size, nlev)
-            coords = (PruneCoords * nlev)(*[self._coords(lv) for lv in levels])
-            thr = np.ascontiguousarray(self.threshold_scheme[l0 - 1:last], dtype=np.float64)
-            segs = np.array([self.scheme.get_segment_idx(lv) for lv in levels], dtype=np.int64)
-            budget = None
-            if self.level_hook is not None:  # a survivor budget the loop applies itself
-                if buf.holds(tree) == 0:
-                    tree = DeviceTree(tree.leaves.clone(), tree.folds.clone(),
-                                      tree.scores.clone(), tree.backtracks)
-                budget = self.level_hook.device_budget(self, buf.cap_par, l0)
-            buf.s.budget = ctypes.addressof(budget) if budget is not None else None
-            first_out = 1 if buf.holds(tree) == 0 else 0
-            self._loop_timer(lib, buf, nlev)
-            args = (coords, thr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
-                    segs.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), nlev, ptr(fold),
-                    stride, ptr(tree.leaves), ptr(tree.folds), tree.size, buf.hist_next,
-                    first_out, ctypes.byref(buf.s))
-            if self._circular:
-                _lib.check(self._timed(
-                    "levels", lib.pyl_circ_levels, ctypes.byref(self._pp),
-                    ctypes.byref(self._qq), flags, *args, ctypes.byref(buf.c), stream_ptr()))
-            else:
-                _lib.check(self._timed("levels", lib.pyl_ep_levels, ctypes.byref(self._pp),
-                                       flags, *args, stream_ptr()))
-            self._read_loop_timer(lib)
-            # one transfer: the records, then the min/max pairs' bits as int64 words
-            info = torch.cat([buf.rec[:nlev + 1].reshape(-1),
-                              buf.minmax[:nlev].view(torch.int64).reshape(-1)]).cpu().numpy()
-            rec = info[:(nlev + 1) * REC_WORDS].reshape(nlev + 1, REC_WORDS)
-            mm = info[(nlev + 1) * REC_WORDS:].view(np.float32).reshape(nlev, 2)
-            stop = ST_OK
-            for i, level in enumerate(levels):
-                r = rec[i]
-                stop = int(r[REC_STATUS])
-                if stop != ST_OK:
-                    break
-                o = 1 if budget is not None else (first_out + i) & 1
-                m, h, nv = int(r[REC_NSURV]), int(r[REC_HIST]), int(r[REC_NVALID])
-                if budget is not None:
-                    m = int(r[REC_NKEPT])
-                sc = buf.scores[o][:m]
-                new_tree = DeviceTree(buf.leaves[o][:m], buf.folds[o][:m], sc,
-                                      buf.hist[h:h + m], scores_ep=sc)
-                st = PruneStats(level=level, seg_idx=int(segs[i]), threshold=float(thr[i]),
-                                score_min=float(mm[i, 0]) if nv else 0.0,
-                                score_max=float(mm[i, 1]) if nv else 0.0,
-                                n_branches=int(r[REC_NPAR]), n_leaves=int(r[REC_NCHILD]),
-                                n_leaves_phy=nv, n_leaves_surv=m)
-                buf.hist_next = h + m
-                self._end_level(st, new_tree, log_file, new_tree.backtracks)
-                if self._prune_complete:
-                    break
-            if stop == ST_OK or self._prune_complete:
-                continue
-            self.loop_handoffs[stop] = self.loop_handoffs.get(stop, 0) + 1
-            if stop == ST_HIST_FULL:
-                buf.new_hist(2 * int(buf.s.cap_hist))
-                continue
-            self.execute_iter(log_file)  # raises on a branch error; replays an overflow
-            if stop == ST_CHILD_CAP:
-                buf.set_children(2 * max(buf.cap_child, self._pstats[-1].n_leaves))
+        levels = range(l0, last + 1)
+        nlev = len(levels)
+        buf = self._loop_buffers(tree.size, nlev)
+        coords = (PruneCoords * nlev)(*[self._coords(lv) for lv in levels])
+        thr = np.ascontiguousarray(self.threshold_scheme[l0 - 1:last], dtype=np.float64)
+        segs = np.array([self.scheme.get_segment_idx(lv) for lv in levels], dtype=np.int64)
+        budget = None
+        if self.level_hook is not None:  # a survivor budget the loop applies itself
+            if buf.holds(tree) == 0:
+                tree = DeviceTree(tree.leaves.clone(), tree.folds.clone(), tree.scores.clone(),
+                                  tree.backtracks)
+            budget = self.level_hook.device_budget(self, buf.cap_par, l0)
+        buf.s.budget = ctypes.addressof(budget) if budget is not None else None
+        first_out = 1 if buf.holds(tree) == 0 else 0
+        self._loop_timer(lib, buf, nlev)
+        args = (coords, thr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                segs.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), nlev, ptr(fold), stride,
+                ptr(tree.leaves), ptr(tree.folds), tree.size, buf.hist_next, first_out,
+                ctypes.byref(buf.s))
+        if self._circular:
+            _lib.check(self._timed(
+                "levels", lib.pyl_circ_levels, ctypes.byref(self._pp), ctypes.byref(self._qq),
+                flags, *args, ctypes.byref(buf.c), stream_ptr()))
+        else:
+            _lib.check(self._timed("levels", lib.pyl_ep_levels, ctypes.byref(self._pp), flags,
+                                   *args, stream_ptr()))
+        return {"levels": levels, "buf": buf, "thr": thr, "segs": segs, "budget": budget,
+                "first_out": first_out}
+
+    def _finish_levels(self, run: dict, log_file: Path | None = None) -> int:
+        """Read the records of an enqueued run (one transfer) and advance the search over
+        its finished levels; a level the loop stopped at is run through execute_iter (or
+        the history is replaced). Returns the stop status."""
+        lib = _lib.load()
+        levels, buf, thr, segs = run["levels"], run["buf"], run["thr"], run["segs"]
+        budget, first_out = run["budget"], run["first_out"]
+        nlev = len(levels)
+        self._read_loop_timer(lib)
+        # one transfer: the records, then the min/max pairs' bits as int64 words
+        info = torch.cat([buf.rec[:nlev + 1].reshape(-1),
+                          buf.minmax[:nlev].view(torch.int64).reshape(-1)]).cpu().numpy()
+        rec = info[:(nlev + 1) * REC_WORDS].reshape(nlev + 1, REC_WORDS)
+        mm = info[(nlev + 1) * REC_WORDS:].view(np.float32).reshape(nlev, 2)
+        stop = ST_OK
+        for i, level in enumerate(levels):
+            r = rec[i]
+            stop = int(r[REC_STATUS])
+            if stop != ST_OK:
+                break
+            o = 1 if budget is not None else (first_out + i) & 1
+            m, h, nv = int(r[REC_NSURV]), int(r[REC_HIST]), int(r[REC_NVALID])
+            if budget is not None:
+                m = int(r[REC_NKEPT])
+            sc = buf.scores[o][:m]
+            new_tree = DeviceTree(buf.leaves[o][:m], buf.folds[o][:m], sc, buf.hist[h:h + m],
+                                  scores_ep=sc)
+            st = PruneStats(level=level, seg_idx=int(segs[i]), threshold=float(thr[i]),
+                            score_min=float(mm[i, 0]) if nv else 0.0,
+                            score_max=float(mm[i, 1]) if nv else 0.0,
+                            n_branches=int(r[REC_NPAR]), n_leaves=int(r[REC_NCHILD]),
+                            n_leaves_phy=nv, n_leaves_surv=m)
+            buf.hist_next = h + m
+            self._end_level(st, new_tree, log_file, new_tree.backtracks)
+            if self._prune_complete:
+                break
+        if stop == ST_OK or self._prune_complete:
+            return stop
+        self.loop_handoffs[stop] = self.loop_handoffs.get(stop, 0) + 1
+        if stop == ST_HIST_FULL:
+            buf.new_hist(2 * int(buf.s.cap_hist))
+            return stop
+        self.execute_iter(log_file)  # raises on a branch error; replays an overflow
+        if stop == ST_CHILD_CAP:
+            buf.set_children(2 * max(buf.cap_child, self._pstats[-1].n_leaves))
+        return stop
 
     def _fn(self, lib, name: str):
         """The real or complex-fold variant of an EP entry point."""
@@ -961,6 +977,11 @@ class Pruning:
                     self._do_ascend()
         if asc is None or self._prune_level not in asc:
             self._do_ascend()
+        return self.report(ref_seg, result_file)
+
+    def report(self, ref_seg: int, result_file: Path | None = None) -> dict:
+        """The result of a finished search: report transform of the survivors, their
+        scores before and after the ascend, and the level statistics (``prune.py:592-613``)."""
         leaves = self._world_tree.leaves.cpu().numpy()
         if not len(leaves):
             report = leaves
