@@ -89,17 +89,27 @@ feat_stats_part_kernel(const float* __restrict__ scores, int64_t n_arg,
 __global__ void __launch_bounds__(kT)
 feat_stats_final_kernel(double* __restrict__ stats, int64_t n_arg,
                         const int64_t* __restrict__ n_dev, uint32_t* __restrict__ shist) {
+  static_assert(kScanBlocks == kT, "one chunk per thread");
+  __shared__ double s_su[kT], s_sq[kT], s_mn[kT], s_mx[kT];
   const int64_t n = count_of(n_arg, n_dev);
-  for (int j = threadIdx.x; j < 2 * kBins; j += kT) shist[j] = 0;
-  if (threadIdx.x != 0) return;
-  double su = 0.0, sq = 0.0, mn = DBL_MAX, mx = -DBL_MAX;
-  for (int b = 0; b < kScanBlocks; ++b) {
-    const double* o = stats + 4 * b;
-    su += o[0];
-    sq += o[1];
-    mn = fmin(mn, o[2]);
-    mx = fmax(mx, o[3]);
+  const int t = threadIdx.x;
+  for (int j = t; j < 2 * kBins; j += kT) shist[j] = 0;
+  s_su[t] = stats[4 * t];
+  s_sq[t] = stats[4 * t + 1];
+  s_mn[t] = stats[4 * t + 2];
+  s_mx[t] = stats[4 * t + 3];
+  __syncthreads();
+  for (int o = kT / 2; o > 0; o >>= 1) {  // a fixed tree: the same sums every run
+    if (t < o) {
+      s_su[t] += s_su[t + o];
+      s_sq[t] += s_sq[t + o];
+      s_mn[t] = fmin(s_mn[t], s_mn[t + o]);
+      s_mx[t] = fmax(s_mx[t], s_mx[t + o]);
+    }
+    __syncthreads();
   }
+  if (t != 0) return;
+  const double su = s_su[0], sq = s_sq[0], mn = s_mn[0], mx = s_mx[0];
   const double mean = n > 0 ? su / (double)n : 0.0;
   const double var = n > 1 ? (sq - su * mean) / (double)(n - 1) : 0.0;
   double* f = stats + 4 * kScanBlocks;
@@ -120,10 +130,17 @@ __global__ void __launch_bounds__(kT)
 feat_hist_kernel(const float* __restrict__ scores, int64_t n_arg,
                  const int64_t* __restrict__ n_dev, const double* __restrict__ stats,
                  uint32_t* __restrict__ shist) {
+  __shared__ uint32_t h[kBins];
   const int64_t n = count_of(n_arg, n_dev);
+  if ((int64_t)blockIdx.x * kT >= n) return;  // the whole block
   const double* f = stats + 4 * kScanBlocks;
+  for (int j = threadIdx.x; j < kBins; j += kT) h[j] = 0;
+  __syncthreads();
   for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT)
-    atomicAdd(&shist[score_bin(scores[i], f)], 1u);
+    atomicAdd(&h[score_bin(scores[i], f)], 1u);
+  __syncthreads();
+  for (int j = threadIdx.x; j < kBins; j += kT)
+    if (h[j]) atomicAdd(&shist[j], h[j]);
 }
 
 // One block: exclusive prefix of the histogram into shist[kBins ..].
@@ -173,7 +190,7 @@ feat_write_kernel(const double* __restrict__ leaves, const float* __restrict__ s
 }
 
 inline unsigned stride_grid(int64_t cap) {
-  return (unsigned)std::max<int64_t>(1, std::min<int64_t>((cap + kT - 1) / kT, 2048));
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>((cap + kT - 1) / kT, 512));
 }
 
 int features_enqueue(const double* leaves, const float* scores, int64_t n_cap,

@@ -584,8 +584,10 @@ template <int B>
 int launch_forward_mfma(const float* th, const float* folds, int64_t n, const int64_t* n_dev,
                         const float* feats, float* logits, int complex_folds, hipStream_t s) {
   constexpr int kW = kThreads / 64;
+  // At most two blocks per CU: a block stages the weights once, then its wavefronts
+  // stride over the samples.
   const unsigned grid = (unsigned)std::max<int64_t>(
-      1, std::min<int64_t>(pyl::ceil_div(n, kW), 2048));
+      1, std::min<int64_t>(pyl::ceil_div(n, kW), 512));
   if (complex_folds)
     cnn_forward_mfma_kernel<B, true><<<grid, kThreads, 0, s>>>(th, folds, n, n_dev, feats, logits);
   else
