@@ -60,6 +60,9 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--nseries", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tile-rows", type=int, default=None,
+                    help="outer rows per 256-row tile of the fused FFA kernels; default: "
+                         "chosen by the autotuner (pyloki_amd/autotune.py) on this device")
     ap.add_argument("--cpu-sample-series", type=int, default=2)
     ap.add_argument("--workload", choices=["ffa", "ep-taylor-jerk", "ep-circular-cnn"],
                     default="ffa",
@@ -422,6 +425,16 @@ def main() -> int:
     for n in lens:
         nlev = int(np.log2(n // cfg_max.bseg_brute))
         trials += plan.levels[nlev - 1].ncoords if nlev > 0 else plan.bf.nfreq
+    from pyloki_amd import autotune
+
+    if args.tile_rows is not None:
+        autotune.set_tile_rows(args.tile_rows)
+        tuned = {"rows": args.tile_rows, "ms": None, "cached": False, "source": "--tile-rows"}
+    else:  # time every candidate tile on this workload, keep the fastest
+        sig = (f"ffa_batch/B{nbins}/nw{nw}/lens{sorted(set(lens))}x{len(lens)}/"
+               f"grid{[int(c) for c in cfg_max.get_param_grid_count(cfg_max.niters_ffa)]}")
+        tuned = autotune.tune_tile_rows(sig, lambda: ffa_batch(plan, dev_series, widths))
+        tuned["source"] = "autotune"
     for _ in range(args.warmup):
         ffa_batch(plan, dev_series, widths)
     torch.cuda.synchronize()
@@ -484,6 +497,9 @@ def main() -> int:
             "trials_per_step": trials,
             "final_grid_2^23": [int(c) for c in cfg_max.get_param_grid_count(cfg_max.niters_ffa)],
             "parallelism": "1 GPU; series concatenated along the segment axis",
+            "tile_outer_rows": tuned["rows"],
+            "tile_source": tuned["source"],
+            "autotune_ms": tuned["ms"],
         },
         "roofline": {
             "kernel": dominant,

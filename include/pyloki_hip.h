@@ -163,6 +163,14 @@ int pyl_ffa_shift_add_score_layout(const float* fold_in_dev, int32_t in_layout,
  * formed in the same order and rounded to FP32 at the same points); the intermediate fold
  * is never written. nbins 32 or 64, at most 16 widths. No reference counterpart as an
  * interface: two unify_fold calls (dyn_pffa.py:435-489) and the final scoring. */
+/* Launch shape of the round-based fused scoring kernels (pyl_ffa_shift_add_score_layout,
+ * pyl_ffa_shift_add2_score_layout; nbins 32/64): a block owns 256 output rows, `rows`
+ * along the outer grid axis by 256 / rows along the inner one. 0 restores the default
+ * (16). Process-wide; set by the autotuner (pyloki_amd/autotune.py). No reference
+ * counterpart: it changes no result, only which rows share a block's L2 reuse. */
+int pyl_ffa_set_tile_rows(int32_t rows);
+int pyl_ffa_get_tile_rows(void);
+
 int pyl_ffa_shift_add2_score_layout(const float* fold_in_dev, int32_t in_layout,
                                     int64_t nseg_in, int64_t ncoords_prev2,
                                     const int32_t* table_prev_dev, int64_t ncoords_prev,

@@ -18,6 +18,7 @@
 //   Rows of consecutive j share their source rows (the previous level's grid is about
 //   half as fine), so each block takes a contiguous run of j and the runs are dealt to
 //   the 8 XCDs in contiguous slabs: neighbours' rows are reused from the same L2.
+#include <atomic>
 #include <algorithm>
 
 #include "pyl_score.hpp"
@@ -584,13 +585,24 @@ extern "C" int pyl_ffa_shift_add(const float* fold_in, int64_t nseg_in, int64_t 
                                   ncoords_cur, nbins, fold_out, PYL_FOLD_DENSE, stream);
 }
 
+// Outer rows per 256-row tile of the round-based fused kernels: 0 = the default (16 x 16
+// tiles), else the value the autotuner measured best (pyl_ffa_set_tile_rows). Fewer outer
+// rows on short outer axes. The rows a tile holds never change a result.
+static std::atomic<int> g_tile_outer{0};
+
+static int tile_outer_rows(int64_t n_outer) {
+  int to = g_tile_outer.load(std::memory_order_relaxed);
+  if (to <= 0) to = 16;
+  while (to > 1 && n_outer < to) to /= 2;
+  return to;
+}
+
 template <int B, typename T, int WC, bool kWrapIn>
 static void launch_rounds(const T* in, int64_t nprev, const int4* tb, int64_t ncur,
                           int64_t n_inner, int64_t nseg_out, const pyl::ScoreConst& k,
                           float* out, hipStream_t s) {
   const int64_t n_outer = ncur / n_inner;
-  int to = 16;  // outer rows per tile: 16 x 16 tiles, fewer outer rows on short outer axes
-  while (to > 1 && n_outer < to) to /= 2;
+  const int to = tile_outer_rows(n_outer);
   const int64_t tiles_inner = pyl::ceil_div(n_inner, 256 / to);
   const int64_t tiles_outer = pyl::ceil_div(n_outer, to);
   const int64_t per_pair = tiles_inner * tiles_outer;
@@ -605,8 +617,7 @@ static void launch_rounds2(const float* in, int64_t nprev2, const int4* tb2, int
                            const int4* tb, int64_t ncur, int64_t n_inner, int64_t nseg_out,
                            const pyl::ScoreConst& k, float* out, hipStream_t s) {
   const int64_t n_outer = ncur / n_inner;
-  int to = 16;
-  while (to > 1 && n_outer < to) to /= 2;
+  const int to = tile_outer_rows(n_outer);
   const int64_t tiles_inner = pyl::ceil_div(n_inner, 256 / to);
   const int64_t tiles_outer = pyl::ceil_div(n_outer, to);
   const int64_t per_pair = tiles_inner * tiles_outer;
@@ -796,3 +807,12 @@ extern "C" int pyl_ffa_level(int32_t nparams, const double* params_dev, const in
   return pyl_ffa_shift_add(fold_in_dev, nseg_in, nprev, table_dev, ncur, nbins, fold_out_dev,
                            stream);
 }
+
+extern "C" int pyl_ffa_set_tile_rows(int32_t rows) {
+  PYL_REQUIRE(rows == 0 || (rows >= 1 && rows <= 256 && (rows & (rows - 1)) == 0),
+              "tile rows must be 0 or a power of two in [1, 256] (got %d)", rows);
+  g_tile_outer.store(rows, std::memory_order_relaxed);
+  return PYL_OK;
+}
+
+extern "C" int pyl_ffa_get_tile_rows(void) { return g_tile_outer.load(std::memory_order_relaxed); }
