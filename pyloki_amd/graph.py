@@ -87,15 +87,25 @@ class CapturedSearch:
     def run(self, dyp) -> dict:
         """The search of ``dyp`` (same configuration): the result dict of
         ``Pruning.execute``. Device views in it (the world tree) live until the next run."""
+        self.launch(dyp)
+        return self.finish()
+
+    def launch(self, dyp) -> None:
+        """Enqueue the search of ``dyp`` on the current stream (no host synchronisation
+        once the graph exists)."""
         src = dyp.fold_device
         if src.shape != self.fold.shape or src.dtype != self.fold.dtype:
             msg = "the series' fold does not match the captured configuration"
             raise ValueError(msg)
         if self.graph is None or self._buffer_ptrs() != self.ptrs:
             self._capture()
-        pr = self.pr
+        torch.cuda.current_stream().wait_stream(torch.cuda.default_stream())
         self.fold.copy_(src)
         self.graph.replay()
+
+    def finish(self) -> dict:
+        """Read the launched search's records and complete it (on the current stream)."""
+        pr = self.pr
         # The host side of initialize(): the same scheme, the replayed seed.
         pr._prune_complete = False
         pr._prune_level = 0
@@ -111,3 +121,39 @@ class CapturedSearch:
         pr.execute_levels()  # levels after a hand-off, if any
         pr._do_ascend()
         return pr.report(self.ref_seg)
+
+
+class SearchBatch:
+    """Many series of one configuration, ``width`` searches at a time on their own
+    streams: each stream replays its own captured search (its own buffers), so the small
+    dependent launches of one search's levels overlap those of the others and the GPU's
+    CUs stay busy on levels that a single search cannot fill. Results equal
+    ``Pruning.execute`` of each series."""
+
+    def __init__(self, dyp, threshold_scheme: np.ndarray, ref_seg: int, width: int = 8,
+                 hook_factory=None, loop_caps: dict | None = None, **kw) -> None:
+        self.streams = [torch.cuda.Stream() for _ in range(width)]
+        self.searches = []
+        for st in self.streams:
+            with torch.cuda.stream(st):
+                hook = hook_factory() if hook_factory is not None else None
+                self.searches.append(CapturedSearch(dyp, threshold_scheme, ref_seg,
+                                                    level_hook=hook, loop_caps=loop_caps,
+                                                    **kw))
+        torch.cuda.synchronize()
+
+    def run(self, dyps: list) -> list[dict]:
+        out = []
+        w = len(self.searches)
+        for b0 in range(0, len(dyps), w):
+            chunk = dyps[b0:b0 + w]
+            for cs, st, d in zip(self.searches, self.streams, chunk):
+                with torch.cuda.stream(st):
+                    cs.launch(d)
+            for cs, st, _ in zip(self.searches, self.streams, chunk):
+                with torch.cuda.stream(st):
+                    out.append(cs.finish())
+        torch.cuda.default_stream().wait_stream(torch.cuda.current_stream())
+        for st in self.streams:
+            torch.cuda.default_stream().wait_stream(st)
+        return out

@@ -81,3 +81,20 @@ def test_graph_circular_replays(gpu_device, kind):
     cs = _replays(dyps, thresholds, dyps[0].nsegments // 2, factory, max_sugg=2**16,
                   batch_size=256)
     print(f"[result] captures {cs.captures}, hand-offs {cs.pr.loop_handoffs}", flush=True)
+
+
+def test_search_batch_equals_eager(gpu_device):
+    from pyloki_amd.cnn import SurvivorBudget
+    from pyloki_amd.graph import SearchBatch
+
+    _case("search batch (streams) vs eager")
+    dyps = [_setup(2, 2**17, 8, seed=s)[1] for s in (4, 5, 6, 7, 8)]
+    thresholds = np.linspace(2.5, 5.5, 7)
+    for factory in (None, lambda: SurvivorBudget(24)):
+        sb = SearchBatch(dyps[0], thresholds, 3, width=3, hook_factory=factory,
+                         max_sugg=2**20, batch_size=64)
+        got = sb.run(dyps)
+        for i, d in enumerate(dyps):
+            pr = Pruning(d, thresholds, max_sugg=2**20, batch_size=64)
+            pr.level_hook = factory() if factory else None
+            _same(got[i], pr.execute(3), f"series {i}")
