@@ -142,7 +142,8 @@ def cpu_baseline(series, nsample: int) -> dict:
 EP_CNN = {"nsamps": 2**24, "tsamp": 64e-6, "nseg": 128, "nbins": 64, "freq": (249.0, 251.0),
           "p_orb": 6 * 3600.0, "x_orb": 1.5, "m_c_max": 0.7, "m_p_min": 1.4, "snr": 15.0,
           "thresholds": (0.5, 8.0), "max_sugg": 2**21, "budgets": (2**16, 2**14),
-          "train_seeds": (1, 2, 3, 4), "test_seed": 9, "epochs": 16}
+          "train_seeds": (1, 2, 3, 4), "test_seed": 9, "epochs": 16,
+          "batch_seeds": tuple(range(9, 17))}
 
 
 def ep_circular_dyp(seed: int, w: dict = EP_CNN):
@@ -195,9 +196,18 @@ def ep_circular_cnn() -> int:
                    "train_seeds": list(w["train_seeds"]), "test_seed": w["test_seed"]},
         "trial_ratio": r["trial_ratio"], "speedup": r["speedup"],
         "exhaustive": ex, "exhaustive_trials_per_s": ex["trials"] / ex["time_s"],
-        "runs": {k: v for k, v in res.items() if k not in ("exhaustive", "train")},
+        "runs": {k: v for k, v in res.items() if k not in ("exhaustive", "train", "ranker")},
         "train": res["train"],
     }
+    # A survey: more series of the configuration, 4 searches at a time on their own streams
+    # (each a captured graph), exhaustive vs the same budgets.
+    series = [ep_circular_dyp(s) for s in w["batch_seeds"]]
+    ranker = res["ranker"]
+    hooks = {"exhaustive": None,
+             f"cnn_{b}": lambda: cnn.SurvivorBudget(b, ranker),
+             f"snr_{b}": lambda: cnn.SurvivorBudget(b)}
+    line["batched"] = cnn.batched_throughput(series, thr, w["max_sugg"], hooks, width=4)
+    line["batched"]["series"] = len(series)
     print(json.dumps(line), flush=True)
     return 0
 

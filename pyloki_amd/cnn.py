@@ -384,7 +384,47 @@ def budget_experiment(make_dyp, train_seeds, test_seed: int, thresholds: np.ndar
         hooks[f"snr_{b}"] = score_budget_hook(b)
     res = compare_searches(lambda: Pruning(dyp, thresholds, max_sugg=max_sugg,
                                            batch_size=batch_size), dyp.nsegments // 2, hooks)
+    res["ranker"] = ranker
     res["train"] = {"samples": int(folds.shape[0]), "positives": int(labels.sum().item()),
                     "epochs": epochs, "loss_first": hist[0], "loss_last": hist[-1],
                     "data_s": t_data, "train_s": t_train}
     return res
+
+
+def batched_throughput(dyps: list, thresholds: np.ndarray, max_sugg: int, hooks: dict,
+                       width: int = 4, batch_size: int = 1024) -> dict:
+    """Searches per second of each named hook factory (None: exhaustive) over the series
+    ``dyps`` of one configuration, ``width`` searches at a time on their own streams
+    (``graph.SearchBatch``), and the recall of each run's candidates against the
+    exhaustive ones, series by series."""
+    from pyloki_amd.graph import SearchBatch
+
+    ref = dyps[0].nsegments // 2
+    out, finals = {}, {}
+    for name, factory in hooks.items():
+        sb = SearchBatch(dyps[0], thresholds, ref, width=width, hook_factory=factory,
+                         max_sugg=max_sugg, batch_size=batch_size)
+        sb.run(dyps[:width])  # warm-up (captures the graphs)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = sb.run(dyps)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        finals[name] = [{r.tobytes() for r in np.ascontiguousarray(x["param_sets"])}
+                        for x in res]
+        out[name] = {"ms_per_search": dt / len(dyps) * 1e3,
+                     "trials": int(sum(sum(st.n_leaves for st in x["level_stats"][1:])
+                                       for x in res))}
+        del sb
+        torch.cuda.empty_cache()
+    ex = finals.get("exhaustive")
+    for name in out:
+        if ex is None:
+            continue
+        hit = sum(len(a & b) for a, b in zip(ex, finals[name]))
+        tot = sum(len(a) for a in ex)
+        out[name]["recall"] = hit / tot if tot else 1.0
+        if name != "exhaustive":
+            out[name]["speedup"] = out["exhaustive"]["ms_per_search"] / out[name]["ms_per_search"]
+            out[name]["trial_ratio"] = out["exhaustive"]["trials"] / max(out[name]["trials"], 1)
+    return out
