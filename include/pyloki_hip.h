@@ -463,6 +463,7 @@ int pyl_ep_timer_read(pyl_ep_timer* t, int32_t* phase_out, float* ms_out, int32_
  * exactly. Scratch arrays hold `cap` rows (the tree buffers' rows). */
 #define PYL_BUDGET_SCORE 1
 #define PYL_BUDGET_CNN 2
+#define PYL_BUDGET_HYBRID 3 /* the larger of the rank fractions by logit and by S/N */
 #define PYL_BUDGET_FP32 0
 #define PYL_BUDGET_BF16 1
 #define PYL_BUDGET_SEL_WORDS 272

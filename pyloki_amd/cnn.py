@@ -24,7 +24,7 @@ from pyloki_amd import _lib
 from pyloki_amd.device import ptr, stream_ptr
 
 NFEAT = 8
-_BUDGET_SCORE, _BUDGET_CNN = 1, 2
+_BUDGET_SCORE, _BUDGET_CNN, _BUDGET_HYBRID = 1, 2, 3
 _BUDGET_SEL_WORDS, _BUDGET_HIST_BINS, _BUDGET_STATS = 272, 1024, 1032
 _EP_SCAN_BLOCKS = 256
 _REC_WORDS, _REC_NKEPT = 12, 10
@@ -233,7 +233,7 @@ class SurvivorBudget:
     level loop applies it on the device (``device_budget``), the per-level path calls it."""
 
     def __init__(self, budget: int, ranker: BranchRanker | None = None,
-                 precision: str = "bf16") -> None:
+                 precision: str = "bf16", hybrid: bool = False) -> None:
         if budget < 1:
             msg = "budget must be >= 1"
             raise ValueError(msg)
@@ -243,6 +243,12 @@ class SurvivorBudget:
         self.budget = int(budget)
         self.ranker = ranker
         self.precision = precision  # of the network's forward pass (bf16: MFMA)
+        # hybrid: rank by the larger of the rank fractions by logit and by S/N (keeps what
+        # either ranking puts near the top)
+        if hybrid and ranker is None:
+            msg = "a hybrid budget needs the ranking network"
+            raise ValueError(msg)
+        self.hybrid = bool(hybrid)
         self._cap = -1
         self._s = None
 
@@ -263,7 +269,8 @@ class SurvivorBudget:
                  "shist": torch.empty(2 * _BUDGET_HIST_BINS, dtype=torch.int32, device=dev),
                  "part": torch.empty(_EP_SCAN_BLOCKS, dtype=torch.int64, device=dev)}
             s = EpBudget()
-            s.kind = _BUDGET_CNN if self.ranker is not None else _BUDGET_SCORE
+            s.kind = (_BUDGET_HYBRID if self.hybrid else _BUDGET_CNN) if self.ranker is not None \
+                else _BUDGET_SCORE
             s.nbins = cfg.nbins
             s.complex_folds = int(bool(cfg.use_fourier))
             s.budget = self.budget

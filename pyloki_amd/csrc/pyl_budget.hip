@@ -189,6 +189,27 @@ feat_write_kernel(const double* __restrict__ leaves, const float* __restrict__ s
   }
 }
 
+// Hybrid key: the larger of the row's rank fraction by the network's logit (from the
+// histogram just built over the logits) and by S/N (feature 3), so that the cut keeps the
+// rows that either ranking puts near the top.
+__global__ void __launch_bounds__(kT)
+hybrid_key_kernel(float* __restrict__ keys, const float* __restrict__ feats, int64_t n_arg,
+                  const int64_t* __restrict__ n_dev, const double* __restrict__ stats,
+                  const uint32_t* __restrict__ shist) {
+  const int64_t n = count_of(n_arg, n_dev);
+  const double* f = stats + 4 * kScanBlocks;
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kT) {
+    float r = 0.0f;
+    if (n > 1) {
+      const int b = score_bin(keys[i], f);
+      r = (float)(((double)shist[kBins + b] + 0.5 * ((double)shist[b] - 1.0)) /
+                  (double)(n - 1));
+    }
+    keys[i] = fmaxf(r, feats[i * kFeat + 3]);
+  }
+}
+
 inline unsigned stride_grid(int64_t cap) {
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>((cap + kT - 1) / kT, 512));
 }
@@ -336,17 +357,18 @@ int pyl::ep_budget_enqueue(const pyl_ep_budget* b, int P, int level, const doubl
                            int64_t n_cap, int64_t* rec, double* o_leaves, float* o_folds,
                            float* o_scores, int32_t* o_hist, hipStream_t s) {
   PYL_REQUIRE(b != nullptr, "null budget");
-  PYL_REQUIRE(b->kind == PYL_BUDGET_SCORE || b->kind == PYL_BUDGET_CNN, "unknown budget kind %d",
-              b->kind);
+  PYL_REQUIRE(b->kind == PYL_BUDGET_SCORE || b->kind == PYL_BUDGET_CNN ||
+                  b->kind == PYL_BUDGET_HYBRID,
+              "unknown budget kind %d", b->kind);
   PYL_REQUIRE(b->budget >= 1, "budget must be >= 1");
   PYL_REQUIRE(n_cap >= 0 && n_cap < 0x7fffffffLL, "too many survivors for the budget");
-  PYL_REQUIRE(b->kept && b->sel && b->part && (b->kind != PYL_BUDGET_CNN ||
-                                               (b->params && b->keys && b->feats &&
-                                                b->stats && b->shist)),
+  const bool net = b->kind != PYL_BUDGET_SCORE;
+  PYL_REQUIRE(b->kept && b->sel && b->part &&
+                  (!net || (b->params && b->keys && b->feats && b->stats && b->shist)),
               "null budget buffer");
   const int64_t* m_dev = rec + kRecNsurv;
   const float* keys = scores;
-  if (b->kind == PYL_BUDGET_CNN) {
+  if (net) {
     int rc = features_enqueue(leaves, scores, n_cap, m_dev, P, level, b->nlevels, b->stats,
                               b->shist, b->feats, s);
     if (rc != PYL_OK) return rc;
@@ -354,6 +376,16 @@ int pyl::ep_budget_enqueue(const pyl_ep_budget* b, int P, int level, const doubl
                                 b->feats, b->keys, b->precision == PYL_BUDGET_BF16, s);
     if (rc != PYL_OK) return rc;
     keys = b->keys;
+    if (b->kind == PYL_BUDGET_HYBRID) {  // the logits' histogram, then the hybrid key
+      feat_stats_part_kernel<<<kScanBlocks, kT, 0, s>>>(b->keys, n_cap, m_dev, b->stats);
+      feat_stats_final_kernel<<<1, kT, 0, s>>>(b->stats, n_cap, m_dev, b->shist);
+      feat_hist_kernel<<<stride_grid(n_cap), kT, 0, s>>>(b->keys, n_cap, m_dev, b->stats,
+                                                         b->shist);
+      feat_cdf_kernel<<<1, kT, 0, s>>>(b->shist);
+      hybrid_key_kernel<<<stride_grid(n_cap), kT, 0, s>>>(b->keys, b->feats, n_cap, m_dev,
+                                                          b->stats, b->shist);
+      PYL_CHECK_LAUNCH("hybrid_key_kernel");
+    }
   }
   sel_init_kernel<<<1, 256, 0, s>>>(m_dev, b->budget, b->sel);
   for (int shift = 24; shift >= 0; shift -= 8) {

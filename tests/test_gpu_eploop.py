@@ -212,6 +212,7 @@ def test_cnn_features_definition(gpu_device):
 
 
 @pytest.mark.parametrize("kind,circular,budget", [("snr", False, 64), ("cnn", False, 64),
+                                                  ("hybrid", False, 64), ("hybrid", True, 512),
                                                   ("snr", True, 512), ("cnn", True, 512)])
 def test_loop_budget_equals_per_level_path(gpu_device, kind, circular, budget):
     from pyloki_amd.cnn import BranchRanker, SurvivorBudget
@@ -225,11 +226,11 @@ def test_loop_budget_equals_per_level_path(gpu_device, kind, circular, budget):
         cfg, dyp, pcfg = _setup(2, 2**18, 16)
         ref = 7
         thresholds = np.linspace(2.0, 5.0, dyp.nsegments - 1)
-    ranker = BranchRanker(cfg.nbins, seed=3) if kind == "cnn" else None
+    ranker = BranchRanker(cfg.nbins, seed=3) if kind != "snr" else None
     runs = []
     for dl in (False, True):
         pr = Pruning(dyp, thresholds, max_sugg=2**16, batch_size=256, device_loop=dl)
-        pr.level_hook = SurvivorBudget(budget, ranker)
+        pr.level_hook = SurvivorBudget(budget, ranker, hybrid=kind == "hybrid")
         runs.append((pr, pr.execute(ref)))
     _same(runs[1], runs[0])
     st = runs[1][0].pstats
