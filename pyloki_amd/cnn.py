@@ -388,6 +388,7 @@ def budget_experiment(make_dyp, train_seeds, test_seed: int, thresholds: np.ndar
     hooks = {}
     for b in budgets:
         hooks[f"cnn_{b}"] = budget_hook(ranker, b)
+        hooks[f"hybrid_{b}"] = SurvivorBudget(b, ranker, hybrid=True)
         hooks[f"snr_{b}"] = score_budget_hook(b)
     res = compare_searches(lambda: Pruning(dyp, thresholds, max_sugg=max_sugg,
                                            batch_size=batch_size), dyp.nsegments // 2, hooks)
