@@ -217,6 +217,15 @@ def ep_circular_cnn() -> int:
     return 0
 
 
+# EP search of a jerk pulsar (Taylor order 3): N = 2^23 samples at 64 us, 64 FFA segments
+# (63 EP levels), bseg_brute 1024, nbins 64, eta 1, f in [495, 505] Hz, accel +-30 m/s^2,
+# jerk +-2 m/s^3; injected P = 2 ms pulsar (a = 5 m/s^2, j = 0.05 m/s^3, so that its
+# acceleration stays inside the grid over the whole observation; 5 % duty, S/N 12); thresholds
+# bound_scheme(63, 5); max_sugg 2^22, batches of 1024 parents; reference segment 32.
+EP_JERK = {"nsamps": 2**23, "tsamp": 64e-6, "nseg": 64, "nbins": 64, "snr": 12.0,
+           "snr_bound": 5.0, "max_sugg": 2**22, "ref_seg": 32, "seed": 7}
+
+
 def ep_jerk_dyp(w: dict = EP_JERK):
     from pyloki_amd.config import ParamLimits, PulsarSearchConfig
     from pyloki_amd.ffa import DynamicProgramming
