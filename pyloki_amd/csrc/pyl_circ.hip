@@ -265,7 +265,7 @@ __device__ __forceinline__ void circ_expand_one(const LevelConst& k, const CircC
 }
 
 template <bool kComplex>
-__global__ void __launch_bounds__(kBlk)
+__global__ void __launch_bounds__(kChildThreads<kComplex>)
 circ_children_kernel(LevelConst k, CircConst q, const double* __restrict__ leaves,
                      const float* __restrict__ folds, pyl_circ_level L,
                      const int64_t* __restrict__ valid, int64_t n_valid_arg,
@@ -323,9 +323,9 @@ circ_children_kernel(LevelConst k, CircConst q, const double* __restrict__ leave
     cfold_rows(reinterpret_cast<const float2*>(seg), reinterpret_cast<const float2*>(folds),
                s_row, s_parent,
                reinterpret_cast<const float2*>(lds + complex_ph_offset(B, k.sc.wmax)), nvalid,
-               B, reinterpret_cast<float2*>(lds), tid, kBlk);
+               B, reinterpret_cast<float2*>(lds), tid, kCBlk);
     __syncthreads();
-    Sc = complex_block_columns(k, lds, nvalid, tid);
+    Sc = complex_block_columns(k, lds, nvalid, tid, kCBlk);
   } else {
 #pragma unroll 4
     for (int r = 0; r < nvalid; ++r) {
@@ -343,7 +343,10 @@ circ_children_kernel(LevelConst k, CircConst q, const double* __restrict__ leave
     __syncthreads();
   }
   float sc = 0.0f;
-  if (tid < nvalid) {
+  if constexpr (kComplex) {
+    sc = complex_score_columns(k, Sc, nvalid, tid, kCBlk);
+    if (tid < nvalid) c_score[v0 + tid] = sc;
+  } else if (tid < nvalid) {
     sc = score_column(k, Sc, tid);
     c_score[v0 + tid] = sc;
   }
@@ -498,7 +501,7 @@ circ_ascend_kernel(LevelConst k, CircConst q, const double* __restrict__ leaves,
   __syncthreads();
   float* Sc = S;
   if constexpr (kComplex) {
-    Sc = complex_block_columns(k, reinterpret_cast<char*>(S), nvalid, tid);
+    Sc = complex_block_columns(k, reinterpret_cast<char*>(S), nvalid, tid, kBlk);
   } else {
     // common.py:432-459: accumulate the rotated rows in segment order, from zero.
     for (int r = 0; r < nvalid; ++r) {
@@ -607,8 +610,8 @@ static int circ_children(const pyl_prune_params* p, const pyl_circ_params* q,
   if constexpr (kComplex)
     PYL_REQUIRE((k.nbins & (k.nbins - 1)) == 0,
                 "complex folds need a power-of-two nbins (%d)", k.nbins);
-  circ_children_kernel<kComplex><<<(unsigned)items_grid(n_valid), kBlk, lds,
-                                   pyl::as_stream(stream)>>>(
+  circ_children_kernel<kComplex><<<(unsigned)items_grid(n_valid), kChildThreads<kComplex>,
+                                   lds, pyl::as_stream(stream)>>>(
       k, cc, leaves, folds, *L, valid, n_valid, nullptr, seg, c_parent, c_row, c_shift, c_phase,
       c_score, minmax);
   PYL_CHECK_LAUNCH("circ_children_kernel");
@@ -962,7 +965,7 @@ int circ_levels(const pyl_prune_params* p, const pyl_circ_params* q,
     PYL_CHECK_LAUNCH("circ_expand_kernel");
     if (tm) tm->end(s);
     if (tm) tm->begin(PYL_EP_PH_CHILDREN, s);
-    circ_children_kernel<kComplex><<<g_child, kBlk, lds, s>>>(
+    circ_children_kernel<kComplex><<<g_child, kChildThreads<kComplex>, lds, s>>>(
         k, cc, leaves, folds, cl, C->valid, 0, rec + kRecNvalid, seg, L->c_parent, L->c_row,
         L->c_shift, L->c_phase, L->c_score, L->minmax + 2 * l);
     PYL_CHECK_LAUNCH("circ_children_kernel");

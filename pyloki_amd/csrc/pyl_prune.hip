@@ -130,7 +130,7 @@ __global__ void cheb_parents_kernel(LevelConst k, const double* __restrict__ lea
 // Folds are real [2][B] rows or, with kComplex, complex64 [2][B/2 + 1] rows (as float2).
 // kCheby: the leaves are Chebyshev coefficients on the current domain (chebyshev.py:180-258).
 template <bool kComplex, bool kCheby>
-__global__ void __launch_bounds__(kBlk)
+__global__ void __launch_bounds__(kChildThreads<kComplex>)
 prune_children_kernel(LevelConst k, const double* __restrict__ leaves,
                       const float* __restrict__ folds, int64_t n_par_arg,
                       const int64_t* __restrict__ n_par_dev,
@@ -209,9 +209,9 @@ prune_children_kernel(LevelConst k, const double* __restrict__ leaves,
     cfold_rows(reinterpret_cast<const float2*>(seg), reinterpret_cast<const float2*>(folds),
                s_row, s_parent,
                reinterpret_cast<const float2*>(lds + complex_ph_offset(B, k.sc.wmax)), nvalid,
-               B, reinterpret_cast<float2*>(lds), tid, kBlk);
+               B, reinterpret_cast<float2*>(lds), tid, kCBlk);
     __syncthreads();
-    Sc = complex_block_columns(k, lds, nvalid, tid);
+    Sc = complex_block_columns(k, lds, nvalid, tid, kCBlk);
   } else {
     // Fold (common.py:389-399: parent row + segment row rotated by the shift) and
     // normalise, lane = bin, transposed into S.
@@ -231,7 +231,10 @@ prune_children_kernel(LevelConst k, const double* __restrict__ leaves,
     __syncthreads();
   }
   float sc = 0.0f;
-  if (tid < nvalid) {
+  if constexpr (kComplex) {
+    sc = complex_score_columns(k, Sc, nvalid, tid, kCBlk);
+    if (tid < nvalid) c_score[c0 + tid] = sc;
+  } else if (tid < nvalid) {
     sc = score_column(k, Sc, tid);
     c_score[c0 + tid] = sc;
   }
@@ -389,7 +392,7 @@ prune_ascend_kernel(LevelConst k, const double* __restrict__ leaves, int64_t n,
   __syncthreads();
   float* Sc = S;
   if constexpr (kComplex) {
-    Sc = complex_block_columns(k, reinterpret_cast<char*>(S), nvalid, tid);
+    Sc = complex_block_columns(k, reinterpret_cast<char*>(S), nvalid, tid, kBlk);
   } else {
     // common.py:432-459: accumulate the rotated rows in segment order, from zero.
     for (int r = 0; r < nvalid; ++r) {
@@ -456,7 +459,8 @@ static int prune_children(const pyl_prune_params* p, const pyl_prune_coords* c,
   if constexpr (kComplex)
     PYL_REQUIRE((k.nbins & (k.nbins - 1)) == 0,
                 "complex folds need a power-of-two nbins (%d)", k.nbins);
-  prune_children_kernel<kComplex, kCheby><<<(unsigned)items_grid(n_child), kBlk, lds,
+  prune_children_kernel<kComplex, kCheby><<<(unsigned)items_grid(n_child),
+                                            kChildThreads<kComplex>, lds,
                                             pyl::as_stream(stream)>>>(
       k, leaves, folds, n_par, nullptr, branch, counts, off, n_child, nullptr, seg, c_parent,
       c_row, c_shift, c_phase, c_score, minmax);
@@ -818,7 +822,7 @@ int ep_levels(const pyl_prune_params* p, const pyl_prune_coords* coords, const d
     PYL_CHECK_LAUNCH("chunk_scan_kernel");
     if (tm) tm->end(s);
     if (tm) tm->begin(PYL_EP_PH_CHILDREN, s);
-    prune_children_kernel<kComplex, kCheby><<<g_child, kBlk, lds, s>>>(
+    prune_children_kernel<kComplex, kCheby><<<g_child, kChildThreads<kComplex>, lds, s>>>(
         k, src, folds, 0, rec + kRecNpar, L->branch, L->counts, L->off, 0, rec + kRecNchild,
         seg, L->c_parent, L->c_row, L->c_shift, L->c_phase, L->c_score, L->minmax + 2 * l);
     PYL_CHECK_LAUNCH("prune_children_kernel");

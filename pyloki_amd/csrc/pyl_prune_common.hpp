@@ -397,7 +397,7 @@ __device__ inline void cfold_rows(const float2* __restrict__ seg,
                                   const float2* ph, int nvalid, int nbins, float2* X,
                                   int tid, int nthreads) {
   const int nb_f = nbins / 2 + 1;
-#pragma unroll 4
+#pragma unroll 8
   for (int r = 0; r < nvalid; ++r) {
     const float2* sr = seg + (int64_t)s_row[r] * 2 * nb_f;
     const float2* fr = folds + (int64_t)s_parent[r] * 2 * nb_f;
@@ -447,24 +447,36 @@ __host__ __device__ inline size_t complex_block_lds(int nbins, int wmax) {
   return base > ph_end ? base : ph_end;
 }
 
+// Threads of a complex children block: four wavefronts share the block's (at most kBlk)
+// children in the fold, the inverse transforms and the widths of the scorer, so that a
+// small level's few blocks still occupy every SIMD of their CUs. Real blocks keep one
+// wavefront (lane = child column).
+constexpr int kCBlk = 4 * kBlk;
+template <bool kComplex>
+constexpr int kChildThreads = kComplex ? kCBlk : kBlk;
+
 // irfft + e/sqrt(v) of the block's rows (lane = bin) into the scorer's columns; the caller
-// has filled Xs and synchronised. Returns after a barrier; thread tid < nvalid then scores
-// column tid.
+// has filled Xs and synchronised. The rows go to the block's wavefronts four children at a
+// time (eight independent accumulations per lane, each one pyl::irfft_bin's, in its order),
+// the last nvalid % 4 rows one at a time. Returns after a barrier; the columns are then
+// scored (complex_score_columns, or score_column by thread tid < nvalid).
 __device__ inline float* complex_block_columns(const LevelConst& k, char* lds, int nvalid,
-                                               int tid) {
+                                               int tid, int nthreads) {
   const int B = k.nbins;
   const int nb_f = B / 2 + 1;
   const float2* Xs = reinterpret_cast<const float2*>(lds);
   float* S = reinterpret_cast<float*>(lds + sizeof(float2) * (size_t)kBlk * 2 * nb_f);
   float* tw = S + (B + k.sc.wmax) * kStride;
-  pyl::fill_twiddles(tw, B, tid, kBlk);
+  pyl::fill_twiddles(tw, B, tid, nthreads);
   __syncthreads();
-  // Four children at a time: eight independent accumulations per lane (each one
-  // pyl::irfft_bin's, in its order) instead of one long dependent chain per row.
-  int r = 0;
-  for (; r + 4 <= nvalid; r += 4) {
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int nwaves = nthreads >> 6;
+  const int nfull = nvalid / 4;
+  for (int g = wave; g < nfull; g += nwaves) {
+    const int r = 4 * g;
     const float2* X = Xs + (size_t)r * 2 * nb_f;
-    for (int b = tid; b < B; b += kBlk) {
+    for (int b = lane; b < B; b += 64) {
       float acc[8];
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
@@ -473,6 +485,7 @@ __device__ inline float* complex_block_columns(const LevelConst& k, char* lds, i
         acc[q] = row[0].x;
         acc[q] += (b & 1) ? -ny : ny;
       }
+#pragma unroll 4
       for (int kk = 1; kk < nb_f - 1; ++kk) {
         const int idx = (kk * b) & (B - 1);
         const float c = tw[idx], sn = tw[B + idx];
@@ -488,9 +501,9 @@ __device__ inline float* complex_block_columns(const LevelConst& k, char* lds, i
         S[b * kStride + r + q] = pyl::norm_ev(acc[2 * q] * inv, acc[2 * q + 1] * inv);
     }
   }
-  for (; r < nvalid; ++r) {
+  for (int r = 4 * nfull + wave; r < nvalid; r += nwaves) {
     const float2* X = Xs + (size_t)r * 2 * nb_f;
-    for (int b = tid; b < B; b += kBlk) {
+    for (int b = lane; b < B; b += 64) {
       const float e = pyl::irfft_bin(X, B, b, tw);
       const float v = pyl::irfft_bin(X + nb_f, B, b, tw);
       S[b * kStride + r] = pyl::norm_ev(e, v);
@@ -498,6 +511,55 @@ __device__ inline float* complex_block_columns(const LevelConst& k, char* lds, i
   }
   __syncthreads();
   return S;
+}
+
+// score_column of the block's columns (nvalid <= kBlk) with the widths spread over the
+// wavefronts: the columns' owners (lanes of wavefront 0) scan, every wavefront takes every
+// nwaves-th width, and the owner keeps the largest S/N. A maximum does not depend on the
+// order of its terms, so thread tid < nvalid gets score_column's value; every thread of
+// the block calls this.
+__device__ inline float complex_score_columns(const LevelConst& k, float* S, int nvalid,
+                                              int tid, int nthreads) {
+  __shared__ double s_best[kCBlk];
+  const int B = k.nbins;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int nwaves = nthreads >> 6;
+  if (tid < nvalid) {
+    float run = S[tid];
+    for (int i = 1; i < B; ++i) {
+      run = run + S[i * kStride + tid];
+      S[i * kStride + tid] = run;
+    }
+    for (int j = 0; j < k.sc.wmax; ++j)
+      S[(B + j) * kStride + tid] = S[j * kStride + tid] + run;
+  }
+  __syncthreads();
+  double best = -(double)FLT_MAX;
+  if (lane < nvalid) {
+    const float total = S[(B - 1) * kStride + lane];
+    for (int w = wave; w < k.sc.nw; w += nwaves) {
+      const int wd = k.sc.widths[w];
+      float dmax = -FLT_MAX;
+      for (int i = 0; i < B; ++i) {
+        const float d = S[(i + wd) * kStride + lane] - S[i * kStride + lane];
+        dmax = d > dmax ? d : dmax;
+      }
+      const double t1 = k.sc.hpb[w] * (double)dmax;
+      const double t2 = k.sc.bb[w] * (double)total;
+      const double snr = t1 - t2;
+      best = snr > best ? snr : best;
+    }
+  }
+  s_best[tid] = best;
+  __syncthreads();
+  if (tid < nvalid) {
+    for (int v = 1; v < nwaves; ++v) {
+      const double o = s_best[v * 64 + tid];
+      best = o > best ? o : best;
+    }
+  }
+  return (float)best;
 }
 
 // Dynamic LDS of the complex materialize blocks: phasors [kBlk][B/2 + 1], rows and parents.
