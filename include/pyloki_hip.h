@@ -432,6 +432,7 @@ int pyl_cheb_ascend_complex(const pyl_prune_params* p, const double* leaves_dev,
 #define PYL_EP_REC_NBATCH 8 /* circular: batches of parents */
 #define PYL_EP_REC_NEXPAND 9 /* circular: held children to expand (0 if the level stops) */
 #define PYL_EP_REC_NKEPT 10 /* survivors kept by the budget (= the next level's parents) */
+#define PYL_EP_REC_NRANK 11 /* survivors the budget ranks: NSURV above the budget, else 0 */
 #define PYL_EP_REC_WORDS 12
 
 #define PYL_EP_ST_OK 0

@@ -229,14 +229,17 @@ int features_enqueue(const double* leaves, const float* scores, int64_t n_cap,
 
 // ---- The cut -------------------------------------------------------------------------
 
-__global__ void sel_init_kernel(const int64_t* __restrict__ m_dev, int64_t budget,
+// The select state, and the count of rows to rank: none when all of them fit the budget
+// (the features, the network and the hybrid key then skip the level).
+__global__ void sel_init_kernel(int64_t* __restrict__ rec, int64_t budget,
                                 uint32_t* __restrict__ sel) {
-  const int64_t m = *m_dev;
+  const int64_t m = rec[kRecNsurv];
   if (threadIdx.x == 0) {
     sel[kSelPrefix] = 0;
     sel[kSelMask] = 0;
     sel[kSelRemain] = (uint32_t)std::min<int64_t>(budget, m);
     sel[kSelAll] = m <= budget ? 1u : 0u;
+    rec[PYL_EP_REC_NRANK] = m <= budget ? 0 : m;
   }
   sel[kSelHist + threadIdx.x] = 0;
 }
@@ -367,27 +370,28 @@ int pyl::ep_budget_enqueue(const pyl_ep_budget* b, int P, int level, const doubl
                   (!net || (b->params && b->keys && b->feats && b->stats && b->shist)),
               "null budget buffer");
   const int64_t* m_dev = rec + kRecNsurv;
+  const int64_t* r_dev = rec + PYL_EP_REC_NRANK;  // 0 when every survivor is kept
+  sel_init_kernel<<<1, 256, 0, s>>>(rec, b->budget, b->sel);
   const float* keys = scores;
   if (net) {
-    int rc = features_enqueue(leaves, scores, n_cap, m_dev, P, level, b->nlevels, b->stats,
+    int rc = features_enqueue(leaves, scores, n_cap, r_dev, P, level, b->nlevels, b->stats,
                               b->shist, b->feats, s);
     if (rc != PYL_OK) return rc;
-    rc = pyl::cnn_forward_count(b->params, folds, n_cap, m_dev, b->nbins, b->complex_folds,
+    rc = pyl::cnn_forward_count(b->params, folds, n_cap, r_dev, b->nbins, b->complex_folds,
                                 b->feats, b->keys, b->precision == PYL_BUDGET_BF16, s);
     if (rc != PYL_OK) return rc;
     keys = b->keys;
     if (b->kind == PYL_BUDGET_HYBRID) {  // the logits' histogram, then the hybrid key
-      feat_stats_part_kernel<<<kScanBlocks, kT, 0, s>>>(b->keys, n_cap, m_dev, b->stats);
-      feat_stats_final_kernel<<<1, kT, 0, s>>>(b->stats, n_cap, m_dev, b->shist);
-      feat_hist_kernel<<<stride_grid(n_cap), kT, 0, s>>>(b->keys, n_cap, m_dev, b->stats,
+      feat_stats_part_kernel<<<kScanBlocks, kT, 0, s>>>(b->keys, n_cap, r_dev, b->stats);
+      feat_stats_final_kernel<<<1, kT, 0, s>>>(b->stats, n_cap, r_dev, b->shist);
+      feat_hist_kernel<<<stride_grid(n_cap), kT, 0, s>>>(b->keys, n_cap, r_dev, b->stats,
                                                          b->shist);
       feat_cdf_kernel<<<1, kT, 0, s>>>(b->shist);
-      hybrid_key_kernel<<<stride_grid(n_cap), kT, 0, s>>>(b->keys, b->feats, n_cap, m_dev,
+      hybrid_key_kernel<<<stride_grid(n_cap), kT, 0, s>>>(b->keys, b->feats, n_cap, r_dev,
                                                           b->stats, b->shist);
       PYL_CHECK_LAUNCH("hybrid_key_kernel");
     }
   }
-  sel_init_kernel<<<1, 256, 0, s>>>(m_dev, b->budget, b->sel);
   for (int shift = 24; shift >= 0; shift -= 8) {
     sel_hist_kernel<<<stride_grid(n_cap), kT, 0, s>>>(keys, m_dev, b->sel, shift);
     sel_pick_kernel<<<1, kT, 0, s>>>(b->sel, shift);
