@@ -11,9 +11,15 @@
 // the host.
 //
 // The network's features need the level's S/N mean, standard deviation and rank: the
-// statistics are reduced in a fixed order (per-chunk partials, then one thread over the
+// statistics are reduced in a fixed order (per-chunk partials, then a fixed tree over the
 // chunks), the rank comes from a 1024-bin histogram between the level's smallest and
 // largest S/N (integer counts), so the features do not depend on scheduling.
+//
+// A level's ranking is a chain of small dependent launches, so each launch does what the
+// next step needs from the previous one's complete output: the histogram blocks reduce the
+// statistics' partials themselves, the feature (key) blocks build the histogram's prefix
+// in LDS, and the last block of each digit histogram (found with a device-scope counter)
+// picks the digit.
 #include <cfloat>
 
 #include "pyl_budget.hpp"
@@ -25,7 +31,8 @@ using namespace pyl_ep;
 
 constexpr int kT = 256;
 constexpr int kBins = PYL_BUDGET_HIST_BINS;
-constexpr int kSelPrefix = 0, kSelMask = 1, kSelRemain = 2, kSelAll = 3, kSelHist = 16;
+constexpr int kSelPrefix = 0, kSelMask = 1, kSelRemain = 2, kSelAll = 3, kSelDone = 4,
+              kSelHist = 16;
 constexpr int kFeat = 8;
 static_assert(kSelHist + 256 == PYL_BUDGET_SEL_WORDS, "select state layout");
 static_assert(4 * kScanBlocks + 8 == PYL_BUDGET_STATS, "statistics layout");
@@ -45,10 +52,13 @@ __device__ __forceinline__ uint32_t key_bits(float x) {
 // Per chunk of the rows (kScanBlocks chunks): sum, sum of squares (FP64), min, max.
 __global__ void __launch_bounds__(kT)
 feat_stats_part_kernel(const float* __restrict__ scores, int64_t n_arg,
-                       const int64_t* __restrict__ n_dev, double* __restrict__ stats) {
+                       const int64_t* __restrict__ n_dev, double* __restrict__ stats,
+                       uint32_t* __restrict__ shist) {
   __shared__ double s_sum[kT], s_sq[kT];
   __shared__ float s_mn[kT], s_mx[kT];
   const int64_t n = count_of(n_arg, n_dev);
+  if (blockIdx.x == 0)  // the histogram of the next launch starts from zero
+    for (int j = threadIdx.x; j < 2 * kBins; j += kT) shist[j] = 0;
   int64_t b0, b1;
   chunk_bounds(n, &b0, &b1);
   double su = 0.0, sq = 0.0;
@@ -84,22 +94,18 @@ feat_stats_part_kernel(const float* __restrict__ scores, int64_t n_arg,
   }
 }
 
-// One block: the chunks in order -> mean, unbiased standard deviation, min, max; clears
-// the score histogram.
-__global__ void __launch_bounds__(kT)
-feat_stats_final_kernel(double* __restrict__ stats, int64_t n_arg,
-                        const int64_t* __restrict__ n_dev, uint32_t* __restrict__ shist) {
+// The chunks' partials in a fixed tree -> mean, unbiased standard deviation, min, max in
+// f[0..3] (every thread of the block; the same values in every block and every run).
+__device__ inline void stats_final(const double* __restrict__ stats, int64_t n, double* f) {
   static_assert(kScanBlocks == kT, "one chunk per thread");
   __shared__ double s_su[kT], s_sq[kT], s_mn[kT], s_mx[kT];
-  const int64_t n = count_of(n_arg, n_dev);
   const int t = threadIdx.x;
-  for (int j = t; j < 2 * kBins; j += kT) shist[j] = 0;
   s_su[t] = stats[4 * t];
   s_sq[t] = stats[4 * t + 1];
   s_mn[t] = stats[4 * t + 2];
   s_mx[t] = stats[4 * t + 3];
   __syncthreads();
-  for (int o = kT / 2; o > 0; o >>= 1) {  // a fixed tree: the same sums every run
+  for (int o = kT / 2; o > 0; o >>= 1) {
     if (t < o) {
       s_su[t] += s_su[t + o];
       s_sq[t] += s_sq[t + o];
@@ -108,15 +114,13 @@ feat_stats_final_kernel(double* __restrict__ stats, int64_t n_arg,
     }
     __syncthreads();
   }
-  if (t != 0) return;
-  const double su = s_su[0], sq = s_sq[0], mn = s_mn[0], mx = s_mx[0];
+  const double su = s_su[0], sq = s_sq[0];
   const double mean = n > 0 ? su / (double)n : 0.0;
   const double var = n > 1 ? (sq - su * mean) / (double)(n - 1) : 0.0;
-  double* f = stats + 4 * kScanBlocks;
   f[0] = mean;
   f[1] = sqrt(fmax(var, 0.0));
-  f[2] = mn;
-  f[3] = mx;
+  f[2] = s_mn[0];
+  f[3] = s_mx[0];
 }
 
 __device__ __forceinline__ int score_bin(float x, const double* f) {
@@ -128,12 +132,14 @@ __device__ __forceinline__ int score_bin(float x, const double* f) {
 
 __global__ void __launch_bounds__(kT)
 feat_hist_kernel(const float* __restrict__ scores, int64_t n_arg,
-                 const int64_t* __restrict__ n_dev, const double* __restrict__ stats,
+                 const int64_t* __restrict__ n_dev, double* __restrict__ stats,
                  uint32_t* __restrict__ shist) {
   __shared__ uint32_t h[kBins];
   const int64_t n = count_of(n_arg, n_dev);
-  if ((int64_t)blockIdx.x * kT >= n) return;  // the whole block
-  const double* f = stats + 4 * kScanBlocks;
+  if (blockIdx.x > 0 && (int64_t)blockIdx.x * kT >= n) return;  // the whole block
+  double f[4];
+  stats_final(stats, n, f);
+  if (blockIdx.x == 0 && threadIdx.x < 4) stats[4 * kScanBlocks + threadIdx.x] = f[threadIdx.x];
   for (int j = threadIdx.x; j < kBins; j += kT) h[j] = 0;
   __syncthreads();
   for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT)
@@ -143,21 +149,25 @@ feat_hist_kernel(const float* __restrict__ scores, int64_t n_arg,
     if (h[j]) atomicAdd(&shist[j], h[j]);
 }
 
-// One block: exclusive prefix of the histogram into shist[kBins ..].
-__global__ void __launch_bounds__(kT) feat_cdf_kernel(uint32_t* __restrict__ shist) {
-  __shared__ uint32_t s_part[kT];
+// The histogram and its exclusive prefix into LDS (every block; integer counts, so the
+// prefix is exact whatever the order).
+__device__ inline void hist_cdf(const uint32_t* __restrict__ shist, uint32_t* cnt,
+                                uint32_t* cdf) {
   constexpr int kPer = kBins / kT;
   const int t = threadIdx.x;
   uint32_t acc = 0;
-  for (int j = 0; j < kPer; ++j) acc += shist[t * kPer + j];
-  s_part[t] = acc;
-  __syncthreads();
-  uint32_t pre = 0;
-  for (int j = 0; j < t; ++j) pre += s_part[j];
   for (int j = 0; j < kPer; ++j) {
-    shist[kBins + t * kPer + j] = pre;
-    pre += shist[t * kPer + j];
+    const uint32_t c = shist[t * kPer + j];
+    cnt[t * kPer + j] = c;
+    acc += c;
   }
+  int64_t tot;
+  uint32_t pre = (uint32_t)(block_scan_i64((int64_t)acc, &tot) - (int64_t)acc);
+  for (int j = 0; j < kPer; ++j) {
+    cdf[t * kPer + j] = pre;
+    pre += cnt[t * kPer + j];
+  }
+  __syncthreads();
 }
 
 __global__ void __launch_bounds__(kT)
@@ -165,7 +175,10 @@ feat_write_kernel(const double* __restrict__ leaves, const float* __restrict__ s
                   int64_t n_arg, const int64_t* __restrict__ n_dev, int P, int level,
                   int nlevels, const double* __restrict__ stats,
                   const uint32_t* __restrict__ shist, float* __restrict__ feats) {
+  __shared__ uint32_t cnt[kBins], cdf[kBins];
   const int64_t n = count_of(n_arg, n_dev);
+  if ((int64_t)blockIdx.x * kT >= n) return;  // the whole block
+  hist_cdf(shist, cnt, cdf);
   const double* f = stats + 4 * kScanBlocks;
   const int kw = P < kFeat - 4 ? P : kFeat - 4;
   for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n;
@@ -177,8 +190,7 @@ feat_write_kernel(const double* __restrict__ leaves, const float* __restrict__ s
     if (n > 1) {
       o[2] = (float)(((double)x - f[0]) / (f[1] + 1e-6));
       const int b = score_bin(x, f);
-      o[3] = (float)(((double)shist[kBins + b] + 0.5 * ((double)shist[b] - 1.0)) /
-                     (double)(n - 1));
+      o[3] = (float)(((double)cdf[b] + 0.5 * ((double)cnt[b] - 1.0)) / (double)(n - 1));
     } else {
       o[2] = 0.0f;
       o[3] = 0.0f;
@@ -196,15 +208,17 @@ __global__ void __launch_bounds__(kT)
 hybrid_key_kernel(float* __restrict__ keys, const float* __restrict__ feats, int64_t n_arg,
                   const int64_t* __restrict__ n_dev, const double* __restrict__ stats,
                   const uint32_t* __restrict__ shist) {
+  __shared__ uint32_t cnt[kBins], cdf[kBins];
   const int64_t n = count_of(n_arg, n_dev);
+  if ((int64_t)blockIdx.x * kT >= n) return;  // the whole block
+  hist_cdf(shist, cnt, cdf);
   const double* f = stats + 4 * kScanBlocks;
   for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * kT) {
     float r = 0.0f;
     if (n > 1) {
       const int b = score_bin(keys[i], f);
-      r = (float)(((double)shist[kBins + b] + 0.5 * ((double)shist[b] - 1.0)) /
-                  (double)(n - 1));
+      r = (float)(((double)cdf[b] + 0.5 * ((double)cnt[b] - 1.0)) / (double)(n - 1));
     }
     keys[i] = fmaxf(r, feats[i * kFeat + 3]);
   }
@@ -217,10 +231,8 @@ inline unsigned stride_grid(int64_t cap) {
 int features_enqueue(const double* leaves, const float* scores, int64_t n_cap,
                      const int64_t* n_dev, int P, int level, int nlevels, double* stats,
                      uint32_t* shist, float* feats, hipStream_t s) {
-  feat_stats_part_kernel<<<kScanBlocks, kT, 0, s>>>(scores, n_cap, n_dev, stats);
-  feat_stats_final_kernel<<<1, kT, 0, s>>>(stats, n_cap, n_dev, shist);
+  feat_stats_part_kernel<<<kScanBlocks, kT, 0, s>>>(scores, n_cap, n_dev, stats, shist);
   feat_hist_kernel<<<stride_grid(n_cap), kT, 0, s>>>(scores, n_cap, n_dev, stats, shist);
-  feat_cdf_kernel<<<1, kT, 0, s>>>(shist);
   feat_write_kernel<<<stride_grid(n_cap), kT, 0, s>>>(leaves, scores, n_cap, n_dev, P, level,
                                                       nlevels, stats, shist, feats);
   PYL_CHECK_LAUNCH("feat_write_kernel");
@@ -239,41 +251,45 @@ __global__ void sel_init_kernel(int64_t* __restrict__ rec, int64_t budget,
     sel[kSelMask] = 0;
     sel[kSelRemain] = (uint32_t)std::min<int64_t>(budget, m);
     sel[kSelAll] = m <= budget ? 1u : 0u;
+    sel[kSelDone] = 0;
     rec[PYL_EP_REC_NRANK] = m <= budget ? 0 : m;
   }
   sel[kSelHist + threadIdx.x] = 0;
 }
 
+// One digit pass: the histogram of the digit at `shift` over the candidates that share the
+// prefix found so far; the last block to finish (a device-scope counter, after a release
+// fence in every block) then picks the digit where the count of candidates from the top
+// reaches the remaining budget, and clears the histogram and the counter for the next pass.
 __global__ void __launch_bounds__(kT)
 sel_hist_kernel(const float* __restrict__ keys, const int64_t* __restrict__ m_dev,
                 uint32_t* __restrict__ sel, int shift) {
   __shared__ uint32_t h[256];
-  h[threadIdx.x] = 0;
+  __shared__ uint32_t suf[257];
+  __shared__ int s_last;
+  const int t = threadIdx.x;
+  if (sel[kSelAll] != 0) return;  // every block: all rows are kept
+  h[t] = 0;
   __syncthreads();
-  if (sel[kSelAll] == 0) {
-    const int64_t m = *m_dev;
-    const uint32_t prefix = sel[kSelPrefix], mask = sel[kSelMask];
-    for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < m;
-         i += (int64_t)gridDim.x * kT) {
-      const uint32_t u = key_bits(keys[i]);
-      if ((u & mask) == prefix) atomicAdd(&h[(u >> shift) & 255u], 1u);
-    }
+  const int64_t m = *m_dev;
+  const uint32_t prefix = sel[kSelPrefix], mask = sel[kSelMask];
+  for (int64_t i = (int64_t)blockIdx.x * kT + t; i < m; i += (int64_t)gridDim.x * kT) {
+    const uint32_t u = key_bits(keys[i]);
+    if ((u & mask) == prefix) atomicAdd(&h[(u >> shift) & 255u], 1u);
   }
   __syncthreads();
-  if (h[threadIdx.x]) atomicAdd(&sel[kSelHist + threadIdx.x], h[threadIdx.x]);
-}
-
-// One block: the digit where the count of candidates from the top reaches the remaining
-// budget; clears the histogram for the next pass.
-__global__ void __launch_bounds__(kT) sel_pick_kernel(uint32_t* __restrict__ sel, int shift) {
-  __shared__ uint32_t suf[257];
-  const int t = threadIdx.x;
-  if (sel[kSelAll] != 0) return;  // the whole block
+  if (h[t]) atomicAdd(&sel[kSelHist + t], h[t]);
+  __threadfence();
+  __syncthreads();
+  if (t == 0) s_last = atomicAdd(&sel[kSelDone], 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
   if (t == 0) {
     uint32_t acc = 0;
     suf[256] = 0;
     for (int d = 255; d >= 0; --d) {
-      acc += sel[kSelHist + d];
+      acc += __hip_atomic_load(&sel[kSelHist + d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       suf[d] = acc;
     }
   }
@@ -281,11 +297,12 @@ __global__ void __launch_bounds__(kT) sel_pick_kernel(uint32_t* __restrict__ sel
   const uint32_t rem = sel[kSelRemain];
   __syncthreads();  // every thread has read the state before it changes
   if (suf[t + 1] < rem && rem <= suf[t]) {
-    sel[kSelPrefix] |= (uint32_t)t << shift;
-    sel[kSelMask] |= 0xffu << shift;
+    sel[kSelPrefix] = prefix | ((uint32_t)t << shift);
+    sel[kSelMask] = mask | (0xffu << shift);
     sel[kSelRemain] = rem - suf[t + 1];
   }
   sel[kSelHist + t] = 0;
+  if (t == 0) sel[kSelDone] = 0;
 }
 
 struct KeepSrc {  // packed: 1 = above the cut, 1 << 32 = at the cut
@@ -382,11 +399,10 @@ int pyl::ep_budget_enqueue(const pyl_ep_budget* b, int P, int level, const doubl
     if (rc != PYL_OK) return rc;
     keys = b->keys;
     if (b->kind == PYL_BUDGET_HYBRID) {  // the logits' histogram, then the hybrid key
-      feat_stats_part_kernel<<<kScanBlocks, kT, 0, s>>>(b->keys, n_cap, r_dev, b->stats);
-      feat_stats_final_kernel<<<1, kT, 0, s>>>(b->stats, n_cap, r_dev, b->shist);
+      feat_stats_part_kernel<<<kScanBlocks, kT, 0, s>>>(b->keys, n_cap, r_dev, b->stats,
+                                                         b->shist);
       feat_hist_kernel<<<stride_grid(n_cap), kT, 0, s>>>(b->keys, n_cap, r_dev, b->stats,
                                                          b->shist);
-      feat_cdf_kernel<<<1, kT, 0, s>>>(b->shist);
       hybrid_key_kernel<<<stride_grid(n_cap), kT, 0, s>>>(b->keys, b->feats, n_cap, r_dev,
                                                           b->stats, b->shist);
       PYL_CHECK_LAUNCH("hybrid_key_kernel");
@@ -394,7 +410,6 @@ int pyl::ep_budget_enqueue(const pyl_ep_budget* b, int P, int level, const doubl
   }
   for (int shift = 24; shift >= 0; shift -= 8) {
     sel_hist_kernel<<<stride_grid(n_cap), kT, 0, s>>>(keys, m_dev, b->sel, shift);
-    sel_pick_kernel<<<1, kT, 0, s>>>(b->sel, shift);
   }
   chunked_scan(KeepSrc{keys, b->sel}, KeepSink{b->kept, b->sel, rec}, m_dev, b->part, s);
   const int row_floats = b->complex_folds ? 4 * (b->nbins / 2 + 1) : 2 * b->nbins;
