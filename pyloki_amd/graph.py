@@ -49,6 +49,20 @@ class CapturedSearch:
         self.captures = 0
         # An eager run sizes the loop's buffers (and the budget's scratch) for this data.
         self.pr.execute(self.ref_seg)
+        if loop_caps is None or "cap_child" not in loop_caps:
+            self._fit_children()
+
+    def _fit_children(self) -> None:
+        """Per-child arrays for twice the eager run's largest level (at least 2^16): the
+        children and materialize launches cover the capacity, and on small levels the
+        blocks beyond the level's children only cost dispatches. A series with a larger
+        level hands that level to the host, which grows the arrays; the graph is then
+        captured again."""
+        need = max([int(st.n_leaves) for st in self.pr._pstats[1:]] + [1])
+        cap = max(2**16, 1 << (2 * need - 1).bit_length())
+        buf = self.pr._loop
+        if buf is not None and cap < buf.cap_child:
+            buf.set_children(cap)
 
     def _buffer_ptrs(self):
         buf = self.pr._loop
