@@ -273,7 +273,7 @@ circ_children_kernel(LevelConst k, CircConst q, const double* __restrict__ leave
                      const float* __restrict__ seg, int32_t* __restrict__ c_parent,
                      int32_t* __restrict__ c_row, int32_t* __restrict__ c_shift,
                      double* __restrict__ c_phase, float* __restrict__ c_score,
-                     float* __restrict__ minmax) {
+                     float* __restrict__ minmax, uint8_t* __restrict__ has_taylor) {
   extern __shared__ float S[];  // [(B + wmax)][kStride]
   __shared__ int32_t s_parent[kBlk], s_row[kBlk], s_shift[kBlk];
   const int tid = threadIdx.x;
@@ -291,6 +291,8 @@ circ_children_kernel(LevelConst k, CircConst q, const double* __restrict__ leave
     ChildLeaf ch;
     circ_decode(k, leaves, L, par, s1, L.c_sub[c], ch);
     const int8_t reg = L.c_region[c];
+    // The level loop's batch flags: a batch holds a Taylor-region child (cleared per level).
+    if (has_taylor && reg == PYL_REGION_TAYLOR) has_taylor[par / q.batch_size] = 1;
     double da[kN], di[kN];
     region_shift(reg, ch.val, q.dt_add, k.tadd, da);
     region_shift(reg, ch.val, q.dt_init, k.tinit, di);
@@ -613,7 +615,7 @@ static int circ_children(const pyl_prune_params* p, const pyl_circ_params* q,
   circ_children_kernel<kComplex><<<(unsigned)items_grid(n_valid), kChildThreads<kComplex>,
                                    lds, pyl::as_stream(stream)>>>(
       k, cc, leaves, folds, *L, valid, n_valid, nullptr, seg, c_parent, c_row, c_shift, c_phase,
-      c_score, minmax);
+      c_score, minmax, nullptr);
   PYL_CHECK_LAUNCH("circ_children_kernel");
   return PYL_OK;
 }
@@ -795,11 +797,13 @@ __global__ void circ_batch_totals_kernel(const int64_t* __restrict__ off1,
                                          const int64_t* __restrict__ held_off,
                                          const int64_t* __restrict__ rec, int64_t batch_size,
                                          int64_t* __restrict__ batch_tot,
-                                         int64_t* __restrict__ batch_nk) {
+                                         int64_t* __restrict__ batch_nk,
+                                         uint8_t* __restrict__ has_taylor) {
   const int64_t nb = rec[PYL_EP_REC_NBATCH];
   const int64_t n_par = rec[kRecNpar];
   for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < nb;
        b += (int64_t)gridDim.x * blockDim.x) {
+    has_taylor[b] = 0;  // set again after the children are scored
     const int64_t p0 = b * batch_size;
     const int64_t p1 = std::min<int64_t>(p0 + batch_size, n_par);
     const int64_t d = held_off[off1[p1]] - held_off[off1[p0]];
@@ -850,25 +854,6 @@ struct ValidSink {
   __device__ void finish(int64_t nv) const { rec[kRecNvalid] = nv; }
 };
 
-__global__ void circ_taylor_flags_clear_kernel(uint8_t* __restrict__ has_taylor,
-                                               const int64_t* __restrict__ rec) {
-  const int64_t nb = rec[PYL_EP_REC_NBATCH];
-  for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < nb;
-       b += (int64_t)gridDim.x * blockDim.x)
-    has_taylor[b] = 0;
-}
-
-// circular.py:506-511: batches whose validated children include a Taylor cell.
-__global__ void circ_taylor_flags_kernel(const int64_t* __restrict__ valid,
-                                         const int8_t* __restrict__ c_region,
-                                         const int32_t* __restrict__ c_parent,
-                                         const int64_t* __restrict__ rec, int64_t batch_size,
-                                         uint8_t* __restrict__ has_taylor) {
-  const int64_t nv = rec[kRecNvalid];
-  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
-       v += (int64_t)gridDim.x * blockDim.x)
-    if (c_region[valid[v]] == PYL_REGION_TAYLOR) has_taylor[c_parent[v] / batch_size] = 1;
-}
 
 template <bool kComplex>
 int circ_levels(const pyl_prune_params* p, const pyl_circ_params* q,
@@ -954,7 +939,7 @@ int circ_levels(const pyl_prune_params* p, const pyl_circ_params* q,
     chunked_scan(HeldSrc{C->s1_expand, C->s1_parent, C->cr_num}, PackedOffSink{C->held_off},
                  rec + PYL_EP_REC_NHELD, L->part, s);
     circ_batch_totals_kernel<<<g_par, tpar, 0, s>>>(C->off1, C->held_off, rec, bs, C->batch_tot,
-                                                   C->batch_nk);
+                                                   C->batch_nk, C->has_taylor);
     chunked_scan(CountSrc{C->batch_tot}, BatchOffSink{C->batch_off, rec, L->cap_child},
                  rec + PYL_EP_REC_NBATCH, L->part, s);
     circ_positions_kernel<<<g_held, tpar, 0, s>>>(C->off1, C->held_off, C->s1_parent,
@@ -967,13 +952,10 @@ int circ_levels(const pyl_prune_params* p, const pyl_circ_params* q,
     if (tm) tm->begin(PYL_EP_PH_CHILDREN, s);
     circ_children_kernel<kComplex><<<g_child, kChildThreads<kComplex>, lds, s>>>(
         k, cc, leaves, folds, cl, C->valid, 0, rec + kRecNvalid, seg, L->c_parent, L->c_row,
-        L->c_shift, L->c_phase, L->c_score, L->minmax + 2 * l);
+        L->c_shift, L->c_phase, L->c_score, L->minmax + 2 * l, C->has_taylor);
     PYL_CHECK_LAUNCH("circ_children_kernel");
     if (tm) tm->end(s);
     if (tm) tm->begin(PYL_EP_PH_SELECT, s);
-    circ_taylor_flags_clear_kernel<<<g_par, tpar, 0, s>>>(C->has_taylor, rec);
-    circ_taylor_flags_kernel<<<g_held, tpar, 0, s>>>(C->valid, C->c_region, L->c_parent, rec,
-                                                    bs, C->has_taylor);
     chunked_scan(PassSrc{L->c_score, thr[l]},
                  SurvSink{L->surv, rec, L->max_sugg, L->cap_hist}, rec + kRecNvalid, L->part,
                  s);
