@@ -906,7 +906,8 @@ int circ_levels(const pyl_prune_params* p, const pyl_circ_params* q,
   const unsigned g_par = strided_grid(L->cap_par, tpar);
   const unsigned g_held = strided_grid(L->cap_child, tpar);
   const unsigned g_child = items_grid_cap(L->cap_child);
-  const unsigned g_surv = items_grid_cap(L->max_sugg);
+  // Survivors are children of the level: the materialize grid covers the smaller capacity.
+  const unsigned g_surv = items_grid_cap(std::min(L->max_sugg, L->cap_child));
   for (int l = 0; l < nlev; ++l) {
     LevelConst k;
     CircConst cc;

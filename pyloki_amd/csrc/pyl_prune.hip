@@ -791,7 +791,8 @@ int ep_levels(const pyl_prune_params* p, const pyl_prune_coords* coords, const d
     if (l == 0) {  // grids for the capacities: blocks past a level's count exit at once
       g_par = strided_grid(L->cap_par, tpar);
       g_child = items_grid_cap(L->cap_child);
-      g_surv = items_grid_cap(L->max_sugg);
+      // Survivors are children of the level: the grid covers the smaller capacity.
+      g_surv = items_grid_cap(std::min(L->max_sugg, L->cap_child));
     }
     PYL_REQUIRE(lds <= kMaxEpLds, "nbins too large for the EP scorer (%d)", k.nbins);
     if constexpr (kComplex)
