@@ -455,16 +455,26 @@ constexpr int kCBlk = 4 * kBlk;
 template <bool kComplex>
 constexpr int kChildThreads = kComplex ? kCBlk : kBlk;
 
-// irfft + e/sqrt(v) of the block's rows (lane = bin) into the scorer's columns; the caller
-// has filled Xs and synchronised. The rows go to the block's wavefronts four children at a
-// time (eight independent accumulations per lane, each one pyl::irfft_bin's, in its order),
-// the last nvalid % 4 rows one at a time. Returns after a barrier; the columns are then
-// scored (complex_score_columns, or score_column by thread tid < nvalid).
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+// irfft + e/sqrt(v) of the block's rows into the scorer's columns, as a real product on the
+// matrix cores: Y[m][b] = sum_j A[m][j] W[j][b] over the 2 (B/2 + 1) real coefficients
+// j = (harmonic s, real or imaginary part) of row m = 2 r + component of child r, with
+//   W[(s, re)][b] = w_s cos(2 pi s b / B), w_s = 1 at s = 0 and s = B/2, else 2,
+//   W[(s, im)][b] = -2 sin(2 pi s b / B), 0 at s = 0 and s = B/2
+// (pyl::irfft_bin's terms: the imaginary parts of the DC and Nyquist harmonics are
+// ignored, as a real inverse transform does), then times 1/B. The block's wavefronts take
+// the 32 x 32 tiles of Y (v_mfma_f32_32x32x2f32, one harmonic per step: lanes 0-31 carry
+// real parts, lanes 32-63 imaginary ones); a lane ends holding bin b of 8 children's e and
+// v. FP32 products and sums in the matrix unit's order: within a few ulps of irfft_bin,
+// not bit-equal to it. The caller has filled Xs [r][component][B/2 + 1] and synchronised;
+// returns after a barrier, the columns then scored by complex_score_columns (or
+// score_column by thread tid < nvalid).
 __device__ inline float* complex_block_columns(const LevelConst& k, char* lds, int nvalid,
                                                int tid, int nthreads) {
   const int B = k.nbins;
   const int nb_f = B / 2 + 1;
-  const float2* Xs = reinterpret_cast<const float2*>(lds);
+  const float* Xf = reinterpret_cast<const float*>(lds);  // row m: 2 nb_f floats
   float* S = reinterpret_cast<float*>(lds + sizeof(float2) * (size_t)kBlk * 2 * nb_f);
   float* tw = S + (B + k.sc.wmax) * kStride;
   pyl::fill_twiddles(tw, B, tid, nthreads);
@@ -472,41 +482,37 @@ __device__ inline float* complex_block_columns(const LevelConst& k, char* lds, i
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int nwaves = nthreads >> 6;
-  const int nfull = nvalid / 4;
-  for (int g = wave; g < nfull; g += nwaves) {
-    const int r = 4 * g;
-    const float2* X = Xs + (size_t)r * 2 * nb_f;
-    for (int b = lane; b < B; b += 64) {
-      float acc[8];
+  const int row = lane & 31, half = lane >> 5;
+  const int m_tiles = (2 * nvalid + 31) / 32;
+  const int n_tiles = (B + 31) / 32;
+  const float inv = 1.0f / (float)B;
+  for (int t = wave; t < m_tiles * n_tiles; t += nwaves) {
+    const int mt = t / n_tiles, nt = t - mt * n_tiles;
+    const int m = 32 * mt + row;
+    const bool mvalid = m < 2 * nvalid;
+    const float* xa = Xf + (size_t)(mvalid ? m : 0) * 2 * nb_f + half;
+    const int b = 32 * nt + row;
+    const bool bvalid = b < B;
+    const float* twp = tw + (half ? B : 0);
+    f32x16_t acc;
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const float2* row = X + (size_t)q * nb_f;  // child r + q / 2, component q % 2
-        const float ny = row[nb_f - 1].x;
-        acc[q] = row[0].x;
-        acc[q] += (b & 1) ? -ny : ny;
-      }
+    for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
 #pragma unroll 4
-      for (int kk = 1; kk < nb_f - 1; ++kk) {
-        const int idx = (kk * b) & (B - 1);
-        const float c = tw[idx], sn = tw[B + idx];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          const float2 xk = X[(size_t)q * nb_f + kk];
-          acc[q] += 2.0f * (xk.x * c - xk.y * sn);
-        }
-      }
-      const float inv = 1.0f / (float)B;
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-        S[b * kStride + r + q] = pyl::norm_ev(acc[2 * q] * inv, acc[2 * q + 1] * inv);
+    for (int sh = 0; sh < nb_f; ++sh) {
+      const float a = mvalid ? xa[2 * sh] : 0.0f;
+      const bool edge = sh == 0 || sh == nb_f - 1;
+      const float wt = edge ? (half ? 0.0f : 1.0f) : (half ? -2.0f : 2.0f);
+      const float w = bvalid ? wt * twp[(sh * b) & (B - 1)] : 0.0f;
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, w, acc, 0, 0, 0);
     }
-  }
-  for (int r = 4 * nfull + wave; r < nvalid; r += nwaves) {
-    const float2* X = Xs + (size_t)r * 2 * nb_f;
-    for (int b = lane; b < B; b += 64) {
-      const float e = pyl::irfft_bin(X, B, b, tw);
-      const float v = pyl::irfft_bin(X + nb_f, B, b, tw);
-      S[b * kStride + r] = pyl::norm_ev(e, v);
+    // acc[i] is row 8 (i / 4) + 4 half + i % 4 of the tile: rows (2 r, 2 r + 1) = (e, v)
+    // of child r sit in acc[i], acc[i + 1] for even i.
+    if (bvalid) {
+#pragma unroll
+      for (int i = 0; i < 16; i += 2) {
+        const int r = (32 * mt + 8 * (i / 4) + 4 * half + (i % 4)) / 2;
+        if (r < nvalid) S[b * kStride + r] = pyl::norm_ev(acc[i] * inv, acc[i + 1] * inv);
+      }
     }
   }
   __syncthreads();
