@@ -440,6 +440,7 @@ int pyl_cheb_ascend_complex(const pyl_prune_params* p, const double* leaves_dev,
 #define PYL_EP_ST_CHILD_CAP 2 /* children exceed cap_child */
 #define PYL_EP_ST_OVERFLOW 3  /* passing children exceed max_sugg */
 #define PYL_EP_ST_HIST_FULL 4 /* backtrack history full */
+#define PYL_EP_ST_BUDGET 5    /* survivors exceed a budget the level was enqueued without */
 
 /* Optional per-phase timing of pyl_ep_levels: HIP events recorded on the stream around
  * each level's phases (PYL_EP_PH_*), read back by pyl_ep_timer_read (which waits for them
@@ -487,6 +488,12 @@ typedef struct pyl_ep_budget {
   double* stats;         /* f64 [PYL_BUDGET_STATS]: per-level score statistics */
   uint32_t* shist;       /* u32 [2 * PYL_BUDGET_HIST_BINS]: score histogram and its prefix */
   int64_t* part;         /* i64 [PYL_EP_SCAN_BLOCKS] */
+  /* Level loop: host flags indexed by search level (0..nlevels), or null. A level
+   * whose flag is 0 is enqueued without the ranking and the cut: its survivors are copied
+   * as they are when they fit the budget, else the level stops with PYL_EP_ST_BUDGET (and
+   * runs per level, where the budget always applies). Results do not depend on the flags;
+   * the launches of a level without the cut are 1 instead of 15. */
+  const uint8_t* level_mask;
 } pyl_ep_budget;
 
 /* The budget on one tree of n rows (the per-level path): writes the kept rows to out_*

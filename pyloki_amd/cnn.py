@@ -251,6 +251,9 @@ class SurvivorBudget:
         self.hybrid = bool(hybrid)
         self._cap = -1
         self._s = None
+        # Level loop: levels that get the ranking and the cut (None: all), see device_budget.
+        self.level_mask: np.ndarray | None = None
+        self._mask = None
 
     def _buffers(self, pr, cap: int):
         """The C struct with scratch for ``cap`` rows (kept alive here)."""
@@ -283,9 +286,22 @@ class SurvivorBudget:
         return self._s
 
     def device_budget(self, pr, cap: int, level0: int):
-        """The budget for the level loop, whose first level is ``level0``."""
+        """The budget for the level loop, whose first level is ``level0``. With
+        ``level_mask`` set (uint8, one flag per search level 0..nsegments - 1), the loop
+        enqueues the ranking and the cut only on flagged levels; another level keeps its
+        survivors when they fit the budget and otherwise runs per level, so the result does
+        not depend on the mask."""
         s = self._buffers(pr, cap)
         s.level0 = int(level0)
+        mask = self.level_mask
+        if mask is not None:
+            mask = np.ascontiguousarray(mask, dtype=np.uint8)
+            if mask.shape != (s.nlevels + 1,):
+                raise ValueError(f"level_mask needs {s.nlevels + 1} flags, got {mask.shape}")
+            self._mask = mask  # kept alive while the struct points at it
+            s.level_mask = mask.ctypes.data
+        else:
+            s.level_mask = None
         return s
 
     def __call__(self, pr, tree):

@@ -364,6 +364,37 @@ budget_gather_kernel(const int64_t* __restrict__ kept, const int64_t* __restrict
   if (lane < nb) o_hist[(rec[kRecHist] + r) * nb + lane] = bt[i * nb + lane];
 }
 
+// A level enqueued without the cut: one wavefront per survivor copies it as the gather
+// does (row r = survivor r) when the survivors fit the budget; else the level stops (no
+// kept rows, no parents for the next level) and runs per level.
+__global__ void __launch_bounds__(kT)
+budget_pass_kernel(int64_t* __restrict__ rec, int64_t budget, int P, int row_floats,
+                   const double* __restrict__ leaves, const float* __restrict__ folds,
+                   const float* __restrict__ scores, const int32_t* __restrict__ bt,
+                   double* __restrict__ o_leaves, float* __restrict__ o_folds,
+                   float* __restrict__ o_scores, int32_t* __restrict__ o_hist) {
+  const int64_t m = rec[kRecNsurv];
+  const int64_t h = rec[kRecHist];
+  const bool over = m > budget;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const int64_t k = over ? 0 : m;
+    if (over) rec[kRecStatus] = PYL_EP_ST_BUDGET;
+    rec[PYL_EP_REC_NKEPT] = k;
+    rec[kRecWords + kRecNpar] = k;
+    rec[kRecWords + kRecHist] = h + k;
+  }
+  const int64_t r = (int64_t)blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
+  if (over || r >= m) return;
+  const int lane = threadIdx.x & 63;
+  const int nl = (P + 2) * 2;
+  const int nb = P + 2;
+  if (lane < nl) o_leaves[r * nl + lane] = leaves[r * nl + lane];
+  for (int j = lane; j < row_floats; j += 64)
+    o_folds[r * row_floats + j] = folds[r * row_floats + j];
+  if (lane == 0) o_scores[r] = scores[r];
+  if (lane < nb) o_hist[(h + r) * nb + lane] = bt[r * nb + lane];
+}
+
 __global__ void budget_record_kernel(int64_t* __restrict__ rec, int64_t n) {
   const int w = threadIdx.x;
   if (w < 2 * kRecWords) rec[w] = 0;
@@ -418,6 +449,26 @@ int pyl::ep_budget_enqueue(const pyl_ep_budget* b, int P, int level, const doubl
                          s>>>(b->kept, rec, P, row_floats, leaves, folds, scores, bt,
                               o_leaves, o_folds, o_scores, o_hist);
   PYL_CHECK_LAUNCH("budget_gather_kernel");
+  return PYL_OK;
+}
+
+int pyl::ep_budget_level_enqueue(const pyl_ep_budget* b, int P, int level,
+                                 const double* leaves, const float* folds,
+                                 const float* scores, const int32_t* bt, int64_t n_cap,
+                                 int64_t* rec, double* o_leaves, float* o_folds,
+                                 float* o_scores, int32_t* o_hist, hipStream_t s) {
+  PYL_REQUIRE(b != nullptr, "null budget");
+  PYL_REQUIRE(level >= 0 && level <= b->nlevels, "level %d outside the search", level);
+  if (b->level_mask == nullptr || b->level_mask[level] != 0)
+    return ep_budget_enqueue(b, P, level, leaves, folds, scores, bt, n_cap, rec, o_leaves,
+                             o_folds, o_scores, o_hist, s);
+  PYL_REQUIRE(b->budget >= 1, "budget must be >= 1");
+  const int row_floats = b->complex_folds ? 4 * (b->nbins / 2 + 1) : 2 * b->nbins;
+  const int64_t rows = std::min<int64_t>(n_cap, b->budget);
+  budget_pass_kernel<<<(unsigned)std::max<int64_t>(1, pyl::ceil_div(rows, kT / 64)), kT, 0,
+                       s>>>(rec, b->budget, P, row_floats, leaves, folds, scores, bt,
+                            o_leaves, o_folds, o_scores, o_hist);
+  PYL_CHECK_LAUNCH("budget_pass_kernel");
   return PYL_OK;
 }
 

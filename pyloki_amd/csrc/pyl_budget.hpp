@@ -21,4 +21,12 @@ int ep_budget_enqueue(const pyl_ep_budget* b, int P, int level, const double* le
                       int64_t n_cap, int64_t* rec, double* o_leaves, float* o_folds,
                       float* o_scores, int32_t* o_hist, hipStream_t s);
 
+// The budget of level `level` in the level loop: ep_budget_enqueue, or, when b->level_mask
+// flags the level off, one launch that copies the survivors to o_* (and their backtracks to
+// the history) when they fit the budget, else stops the level with PYL_EP_ST_BUDGET.
+int ep_budget_level_enqueue(const pyl_ep_budget* b, int P, int level, const double* leaves,
+                            const float* folds, const float* scores, const int32_t* bt,
+                            int64_t n_cap, int64_t* rec, double* o_leaves, float* o_folds,
+                            float* o_scores, int32_t* o_hist, hipStream_t s);
+
 }  // namespace pyl

@@ -972,9 +972,10 @@ int circ_levels(const pyl_prune_params* p, const pyl_circ_params* q,
     if (tm) tm->end(s);
     if (bud) {
       if (tm) tm->begin(PYL_EP_PH_BUDGET, s);
-      rc = pyl::ep_budget_enqueue(bud, kP, bud->level0 + l, L->leaves[0], L->folds[0],
-                                  L->scores[0], bud->bt_tmp, L->max_sugg, rec, L->leaves[1],
-                                  L->folds[1], L->scores[1], L->hist, s);
+      rc = pyl::ep_budget_level_enqueue(bud, kP, bud->level0 + l, L->leaves[0],
+                                        L->folds[0], L->scores[0], bud->bt_tmp, L->max_sugg,
+                                        rec, L->leaves[1], L->folds[1], L->scores[1], L->hist,
+                                        s);
       if (rc != PYL_OK) return rc;
       if (tm) tm->end(s);
     }

@@ -51,6 +51,7 @@ class CapturedSearch:
         self.pr.execute(self.ref_seg)
         if loop_caps is None or "cap_child" not in loop_caps:
             self._fit_children()
+        self._fit_budget()
 
     def _fit_children(self) -> None:
         """Per-child arrays for twice the eager run's largest level (at least 2^16): the
@@ -63,6 +64,23 @@ class CapturedSearch:
         buf = self.pr._loop
         if buf is not None and cap < buf.cap_child:
             buf.set_children(cap)
+
+    def _fit_budget(self) -> None:
+        """A survivor budget's ranking and cut only on the levels where the eager run's
+        survivors (before the cut) exceeded a quarter of the budget, or where it stopped:
+        the other levels keep their survivors in one launch instead of about 15 (a search's
+        survivors rise to the budget and fall well below it on the last levels). A series
+        whose survivors exceed the budget on an unflagged level runs that level per level,
+        so results do not depend on this."""
+        hook = self.pr.level_hook
+        if hook is None or not hasattr(hook, "level_mask") or hook.level_mask is not None:
+            return
+        nlast = self.pr.dyp.nsegments - 1
+        mask = np.zeros(nlast + 1, dtype=np.uint8)
+        for level in range(1, nlast + 1):
+            st = self.pr.loop_levels.get(level)
+            mask[level] = st is None or st[0] != 0 or st[1] > hook.budget // 4
+        hook.level_mask = mask
 
     def _buffer_ptrs(self):
         buf = self.pr._loop

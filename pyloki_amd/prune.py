@@ -88,7 +88,7 @@ EP_COMPLEX, EP_CHEBYSHEV = 1, 2
 EP_SCAN_BLOCKS = 256
 REC_NPAR, REC_NCHILD, REC_NSURV, REC_STATUS, REC_ERR, REC_HIST, REC_NVALID = range(7)
 REC_WORDS = 12
-ST_OK, ST_BRANCH, ST_CHILD_CAP, ST_OVERFLOW, ST_HIST_FULL = range(5)
+ST_OK, ST_BRANCH, ST_CHILD_CAP, ST_OVERFLOW, ST_HIST_FULL, ST_BUDGET = range(6)
 
 
 class EpLoop(ctypes.Structure):
@@ -110,7 +110,8 @@ class EpBudget(ctypes.Structure):
                 ("level0", ctypes.c_int32), ("precision", ctypes.c_int32),
                 ("budget", ctypes.c_int64)] + [
         (name, ctypes.c_void_p) for name in (
-            "params", "keys", "feats", "kept", "bt_tmp", "sel", "stats", "shist", "part")]
+            "params", "keys", "feats", "kept", "bt_tmp", "sel", "stats", "shist", "part",
+            "level_mask")]
 
 
 class CircLoop(ctypes.Structure):
@@ -458,6 +459,8 @@ class Pruning:
         self._timer = None
         # Levels the loop handed to the host, by status (ST_*).
         self.loop_handoffs: dict[int, int] = {}
+        # (status, survivors before any budget) of every level the loop ran, by level.
+        self.loop_levels: dict[int, tuple[int, int]] = {}
 
     def _timed(self, phase: str, fn, *args):
         if not self.timing:
@@ -725,6 +728,7 @@ class Pruning:
         for i, level in enumerate(levels):
             r = rec[i]
             stop = int(r[REC_STATUS])
+            self.loop_levels[level] = (stop, int(r[REC_NSURV]))
             if stop != ST_OK:
                 break
             o = 1 if budget is not None else (first_out + i) & 1

@@ -37,7 +37,7 @@ def test_bindings_cover_declarations():
 
 def test_abi_version_and_error_path_without_gpu():
     lib = _lib.load()
-    assert lib.pyl_abi_version() == 105
+    assert lib.pyl_abi_version() == 106
     # An invalid argument is rejected on the host before anything touches the GPU.
     rc = lib.pyl_ffa_resolve(0, None, None, None, None, 1, 1.0, 64, None, None, None)
     assert rc == _lib.PYL_ERR_INVALID

@@ -11,7 +11,7 @@ import pytest
 import torch
 
 from oracle import prune_oracle as po
-from pyloki_amd.prune import ST_CHILD_CAP, ST_HIST_FULL, ST_OVERFLOW, Pruning
+from pyloki_amd.prune import ST_BUDGET, ST_CHILD_CAP, ST_HIST_FULL, ST_OVERFLOW, Pruning
 from tests.test_gpu_cheby import _setup
 from tests.test_gpu_circ import _setup as _circ_setup
 
@@ -238,3 +238,40 @@ def test_loop_budget_equals_per_level_path(gpu_device, kind, circular, budget):
     assert any(s.n_leaves_surv == budget for s in st[1:])  # the cut was applied
     print(f"[result] levels {runs[1][0].prune_level}, leaves {sum(s.n_leaves for s in st)}, "
           f"final {len(runs[1][1]['scores'])}", flush=True)
+
+
+@pytest.mark.parametrize("kind,circular,budget", [("snr", False, 64), ("hybrid", True, 512)])
+@pytest.mark.parametrize("mask", ["none_flagged", "second_half"])
+def test_loop_budget_level_mask(gpu_device, kind, circular, budget, mask):
+    """Levels enqueued without the ranking and the cut keep their survivors when they fit
+    the budget and otherwise stop and run per level: the result equals the per-level path
+    whatever the mask."""
+    from pyloki_amd.cnn import BranchRanker, SurvivorBudget
+
+    _case(f"ep loop budget mask={mask} kind={kind} circular={circular} budget={budget}")
+    if circular:
+        cfg, dyp, pcfg = _circ_setup(use_fourier=True)
+        ref = dyp.nsegments // 2
+        thresholds = np.linspace(2.0, 8.0, dyp.nsegments - 1)
+    else:
+        cfg, dyp, pcfg = _setup(2, 2**18, 16)
+        ref = 7
+        thresholds = np.linspace(2.0, 5.0, dyp.nsegments - 1)
+    ranker = BranchRanker(cfg.nbins, seed=3) if kind != "snr" else None
+    nlev = dyp.nsegments - 1
+    flags = np.zeros(nlev + 1, dtype=np.uint8)
+    if mask == "second_half":
+        flags[nlev // 2:] = 1
+    runs = []
+    for dl in (False, True):
+        pr = Pruning(dyp, thresholds, max_sugg=2**16, batch_size=256, device_loop=dl)
+        pr.level_hook = SurvivorBudget(budget, ranker, hybrid=kind == "hybrid")
+        if dl:
+            pr.level_hook.level_mask = flags
+        runs.append((pr, pr.execute(ref)))
+    _same(runs[1], runs[0])
+    pr = runs[1][0]
+    assert any(s.n_leaves_surv == budget for s in pr.pstats[1:])  # the cut was applied
+    if mask == "none_flagged":  # every cut came from a level stopped and run per level
+        assert pr.loop_handoffs.get(ST_BUDGET, 0) > 0
+    print(f"[result] mask {mask}: handoffs {pr.loop_handoffs}", flush=True)
