@@ -265,7 +265,7 @@ __device__ __forceinline__ void circ_expand_one(const LevelConst& k, const CircC
 }
 
 template <bool kComplex>
-__global__ void __launch_bounds__(kChildThreads<kComplex>)
+__global__ void __launch_bounds__(kCBlk)
 circ_children_kernel(LevelConst k, CircConst q, const double* __restrict__ leaves,
                      const float* __restrict__ folds, pyl_circ_level L,
                      const int64_t* __restrict__ valid, int64_t n_valid_arg,
@@ -329,12 +329,13 @@ circ_children_kernel(LevelConst k, CircConst q, const double* __restrict__ leave
     __syncthreads();
     Sc = complex_block_columns(k, lds, nvalid, tid, kCBlk);
   } else {
-#pragma unroll 4
-    for (int r = 0; r < nvalid; ++r) {
+    // Rows over the block's wavefronts, lane = bin.
+#pragma unroll 2
+    for (int r = tid >> 6; r < nvalid; r += kCBlk / 64) {
       const float* fr = folds + (int64_t)s_parent[r] * 2 * B;
       const float* sr = seg + (int64_t)s_row[r] * 2 * B;
       const int sh = s_shift[r];
-      for (int b = tid; b < B; b += kBlk) {
+      for (int b = tid & 63; b < B; b += 64) {
         int src = b - sh;
         if (src < 0) src += B;
         const float e = fr[b] + sr[src];
@@ -344,14 +345,8 @@ circ_children_kernel(LevelConst k, CircConst q, const double* __restrict__ leave
     }
     __syncthreads();
   }
-  float sc = 0.0f;
-  if constexpr (kComplex) {
-    sc = complex_score_columns(k, Sc, nvalid, tid, kCBlk);
-    if (tid < nvalid) c_score[v0 + tid] = sc;
-  } else if (tid < nvalid) {
-    sc = score_column(k, Sc, tid);
-    c_score[v0 + tid] = sc;
-  }
+  const float sc = block_score_columns(k, Sc, nvalid, tid, kCBlk);
+  if (tid < nvalid) c_score[v0 + tid] = sc;
   block_minmax(sc, tid < nvalid, minmax);
 }
 
@@ -612,7 +607,7 @@ static int circ_children(const pyl_prune_params* p, const pyl_circ_params* q,
   if constexpr (kComplex)
     PYL_REQUIRE((k.nbins & (k.nbins - 1)) == 0,
                 "complex folds need a power-of-two nbins (%d)", k.nbins);
-  circ_children_kernel<kComplex><<<(unsigned)items_grid(n_valid), kChildThreads<kComplex>,
+  circ_children_kernel<kComplex><<<(unsigned)items_grid(n_valid), kCBlk,
                                    lds, pyl::as_stream(stream)>>>(
       k, cc, leaves, folds, *L, valid, n_valid, nullptr, seg, c_parent, c_row, c_shift, c_phase,
       c_score, minmax, nullptr);
@@ -951,7 +946,7 @@ int circ_levels(const pyl_prune_params* p, const pyl_circ_params* q,
     PYL_CHECK_LAUNCH("circ_expand_kernel");
     if (tm) tm->end(s);
     if (tm) tm->begin(PYL_EP_PH_CHILDREN, s);
-    circ_children_kernel<kComplex><<<g_child, kChildThreads<kComplex>, lds, s>>>(
+    circ_children_kernel<kComplex><<<g_child, kCBlk, lds, s>>>(
         k, cc, leaves, folds, cl, C->valid, 0, rec + kRecNvalid, seg, L->c_parent, L->c_row,
         L->c_shift, L->c_phase, L->c_score, L->minmax + 2 * l, C->has_taylor);
     PYL_CHECK_LAUNCH("circ_children_kernel");

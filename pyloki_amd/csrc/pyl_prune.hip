@@ -130,7 +130,7 @@ __global__ void cheb_parents_kernel(LevelConst k, const double* __restrict__ lea
 // Folds are real [2][B] rows or, with kComplex, complex64 [2][B/2 + 1] rows (as float2).
 // kCheby: the leaves are Chebyshev coefficients on the current domain (chebyshev.py:180-258).
 template <bool kComplex, bool kCheby>
-__global__ void __launch_bounds__(kChildThreads<kComplex>)
+__global__ void __launch_bounds__(kCBlk)
 prune_children_kernel(LevelConst k, const double* __restrict__ leaves,
                       const float* __restrict__ folds, int64_t n_par_arg,
                       const int64_t* __restrict__ n_par_dev,
@@ -214,13 +214,13 @@ prune_children_kernel(LevelConst k, const double* __restrict__ leaves,
     Sc = complex_block_columns(k, lds, nvalid, tid, kCBlk);
   } else {
     // Fold (common.py:389-399: parent row + segment row rotated by the shift) and
-    // normalise, lane = bin, transposed into S.
-#pragma unroll 4
-    for (int r = 0; r < nvalid; ++r) {
+    // normalise, rows over the block's wavefronts, lane = bin, transposed into S.
+#pragma unroll 2
+    for (int r = tid >> 6; r < nvalid; r += kCBlk / 64) {
       const float* fr = folds + (int64_t)s_parent[r] * 2 * B;
       const float* sr = seg + (int64_t)s_row[r] * 2 * B;
       const int sh = s_shift[r];
-      for (int b = tid; b < B; b += kBlk) {
+      for (int b = tid & 63; b < B; b += 64) {
         int src = b - sh;
         if (src < 0) src += B;
         const float e = fr[b] + sr[src];
@@ -230,14 +230,8 @@ prune_children_kernel(LevelConst k, const double* __restrict__ leaves,
     }
     __syncthreads();
   }
-  float sc = 0.0f;
-  if constexpr (kComplex) {
-    sc = complex_score_columns(k, Sc, nvalid, tid, kCBlk);
-    if (tid < nvalid) c_score[c0 + tid] = sc;
-  } else if (tid < nvalid) {
-    sc = score_column(k, Sc, tid);
-    c_score[c0 + tid] = sc;
-  }
+  const float sc = block_score_columns(k, Sc, nvalid, tid, kCBlk);
+  if (tid < nvalid) c_score[c0 + tid] = sc;
   // Block min/max of the scores, then one atomic per wavefront.
   block_minmax(sc, tid < nvalid, minmax);
 }
@@ -460,7 +454,7 @@ static int prune_children(const pyl_prune_params* p, const pyl_prune_coords* c,
     PYL_REQUIRE((k.nbins & (k.nbins - 1)) == 0,
                 "complex folds need a power-of-two nbins (%d)", k.nbins);
   prune_children_kernel<kComplex, kCheby><<<(unsigned)items_grid(n_child),
-                                            kChildThreads<kComplex>, lds,
+                                            kCBlk, lds,
                                             pyl::as_stream(stream)>>>(
       k, leaves, folds, n_par, nullptr, branch, counts, off, n_child, nullptr, seg, c_parent,
       c_row, c_shift, c_phase, c_score, minmax);
@@ -823,7 +817,7 @@ int ep_levels(const pyl_prune_params* p, const pyl_prune_coords* coords, const d
     PYL_CHECK_LAUNCH("chunk_scan_kernel");
     if (tm) tm->end(s);
     if (tm) tm->begin(PYL_EP_PH_CHILDREN, s);
-    prune_children_kernel<kComplex, kCheby><<<g_child, kChildThreads<kComplex>, lds, s>>>(
+    prune_children_kernel<kComplex, kCheby><<<g_child, kCBlk, lds, s>>>(
         k, src, folds, 0, rec + kRecNpar, L->branch, L->counts, L->off, 0, rec + kRecNchild,
         seg, L->c_parent, L->c_row, L->c_shift, L->c_phase, L->c_score, L->minmax + 2 * l);
     PYL_CHECK_LAUNCH("prune_children_kernel");

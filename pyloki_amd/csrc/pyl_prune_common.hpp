@@ -447,13 +447,10 @@ __host__ __device__ inline size_t complex_block_lds(int nbins, int wmax) {
   return base > ph_end ? base : ph_end;
 }
 
-// Threads of a complex children block: four wavefronts share the block's (at most kBlk)
-// children in the fold, the inverse transforms and the widths of the scorer, so that a
-// small level's few blocks still occupy every SIMD of their CUs. Real blocks keep one
-// wavefront (lane = child column).
+// Threads of an EP children block: four wavefronts share the block's (at most kBlk)
+// children in the fold, the inverse transforms (complex folds) and the widths of the
+// scorer, so that a small level's few blocks still occupy every SIMD of their CUs.
 constexpr int kCBlk = 4 * kBlk;
-template <bool kComplex>
-constexpr int kChildThreads = kComplex ? kCBlk : kBlk;
 
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
 
@@ -468,7 +465,7 @@ typedef float f32x16_t __attribute__((ext_vector_type(16)));
 // real parts, lanes 32-63 imaginary ones); a lane ends holding bin b of 8 children's e and
 // v. FP32 products and sums in the matrix unit's order: within a few ulps of irfft_bin,
 // not bit-equal to it. The caller has filled Xs [r][component][B/2 + 1] and synchronised;
-// returns after a barrier, the columns then scored by complex_score_columns (or
+// returns after a barrier, the columns then scored by block_score_columns (or
 // score_column by thread tid < nvalid).
 __device__ inline float* complex_block_columns(const LevelConst& k, char* lds, int nvalid,
                                                int tid, int nthreads) {
@@ -524,7 +521,7 @@ __device__ inline float* complex_block_columns(const LevelConst& k, char* lds, i
 // nwaves-th width, and the owner keeps the largest S/N. A maximum does not depend on the
 // order of its terms, so thread tid < nvalid gets score_column's value; every thread of
 // the block calls this.
-__device__ inline float complex_score_columns(const LevelConst& k, float* S, int nvalid,
+__device__ inline float block_score_columns(const LevelConst& k, float* S, int nvalid,
                                               int tid, int nthreads) {
   __shared__ double s_best[kCBlk];
   const int B = k.nbins;
