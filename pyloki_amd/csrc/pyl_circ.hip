@@ -351,7 +351,7 @@ circ_children_kernel(LevelConst k, CircConst q, const double* __restrict__ leave
 }
 
 template <bool kComplex>
-__global__ void __launch_bounds__(kBlk)
+__global__ void __launch_bounds__(kCBlk)
 circ_materialize_kernel(LevelConst k, CircConst q, const double* __restrict__ leaves,
                         const float* __restrict__ folds, pyl_circ_level L,
                         const int64_t* __restrict__ valid, const float* __restrict__ seg,
@@ -364,6 +364,7 @@ circ_materialize_kernel(LevelConst k, CircConst q, const double* __restrict__ le
                         float* __restrict__ o_folds, float* __restrict__ o_scores,
                         int32_t* __restrict__ o_bt_base, const int64_t* __restrict__ bt_row0) {
   __shared__ int64_t s_v[kBlk];
+  __shared__ int32_t s_fpar[kBlk], s_frow[kBlk], s_fsh[kBlk];  // real folds
   extern __shared__ float2 s_ph[];  // complex: [kBlk][B/2 + 1] phasors, then rows, parents
   const int tid = threadIdx.x;
   const int64_t n_surv = dev_count(n_surv_arg, n_dev);
@@ -423,16 +424,23 @@ circ_materialize_kernel(LevelConst k, CircConst q, const double* __restrict__ le
     __syncthreads();
     cfold_rows(reinterpret_cast<const float2*>(seg), reinterpret_cast<const float2*>(folds),
                s_rw, s_pa, s_ph, nvalid, B,
-               reinterpret_cast<float2*>(o_folds) + s0 * 2 * (B / 2 + 1), tid, kBlk);
+               reinterpret_cast<float2*>(o_folds) + s0 * 2 * (B / 2 + 1), tid, kCBlk);
   } else {
+    if (tid < nvalid) {
+      const int64_t v = s_v[tid];
+      s_fpar[tid] = c_parent[v];
+      s_frow[tid] = c_row[v];
+      s_fsh[tid] = c_shift[v];
+    }
     __syncthreads();
-    for (int r = 0; r < nvalid; ++r) {
-      const int64_t v = s_v[r];
-      const float* fr = folds + (int64_t)c_parent[v] * 2 * B;
-      const float* sr = seg + (int64_t)c_row[v] * 2 * B;
-      const int sh = c_shift[v];
+    // Rows over the block's wavefronts, lane = bin.
+#pragma unroll 2
+    for (int r = tid >> 6; r < nvalid; r += kCBlk / 64) {
+      const float* fr = folds + (int64_t)s_fpar[r] * 2 * B;
+      const float* sr = seg + (int64_t)s_frow[r] * 2 * B;
+      const int sh = s_fsh[r];
       float* of = o_folds + (s0 + r) * 2 * B;
-      for (int b = tid; b < B; b += kBlk) {
+      for (int b = tid & 63; b < B; b += 64) {
         int src = b - sh;
         if (src < 0) src += B;
         of[b] = fr[b] + sr[src];
@@ -654,7 +662,7 @@ static int circ_materialize(const pyl_prune_params* p, const pyl_circ_params* q,
   PYL_REQUIRE(leaves && folds && valid && seg && c_parent && c_row && c_shift && c_phase &&
                   c_score && has_taylor && surv && o_leaves && o_folds && o_scores && o_bt,
               "null device pointer");
-  circ_materialize_kernel<kComplex><<<(unsigned)items_grid(n_surv), kBlk,
+  circ_materialize_kernel<kComplex><<<(unsigned)items_grid(n_surv), kCBlk,
                                       materialize_lds(kComplex, k.nbins),
                                       pyl::as_stream(stream)>>>(
       k, cc, leaves, folds, *L, valid, seg, c_parent, c_row, c_shift, c_phase, c_score,
@@ -958,7 +966,7 @@ int circ_levels(const pyl_prune_params* p, const pyl_circ_params* q,
     PYL_CHECK_LAUNCH("chunk_scan_kernel");
     if (tm) tm->end(s);
     if (tm) tm->begin(PYL_EP_PH_MATERIALIZE, s);
-    circ_materialize_kernel<kComplex><<<g_surv, kBlk, materialize_lds(kComplex, k.nbins), s>>>(
+    circ_materialize_kernel<kComplex><<<g_surv, kCBlk, materialize_lds(kComplex, k.nbins), s>>>(
         k, cc, leaves, folds, cl, C->valid, seg, L->c_parent, L->c_row, L->c_shift,
         L->c_phase, L->c_score, C->has_taylor, L->surv, 0, rec + kRecWords + kRecNpar,
         L->leaves[o], L->folds[o], L->scores[o], bud ? bud->bt_tmp : L->hist,

@@ -237,7 +237,7 @@ prune_children_kernel(LevelConst k, const double* __restrict__ leaves,
 }
 
 template <bool kComplex>
-__global__ void __launch_bounds__(kBlk)
+__global__ void __launch_bounds__(kCBlk)
 prune_materialize_kernel(LevelConst k, const double* __restrict__ leaves,
                          const float* __restrict__ folds,
                          const double* __restrict__ branch, const int32_t* __restrict__ counts,
@@ -251,6 +251,7 @@ prune_materialize_kernel(LevelConst k, const double* __restrict__ leaves,
                          float* __restrict__ o_scores, int32_t* __restrict__ o_bt_base,
                          const int64_t* __restrict__ bt_row0) {
   __shared__ int64_t s_child[kBlk];
+  __shared__ int32_t s_fpar[kBlk], s_frow[kBlk], s_fsh[kBlk];  // real folds
   extern __shared__ float2 s_ph[];  // complex: [kBlk][B/2 + 1] phasors, then rows, parents
   const int tid = threadIdx.x;
   const int64_t n_surv = dev_count(n_surv_arg, n_surv_dev);
@@ -299,16 +300,23 @@ prune_materialize_kernel(LevelConst k, const double* __restrict__ leaves,
     __syncthreads();
     cfold_rows(reinterpret_cast<const float2*>(seg), reinterpret_cast<const float2*>(folds),
                s_rw, s_pa, s_ph, nvalid, B,
-               reinterpret_cast<float2*>(o_folds) + s0 * 2 * (B / 2 + 1), tid, kBlk);
+               reinterpret_cast<float2*>(o_folds) + s0 * 2 * (B / 2 + 1), tid, kCBlk);
   } else {
+    if (tid < nvalid) {
+      const int64_t c = s_child[tid];
+      s_fpar[tid] = c_parent[c];
+      s_frow[tid] = c_row[c];
+      s_fsh[tid] = c_shift[c];
+    }
     __syncthreads();
-    for (int r = 0; r < nvalid; ++r) {
-      const int64_t c = s_child[r];
-      const float* fr = folds + (int64_t)c_parent[c] * 2 * B;
-      const float* sr = seg + (int64_t)c_row[c] * 2 * B;
-      const int sh = c_shift[c];
+    // Rows over the block's wavefronts, lane = bin.
+#pragma unroll 2
+    for (int r = tid >> 6; r < nvalid; r += kCBlk / 64) {
+      const float* fr = folds + (int64_t)s_fpar[r] * 2 * B;
+      const float* sr = seg + (int64_t)s_frow[r] * 2 * B;
+      const int sh = s_fsh[r];
       float* of = o_folds + (s0 + r) * 2 * B;
-      for (int b = tid; b < B; b += kBlk) {
+      for (int b = tid & 63; b < B; b += 64) {
         int src = b - sh;
         if (src < 0) src += B;
         of[b] = fr[b] + sr[src];
@@ -503,7 +511,7 @@ static int prune_materialize(const pyl_prune_params* p, const pyl_prune_coords* 
                   c_shift && c_phase && c_score && surv && o_leaves && o_folds && o_scores &&
                   o_bt,
               "null device pointer");
-  prune_materialize_kernel<kComplex><<<(unsigned)items_grid(n_surv), kBlk,
+  prune_materialize_kernel<kComplex><<<(unsigned)items_grid(n_surv), kCBlk,
                                        materialize_lds(kComplex, k.nbins),
                                        pyl::as_stream(stream)>>>(
       k, leaves, folds, branch, counts, off, seg, c_parent, c_row, c_shift, c_phase, c_score,
@@ -829,7 +837,7 @@ int ep_levels(const pyl_prune_params* p, const pyl_prune_coords* coords, const d
     PYL_CHECK_LAUNCH("chunk_scan_kernel");
     if (tm) tm->end(s);
     if (tm) tm->begin(PYL_EP_PH_MATERIALIZE, s);
-    prune_materialize_kernel<kComplex><<<g_surv, kBlk, materialize_lds(kComplex, k.nbins), s>>>(
+    prune_materialize_kernel<kComplex><<<g_surv, kCBlk, materialize_lds(kComplex, k.nbins), s>>>(
         k, src, folds, L->branch, L->counts, L->off, seg, L->c_parent, L->c_row, L->c_shift,
         L->c_phase, L->c_score, L->surv, 0, rec + kRecWords + kRecNpar, L->leaves[o],
         L->folds[o], L->scores[o], bud ? bud->bt_tmp : L->hist, bud ? nullptr : rec + kRecHist);
