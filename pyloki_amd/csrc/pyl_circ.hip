@@ -330,7 +330,7 @@ circ_children_kernel(LevelConst k, CircConst q, const double* __restrict__ leave
     Sc = complex_block_columns(k, lds, nvalid, tid, kCBlk);
   } else {
     // Rows over the block's wavefronts, lane = bin.
-#pragma unroll 2
+#pragma unroll 4
     for (int r = tid >> 6; r < nvalid; r += kCBlk / 64) {
       const float* fr = folds + (int64_t)s_parent[r] * 2 * B;
       const float* sr = seg + (int64_t)s_row[r] * 2 * B;
@@ -434,7 +434,7 @@ circ_materialize_kernel(LevelConst k, CircConst q, const double* __restrict__ le
     }
     __syncthreads();
     // Rows over the block's wavefronts, lane = bin.
-#pragma unroll 2
+#pragma unroll 4
     for (int r = tid >> 6; r < nvalid; r += kCBlk / 64) {
       const float* fr = folds + (int64_t)s_fpar[r] * 2 * B;
       const float* sr = seg + (int64_t)s_frow[r] * 2 * B;

@@ -153,9 +153,11 @@ prune_children_kernel(LevelConst k, const double* __restrict__ leaves,
   const int B = k.nbins;
   const int nb_f = B / 2 + 1;
   const int nvalid = (int)std::min<int64_t>(per, n_child - c0);
+  __shared__ int64_t s_range[2];
+  block_parent_range(off, n_par, c0, c0 + nvalid - 1, s_range);
   if (tid < nvalid) {
     const int64_t c = c0 + tid;
-    const int64_t par = find_parent(off, n_par, c);
+    const int64_t par = find_parent_in(off, s_range[0], s_range[1] + 1, c);
     const double* lf = leaves + par * (P + 2) * 2;
     ChildLeaf ch;
     decode_child(k, lf, branch + par * P * 2, counts + par * P, c - off[par], ch);
@@ -215,7 +217,7 @@ prune_children_kernel(LevelConst k, const double* __restrict__ leaves,
   } else {
     // Fold (common.py:389-399: parent row + segment row rotated by the shift) and
     // normalise, rows over the block's wavefronts, lane = bin, transposed into S.
-#pragma unroll 2
+#pragma unroll 4
     for (int r = tid >> 6; r < nvalid; r += kCBlk / 64) {
       const float* fr = folds + (int64_t)s_parent[r] * 2 * B;
       const float* sr = seg + (int64_t)s_row[r] * 2 * B;
@@ -310,7 +312,7 @@ prune_materialize_kernel(LevelConst k, const double* __restrict__ leaves,
     }
     __syncthreads();
     // Rows over the block's wavefronts, lane = bin.
-#pragma unroll 2
+#pragma unroll 4
     for (int r = tid >> 6; r < nvalid; r += kCBlk / 64) {
       const float* fr = folds + (int64_t)s_fpar[r] * 2 * B;
       const float* sr = seg + (int64_t)s_frow[r] * 2 * B;

@@ -269,6 +269,41 @@ __device__ __forceinline__ int64_t find_parent(const int64_t* __restrict__ off, 
   return lo;
 }
 
+// The parents of the consecutive children c_first..c_last of a block: wavefront 0 finds
+// the parent of c_first, wavefront 1 that of c_last, each by a 64-ary search (a round
+// probes 64 evenly spaced offsets and keeps the interval where off <= c stops holding:
+// about 4 dependent loads for 4 M parents instead of 22). Every thread of the block calls
+// this; it returns after a barrier with s_range = {parent of c_first, parent of c_last}.
+__device__ inline void block_parent_range(const int64_t* __restrict__ off, int64_t n_par,
+                                          int64_t c_first, int64_t c_last, int64_t* s_range) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (wave < 2) {
+    const int64_t c = wave == 0 ? c_first : c_last;
+    int64_t lo = 0, hi = n_par;  // off[lo] <= c < off[hi]
+    while (hi - lo > 1) {
+      const int64_t step = (hi - lo + 63) / 64;
+      const int64_t q = lo + (int64_t)lane * step;
+      const unsigned long long m = __ballot(q < hi && off[q] <= c);  // a prefix of lanes
+      const int64_t nlo = lo + (int64_t)(__popcll(m) - 1) * step;
+      hi = std::min<int64_t>(nlo + step, hi);
+      lo = nlo;
+    }
+    if (lane == 0) s_range[wave] = lo;
+  }
+  __syncthreads();
+}
+
+// find_parent within [lo, hi) where off[lo] <= c < off[hi].
+__device__ __forceinline__ int64_t find_parent_in(const int64_t* __restrict__ off, int64_t lo,
+                                                  int64_t hi, int64_t c) {
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (off[mid] <= c) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
 __device__ __forceinline__ void decode_child(const LevelConst& k, const double* __restrict__ lf,
                                              const double* __restrict__ br,
                                              const int32_t* __restrict__ cn, int64_t local,
