@@ -270,7 +270,7 @@ __device__ __forceinline__ int64_t find_parent(const int64_t* __restrict__ off, 
 }
 
 // The parents of the consecutive children c_first..c_last of a block: wavefront 0 finds
-// the parent of c_first, wavefront 1 that of c_last, each by a 64-ary search (a round
+// the parent of c_first, wavefront 1 that of c_last, each by a 64-ary search (each step
 // probes 64 evenly spaced offsets and keeps the interval where off <= c stops holding:
 // about 4 dependent loads for 4 M parents instead of 22). Every thread of the block calls
 // this; it returns after a barrier with s_range = {parent of c_first, parent of c_last}.
