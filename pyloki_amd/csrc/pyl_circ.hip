@@ -451,7 +451,7 @@ circ_materialize_kernel(LevelConst k, CircConst q, const double* __restrict__ le
 }
 
 template <bool kComplex>
-__global__ void __launch_bounds__(kBlk)
+__global__ void __launch_bounds__(kCBlk)
 circ_ascend_kernel(LevelConst k, CircConst q, const double* __restrict__ leaves, int64_t n,
                    const float* __restrict__ dyp, int64_t ncoords,
                    const int64_t* __restrict__ segi, const double* __restrict__ segt,
@@ -506,13 +506,14 @@ circ_ascend_kernel(LevelConst k, CircConst q, const double* __restrict__ leaves,
   __syncthreads();
   float* Sc = S;
   if constexpr (kComplex) {
-    Sc = complex_block_columns(k, reinterpret_cast<char*>(S), nvalid, tid, kBlk);
+    Sc = complex_block_columns(k, reinterpret_cast<char*>(S), nvalid, tid, kCBlk);
   } else {
-    // common.py:432-459: accumulate the rotated rows in segment order, from zero.
-    for (int r = 0; r < nvalid; ++r) {
+    // common.py:432-459: accumulate the rotated rows in segment order, from zero (rows
+    // over the block's wavefronts, lane = bin).
+    for (int r = tid >> 6; r < nvalid; r += kCBlk / 64) {
       const int64_t sv = s0 + r;
       float* of = o_folds + sv * 2 * B;
-      for (int b = tid; b < B; b += kBlk) {
+      for (int b = tid & 63; b < B; b += 64) {
         float ae = 0.0f, av = 0.0f;
         for (int s = 0; s < nseg; ++s) {
           const float* rowp =
@@ -529,7 +530,8 @@ circ_ascend_kernel(LevelConst k, CircConst q, const double* __restrict__ leaves,
     }
     __syncthreads();
   }
-  if (tid < nvalid) o_scores[s0 + tid] = score_column(k, Sc, tid);
+  const float sc = block_score_columns(k, Sc, nvalid, tid, kCBlk);
+  if (tid < nvalid) o_scores[s0 + tid] = sc;
 }
 
 
@@ -726,7 +728,7 @@ static int circ_ascend(const pyl_prune_params* p, const pyl_circ_params* q,
   PYL_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&d_ph), sizeof(double) * n * nseg, s));
   PYL_CHECK_HIP(hipMemcpyAsync(d_idx, seg_idx, sizeof(int64_t) * nseg, hipMemcpyHostToDevice, s));
   PYL_CHECK_HIP(hipMemcpyAsync(d_t0, seg_t0, sizeof(double) * nseg, hipMemcpyHostToDevice, s));
-  circ_ascend_kernel<kComplex><<<(unsigned)pyl::ceil_div(n, kBlk), kBlk, lds, s>>>(
+  circ_ascend_kernel<kComplex><<<(unsigned)pyl::ceil_div(n, kBlk), kCBlk, lds, s>>>(
       k, cc, leaves, n, dyp, ncoords, d_idx, d_t0, nseg, t0_cur, d_row, d_ph, o_folds,
       o_scores);
   PYL_CHECK_LAUNCH("circ_ascend_kernel");

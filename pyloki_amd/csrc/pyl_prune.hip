@@ -331,7 +331,7 @@ prune_materialize_kernel(LevelConst k, const double* __restrict__ leaves,
 // kCheby: cseg holds per segment the transposed transfer matrix from the survivors' domain
 // to the segment's reference time (chebyshev.py:261-349); unused otherwise.
 template <bool kComplex, bool kCheby>
-__global__ void __launch_bounds__(kBlk)
+__global__ void __launch_bounds__(kCBlk)
 prune_ascend_kernel(LevelConst k, const double* __restrict__ leaves, int64_t n,
                     const float* __restrict__ dyp, int64_t ncoords, const int64_t* __restrict__ segi,
                     const double* __restrict__ segt, const double* __restrict__ cseg,
@@ -396,13 +396,14 @@ prune_ascend_kernel(LevelConst k, const double* __restrict__ leaves, int64_t n,
   __syncthreads();
   float* Sc = S;
   if constexpr (kComplex) {
-    Sc = complex_block_columns(k, reinterpret_cast<char*>(S), nvalid, tid, kBlk);
+    Sc = complex_block_columns(k, reinterpret_cast<char*>(S), nvalid, tid, kCBlk);
   } else {
-    // common.py:432-459: accumulate the rotated rows in segment order, from zero.
-    for (int r = 0; r < nvalid; ++r) {
+    // common.py:432-459: accumulate the rotated rows in segment order, from zero (rows
+    // over the block's wavefronts, lane = bin).
+    for (int r = tid >> 6; r < nvalid; r += kCBlk / 64) {
       const int64_t sv = s0 + r;
       float* of = o_folds + sv * 2 * B;
-      for (int b = tid; b < B; b += kBlk) {
+      for (int b = tid & 63; b < B; b += 64) {
         float ae = 0.0f, av = 0.0f;
         for (int s = 0; s < nseg; ++s) {
           const float* rowp =
@@ -419,7 +420,8 @@ prune_ascend_kernel(LevelConst k, const double* __restrict__ leaves, int64_t n,
     }
     __syncthreads();
   }
-  if (tid < nvalid) o_scores[s0 + tid] = score_column(k, Sc, tid);
+  const float sc = block_score_columns(k, Sc, nvalid, tid, kCBlk);
+  if (tid < nvalid) o_scores[s0 + tid] = sc;
 }
 
 }  // namespace
@@ -590,7 +592,7 @@ static int prune_ascend(const pyl_prune_params* p, const double* leaves, int64_t
     PYL_CHECK_HIP(hipMemcpyAsync(d_c, cseg.data(), sizeof(double) * cseg.size(),
                                  hipMemcpyHostToDevice, s));
   }
-  prune_ascend_kernel<kComplex, kCheby><<<(unsigned)pyl::ceil_div(n, kBlk), kBlk, lds, s>>>(
+  prune_ascend_kernel<kComplex, kCheby><<<(unsigned)pyl::ceil_div(n, kBlk), kCBlk, lds, s>>>(
       k, leaves, n, dyp, ncoords, d_idx, d_t0, d_c, nseg, t0_cur, d_row, d_ph, o_folds,
       o_scores);
   PYL_CHECK_LAUNCH("prune_ascend_kernel");
