@@ -63,7 +63,6 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--tile-rows", type=int, default=None,
                     help="outer rows per 256-row tile of the fused FFA kernels; default: "
                          "chosen by the autotuner (pyloki_amd/autotune.py) on this device")
-    ap.add_argument("--cpu-sample-series", type=int, default=2)
     ap.add_argument("--workload", choices=["ffa", "ep-taylor-jerk", "ep-circular-cnn"],
                     default="ffa",
                     help="ffa: the headline FFA batch (default); ep-taylor-jerk: the EP "
@@ -111,27 +110,65 @@ def make_series(nseries: int):
     return out
 
 
-def cpu_baseline(series, nsample: int) -> dict:
-    """The oracle (single-threaded C restatement) on the first ``nsample`` 2^22 series."""
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _all_threads() -> int:
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return n
+
+
+def cpu_baseline(series, gpu_out, reps: int = 5) -> dict:
+    """The oracle (the C restatement; OpenMP over the reference's prange loops: segments,
+    parameter sets, profiles) on one 2^22 series of the timed batch: one warm-up, then the
+    median of ``reps`` runs, at one thread and at every core this process may use. The
+    oracle's S/N grid of that series is also compared with the GPU's grid from the timed
+    step (the parity column)."""
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     from oracle import oracle as orc
 
-    picked = [ts for ts in series if ts.nsamps == 2**22][:nsample]
-    if not picked:
-        picked = series[:nsample]
-    trials = 0
-    t0 = time.perf_counter()
-    for ts in picked:
-        snr = orc.ffa_search_snr(ts.ts_e, ts.ts_v, make_config(ts.nsamps))
-        trials += snr.size // snr.shape[-1]
-    dt = time.perf_counter() - t0
+    pick = next((i for i, ts in enumerate(series) if ts.nsamps == 2**22), 0)
+    ts = series[pick]
+    cfg = make_config(ts.nsamps)
+    runs = {}
+    snr = None
+    for th in (1, _all_threads()):
+        orc.set_threads(th)
+        snr = orc.ffa_search_snr(ts.ts_e, ts.ts_v, cfg)  # warm-up
+        dts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            snr = orc.ffa_search_snr(ts.ts_e, ts.ts_v, cfg)
+            dts.append(time.perf_counter() - t0)
+        runs[th] = float(np.median(dts))
+    trials = snr.size // snr.shape[-1]
+    got = gpu_out[pick].cpu().numpy()
+    exp = snr.reshape(got.shape)
+    th_all = _all_threads()
     return {
-        "value": trials / dt,
+        "value": trials / runs[th_all],
         "unit": "trials/s",
-        "cores": 1,
+        "cores": th_all,
         "kind": "port",
-        "sample": f"{len(picked)} series of {picked[0].nsamps} samples (same grids), "
-                  f"{trials} trials in {dt:.2f} s",
+        "sample": f"series {pick} of the batch ({ts.nsamps} samples, {trials} trials, same "
+                  f"grids): median of {reps} runs after one warm-up",
+        "single_thread": {"value": trials / runs[1], "unit": "trials/s", "cores": 1,
+                          "median_s": runs[1]},
+        "all_cores_median_s": runs[th_all],
+        "nproc": os.cpu_count(),
+        "cpu_model": _cpu_model(),
+        "parity": {"series": pick, "snr_bit_exact": bool(np.array_equal(got, exp)),
+                   "mismatches": int(np.sum(got != exp))},
     }
 
 
@@ -147,7 +184,7 @@ EP_CNN = {"nsamps": 2**24, "tsamp": 64e-6, "nseg": 128, "nbins": 64, "freq": (24
           "batch_seeds": tuple(range(9, 17))}
 
 
-def ep_circular_dyp(seed: int, w: dict = EP_CNN):
+def ep_circular_dyp(seed: int, w: dict = EP_CNN, use_fourier: bool = True):
     from pyloki_amd.config import ParamLimits, PulsarSearchConfig
     from pyloki_amd.ffa import DynamicProgramming
     from pyloki_amd.simulation.pulse import PulseSignalConfig
@@ -158,7 +195,8 @@ def ep_circular_dyp(seed: int, w: dict = EP_CNN):
     cfg = PulsarSearchConfig(nsamps=n, tsamp=w["tsamp"], nbins=w["nbins"], eta=1.0,
                              param_limits=lim, bseg_brute=n // 1024, bseg_ffa=n // w["nseg"],
                              prune_poly_order=5, p_orb_min=w["p_orb"], m_c_max=w["m_c_max"],
-                             m_p_min=w["m_p_min"], ducy_max=0.3, wtsp=1.5, branch_max=256)
+                             m_p_min=w["m_p_min"], ducy_max=0.3, wtsp=1.5, branch_max=256,
+                             use_fourier=use_fourier)
     rng = np.random.default_rng(seed)
     f0 = 0.5 * (w["freq"][0] + w["freq"][1]) * (1 + 1e-4 * rng.uniform(-1, 1))
     ts = PulseSignalConfig(period=1 / f0, dt=w["tsamp"], nsamps=n, snr=w["snr"], ducy=0.05,
@@ -539,7 +577,9 @@ def main() -> int:
         },
     }
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(series, args.cpu_sample_series)
+        cb = cpu_baseline(series, res)
+        line["parity"] = cb.pop("parity")
+        line["cpu_baseline"] = cb
     print(json.dumps(line), flush=True)
     return 0
 

@@ -63,6 +63,10 @@ def lib() -> ctypes.CDLL:
         if not _LIB_PATH.exists():
             build()
         L = ctypes.CDLL(str(_LIB_PATH))
+        L.orc_set_threads.restype = None
+        L.orc_set_threads.argtypes = [_int]
+        L.orc_get_threads.restype = _int
+        L.orc_get_threads.argtypes = []
         L.orc_phase_idx.restype = _dbl
         L.orc_phase_idx.argtypes = [_dbl, _dbl, _i64, _dbl]
         L.orc_phase_idx_int.restype = _i64
@@ -110,6 +114,16 @@ def lib() -> ctypes.CDLL:
         L.orc_boxcar_snr_f64.argtypes = [_f64p, _i64, _i64, _i64p, _i64, _f64p]
         _lib = L
     return _lib
+
+
+def set_threads(n: int) -> None:
+    """Threads of the oracle's parallel loops (the reference's prange loops); results do
+    not depend on it."""
+    lib().orc_set_threads(int(n))
+
+
+def get_threads() -> int:
+    return int(lib().orc_get_threads())
 
 
 def phase_idx(t: float, f: float, nbins: int, delay: float = 0.0) -> float:

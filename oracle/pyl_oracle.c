@@ -18,9 +18,32 @@
 #include <stdlib.h>
 #include <string.h>
 #include <float.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define ORC_C_VAL 299792458.0
 #define ORC_FLOAT_EPSILON 1e-6
+
+/* Threads of the parallel loops below. Every output element is computed by one thread
+ * in the reference's order, so results do not depend on the count. The parallel loops
+ * are the reference's prange loops: segments in the brute fold (fold.py:161), parameter
+ * sets in unify_fold (dyn_pffa.py:465) and profiles in the scorer (scoring.py:151). */
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+#else
+    (void)n;
+#endif
+}
+
+int orc_get_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
 
 /* utils/psr_utils.py:12-46 get_phase_idx */
 double orc_phase_idx(double proper_time, double freq, int64_t nbins, double delay) {
@@ -123,12 +146,14 @@ void orc_brutefold(const float* ts_e, const float* ts_v, int64_t nsamps,
                    double tsamp, double t_ref, float* out) {
     int64_t nseg = (nsamps + seg_len - 1) / seg_len;
     int32_t* pmap = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nfreq * seg_len));
+#pragma omp parallel for schedule(static)
     for (int64_t f = 0; f < nfreq; ++f)
         for (int64_t i = 0; i < seg_len; ++i) {
             double pt = (double)i * tsamp - t_ref;
             pmap[f * seg_len + i] = (int32_t)orc_phase_idx_int(pt, freqs[f], nbins, 0.0);
         }
     memset(out, 0, sizeof(float) * (size_t)(nseg * nfreq * 2 * nbins));
+#pragma omp parallel for schedule(static)
     for (int64_t s = 0; s < nseg; ++s) {
         int64_t start = s * seg_len;
         int64_t len = nsamps - start < seg_len ? nsamps - start : seg_len;
@@ -173,6 +198,7 @@ void orc_unify_fold(const float* fold_in, int64_t nseg_in, int P, const int64_t*
     for (int k = 1; k < P; ++k) offs[k] = offs[k - 1] + counts_cur[k - 1];
     int64_t nseg_out = nseg_in / 2;
     int64_t rowlen = 2 * nbins;
+#pragma omp parallel for schedule(static)
     for (int64_t j = 0; j < ncur; ++j) {
         double pset[8];
         int64_t r = j;
@@ -235,8 +261,11 @@ void orc_boxcar_snr_2d(const float* norm, int64_t n, int64_t nbins, const int64_
                        int64_t nw, double stdnoise, float* out) {
     int64_t wmax = 0;
     for (int64_t k = 0; k < nw; ++k) if (widths[k] > wmax) wmax = widths[k];
-    float* ps = (float*)malloc(sizeof(float) * (size_t)(nbins + wmax));
     double inv_std = 1.0 / stdnoise;
+#pragma omp parallel
+    {
+    float* ps = (float*)malloc(sizeof(float) * (size_t)(nbins + wmax));
+#pragma omp for schedule(static)
     for (int64_t p = 0; p < n; ++p) {
         const float* x = norm + p * nbins;
         orc_circular_prefix_sum(x, nbins, ps, nbins + wmax);
@@ -253,6 +282,7 @@ void orc_boxcar_snr_2d(const float* norm, int64_t n, int64_t nbins, const int64_
         }
     }
     free(ps);
+    }
 }
 
 /* detection/scoring.py:249-282 snr_score_batch_func: folds [n][2][nbins] -> max S/N */
@@ -260,8 +290,11 @@ void orc_snr_score_batch(const float* folds, int64_t n, int64_t nbins, const int
                          int64_t nw, float* out) {
     int64_t wmax = 0;
     for (int64_t k = 0; k < nw; ++k) if (widths[k] > wmax) wmax = widths[k];
+#pragma omp parallel
+    {
     float* ps = (float*)malloc(sizeof(float) * (size_t)(nbins + wmax));
     float* fn = (float*)malloc(sizeof(float) * (size_t)nbins);
+#pragma omp for schedule(static)
     for (int64_t p = 0; p < n; ++p) {
         const float* e = folds + p * 2 * nbins;
         const float* v = e + nbins;
@@ -282,10 +315,12 @@ void orc_snr_score_batch(const float* folds, int64_t n, int64_t nbins, const int
     }
     free(fn);
     free(ps);
+    }
 }
 
 /* ffa.py:382-411 get_fold_norm (real path): e / sqrt(v) in FP32 */
 void orc_fold_norm(const float* folds, int64_t n, int64_t nbins, float* out) {
+#pragma omp parallel for schedule(static)
     for (int64_t p = 0; p < n; ++p)
         for (int64_t j = 0; j < nbins; ++j)
             out[p * nbins + j] = folds[p * 2 * nbins + j] / sqrtf(folds[p * 2 * nbins + nbins + j]);
@@ -300,6 +335,7 @@ void orc_brutefold_f64(const float* ts_e, const float* ts_v, int64_t nsamps,
                        double tsamp, double t_ref, double* out) {
     int64_t nseg = (nsamps + seg_len - 1) / seg_len;
     memset(out, 0, sizeof(double) * (size_t)(nseg * nfreq * 2 * nbins));
+#pragma omp parallel for schedule(static)
     for (int64_t f = 0; f < nfreq; ++f)
         for (int64_t i = 0; i < seg_len; ++i) {
             double pt = (double)i * tsamp - t_ref;
@@ -323,6 +359,7 @@ void orc_unify_fold_f64(const double* fold_in, int64_t nseg_in, int P, const int
     offs[0] = 0;
     for (int k = 1; k < P; ++k) offs[k] = offs[k - 1] + counts_cur[k - 1];
     int64_t nseg_out = nseg_in / 2, rowlen = 2 * nbins;
+#pragma omp parallel for schedule(static)
     for (int64_t j = 0; j < ncur; ++j) {
         double pset[8];
         int64_t r = j;
@@ -360,7 +397,10 @@ void orc_boxcar_snr_f64(const double* folds, int64_t n, int64_t nbins, const int
                         int64_t nw, double* out) {
     int64_t wmax = 0;
     for (int64_t k = 0; k < nw; ++k) if (widths[k] > wmax) wmax = widths[k];
+#pragma omp parallel
+    {
     double* ps = (double*)malloc(sizeof(double) * (size_t)(nbins + wmax));
+#pragma omp for schedule(static)
     for (int64_t p = 0; p < n; ++p) {
         const double* e = folds + p * 2 * nbins;
         const double* v = e + nbins;
@@ -380,6 +420,7 @@ void orc_boxcar_snr_f64(const double* folds, int64_t n, int64_t nbins, const int
         }
     }
     free(ps);
+    }
 }
 
 /* ---- BF16 storage emulation (FP32 arithmetic, every stored fold element rounded to
@@ -416,6 +457,7 @@ void orc_unify_fold_bf16(const uint16_t* fold_in, int64_t nseg_in, int P, const 
     offs[0] = 0;
     for (int k = 1; k < P; ++k) offs[k] = offs[k - 1] + counts_cur[k - 1];
     int64_t nseg_out = nseg_in / 2, rowlen = 2 * nbins;
+#pragma omp parallel for schedule(static)
     for (int64_t j = 0; j < ncur; ++j) {
         double pset[8];
         int64_t r = j;
