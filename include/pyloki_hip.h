@@ -182,6 +182,19 @@ int pyl_ffa_level(int32_t nparams, const double* params_dev, const int64_t* coun
                   const int64_t* counts_prev, const double* param_limits, int32_t ffa_level,
                   double tseg_brute, int32_t nbins, const float* fold_in_dev, int64_t nseg_in,
                   int32_t* table_dev, float* fold_out_dev, void* stream);
+/* pyl_ffa_level_noalloc: no counterpart in the reference. pyl_ffa_level for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops): it
+ * takes no workspace and calls no allocator; the resolve table lives in a fixed device
+ * array of the library, at most pyl_ffa_level_noalloc_max_coords() parameter sets
+ * (2^21, 32 MiB per GPU, held until the process ends). Larger levels are rejected with
+ * PYL_ERR_INVALID. Calls on one GPU share that array: calls that overlap on one GPU (two
+ * streams, two threads) race and must be serialised by the caller. */
+int pyl_ffa_level_noalloc(int32_t nparams, const double* params_dev, const int64_t* counts_cur,
+                          const int64_t* counts_prev, const double* param_limits,
+                          int32_t ffa_level, double tseg_brute, int32_t nbins,
+                          const float* fold_in_dev, int64_t nseg_in, float* fold_out_dev,
+                          void* stream);
+int64_t pyl_ffa_level_noalloc_max_coords(void);
 
 /* ------------------------------------------------------------------------------------
  * Boxcar matched-filter S/N.

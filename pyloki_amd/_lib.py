@@ -60,6 +60,9 @@ SIGNATURES: list[tuple[str, object, list[object]]] = [
     ("pyl_ffa_get_tile_rows", ctypes.c_int, []),
     ("pyl_ffa_level", ctypes.c_int,
      [_c_i32, _vp, _i64p, _i64p, _f64p, _c_i32, _c_dbl, _c_i32, _vp, _c_i64, _vp, _vp, _vp]),
+    ("pyl_ffa_level_noalloc", ctypes.c_int,
+     [_c_i32, _vp, _i64p, _i64p, _f64p, _c_i32, _c_dbl, _c_i32, _vp, _c_i64, _vp, _vp]),
+    ("pyl_ffa_level_noalloc_max_coords", ctypes.c_int64, []),
     ("pyl_ffa_shift_add_score", ctypes.c_int,
      [_vp, _c_i64, _c_i64, _vp, _c_i64, _c_i64, _c_i32, _i64p, _c_i32, _c_dbl, _vp, _vp]),
     ("pyl_ffa_shift_add_layout", ctypes.c_int,

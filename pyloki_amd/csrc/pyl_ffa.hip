@@ -18,8 +18,9 @@
 //   Rows of consecutive j share their source rows (the previous level's grid is about
 //   half as fine), so each block takes a contiguous run of j and the runs are dealt to
 //   the 8 XCDs in contiguous slabs: neighbours' rows are reused from the same L2.
-#include <atomic>
 #include <algorithm>
+#include <atomic>
+#include <mutex>
 
 #include "pyl_score.hpp"
 
@@ -816,3 +817,60 @@ extern "C" int pyl_ffa_set_tile_rows(int32_t rows) {
 }
 
 extern "C" int pyl_ffa_get_tile_rows(void) { return g_tile_outer.load(std::memory_order_relaxed); }
+
+// ---- One FFA level without any allocation -------------------------------------------
+// The resolve table of pyl_ffa_level lives in a file-scope device array instead of a
+// caller workspace: at most kNoAllocMaxCoords parameter sets (16 B each, 32 MiB), held by
+// every GPU the library runs on until the process ends. Host code never passes the array's
+// name to a kernel (that name is a host-side stand-in); it passes the device address that
+// hipGetSymbolAddress returns for the current device, looked up once per device on the
+// first call there. The resolve kernel writes every entry the shift-add reads, so nothing
+// needs clearing; two calls that overlap on one GPU race on the table.
+constexpr int64_t kNoAllocMaxCoords = (int64_t)1 << 21;
+constexpr int kNoAllocMaxDevices = 64;
+
+static __device__ __align__(256) int4 scratch[kNoAllocMaxCoords];
+
+static std::mutex g_scratch_mu;
+static int4* g_scratch_dev[kNoAllocMaxDevices] = {};
+
+static int scratch_table(int4** out) {
+  int dev = -1;
+  PYL_CHECK_HIP(hipGetDevice(&dev));
+  PYL_REQUIRE(dev >= 0 && dev < kNoAllocMaxDevices, "device %d is beyond the %d supported", dev,
+              kNoAllocMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_dev[dev] == nullptr) {
+    void* p = nullptr;
+    PYL_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    PYL_REQUIRE(p != nullptr, "no device address for the level table on device %d", dev);
+    g_scratch_dev[dev] = static_cast<int4*>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return PYL_OK;
+}
+
+extern "C" int64_t pyl_ffa_level_noalloc_max_coords(void) { return kNoAllocMaxCoords; }
+
+extern "C" int pyl_ffa_level_noalloc(int32_t nparams, const double* params_dev,
+                                     const int64_t* counts_cur, const int64_t* counts_prev,
+                                     const double* param_limits, int32_t ffa_level,
+                                     double tseg_brute, int32_t nbins, const float* fold_in_dev,
+                                     int64_t nseg_in, float* fold_out_dev, void* stream) {
+  PYL_REQUIRE(nparams >= 1 && nparams <= kMaxP, "nparams must be in [1, 5], got %d", nparams);
+  PYL_REQUIRE(counts_cur != nullptr, "null pointer argument");
+  int64_t ncur = 1;
+  for (int k = 0; k < nparams; ++k) {
+    PYL_REQUIRE(counts_cur[k] > 0, "grid counts must be positive");
+    PYL_REQUIRE(counts_cur[k] <= kNoAllocMaxCoords && ncur <= kNoAllocMaxCoords / counts_cur[k],
+                "the level has more than %lld parameter sets, the maximum without allocation",
+                (long long)kNoAllocMaxCoords);
+    ncur *= counts_cur[k];
+  }
+  int4* table = nullptr;
+  const int rc = scratch_table(&table);
+  if (rc != PYL_OK) return rc;
+  return pyl_ffa_level(nparams, params_dev, counts_cur, counts_prev, param_limits, ffa_level,
+                       tseg_brute, nbins, fold_in_dev, nseg_in, reinterpret_cast<int32_t*>(table),
+                       fold_out_dev, stream);
+}

@@ -44,6 +44,22 @@ def test_abi_version_and_error_path_without_gpu():
     assert b"nparams" in lib.pyl_last_error()
 
 
+def test_noalloc_level_rejects_oversized_levels_on_the_host():
+    """The no-allocation FFA level refuses a level beyond its fixed table before it looks
+    for a device."""
+    import ctypes
+
+    import numpy as np
+
+    lib = _lib.load()
+    cmax = int(lib.pyl_ffa_level_noalloc_max_coords())
+    counts = np.array([3, cmax], dtype=np.int64)
+    p = counts.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+    rc = lib.pyl_ffa_level_noalloc(2, None, p, p, None, 1, 1.0, 64, None, 2, None, None)
+    assert rc == _lib.PYL_ERR_INVALID
+    assert b"maximum without allocation" in lib.pyl_last_error()
+
+
 def test_struct_layouts_match_the_header(tmp_path):
     """The ctypes mirrors of the ABI structs have the C compiler's size and field offsets."""
     import shutil
