@@ -58,6 +58,22 @@ int pyl_brutefold_execute(const pyl_bf_plan* plan, const float* ts_e_dev, const 
 int pyl_brutefold_execute_bf16(const pyl_bf_plan* plan, const float* ts_e_dev,
                                const float* ts_v_dev, int64_t nsamps, uint16_t* fold_dev,
                                void* stream);
+/* A batch of series sharing the plan in one launch (per 32 series): ts_e / ts_v are host
+ * arrays of nseries device pointers, nsamps a host array of lengths. Series i's rows are
+ * segments [sum_{j<i} ceil(nsamps[j]/seg_len), ...) of fold_dev, in the given order; each
+ * series equals its own pyl_brutefold_execute. (FFATaylorDPFuncts.init per series.) */
+int pyl_brutefold_execute_batch(const pyl_bf_plan* plan, const float* const* ts_e_dev,
+                                const float* const* ts_v_dev, const int64_t* nsamps,
+                                int32_t nseries, float* fold_dev, void* stream);
+int pyl_brutefold_execute_batch_bf16(const pyl_bf_plan* plan, const float* const* ts_e_dev,
+                                     const float* const* ts_v_dev, const int64_t* nsamps,
+                                     int32_t nseries, uint16_t* fold_dev, void* stream);
+/* Phase bins of the plan (nfreq x seg_len FP64 evaluations of psr_utils.py:49-56) that a
+ * +-1 ulp change of the FP64 phase would move to another bin: the bound on disagreement
+ * with an evaluation that rounds differently in the last bit. */
+int pyl_brutefold_plan_boundary_count(const pyl_bf_plan* plan, int64_t* count_out);
+/* Launch shape: segments staged per block (0 = default, 1, 2, 4); never changes a result. */
+int pyl_brutefold_set_segments_per_block(int32_t segs);
 
 /* ------------------------------------------------------------------------------------
  * Complex (Fourier-domain) FFA, use_fourier=True.

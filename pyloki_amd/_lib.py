@@ -41,6 +41,11 @@ SIGNATURES: list[tuple[str, object, list[object]]] = [
     ("pyl_brutefold_plan_destroy", ctypes.c_int, [_vp]),
     ("pyl_brutefold_execute", ctypes.c_int, [_vp, _vp, _vp, _c_i64, _vp, _vp]),
     ("pyl_brutefold_execute_bf16", ctypes.c_int, [_vp, _vp, _vp, _c_i64, _vp, _vp]),
+    ("pyl_brutefold_execute_batch", ctypes.c_int, [_vp, _vp, _vp, _i64p, _c_i32, _vp, _vp]),
+    ("pyl_brutefold_execute_batch_bf16", ctypes.c_int,
+     [_vp, _vp, _vp, _i64p, _c_i32, _vp, _vp]),
+    ("pyl_brutefold_plan_boundary_count", ctypes.c_int, [_vp, _i64p]),
+    ("pyl_brutefold_set_segments_per_block", ctypes.c_int, [_c_i32]),
     ("pyl_cfold_plan_create", ctypes.c_int,
      [_f64p, _c_i64, _c_i64, _c_i32, _c_dbl, _c_dbl, _vp, ctypes.POINTER(_vp)]),
     ("pyl_cfold_plan_destroy", ctypes.c_int, [_vp]),

@@ -289,10 +289,7 @@ def ffa_batch(plan: FFAPlan, series: list[tuple[torch.Tensor, torch.Tensor]],
         raise ValueError(msg)
     dev = series[0][0].device
     fold = torch.empty((sum(nseg), plan.bf.nfreq, 2, plan.cfg.nbins), dtype=dtype, device=dev)
-    off = 0
-    for i, n in zip(order, nseg):
-        plan.bf.fold(series[i][0], series[i][1], out=fold[off:off + n])
-        off += n
+    plan.bf.fold_batch([series[i] for i in order], fold)
     results: dict[int, torch.Tensor] = {}
     level = 0
     nbins = plan.cfg.nbins

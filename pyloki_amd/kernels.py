@@ -61,6 +61,43 @@ class BruteFoldPlan:
         _lib.check(fn(self._handle, ptr(ts_e), ptr(ts_v), nsamps, ptr(out), stream_ptr()))
         return out
 
+    def fold_batch(self, series: list[tuple[torch.Tensor, torch.Tensor]], out: torch.Tensor,
+                   ) -> torch.Tensor:
+        """Fold many series (same plan) in one launch per 32 series: series i's rows are
+        the next ceil(len_i / seg_len) segments of ``out`` [sum nseg, nfreq, 2, nbins], in
+        the given order."""
+        n = len(series)
+        if n == 0:
+            return out
+        nseg = 0
+        for e, v in series:
+            if e.shape != v.shape or e.dim() != 1 or e.dtype != torch.float32 or \
+                    v.dtype != torch.float32 or not e.is_contiguous() or not v.is_contiguous():
+                msg = "ts_e, ts_v must be contiguous 1-D float32 of the same length."
+                raise ValueError(msg)
+            nseg += -(-e.shape[0] // self.seg_len)
+        shape = (nseg, self.nfreq, 2, self.nbins)
+        if tuple(out.shape) != shape or out.dtype not in (torch.float32, torch.bfloat16) or \
+                not out.is_contiguous():
+            msg = f"out must be contiguous float32 or bfloat16 of shape {shape}"
+            raise ValueError(msg)
+        pe = (ctypes.c_void_p * n)(*[e.data_ptr() for e, _ in series])
+        pv = (ctypes.c_void_p * n)(*[v.data_ptr() for _, v in series])
+        lens = _lib.i64_array([e.shape[0] for e, _ in series])
+        lib = _lib.load()
+        fn = (lib.pyl_brutefold_execute_batch if out.dtype == torch.float32
+              else lib.pyl_brutefold_execute_batch_bf16)
+        _lib.check(fn(self._handle, ctypes.cast(pe, ctypes.c_void_p),
+                      ctypes.cast(pv, ctypes.c_void_p), lens, n, ptr(out), stream_ptr()))
+        return out
+
+    @property
+    def boundary_count(self) -> int:
+        """Phase bins that a +-1 ulp change of their FP64 phase would move."""
+        c = ctypes.c_int64(0)
+        _lib.check(_lib.load().pyl_brutefold_plan_boundary_count(self._handle, ctypes.byref(c)))
+        return int(c.value)
+
     def __del__(self) -> None:
         h = getattr(self, "_handle", None)
         if h is not None and h.value:

@@ -299,3 +299,43 @@ def test_ffa_batch_matches_single_series(gpu_device, lens):
                                  use_fourier=False)
         exp = orc.ffa_search_snr(e.cpu().numpy(), v.cpu().numpy(), sub)
         np.testing.assert_array_equal(snr.cpu().numpy(), exp.reshape(-1, len(widths)))
+
+
+@pytest.mark.parametrize("seg_len,nbins,segs", [
+    (4096, 64, 0), (4096, 64, 1), (2048, 64, 4), (8192, 32, 0), (1000, 100, 2),
+    (16384, 32, 0),   # beyond the LDS budget: the global path
+    (70000, 64, 0),   # more than 65536 samples per segment: u32 sample lists
+])
+def test_brutefold_batch_bit_exact(gpu_device, seg_len, nbins, segs):
+    """A batch of series of different lengths (partial last segments included) in one
+    launch: every series' rows equal the oracle's fold of that series."""
+    _case(f"brutefold batch seg_len={seg_len} nbins={nbins} segs_per_block={segs}")
+    lib = _lib.load()
+    rng = np.random.default_rng(seg_len + nbins)
+    lens = [seg_len * 5 + 123, seg_len * 3, seg_len * 7 - 5, seg_len, 17] if seg_len < 70000 \
+        else [seg_len * 2 + 9, seg_len]
+    tsamp = 64e-6
+    freqs = np.sort(rng.uniform(150.0, 210.0, size=11))
+    t_ref = seg_len * tsamp / 2
+    plan = kernels.BruteFoldPlan(freqs, seg_len, nbins, tsamp, t_ref)
+    host = [(rng.normal(size=n).astype(np.float32), rng.uniform(0.5, 1.5, n).astype(np.float32))
+            for n in lens]
+    dev = [(_dev(e), _dev(v)) for e, v in host]
+    nseg = [-(-n // seg_len) for n in lens]
+    _lib.check(lib.pyl_brutefold_set_segments_per_block(segs))
+    try:
+        out = torch.empty((sum(nseg), len(freqs), 2, nbins), dtype=torch.float32, device="cuda")
+        plan.fold_batch(dev, out)
+        out16 = torch.empty_like(out, dtype=torch.bfloat16)
+        plan.fold_batch(dev, out16)
+    finally:
+        _lib.check(lib.pyl_brutefold_set_segments_per_block(0))
+    got = out.cpu().numpy()
+    got16 = out16.view(torch.int16).cpu().numpy().astype(np.uint16)
+    off = 0
+    for (e, v), n in zip(host, nseg):
+        exp = orc.brutefold(e, v, freqs, seg_len, nbins, tsamp, t_ref)
+        np.testing.assert_array_equal(got[off:off + n], exp)
+        np.testing.assert_array_equal(got16[off:off + n], orc.round_bf16(exp))
+        off += n
+    assert plan.boundary_count >= 0
