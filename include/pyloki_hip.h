@@ -194,6 +194,17 @@ int pyl_ffa_shift_add2_score_layout(const float* fold_in_dev, int32_t in_layout,
                                     int64_t n_inner, int32_t nbins, const int64_t* widths,
                                     int32_t nw, double stdnoise, float* snr_out_dev,
                                     void* stream);
+/* pyl_ffa_shift_add2_layout: two consecutive levels in one pass (FP32, input
+ * [nseg_in][ncoords_prev2][2][nbins] or its PYL_FOLD_WRAP form, nseg_in a multiple of 4):
+ * equal bit for bit to pyl_ffa_shift_add with table_prev (level L+1) followed by
+ * pyl_ffa_shift_add with table_dev (level L+2), but the intermediate fold is never stored.
+ * Output [nseg_in/4][ncoords_cur][2][nbins] (dense) or [..][2][nbins + 4] (wrap).
+ * No reference counterpart: the reference runs unify_fold once per level
+ * (ffa.py:332-380). */
+int pyl_ffa_shift_add2_layout(const float* fold_in_dev, int32_t in_layout, int64_t nseg_in,
+                              int64_t ncoords_prev2, const int32_t* table_prev_dev, int64_t ncoords_prev,
+                              const int32_t* table_dev, int64_t ncoords_cur, int32_t nbins,
+                              float* fold_out_dev, int32_t out_layout, void* stream);
 int pyl_ffa_level(int32_t nparams, const double* params_dev, const int64_t* counts_cur,
                   const int64_t* counts_prev, const double* param_limits, int32_t ffa_level,
                   double tseg_brute, int32_t nbins, const float* fold_in_dev, int64_t nseg_in,

@@ -75,6 +75,8 @@ SIGNATURES: list[tuple[str, object, list[object]]] = [
     ("pyl_ffa_shift_add_score_layout", ctypes.c_int,
      [_vp, _c_i32, _c_i64, _c_i64, _vp, _c_i64, _c_i64, _c_i32, _i64p, _c_i32, _c_dbl, _vp,
       _vp]),
+    ("pyl_ffa_shift_add2_layout", ctypes.c_int,
+     [_vp, _c_i32, _c_i64, _c_i64, _vp, _c_i64, _vp, _c_i64, _c_i32, _vp, _c_i32, _vp]),
     ("pyl_ffa_shift_add2_score_layout", ctypes.c_int,
      [_vp, _c_i32, _c_i64, _c_i64, _vp, _c_i64, _vp, _c_i64, _c_i64, _c_i32, _i64p, _c_i32,
       _c_dbl, _vp, _vp]),

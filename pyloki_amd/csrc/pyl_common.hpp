@@ -116,6 +116,15 @@ __device__ __forceinline__ float4 norm_ev4(float4 e, float4 v) {
   return make_float4(e.x / sqrtf(v.x), e.y / sqrtf(v.y), e.z / sqrtf(v.z), e.w / sqrtf(v.w));
 }
 
+// max(m, a, b) as the reference's sequential `if d > m: m = d` over a then b leaves it,
+// for every maximum other than a zero: both skip NaN operands (arithmetic NaNs are quiet,
+// and IEEE maxNum returns the other operand) and agree on every non-zero value. A zero
+// maximum may differ in sign (the sequential form keeps the first zero it meets), so
+// callers recompute a zero maximum sequentially. One v_max3_f32.
+__device__ __forceinline__ float max3_seq(float m, float a, float b) {
+  return fmaxf(m, fmaxf(a, b));
+}
+
 // x**n for an integer n by binary powering. The reference raises floats to integer
 // arrays inside numba fastmath code (`delta_t**powers`, transforms.py:43;
 // `(tobs - t_ref)**(k+1)`, psr_utils.py:89); LLVM's fast-math simplification turns

@@ -216,6 +216,59 @@ ffa_shift_add_kernel(const T* __restrict__ in, int64_t ncoords_prev,
   if (kWrapOut && c4 == 0) store4<T>(o + B, 0, r);  // the wrap copy of bins 0..3
 }
 
+// Two levels in one pass (FP32, dense or wrap input): output row (q, j) of level L+2 from the
+// four rows of segments 4q .. 4q+3 at level L that feed it. table_b[j] names the level-L+1
+// rows (tail of segment 2q, head of segment 2q+1) and their shifts; table_a names, for
+// each of those, its two level-L rows and shifts. The rotations compose (rotating a sum is
+// the sum of the rotated rows), so the tail part is the pair sum the intermediate level
+// would store, rounded once, at the composed shifts; likewise the head part; then
+// tail + head. Equal bit for bit to two ffa_shift_add_kernel launches; the intermediate
+// fold never reaches HBM.
+template <int B, bool kWrapIn, bool kWrapOut>
+__global__ void __launch_bounds__(256)
+ffa_shift_add2_kernel(const float* __restrict__ in, int64_t ncoords_prev2,
+                      const int4* __restrict__ table_a, const int4* __restrict__ table_b,
+                      int64_t ncoords_cur, int64_t nseg_out, int64_t nblk_j,
+                      float* __restrict__ out) {
+  constexpr int CPR = B / 4;
+  constexpr int TPR = 2 * CPR;
+  constexpr int RPB = 256 / TPR;
+  using Li = RowLayout<B, kWrapIn>;
+  using Lo = RowLayout<B, kWrapOut>;
+  constexpr int64_t kRow = Li::kRow;
+  const int64_t vb = xcd_slab_block(blockIdx.x, gridDim.x);
+  const int64_t q = vb / nblk_j;
+  const int64_t jb = vb - q * nblk_j;
+  if (q >= nseg_out) return;
+  const int t = threadIdx.x;
+  const int64_t j = jb * RPB + t / TPR;
+  if (j >= ncoords_cur) return;
+  const int e4 = t % TPR;
+  const int comp = e4 / CPR;
+  const int c4 = e4 % CPR;
+  const int4 tb = table_b[j];
+  const int4 ta = table_a[tb.x];
+  const int4 ha = table_a[tb.z];
+  const int64_t seg = ncoords_prev2 * kRow;
+  const float* base = in + (4 * q) * seg + comp * Li::kComp;
+  const float* t1 = base + (int64_t)ta.x * kRow;
+  const float* t2 = base + seg + (int64_t)ta.z * kRow;
+  const float* h1 = base + 2 * seg + (int64_t)ha.x * kRow;
+  const float* h2 = base + 3 * seg + (int64_t)ha.z * kRow;
+  const float4 a1 = row_chunk<B, float, kWrapIn>(t1, c4, (tb.y + ta.y) & (B - 1));
+  const float4 a2 = row_chunk<B, float, kWrapIn>(t2, c4, (tb.y + ta.w) & (B - 1));
+  const float4 b1 = row_chunk<B, float, kWrapIn>(h1, c4, (tb.w + ha.y) & (B - 1));
+  const float4 b2 = row_chunk<B, float, kWrapIn>(h2, c4, (tb.w + ha.w) & (B - 1));
+  float4 r;
+  r.x = (a1.x + a2.x) + (b1.x + b2.x);
+  r.y = (a1.y + a2.y) + (b1.y + b2.y);
+  r.z = (a1.z + a2.z) + (b1.z + b2.z);
+  r.w = (a1.w + a2.w) + (b1.w + b2.w);
+  float* o = out + (q * ncoords_cur + j) * Lo::kRow + comp * Lo::kComp;
+  store4<float>(o, c4, r);
+  if (kWrapOut && c4 == 0) store4<float>(o + B, 0, r);  // the wrap copy of bins 0..3
+}
+
 // Generic nbins: one thread per output element.
 __global__ void ffa_shift_add_generic_kernel(const float* __restrict__ in, int64_t ncoords_prev,
                                              const int4* __restrict__ table, int64_t ncoords_cur,
@@ -450,12 +503,47 @@ ffa_shift_add_score_rounds_kernel(const T* __restrict__ in, int64_t ncoords_prev
       const float* Sw = S + k.widths[m] * kLds + tid;
       float dm = dmax[m];
 #pragma unroll
-      for (int b = 0; b < kSeg; ++b) {
-        const float d = Sw[b * kLds] - x[rd * kSeg + b];
-        dm = d > dm ? d : dm;
+      for (int b = 0; b < kSeg; b += 2) {
+        const float d0 = Sw[b * kLds] - x[rd * kSeg + b];
+        const float d1 = Sw[(b + 1) * kLds] - x[rd * kSeg + b + 1];
+        dm = pyl::max3_seq(dm, d0, d1);
       }
       dmax[m] = dm;
     }
+  }
+  // A zero maximum takes the sign of the first zero difference in the reference's order
+  // (later zeros never replace it): a second pass over the windows finds it. Real data
+  // reach this only for all-zero stretches of a profile.
+  uint32_t zero = 0;
+#pragma unroll 1
+  for (int m = 0; m < nw; ++m) zero |= (dmax[m] == 0.0f ? 1u : 0u) << m;
+  if (zero) {
+    uint32_t neg = 0;
+#pragma unroll
+    for (int rd = 0; rd < kRounds; ++rd) {
+#pragma unroll
+      for (int jw = 1; jw < kWin; ++jw) {
+        const int g = rd * kSeg + jw;
+        S[jw * kLds + tid] = g < B ? x[g < B ? g : 0] : (x[g >= B ? g - B : 0] + total);
+      }
+      S[tid] = x[rd * kSeg];
+#pragma unroll 1
+      for (int m = 0; m < nw; ++m) {
+        if (!((zero >> m) & 1u)) continue;
+        const float* Sw = S + k.widths[m] * kLds + tid;
+        for (int b = 0; b < kSeg; ++b) {
+          const float d = Sw[b * kLds] - S[b * kLds + tid];
+          if (d == 0.0f) {
+            neg |= (__float_as_uint(d) >> 31) << m;
+            zero &= ~(1u << m);
+            break;
+          }
+        }
+      }
+    }
+#pragma unroll 1
+    for (int m = 0; m < nw; ++m)
+      if (dmax[m] == 0.0f) dmax[m] = ((neg >> m) & 1u) ? -0.0f : 0.0f;
   }
   float* o = out + (q * ncoords_cur + j) * nw;
 #pragma unroll 1
@@ -789,6 +877,56 @@ extern "C" int pyl_ffa_shift_add_bf16(const uint16_t* fold_in, int64_t nseg_in,
     default: launch_shift_add<256, uint16_t, false, false>(fold_in, ncoords_prev, tb, ncoords_cur, nseg_out, fold_out, s); break;
   }
   PYL_CHECK_LAUNCH("ffa_shift_add_kernel<bf16>");
+  return PYL_OK;
+}
+
+template <int B>
+static void launch_shift_add2(const float* in, bool wrap_in, int64_t nprev2, const int4* ta,
+                              const int4* tb, int64_t ncur, int64_t nseg_out, float* out,
+                              bool wrap_out, hipStream_t s) {
+  constexpr int RPB = 256 / (B / 2);
+  const int64_t nblk_j = pyl::ceil_div(ncur, RPB);
+  const unsigned nblocks = (unsigned)(nblk_j * nseg_out);
+#define PYL_SA2(WI, WO)                                                                 \
+  ffa_shift_add2_kernel<B, WI, WO><<<nblocks, 256, 0, s>>>(in, nprev2, ta, tb, ncur,     \
+                                                          nseg_out, nblk_j, out)
+  if (wrap_in) {
+    if (wrap_out) PYL_SA2(true, true);
+    else PYL_SA2(true, false);
+  } else {
+    if (wrap_out) PYL_SA2(false, true);
+    else PYL_SA2(false, false);
+  }
+#undef PYL_SA2
+}
+
+extern "C" int pyl_ffa_shift_add2_layout(const float* fold_in, int32_t in_layout, int64_t nseg_in,
+                                         int64_t ncoords_prev2, const int32_t* table_prev,
+                                         int64_t ncoords_prev, const int32_t* table_dev,
+                                         int64_t ncoords_cur, int32_t nbins, float* fold_out,
+                                         int32_t out_layout, void* stream) {
+  PYL_REQUIRE(fold_in && table_prev && table_dev && fold_out, "null device pointer");
+  PYL_REQUIRE(nseg_in >= 4 && nseg_in % 4 == 0,
+              "two fused levels need a multiple of 4 segments, got %lld", (long long)nseg_in);
+  PYL_REQUIRE(ncoords_prev2 > 0 && ncoords_prev > 0 && ncoords_cur > 0, "empty parameter grid");
+  PYL_REQUIRE(nbins == 32 || nbins == 64 || nbins == 128 || nbins == 256,
+              "two fused levels support nbins in {32, 64, 128, 256}, got %d", nbins);
+  PYL_REQUIRE(valid_layout(in_layout) && valid_layout(out_layout), "invalid fold layout");
+  PYL_REQUIRE(ncoords_prev2 * nseg_in <= ((int64_t)1 << 40), "fold too large");
+  const int64_t nseg_out = nseg_in / 4;
+  PYL_REQUIRE(pyl::ceil_div(ncoords_cur, 256 / (nbins / 2)) * nseg_out <= 0x7fffffff,
+              "too many output rows for one launch");
+  hipStream_t s = pyl::as_stream(stream);
+  const int4* ta = reinterpret_cast<const int4*>(table_prev);
+  const int4* tb = reinterpret_cast<const int4*>(table_dev);
+  const bool wi = in_layout == PYL_FOLD_WRAP, wo = out_layout == PYL_FOLD_WRAP;
+  switch (nbins) {
+    case 32: launch_shift_add2<32>(fold_in, wi, ncoords_prev2, ta, tb, ncoords_cur, nseg_out, fold_out, wo, s); break;
+    case 64: launch_shift_add2<64>(fold_in, wi, ncoords_prev2, ta, tb, ncoords_cur, nseg_out, fold_out, wo, s); break;
+    case 128: launch_shift_add2<128>(fold_in, wi, ncoords_prev2, ta, tb, ncoords_cur, nseg_out, fold_out, wo, s); break;
+    default: launch_shift_add2<256>(fold_in, wi, ncoords_prev2, ta, tb, ncoords_cur, nseg_out, fold_out, wo, s); break;
+  }
+  PYL_CHECK_LAUNCH("ffa_shift_add2_kernel");
   return PYL_OK;
 }
 

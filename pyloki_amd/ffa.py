@@ -345,52 +345,53 @@ def ffa_batch(plan: FFAPlan, series: list[tuple[torch.Tensor, torch.Tensor]],
 
 def _batch_two_levels(plan: FFAPlan, fold: torch.Tensor, nseg: list[int], order: list[int],
                       widths: np.ndarray, stdnoise: float, results: dict) -> None:
-    """The level loop of ``ffa_batch`` with every series' last two levels fused with its
-    scoring (``pyl_ffa_shift_add2_score_layout``): the level before them (a series with 8
-    segments) writes the "wrap" row layout, a series with 4 segments runs its last two
-    levels and scoring in one launch, and only a series with 2 segments after the brute
-    fold uses the one-level fused kernel. ``fold`` holds the brute folds, longest series
-    first; the series with one segment have been scored already."""
+    """The level loop of ``ffa_batch`` for FP32 folds with nbins 32/64 and at most 16
+    widths. Series of one length (contiguous in ``fold``, longest first) run together:
+    their levels two at a time (``pyl_ffa_shift_add2_layout``, the intermediate fold never
+    stored; an odd count ends with one single level), every fold after the brute fold in
+    the "wrap" row layout (each rotated 4-bin chunk one load), and the last two levels
+    fused with scoring (``pyl_ffa_shift_add2_score_layout``). A series with 2
+    segments after the brute fold uses the one-level fused kernel; the series with one
+    segment have been scored already."""
     nser = len(nseg)
-    level = 0
-    fin2_src: torch.Tensor | None = None  # the previous level's output for the 8-segment group
-    while True:
-        big = [k for k in range(nser) if nseg[k] > 8]
-        pre2 = [k for k in range(nser) if nseg[k] == 8]
-        fin2 = [k for k in range(nser) if nseg[k] == 4]
-        fin = [k for k in range(nser) if nseg[k] == 2]
-        if not big and not pre2 and not fin2 and not fin:
-            break
-        lp = plan.levels[level]
-        nb = sum(nseg[k] for k in big)
-        npre2 = 8 * len(pre2)
-        off = nb + npre2
-        if fin2:
-            lp1 = plan.levels[level + 1]
-            if fin2_src is not None:
-                part, lay = fin2_src, "wrap"
-            else:  # series that had 4 segments after the brute fold
-                part, lay = fold[off:off + 4 * len(fin2)], "dense"
-                off += 4 * len(fin2)
-            snr = kernels.ffa_shift_add2_score(part, lp.table, lp1.table, lp.ncoords,
-                                               lp1.ncoords, widths, stdnoise,
-                                               n_inner=int(lp1.counts_cur[-1]), in_layout=lay)
-            for m, k in enumerate(fin2):
-                results[order[k]] = snr[m]
-        if fin:  # series that had 2 segments after the brute fold
-            snr = kernels.ffa_shift_add_score(fold[off:off + 2 * len(fin)], lp.table,
-                                              lp.ncoords, widths, stdnoise,
-                                              n_inner=int(lp.counts_cur[-1]))
-            for m, k in enumerate(fin):
-                results[order[k]] = snr[m]
-        fin2_src = None
-        if pre2:
-            fin2_src = kernels.ffa_shift_add(fold[nb:nb + npre2], lp.table, lp.ncoords,
-                                             out_layout="wrap")
-        if big:
-            fold = kernels.ffa_shift_add(fold[:nb], lp.table, lp.ncoords)
-        for k in big + pre2:
-            nseg[k] //= 2
-        for k in fin2 + fin:
-            nseg[k] = 1
-        level += 1
+    off, k = 0, 0
+    while k < nser:
+        n0 = nseg[k]
+        k1 = k
+        while k1 < nser and nseg[k1] == n0:
+            k1 += 1
+        part = fold[off:off + (k1 - k) * n0]
+        off += (k1 - k) * n0
+        if n0 > 1:
+            snr = _class_levels(plan, part, n0, widths, stdnoise)
+            for m in range(k1 - k):
+                results[order[k + m]] = snr[m]
+        k = k1
+
+
+def _class_levels(plan: FFAPlan, fold: torch.Tensor, n0: int, widths: np.ndarray,
+                  stdnoise: float) -> torch.Tensor:
+    """Every level of series with ``n0`` segments each (concatenated in ``fold``) and the
+    per-width S/N of their last level: [nseries, ncoords_final, n_w]."""
+    left = int(math.log2(n0))
+    level, lay = 0, "dense"
+    while left > 2:  # every level written in the wrap layout, read with one load per chunk
+        if left == 3:
+            lp = plan.levels[level]
+            fold = kernels.ffa_shift_add(fold, lp.table, lp.ncoords, in_layout=lay,
+                                         out_layout="wrap")
+            level, left = level + 1, left - 1
+        else:
+            a, b = plan.levels[level], plan.levels[level + 1]
+            fold = kernels.ffa_shift_add2(fold, a.table, b.table, a.ncoords, b.ncoords,
+                                          in_layout=lay, out_layout="wrap")
+            level, left = level + 2, left - 2
+        lay = "wrap"
+    if left == 2:
+        a, b = plan.levels[level], plan.levels[level + 1]
+        return kernels.ffa_shift_add2_score(fold, a.table, b.table, a.ncoords, b.ncoords,
+                                            widths, stdnoise, n_inner=int(b.counts_cur[-1]),
+                                            in_layout=lay)
+    lp = plan.levels[level]
+    return kernels.ffa_shift_add_score(fold, lp.table, lp.ncoords, widths, stdnoise,
+                                       n_inner=int(lp.counts_cur[-1]), in_layout=lay)

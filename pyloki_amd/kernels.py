@@ -325,6 +325,36 @@ def ffa_shift_add_score(fold_in: torch.Tensor, table: torch.Tensor, ncoords_cur:
     return out
 
 
+def ffa_shift_add2(fold_in: torch.Tensor, table_prev: torch.Tensor, table: torch.Tensor,
+                   ncoords_prev: int, ncoords_cur: int, out: torch.Tensor | None = None, *,
+                   in_layout: str = "dense", out_layout: str = "dense") -> torch.Tensor:
+    """Two levels in one pass: [nseg, nprev2, 2, B(+4)] -> [nseg/4, ncur, 2, B(+4)], equal
+    bit for bit to ``ffa_shift_add`` with ``table_prev`` then with ``table``; the
+    intermediate fold is never stored. FP32, nbins 32..256 (powers of two)."""
+    if fold_in.dtype != torch.float32:
+        msg = "two fused levels need FP32 folds"
+        raise ValueError(msg)
+    nseg, nprev2, nbins = _check_level_input(fold_in, table_prev, ncoords_prev, in_layout)
+    if nseg % 4:
+        msg = f"two fused levels need a multiple of 4 segments, got {nseg}"
+        raise ValueError(msg)
+    if tuple(table.shape) != (ncoords_cur, 4):
+        msg = "table does not match ncoords_cur"
+        raise ValueError(msg)
+    lay_out = _layout(out_layout)
+    shape = (nseg // 4, ncoords_cur, 2, nbins + (WRAP_PAD if lay_out == _lib.PYL_FOLD_WRAP
+                                                  else 0))
+    if out is None:
+        out = torch.empty(shape, dtype=torch.float32, device=fold_in.device)
+    elif tuple(out.shape) != shape or out.dtype != torch.float32 or not out.is_contiguous():
+        msg = f"out must be contiguous float32 with shape {shape}"
+        raise ValueError(msg)
+    _lib.check(_lib.load().pyl_ffa_shift_add2_layout(
+        ptr(fold_in), _layout(in_layout), nseg, nprev2, ptr(table_prev), ncoords_prev,
+        ptr(table), ncoords_cur, nbins, ptr(out), lay_out, stream_ptr()))
+    return out
+
+
 def ffa_shift_add2_score(fold_in: torch.Tensor, table_prev: torch.Tensor, table: torch.Tensor,
                          ncoords_prev: int, ncoords_cur: int, widths: np.ndarray,
                          stdnoise: float = 1.0, out: torch.Tensor | None = None,

@@ -339,3 +339,81 @@ def test_brutefold_batch_bit_exact(gpu_device, seg_len, nbins, segs):
         np.testing.assert_array_equal(got16[off:off + n], orc.round_bf16(exp))
         off += n
     assert plan.boundary_count >= 0
+
+
+@pytest.mark.parametrize("P,nbins", [(2, 64), (2, 32), (3, 128), (1, 256)])
+def test_shift_add2_equals_two_levels(gpu_device, P, nbins):
+    """Two levels in one pass (dense and wrap output) equal two single levels bit for bit,
+    at every pair of consecutive levels of a small search."""
+    _case(f"shift_add2 P={P} nbins={nbins}")
+    nsamps = 2**17
+    cfg = _cfg(P, nsamps, nbins, nsamps // 64)
+    plan = FFAPlan(cfg)
+    rng = np.random.default_rng(P * nbins)
+    e = _dev(rng.normal(size=nsamps).astype(np.float32))
+    v = _dev(rng.uniform(0.5, 1.5, nsamps).astype(np.float32))
+    folds = plan.run(e, v, keep=True)
+    for lv in range(0, len(plan.levels) - 1):
+        if folds[lv].shape[0] % 4:
+            continue
+        a, b = plan.levels[lv], plan.levels[lv + 1]
+        got = kernels.ffa_shift_add2(folds[lv], a.table, b.table, a.ncoords, b.ncoords)
+        np.testing.assert_array_equal(got.cpu().numpy(), folds[lv + 2].cpu().numpy())
+        gw = kernels.ffa_shift_add2(folds[lv], a.table, b.table, a.ncoords, b.ncoords,
+                                    out_layout="wrap").cpu().numpy()
+        exp = folds[lv + 2].cpu().numpy()
+        np.testing.assert_array_equal(gw[..., :nbins], exp)
+        np.testing.assert_array_equal(gw[..., nbins:], exp[..., :4])
+        src = folds[lv]
+        wrap_in = torch.cat([src, src[..., :4]], dim=-1).contiguous()
+        gi = kernels.ffa_shift_add2(wrap_in, a.table, b.table, a.ncoords, b.ncoords,
+                                    in_layout="wrap", out_layout="wrap").cpu().numpy()
+        np.testing.assert_array_equal(gi, gw)
+
+
+def _np_level(fold: np.ndarray, table: np.ndarray) -> np.ndarray:
+    """One level in numpy FP32 from a resolve table ([tail row, shift, head row, shift])."""
+    nseg, nprev, _, B = fold.shape
+    out = np.empty((nseg // 2, len(table), 2, B), dtype=np.float32)
+    for q in range(nseg // 2):
+        for j, (rt, st, rh, sh) in enumerate(table):
+            out[q, j] = np.roll(fold[2 * q, rt], st, axis=-1) + np.roll(fold[2 * q + 1, rh], sh,
+                                                                          axis=-1)
+    return out
+
+
+@pytest.mark.parametrize("two", [False, True])
+def test_fused_score_signed_zero_profiles(gpu_device, two):
+    """Profiles that are zero over whole stretches (+0.0 and -0.0 mixed, as zero-filled gaps
+    of a series give): the fused scorers' S/N equals the sequential reference bit for bit,
+    the sign of a zero S/N included."""
+    _case(f"fused score signed zeros two={two}")
+    B, widths = 64, np.array([1, 2, 3, 5, 8, 12])
+    rng = np.random.default_rng(3 + two)
+    nprev, ncur = 300, 512
+    nseg = 4 if two else 2
+    e = rng.choice(np.array([0.0, -0.0, 0.0, -0.0, 1.0, -1.0], dtype=np.float32),
+                   size=(nseg, nprev, B))
+    e[:, : nprev // 2] = rng.choice(np.array([0.0, -0.0], dtype=np.float32),
+                                     size=(nseg, nprev // 2, B))
+    fold = np.stack([e, np.ones_like(e)], axis=2).astype(np.float32)
+    if two:
+        tprev = np.stack([rng.integers(0, nprev, ncur), rng.integers(0, B, ncur),
+                          rng.integers(0, nprev, ncur), rng.integers(0, B, ncur)], 1)
+        table = np.stack([rng.integers(0, ncur, ncur), rng.integers(0, B, ncur),
+                          rng.integers(0, ncur, ncur), rng.integers(0, B, ncur)], 1)
+        mid = _np_level(fold, tprev)
+        last = _np_level(mid, table)
+        got = kernels.ffa_shift_add2_score(_dev(fold), _dev(tprev.astype(np.int32)),
+                                           _dev(table.astype(np.int32)), ncur, ncur, widths,
+                                           n_inner=ncur)
+    else:
+        table = np.stack([rng.integers(0, nprev, ncur), rng.integers(0, B, ncur),
+                          rng.integers(0, nprev, ncur), rng.integers(0, B, ncur)], 1)
+        last = _np_level(fold, table)
+        got = kernels.ffa_shift_add_score(_dev(fold), _dev(table.astype(np.int32)), ncur,
+                                          widths, n_inner=ncur)
+    exp = orc.boxcar_snr(orc.fold_norm(last), widths)
+    g = got.cpu().numpy()
+    np.testing.assert_array_equal(g.view(np.uint32), exp.view(np.uint32))
+    assert np.sum(exp == 0) > 0
