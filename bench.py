@@ -29,17 +29,18 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                               "ffa_batch_accel32_hbm_traffic.md")
+VALU_PEAK_TOPS = 39.32  # VALU lane-ops/s (1e12): 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
+COUNTER_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                               "ffa_batch_accel32_counters.md")
 
 
-def committed_counters(kernel: str) -> dict | None:
-    """HBM traffic and VALU issue figures of ``kernel`` on this workload, measured with
-    rocprofv3 PMC passes (FETCH_SIZE, WRITE_SIZE scaled by a 1 GiB copy calibration; SQ
-    instruction counters) and committed as a JSON block in the traffic profile. None when
-    the profile does not cover the kernel."""
+def committed_counters(tile_rows: int) -> dict | None:
+    """Per-kernel counters of one benchmark step at ``tile_rows`` (VALU and LDS
+    instructions, HBM bytes from FETCH_SIZE / WRITE_SIZE scaled by the read/copy probe
+    calibration), measured with rocprofv3 PMC passes and committed as a JSON block in the
+    counter profile. None when the profile does not cover that tile."""
     try:
-        text = open(TRAFFIC_PROFILE).read()
+        text = open(COUNTER_PROFILE).read()
     except OSError:
         return None
     start = text.find("```json")
@@ -50,7 +51,7 @@ def committed_counters(kernel: str) -> dict | None:
         data = json.loads(text[start + 7:end])
     except ValueError:
         return None
-    return data.get("kernels", {}).get(kernel)
+    return data.get("tiles", {}).get(str(tile_rows))
 
 
 def parse() -> argparse.Namespace:
@@ -69,23 +70,44 @@ def parse() -> argparse.Namespace:
                          "search of a jerk pulsar (trials/s); ep-circular-cnn: the "
                          "circular-orbit EP with a learned survivor budget vs exhaustive")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="only copy 1 GiB device-to-device 4 times (HBM counter calibration)")
+                    help="only run the HBM read and copy probes (1 GiB, 4 launches each): "
+                         "the counter calibration and the attainable bandwidth")
     return ap.parse_args()
 
 
 def copy_calibration() -> int:
-    """A plain device copy of known size: 4 launches, each reading and writing 1 GiB.
-    Profiling it with the FETCH_SIZE / WRITE_SIZE counters gives the factor that turns
-    those counters into bytes on this device."""
+    """HBM probes of known size (pyl_hbm_read, pyl_hbm_copy): 4 launches each over 1 GiB,
+    timed with HIP events. Profiled with FETCH_SIZE / WRITE_SIZE they give the factors that
+    turn those counters into bytes; timed, the attainable HBM bandwidth on this device."""
     import torch
 
-    n = (1 << 30) // 4
-    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    from pyloki_amd import _lib
+    from pyloki_amd.device import ptr, stream_ptr
+
+    lib = _lib.load()
+    n4 = (1 << 30) // 16
+    src = torch.ones(4 * n4, dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
-    for _ in range(4):
-        dst.copy_(src)
-    torch.cuda.synchronize()
-    print(json.dumps({"copy_calibration_bytes_per_launch": n * 4, "launches": 4}))
+    nblk = 256 * 32
+    part = torch.empty(nblk, dtype=torch.float32, device="cuda")
+    out = {"bytes_per_launch": n4 * 16, "launches": 4}
+    for name, fn in (("read", lambda: lib.pyl_hbm_read(ptr(src), n4, ptr(part), nblk,
+                                                        stream_ptr())),
+                     ("copy", lambda: lib.pyl_hbm_copy(ptr(src), ptr(dst), n4, nblk,
+                                                        stream_ptr()))):
+        _lib.check(fn())  # warm-up
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(4):
+            _lib.check(fn())
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 4
+        moved = n4 * 16 * (2 if name == "copy" else 1)
+        out[f"{name}_ms"] = ms
+        out[f"{name}_gbs"] = moved / (ms * 1e-3) / 1e9
+    print(json.dumps(out))
     return 0
 
 
@@ -420,58 +442,64 @@ def main() -> int:
           f"{sorted(set(lens))} counts {[lens.count(n) for n in sorted(set(lens))]}",
           file=sys.stderr, flush=True)
 
-    # Live timing of the FFA level kernels: HIP events around every launch, on the stream
-    # the kernel is launched on (torch's current stream, which the wrappers pass down).
+    # Live timing of every kernel of the step: HIP events around each launch, on the stream
+    # it is launched on (torch's current stream, which the wrappers pass down).
     nbins = cfg_max.nbins
     nw = len(widths)
-    # Algorithmic bytes per output row: every output row reads its 2 source rows and writes
-    # its row (shift-add) or its per-width S/N (fused last level + score).
-    # The last two levels fused with scoring read the 4 rows of the level before both and
-    # write the S/N; the intermediate level's rows are never stored.
+    row = 2 * nbins * 4  # bytes of one FP32 fold row
+    # name -> (module attribute, rows out, compulsory bytes, algorithmic bytes) per launch.
+    # Compulsory bytes: the input fold read once plus the output written once. Algorithmic
+    # bytes: every output row's source rows (2 per level) plus its output.
     two = "ffa_shift_add_score_rounds_kernel<64, two levels>"
-    row_bytes = {
-        "ffa_shift_add_kernel<64>": 3 * 2 * nbins * 4,
-        "ffa_shift_add_score_rounds_kernel<64>": 2 * 2 * nbins * 4 + 4 * nw,
-        two: 4 * 2 * nbins * 4 + 4 * nw,
-    }
-    events: dict[str, list] = {k: [] for k in row_bytes}
+    names = {"ffa_shift_add": "ffa_shift_add_kernel<64>",
+             "ffa_shift_add2": "ffa_shift_add2_kernel<64>",
+             "ffa_shift_add_score": "ffa_shift_add_score_rounds_kernel<64>",
+             "ffa_shift_add2_score": two}
+    src_rows = {"ffa_shift_add": 2, "ffa_shift_add2": 4, "ffa_shift_add_score": 2,
+                "ffa_shift_add2_score": 4}
+    events: dict[str, list] = {}
     record = {"on": False}
-    orig_sa, orig_sas = kernels.ffa_shift_add, kernels.ffa_shift_add_score
+    origs = {k: getattr(kernels, k) for k in names}
 
-    def _timed(name, fn):
-        def wrapper(fold_in, table, ncoords_cur, *a, **kw):
+    def _timed(attr):
+        fn = origs[attr]
+
+        def wrapper(fold_in, *a, **kw):
             if not record["on"]:
-                return fn(fold_in, table, ncoords_cur, *a, **kw)
-            s = torch.cuda.current_stream()
+                return fn(fold_in, *a, **kw)
+            st = torch.cuda.current_stream()
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)
-            e0.record(s)
-            res = fn(fold_in, table, ncoords_cur, *a, **kw)
-            e1.record(s)
-            # rows out, and the compulsory bytes: the input fold read once + the output
-            compulsory = fold_in.numel() * fold_in.element_size() + res.numel() * res.element_size()
-            events[name].append((e0, e1, (fold_in.shape[0] // 2) * ncoords_cur, compulsory))
+            e0.record(st)
+            res = fn(fold_in, *a, **kw)
+            e1.record(st)
+            nrows = res.shape[0] * res.shape[1]
+            comp = fold_in.numel() * fold_in.element_size() + res.numel() * res.element_size()
+            out_b = res.numel() * res.element_size() // nrows
+            algo = nrows * (src_rows[attr] * row + out_b)
+            events.setdefault(names[attr], []).append((e0, e1, comp, algo))
             return res
         return wrapper
 
-    kernels.ffa_shift_add = _timed("ffa_shift_add_kernel<64>", orig_sa)
-    kernels.ffa_shift_add_score = _timed("ffa_shift_add_score_rounds_kernel<64>", orig_sas)
-    orig_s2 = kernels.ffa_shift_add2_score
+    for attr in names:
+        setattr(kernels, attr, _timed(attr))
+    bf_name = "bf_fold_lds_kernel<2, float>"
+    orig_bf = plan.bf.fold_batch
 
-    def _timed2(fold_in, table_prev, table, ncoords_prev, ncoords_cur, *a, **kw):
+    def _timed_bf(b, out):
         if not record["on"]:
-            return orig_s2(fold_in, table_prev, table, ncoords_prev, ncoords_cur, *a, **kw)
-        s = torch.cuda.current_stream()
+            return orig_bf(b, out)
+        st = torch.cuda.current_stream()
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
-        e0.record(s)
-        res = orig_s2(fold_in, table_prev, table, ncoords_prev, ncoords_cur, *a, **kw)
-        e1.record(s)
-        compulsory = fold_in.numel() * fold_in.element_size() + res.numel() * res.element_size()
-        events[two].append((e0, e1, (fold_in.shape[0] // 4) * ncoords_cur, compulsory))
+        e0.record(st)
+        res = orig_bf(b, out)
+        e1.record(st)
+        comp = sum(e.numel() * 8 for e, _ in b) + out.numel() * out.element_size()
+        events.setdefault(bf_name, []).append((e0, e1, comp, comp))
         return res
 
-    kernels.ffa_shift_add2_score = _timed2
+    plan.bf.fold_batch = _timed_bf
 
     # Trials per step: the final grid of every series (known from the plan, no run needed).
     trials = 0
@@ -499,36 +527,68 @@ def main() -> int:
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
 
-    # Separate instrumented pass for the kernel roofline (events perturb the timed loop).
+    # Separate instrumented pass for the per-kernel times (events perturb the timed loop).
+    nrec = max(1, min(args.steps, 3))
     record["on"] = True
-    for _ in range(max(1, min(args.steps, 3))):
+    for _ in range(nrec):
         ffa_batch(plan, dev_series, widths)
     torch.cuda.synchronize()
     record["on"] = False
-    kernels.ffa_shift_add, kernels.ffa_shift_add_score = orig_sa, orig_sas
-    kernels.ffa_shift_add2_score = orig_s2
+    for attr, fn in origs.items():
+        setattr(kernels, attr, fn)
+    plan.bf.fold_batch = orig_bf
     kstats = {}
     for name, evs in events.items():
-        if not evs:
-            continue
         tot_ms = sum(ev[0].elapsed_time(ev[1]) for ev in evs)
-        rows = sum(ev[2] for ev in evs)
-        avg_ms = tot_ms / len(evs)
-        avg_bytes = rows * row_bytes[name] / len(evs)
-        avg_comp = sum(ev[3] for ev in evs) / len(evs)
+        n = len(evs)
         kstats[name] = {
-            "launches": len(evs),
-            "total_ms_per_step": tot_ms / max(1, min(args.steps, 3)),
-            "avg_launch_ms": avg_ms,
-            "algorithmic_bytes_per_launch": avg_bytes,
-            "achieved_gbs": avg_bytes / (avg_ms * 1e-3) / 1e9,
-            "compulsory_bytes_per_launch": avg_comp,
-            "compulsory_gbs": avg_comp / (avg_ms * 1e-3) / 1e9,
+            "launches_per_step": n / nrec,
+            "ms_per_step": tot_ms / nrec,
+            "avg_launch_ms": tot_ms / n,
+            "compulsory_bytes_per_launch": sum(ev[2] for ev in evs) / n,
+            "algorithmic_bytes_per_launch": sum(ev[3] for ev in evs) / n,
         }
-    dominant = max(kstats, key=lambda k: kstats[k]["total_ms_per_step"])
-    achieved = kstats[dominant]["achieved_gbs"]
-    counters = committed_counters(dominant)
+        kstats[name]["compulsory_gbs"] = (kstats[name]["compulsory_bytes_per_launch"]
+                                          / (kstats[name]["avg_launch_ms"] * 1e-3) / 1e9)
+    dominant = max(kstats, key=lambda k: kstats[k]["ms_per_step"])
+    tile = int(tuned["rows"]) if tuned.get("rows") else 16
+    counters = committed_counters(tile)
+    prof = (counters or {}).get("kernels", {})
+    for name, st in kstats.items():
+        c = prof.get(name)
+        if not c:
+            continue
+        per_launch = 1.0 / max(c["launches_per_step"], 1)
+        if c.get("hbm_bytes_per_step") is not None:
+            st["hbm_bytes_per_launch"] = c["hbm_bytes_per_step"] * per_launch
+            st["hbm_gbs"] = st["hbm_bytes_per_launch"] / (st["avg_launch_ms"] * 1e-3) / 1e9
+            st["hbm_over_compulsory"] = st["hbm_bytes_per_launch"] / st[
+                "compulsory_bytes_per_launch"]
+        if c.get("valu_insts_per_step") is not None:
+            ops = c["valu_insts_per_step"] * 64 / (st["ms_per_step"] * 1e-3) / 1e12
+            st["valu_tops"] = ops
+            st["valu_frac"] = ops / VALU_PEAK_TOPS
+    dk = kstats[dominant]
     ms_per_step = wall / args.steps * 1e3
+    if "valu_tops" in dk:
+        roof = {"kernel": dominant, "bound": "valu", "achieved": dk["valu_tops"],
+                "peak": VALU_PEAK_TOPS, "unit": "TOPS (VALU lane-ops)",
+                "frac": dk["valu_frac"]}
+    else:  # no counter profile at this tile: the HBM view only
+        roof = {"kernel": dominant, "bound": "hbm", "achieved": dk["compulsory_gbs"],
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": dk["compulsory_gbs"] / HBM_PEAK_GBS}
+    roof.update({
+        "traffic": dk.get("hbm_bytes_per_launch"),
+        "hbm_gbs": dk.get("hbm_gbs"),
+        "hbm_frac": (dk["hbm_gbs"] / HBM_PEAK_GBS) if "hbm_gbs" in dk else None,
+        "compulsory_bytes_per_launch": dk["compulsory_bytes_per_launch"],
+        "hbm_over_compulsory": dk.get("hbm_over_compulsory"),
+        "avg_launch_ms": dk["avg_launch_ms"],
+        "launches_per_step": dk["launches_per_step"],
+        "counters": (os.path.relpath(COUNTER_PROFILE, os.path.dirname(os.path.abspath(
+            __file__))) + f" (tile {tile})") if counters else None,
+        "kernels": kstats,
+    })
     line = {
         "metric": "param trials folded+scored/sec & %HBM peak; EP recall vs exhaustive at equal S/N",
         "value": trials / (wall / args.steps),
@@ -549,32 +609,13 @@ def main() -> int:
             "samples_total": int(sum(lens)),
             "trials_per_step": trials,
             "final_grid_2^23": [int(c) for c in cfg_max.get_param_grid_count(cfg_max.niters_ffa)],
-            "parallelism": "1 GPU; series concatenated along the segment axis",
+            "parallelism": "1 GPU; series of one length in one launch per level pair",
             "tile_outer_rows": tuned["rows"],
             "tile_source": tuned["source"],
             "autotune_ms": tuned["ms"],
         },
-        "roofline": {
-            "kernel": dominant,
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": counters["hbm_bytes_per_launch"] if counters else None,
-            "traffic_source": (os.path.relpath(TRAFFIC_PROFILE, os.path.dirname(
-                os.path.abspath(__file__))) if counters else None),
-            "valu_issue_frac": counters.get("valu_issue_frac") if counters else None,
-            # Per-row algorithmic bytes count both source rows of every output row; the
-            # kernels' tiles reuse source rows from L2, so HBM traffic can be below them
-            # (frac above 1). Compulsory bytes: the input fold read once plus the output.
-            "compulsory_bytes_per_launch": kstats[dominant]["compulsory_bytes_per_launch"],
-            "compulsory_frac": kstats[dominant]["compulsory_gbs"] / HBM_PEAK_GBS,
-            "launches_timed": kstats[dominant]["launches"],
-            "avg_launch_ms": kstats[dominant]["avg_launch_ms"],
-            "algorithmic_bytes_per_launch": kstats[dominant]["algorithmic_bytes_per_launch"],
-            "kernels": kstats,
-        },
+        "roofline": roof,
+        "boundary_evals": plan.boundary_counts(),
     }
     if not args.no_cpu_baseline:
         cb = cpu_baseline(series, res)

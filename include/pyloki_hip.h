@@ -126,6 +126,16 @@ int pyl_ffa_resolve(int32_t nparams, const double* params_dev, const int64_t* co
                     const int64_t* counts_prev, const double* param_limits, int32_t ffa_level,
                     double tseg_brute, int32_t nbins, int32_t* table_dev, double* phases_dev,
                     void* stream);
+/* pyl_ffa_resolve_boundary_count: the resolve of one level without writing a table: adds
+ * to counts_dev[0] the grid indices (int(raw + 1e-6), psr_utils.py:508) and to
+ * counts_dev[1] the bin shifts (int(f32(x) + 0.5f), common.py:313-318) that a +-1 ulp
+ * change of their FP64 input would flip -- the bound on disagreement with an evaluation
+ * that differs in the last bit (the reference runs under numba fastmath). */
+int pyl_ffa_resolve_boundary_count(int32_t nparams, const double* params_dev,
+                                   const int64_t* counts_cur, const int64_t* counts_prev,
+                                   const double* param_limits, int32_t ffa_level,
+                                   double tseg_brute, int32_t nbins,
+                                   unsigned long long* counts_dev, void* stream);
 int pyl_ffa_shift_add(const float* fold_in_dev, int64_t nseg_in, int64_t ncoords_prev,
                       const int32_t* table_dev, int64_t ncoords_cur, int32_t nbins,
                       float* fold_out_dev, void* stream);
@@ -775,6 +785,17 @@ int pyl_cnn_forward_bf16(const float* params, const float* folds, int64_t n, int
                          int complex_folds, const float* feats, float* logits, void* stream);
 int pyl_cnn_adam(float* params_dev, const float* grad_dev, float* m_dev, float* v_dev,
                  int64_t nparams, float lr, float beta1, float beta2, float eps, int step,
+                 void* stream);
+
+/* ------------------------------------------------------------------------------------
+ * Bandwidth probes of known size (no reference counterpart; measurement only):
+ * pyl_hbm_read loads n_float4 16-byte elements once each (one float per block written to
+ * partial_dev [nblocks]); pyl_hbm_copy copies n_float4 elements. Used to measure the
+ * attainable HBM bandwidth and to scale the FETCH_SIZE / WRITE_SIZE counters to bytes.
+ * ------------------------------------------------------------------------------------ */
+int pyl_hbm_read(const float* src_dev, int64_t n_float4, float* partial_dev, int32_t nblocks,
+                 void* stream);
+int pyl_hbm_copy(const float* src_dev, float* dst_dev, int64_t n_float4, int32_t nblocks,
                  void* stream);
 
 #ifdef __cplusplus

@@ -60,6 +60,8 @@ SIGNATURES: list[tuple[str, object, list[object]]] = [
      [_vp, _c_i64, _c_i64, _vp, _c_i64, _c_i64, _c_i32, _i64p, _c_i32, _c_dbl, _vp, _vp]),
     ("pyl_ffa_resolve", ctypes.c_int,
      [_c_i32, _vp, _i64p, _i64p, _f64p, _c_i32, _c_dbl, _c_i32, _vp, _vp, _vp]),
+    ("pyl_ffa_resolve_boundary_count", ctypes.c_int,
+     [_c_i32, _vp, _i64p, _i64p, _f64p, _c_i32, _c_dbl, _c_i32, _vp, _vp]),
     ("pyl_ffa_shift_add", ctypes.c_int, [_vp, _c_i64, _c_i64, _vp, _c_i64, _c_i32, _vp, _vp]),
     ("pyl_ffa_set_tile_rows", ctypes.c_int, [_c_i32]),
     ("pyl_ffa_get_tile_rows", ctypes.c_int, []),
@@ -80,6 +82,8 @@ SIGNATURES: list[tuple[str, object, list[object]]] = [
     ("pyl_ffa_shift_add2_score_layout", ctypes.c_int,
      [_vp, _c_i32, _c_i64, _c_i64, _vp, _c_i64, _vp, _c_i64, _c_i64, _c_i32, _i64p, _c_i32,
       _c_dbl, _vp, _vp]),
+    ("pyl_hbm_read", ctypes.c_int, [_vp, _c_i64, _vp, _c_i32, _vp]),
+    ("pyl_hbm_copy", ctypes.c_int, [_vp, _vp, _c_i64, _c_i32, _vp]),
     ("pyl_score_boxcar", ctypes.c_int,
      [_vp, _c_i64, _c_i32, _c_i32, _i64p, _c_i32, _c_dbl, _c_i32, _vp, _vp]),
     ("pyl_circ_branch", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),

@@ -59,14 +59,36 @@ __device__ __host__ inline int shift_int(double iphase, int nbins) {
 }
 
 // psr_utils.py:466-514: truncation of the grid position with a 1e-6 guard.
-__device__ __host__ inline int64_t nearest_idx(double val, int64_t n_grid, double vmin,
-                                               double vmax) {
-  if (n_grid <= 1) return 0;
-  double raw = (double)n_grid * (val - vmin) / (vmax - vmin);
+__device__ __host__ inline double nearest_raw(double val, int64_t n_grid, double vmin,
+                                              double vmax) {
+  return (double)n_grid * (val - vmin) / (vmax - vmin);
+}
+
+__device__ __host__ inline int64_t idx_of_raw(double raw, int64_t n_grid) {
   int64_t idx = (int64_t)(raw + kFloatEps);
   if (idx < 0) idx = 0;
   else if (idx >= n_grid) idx = n_grid - 1;
   return idx;
+}
+
+__device__ __host__ inline int64_t nearest_idx(double val, int64_t n_grid, double vmin,
+                                               double vmax) {
+  if (n_grid <= 1) return 0;
+  return idx_of_raw(nearest_raw(val, n_grid, vmin, vmax), n_grid);
+}
+
+// Whether a +-1 ulp change of the FP64 value an integer decision is taken on would change
+// it: the grid index (int(raw + 1e-6)) and the bin shift (int(f32(x) + 0.5f)).
+__device__ inline bool idx_flips(double raw, int64_t n_grid) {
+  const int64_t i = idx_of_raw(raw, n_grid);
+  return idx_of_raw(nextafter(raw, -INFINITY), n_grid) != i ||
+         idx_of_raw(nextafter(raw, INFINITY), n_grid) != i;
+}
+
+__device__ inline bool shift_flips(double x, int nbins) {
+  const int b = shift_int(x, nbins);
+  return shift_int(nextafter(x, -INFINITY), nbins) != b ||
+         shift_int(nextafter(x, INFINITY), nbins) != b;
 }
 
 __device__ __host__ inline double fact_d(int n) {

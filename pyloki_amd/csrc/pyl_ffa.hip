@@ -45,8 +45,11 @@ struct ResolveArgs {
   double tmat[2][(kMaxP + 1) * (kMaxP + 1)];  // Taylor shift matrix per half (row-major)
 };
 
+// With `boundary` set the kernel writes no table and instead counts the decisions a +-1 ulp
+// change of their FP64 input would flip: boundary[0] grid indices, boundary[1] shifts.
 __global__ void ffa_resolve_kernel(const double* __restrict__ params, ResolveArgs a,
-                                   int4* __restrict__ table, double2* __restrict__ phases) {
+                                   int4* __restrict__ table, double2* __restrict__ phases,
+                                   unsigned long long* __restrict__ boundary = nullptr) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= a.ncoords) return;
   const int P = a.P;
@@ -59,6 +62,7 @@ __global__ void ffa_resolve_kernel(const double* __restrict__ params, ResolveArg
   }
   int out[4];
   double ph[2];
+  unsigned n_idx = 0, n_shift = 0;
   for (int latter = 0; latter < 2; ++latter) {
     double pnew[kMaxP];
     double delay;
@@ -89,6 +93,19 @@ __global__ void ffa_resolve_kernel(const double* __restrict__ params, ResolveArg
     out[2 * latter] = (int)flat;
     out[2 * latter + 1] = pyl::shift_int(phase, a.nbins);
     ph[latter] = phase;
+    if (boundary) {
+      for (int k = 0; k < P; ++k)
+        if (a.counts_prev[k] > 1)
+          n_idx += pyl::idx_flips(
+              pyl::nearest_raw(pnew[k], a.counts_prev[k], a.lim_lo[k], a.lim_hi[k]),
+              a.counts_prev[k]);
+      n_shift += pyl::shift_flips(phase, a.nbins);
+    }
+  }
+  if (boundary) {
+    if (n_idx) atomicAdd(&boundary[0], (unsigned long long)n_idx);
+    if (n_shift) atomicAdd(&boundary[1], (unsigned long long)n_shift);
+    return;
   }
   table[j] = make_int4(out[0], out[1], out[2], out[3]);
   if (phases) phases[j] = make_double2(ph[0], ph[1]);
@@ -556,13 +573,14 @@ ffa_shift_add_score_rounds_kernel(const T* __restrict__ in, int64_t ncoords_prev
 
 }  // namespace
 
-extern "C" int pyl_ffa_resolve(int32_t nparams, const double* params_dev,
-                               const int64_t* counts_cur, const int64_t* counts_prev,
-                               const double* param_limits, int32_t ffa_level, double tseg_brute,
-                               int32_t nbins, int32_t* table_dev, double* phases_dev,
-                               void* stream) {
+static int ffa_resolve_impl(int32_t nparams, const double* params_dev,
+                            const int64_t* counts_cur, const int64_t* counts_prev,
+                            const double* param_limits, int32_t ffa_level, double tseg_brute,
+                            int32_t nbins, int32_t* table_dev, double* phases_dev,
+                            unsigned long long* boundary_dev, void* stream) {
   PYL_REQUIRE(nparams >= 1 && nparams <= kMaxP, "nparams must be in [1, 5], got %d", nparams);
-  PYL_REQUIRE(params_dev && counts_cur && counts_prev && param_limits && table_dev,
+  PYL_REQUIRE(params_dev && counts_cur && counts_prev && param_limits &&
+                  (table_dev || boundary_dev),
               "null pointer argument");
   PYL_REQUIRE(ffa_level >= 1 && ffa_level < 62, "ffa_level must be >= 1, got %d", ffa_level);
   PYL_REQUIRE(nbins > 0, "Number of bins must be positive.");
@@ -603,9 +621,28 @@ extern "C" int pyl_ffa_resolve(int32_t nparams, const double* params_dev,
   const int threads = 128;
   ffa_resolve_kernel<<<(unsigned)pyl::ceil_div(ncur, threads), threads, 0, s>>>(
       params_dev, a, reinterpret_cast<int4*>(table_dev),
-      reinterpret_cast<double2*>(phases_dev));
+      reinterpret_cast<double2*>(phases_dev), boundary_dev);
   PYL_CHECK_LAUNCH("ffa_resolve_kernel");
   return PYL_OK;
+}
+
+extern "C" int pyl_ffa_resolve(int32_t nparams, const double* params_dev,
+                               const int64_t* counts_cur, const int64_t* counts_prev,
+                               const double* param_limits, int32_t ffa_level, double tseg_brute,
+                               int32_t nbins, int32_t* table_dev, double* phases_dev,
+                               void* stream) {
+  return ffa_resolve_impl(nparams, params_dev, counts_cur, counts_prev, param_limits, ffa_level,
+                          tseg_brute, nbins, table_dev, phases_dev, nullptr, stream);
+}
+
+extern "C" int pyl_ffa_resolve_boundary_count(int32_t nparams, const double* params_dev,
+                                              const int64_t* counts_cur,
+                                              const int64_t* counts_prev,
+                                              const double* param_limits, int32_t ffa_level,
+                                              double tseg_brute, int32_t nbins,
+                                              unsigned long long* counts_dev, void* stream) {
+  return ffa_resolve_impl(nparams, params_dev, counts_cur, counts_prev, param_limits, ffa_level,
+                          tseg_brute, nbins, nullptr, nullptr, counts_dev, stream);
 }
 
 template <int B, typename T, bool kWrapIn, bool kWrapOut>

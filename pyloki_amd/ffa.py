@@ -77,6 +77,32 @@ class FFAPlan:
                                          ncur, phases))
             counts_prev = counts_cur
 
+    def boundary_counts(self) -> dict:
+        """FP64 decisions of the plan that a +-1 ulp change of their input would flip:
+        brute-fold phase bins, and the grid indices and bin shifts of every level's resolve
+        (the honest bound on disagreement with an evaluation that rounds differently)."""
+        from pyloki_amd import _lib
+        from pyloki_amd.device import ptr, stream_ptr
+
+        cnt = torch.zeros(2, dtype=torch.int64, device=self.levels[0].params_dev.device) \
+            if self.levels else None
+        lib = _lib.load()
+        lim = np.ascontiguousarray(self.cfg.param_limits, dtype=np.float64).reshape(-1)
+        n_idx = n_shift = 0
+        for lp in self.levels:
+            _lib.check(lib.pyl_ffa_resolve_boundary_count(
+                len(lp.counts_cur), ptr(lp.params_dev), _lib.i64_array(lp.counts_cur),
+                _lib.i64_array(lp.counts_prev), _lib.f64_array(lim), lp.level,
+                self.cfg.tseg_brute, self.cfg.nbins, ptr(cnt), stream_ptr()))
+            n_idx += 2 * lp.ncoords * int(np.sum(np.asarray(lp.counts_prev) > 1))
+            n_shift += 2 * lp.ncoords
+        got = cnt.cpu().numpy() if cnt is not None else np.zeros(2, dtype=np.int64)
+        bf = 0 if self.complex else self.bf.boundary_count
+        nbf = 0 if self.complex else self.bf.nfreq * self.cfg.bseg_brute
+        return {"phase_bins": int(bf), "phase_bins_of": int(nbf),
+                "grid_indices": int(got[0]), "grid_indices_of": int(n_idx),
+                "shifts": int(got[1]), "shifts_of": int(n_shift)}
+
     def level(self, fold: torch.Tensor, lp: LevelPlan) -> torch.Tensor:
         """One level's shift-add (real, or complex with fractional phase shifts)."""
         if self.complex:
