@@ -20,7 +20,7 @@ import numpy as np
 import torch
 
 from pyloki_amd import _lib
-from pyloki_amd.prune import DeviceTree, Pruning, PruneStats
+from pyloki_amd.prune import DeviceTree, Pruning, PruneStats, PruneStatsCollection
 
 
 class _FoldView:
@@ -83,15 +83,30 @@ class CapturedSearch:
         hook.level_mask = mask
 
     def _buffer_ptrs(self):
+        """Everything the captured launches hold by address or by value that may change
+        between runs: the loop's buffers, and a survivor budget's scratch pointers, budget,
+        kind, precision and level flags. The graph is captured again when any changes."""
         buf = self.pr._loop
-        return (int(buf.s.hist or 0), int(buf.s.cap_child), int(buf.s.c_score or 0))
+        key = [int(buf.s.hist or 0), int(buf.s.cap_child), int(buf.s.c_score or 0)]
+        hook = self.pr.level_hook
+        s = getattr(hook, "_s", None)
+        if s is not None:
+            key += [int(getattr(s, f) or 0) for f in ("params", "keys", "feats", "kept",
+                                                     "bt_tmp", "sel", "stats", "shist",
+                                                     "part")]
+            key += [int(s.budget), int(s.kind), int(s.precision), int(s.nlevels)]
+            mask = getattr(hook, "level_mask", None)
+            key.append(None if mask is None
+                       else bytes(np.ascontiguousarray(mask, dtype=np.uint8)))
+        return tuple(key)
 
     def _capture(self) -> None:
         pr = self.pr
         pr.initialize(self.ref_seg)
         seed = pr.world_tree
         self.seed = DeviceTree(seed.leaves.clone(), seed.folds.clone(), seed.scores.clone(),
-                               seed.backtracks.clone(), scores_ep=seed.scores)
+                               seed.backtracks.clone())
+        self.seed.scores_ep = self.seed.scores  # the replayed seed scores
         n = seed.size
         nlast = pr.dyp.nsegments - 1
         buf = pr._loop_buffers(n, nlast)
@@ -145,10 +160,11 @@ class CapturedSearch:
         pr._world_tree = self.seed
         pr._loop.hist_next = 0
         n = self.seed.size
-        pr._pstats = [PruneStats(level=0, seg_idx=pr.scheme.ref_idx, threshold=0,
-                                 score_min=self.seed.score_min,
-                                 score_max=self.seed.score_max, n_branches=n, n_leaves=n,
-                                 n_leaves_phy=n, n_leaves_surv=n)]
+        pr._pstats = PruneStatsCollection()
+        pr._pstats.append(PruneStats(level=0, seg_idx=pr.scheme.ref_idx, threshold=0,
+                                     score_min=self.seed.score_min,
+                                     score_max=self.seed.score_max, n_branches=n, n_leaves=n,
+                                     n_leaves_phy=n, n_leaves_surv=n))
         pr._finish_levels(self.run_ctx)
         pr.execute_levels()  # levels after a hand-off, if any
         pr._do_ascend()

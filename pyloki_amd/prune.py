@@ -26,6 +26,8 @@ import torch
 
 from pyloki_amd import _lib
 from pyloki_amd.device import ptr, stream_ptr, to_device
+from pyloki_amd.io.cands import (PruneResultWriter, PruneStats, PruneStatsCollection,
+                                 mkdir_p)
 from pyloki_amd.utils import transforms
 from pyloki_amd.utils.misc import C_VAL
 from pyloki_amd.utils.snail import MiddleOutScheme
@@ -214,39 +216,6 @@ class LoopBuffers:
             if tree.leaves.numel() and tree.leaves.data_ptr() == self.leaves[j].data_ptr():
                 return j
         return -1
-
-
-@dataclass
-class PruneStats:
-    """Per-level statistics (``io/cands.py:16-101``)."""
-
-    level: int
-    seg_idx: int
-    threshold: float
-    score_min: float = 0.0
-    score_max: float = 0.0
-    n_branches: int = 1
-    n_leaves: int = 1
-    n_leaves_phy: int = 1
-    n_leaves_surv: int = 1
-
-    @property
-    def lb_leaves(self) -> float:
-        return float(np.round(np.log2(max(self.n_leaves, 1)), 2))
-
-    @property
-    def branch_frac(self) -> float:
-        return 0.0 if self.n_branches == 0 else float(np.round(self.n_leaves_phy / self.n_branches, 2))
-
-    @property
-    def surv_frac(self) -> float:
-        return 0.0 if self.n_leaves_phy == 0 else float(np.round(self.n_leaves_surv / self.n_leaves_phy, 2))
-
-    def get_summary(self) -> str:
-        return (f"Prune level: {self.level:3d}, seg_idx: {self.seg_idx:3d}, "
-                f"leaves: {self.lb_leaves:5.2f}, branch_frac: {self.branch_frac:5.2f},"
-                f"score thresh: {self.threshold:5.2f}, max: {self.score_max:5.2f}, "
-                f"min: {self.score_min:5.2f}, P(surv): {self.surv_frac:4.2f}\n")
 
 
 @dataclass
@@ -548,7 +517,7 @@ class Pruning:
         return self._world_tree
 
     @property
-    def pstats(self) -> list[PruneStats]:
+    def pstats(self) -> PruneStatsCollection:
         return self._pstats
 
     @property
@@ -583,10 +552,11 @@ class Pruning:
         bt = torch.zeros((n, P + 2), dtype=torch.int32, device=folds.device)
         self._world_tree = DeviceTree(to_device(leaves), folds.clone(), scores, bt)
         self._backtrack_hist = []
-        self._pstats = [PruneStats(level=0, seg_idx=self.scheme.ref_idx, threshold=0,
-                                   score_min=self._world_tree.score_min,
-                                   score_max=self._world_tree.score_max, n_branches=n,
-                                   n_leaves=n, n_leaves_phy=n, n_leaves_surv=n)]
+        self._pstats = PruneStatsCollection()
+        self._pstats.append(PruneStats(level=0, seg_idx=self.scheme.ref_idx, threshold=0,
+                                       score_min=self._world_tree.score_min,
+                                       score_max=self._world_tree.score_max, n_branches=n,
+                                       n_leaves=n, n_leaves_phy=n, n_leaves_surv=n))
         if log_file is not None:
             with Path(log_file).open("a") as f:
                 f.write(self._pstats[0].get_summary())
@@ -963,9 +933,19 @@ class Pruning:
 
     def execute(self, ref_seg: int, ascend_levels: list[int] | None = None,
                 outdir: str | Path = "./", log_file: Path | None = None,  # noqa: ARG002
-                result_file: Path | None = None, **_: object) -> dict:
-        """Seed, prune every level, ascend, report (``prune.py:526-613``)."""
+                result_file: Path | None = None, shared_progress=None,  # noqa: ANN001, ARG002
+                task_id: int | None = None) -> dict:
+        """Seed, prune every level, ascend, report (``prune.py:526-613``).
+
+        The run is named ``f"{ref_seg:03d}_{task_id:02d}"``; with ``result_file`` it is
+        appended to that results file (``PruneResultWriter``) and with ``log_file`` the
+        level summaries go there. (The reference writes temporary files in ``outdir``
+        when neither is given; here nothing is written then.) The run is also returned:
+        run name, scheme, reported parameters, scores, EP scores, level statistics."""
         asc = frozenset(ascend_levels) if ascend_levels is not None else None
+        if log_file is not None:
+            with Path(log_file).open("a") as f:
+                f.write(f"Pruning log for ref segment: {ref_seg}\n")
         self.initialize(ref_seg, log_file)
         nlast = self.dyp.nsegments - 1
         if self._use_device_loop():
@@ -981,9 +961,18 @@ class Pruning:
                     self._do_ascend()
         if asc is None or self._prune_level not in asc:
             self._do_ascend()
-        return self.report(ref_seg, result_file)
+        res = self.report(ref_seg, task_id=0 if task_id is None else task_id)
+        if result_file is not None:
+            with PruneResultWriter(result_file, mode="a") as writer:
+                writer.write_run_results(res["run_name"], res["scheme"], res["param_sets"],
+                                         res["scores"], res["scores_ep"], self._pstats)
+        if log_file is not None:
+            with Path(log_file).open("a") as f:
+                f.write(f"Pruning run complete for ref segment: {ref_seg}\n")
+                f.write(f"Time: {self._pstats.get_timer_summary()}\n\n")
+        return res
 
-    def report(self, ref_seg: int, result_file: Path | None = None) -> dict:
+    def report(self, ref_seg: int, task_id: int = 0) -> dict:
         """The result of a finished search: report transform of the survivors, their
         scores before and after the ascend, and the level statistics (``prune.py:592-613``)."""
         leaves = self._world_tree.leaves.cpu().numpy()
@@ -993,17 +982,14 @@ class Pruning:
             report = poly_chebyshev_report_batch(leaves, self.scheme.get_report_coord(True))
         else:
             report = poly_taylor_report_batch(leaves)
-        res = {
-            "run_name": f"{ref_seg:03d}",
+        return {
+            "run_name": f"{ref_seg:03d}_{task_id:02d}",
             "scheme": self.scheme.data.copy(),
             "param_sets": report,
             "scores": self._world_tree.scores.cpu().numpy(),
             "scores_ep": self._world_tree.scores_ep.cpu().numpy(),
             "level_stats": self._pstats,
         }
-        if result_file is not None:
-            save_run(result_file, res)
-        return res
 
 
 def poly_taylor_seed(param_arr: list[np.ndarray], dparams: np.ndarray,
@@ -1073,15 +1059,6 @@ def poly_chebyshev_report_batch(leaves: np.ndarray,
     return out
 
 
-def save_run(path: str | Path, res: dict) -> None:
-    """Per-run results as ``.npz`` (the reference writes the same datasets to HDF5,
-    ``io/cands.py:324-354``; h5py is not part of this package's dependencies)."""
-    stats = np.array([(s.level, s.seg_idx, s.threshold, s.score_min, s.score_max, s.n_branches,
-                       s.n_leaves, s.n_leaves_phy, s.n_leaves_surv) for s in res["level_stats"]])
-    np.savez(path, scheme=res["scheme"], param_sets=res["param_sets"], scores=res["scores"],
-             scores_ep=res["scores_ep"], level_stats=stats)
-
-
 def determine_ref_segs_pareto(nsegments: int, n_runs: int) -> list[int]:
     """Pareto-optimal reference segments (``np_utils.py:412-443``)."""
     if not (1 <= n_runs <= nsegments):
@@ -1100,28 +1077,44 @@ def determine_ref_segs_pareto(nsegments: int, n_runs: int) -> list[int]:
 
 
 def prune_dyp_tree(dyp, threshold_scheme: np.ndarray, n_runs: int | None = None,
-                   ref_segs: list[int] | None = None, max_sugg: int = 2**17,
-                   batch_size: int = 1024, outdir: str | Path = "./",
-                   file_prefix: str = "test", poly_basis: str = "taylor", n_workers: int = 1,  # noqa: ARG001
-                   *, use_moving_grid: bool = True, ascend_levels: list[int] | None = None,
-                   **_: object) -> list[dict]:
-    """Run the EP search from several reference segments (``prune.py:824-932``).
+                   ref_segs: list[int] | None = None, ascend_levels: list[int] | None = None,
+                   max_sugg: int = 2**18, batch_size: int = 1024, outdir: str | Path = "./",
+                   file_prefix: str = "test", poly_basis: str = "taylor", *,
+                   use_moving_grid: bool = True, n_workers: int = 4) -> str:
+    """Run the EP search from several reference segments and write every run to one results
+    file; returns its path (``prune.py:824-932``).
 
-    The reference fans runs out over worker processes; here every run uses the GPU in
-    turn (each run's level work already fills the device). Returns the per-run results.
+    The reference fans the runs out over ``n_workers`` processes. Here every run uses the
+    GPU in turn (one run's levels already fill the device); ``n_workers`` is checked as the
+    reference checks it and sets the task ids of the run names (0 for every run with one
+    worker, else the run's index), so the file's run names are the reference's.
     """
-    if ref_segs is None:
-        if n_runs is None:
-            msg = "Either n_runs or ref_segs must be provided"
+    if not isinstance(n_workers, (int, np.integer)):
+        msg = f"n_workers must be an integer, got {type(n_workers)}"
+        raise TypeError(msg)
+    if n_workers < 1:
+        msg = f"n_workers must be greater than 0, got {n_workers}"
+        raise ValueError(msg)
+    filebase = f"{file_prefix}_pruning_nstages_{dyp.nsegments}"
+    log_file = Path(outdir) / f"{filebase}_log.txt"
+    result_file = Path(outdir) / f"{filebase}_results.npz"
+    outdir = mkdir_p(outdir, must_not_exist=[log_file.name, result_file.name])
+    # n_runs takes precedence over ref_segs
+    if n_runs is not None:
+        if not 1 <= n_runs <= dyp.nsegments:
+            msg = f"n_runs must be between 1 and {dyp.nsegments}, got {n_runs}"
             raise ValueError(msg)
         ref_segs = determine_ref_segs_pareto(dyp.nsegments, n_runs)
-    out = []
+    elif ref_segs is None:
+        msg = "Either n_runs or ref_segs must be provided"
+        raise ValueError(msg)
+    n_workers_act = min(n_workers, len(ref_segs))
+    log_file.write_text("Pruning log\n")
+    with PruneResultWriter(result_file) as writer:
+        writer.write_metadata(dyp.cfg.param_names, dyp.nsegments, max_sugg, threshold_scheme)
+    pr = Pruning(dyp, threshold_scheme, max_sugg=max_sugg, batch_size=batch_size,
+                 poly_basis=poly_basis, use_moving_grid=use_moving_grid)
     for task, ref in enumerate(ref_segs):
-        pr = Pruning(dyp, threshold_scheme, max_sugg, batch_size, poly_basis,
-                     use_moving_grid=use_moving_grid)
-        res = pr.execute(int(ref), ascend_levels=ascend_levels)
-        res["run_name"] = f"{int(ref):03d}_{task:02d}"
-        out.append(res)
-    if outdir is not None and file_prefix:
-        Path(outdir).mkdir(parents=True, exist_ok=True)
-    return out
+        pr.execute(int(ref), ascend_levels=ascend_levels, outdir=outdir, log_file=log_file,
+                   result_file=result_file, task_id=0 if n_workers_act == 1 else task)
+    return result_file.as_posix()

@@ -88,7 +88,7 @@ def test_prune_levels_bit_exact(gpu_device, P, nsamps, nseg, thr, max_sugg, ref)
         np.testing.assert_array_equal(pr.world_tree.scores_ep.cpu().numpy(), t.scores_ep)
 
 
-def test_prune_execute_and_report(gpu_device):
+def test_prune_execute_and_report(gpu_device, tmp_path):
     _case("prune execute + report")
     cfg, dyp, pcfg = _setup(2, 2**17, 8)
     thresholds = np.linspace(2.0, 5.0, 7)
@@ -99,8 +99,21 @@ def test_prune_execute_and_report(gpu_device):
     np.testing.assert_array_equal(res["scores"], exp["tree"].scores)
     best = int(np.argmax(res["scores"]))
     assert abs(res["param_sets"][best, -3, 0] - 200.37) < 0.05
-    runs = prune_dyp_tree(dyp, thresholds, n_runs=2, max_sugg=2**18, outdir=None)
-    assert [r["run_name"] for r in runs] == ["002_00", "005_01"]
+    path = prune_dyp_tree(dyp, thresholds, n_runs=2, max_sugg=2**18, outdir=tmp_path)
+    assert path.endswith("test_pruning_nstages_8_results.npz")
+    from pyloki_amd.io.cands import read_results
+    from pyloki_amd.periodogram import PruningStatsPlotter, ScatteredPeriodogram
+
+    runs = read_results(path)["runs"]
+    assert list(runs) == ["002_00", "005_01"]
+    r5 = Pruning(dyp, thresholds, max_sugg=2**18).execute(5)
+    np.testing.assert_array_equal(runs["005_01"]["param_sets"], r5["param_sets"])
+    np.testing.assert_array_equal(runs["005_01"]["scores"], r5["scores"])
+    np.testing.assert_array_equal(runs["005_01"]["scores_ep"], r5["scores_ep"])
+    sp = ScatteredPeriodogram.load(path)
+    assert sp.n_runs == 2 and len(sp.data) == len(runs["002_00"]["scores"]) + len(r5["scores"])
+    assert PruningStatsPlotter.load(path).n_runs == 2
+    assert (tmp_path / "test_pruning_nstages_8_log.txt").read_text().startswith("Pruning log")
 
 
 def test_prune_branch_max_error(gpu_device):
