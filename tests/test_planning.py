@@ -144,3 +144,61 @@ def test_branching_pattern_matches_notebook(name, key, ref):
     bp = cfg.generate_branching_pattern("poly_taylor_moving", ref)
     # numpy printed 8 decimals (9 significant digits)
     np.testing.assert_allclose(bp, np.array(g[key]), rtol=6e-9, atol=5e-9)
+
+
+def _circ_cfg(g, limits):
+    fc = g["from_circular"]
+    return PulsarSearchConfig(
+        nsamps=g["nsamps"], tsamp=g["tsamp"], nbins=g["nbins"], eta=g["eta"],
+        param_limits=limits, bseg_brute=g["bseg_brute"], bseg_ffa=g["bseg_ffa"],
+        prune_poly_order=g["prune_poly_order"], m_p_min=fc["m_p_min"], m_c_max=fc["m_c_max"],
+        p_orb_min=fc["p_orb_min"], ducy_max=g["ducy_max"], wtsp=g["wtsp"], use_fourier=True,
+        branch_max=g["branch_max"],
+    )
+
+
+def test_ep_circular_planning_matches_notebook():
+    """examples/pyloki_ep_circular.ipynb cells 2-3: fold dims and leaves of every FFA
+    level, level-0 and final grid sizes, and the 127-level circ_taylor_moving branching
+    pattern (core/circular.py:905-1010) at ref_seg 0, from the limits the notebook
+    printed."""
+    g = GOLD["ep_circular"]
+    cfg = _circ_cfg(g, np.array(g["param_limits_8dp"]))
+    assert cfg.niters_ffa + 1 == len(g["fold_dims"])
+    for level, dims in enumerate(g["fold_dims"]):
+        counts = cfg.get_param_grid_count(level)
+        assert [g["nsamps"] // (g["bseg_brute"] * 2**level), *counts.tolist()] == dims[:-2]
+        assert round(math.log2(int(np.prod(counts))), 2) == g["leaves_log2"][level]
+    np.testing.assert_allclose(cfg.get_dparams(0), g["grid_sizes_level0"], rtol=6e-9)
+    np.testing.assert_allclose(cfg.get_dparams(cfg.niters_ffa)[:4], g["grid_sizes_final_first4"],
+                               rtol=6e-9)
+    bp = cfg.generate_branching_pattern("circ_taylor_moving", 0)
+    exp = np.array(g["branching_pattern_circ_taylor_moving_ref0"])
+    assert bp.shape == exp.shape == (127,)
+    np.testing.assert_allclose(bp, exp, rtol=6e-9, atol=5e-9)
+
+
+def test_circ_branching_pattern_rules():
+    """The circular pattern's own rules, against the Taylor pattern of the same grid:
+    levels before any snap split agree with the Taylor pattern (crackle never splits there
+    either); the first level a frequency's snap splits is halved; five parameters only."""
+    g = GOLD["ep_circular"]
+    cfg = _circ_cfg(g, np.array(g["param_limits_8dp"]))
+    circ = cfg.generate_branching_pattern("circ_taylor_fixed", 0)
+    taylor = cfg.generate_branching_pattern("poly_taylor_fixed", 0)
+    assert np.all(circ <= taylor + 1e-12)
+    first = int(np.argmax(~np.isclose(circ, taylor)))
+    assert first > 0
+    np.testing.assert_array_equal(circ[:first], taylor[:first])
+    moving = cfg.generate_branching_pattern("circ_taylor_moving", 0)
+    assert not np.array_equal(moving, circ)
+    jerk = GOLD["ep_jerk"]
+    fu = jerk["from_upper"]
+    p = ParamLimits.from_upper(tuple(fu["freq"]), fu["true_d_params"],
+                               tuple(fu["d_upper_limits"]), jerk["nsamps"] * jerk["tsamp"])
+    cfg3 = PulsarSearchConfig(
+        nsamps=jerk["nsamps"], tsamp=jerk["tsamp"], nbins=jerk["nbins"], eta=jerk["eta"],
+        param_limits=p.limits, bseg_brute=jerk["bseg_brute"], bseg_ffa=jerk["bseg_ffa"],
+        prune_poly_order=3, branch_max=jerk["branch_max"])
+    with pytest.raises(ValueError, match="exactly 5 parameters"):
+        cfg3.generate_branching_pattern("circ_taylor_moving", 0)
