@@ -788,6 +788,66 @@ int pyl_cnn_adam(float* params_dev, const float* grad_dev, float* m_dev, float* 
                  void* stream);
 
 /* ------------------------------------------------------------------------------------
+ * Monte-Carlo threshold design (detection/thresholding.py).
+ *
+ * pyl_thresh_simulate replaces, per task, one simulate_folds (:45-95) of a fold set
+ * followed by its scoring (prune_folds :98-134 / fold_max_scores :483-499: folds / sqrt(var)
+ * in FP64 rounded to FP32, boxcar S/N of every width with stdnoise 1, maximum over widths)
+ * and either the survivor counts of a list of thresholds (PYL_SIM_COUNT) or the survivors
+ * themselves (PYL_SIM_COMPACT: prune_folds's `score > thr` selection, stable order).
+ *
+ * Row i of a task's simulated folds (i < n_sim) is
+ *   f32( f64( src[src_row + i % n_prev][j] + f32(sqrt(var_add) * z) ) + prof_scaled[hyp][j] )
+ * with src rows all zero when src_row < 0 and z = normal deviate (seed, stream, row i,
+ * bin j) of pyl_rng_normal. A task simulated twice with the same stream gives the same
+ * folds, which is how survivors are materialised after the caller chose which to keep.
+ * Tasks run concurrently (one wavefront each); dst rows of different tasks must not
+ * overlap. nbins in {16, 32, 64}; widths as for pyl_score_boxcar; thr_count <= 256 (the
+ * kernel counts at most the first 256 thresholds of a task).
+ * ------------------------------------------------------------------------------------ */
+#define PYL_SIM_COUNT 0
+#define PYL_SIM_COMPACT 1
+#define PYL_SIM_NO_NOISE 1 /* PylSimTask.flags */
+
+typedef struct {
+  int64_t src_row;   /* first source row in src_dev; < 0: the n_prev source rows are zero */
+  int64_t dst_row;   /* PYL_SIM_COMPACT: first output row in dst_dev */
+  int64_t score_off; /* first score written to scores_dev (when scores_dev is not NULL) */
+  int64_t count_off; /* first count written to counts_dev */
+  uint64_t stream;   /* RNG stream of this simulation */
+  double var_norm;   /* variance the simulated folds are normalised by (var_cur + var_add) */
+  double thr;        /* PYL_SIM_COMPACT: rows with score > thr are kept */
+  int32_t n_prev;    /* source rows (> 0) */
+  int32_t n_sim;     /* rows simulated (> 0; simulate_folds: n_prev * ceil(ntrials / n_prev)) */
+  int32_t hyp;       /* row of prof_scaled_dev added: 0 (H0) or 1 (H1) */
+  int32_t thr_first; /* PYL_SIM_COUNT: first of this task's thresholds in thr_dev */
+  int32_t thr_count; /* PYL_SIM_COUNT: number of thresholds */
+  int32_t flags;     /* PYL_SIM_NO_NOISE: row i is src row i % n_prev itself (scoring only) */
+} PylSimTask;
+
+/* tasks_dev: PylSimTask [ntasks]; prof_scaled_dev: f64 [2][nbins] (profile * bias of H0 and
+ * H1); thr_dev: f64 thresholds. PYL_SIM_COUNT: counts_dev[count_off + t] = #(score >
+ * thr_dev[thr_first + t]) and, if scores_dev, scores_dev[score_off + i] = score of row i.
+ * PYL_SIM_COMPACT: survivors to dst_dev rows dst_row.., their scores to scores_dev
+ * [score_off..] (if not NULL), their number to counts_dev[count_off]. */
+int pyl_thresh_simulate(const float* src_dev, const PylSimTask* tasks_dev, int32_t ntasks,
+                        int32_t nbins, const double* prof_scaled_dev, double var_add,
+                        uint64_t seed, const int64_t* widths, int32_t nw, const double* thr_dev,
+                        int32_t mode, float* dst_dev, float* scores_dev, int32_t* counts_dev,
+                        void* stream);
+
+/* Normal deviates (no reference counterpart: the reference draws numpy Generator.normal,
+ * whose bits are not reproducible on a GPU): out_dev[i][j] = f32(std * z(seed, stream, i,
+ * j)) for i < n_rows, j < row_len (row_len a multiple of 4). z: Philox4x32-10 of counter
+ * {j / 4, i, stream} and key seed, Box-Muller with an IEEE-basic-operation ln and sincos. */
+int pyl_rng_normal(uint64_t seed, uint64_t stream, int64_t n_rows, int32_t row_len, double std,
+                   float* out_dev, void* stream_);
+
+/* Rows gather: dst_dev[r] = src_dev[idx[r]] (rows of row_len floats); idx_dev int64 [n]. */
+int pyl_rows_gather(const float* src_dev, const int64_t* idx_dev, int64_t n, int32_t row_len,
+                    float* dst_dev, void* stream);
+
+/* ------------------------------------------------------------------------------------
  * Bandwidth probes of known size (no reference counterpart; measurement only):
  * pyl_hbm_read loads n_float4 16-byte elements once each (one float per block written to
  * partial_dev [nblocks]); pyl_hbm_copy copies n_float4 elements. Used to measure the

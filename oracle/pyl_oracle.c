@@ -490,3 +490,134 @@ void orc_unify_fold_bf16(const uint16_t* fold_in, int64_t nseg_in, int P, const 
         }
     }
 }
+
+/* ---------------------------------------------------------------------------------------
+ * Monte-Carlo threshold design (detection/thresholding.py:45-95 simulate_folds).
+ *
+ * The reference draws numpy Generator.normal (bits not reproducible off the CPU). The
+ * GPU path draws its deviates from Philox4x32-10 (Salmon et al., SC'11; counter {quad,
+ * row, stream_lo, stream_hi}, key = 64-bit seed) through Box-Muller with a logarithm and
+ * a sincos(2 pi u) built from IEEE basic operations; this is the same published
+ * generator and the same series, written out independently, so the simulated folds can be
+ * compared bit for bit. Pinned by the Random123 known-answer vectors (philox) and against
+ * libm (ln, sin, cos) in tests/test_thresh_oracle.py.
+ * ------------------------------------------------------------------------------------- */
+static uint32_t orc_mulhi32(uint32_t a, uint32_t b) {
+  return (uint32_t)(((uint64_t)a * (uint64_t)b) >> 32);
+}
+
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+  uint32_t k0 = key[0], k1 = key[1];
+  for (int r = 0; r < 10; ++r) {
+    const uint32_t h0 = orc_mulhi32(0xD2511F53u, c0), l0 = 0xD2511F53u * c0;
+    const uint32_t h1 = orc_mulhi32(0xCD9E8D57u, c2), l1 = 0xCD9E8D57u * c2;
+    const uint32_t n0 = h1 ^ c1 ^ k0, n2 = h0 ^ c3 ^ k1;
+    c0 = n0; c1 = l1; c2 = n2; c3 = l0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* ln x, x normal in (0, 1): x = m 2^e, m in [sqrt(1/2), sqrt(2)); atanh series in
+ * s = (m-1)/(m+1) to s^21, Horner from the highest term. */
+double orc_ln_det(double x) {
+  static const double c[11] = {
+      1.0, 0x1.5555555555555p-2, 0x1.999999999999ap-3, 0x1.2492492492492p-3,
+      0x1.c71c71c71c71cp-4, 0x1.745d1745d1746p-4, 0x1.3b13b13b13b14p-4,
+      0x1.1111111111111p-4, 0x1.e1e1e1e1e1e1ep-5, 0x1.af286bca1af28p-5,
+      0x1.8618618618618p-5};
+  uint64_t b;
+  memcpy(&b, &x, sizeof b);
+  int e = (int)((b >> 52) & 0x7ff) - 1022;
+  const uint64_t mb = (b & 0x000fffffffffffffull) | 0x3fe0000000000000ull;
+  double m;
+  memcpy(&m, &mb, sizeof m);
+  if (m < 0x1.6a09e667f3bcdp-1) { m = m * 2.0; e -= 1; }
+  const double s = (m - 1.0) / (m + 1.0);
+  const double s2 = s * s;
+  double p = c[10];
+  for (int k = 9; k >= 0; --k) p = p * s2 + c[k];
+  return (double)e * 0x1.62e42fefa39efp-1 + 2.0 * s * p;
+}
+
+/* sin and cos of 2 pi u, u in [0, 1): exact quadrant split of 4u, first-quadrant angle
+ * <= pi/4, Taylor series to a^17 (sin) and a^18 (cos). */
+void orc_sincos_2pi(double u, double* sn, double* cs) {
+  static const double cs_k[9] = {
+      1.0, -0x1.5555555555555p-3, 0x1.1111111111111p-7, -0x1.a01a01a01a01ap-13,
+      0x1.71de3a556c734p-19, -0x1.ae64567f544e4p-26, 0x1.6124613a86d09p-33,
+      -0x1.ae7f3e733b81fp-41, 0x1.952c77030ad4ap-49};
+  static const double cc_k[10] = {
+      1.0, -0.5, 0x1.5555555555555p-5, -0x1.6c16c16c16c17p-10, 0x1.a01a01a01a01ap-16,
+      -0x1.27e4fb7789f5cp-22, 0x1.1eed8eff8d898p-29, -0x1.93974a8c07c9dp-37,
+      0x1.ae7f3e733b81fp-45, -0x1.6827863b97d97p-53};
+  const double t = 4.0 * u;
+  const int q = (int)t;
+  const double f = t - (double)q;
+  const int swap = f > 0.5;
+  const double g = swap ? 1.0 - f : f;
+  const double a = g * 0x1.921fb54442d18p+0;
+  const double a2 = a * a;
+  double ps = cs_k[8];
+  for (int k = 7; k >= 0; --k) ps = ps * a2 + cs_k[k];
+  double pc = cc_k[9];
+  for (int k = 8; k >= 0; --k) pc = pc * a2 + cc_k[k];
+  double s0 = a * ps, c0 = pc;
+  if (swap) { const double tmp = s0; s0 = c0; c0 = tmp; }
+  switch (q & 3) {
+    case 0: *sn = s0; *cs = c0; break;
+    case 1: *sn = c0; *cs = -s0; break;
+    case 2: *sn = -s0; *cs = -c0; break;
+    default: *sn = -c0; *cs = s0; break;
+  }
+}
+
+void orc_normal4(uint64_t seed, uint64_t stream, uint32_t row, uint32_t quad, double z[4]) {
+  const uint32_t ctr[4] = {quad, row, (uint32_t)stream, (uint32_t)(stream >> 32)};
+  const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  uint32_t r[4];
+  orc_philox4x32_10(ctr, key, r);
+  double sn, cs;
+  const double r0 = sqrt(-2.0 * orc_ln_det(((double)r[0] + 0.5) * 0x1p-32));
+  orc_sincos_2pi(((double)r[1] + 0.5) * 0x1p-32, &sn, &cs);
+  z[0] = r0 * cs;
+  z[1] = r0 * sn;
+  const double r1 = sqrt(-2.0 * orc_ln_det(((double)r[2] + 0.5) * 0x1p-32));
+  orc_sincos_2pi(((double)r[3] + 0.5) * 0x1p-32, &sn, &cs);
+  z[2] = r1 * cs;
+  z[3] = r1 * sn;
+}
+
+/* out[i][j] = f32(std * z(seed, stream, i, j)); row_len a multiple of 4. */
+void orc_rng_normal(uint64_t seed, uint64_t stream, int64_t n_rows, int64_t row_len, double sd,
+                    float* out) {
+  for (int64_t i = 0; i < n_rows; ++i)
+    for (int64_t q = 0; q < row_len / 4; ++q) {
+      double z[4];
+      orc_normal4(seed, stream, (uint32_t)i, (uint32_t)q, z);
+      for (int r = 0; r < 4; ++r) out[i * row_len + 4 * q + r] = (float)(sd * z[r]);
+    }
+}
+
+/* thresholding.py:45-95: folds_sim[i][j] = folds[i % n_prev][j] + noise[i][j] +
+ * profile_scaled[j] -- the first addition in FP32 (both operands FP32), the second in FP64
+ * (profile_scaled is FP64), stored as FP32. src == NULL: zero source rows. */
+void orc_simulate_folds(const float* src, int64_t n_prev, int64_t n_sim, int64_t nbins,
+                        const double* prof_scaled, double var_add, uint64_t seed,
+                        uint64_t stream, float* out) {
+  const double sd = sqrt(var_add);
+  for (int64_t i = 0; i < n_sim; ++i)
+    for (int64_t q = 0; q < nbins / 4; ++q) {
+      double z[4];
+      orc_normal4(seed, stream, (uint32_t)i, (uint32_t)q, z);
+      for (int r = 0; r < 4; ++r) {
+        const int64_t j = 4 * q + r;
+        const float noise = (float)(sd * z[r]);
+        const float base = src ? src[(i % n_prev) * nbins + j] : 0.0f;
+        const float a = base + noise;
+        out[i * nbins + j] = (float)((double)a + prof_scaled[j]);
+      }
+    }
+}

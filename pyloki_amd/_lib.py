@@ -24,6 +24,9 @@ PYL_FOLD_DENSE = 0
 PYL_FOLD_WRAP = 1
 PYL_SCORE_PER_WIDTH = 0
 PYL_SCORE_MAX = 1
+PYL_SIM_COUNT = 0
+PYL_SIM_COMPACT = 1
+PYL_SIM_NO_NOISE = 1
 
 _c_i32 = ctypes.c_int32
 _c_i64 = ctypes.c_int64
@@ -82,6 +85,12 @@ SIGNATURES: list[tuple[str, object, list[object]]] = [
     ("pyl_ffa_shift_add2_score_layout", ctypes.c_int,
      [_vp, _c_i32, _c_i64, _c_i64, _vp, _c_i64, _vp, _c_i64, _c_i64, _c_i32, _i64p, _c_i32,
       _c_dbl, _vp, _vp]),
+    ("pyl_thresh_simulate", ctypes.c_int,
+     [_vp, _vp, _c_i32, _c_i32, _vp, _c_dbl, ctypes.c_uint64, _i64p, _c_i32, _vp, _c_i32, _vp,
+      _vp, _vp, _vp]),
+    ("pyl_rng_normal", ctypes.c_int,
+     [ctypes.c_uint64, ctypes.c_uint64, _c_i64, _c_i32, _c_dbl, _vp, _vp]),
+    ("pyl_rows_gather", ctypes.c_int, [_vp, _vp, _c_i64, _c_i32, _vp, _vp]),
     ("pyl_hbm_read", ctypes.c_int, [_vp, _c_i64, _vp, _c_i32, _vp]),
     ("pyl_hbm_copy", ctypes.c_int, [_vp, _vp, _c_i64, _c_i32, _vp]),
     ("pyl_score_boxcar", ctypes.c_int,

@@ -76,3 +76,15 @@ class PulseSignalConfig:
         ts_e = (noise + (amp * tmpl).astype(np.float32)).astype(np.float32)
         ts_v = np.ones(self.nsamps, dtype=np.float32)
         return TimeSeries(ts_e, ts_v, self.dt)
+
+
+def generate_folded_profile(nbins: int = 100, ducy: float = 0.1, center: float = 0.5
+                            ) -> np.ndarray:
+    """Unit-energy Gaussian folded profile (``pulse.py:512-523``): full width at a tenth
+    of the peak ``ducy`` of a turn, peak at phase ``center``, wrapped; FP64."""
+    phase = np.linspace(0, 1, nbins, endpoint=False)
+    sigma = ducy / (2 * np.sqrt(2 * np.log(10)))
+    wrapped = np.mod(phase - center + 0.5, 1) - 0.5
+    profile = np.exp(-(wrapped**2) / (2 * sigma**2))
+    profile = profile / profile.max()
+    return profile / np.sqrt(np.sum(profile**2))

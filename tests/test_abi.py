@@ -98,3 +98,32 @@ def ctypes_size(t) -> int:
     import ctypes
 
     return ctypes.sizeof(t)
+
+
+def test_sim_task_dtype_matches_the_header(tmp_path):
+    """The numpy record of PylSimTask has the C compiler's size and field offsets."""
+    import shutil
+    import subprocess
+
+    import pytest
+
+    from pyloki_amd.detection.thresholding import TASK_DTYPE
+
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not available")
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "pyloki_hip.h"',
+             "int main(void) {", 'printf("size %zu\\n", sizeof(PylSimTask));']
+    for name in TASK_DTYPE.names:
+        lines.append(f'printf("{name} %zu\\n", offsetof(PylSimTask, {name}));')
+    lines.append("return 0; }")
+    src = tmp_path / "task.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "task"
+    inc = Path(__file__).resolve().parent.parent / "include"
+    subprocess.run([gcc, "-I", str(inc), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    for line in out.splitlines():
+        field, val = line.split()
+        got = TASK_DTYPE.itemsize if field == "size" else TASK_DTYPE.fields[field][1]
+        assert got == int(val), (field, got, val)
