@@ -252,7 +252,7 @@ int pyl_score_boxcar(const float* folds_dev, int64_t n, int32_t nbins, int32_t l
                      float* out_dev, void* stream);
 
 /* ------------------------------------------------------------------------------------
- * EP (enumerate-and-prune) Taylor search, real folds, moving grid.
+ * EP (enumerate-and-prune) Taylor search, real folds, moving or fixed grid.
  * One pruning level of prune.py:165-332 pruning_iteration_batched is three calls:
  *   pyl_prune_branch      core/taylor.py:68-169 poly_taylor_branch_batch: per parent and
  *                         parameter the child count, first child centre and step
@@ -292,6 +292,10 @@ typedef struct pyl_prune_params {
   double accel_lim[2];
   double freq_lim[2];
   int32_t widths[PYL_MAX_WIDTHS];
+  int32_t fixed_grid;     /* 0: moving grid; 1: fixed grid (use_moving_grid=False): the
+                             coords are the fixed ones (snail.py:286-345), the resolve is the
+                             fixed variant (taylor.py:244-277, circular.py:531-590,
+                             chebyshev.py:261-300) and the transform is the identity */
 } pyl_prune_params;
 
 typedef struct pyl_prune_coords {

@@ -267,6 +267,21 @@ def cheby_resolve_batch(leaves, coord_add, coord_cur, coord_init, counts_init, l
     return idx, phase
 
 
+def cheby_fixed_resolve_batch(leaves, coord_add, coord_cur_fixed, coord_init, counts_init,
+                              limits, nbins: int):
+    """poly_chebyshev_fixed_resolve_batch (chebyshev.py:261-300): the leaf's domain is
+    (t_init, scale of the fixed coord); velocity and delay at t_add used whole."""
+    scale = coord_cur_fixed[1]
+    t0_init = coord_init[0]
+    vec = leaves[:, :-1, 0]
+    f0 = leaves[:, -1, 0]
+    d_add = cheby_param_shift(vec, t0_init, scale, coord_add[0])
+    freq = f0 * (1 - d_add[:, -2] / C_VAL)
+    delay = d_add[:, -1] / C_VAL
+    phase = po.phase_idx_vec(coord_add[0] - t0_init, f0, nbins, delay)
+    return po.nearest_2d(d_add[:, -3], freq, counts_init, limits), phase
+
+
 def cheby_transform_batch(leaves, coord_next, coord_cur, tiling: str):
     out = np.zeros_like(leaves)
     out[:, :-1] = shift_cheby_full(leaves[:, :-1], coord_next, coord_cur, tiling)
@@ -275,16 +290,29 @@ def cheby_transform_batch(leaves, coord_next, coord_cur, tiling: str):
 
 
 def cheby_prune_level(tree: po.Tree, fold_segment, coords: dict, threshold: float, cfg: dict,
-                      max_sugg: int, batch_size: int = 1024):
+                      max_sugg: int, batch_size: int = 1024, moving: bool = True):
     P = cfg["poly_order"]
     tiling = cfg["tiling"]
+    if moving:
+        def resolve(ch):
+            return cheby_resolve_batch(ch, coords["add"], coords["cur"], coords["init"],
+                                       cfg["counts_init"], cfg["param_limits"], cfg["nbins"])
+
+        def transform(ch):
+            return cheby_transform_batch(ch, coords["next"], coords["cur"], tiling)
+    else:
+        def resolve(ch):
+            return cheby_fixed_resolve_batch(ch, coords["add"], coords["cur"], coords["init"],
+                                             cfg["counts_init"], cfg["param_limits"],
+                                             cfg["nbins"])
+
+        def transform(ch):
+            return ch
     return po.level_loop(
         tree, fold_segment, threshold, cfg, max_sugg, batch_size,
         lambda lv: cheby_branch_batch(lv, coords["cur"], coords["prev"], cfg["nbins"],
                                       cfg["eta"], P, cfg["branch_max"], tiling),
-        lambda ch: cheby_resolve_batch(ch, coords["add"], coords["cur"], coords["init"],
-                                       cfg["counts_init"], cfg["param_limits"], cfg["nbins"]),
-        lambda ch: cheby_transform_batch(ch, coords["next"], coords["cur"], tiling))
+        resolve, transform)
 
 
 def cheby_ascend(tree: po.Tree, dyp_fold, idx_segments, coord_segments, coord_cur,
@@ -326,15 +354,16 @@ def cheby_report_batch(leaves: np.ndarray, coord_report) -> np.ndarray:
     return out
 
 
-def level_coords(scheme: po.MiddleOut, level: int) -> dict:
-    return {"init": scheme.get_coord(0), "cur": scheme.get_current_coord(level),
-            "prev": scheme.get_previous_coord(level), "next": scheme.get_coord(level),
+def level_coords(scheme: po.MiddleOut, level: int, moving: bool = True) -> dict:
+    return {"init": scheme.get_coord(0), "cur": scheme.get_current_coord(level, moving),
+            "prev": scheme.get_previous_coord(level, moving), "next": scheme.get_coord(level),
             "add": scheme.get_segment_coord(level)}
 
 
 def cheby_prune_run(dyp_fold, param_arr, dparams, tseg, thresholds, ref_seg, cfg, max_sugg,
-                    batch_size=1024, nlevels=None):
-    """``Pruning.execute`` with the Chebyshev plugin (moving grid)."""
+                    batch_size=1024, nlevels=None, moving=True):
+    """``Pruning.execute`` with the Chebyshev plugin (moving or fixed grid; the fixed
+    report converts with the fixed report coord, dyn_poly_cheby.py:824-839)."""
     nseg = dyp_fold.shape[0]
     scheme = po.MiddleOut(nseg, ref_seg, tseg)
     P = cfg["poly_order"]
@@ -350,9 +379,9 @@ def cheby_prune_run(dyp_fold, param_arr, dparams, tseg, thresholds, ref_seg, cfg
     level = 0
     for level in range(1, nl + 1):
         tree, st, surv = cheby_prune_level(tree, dyp_fold[scheme.get_segment_idx(level)],
-                                           level_coords(scheme, level),
+                                           level_coords(scheme, level, moving),
                                            float(thresholds[level - 1]), cfg, max_sugg,
-                                           batch_size)
+                                           batch_size, moving)
         stats.append(st)
         survivors.append(surv)
         trees.append(tree)
@@ -360,8 +389,9 @@ def cheby_prune_run(dyp_fold, param_arr, dparams, tseg, thresholds, ref_seg, cfg
             break
     if tree.size > 0:
         idx_seg, coord_seg = scheme.get_segment_coords_so_far(level)
-        tree = cheby_ascend(tree, dyp_fold, idx_seg, coord_seg, scheme.get_coord(level), cfg)
-    report = (cheby_report_batch(tree.leaves, scheme.get_coord(nseg - 1)) if tree.size
+        tree = cheby_ascend(tree, dyp_fold, idx_seg, coord_seg,
+                            scheme.get_previous_coord(level + 1, moving), cfg)
+    report = (cheby_report_batch(tree.leaves, scheme.get_report_coord(moving)) if tree.size
               else tree.leaves)
     return {"tree": tree, "report": report, "stats": stats, "survivors": survivors,
             "trees": trees, "scheme": scheme, "seed": seed}

@@ -98,13 +98,35 @@ class MiddleOut:
     def get_segment_coord(self, level: int) -> tuple[float, float]:
         return (self.get_segment_idx(level) + 0.5) * self.tsegment, 0.5 * self.tsegment
 
-    def get_current_coord(self, level: int) -> tuple[float, float]:
+    def _fixed(self, level: int) -> tuple[float, float]:
+        """snail.py:297-303: the span from the reference segment to the far edge of the
+        segments so far, less the initial half-width, referenced to t_init."""
+        t0_init, scale_init = self.get_coord(0)
+        t0_next, scale_next = self.get_coord(level)
+        left, right = t0_next - scale_next, t0_next + scale_next
+        return t0_init, max(abs(left - t0_init), abs(right - t0_init)) - scale_init
+
+    def get_current_coord(self, level: int, moving: bool = True) -> tuple[float, float]:
+        """snail.py:286-303."""
         if level == 0:
             return self.get_coord(0)
-        return self.get_coord(level - 1)[0], self.get_coord(level)[1]
+        if moving:
+            return self.get_coord(level - 1)[0], self.get_coord(level)[1]
+        return self._fixed(level)
 
-    def get_previous_coord(self, level: int) -> tuple[float, float]:
-        return self.get_coord(level - 1)
+    def get_previous_coord(self, level: int, moving: bool = True) -> tuple[float, float]:
+        """snail.py:307-326."""
+        lv = level - 1
+        if lv == 0 or moving:
+            return self.get_coord(lv)
+        return self._fixed(lv)
+
+    def get_report_coord(self, moving: bool = True) -> tuple[float, float]:
+        """snail.py:330-345: the fixed one keeps t0 of the last level."""
+        lv = self.nsegments - 1
+        if moving:
+            return self.get_coord(lv)
+        return self.get_coord(lv)[0], self._fixed(lv)[1]
 
     def get_segment_coords_so_far(self, level: int):
         idx = np.array([self.get_segment_idx(i) for i in range(level + 1)], dtype=np.int64)
@@ -297,6 +319,20 @@ def resolve_batch(leaves, coord_add, coord_cur, coord_init, counts_init, limits,
     phase = phase_idx_vec(t0_add - t0_init, f0, nbins, delay)
     idx = nearest_2d(accel, freq, counts_init, limits)
     return idx, phase
+
+
+def fixed_resolve_batch(leaves, coord_add, coord_init, counts_init, limits, nbins):
+    """poly_taylor_fixed_resolve_batch (taylor.py:244-277): the leaf is referenced to
+    t_init; velocity and delay at t_add are used whole."""
+    t0_init, t0_add = coord_init[0], coord_add[0]
+    vec = leaves[:, :-1, 0]
+    f0 = leaves[:, -1, 0]
+    d_add = shift_taylor(vec, t0_add - t0_init)
+    accel = d_add[:, -3]
+    freq = f0 * (1 - d_add[:, -2] / C_VAL)
+    delay = d_add[:, -1] / C_VAL
+    phase = phase_idx_vec(t0_add - t0_init, f0, nbins, delay)
+    return nearest_2d(accel, freq, counts_init, limits), phase
 
 
 def seg_rows(fold_segment: np.ndarray, idx: np.ndarray) -> np.ndarray:
@@ -506,16 +542,29 @@ def level_loop(tree: Tree, fold_segment: np.ndarray, threshold: float, cfg: dict
 
 
 def prune_level(tree: Tree, fold_segment: np.ndarray, coords: dict, threshold: float,
-                cfg: dict, max_sugg: int, batch_size: int = 1024):
-    """One ``pruning_iteration_batched`` with the Taylor plugin."""
+                cfg: dict, max_sugg: int, batch_size: int = 1024, moving: bool = True):
+    """One ``pruning_iteration_batched`` with the Taylor plugin; on the fixed grid the
+    resolve is the fixed one and the transform the identity (dyn_poly_taylor.py:483-568)."""
     P = cfg["poly_order"]
+    if moving:
+        def resolve(ch):
+            return resolve_batch(ch, coords["add"], coords["cur"], coords["init"],
+                                 cfg["counts_init"], cfg["param_limits"], cfg["nbins"])
+
+        def transform(ch):
+            return transform_batch(ch, coords["next"], coords["cur"], cfg["tiling"])
+    else:
+        def resolve(ch):
+            return fixed_resolve_batch(ch, coords["add"], coords["init"], cfg["counts_init"],
+                                       cfg["param_limits"], cfg["nbins"])
+
+        def transform(ch):
+            return ch
     return level_loop(
         tree, fold_segment, threshold, cfg, max_sugg, batch_size,
         lambda lv: branch_batch(lv, coords["cur"], cfg["nbins"], cfg["eta"], P,
                                 cfg["branch_max"]),
-        lambda ch: resolve_batch(ch, coords["add"], coords["cur"], coords["init"],
-                                 cfg["counts_init"], cfg["param_limits"], cfg["nbins"]),
-        lambda ch: transform_batch(ch, coords["next"], coords["cur"], cfg["tiling"]))
+        resolve, transform)
 
 
 def ascend(tree: Tree, dyp_fold: np.ndarray, idx_segments, coord_segments, coord_cur,
@@ -558,8 +607,10 @@ def report_batch(leaves: np.ndarray) -> np.ndarray:
 
 def prune_run(dyp_fold: np.ndarray, param_arr: list[np.ndarray], dparams: np.ndarray,
               tseg: float, thresholds: np.ndarray, ref_seg: int, cfg: dict, max_sugg: int,
-              batch_size: int = 1024, nlevels: int | None = None):
-    """``Pruning.execute`` (moving grid, real folds): seed, levels, final ascend, report.
+              batch_size: int = 1024, nlevels: int | None = None, moving: bool = True):
+    """``Pruning.execute`` (real folds, moving or fixed grid): seed, levels, final ascend
+    (at coord_end, prune.py:680-683), report (the fixed grid's report drops its re-centring,
+    dyn_poly_taylor.py:676-690).
 
     ``dyp_fold``: [nseg, *counts, 2, B]. Returns dict with the final tree, report, per-level
     stats and per-level survivor child ids.
@@ -580,13 +631,13 @@ def prune_run(dyp_fold: np.ndarray, param_arr: list[np.ndarray], dparams: np.nda
         seg_idx = scheme.get_segment_idx(level)
         coords = {
             "init": scheme.get_coord(0),
-            "cur": scheme.get_current_coord(level),
-            "prev": scheme.get_previous_coord(level),
+            "cur": scheme.get_current_coord(level, moving),
+            "prev": scheme.get_previous_coord(level, moving),
             "next": scheme.get_coord(level),
             "add": scheme.get_segment_coord(level),
         }
         tree, st, surv = prune_level(tree, dyp_fold[seg_idx], coords, float(thresholds[level - 1]),
-                                     cfg, max_sugg, batch_size)
+                                     cfg, max_sugg, batch_size, moving)
         stats.append(st)
         survivors.append(surv)
         trees.append(tree)
@@ -595,7 +646,8 @@ def prune_run(dyp_fold: np.ndarray, param_arr: list[np.ndarray], dparams: np.nda
     final_level = level
     if tree.size > 0:
         idx_seg, coord_seg = scheme.get_segment_coords_so_far(final_level)
-        tree = ascend(tree, dyp_fold, idx_seg, coord_seg, scheme.get_coord(final_level), cfg)
+        tree = ascend(tree, dyp_fold, idx_seg, coord_seg,
+                      scheme.get_previous_coord(final_level + 1, moving), cfg)
     return {"tree": tree, "report": report_batch(tree.leaves) if tree.size else tree.leaves,
             "stats": stats, "survivors": survivors, "trees": trees, "scheme": scheme}
 
@@ -804,6 +856,21 @@ def circ_resolve_batch(leaves, coord_add, coord_cur, coord_init, counts_init, li
     return nearest_2d(accel, freq, counts_init, limits), phase
 
 
+def circ_fixed_resolve_batch(leaves, coord_add, coord_init, counts_init, limits, nbins, sig):
+    """circ_taylor_fixed_resolve_batch (circular.py:531-590): shifts from t_init, velocity
+    and delay used whole, and no basis-flag writes."""
+    t0_init, t0_add = coord_init[0], coord_add[0]
+    vec = leaves[:, :-1, 0]
+    f0 = leaves[:, -1, 0]
+    ms, mc, mt = circ_mask(leaves, sig)
+    d_add = circ_shift_all(vec, t0_add - t0_init, ms, mc, mt)
+    accel = d_add[:, -3]
+    freq = f0 * (1 - d_add[:, -2] / C_VAL)
+    delay = d_add[:, -1] / C_VAL
+    phase = phase_idx_vec(t0_add - t0_init, f0, nbins, delay)
+    return nearest_2d(accel, freq, counts_init, limits), phase
+
+
 def circ_transform_batch(leaves, coord_next, coord_cur, tiling, sig):
     ms, mc, mt = circ_mask(leaves, sig)
     dt = coord_next[0] - coord_cur[0]
@@ -819,7 +886,7 @@ def circ_transform_batch(leaves, coord_next, coord_cur, tiling, sig):
 
 
 def circ_prune_level(tree: Tree, fold_segment, coords, threshold, cfg, max_sugg,
-                     batch_size=1024):
+                     batch_size=1024, moving=True):
     """pruning_iteration_batched with the circular plugin (prune.py:165-332)."""
     P = 5
     B = tree.folds.shape[-1]
@@ -842,9 +909,14 @@ def circ_prune_level(tree: Tree, fold_segment, coords, threshold, cfg, max_sugg,
         n_phy += len(ch)
         if len(ch) == 0:
             continue
-        idx, phase = circ_resolve_batch(ch, coords["add"], coords["cur"], coords["init"],
-                                        cfg["counts_init"], cfg["param_limits"], cfg["nbins"],
-                                        cfg["prop_sig"])
+        if moving:
+            idx, phase = circ_resolve_batch(ch, coords["add"], coords["cur"], coords["init"],
+                                            cfg["counts_init"], cfg["param_limits"],
+                                            cfg["nbins"], cfg["prop_sig"])
+        else:
+            idx, phase = circ_fixed_resolve_batch(ch, coords["add"], coords["init"],
+                                                  cfg["counts_init"], cfg["param_limits"],
+                                                  cfg["nbins"], cfg["prop_sig"])
         seg = seg_rows(fold_segment, idx)
         parents = np.arange(b0, b1)[org]
         comb, scores = _fold_and_score(seg, phase, tree.folds, parents, cfg["widths"])
@@ -852,8 +924,8 @@ def circ_prune_level(tree: Tree, fold_segment, coords, threshold, cfg, max_sugg,
         smax = max(smax, float(scores.max()))
         fi = np.where(scores.astype(np.float64) >= max(threshold, cur_thr))[0]
         if len(fi):
-            trans = circ_transform_batch(ch[fi], coords["next"], coords["cur"], cfg["tiling"],
-                                         cfg["prop_sig"])
+            trans = (circ_transform_batch(ch[fi], coords["next"], coords["cur"], cfg["tiling"],
+                                          cfg["prop_sig"]) if moving else ch[fi])
             bt = np.empty((len(fi), P + 2), dtype=np.int32)
             bt[:, 0] = parents[fi]
             bt[:, 1: idx.shape[1] + 1] = idx[fi]
@@ -890,8 +962,8 @@ def circ_ascend(tree: Tree, dyp_fold, idx_segments, coord_segments, coord_cur, c
 
 
 def circ_prune_run(dyp_fold, param_arr, dparams, tseg, thresholds, ref_seg, cfg, max_sugg,
-                   batch_size=1024, nlevels=None):
-    """Pruning.execute with the circular plugin (moving grid, real folds)."""
+                   batch_size=1024, nlevels=None, moving=True):
+    """Pruning.execute with the circular plugin (real folds, moving or fixed grid)."""
     nseg = dyp_fold.shape[0]
     scheme = MiddleOut(nseg, ref_seg, tseg)
     leaves = taylor_seed(param_arr, dparams, 5)
@@ -904,11 +976,11 @@ def circ_prune_run(dyp_fold, param_arr, dparams, tseg, thresholds, ref_seg, cfg,
     nl = nseg - 1 if nlevels is None else nlevels
     level = 0
     for level in range(1, nl + 1):
-        coords = {"init": scheme.get_coord(0), "cur": scheme.get_current_coord(level),
+        coords = {"init": scheme.get_coord(0), "cur": scheme.get_current_coord(level, moving),
                   "next": scheme.get_coord(level), "add": scheme.get_segment_coord(level)}
         tree, st, surv = circ_prune_level(tree, dyp_fold[scheme.get_segment_idx(level)], coords,
                                           float(thresholds[level - 1]), cfg, max_sugg,
-                                          batch_size)
+                                          batch_size, moving)
         stats.append(st)
         survivors.append(surv)
         trees.append(tree)
@@ -916,6 +988,7 @@ def circ_prune_run(dyp_fold, param_arr, dparams, tseg, thresholds, ref_seg, cfg,
             break
     if tree.size > 0:
         idx_seg, coord_seg = scheme.get_segment_coords_so_far(level)
-        tree = circ_ascend(tree, dyp_fold, idx_seg, coord_seg, scheme.get_coord(level), cfg)
+        tree = circ_ascend(tree, dyp_fold, idx_seg, coord_seg,
+                           scheme.get_previous_coord(level + 1, moving), cfg)
     return {"tree": tree, "report": report_batch(tree.leaves) if tree.size else tree.leaves,
             "stats": stats, "survivors": survivors, "trees": trees, "scheme": scheme}

@@ -296,6 +296,11 @@ circ_children_kernel(LevelConst k, CircConst q, const double* __restrict__ leave
     double da[kN], di[kN];
     region_shift(reg, ch.val, q.dt_add, k.tadd, da);
     region_shift(reg, ch.val, q.dt_init, k.tinit, di);
+    if (k.fixed) {
+      // circular.py:531-590: velocity and delay at t_add taken whole.
+      di[kN - 2] = 0.0;
+      di[kN - 1] = 0.0;
+    }
     const double accel = da[kN - 3];
     const double vel = da[kN - 2] - di[kN - 2];
     const double freq = ch.f0 * (1.0 - vel / pyl::kCVal);
@@ -385,7 +390,9 @@ circ_materialize_kernel(LevelConst k, CircConst q, const double* __restrict__ le
     circ_decode(k, leaves, L, par, s1, L.c_sub[c], ch);
     const int8_t reg = L.c_region[c];
     double* ol = o_leaves + s * kRows * 2;
-    if (reg == PYL_REGION_TAYLOR) {
+    if (k.fixed) {
+      keep_leaf(k, ch, ol);  // no transform on the fixed grid
+    } else if (reg == PYL_REGION_TAYLOR) {
       taylor_full(k, k.tnext, ch, ol);  // transforms.py:93-137
     } else {
       // transforms.py:384-432: values moved along the orbit, errors unchanged.
@@ -398,9 +405,12 @@ circ_materialize_kernel(LevelConst k, CircConst q, const double* __restrict__ le
     }
     // circular.py:478-511: snap -> 1, crackle -> 2, crackle reset to 0 when the batch
     // holds a Taylor cell; Taylor cells keep the parent's flag.
+    // The fixed-grid resolve (circular.py:531-590) leaves the flag as it is.
     double flag = ch.flag;
-    if (reg == PYL_REGION_SNAP) flag = 1.0;
-    else if (reg == PYL_REGION_CRACKLE) flag = has_taylor[par / q.batch_size] ? 0.0 : 2.0;
+    if (!k.fixed) {
+      if (reg == PYL_REGION_SNAP) flag = 1.0;
+      else if (reg == PYL_REGION_CRACKLE) flag = has_taylor[par / q.batch_size] ? 0.0 : 2.0;
+    }
     ol[2 * (kP + 1)] = ch.f0;
     ol[2 * (kP + 1) + 1] = flag;
     o_scores[s] = c_score[v];

@@ -184,6 +184,12 @@ prune_children_kernel(LevelConst k, const double* __restrict__ leaves,
       v_init = mat_row(k.tinit, ch.val, n, P - 1);
       d_init = mat_row(k.tinit, ch.val, n, P);
     }
+    if (k.fixed) {
+      // taylor.py:244-277 / chebyshev.py:261-300: the leaf lives at t_init, so the
+      // velocity and delay at t_add are taken whole (nothing subtracted).
+      v_init = 0.0;
+      d_init = 0.0;
+    }
     const double vel = v_add - v_init;
     const double freq = ch.f0 * (1.0 - vel / pyl::kCVal);
     const double delay = (d_add - d_init) / pyl::kCVal;
@@ -275,8 +281,11 @@ prune_materialize_kernel(LevelConst k, const double* __restrict__ leaves,
     ChildLeaf ch;
     decode_child(k, lf, branch + par * P * 2, counts + par * P, c - off[par], ch);
     double* ol = o_leaves + s * (P + 2) * 2;
-    // transforms.py:93-137 shift_taylor_full to coord_next.
-    taylor_full(k, k.tnext, ch, ol);
+    // transforms.py:93-137 shift_taylor_full to coord_next (fixed grid: unchanged).
+    if (k.fixed)
+      keep_leaf(k, ch, ol);
+    else
+      taylor_full(k, k.tnext, ch, ol);
     ol[2 * (P + 1)] = ch.f0;
     ol[2 * (P + 1) + 1] = ch.flag;
     o_scores[s] = c_score[c];

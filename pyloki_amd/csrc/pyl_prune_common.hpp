@@ -20,6 +20,7 @@ constexpr int kStride = pyl::kScoreStride;
 
 struct LevelConst {
   int32_t P, nbins, branch_max, tiling;
+  int32_t fixed;  // fixed grid: resolve from coord_init without the init offsets, no transform
   int32_t nparams_ffa, n_accel, n_freq;
   double eta;
   double dpf[kMaxP];          // poly_taylor_step_f(P, scale_cur) row order d_P..d_1
@@ -58,6 +59,8 @@ int build_const(const pyl_prune_params* p, const pyl_prune_coords* c, LevelConst
   k->n_accel = p->n_accel;
   k->n_freq = p->n_freq;
   k->eta = p->eta;
+  PYL_REQUIRE(p->fixed_grid == 0 || p->fixed_grid == 1, "fixed_grid must be 0 or 1");
+  k->fixed = p->fixed_grid;
   if (c) {
     // psr_utils.py:59-96 (t_ref = 0, Chebyshev coarsening), reversed to d_P..d_1.
     const double dphi = p->eta / p->nbins;
@@ -339,6 +342,19 @@ __device__ __forceinline__ double mat_row(const double* __restrict__ T, const do
 // (scoring.py:249-282).
 __device__ __forceinline__ float score_column(const LevelConst& k, float* S, int tid) {
   return (float)pyl::score_column<false>(k.sc, k.nbins, S, tid, nullptr);
+}
+
+// The fixed grid's transform (dyn_poly_taylor.py:554-568, dyn_poly_cheby.py:698-712,
+// dyn_circular_taylor.py:600-615): the child leaf as branched.
+__device__ __forceinline__ void keep_leaf(const LevelConst& k, const ChildLeaf& ch, double* ol) {
+  const int n = k.P + 1;
+#pragma unroll
+  for (int i = 0; i < kMaxN; ++i) {
+    if (i < n) {
+      ol[2 * i] = ch.val[i];
+      ol[2 * i + 1] = ch.err[i];
+    }
+  }
 }
 
 // transforms.py:93-137 shift_taylor_full: the Taylor translation T of the values and the

@@ -2,9 +2,16 @@
 
 Mirrors ``pyloki.prune`` (``src/pyloki/prune.py``) for the Taylor and circular bases
 (``poly_basis="taylor"``; circular when ``prune_poly_order == 5``), with real or complex
-(``use_fourier=True``) folds and a moving grid (``use_moving_grid=True``):
-seed from the FFA fold of the reference segment, one pruning level per remaining segment
-(middle-out order), the final ascend (re-integration + rescoring) and the report transform.
+(``use_fourier=True``) folds and a moving or fixed grid (``use_moving_grid``): seed from the
+FFA fold of the reference segment, one pruning level per remaining segment (middle-out
+order), the final ascend (re-integration + rescoring) and the report transform.
+
+On the fixed grid every leaf stays referenced to the reference segment: the coordinates
+are ``MiddleOutScheme``'s fixed ones (``snail.py:286-345``), the resolve is the fixed
+variant and the transform is the identity. Its report is the moving grid's report of
+those leaves: the reference computes a re-centring there and drops the result
+(``dyn_poly_taylor.py:676-690``, ``dyn_circular_taylor.py:725-740``); the Chebyshev report
+converts with the fixed report coordinate (``dyn_poly_cheby.py:824-839``).
 
 Every per-leaf step runs in the HIP library (``pyl_prune_*``). The host decides which
 children survive: when the children passing the threshold fit the survivor buffer
@@ -50,6 +57,7 @@ class PruneParams(ctypes.Structure):
         ("accel_lim", ctypes.c_double * 2),
         ("freq_lim", ctypes.c_double * 2),
         ("widths", ctypes.c_int32 * 64),
+        ("fixed_grid", ctypes.c_int32),
     ]
 
 
@@ -370,9 +378,6 @@ class Pruning:
             raise ValueError(msg)
         self._cheby = poly_basis == "chebyshev"
         self._complex = bool(cfg.use_fourier)  # complex64 folds (Fourier domain)
-        if not use_moving_grid:
-            msg = "only the moving grid is supported"
-            raise NotImplementedError(msg)
         if cfg.nparams != cfg.prune_poly_order or cfg.nparams < 2:
             msg = "Taylor pruning needs prune_poly_order == number of FFA parameters (>= 2)"
             raise ValueError(msg)
@@ -399,6 +404,7 @@ class Pruning:
         pp.freq_lim[0], pp.freq_lim[1] = (float(x) for x in cfg.param_limits[-1])
         for i, w in enumerate(widths.tolist()):
             pp.widths[i] = w
+        pp.fixed_grid = 0 if use_moving_grid else 1
         self._pp = pp
         self._qq = None
         if self._circular:
@@ -564,12 +570,12 @@ class Pruning:
     def _coords(self, level: int) -> PruneCoords:
         s = self.scheme
         c = PruneCoords()
+        mg = self._use_moving_grid
         c.t0_init = s.get_coord(0)[0]
-        cur = s.get_current_coord(level, True)
-        c.t0_cur, c.scale_cur = cur
+        c.t0_cur, c.scale_cur = s.get_current_coord(level, mg)
         c.t0_next, c.scale_next = s.get_coord(level)
         c.t0_add = s.get_segment_coord(level)[0]
-        c.t0_prev, c.scale_prev = s.get_previous_coord(level, True)
+        c.t0_prev, c.scale_prev = s.get_previous_coord(level, mg)
         return c
 
     def execute_iter(self, log_file: Path | None = None) -> None:
@@ -907,7 +913,9 @@ class Pruning:
             return
         tree = self._world_tree
         seg_idx, seg_coords = self.scheme.get_segment_coords_so_far(self._prune_level)
-        t0_cur, scale_cur = self.scheme.get_coord(self._prune_level)
+        # coord_end (prune.py:680-683): where the survivors' parameters are referenced.
+        t0_cur, scale_cur = self.scheme.get_previous_coord(self._prune_level + 1,
+                                                           self._use_moving_grid)
         n = tree.size
         folds = self._new_folds(n, tree.folds.device)
         scores = torch.empty(n, dtype=torch.float32, device=tree.folds.device)
@@ -979,7 +987,8 @@ class Pruning:
         if not len(leaves):
             report = leaves
         elif self._cheby:
-            report = poly_chebyshev_report_batch(leaves, self.scheme.get_report_coord(True))
+            report = poly_chebyshev_report_batch(
+                leaves, self.scheme.get_report_coord(self._use_moving_grid))
         else:
             report = poly_taylor_report_batch(leaves)
         return {

@@ -205,3 +205,139 @@ def test_search_recovers_injected_signal():
     # the injected acceleration lies inside the best leaf's acceleration cell
     assert abs(tree.leaves[best, 0, 0] - 12.0) <= tree.leaves[best, 0, 1] / 2
     assert tree.scores[best] > 15
+
+
+def test_fixed_grid_coords_by_hand_and_product_scheme():
+    """snail.py:286-345 fixed coordinates: t_init with the half-width from t_init to the far
+    edge of the segments so far, less the initial half-width; the report keeps the last
+    level's t0. The product's MiddleOutScheme gives the same coordinates."""
+    from pyloki_amd.utils.snail import MiddleOutScheme
+
+    s = po.MiddleOut(8, 3, 1.0)  # order 3, 2, 4, 1, 5, 0, 6, 7
+    assert s.get_current_coord(0, False) == (3.5, 0.5)
+    # level 1: segments 2..3 -> edges 2 and 4 -> max(1.5, 0.5) - 0.5
+    assert s.get_current_coord(1, False) == (3.5, 1.0)
+    assert s.get_current_coord(2, False) == (3.5, 1.0)   # 2..4: edges 2, 5 -> 1.5
+    assert s.get_current_coord(7, False) == (3.5, 4.0)   # 0..7: max(3.5, 4.5) - 0.5
+    assert s.get_previous_coord(1, False) == s.get_coord(0)
+    assert s.get_previous_coord(3, False) == s.get_current_coord(2, False)
+    assert s.get_report_coord(False) == (4.0, 4.0)
+    assert s.get_report_coord(True) == s.get_coord(7)
+    for nseg, ref, ts in ((8, 3, 1.0), (11, 0, 0.7), (11, 10, 2.5), (16, 7, 1.3)):
+        o, p = po.MiddleOut(nseg, ref, ts), MiddleOutScheme(nseg, ref, ts)
+        for mv in (True, False):
+            for lv in range(nseg):
+                assert p.get_current_coord(lv, mv) == o.get_current_coord(lv, mv)
+            for lv in range(1, nseg + 1):
+                assert p.get_previous_coord(lv, mv) == o.get_previous_coord(lv, mv)
+            assert p.get_report_coord(mv) == o.get_report_coord(mv)
+
+
+def test_fixed_resolve_vs_exact_rational():
+    """taylor.py:244-277: velocity and delay at t_add are the whole polynomial values (the
+    moving grid's init offsets are not subtracted), phase from t_add - t_init."""
+    P = 2
+    lv = _leaves(400, P)
+    lv[:, 2, 0] = 0.0  # the fixed grid keeps d0 = 0 at t_init
+    limits = np.array([[-50.0, 50.0], [190.0, 210.0]])
+    counts = np.array([17, 4001])
+    idx, phase = po.fixed_resolve_batch(lv, (30.0, 1.0), (16.0, 1.0), counts, limits, 64)
+    C = Fr(po.C_VAL)
+    checked = 0
+    for n in range(len(lv)):
+        a, v, f0 = Fr(lv[n, 0, 0]), Fr(lv[n, 1, 0]), Fr(lv[n, 3, 0])
+        dt = Fr(30.0 - 16.0)
+        v_add, d_add = v + a * dt, v * dt + a * dt * dt / 2
+        freq = f0 * (1 - v_add / C)
+        x = (dt - d_add / C) * f0
+        pos = (x - math.floor(x)) * 64
+        ra = (a - Fr(-50)) * Fr(17) / Fr(100) + Fr(1, 10**6)
+        rf = (freq - Fr(190)) * Fr(4001) / Fr(20) + Fr(1, 10**6)
+        dist = min(abs(ra - round(ra)), abs(rf - round(rf)),
+                   abs(pos + Fr(1, 2) - round(pos + Fr(1, 2))))
+        if dist < 1e-6:
+            continue
+        assert idx[n, 0] == min(max(math.floor(ra), 0), 16)
+        assert idx[n, 1] == min(max(math.floor(rf), 0), 4000)
+        assert orc.shift_int(phase[n], 64) == math.floor(pos + Fr(1, 2)) % 64
+        checked += 1
+    assert checked > 390
+    # the moving resolve from the same reference time subtracts the init velocity: with
+    # velocities of a few hundred km/s the frequency bins differ
+    big = lv.copy()
+    big[:, 1, 0] = RNG.uniform(1e5, 3e5, len(big))
+    im, _ = po.resolve_batch(big, (30.0, 1.0), (16.0, 1.0), (16.0, 1.0), counts, limits, 64)
+    fx, _ = po.fixed_resolve_batch(big, (30.0, 1.0), (16.0, 1.0), counts, limits, 64)
+    assert np.mean(im[:, 1] != fx[:, 1]) > 0.9
+
+
+def test_fixed_resolve_circular_and_chebyshev_identities():
+    """circular.py:531-590 and chebyshev.py:261-300: the fixed resolve is the moving one
+    from t_cur = t_init without the init velocity and delay, and the circular fixed resolve
+    writes no basis flags."""
+    from oracle import cheby_oracle as co
+
+    n = 300
+    lv = np.zeros((n, 7, 2))
+    lv[:, 0, 0] = RNG.uniform(-1e-6, 1e-6, n)
+    lv[:, 1, 0] = RNG.uniform(-3e-3, 3e-3, n)
+    lv[:, 2, 0] = RNG.uniform(-0.3, 0.3, n)
+    lv[:, 3, 0] = RNG.uniform(-30, 30, n)
+    lv[:, 4, 0] = RNG.uniform(1e5, 3e5, n)  # velocities large enough to move a bin
+    lv[:, :5, 1] = np.abs(lv[:, :5, 0]) * RNG.uniform(0.01, 2, (n, 5))
+    lv[:, 6, 0] = RNG.uniform(199, 201, n)
+    lv[:, 6, 1] = 7.0
+    limits = np.array([[-1, 1], [-1, 1], [-1, 1], [-40, 40], [190, 210.0]])
+    counts = np.array([1, 1, 1, 9, 2001])
+    init, add = (16.0, 1.0), (30.0, 1.0)
+    before = lv.copy()
+    fi, fp = po.circ_fixed_resolve_batch(lv, add, init, counts, limits, 64, 2.0)
+    np.testing.assert_array_equal(lv, before)  # no flag writes
+    acc, freq_mov, _, masks = po.circ_resolve_values(lv, add, init, init, 2.0)
+    assert masks[0].any() and masks[2].any()
+    d_add = po.circ_shift_all(lv[:, :-1, 0], 14.0, *masks)
+    f0 = lv[:, -1, 0]
+    f_fixed = f0 * (1 - d_add[:, -2] / po.C_VAL)
+    np.testing.assert_array_equal(fi, po.nearest_2d(acc, f_fixed, counts, limits))
+    np.testing.assert_array_equal(fp, po.phase_idx_vec(14.0, f0, 64, d_add[:, -1] / po.C_VAL))
+    moving = po.nearest_2d(acc, freq_mov, counts, limits)
+    assert np.mean(moving[:, -1] != fi[:, -1]) > 0.9
+    # Chebyshev: the fixed resolve's implied velocity exceeds the moving one's by v(t_init)
+    P = 2
+    cl = np.zeros((n, P + 2, 2))
+    cl[:, 0, 0] = RNG.uniform(-3, 3, n)
+    cl[:, 1, 0] = RNG.uniform(-50, 50, n)
+    cl[:, 2, 0] = RNG.uniform(-2, 2, n)
+    cl[:, 3, 0] = RNG.uniform(195, 205, n)
+    lim2 = np.array([[-50.0, 50.0], [190.0, 210.0]])
+    cnt2 = np.array([17, 4001])
+    cur = (16.0, 9.0)
+    fi2, fp2 = co.cheby_fixed_resolve_batch(cl, add, cur, init, cnt2, lim2, 64)
+    d_add = co.cheby_param_shift(cl[:, :-1, 0], 16.0, 9.0, 30.0)
+    d_in = co.cheby_param_shift(cl[:, :-1, 0], 16.0, 9.0, 16.0)
+    f_fix = cl[:, -1, 0] * (1 - d_add[:, -2] / po.C_VAL)
+    np.testing.assert_array_equal(fi2, po.nearest_2d(d_add[:, -3], f_fix, cnt2, lim2))
+    im2, pm2 = co.cheby_resolve_batch(cl, add, cur, init, cnt2, lim2, 64)
+    d_mov = (d_add[:, -1] - d_in[:, -1]) / po.C_VAL
+    np.testing.assert_array_equal(pm2, po.phase_idx_vec(14.0, cl[:, -1, 0], 64, d_mov))
+    np.testing.assert_array_equal(fp2, po.phase_idx_vec(14.0, cl[:, -1, 0], 64,
+                                                        d_add[:, -1] / po.C_VAL))
+
+
+def test_fixed_grid_search_recovers_injected_signal():
+    """The whole fixed-grid run (moving=False) finds the injected pulsar as the moving one
+    does; its survivors stay referenced to the reference segment."""
+    cfg, dyp_fold, pa, pcfg = _small_search()
+    thr = np.array([2.0, 2.5, 3.0, 3.5, 4.0, 4.5, 5.0])
+    dp = cfg.get_dparams_actual(cfg.niters_ffa)
+    res = po.prune_run(dyp_fold, pa, dp, cfg.tseg_ffa, thr, ref_seg=3, cfg=pcfg, max_sugg=2**16,
+                       moving=False)
+    mov = po.prune_run(dyp_fold, pa, dp, cfg.tseg_ffa, thr, ref_seg=3, cfg=pcfg, max_sugg=2**16)
+    tree = res["tree"]
+    assert tree.size > 0
+    best = int(np.argmax(tree.scores))
+    assert abs(res["report"][best][-3, 0] - 200.37) < 0.02
+    assert abs(tree.leaves[best, 0, 0] - 12.0) <= tree.leaves[best, 0, 1] / 2
+    assert tree.scores[best] > 15
+    assert np.all(tree.leaves[:, -2, 0] == 0.0)   # d0 stays 0: never re-centred
+    assert float(tree.scores.max()) >= 0.85 * float(mov["tree"].scores.max())
