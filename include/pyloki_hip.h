@@ -495,6 +495,7 @@ int pyl_cheb_ascend_complex(const pyl_prune_params* p, const double* leaves_dev,
 #define PYL_EP_ST_OVERFLOW 3  /* passing children exceed max_sugg */
 #define PYL_EP_ST_HIST_FULL 4 /* backtrack history full */
 #define PYL_EP_ST_BUDGET 5    /* survivors exceed a budget the level was enqueued without */
+#define PYL_EP_ST_TREE_OVERFLOW 6 /* forest: one tree's passing children exceed its max_sugg */
 
 /* Optional per-phase timing of pyl_ep_levels: HIP events recorded on the stream around
  * each level's phases (PYL_EP_PH_*), read back by pyl_ep_timer_read (which waits for them
@@ -754,6 +755,60 @@ int pyl_circ_levels(const pyl_prune_params* p, const pyl_circ_params* q, int32_t
                     int64_t seg_stride, const double* in_leaves_dev, const float* in_folds_dev,
                     int64_t n_in, int64_t hist_row0, int32_t first_out, const pyl_ep_loop* L,
                     const pyl_circ_loop* C, void* stream);
+
+/* A forest: ntree searches of one configuration (same grids, reference segment, coords
+ * and thresholds; one series each) through one run of the circular level loop. The
+ * reference fans the searches of a survey out over processes (prune.py:892-922, one
+ * Pruning per task); here every launch of a level serves all of them.
+ * A level's parents are the trees' parents one tree after the other, and so are its held
+ * children, its batches (each tree's own batches of batch_size parents), its children,
+ * survivors and kept rows. Per level l, tab + l * PYL_EP_FOREST_TAB * (ntree + 1) holds,
+ * for tree t, the offsets (each [ntree + 1], non-decreasing) of its parents (PO), batches
+ * (BO), children (CB), scored children (VB), survivors (SO) and kept rows (KO: the budget's,
+ * else SO) in those sequences; row l + 1's PO is level l's KO. Level records hold the
+ * forest's totals; minmax holds every tree's children's score min/max per level.
+ * Each tree's results equal those of pyl_circ_levels on that tree alone, bit for bit: its
+ * backtracks hold parent indices within its own parents. Besides the single loop's stops
+ * (PYL_EP_ST_OVERFLOW: the forest's survivor buffer L->max_sugg), a level stops with
+ * PYL_EP_ST_TREE_OVERFLOW when any tree's survivors exceed F->max_sugg (the caller then
+ * runs the trees one by one from that level) and with PYL_EP_ST_BUDGET when a level
+ * enqueued without the cut finds any tree's survivors above the budget.
+ * The FFA folds of the trees are stacked: tree t's segment rows of level l start at
+ * dyp_fold_dev + t * tree_stride + seg_idx[l] * seg_stride floats. L->max_sugg and
+ * L->cap_* are the forest's capacities (all trees together); n_in: host [ntree], the
+ * input trees' sizes (their rows one after the other in in_*). With L->budget, every tree
+ * keeps at most budget->budget survivors per level, ranked and cut per tree (the network's
+ * level statistics are per tree); budget scratch is sized for L->max_sugg rows. */
+#define PYL_EP_FOREST_MAX 64
+#define PYL_EP_FOREST_TAB 6
+#define PYL_EP_TAB_PO 0
+#define PYL_EP_TAB_BO 1
+#define PYL_EP_TAB_CB 2
+#define PYL_EP_TAB_VB 3
+#define PYL_EP_TAB_SO 4
+#define PYL_EP_TAB_KO 5
+typedef struct pyl_ep_forest {
+  int32_t ntree;       /* trees, 1..PYL_EP_FOREST_MAX */
+  int32_t reserved;
+  int64_t tree_stride; /* floats between consecutive trees' FFA folds */
+  int64_t max_sugg;    /* survivor buffer of each tree (the reference's max_sugg) */
+  int64_t* tab;        /* i64 [nlev + 1][PYL_EP_FOREST_TAB][ntree + 1] (written) */
+  float* minmax;       /* f32 [nlev][ntree][2] (written) */
+  /* per-tree budget state (with L->budget) */
+  uint32_t* sel;       /* u32 [ntree][PYL_BUDGET_SEL_WORDS] */
+  double* stats;       /* f64 [ntree][PYL_BUDGET_STATS] */
+  uint32_t* shist;     /* u32 [ntree][2 * PYL_BUDGET_HIST_BINS] */
+  int64_t* part;       /* i64 [ntree][PYL_EP_SCAN_BLOCKS] */
+  int64_t* kcount;     /* i64 [ntree] */
+} pyl_ep_forest;
+
+int pyl_circ_forest_levels(const pyl_prune_params* p, const pyl_circ_params* q, int32_t flags,
+                           const pyl_prune_coords* coords, const double* thresholds,
+                           const int64_t* seg_idx, int32_t nlev, const float* dyp_fold_dev,
+                           int64_t seg_stride, const double* in_leaves_dev,
+                           const float* in_folds_dev, const int64_t* n_in, int64_t hist_row0,
+                           int32_t first_out, const pyl_ep_loop* L, const pyl_circ_loop* C,
+                           const pyl_ep_forest* F, void* stream);
 
 /* ------------------------------------------------------------------------------------
  * Learned branch ranking (no counterpart in the reference). A small 1-D CNN scores EP

@@ -441,6 +441,46 @@ def batched_throughput(dyps: list, thresholds: np.ndarray, max_sugg: int, hooks:
                                        for x in res))}
         del sb
         torch.cuda.empty_cache()
+    return _survey_summary(out, finals)
+
+
+def forest_throughput(dyps: list, thresholds: np.ndarray, max_sugg: int, hooks: dict,
+                      batch_size: int = 1024, ntree: int | None = None) -> dict:
+    """``batched_throughput`` with the series searched ``ntree`` at a time (default: all)
+    as the trees of one forest (``forest.ForestSearch``): every launch of a level serves
+    all of them. The first pass over the series (untimed) sizes the buffers."""
+    from pyloki_amd.forest import ForestSearch
+
+    ref = dyps[0].nsegments // 2
+    T = len(dyps) if ntree is None else int(ntree)
+    out, finals = {}, {}
+    for name, factory in hooks.items():
+        fs = ForestSearch(dyps[0], thresholds, ref, T, hook_factory=factory,
+                          max_sugg=max_sugg, batch_size=batch_size)
+        for b0 in range(0, len(dyps), T):  # warm-up
+            fs.run(dyps[b0:b0 + T])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = []
+        for b0 in range(0, len(dyps), T):
+            res += fs.run(dyps[b0:b0 + T])
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        finals[name] = [{r.tobytes() for r in np.ascontiguousarray(x["param_sets"])}
+                        for x in res]
+        out[name] = {"ms_per_search": dt / len(dyps) * 1e3,
+                     "trials": int(sum(sum(st.n_leaves for st in x["level_stats"][1:])
+                                       for x in res)),
+                     "resumes": {int(k): v for k, v in fs.stats["resumes"].items()},
+                     "fallbacks": fs.stats["fallbacks"]}
+        del fs
+        torch.cuda.empty_cache()
+    return _survey_summary(out, finals)
+
+
+def _survey_summary(out: dict, finals: dict) -> dict:
+    """Recall of every run's candidates against the exhaustive ones (series by series),
+    and the speedup and trial ratio over the exhaustive search."""
     ex = finals.get("exhaustive")
     for name in out:
         if ex is None:

@@ -25,4 +25,4 @@ void taylor_matrix(int n, double dt, double* out) {
 
 extern "C" const char* pyl_last_error(void) { return pyl::g_last_error.c_str(); }
 
-extern "C" int pyl_abi_version(void) { return 106; }
+extern "C" int pyl_abi_version(void) { return 107; }

@@ -41,6 +41,21 @@ __device__ __forceinline__ int64_t count_of(int64_t n, const int64_t* n_dev) {
   return n_dev ? *n_dev : n;
 }
 
+// Rows of the block's tree and the first of them. One tree: [0, count_of(n, n_dev)). A
+// forest (off not null): tree blockIdx.y's rows [off[y], off[y + 1]), or none when n_dev
+// reads 0 (no tree needs ranking). Per-tree state is offset by blockIdx.y (0 for one tree).
+__device__ __forceinline__ int64_t tree_rows(int64_t n, const int64_t* n_dev,
+                                             const int64_t* off, int64_t* base) {
+  if (!off) {
+    *base = 0;
+    return count_of(n, n_dev);
+  }
+  const int y = blockIdx.y;
+  *base = off[y];
+  if (n_dev && *n_dev == 0) return 0;
+  return off[y + 1] - off[y];
+}
+
 // Order-preserving image of a float: larger key, larger unsigned value.
 __device__ __forceinline__ uint32_t key_bits(float x) {
   const uint32_t b = __float_as_uint(x);
@@ -53,10 +68,14 @@ __device__ __forceinline__ uint32_t key_bits(float x) {
 __global__ void __launch_bounds__(kT)
 feat_stats_part_kernel(const float* __restrict__ scores, int64_t n_arg,
                        const int64_t* __restrict__ n_dev, double* __restrict__ stats,
-                       uint32_t* __restrict__ shist) {
+                       uint32_t* __restrict__ shist, const int64_t* __restrict__ off = nullptr) {
   __shared__ double s_sum[kT], s_sq[kT];
   __shared__ float s_mn[kT], s_mx[kT];
-  const int64_t n = count_of(n_arg, n_dev);
+  int64_t base;
+  const int64_t n = tree_rows(n_arg, n_dev, off, &base);
+  scores += base;
+  stats += (int64_t)blockIdx.y * PYL_BUDGET_STATS;
+  shist += (int64_t)blockIdx.y * 2 * kBins;
   if (blockIdx.x == 0)  // the histogram of the next launch starts from zero
     for (int j = threadIdx.x; j < 2 * kBins; j += kT) shist[j] = 0;
   int64_t b0, b1;
@@ -133,9 +152,13 @@ __device__ __forceinline__ int score_bin(float x, const double* f) {
 __global__ void __launch_bounds__(kT)
 feat_hist_kernel(const float* __restrict__ scores, int64_t n_arg,
                  const int64_t* __restrict__ n_dev, double* __restrict__ stats,
-                 uint32_t* __restrict__ shist) {
+                 uint32_t* __restrict__ shist, const int64_t* __restrict__ off = nullptr) {
   __shared__ uint32_t h[kBins];
-  const int64_t n = count_of(n_arg, n_dev);
+  int64_t base;
+  const int64_t n = tree_rows(n_arg, n_dev, off, &base);
+  scores += base;
+  stats += (int64_t)blockIdx.y * PYL_BUDGET_STATS;
+  shist += (int64_t)blockIdx.y * 2 * kBins;
   if (blockIdx.x > 0 && (int64_t)blockIdx.x * kT >= n) return;  // the whole block
   double f[4];
   stats_final(stats, n, f);
@@ -174,9 +197,16 @@ __global__ void __launch_bounds__(kT)
 feat_write_kernel(const double* __restrict__ leaves, const float* __restrict__ scores,
                   int64_t n_arg, const int64_t* __restrict__ n_dev, int P, int level,
                   int nlevels, const double* __restrict__ stats,
-                  const uint32_t* __restrict__ shist, float* __restrict__ feats) {
+                  const uint32_t* __restrict__ shist, float* __restrict__ feats,
+                  const int64_t* __restrict__ off = nullptr) {
   __shared__ uint32_t cnt[kBins], cdf[kBins];
-  const int64_t n = count_of(n_arg, n_dev);
+  int64_t base;
+  const int64_t n = tree_rows(n_arg, n_dev, off, &base);
+  leaves += base * (P + 2) * 2;
+  scores += base;
+  feats += base * kFeat;
+  stats += (int64_t)blockIdx.y * PYL_BUDGET_STATS;
+  shist += (int64_t)blockIdx.y * 2 * kBins;
   if ((int64_t)blockIdx.x * kT >= n) return;  // the whole block
   hist_cdf(shist, cnt, cdf);
   const double* f = stats + 4 * kScanBlocks;
@@ -207,9 +237,14 @@ feat_write_kernel(const double* __restrict__ leaves, const float* __restrict__ s
 __global__ void __launch_bounds__(kT)
 hybrid_key_kernel(float* __restrict__ keys, const float* __restrict__ feats, int64_t n_arg,
                   const int64_t* __restrict__ n_dev, const double* __restrict__ stats,
-                  const uint32_t* __restrict__ shist) {
+                  const uint32_t* __restrict__ shist, const int64_t* __restrict__ off = nullptr) {
   __shared__ uint32_t cnt[kBins], cdf[kBins];
-  const int64_t n = count_of(n_arg, n_dev);
+  int64_t base;
+  const int64_t n = tree_rows(n_arg, n_dev, off, &base);
+  keys += base;
+  feats += base * kFeat;
+  stats += (int64_t)blockIdx.y * PYL_BUDGET_STATS;
+  shist += (int64_t)blockIdx.y * 2 * kBins;
   if ((int64_t)blockIdx.x * kT >= n) return;  // the whole block
   hist_cdf(shist, cnt, cdf);
   const double* f = stats + 4 * kScanBlocks;
@@ -228,13 +263,19 @@ inline unsigned stride_grid(int64_t cap) {
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>((cap + kT - 1) / kT, 512));
 }
 
+// off / ntree: a forest's trees (off = their survivor offsets), each with its own
+// statistics; n_cap then bounds one tree's rows.
 int features_enqueue(const double* leaves, const float* scores, int64_t n_cap,
                      const int64_t* n_dev, int P, int level, int nlevels, double* stats,
-                     uint32_t* shist, float* feats, hipStream_t s) {
-  feat_stats_part_kernel<<<kScanBlocks, kT, 0, s>>>(scores, n_cap, n_dev, stats, shist);
-  feat_hist_kernel<<<stride_grid(n_cap), kT, 0, s>>>(scores, n_cap, n_dev, stats, shist);
-  feat_write_kernel<<<stride_grid(n_cap), kT, 0, s>>>(leaves, scores, n_cap, n_dev, P, level,
-                                                      nlevels, stats, shist, feats);
+                     uint32_t* shist, float* feats, hipStream_t s,
+                     const int64_t* off = nullptr, int ntree = 1) {
+  const unsigned T = (unsigned)ntree;
+  feat_stats_part_kernel<<<dim3(kScanBlocks, T), kT, 0, s>>>(scores, n_cap, n_dev, stats,
+                                                             shist, off);
+  feat_hist_kernel<<<dim3(stride_grid(n_cap), T), kT, 0, s>>>(scores, n_cap, n_dev, stats,
+                                                              shist, off);
+  feat_write_kernel<<<dim3(stride_grid(n_cap), T), kT, 0, s>>>(
+      leaves, scores, n_cap, n_dev, P, level, nlevels, stats, shist, feats, off);
   PYL_CHECK_LAUNCH("feat_write_kernel");
   return PYL_OK;
 }
@@ -263,15 +304,18 @@ __global__ void sel_init_kernel(int64_t* __restrict__ rec, int64_t budget,
 // reaches the remaining budget, and clears the histogram and the counter for the next pass.
 __global__ void __launch_bounds__(kT)
 sel_hist_kernel(const float* __restrict__ keys, const int64_t* __restrict__ m_dev,
-                uint32_t* __restrict__ sel, int shift) {
+                uint32_t* __restrict__ sel, int shift, const int64_t* __restrict__ off = nullptr) {
   __shared__ uint32_t h[256];
   __shared__ uint32_t suf[257];
   __shared__ int s_last;
   const int t = threadIdx.x;
+  sel += (int64_t)blockIdx.y * PYL_BUDGET_SEL_WORDS;  // a forest: the block's tree
   if (sel[kSelAll] != 0) return;  // every block: all rows are kept
   h[t] = 0;
   __syncthreads();
-  const int64_t m = *m_dev;
+  int64_t base;
+  const int64_t m = tree_rows(0, m_dev, off, &base);
+  keys += base;
   const uint32_t prefix = sel[kSelPrefix], mask = sel[kSelMask];
   for (int64_t i = (int64_t)blockIdx.x * kT + t; i < m; i += (int64_t)gridDim.x * kT) {
     const uint32_t u = key_bits(keys[i]);
@@ -401,7 +445,191 @@ __global__ void budget_record_kernel(int64_t* __restrict__ rec, int64_t n) {
   if (w == kRecNsurv) rec[w] = n;
 }
 
+// ---- Forests: the budget per tree ----------------------------------------------------
+// Tree t's survivors are rows [so[t], so[t + 1]) of the level's materialised rows; each
+// tree is ranked and cut on its own (its own select state, statistics and histogram), its
+// kept rows' indices go to kept[so[t]..], and the kept rows of all trees are gathered one
+// tree after the other (ko = their offsets, the next level's parents).
+
+// One block per tree. The level's rows are ranked (NRANK = NSURV) when any tree's
+// survivors exceed the budget; the record starts at zero, so the blocks that find one
+// all write the same value.
+__global__ void fsel_init_kernel(int64_t* __restrict__ rec, const int64_t* __restrict__ so,
+                                 int64_t budget, uint32_t* __restrict__ sel) {
+  const int y = blockIdx.x;
+  const int64_t m = so[y + 1] - so[y];
+  uint32_t* st = sel + (int64_t)y * PYL_BUDGET_SEL_WORDS;
+  if (threadIdx.x == 0) {
+    st[kSelPrefix] = 0;
+    st[kSelMask] = 0;
+    st[kSelRemain] = (uint32_t)std::min<int64_t>(budget, m);
+    st[kSelAll] = m <= budget ? 1u : 0u;
+    st[kSelDone] = 0;
+    if (m > budget) rec[PYL_EP_REC_NRANK] = rec[kRecNsurv];
+  }
+  st[kSelHist + threadIdx.x] = 0;
+}
+
+struct FKeepSrc {
+  const float* keys;
+  const uint32_t* sel;
+  __device__ int64_t operator()(int64_t i, int y) const {
+    return KeepSrc{keys, sel + (int64_t)y * PYL_BUDGET_SEL_WORDS}(i);
+  }
+};
+
+struct FKeepSink {
+  int64_t* kept;
+  const uint32_t* sel;
+  const int64_t* so;
+  int64_t* kcount;
+  __device__ void operator()(int64_t i, int64_t excl, int64_t x, int y) const {
+    if (!x) return;
+    const uint32_t* st = sel + (int64_t)y * PYL_BUDGET_SEL_WORDS;
+    int64_t* kt = kept + so[y];
+    const int64_t above = excl & 0xffffffffLL, at = excl >> 32;
+    if (st[kSelAll]) {
+      kt[above] = i;
+      return;
+    }
+    const int64_t need = st[kSelRemain];
+    if (x & 0xffffffffLL) kt[above + (at < need ? at : need)] = i;
+    else if (at < need) kt[above + at] = i;
+  }
+  __device__ void finish(int y, int64_t tot) const {
+    const uint32_t* st = sel + (int64_t)y * PYL_BUDGET_SEL_WORDS;
+    const int64_t need = st[kSelRemain];
+    kcount[y] = st[kSelAll] ? so[y + 1] - so[y]
+                            : (tot & 0xffffffffLL) + std::min<int64_t>(tot >> 32, need);
+  }
+};
+
+// The kept rows' offsets per tree (the next level's parents) and the level's totals. With
+// pass set (a level enqueued without the cut) every tree keeps its survivors, or, when
+// any tree's exceed the budget, the level stops with PYL_EP_ST_BUDGET and keeps none.
+__global__ void fkeep_offsets_kernel(int64_t* __restrict__ rec, int64_t* __restrict__ tab,
+                                     int64_t* __restrict__ tab_next, int T,
+                                     const int64_t* __restrict__ kcount, int pass,
+                                     int64_t budget) {
+  if (threadIdx.x != 0) return;
+  const int64_t* so = tab + PYL_EP_TAB_SO * (T + 1);
+  int64_t* ko = tab + PYL_EP_TAB_KO * (T + 1);
+  bool over = false;
+  if (pass)
+    for (int t = 0; t < T; ++t) over = over || so[t + 1] - so[t] > budget;
+  if (over && rec[kRecStatus] == 0) rec[kRecStatus] = PYL_EP_ST_BUDGET;
+  int64_t acc = 0;
+  for (int t = 0; t < T; ++t) {
+    ko[t] = acc;
+    tab_next[t] = acc;  // PO of the next level
+    acc += over ? 0 : (pass ? so[t + 1] - so[t] : kcount[t]);
+  }
+  ko[T] = acc;
+  tab_next[T] = acc;
+  rec[PYL_EP_REC_NKEPT] = acc;
+  rec[kRecWords + kRecNpar] = acc;
+  rec[kRecWords + kRecHist] = rec[kRecHist] + acc;
+}
+
+// One wavefront per kept row r (tree t, ko[t] <= r < ko[t + 1]): row kept[so[t] + r -
+// ko[t]] of the materialised rows (row r itself when kept is null), into o_* and history
+// row HIST + r.
+__global__ void __launch_bounds__(kT)
+fgather_kernel(const int64_t* __restrict__ tab, int T, const int64_t* __restrict__ kept,
+               const int64_t* __restrict__ rec, int P, int row_floats,
+               const double* __restrict__ leaves, const float* __restrict__ folds,
+               const float* __restrict__ scores, const int32_t* __restrict__ bt,
+               double* __restrict__ o_leaves, float* __restrict__ o_folds,
+               float* __restrict__ o_scores, int32_t* __restrict__ o_hist) {
+  const int64_t k = rec[kRecWords + kRecNpar];
+  const int lane = threadIdx.x & 63;
+  const int64_t r = (int64_t)blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
+  if (r >= k) return;
+  int64_t i = r;
+  if (kept) {
+    const int64_t* so = tab + PYL_EP_TAB_SO * (T + 1);
+    const int64_t* ko = tab + PYL_EP_TAB_KO * (T + 1);
+    int lo = 0, hi = T;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (ko[mid] <= r) lo = mid;
+      else hi = mid;
+    }
+    i = kept[so[lo] + r - ko[lo]];
+  }
+  const int nl = (P + 2) * 2;
+  const int nb = P + 2;
+  if (lane < nl) o_leaves[r * nl + lane] = leaves[i * nl + lane];
+  for (int j = lane; j < row_floats; j += 64)
+    o_folds[r * row_floats + j] = folds[i * row_floats + j];
+  if (lane == 0) o_scores[r] = scores[i];
+  if (lane < nb) o_hist[(rec[kRecHist] + r) * nb + lane] = bt[i * nb + lane];
+}
+
 }  // namespace
+
+int pyl::ep_budget_forest_enqueue(const pyl_ep_budget* b, const pyl_ep_forest* F, int P,
+                                  int level, int64_t* tab, int64_t* tab_next,
+                                  const double* leaves, const float* folds,
+                                  const float* scores, const int32_t* bt, int64_t n_cap,
+                                  int64_t* rec, double* o_leaves, float* o_folds,
+                                  float* o_scores, int32_t* o_hist, hipStream_t s) {
+  PYL_REQUIRE(b != nullptr && F != nullptr, "null budget");
+  PYL_REQUIRE(level >= 0 && level <= b->nlevels, "level %d outside the search", level);
+  PYL_REQUIRE(b->kind == PYL_BUDGET_SCORE || b->kind == PYL_BUDGET_CNN ||
+                  b->kind == PYL_BUDGET_HYBRID,
+              "unknown budget kind %d", b->kind);
+  PYL_REQUIRE(b->budget >= 1, "budget must be >= 1");
+  PYL_REQUIRE(n_cap >= 0 && n_cap < 0x7fffffffLL, "too many survivors for the budget");
+  const int T = F->ntree;
+  const int row_floats = b->complex_folds ? 4 * (b->nbins / 2 + 1) : 2 * b->nbins;
+  const int64_t* so = tab + PYL_EP_TAB_SO * (T + 1);
+  const int64_t rows = std::min<int64_t>(n_cap, (int64_t)T * b->budget);
+  const unsigned g_rows = (unsigned)std::max<int64_t>(1, pyl::ceil_div(rows, kT / 64));
+  if (b->level_mask != nullptr && b->level_mask[level] == 0) {
+    fkeep_offsets_kernel<<<1, 64, 0, s>>>(rec, tab, tab_next, T, F->kcount, 1, b->budget);
+    fgather_kernel<<<g_rows, kT, 0, s>>>(tab, T, nullptr, rec, P, row_floats, leaves, folds,
+                                         scores, bt, o_leaves, o_folds, o_scores, o_hist);
+    PYL_CHECK_LAUNCH("fgather_kernel");
+    return PYL_OK;
+  }
+  const bool net = b->kind != PYL_BUDGET_SCORE;
+  PYL_REQUIRE(b->kept && F->sel && F->part && F->kcount &&
+                  (!net || (b->params && b->keys && b->feats && F->stats && F->shist)),
+              "null budget buffer");
+  const int64_t* r_dev = rec + PYL_EP_REC_NRANK;  // 0 when every tree's survivors fit
+  const int64_t tree_cap = std::min<int64_t>(n_cap, F->max_sugg);
+  fsel_init_kernel<<<(unsigned)T, 256, 0, s>>>(rec, so, b->budget, F->sel);
+  const float* keys = scores;
+  if (net) {
+    int rc = features_enqueue(leaves, scores, tree_cap, r_dev, P, level, b->nlevels, F->stats,
+                              F->shist, b->feats, s, so, T);
+    if (rc != PYL_OK) return rc;
+    rc = pyl::cnn_forward_count(b->params, folds, n_cap, r_dev, b->nbins, b->complex_folds,
+                                b->feats, b->keys, b->precision == PYL_BUDGET_BF16, s);
+    if (rc != PYL_OK) return rc;
+    keys = b->keys;
+    if (b->kind == PYL_BUDGET_HYBRID) {
+      const dim3 gs(kScanBlocks, (unsigned)T), gr(stride_grid(tree_cap), (unsigned)T);
+      feat_stats_part_kernel<<<gs, kT, 0, s>>>(b->keys, tree_cap, r_dev, F->stats, F->shist,
+                                               so);
+      feat_hist_kernel<<<gr, kT, 0, s>>>(b->keys, tree_cap, r_dev, F->stats, F->shist, so);
+      hybrid_key_kernel<<<gr, kT, 0, s>>>(b->keys, b->feats, tree_cap, r_dev, F->stats,
+                                          F->shist, so);
+      PYL_CHECK_LAUNCH("hybrid_key_kernel");
+    }
+  }
+  const dim3 gsel(stride_grid(tree_cap), (unsigned)T);
+  for (int shift = 24; shift >= 0; shift -= 8)
+    sel_hist_kernel<<<gsel, kT, 0, s>>>(keys, nullptr, F->sel, shift, so);
+  chunked_scan_seg(FKeepSrc{keys, F->sel}, FKeepSink{b->kept, F->sel, so, F->kcount}, so, T,
+                   F->part, s);
+  fkeep_offsets_kernel<<<1, 64, 0, s>>>(rec, tab, tab_next, T, F->kcount, 0, b->budget);
+  fgather_kernel<<<g_rows, kT, 0, s>>>(tab, T, b->kept, rec, P, row_floats, leaves, folds,
+                                       scores, bt, o_leaves, o_folds, o_scores, o_hist);
+  PYL_CHECK_LAUNCH("fgather_kernel");
+  return PYL_OK;
+}
 
 int pyl::ep_budget_enqueue(const pyl_ep_budget* b, int P, int level, const double* leaves,
                            const float* folds, const float* scores, const int32_t* bt,

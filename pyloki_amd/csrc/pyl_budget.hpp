@@ -29,4 +29,15 @@ int ep_budget_level_enqueue(const pyl_ep_budget* b, int P, int level, const doub
                             int64_t n_cap, int64_t* rec, double* o_leaves, float* o_folds,
                             float* o_scores, int32_t* o_hist, hipStream_t s);
 
+// The budget of a forest's level (pyl_ep_forest): every tree ranked and cut on its own.
+// tab is the level's tree table (its SO row read, KO written), tab_next the next level's
+// (PO written); the record as in ep_budget_enqueue, with the forest's totals. With
+// b->level_mask flagging the level off, one pass as in ep_budget_level_enqueue (any tree
+// above the budget stops the level).
+int ep_budget_forest_enqueue(const pyl_ep_budget* b, const pyl_ep_forest* F, int P, int level,
+                             int64_t* tab, int64_t* tab_next, const double* leaves,
+                             const float* folds, const float* scores, const int32_t* bt,
+                             int64_t n_cap, int64_t* rec, double* o_leaves, float* o_folds,
+                             float* o_scores, int32_t* o_hist, hipStream_t s);
+
 }  // namespace pyl

@@ -131,6 +131,84 @@ __device__ __forceinline__ int64_t dev_count(int64_t n, const int64_t* n_dev) {
   return n_dev ? *n_dev : n;
 }
 
+// ---- Forests: several trees of one configuration in one run of the level loop ---------
+// A level's parents (and so its held children, batches, children and survivors) are the
+// trees' ones one after the other. Per level, the tree tables hold, for tree t, the offsets
+// po (parents), bo (batches), cb (children), vb (validated children), so (survivors) and
+// ko (kept rows, the next level's parents), each [T + 1]. Batches never straddle trees:
+// tree t's batches are the reference's batches of its own parents.
+constexpr int kTabArrays = 6;
+struct FTab {
+  const int64_t* tab;   // this level's tables; null: a single tree
+  int32_t T;
+  int64_t row_stride;   // segment rows between consecutive trees' FFA folds
+  float* minmax;        // this level's per-tree [T][2] (forest only)
+};
+
+// Largest t in [0, T) with off[t] <= x (off non-decreasing): the tree holding item x.
+__device__ __forceinline__ int f_tree(const int64_t* off, int T, int64_t x) {
+  int lo = 0, hi = T;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= x) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Batch of parent par, and the parent range of batch b.
+__device__ __forceinline__ int64_t f_batch(const FTab& f, int64_t par, int64_t bs) {
+  if (!f.tab) return par / bs;
+  const int64_t* po = f.tab;
+  const int64_t* bo = f.tab + (f.T + 1);
+  const int t = f_tree(po, f.T, par);
+  return bo[t] + (par - po[t]) / bs;
+}
+__device__ __forceinline__ void f_batch_parents(const FTab& f, int64_t b, int64_t bs,
+                                                int64_t n_par, int64_t* p0, int64_t* p1) {
+  if (!f.tab) {
+    *p0 = b * bs;
+    *p1 = std::min<int64_t>(*p0 + bs, n_par);
+    return;
+  }
+  const int64_t* po = f.tab;
+  const int64_t* bo = f.tab + (f.T + 1);
+  const int t = f_tree(bo, f.T, b);
+  *p0 = po[t] + (b - bo[t]) * bs;
+  *p1 = std::min<int64_t>(*p0 + bs, po[t + 1]);
+}
+// Segment row of a child of parent par in the stacked trees' folds.
+__device__ __forceinline__ int32_t f_row(const FTab& f, int64_t par, int32_t row) {
+  if (!f.tab) return row;
+  return row + (int32_t)(f_tree(f.tab, f.T, par) * f.row_stride);
+}
+
+// Per-tree score minimum / maximum of a block's children (one atomic pair per wavefront
+// when its children are of one tree, else per child).
+__device__ __forceinline__ void forest_minmax(float sc, bool valid, int t, float* mm) {
+  int tlo = valid ? t : INT_MAX, thi = valid ? t : -1;
+  for (int o = 32; o > 0; o >>= 1) {
+    tlo = min(tlo, __shfl_xor(tlo, o));
+    thi = max(thi, __shfl_xor(thi, o));
+  }
+  if (thi < 0) return;  // no child in the wavefront
+  if (tlo == thi) {
+    float mn = valid ? sc : FLT_MAX;
+    float mx = valid ? sc : -FLT_MAX;
+    for (int o = 32; o > 0; o >>= 1) {
+      mn = fminf(mn, __shfl_xor(mn, o));
+      mx = fmaxf(mx, __shfl_xor(mx, o));
+    }
+    if ((threadIdx.x & 63) == 0) {
+      atomic_min_f32(&mm[2 * tlo], mn);
+      atomic_max_f32(&mm[2 * tlo + 1], mx);
+    }
+  } else if (valid) {
+    atomic_min_f32(&mm[2 * t], sc);
+    atomic_max_f32(&mm[2 * t + 1], sc);
+  }
+}
+
 // Per-item bodies of the thread-per-item kernels (defined below the kernels).
 __device__ __forceinline__ void circ_branch_one(const LevelConst& k, const CircConst& q,
                                                 const double* __restrict__ leaves,
@@ -264,7 +342,7 @@ __device__ __forceinline__ void circ_expand_one(const LevelConst& k, const CircC
   }
 }
 
-template <bool kComplex>
+template <bool kComplex, bool kForest = false>
 __global__ void __launch_bounds__(kCBlk)
 circ_children_kernel(LevelConst k, CircConst q, const double* __restrict__ leaves,
                      const float* __restrict__ folds, pyl_circ_level L,
@@ -273,7 +351,8 @@ circ_children_kernel(LevelConst k, CircConst q, const double* __restrict__ leave
                      const float* __restrict__ seg, int32_t* __restrict__ c_parent,
                      int32_t* __restrict__ c_row, int32_t* __restrict__ c_shift,
                      double* __restrict__ c_phase, float* __restrict__ c_score,
-                     float* __restrict__ minmax, uint8_t* __restrict__ has_taylor) {
+                     float* __restrict__ minmax, uint8_t* __restrict__ has_taylor,
+                     FTab f = FTab{}) {
   extern __shared__ float S[];  // [(B + wmax)][kStride]
   __shared__ int32_t s_parent[kBlk], s_row[kBlk], s_shift[kBlk];
   const int tid = threadIdx.x;
@@ -292,7 +371,10 @@ circ_children_kernel(LevelConst k, CircConst q, const double* __restrict__ leave
     circ_decode(k, leaves, L, par, s1, L.c_sub[c], ch);
     const int8_t reg = L.c_region[c];
     // The level loop's batch flags: a batch holds a Taylor-region child (cleared per level).
-    if (has_taylor && reg == PYL_REGION_TAYLOR) has_taylor[par / q.batch_size] = 1;
+    if (has_taylor && reg == PYL_REGION_TAYLOR) {
+      if constexpr (kForest) has_taylor[f_batch(f, par, q.batch_size)] = 1;
+      else has_taylor[par / q.batch_size] = 1;
+    }
     double da[kN], di[kN];
     region_shift(reg, ch.val, q.dt_add, k.tadd, da);
     region_shift(reg, ch.val, q.dt_init, k.tinit, di);
@@ -315,7 +397,8 @@ circ_children_kernel(LevelConst k, CircConst q, const double* __restrict__ leave
     c_shift[v] = sh;
     c_phase[v] = phase;
     s_parent[tid] = (int32_t)par;
-    s_row[tid] = row;
+    if constexpr (kForest) s_row[tid] = f_row(f, par, row);  // the tree's own fold
+    else s_row[tid] = row;
     s_shift[tid] = sh;
     if constexpr (kComplex)
       cphasors(phase, B,
@@ -352,10 +435,15 @@ circ_children_kernel(LevelConst k, CircConst q, const double* __restrict__ leave
   }
   const float sc = block_score_columns(k, Sc, nvalid, tid, kCBlk);
   if (tid < nvalid) c_score[v0 + tid] = sc;
-  block_minmax(sc, tid < nvalid, minmax);
+  if constexpr (kForest) {
+    const int t = tid < nvalid ? f_tree(f.tab, f.T, s_parent[tid]) : 0;
+    forest_minmax(sc, tid < nvalid, t, f.minmax);
+  } else {
+    block_minmax(sc, tid < nvalid, minmax);
+  }
 }
 
-template <bool kComplex>
+template <bool kComplex, bool kForest = false>
 __global__ void __launch_bounds__(kCBlk)
 circ_materialize_kernel(LevelConst k, CircConst q, const double* __restrict__ leaves,
                         const float* __restrict__ folds, pyl_circ_level L,
@@ -367,7 +455,8 @@ circ_materialize_kernel(LevelConst k, CircConst q, const double* __restrict__ le
                         int64_t n_surv_arg, const int64_t* __restrict__ n_dev,
                         double* __restrict__ o_leaves,
                         float* __restrict__ o_folds, float* __restrict__ o_scores,
-                        int32_t* __restrict__ o_bt_base, const int64_t* __restrict__ bt_row0) {
+                        int32_t* __restrict__ o_bt_base, const int64_t* __restrict__ bt_row0,
+                        FTab f = FTab{}) {
   __shared__ int64_t s_v[kBlk];
   __shared__ int32_t s_fpar[kBlk], s_frow[kBlk], s_fsh[kBlk];  // real folds
   extern __shared__ float2 s_ph[];  // complex: [kBlk][B/2 + 1] phasors, then rows, parents
@@ -408,14 +497,21 @@ circ_materialize_kernel(LevelConst k, CircConst q, const double* __restrict__ le
     // The fixed-grid resolve (circular.py:531-590) leaves the flag as it is.
     double flag = ch.flag;
     if (!k.fixed) {
-      if (reg == PYL_REGION_SNAP) flag = 1.0;
-      else if (reg == PYL_REGION_CRACKLE) flag = has_taylor[par / q.batch_size] ? 0.0 : 2.0;
+      if (reg == PYL_REGION_SNAP) {
+        flag = 1.0;
+      } else if (reg == PYL_REGION_CRACKLE) {
+        int64_t b;
+        if constexpr (kForest) b = f_batch(f, par, q.batch_size);
+        else b = par / q.batch_size;
+        flag = has_taylor[b] ? 0.0 : 2.0;
+      }
     }
     ol[2 * (kP + 1)] = ch.f0;
     ol[2 * (kP + 1) + 1] = flag;
     o_scores[s] = c_score[v];
     int32_t* bt = o_bt + s * kRows;
-    bt[0] = (int32_t)par;
+    if constexpr (kForest) bt[0] = (int32_t)(par - f.tab[f_tree(f.tab, f.T, par)]);  // in its tree
+    else bt[0] = (int32_t)par;
     const int32_t row = c_row[v];
     for (int qq = 1; qq <= kP; ++qq) bt[qq] = 0;
     bt[kP - 1] = row / k.n_freq;
@@ -428,7 +524,8 @@ circ_materialize_kernel(LevelConst k, CircConst q, const double* __restrict__ le
     int32_t* s_rw = reinterpret_cast<int32_t*>(s_ph + (size_t)kBlk * (B / 2 + 1));
     int32_t* s_pa = s_rw + kBlk;
     if (tid < nvalid) {
-      s_rw[tid] = c_row[s_v[tid]];
+      if constexpr (kForest) s_rw[tid] = f_row(f, c_parent[s_v[tid]], c_row[s_v[tid]]);
+      else s_rw[tid] = c_row[s_v[tid]];
       s_pa[tid] = c_parent[s_v[tid]];
     }
     __syncthreads();
@@ -439,7 +536,8 @@ circ_materialize_kernel(LevelConst k, CircConst q, const double* __restrict__ le
     if (tid < nvalid) {
       const int64_t v = s_v[tid];
       s_fpar[tid] = c_parent[v];
-      s_frow[tid] = c_row[v];
+      if constexpr (kForest) s_frow[tid] = f_row(f, c_parent[v], c_row[v]);
+      else s_frow[tid] = c_row[v];
       s_fsh[tid] = c_shift[v];
     }
     __syncthreads();
@@ -776,6 +874,7 @@ struct HeldOffSink {
   int64_t* off1;
   int64_t* rec;
   int64_t cap_child, max_sugg, cap_hist, batch_size, branch_max;
+  const int64_t* nb_forest;  // a forest's batch count (its bo[T]), else null
   __device__ void operator()(int64_t i, int64_t excl, int64_t x) const { off1[i + 1] = excl + x; }
   __device__ void finish(int64_t n_s1) const {
     off1[0] = 0;
@@ -786,7 +885,8 @@ struct HeldOffSink {
     if (st == 0 && rec[kRecHist] + max_sugg > cap_hist) st = PYL_EP_ST_HIST_FULL;
     rec[kRecStatus] = st;
     rec[PYL_EP_REC_NHELD] = st ? 0 : n_s1;
-    rec[PYL_EP_REC_NBATCH] = st ? 0 : (rec[kRecNpar] + batch_size - 1) / batch_size;
+    rec[PYL_EP_REC_NBATCH] =
+        st ? 0 : (nb_forest ? *nb_forest : (rec[kRecNpar] + batch_size - 1) / batch_size);
   }
 };
 
@@ -813,14 +913,14 @@ __global__ void circ_batch_totals_kernel(const int64_t* __restrict__ off1,
                                          const int64_t* __restrict__ rec, int64_t batch_size,
                                          int64_t* __restrict__ batch_tot,
                                          int64_t* __restrict__ batch_nk,
-                                         uint8_t* __restrict__ has_taylor) {
+                                         uint8_t* __restrict__ has_taylor, FTab f) {
   const int64_t nb = rec[PYL_EP_REC_NBATCH];
   const int64_t n_par = rec[kRecNpar];
   for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < nb;
        b += (int64_t)gridDim.x * blockDim.x) {
     has_taylor[b] = 0;  // set again after the children are scored
-    const int64_t p0 = b * batch_size;
-    const int64_t p1 = std::min<int64_t>(p0 + batch_size, n_par);
+    int64_t p0, p1;
+    f_batch_parents(f, b, batch_size, n_par, &p0, &p1);
     const int64_t d = held_off[off1[p1]] - held_off[off1[p0]];
     const int64_t nk = d & 0xffffffffLL;
     batch_nk[b] = nk;
@@ -850,12 +950,14 @@ __global__ void circ_positions_kernel(const int64_t* __restrict__ off1,
                                       const int64_t* __restrict__ batch_off,
                                       const int64_t* __restrict__ batch_nk,
                                       const int64_t* __restrict__ n_dev, int64_t batch_size,
-                                      int64_t* __restrict__ s1_pos) {
+                                      int64_t* __restrict__ s1_pos, FTab f) {
   const int64_t n = *n_dev;
   for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < n;
        s += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t b = s1_parent[s] / batch_size;
-    const int64_t d = held_off[s] - held_off[off1[b * batch_size]];
+    const int64_t b = f_batch(f, s1_parent[s], batch_size);
+    int64_t p0, p1;
+    f_batch_parents(f, b, batch_size, INT64_MAX, &p0, &p1);
+    const int64_t d = held_off[s] - held_off[off1[p0]];
     s1_pos[s] = batch_off[b] + (s1_expand[s] ? batch_nk[b] + (d >> 32) : (d & 0xffffffffLL));
   }
 }
@@ -869,13 +971,116 @@ struct ValidSink {
   __device__ void finish(int64_t nv) const { rec[kRecNvalid] = nv; }
 };
 
+// ---- Forest tables -------------------------------------------------------------------
+struct TreeSizes {
+  int64_t n[PYL_EP_FOREST_MAX];
+};
 
-template <bool kComplex>
+// One block: every level's tables cleared, per-tree min/max reset, and level 0's parent
+// offsets from the input trees' sizes.
+__global__ void forest_init_kernel(int64_t* __restrict__ tab, float* __restrict__ minmax,
+                                   int T, int nlev, TreeSizes n_in) {
+  const int64_t words = (int64_t)(nlev + 1) * PYL_EP_FOREST_TAB * (T + 1);
+  for (int64_t i = threadIdx.x; i < words; i += blockDim.x) tab[i] = 0;
+  for (int64_t i = threadIdx.x; i < (int64_t)nlev * T; i += blockDim.x) {
+    minmax[2 * i] = INFINITY;
+    minmax[2 * i + 1] = -INFINITY;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t acc = 0;
+    for (int t = 0; t < T; ++t) {
+      tab[t] = acc;
+      acc += n_in.n[t];
+    }
+    tab[T] = acc;
+  }
+}
+
+// A level's batch offsets: each tree's parents in batches of bs (circular.py:359-394).
+__global__ void forest_begin_kernel(int64_t* __restrict__ tab, int T, int64_t bs) {
+  if (threadIdx.x != 0) return;
+  const int64_t* po = tab + PYL_EP_TAB_PO * (T + 1);
+  int64_t* bo = tab + PYL_EP_TAB_BO * (T + 1);
+  int64_t acc = 0;
+  for (int t = 0; t < T; ++t) {
+    bo[t] = acc;
+    acc += (po[t + 1] - po[t] + bs - 1) / bs;
+  }
+  bo[T] = acc;
+}
+
+// First index i in [0, n) with a[i] >= x (a increasing), else n.
+__device__ __forceinline__ int64_t lower_bound_i64(const int64_t* a, int64_t n, int64_t x) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (a[mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// After the survivor scan: every tree's children (its batches' first child), scored
+// children and survivors; a tree with more survivors than tree_max stops the level
+// (PYL_EP_ST_TREE_OVERFLOW: nothing is materialised). Without a budget, the survivors are the
+// next level's parents. One block of at least T + 1 threads.
+__global__ void forest_split_kernel(int64_t* __restrict__ rec, int64_t* __restrict__ tab,
+                                    int64_t* __restrict__ tab_next, int T,
+                                    const int64_t* __restrict__ batch_off,
+                                    const int64_t* __restrict__ valid,
+                                    const int64_t* __restrict__ surv, int64_t tree_max,
+                                    int budget) {
+  __shared__ int s_over;
+  const int t = threadIdx.x;
+  const int64_t* bo = tab + PYL_EP_TAB_BO * (T + 1);
+  int64_t* cb = tab + PYL_EP_TAB_CB * (T + 1);
+  int64_t* vb = tab + PYL_EP_TAB_VB * (T + 1);
+  int64_t* so = tab + PYL_EP_TAB_SO * (T + 1);
+  int64_t* ko = tab + PYL_EP_TAB_KO * (T + 1);
+  if (t == 0) s_over = 0;
+  __syncthreads();
+  const bool ok = rec[kRecStatus] == 0;
+  if (t <= T) {
+    int64_t c = 0, v = 0, sv = 0;
+    if (ok) {
+      const int64_t nb = rec[PYL_EP_REC_NBATCH], nc = rec[kRecNchild];
+      c = (t < T && bo[t] < nb) ? batch_off[bo[t]] : nc;
+      v = lower_bound_i64(valid, rec[kRecNvalid], c);
+      sv = lower_bound_i64(surv, rec[kRecNsurv], v);
+    }
+    cb[t] = c;
+    vb[t] = v;
+    so[t] = sv;
+  }
+  __syncthreads();
+  if (t < T && so[t + 1] - so[t] > tree_max) s_over = 1;
+  __syncthreads();
+  if (s_over) {
+    if (t <= T) so[t] = 0;
+    if (t == 0) {
+      rec[kRecStatus] = PYL_EP_ST_TREE_OVERFLOW;
+      rec[kRecNsurv] = 0;
+      rec[kRecWords + kRecNpar] = 0;
+      rec[kRecWords + kRecHist] = rec[kRecHist];
+    }
+  }
+  __syncthreads();
+  if (!budget && t <= T) {
+    ko[t] = so[t];
+    tab_next[t] = so[t];  // PO of the next level
+  }
+}
+
+
+// One tree (F null), or a forest of F->ntree trees whose input sizes are n_tree.
+template <bool kComplex, bool kForest>
 int circ_levels(const pyl_prune_params* p, const pyl_circ_params* q,
                 const pyl_prune_coords* coords, const double* thr, const int64_t* seg_idx,
                 int32_t nlev, const float* fold, int64_t seg_stride, const double* in_leaves,
                 const float* in_folds, int64_t n_in, int64_t hist_row0, int32_t first_out,
-                const pyl_ep_loop* L, const pyl_circ_loop* C, hipStream_t s) {
+                const pyl_ep_loop* L, const pyl_circ_loop* C, const pyl_ep_forest* F,
+                const int64_t* n_tree, hipStream_t s) {
   PYL_REQUIRE(L != nullptr && C != nullptr && coords != nullptr && thr != nullptr &&
                   seg_idx != nullptr,
               "null host argument");
@@ -901,9 +1106,41 @@ int circ_levels(const pyl_prune_params* p, const pyl_circ_params* q,
               "the input tree is the first output buffer");
   PYL_REQUIRE(!bud || bud->bt_tmp, "null budget buffer");
   PYL_REQUIRE((int64_t)p->n_accel * p->n_freq <= 0x7fffffff, "FFA grid too large");
+  int T = 1;
+  int64_t row_stride = 0;
+  if constexpr (kForest) {
+    PYL_REQUIRE(F != nullptr && n_tree != nullptr, "null forest");
+    T = F->ntree;
+    PYL_REQUIRE(T >= 1 && T <= PYL_EP_FOREST_MAX, "ntree must be in [1, %d] (got %d)",
+                PYL_EP_FOREST_MAX, T);
+    PYL_REQUIRE(F->tab && F->minmax, "null forest table");
+    PYL_REQUIRE(F->max_sugg >= 1, "invalid per-tree max_sugg");
+    PYL_REQUIRE(!bud || (F->sel && F->part && F->kcount), "null forest budget state");
+    const int64_t row_floats = kComplex ? 4 * (p->nbins / 2 + 1) : 2 * (int64_t)p->nbins;
+    PYL_REQUIRE(F->tree_stride >= 0 && F->tree_stride % row_floats == 0,
+                "tree_stride must be a multiple of a fold row (%lld floats)",
+                (long long)row_floats);
+    row_stride = F->tree_stride / row_floats;
+    PYL_REQUIRE((T - 1) * row_stride + (int64_t)p->n_accel * p->n_freq <= 0x7fffffff,
+                "stacked folds too large for 32-bit row indices");
+    TreeSizes sz{};
+    int64_t tot = 0;
+    for (int t = 0; t < T; ++t) {
+      PYL_REQUIRE(n_tree[t] >= 0, "negative tree size");
+      sz.n[t] = n_tree[t];
+      tot += n_tree[t];
+    }
+    PYL_REQUIRE(tot == n_in, "tree sizes do not add up to the input rows");
+    // every tree's batches together: at most cap_par (the per-batch arrays)
+    PYL_REQUIRE(q->batch_size == 1 || L->cap_par >= 2 * (int64_t)T,
+                "cap_par must be at least twice the number of trees");
+    forest_init_kernel<<<1, 256, 0, s>>>(F->tab, F->minmax, T, nlev, sz);
+    PYL_CHECK_LAUNCH("forest_init_kernel");
+  }
   ep_records_init_kernel<<<(unsigned)pyl::ceil_div(nlev + 1, 64), 64, 0, s>>>(
       L->rec, L->minmax, nlev, n_in, hist_row0);
   PYL_CHECK_LAUNCH("ep_records_init_kernel");
+  const int64_t tabw = (int64_t)PYL_EP_FOREST_TAB * (T + 1);
   pyl_circ_level cl{};
   cl.branch = L->branch;
   cl.counts = L->counts;
@@ -941,34 +1178,42 @@ int circ_levels(const pyl_prune_params* p, const pyl_circ_params* q,
     const int o = bud ? 0 : (first_out + l) & 1;
     const float* seg = fold + seg_idx[l] * seg_stride;
     const int64_t bs = cc.batch_size;
+    FTab ft{};
+    int64_t* tab = nullptr;
+    if constexpr (kForest) {
+      tab = F->tab + l * tabw;
+      ft = FTab{tab, T, row_stride, F->minmax + (int64_t)l * T * 2};
+    }
     pyl_ep_timer* tm = L->timer;
     if (tm) tm->begin(PYL_EP_PH_BRANCH, s);
+    if constexpr (kForest) forest_begin_kernel<<<1, 64, 0, s>>>(tab, T, bs);
     circ_branch_kernel<<<g_par, tpar, 0, s>>>(
         k, cc, leaves, cl, rec + kRecNpar, L->nchild,
         reinterpret_cast<unsigned long long*>(rec + kRecErr));
     chunked_scan(CountSrc{L->nchild},
                  HeldOffSink{C->off1, rec, L->cap_child, L->max_sugg, L->cap_hist, bs,
-                             (int64_t)std::max(p->branch_max, 1)},
+                             (int64_t)std::max(p->branch_max, 1),
+                             kForest ? tab + PYL_EP_TAB_BO * (T + 1) + T : nullptr},
                  rec + kRecNpar, L->part, s);
     circ_stage1_kernel<<<g_held, tpar, 0, s>>>(k, cc, leaves, cl, rec + kRecNpar,
                                               rec + PYL_EP_REC_NHELD);
     chunked_scan(HeldSrc{C->s1_expand, C->s1_parent, C->cr_num}, PackedOffSink{C->held_off},
                  rec + PYL_EP_REC_NHELD, L->part, s);
     circ_batch_totals_kernel<<<g_par, tpar, 0, s>>>(C->off1, C->held_off, rec, bs, C->batch_tot,
-                                                   C->batch_nk, C->has_taylor);
+                                                   C->batch_nk, C->has_taylor, ft);
     chunked_scan(CountSrc{C->batch_tot}, BatchOffSink{C->batch_off, rec, L->cap_child},
                  rec + PYL_EP_REC_NBATCH, L->part, s);
     circ_positions_kernel<<<g_held, tpar, 0, s>>>(C->off1, C->held_off, C->s1_parent,
                                                  C->s1_expand, C->batch_off, C->batch_nk,
-                                                 rec + PYL_EP_REC_NEXPAND, bs, C->s1_pos);
+                                                 rec + PYL_EP_REC_NEXPAND, bs, C->s1_pos, ft);
     circ_expand_kernel<<<g_held, tpar, 0, s>>>(k, cc, leaves, cl, rec + PYL_EP_REC_NEXPAND);
     chunked_scan(FlagSrc{C->c_keep}, ValidSink{C->valid, rec}, rec + kRecNchild, L->part, s);
     PYL_CHECK_LAUNCH("circ_expand_kernel");
     if (tm) tm->end(s);
     if (tm) tm->begin(PYL_EP_PH_CHILDREN, s);
-    circ_children_kernel<kComplex><<<g_child, kCBlk, lds, s>>>(
+    circ_children_kernel<kComplex, kForest><<<g_child, kCBlk, lds, s>>>(
         k, cc, leaves, folds, cl, C->valid, 0, rec + kRecNvalid, seg, L->c_parent, L->c_row,
-        L->c_shift, L->c_phase, L->c_score, L->minmax + 2 * l, C->has_taylor);
+        L->c_shift, L->c_phase, L->c_score, L->minmax + 2 * l, C->has_taylor, ft);
     PYL_CHECK_LAUNCH("circ_children_kernel");
     if (tm) tm->end(s);
     if (tm) tm->begin(PYL_EP_PH_SELECT, s);
@@ -976,21 +1221,33 @@ int circ_levels(const pyl_prune_params* p, const pyl_circ_params* q,
                  SurvSink{L->surv, rec, L->max_sugg, L->cap_hist}, rec + kRecNvalid, L->part,
                  s);
     PYL_CHECK_LAUNCH("chunk_scan_kernel");
+    if constexpr (kForest) {
+      forest_split_kernel<<<1, 128, 0, s>>>(rec, tab, tab + tabw, T, C->batch_off, C->valid,
+                                            L->surv, F->max_sugg, bud ? 1 : 0);
+      PYL_CHECK_LAUNCH("forest_split_kernel");
+    }
     if (tm) tm->end(s);
     if (tm) tm->begin(PYL_EP_PH_MATERIALIZE, s);
-    circ_materialize_kernel<kComplex><<<g_surv, kCBlk, materialize_lds(kComplex, k.nbins), s>>>(
-        k, cc, leaves, folds, cl, C->valid, seg, L->c_parent, L->c_row, L->c_shift,
-        L->c_phase, L->c_score, C->has_taylor, L->surv, 0, rec + kRecWords + kRecNpar,
-        L->leaves[o], L->folds[o], L->scores[o], bud ? bud->bt_tmp : L->hist,
-        bud ? nullptr : rec + kRecHist);
+    circ_materialize_kernel<kComplex, kForest>
+        <<<g_surv, kCBlk, materialize_lds(kComplex, k.nbins), s>>>(
+            k, cc, leaves, folds, cl, C->valid, seg, L->c_parent, L->c_row, L->c_shift,
+            L->c_phase, L->c_score, C->has_taylor, L->surv, 0, rec + kRecWords + kRecNpar,
+            L->leaves[o], L->folds[o], L->scores[o], bud ? bud->bt_tmp : L->hist,
+            bud ? nullptr : rec + kRecHist, ft);
     PYL_CHECK_LAUNCH("circ_materialize_kernel");
     if (tm) tm->end(s);
     if (bud) {
       if (tm) tm->begin(PYL_EP_PH_BUDGET, s);
-      rc = pyl::ep_budget_level_enqueue(bud, kP, bud->level0 + l, L->leaves[0],
-                                        L->folds[0], L->scores[0], bud->bt_tmp, L->max_sugg,
-                                        rec, L->leaves[1], L->folds[1], L->scores[1], L->hist,
-                                        s);
+      if constexpr (kForest)
+        rc = pyl::ep_budget_forest_enqueue(bud, F, kP, bud->level0 + l, tab, tab + tabw,
+                                           L->leaves[0], L->folds[0], L->scores[0],
+                                           bud->bt_tmp, L->max_sugg, rec, L->leaves[1],
+                                           L->folds[1], L->scores[1], L->hist, s);
+      else
+        rc = pyl::ep_budget_level_enqueue(bud, kP, bud->level0 + l, L->leaves[0],
+                                          L->folds[0], L->scores[0], bud->bt_tmp, L->max_sugg,
+                                          rec, L->leaves[1], L->folds[1], L->scores[1],
+                                          L->hist, s);
       if (rc != PYL_OK) return rc;
       if (tm) tm->end(s);
     }
@@ -1009,7 +1266,28 @@ extern "C" int pyl_circ_levels(const pyl_prune_params* p, const pyl_circ_params*
                                const pyl_circ_loop* C, void* stream) {
   PYL_REQUIRE(p != nullptr && q != nullptr, "null parameters");
   PYL_REQUIRE((flags & ~PYL_EP_COMPLEX) == 0, "unknown flags %d", flags);
-  auto* f = (flags & PYL_EP_COMPLEX) ? circ_levels<true> : circ_levels<false>;
+  auto* f = (flags & PYL_EP_COMPLEX) ? circ_levels<true, false> : circ_levels<false, false>;
   return f(p, q, coords, thresholds, seg_idx, nlev, dyp_fold, seg_stride, in_leaves, in_folds,
-           n_in, hist_row0, first_out, L, C, pyl::as_stream(stream));
+           n_in, hist_row0, first_out, L, C, nullptr, nullptr, pyl::as_stream(stream));
+}
+
+extern "C" int pyl_circ_forest_levels(const pyl_prune_params* p, const pyl_circ_params* q,
+                                      int32_t flags, const pyl_prune_coords* coords,
+                                      const double* thresholds, const int64_t* seg_idx,
+                                      int32_t nlev, const float* dyp_fold, int64_t seg_stride,
+                                      const double* in_leaves, const float* in_folds,
+                                      const int64_t* n_in, int64_t hist_row0,
+                                      int32_t first_out, const pyl_ep_loop* L,
+                                      const pyl_circ_loop* C, const pyl_ep_forest* F,
+                                      void* stream) {
+  PYL_REQUIRE(p != nullptr && q != nullptr && F != nullptr && n_in != nullptr,
+              "null parameters");
+  PYL_REQUIRE((flags & ~PYL_EP_COMPLEX) == 0, "unknown flags %d", flags);
+  PYL_REQUIRE(F->ntree >= 1 && F->ntree <= PYL_EP_FOREST_MAX, "ntree must be in [1, %d]",
+              PYL_EP_FOREST_MAX);
+  int64_t tot = 0;
+  for (int t = 0; t < F->ntree; ++t) tot += n_in[t];
+  auto* f = (flags & PYL_EP_COMPLEX) ? circ_levels<true, true> : circ_levels<false, true>;
+  return f(p, q, coords, thresholds, seg_idx, nlev, dyp_fold, seg_stride, in_leaves, in_folds,
+           tot, hist_row0, first_out, L, C, F, n_in, pyl::as_stream(stream));
 }

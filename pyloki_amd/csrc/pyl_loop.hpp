@@ -151,6 +151,66 @@ void chunked_scan(Src src, Sink sink, const int64_t* n_dev, int64_t* part, hipSt
   chunk_scan_kernel<Src, Sink><<<kScanBlocks, kScanThreads, 0, s>>>(src, sink, n_dev, part);
 }
 
+// Chunked scans over the trees of a forest at once: tree y = blockIdx.y holds the items
+// [off[y], off[y + 1]) and its chunk sums in part + y * kScanBlocks; src(i, y), then
+// sink(i, exclusive prefix within the tree, value, y), and sink.finish(y, total) once per
+// tree. kSegScanBlocks chunks per tree.
+constexpr int kSegScanBlocks = 64;
+
+__device__ __forceinline__ void seg_chunk_bounds(const int64_t* off, int64_t* b0, int64_t* b1) {
+  const int64_t lo = off[blockIdx.y], n = off[blockIdx.y + 1] - lo;
+  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  *b0 = lo + std::min<int64_t>(n, (int64_t)blockIdx.x * chunk);
+  *b1 = lo + std::min<int64_t>(n, (int64_t)(blockIdx.x + 1) * chunk);
+}
+
+template <class Src>
+__global__ void __launch_bounds__(kScanThreads)
+chunk_sum_seg_kernel(Src src, const int64_t* __restrict__ off, int64_t* __restrict__ part) {
+  int64_t b0, b1;
+  seg_chunk_bounds(off, &b0, &b1);
+  int64_t acc = 0;
+  for (int64_t i = b0 + threadIdx.x; i < b1; i += kScanThreads) acc += src(i, blockIdx.y);
+  acc = block_sum_i64(acc);
+  if (threadIdx.x == 0) part[(int64_t)blockIdx.y * kScanBlocks + blockIdx.x] = acc;
+}
+
+template <class Src, class Sink>
+__global__ void __launch_bounds__(kScanThreads)
+chunk_scan_seg_kernel(Src src, Sink sink, const int64_t* __restrict__ off,
+                      const int64_t* __restrict__ part) {
+  const int y = blockIdx.y;
+  int64_t b0, b1;
+  seg_chunk_bounds(off, &b0, &b1);
+  const int64_t* pt = part + (int64_t)y * kScanBlocks;
+  int64_t pre = 0, tot = 0;
+  for (int i = threadIdx.x; i < (int)gridDim.x; i += kScanThreads) {
+    const int64_t v = pt[i];
+    pre += i < (int)blockIdx.x ? v : 0;
+    tot += v;
+  }
+  pre = block_sum_i64(pre);
+  tot = block_sum_i64(tot);
+  for (int64_t base = b0; base < b1; base += kScanThreads) {
+    const int64_t i = base + threadIdx.x;
+    const int64_t x = i < b1 ? src(i, y) : 0;
+    int64_t t;
+    const int64_t inc = block_scan_i64(x, &t);
+    if (i < b1) sink(i, pre + inc - x, x, y);
+    pre += t;
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) sink.finish(y, tot);
+}
+
+template <class Src, class Sink>
+void chunked_scan_seg(Src src, Sink sink, const int64_t* off, int ntree, int64_t* part,
+                      hipStream_t s) {
+  static_assert(kSegScanBlocks <= kScanBlocks, "chunk sums per tree");
+  const dim3 g(kSegScanBlocks, (unsigned)ntree);
+  chunk_sum_seg_kernel<Src><<<g, kScanThreads, 0, s>>>(src, off, part);
+  chunk_scan_seg_kernel<Src, Sink><<<g, kScanThreads, 0, s>>>(src, sink, off, part);
+}
+
 // Children of a level: off[i + 1] = inclusive count; the total becomes the level's child
 // count unless the branch step failed, the children exceed the buffers or the history
 // could not take a full survivor buffer (checked here, before any child is scored).

@@ -98,7 +98,7 @@ EP_COMPLEX, EP_CHEBYSHEV = 1, 2
 EP_SCAN_BLOCKS = 256
 REC_NPAR, REC_NCHILD, REC_NSURV, REC_STATUS, REC_ERR, REC_HIST, REC_NVALID = range(7)
 REC_WORDS = 12
-ST_OK, ST_BRANCH, ST_CHILD_CAP, ST_OVERFLOW, ST_HIST_FULL, ST_BUDGET = range(6)
+ST_OK, ST_BRANCH, ST_CHILD_CAP, ST_OVERFLOW, ST_HIST_FULL, ST_BUDGET, ST_TREE_OVERFLOW = range(7)
 
 
 class EpLoop(ctypes.Structure):
@@ -122,6 +122,17 @@ class EpBudget(ctypes.Structure):
         (name, ctypes.c_void_p) for name in (
             "params", "keys", "feats", "kept", "bt_tmp", "sel", "stats", "shist", "part",
             "level_mask")]
+
+
+EP_FOREST_MAX, EP_FOREST_TAB = 64, 6
+TAB_PO, TAB_BO, TAB_CB, TAB_VB, TAB_SO, TAB_KO = range(6)
+
+
+class EpForest(ctypes.Structure):
+    _fields_ = [("ntree", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("tree_stride", ctypes.c_int64), ("max_sugg", ctypes.c_int64)] + [
+        (name, ctypes.c_void_p) for name in ("tab", "minmax", "sel", "stats", "shist", "part",
+                                             "kcount")]
 
 
 class CircLoop(ctypes.Structure):

@@ -37,7 +37,7 @@ def test_bindings_cover_declarations():
 
 def test_abi_version_and_error_path_without_gpu():
     lib = _lib.load()
-    assert lib.pyl_abi_version() == 106
+    assert lib.pyl_abi_version() == 107
     # An invalid argument is rejected on the host before anything touches the GPU.
     rc = lib.pyl_ffa_resolve(0, None, None, None, None, 1, 1.0, 64, None, None, None)
     assert rc == _lib.PYL_ERR_INVALID
@@ -73,7 +73,9 @@ def test_struct_layouts_match_the_header(tmp_path):
 
         pytest.skip("gcc not available")
     structs = {"pyl_prune_params": prune.PruneParams, "pyl_prune_coords": prune.PruneCoords,
-               "pyl_circ_params": prune.CircParams, "pyl_circ_level": prune.CircLevel}
+               "pyl_circ_params": prune.CircParams, "pyl_circ_level": prune.CircLevel,
+               "pyl_ep_loop": prune.EpLoop, "pyl_ep_budget": prune.EpBudget,
+               "pyl_circ_loop": prune.CircLoop, "pyl_ep_forest": prune.EpForest}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "pyloki_hip.h"',
              "int main(void) {"]
     for cname, py in structs.items():

@@ -1,0 +1,123 @@
+"""Many circular-orbit EP searches through one level loop (``forest.ForestSearch``)
+against eager ``Pruning.execute`` of each series alone: reports, scores, EP scores, level
+statistics and every level's backtracks identical, with and without a survivor budget
+(S/N, network, hybrid), on real and complex folds; also when levels stop on the device
+(buffers too small: grown and resumed; a tree over its survivor limit: the trees go on
+alone) and for a forest that runs fewer series than it holds."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from pyloki_amd.forest import ForestSearch
+from pyloki_amd.prune import (ST_CHILD_CAP, ST_HIST_FULL, ST_OVERFLOW, Pruning)
+from tests.test_gpu_circ import _setup as _circ_setup
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(msg: str) -> None:
+    print(f"[case] {msg}", flush=True)
+
+
+def _series(seeds, use_fourier: bool):
+    from pyloki_amd.ffa import DynamicProgramming
+    from pyloki_amd.simulation.pulse import PulseSignalConfig
+
+    cfg, dyp0, _ = _circ_setup(use_fourier=use_fourier)
+    out = []
+    for s in seeds:
+        sim = PulseSignalConfig(period=1 / 200.17, dt=cfg.tsamp, nsamps=cfg.nsamps, snr=30.0,
+                                ducy=0.1,
+                                mod_kwargs={"p_orb": 30.0, "x_orb": 0.002, "phi": 0.1 * s})
+        dyp = DynamicProgramming(sim.generate(seed=s), cfg)
+        dyp.initialize()
+        dyp.execute()
+        out.append(dyp)
+    return cfg, out
+
+
+def _eager(dyp, thresholds, ref, hook_factory, **kw):
+    pr = Pruning(dyp, thresholds, **kw)
+    pr.level_hook = hook_factory() if hook_factory else None
+    res = pr.execute(ref)
+    return res, [b.cpu().numpy() for b in pr.backtrack_hist]
+
+
+def _check(fs: ForestSearch, got: list, dyps, thresholds, ref, hook_factory, **kw) -> None:
+    for t, d in enumerate(dyps):
+        exp, bt = _eager(d, thresholds, ref, hook_factory, **kw)
+        what = f"tree {t}"
+        np.testing.assert_array_equal(got[t]["param_sets"], exp["param_sets"], err_msg=what)
+        np.testing.assert_array_equal(got[t]["scores"], exp["scores"], err_msg=what)
+        np.testing.assert_array_equal(got[t]["scores_ep"], exp["scores_ep"], err_msg=what)
+        assert got[t]["level_stats"] == exp["level_stats"], what
+        fb = [b.cpu().numpy() for b in fs.prs[t].backtrack_hist]
+        assert len(fb) == len(bt), what
+        for lv, (a, b) in enumerate(zip(fb, bt)):
+            np.testing.assert_array_equal(a, b, err_msg=f"{what} level {lv + 1} backtracks")
+
+
+def _budget(kind: str, nbins: int):
+    from pyloki_amd.cnn import BranchRanker, SurvivorBudget
+
+    if kind == "none":
+        return None
+    if kind == "snr":
+        return lambda: SurvivorBudget(256)
+    ranker = BranchRanker(nbins, seed=2)
+    return lambda: SurvivorBudget(256, ranker, hybrid=kind == "hybrid")
+
+
+@pytest.mark.parametrize("use_fourier", [False, True])
+@pytest.mark.parametrize("kind", ["none", "snr", "cnn", "hybrid"])
+def test_forest_equals_eager(gpu_device, use_fourier, kind):
+    _case(f"forest of 4 circular trees, complex={use_fourier}, budget={kind}")
+    cfg, dyps = _series((3, 4, 5, 6), use_fourier)
+    thresholds = np.linspace(2.0, 8.0, dyps[0].nsegments - 1)
+    ref = dyps[0].nsegments // 2
+    factory = _budget(kind, cfg.nbins)
+    kw = {"max_sugg": 2**20, "batch_size": 256}
+    fs = ForestSearch(dyps[0], thresholds, ref, 4, hook_factory=factory, **kw)
+    got = fs.run(dyps)
+    print(f"[result] resumes {fs.stats}", flush=True)
+    _check(fs, got, dyps, thresholds, ref, factory, **kw)
+    assert fs.stats["fallbacks"] == 0
+    # a second run, fewer series than the forest holds, in another order
+    sub = [dyps[2], dyps[0], dyps[3]]
+    got = fs.run(sub)
+    _check(fs, got, sub, thresholds, ref, factory, **kw)
+
+
+@pytest.mark.parametrize("kind", ["none", "snr"])
+def test_forest_resumes_with_grown_buffers(gpu_device, kind):
+    _case(f"forest with small buffers, budget={kind}")
+    cfg, dyps = _series((3, 4, 5), True)
+    thresholds = np.linspace(2.0, 8.0, dyps[0].nsegments - 1)
+    ref = dyps[0].nsegments // 2
+    factory = _budget(kind, cfg.nbins)
+    kw = {"max_sugg": 2**20, "batch_size": 256}
+    fs = ForestSearch(dyps[0], thresholds, ref, 3, hook_factory=factory, **kw)
+    fs.max_tot, fs.cap_child, fs.cap_hist = 512, 2048, 1024
+    fs.cap_par = max(fs.max_tot, 3 * fs.n_seed)
+    fs._alloc()
+    got = fs.run(dyps)
+    print(f"[result] {fs.stats}", flush=True)
+    _check(fs, got, dyps, thresholds, ref, factory, **kw)
+    res = fs.stats["resumes"]
+    assert res.get(ST_CHILD_CAP) and res.get(ST_OVERFLOW) and res.get(ST_HIST_FULL), res
+    assert fs.stats["fallbacks"] == 0
+
+
+def test_forest_tree_over_its_limit_goes_on_alone(gpu_device):
+    _case("forest with max_sugg 64 (the overflow rule)")
+    cfg, dyps = _series((3, 4, 5), False)
+    thresholds = np.linspace(2.0, 8.0, dyps[0].nsegments - 1)
+    ref = dyps[0].nsegments // 2
+    kw = {"max_sugg": 64, "batch_size": 32}
+    fs = ForestSearch(dyps[0], thresholds, ref, 3, **kw)
+    got = fs.run(dyps)
+    print(f"[result] {fs.stats}", flush=True)
+    assert fs.stats["fallbacks"] == 1
+    _check(fs, got, dyps, thresholds, ref, None, **kw)
