@@ -778,7 +778,11 @@ int pyl_circ_levels(const pyl_prune_params* p, const pyl_circ_params* q, int32_t
  * L->cap_* are the forest's capacities (all trees together); n_in: host [ntree], the
  * input trees' sizes (their rows one after the other in in_*). With L->budget, every tree
  * keeps at most budget->budget survivors per level, ranked and cut per tree (the network's
- * level statistics are per tree); budget scratch is sized for L->max_sugg rows. */
+ * level statistics are per tree); budget scratch is sized for L->max_sugg rows. A
+ * PYL_BUDGET_SCORE budget is cut on the survivors' scores (copied to budget->keys) before
+ * anything is materialised: only the kept survivors are, into tree buffer
+ * (first_out + l) % 2 as without a budget (budget->bt_tmp unused); the network's budgets
+ * use the buffers as pyl_ep_loop describes. */
 #define PYL_EP_FOREST_MAX 64
 #define PYL_EP_FOREST_TAB 6
 #define PYL_EP_TAB_PO 0
@@ -809,6 +813,18 @@ int pyl_circ_forest_levels(const pyl_prune_params* p, const pyl_circ_params* q, 
                            const float* in_folds_dev, const int64_t* n_in, int64_t hist_row0,
                            int32_t first_out, const pyl_ep_loop* L, const pyl_circ_loop* C,
                            const pyl_ep_forest* F, void* stream);
+
+/* pyl_circ_ascend(_complex) of a forest's final trees in one launch: tree t's leaves are
+ * rows tree_off[t]..tree_off[t + 1] (host [ntree + 1], tree_off[0] = 0) of leaves_dev and
+ * its FFA fold starts dyp_fold_dev + t * tree_stride floats; out_* as pyl_circ_ascend for
+ * the tree_off[ntree] rows. flags: PYL_EP_COMPLEX. Replaces one Pruning's ascend
+ * (prune.py:671-693) per task. */
+int pyl_circ_forest_ascend(const pyl_prune_params* p, const pyl_circ_params* q, int32_t flags,
+                           const double* leaves_dev, const int64_t* tree_off, int32_t ntree,
+                           const float* dyp_fold_dev, int64_t tree_stride, int64_t ncoords,
+                           const int64_t* seg_idx, const double* seg_t0, int32_t nseg,
+                           double t0_cur, float* out_folds_dev, float* out_scores_dev,
+                           void* stream);
 
 /* ------------------------------------------------------------------------------------
  * Learned branch ranking (no counterpart in the reference). A small 1-D CNN scores EP

@@ -146,6 +146,9 @@ SIGNATURES: list[tuple[str, object, list[object]]] = [
     ("pyl_circ_forest_levels", ctypes.c_int,
      [_vp, _vp, _c_i32, _vp, _f64p, _i64p, _c_i32, _vp, _c_i64, _vp, _vp, _i64p, _c_i64,
       _c_i32, _vp, _vp, _vp, _vp]),
+    ("pyl_circ_forest_ascend", ctypes.c_int,
+     [_vp, _vp, _c_i32, _vp, _i64p, _c_i32, _vp, _c_i64, _c_i64, _i64p, _f64p, _c_i32,
+      _c_dbl, _vp, _vp, _vp]),
     ("pyl_ep_levels", ctypes.c_int,
      [_vp, _c_i32, _vp, _f64p, _i64p, _c_i32, _vp, _c_i64, _vp, _vp, _c_i64, _c_i64, _c_i32,
       _vp, _vp]),

@@ -263,6 +263,12 @@ inline unsigned stride_grid(int64_t cap) {
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>((cap + kT - 1) / kT, 512));
 }
 
+// Blocks per tree of a forest's strided launch: about 512 in all (the launch's count is
+// fixed at enqueue time, and most trees hold far fewer rows than the capacity).
+inline unsigned forest_grid(int64_t cap, int ntree) {
+  return std::min<unsigned>(stride_grid(cap), (unsigned)std::max(1, 512 / ntree));
+}
+
 // off / ntree: a forest's trees (off = their survivor offsets), each with its own
 // statistics; n_cap then bounds one tree's rows.
 int features_enqueue(const double* leaves, const float* scores, int64_t n_cap,
@@ -270,11 +276,11 @@ int features_enqueue(const double* leaves, const float* scores, int64_t n_cap,
                      uint32_t* shist, float* feats, hipStream_t s,
                      const int64_t* off = nullptr, int ntree = 1) {
   const unsigned T = (unsigned)ntree;
+  const unsigned gx = forest_grid(n_cap, ntree);
   feat_stats_part_kernel<<<dim3(kScanBlocks, T), kT, 0, s>>>(scores, n_cap, n_dev, stats,
                                                              shist, off);
-  feat_hist_kernel<<<dim3(stride_grid(n_cap), T), kT, 0, s>>>(scores, n_cap, n_dev, stats,
-                                                              shist, off);
-  feat_write_kernel<<<dim3(stride_grid(n_cap), T), kT, 0, s>>>(
+  feat_hist_kernel<<<dim3(gx, T), kT, 0, s>>>(scores, n_cap, n_dev, stats, shist, off);
+  feat_write_kernel<<<dim3(gx, T), kT, 0, s>>>(
       leaves, scores, n_cap, n_dev, P, level, nlevels, stats, shist, feats, off);
   PYL_CHECK_LAUNCH("feat_write_kernel");
   return PYL_OK;
@@ -566,6 +572,26 @@ fgather_kernel(const int64_t* __restrict__ tab, int T, const int64_t* __restrict
   if (lane < nb) o_hist[(rec[kRecHist] + r) * nb + lane] = bt[i * nb + lane];
 }
 
+// The cut of every tree of a forest on `keys` (survivor order): its kept rows' indices,
+// the trees' kept offsets and the level's totals (the select state initialised first
+// unless `inited`).
+int forest_cut(const pyl_ep_budget* b, const pyl_ep_forest* F, int64_t* tab,
+               int64_t* tab_next, const float* keys, int64_t n_cap, int64_t* rec, bool inited,
+               hipStream_t s) {
+  const int T = F->ntree;
+  const int64_t* so = tab + PYL_EP_TAB_SO * (T + 1);
+  const int64_t tree_cap = std::min<int64_t>(n_cap, F->max_sugg);
+  if (!inited) fsel_init_kernel<<<(unsigned)T, 256, 0, s>>>(rec, so, b->budget, F->sel);
+  const dim3 gsel(forest_grid(tree_cap, T), (unsigned)T);
+  for (int shift = 24; shift >= 0; shift -= 8)
+    sel_hist_kernel<<<gsel, kT, 0, s>>>(keys, nullptr, F->sel, shift, so);
+  chunked_scan_seg(FKeepSrc{keys, F->sel}, FKeepSink{b->kept, F->sel, so, F->kcount}, so, T,
+                   F->part, s);
+  fkeep_offsets_kernel<<<1, 64, 0, s>>>(rec, tab, tab_next, T, F->kcount, 0, b->budget);
+  PYL_CHECK_LAUNCH("fkeep_offsets_kernel");
+  return PYL_OK;
+}
+
 }  // namespace
 
 int pyl::ep_budget_forest_enqueue(const pyl_ep_budget* b, const pyl_ep_forest* F, int P,
@@ -599,9 +625,9 @@ int pyl::ep_budget_forest_enqueue(const pyl_ep_budget* b, const pyl_ep_forest* F
               "null budget buffer");
   const int64_t* r_dev = rec + PYL_EP_REC_NRANK;  // 0 when every tree's survivors fit
   const int64_t tree_cap = std::min<int64_t>(n_cap, F->max_sugg);
-  fsel_init_kernel<<<(unsigned)T, 256, 0, s>>>(rec, so, b->budget, F->sel);
   const float* keys = scores;
   if (net) {
+    fsel_init_kernel<<<(unsigned)T, 256, 0, s>>>(rec, so, b->budget, F->sel);
     int rc = features_enqueue(leaves, scores, tree_cap, r_dev, P, level, b->nlevels, F->stats,
                               F->shist, b->feats, s, so, T);
     if (rc != PYL_OK) return rc;
@@ -610,7 +636,7 @@ int pyl::ep_budget_forest_enqueue(const pyl_ep_budget* b, const pyl_ep_forest* F
     if (rc != PYL_OK) return rc;
     keys = b->keys;
     if (b->kind == PYL_BUDGET_HYBRID) {
-      const dim3 gs(kScanBlocks, (unsigned)T), gr(stride_grid(tree_cap), (unsigned)T);
+      const dim3 gs(kScanBlocks, (unsigned)T), gr(forest_grid(tree_cap, T), (unsigned)T);
       feat_stats_part_kernel<<<gs, kT, 0, s>>>(b->keys, tree_cap, r_dev, F->stats, F->shist,
                                                so);
       feat_hist_kernel<<<gr, kT, 0, s>>>(b->keys, tree_cap, r_dev, F->stats, F->shist, so);
@@ -619,16 +645,30 @@ int pyl::ep_budget_forest_enqueue(const pyl_ep_budget* b, const pyl_ep_forest* F
       PYL_CHECK_LAUNCH("hybrid_key_kernel");
     }
   }
-  const dim3 gsel(stride_grid(tree_cap), (unsigned)T);
-  for (int shift = 24; shift >= 0; shift -= 8)
-    sel_hist_kernel<<<gsel, kT, 0, s>>>(keys, nullptr, F->sel, shift, so);
-  chunked_scan_seg(FKeepSrc{keys, F->sel}, FKeepSink{b->kept, F->sel, so, F->kcount}, so, T,
-                   F->part, s);
-  fkeep_offsets_kernel<<<1, 64, 0, s>>>(rec, tab, tab_next, T, F->kcount, 0, b->budget);
+  int rc = forest_cut(b, F, tab, tab_next, keys, n_cap, rec, net, s);
+  if (rc != PYL_OK) return rc;
   fgather_kernel<<<g_rows, kT, 0, s>>>(tab, T, b->kept, rec, P, row_floats, leaves, folds,
                                        scores, bt, o_leaves, o_folds, o_scores, o_hist);
   PYL_CHECK_LAUNCH("fgather_kernel");
   return PYL_OK;
+}
+
+int pyl::ep_budget_forest_select(const pyl_ep_budget* b, const pyl_ep_forest* F, int level,
+                                 int64_t* tab, int64_t* tab_next, const float* keys,
+                                 int64_t n_cap, int64_t* rec, hipStream_t s) {
+  PYL_REQUIRE(b != nullptr && F != nullptr, "null budget");
+  PYL_REQUIRE(level >= 0 && level <= b->nlevels, "level %d outside the search", level);
+  PYL_REQUIRE(b->kind == PYL_BUDGET_SCORE, "the select-first cut ranks by S/N only");
+  PYL_REQUIRE(b->budget >= 1, "budget must be >= 1");
+  PYL_REQUIRE(n_cap >= 0 && n_cap < 0x7fffffffLL, "too many survivors for the budget");
+  const int T = F->ntree;
+  if (b->level_mask != nullptr && b->level_mask[level] == 0) {
+    fkeep_offsets_kernel<<<1, 64, 0, s>>>(rec, tab, tab_next, T, F->kcount, 1, b->budget);
+    PYL_CHECK_LAUNCH("fkeep_offsets_kernel");
+    return PYL_OK;
+  }
+  PYL_REQUIRE(b->kept && keys && F->sel && F->part && F->kcount, "null budget buffer");
+  return forest_cut(b, F, tab, tab_next, keys, n_cap, rec, false, s);
 }
 
 int pyl::ep_budget_enqueue(const pyl_ep_budget* b, int P, int level, const double* leaves,

@@ -40,4 +40,13 @@ int ep_budget_forest_enqueue(const pyl_ep_budget* b, const pyl_ep_forest* F, int
                              int64_t n_cap, int64_t* rec, double* o_leaves, float* o_folds,
                              float* o_scores, int32_t* o_hist, hipStream_t s);
 
+// The S/N cut of a forest's level before anything is materialised: keys are the
+// survivors' scores in survivor order; b->kept receives every tree's kept survivors
+// (tree t's at b->kept[SO[t]..], survivor indices), tab's KO row, tab_next's PO row and the
+// record's NKEPT, next NPAR and next HIST are written. With b->level_mask flagging the
+// level off: every tree keeps its survivors, or the level stops with PYL_EP_ST_BUDGET.
+int ep_budget_forest_select(const pyl_ep_budget* b, const pyl_ep_forest* F, int level,
+                            int64_t* tab, int64_t* tab_next, const float* keys, int64_t n_cap,
+                            int64_t* rec, hipStream_t s);
+
 }  // namespace pyl

@@ -183,6 +183,22 @@ __device__ __forceinline__ int32_t f_row(const FTab& f, int64_t par, int32_t row
   return row + (int32_t)(f_tree(f.tab, f.T, par) * f.row_stride);
 }
 
+// Rows a forest's materialisation writes when a budget's cut comes first: output row r
+// is survivor kept[SO[t] + r - KO[t]] of tree t (KO[t] <= r < KO[t + 1]); without kept,
+// survivor r.
+struct KMap {
+  const int64_t* kept;
+  const int64_t* tab;
+  int32_t T;
+  __device__ __forceinline__ int64_t operator()(int64_t r) const {
+    if (!kept) return r;
+    const int64_t* so = tab + PYL_EP_TAB_SO * (T + 1);
+    const int64_t* ko = tab + PYL_EP_TAB_KO * (T + 1);
+    const int t = f_tree(ko, T, r);
+    return kept[so[t] + r - ko[t]];
+  }
+};
+
 // Per-tree score minimum / maximum of a block's children (one atomic pair per wavefront
 // when its children are of one tree, else per child).
 __device__ __forceinline__ void forest_minmax(float sc, bool valid, int t, float* mm) {
@@ -456,7 +472,7 @@ circ_materialize_kernel(LevelConst k, CircConst q, const double* __restrict__ le
                         double* __restrict__ o_leaves,
                         float* __restrict__ o_folds, float* __restrict__ o_scores,
                         int32_t* __restrict__ o_bt_base, const int64_t* __restrict__ bt_row0,
-                        FTab f = FTab{}) {
+                        FTab f = FTab{}, KMap km = KMap{}) {
   __shared__ int64_t s_v[kBlk];
   __shared__ int32_t s_fpar[kBlk], s_frow[kBlk], s_fsh[kBlk];  // real folds
   extern __shared__ float2 s_ph[];  // complex: [kBlk][B/2 + 1] phasors, then rows, parents
@@ -470,7 +486,7 @@ circ_materialize_kernel(LevelConst k, CircConst q, const double* __restrict__ le
   const int B = k.nbins;
   if (tid < nvalid) {
     const int64_t s = s0 + tid;
-    const int64_t v = surv[s];
+    const int64_t v = surv[km(s)];
     s_v[tid] = v;
     const int64_t c = valid[v];
     const int64_t s1 = L.c_s1[c];
@@ -565,7 +581,10 @@ circ_ascend_kernel(LevelConst k, CircConst q, const double* __restrict__ leaves,
                    const int64_t* __restrict__ segi, const double* __restrict__ segt,
                    int32_t nseg, double t0_cur, int32_t* __restrict__ scratch_row,
                    double* __restrict__ scratch_phase, float* __restrict__ o_folds,
-                   float* __restrict__ o_scores) {
+                   float* __restrict__ o_scores, const int64_t* __restrict__ toff = nullptr,
+                   int T = 1, int64_t tree_stride = 0) {
+  // A forest (toff not null): row r belongs to tree t with toff[t] <= r < toff[t + 1],
+  // whose FFA fold starts tree_stride * t floats into dyp.
   extern __shared__ float S[];
   const int tid = threadIdx.x;
   const int64_t s0 = (int64_t)blockIdx.x * kBlk;
@@ -602,7 +621,8 @@ circ_ascend_kernel(LevelConst k, CircConst q, const double* __restrict__ leaves,
       const int nb_f = B / 2 + 1;
       float2* X = reinterpret_cast<float2*>(S) + (size_t)tid * 2 * nb_f;
       for (int i = 0; i < 2 * nb_f; ++i) X[i] = make_float2(0.0f, 0.0f);
-      const float2* dc = reinterpret_cast<const float2*>(dyp);
+      const float2* dc = reinterpret_cast<const float2*>(
+          dyp + (toff ? f_tree(toff, T, s0 + tid) * tree_stride : 0));
       for (int s = 0; s < nseg; ++s) {
         const int64_t rr = (int64_t)segi[s] * ncoords + scratch_row[(s0 + tid) * nseg + s];
         cascend_add(dc + rr * 2 * nb_f, scratch_phase[(s0 + tid) * nseg + s], B, X);
@@ -621,11 +641,12 @@ circ_ascend_kernel(LevelConst k, CircConst q, const double* __restrict__ leaves,
     for (int r = tid >> 6; r < nvalid; r += kCBlk / 64) {
       const int64_t sv = s0 + r;
       float* of = o_folds + sv * 2 * B;
+      const float* dr = dyp + (toff ? f_tree(toff, T, sv) * tree_stride : 0);
       for (int b = tid & 63; b < B; b += 64) {
         float ae = 0.0f, av = 0.0f;
         for (int s = 0; s < nseg; ++s) {
           const float* rowp =
-              dyp + ((int64_t)segi[s] * ncoords + scratch_row[sv * nseg + s]) * 2 * B;
+              dr + ((int64_t)segi[s] * ncoords + scratch_row[sv * nseg + s]) * 2 * B;
           int src = b - pyl::shift_int(scratch_phase[sv * nseg + s], B);
           if (src < 0) src += B;
           ae = ae + rowp[src];
@@ -808,11 +829,14 @@ extern "C" int pyl_circ_materialize_complex(
                                 o_folds, o_scores, o_bt, stream);
 }
 
+// tree_off (host [T + 1]) and tree_stride: a forest's trees (null: one tree).
 template <bool kComplex>
 static int circ_ascend(const pyl_prune_params* p, const pyl_circ_params* q,
                        const double* leaves, int64_t n, const float* dyp, int64_t ncoords,
                        const int64_t* seg_idx, const double* seg_t0, int32_t nseg,
-                       double t0_cur, float* o_folds, float* o_scores, void* stream) {
+                       double t0_cur, float* o_folds, float* o_scores, void* stream,
+                       const int64_t* tree_off = nullptr, int T = 1,
+                       int64_t tree_stride = 0) {
   LevelConst k;
   CircConst cc;
   int rc = build_circ(p, q, nullptr, &k, &cc);
@@ -836,10 +860,17 @@ static int circ_ascend(const pyl_prune_params* p, const pyl_circ_params* q,
   PYL_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&d_ph), sizeof(double) * n * nseg, s));
   PYL_CHECK_HIP(hipMemcpyAsync(d_idx, seg_idx, sizeof(int64_t) * nseg, hipMemcpyHostToDevice, s));
   PYL_CHECK_HIP(hipMemcpyAsync(d_t0, seg_t0, sizeof(double) * nseg, hipMemcpyHostToDevice, s));
+  int64_t* d_off = nullptr;
+  if (tree_off) {
+    PYL_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&d_off), sizeof(int64_t) * (T + 1), s));
+    PYL_CHECK_HIP(
+        hipMemcpyAsync(d_off, tree_off, sizeof(int64_t) * (T + 1), hipMemcpyHostToDevice, s));
+  }
   circ_ascend_kernel<kComplex><<<(unsigned)pyl::ceil_div(n, kBlk), kCBlk, lds, s>>>(
       k, cc, leaves, n, dyp, ncoords, d_idx, d_t0, nseg, t0_cur, d_row, d_ph, o_folds,
-      o_scores);
+      o_scores, d_off, T, tree_stride);
   PYL_CHECK_LAUNCH("circ_ascend_kernel");
+  if (d_off) PYL_CHECK_HIP(hipFreeAsync(d_off, s));
   PYL_CHECK_HIP(hipFreeAsync(d_idx, s));
   PYL_CHECK_HIP(hipFreeAsync(d_t0, s));
   PYL_CHECK_HIP(hipFreeAsync(d_row, s));
@@ -863,6 +894,27 @@ extern "C" int pyl_circ_ascend_complex(const pyl_prune_params* p, const pyl_circ
                                        float* o_folds, float* o_scores, void* stream) {
   return circ_ascend<true>(p, q, leaves, n, dyp, ncoords, seg_idx, seg_t0, nseg, t0_cur,
                            o_folds, o_scores, stream);
+}
+
+extern "C" int pyl_circ_forest_ascend(const pyl_prune_params* p, const pyl_circ_params* q,
+                                      int32_t flags, const double* leaves,
+                                      const int64_t* tree_off, int32_t ntree,
+                                      const float* dyp, int64_t tree_stride, int64_t ncoords,
+                                      const int64_t* seg_idx, const double* seg_t0,
+                                      int32_t nseg, double t0_cur, float* o_folds,
+                                      float* o_scores, void* stream) {
+  PYL_REQUIRE(p != nullptr && q != nullptr && tree_off != nullptr, "null parameters");
+  PYL_REQUIRE((flags & ~PYL_EP_COMPLEX) == 0, "unknown flags %d", flags);
+  PYL_REQUIRE(ntree >= 1 && ntree <= PYL_EP_FOREST_MAX, "ntree must be in [1, %d]",
+              PYL_EP_FOREST_MAX);
+  PYL_REQUIRE(tree_off[0] == 0, "tree offsets must start at 0");
+  for (int t = 0; t < ntree; ++t)
+    PYL_REQUIRE(tree_off[t + 1] >= tree_off[t], "tree offsets must not decrease");
+  PYL_REQUIRE(tree_stride >= 0, "negative tree_stride");
+  const int64_t n = tree_off[ntree];
+  auto* f = (flags & PYL_EP_COMPLEX) ? circ_ascend<true> : circ_ascend<false>;
+  return f(p, q, leaves, n, dyp, ncoords, seg_idx, seg_t0, nseg, t0_cur, o_folds, o_scores,
+           stream, tree_off, ntree, tree_stride);
 }
 
 // ---- The sync-free level loop (circular basis) ---------------------------------------
@@ -1101,10 +1153,15 @@ int circ_levels(const pyl_prune_params* p, const pyl_circ_params* q,
                   C->has_taylor,
               "null device pointer");
   const pyl_ep_budget* bud = L->budget;
-  const int out0 = bud ? 0 : first_out;  // with a budget: materialise into 0, keep into 1
+  // A forest's S/N budget cuts before anything is materialised: only the kept survivors
+  // are materialised, into the tree buffers in turn as without a budget.
+  const bool cut_first = kForest && bud && bud->kind == PYL_BUDGET_SCORE;
+  const bool pingpong = !bud || cut_first;
+  const int out0 = pingpong ? first_out : 0;  // a budget: materialise into 0, keep into 1
   PYL_REQUIRE(in_leaves != L->leaves[out0] && in_folds != L->folds[out0],
               "the input tree is the first output buffer");
   PYL_REQUIRE(!bud || bud->bt_tmp, "null budget buffer");
+  PYL_REQUIRE(!cut_first || (bud->keys && bud->kept), "null budget buffer");
   PYL_REQUIRE((int64_t)p->n_accel * p->n_freq <= 0x7fffffff, "FFA grid too large");
   int T = 1;
   int64_t row_stride = 0;
@@ -1172,10 +1229,10 @@ int circ_levels(const pyl_prune_params* p, const pyl_circ_params* q,
                   "complex folds need a power-of-two nbins (%d)", k.nbins);
     PYL_REQUIRE(seg_idx[l] >= 0, "negative segment index");
     int64_t* rec = L->rec + (int64_t)l * kRecWords;
-    const int in_buf = bud ? 1 : (first_out + l - 1) & 1;
+    const int in_buf = pingpong ? (first_out + l - 1) & 1 : 1;
     const double* leaves = l == 0 ? in_leaves : L->leaves[in_buf];
     const float* folds = l == 0 ? in_folds : L->folds[in_buf];
-    const int o = bud ? 0 : (first_out + l) & 1;
+    const int o = pingpong ? (first_out + l) & 1 : 0;
     const float* seg = fold + seg_idx[l] * seg_stride;
     const int64_t bs = cc.batch_size;
     FTab ft{};
@@ -1218,13 +1275,22 @@ int circ_levels(const pyl_prune_params* p, const pyl_circ_params* q,
     if (tm) tm->end(s);
     if (tm) tm->begin(PYL_EP_PH_SELECT, s);
     chunked_scan(PassSrc{L->c_score, thr[l]},
-                 SurvSink{L->surv, rec, L->max_sugg, L->cap_hist}, rec + kRecNvalid, L->part,
-                 s);
+                 SurvSink{L->surv, rec, L->max_sugg, L->cap_hist, L->c_score,
+                          cut_first ? bud->keys : nullptr},
+                 rec + kRecNvalid, L->part, s);
     PYL_CHECK_LAUNCH("chunk_scan_kernel");
+    KMap km{};
     if constexpr (kForest) {
       forest_split_kernel<<<1, 128, 0, s>>>(rec, tab, tab + tabw, T, C->batch_off, C->valid,
                                             L->surv, F->max_sugg, bud ? 1 : 0);
       PYL_CHECK_LAUNCH("forest_split_kernel");
+      if (cut_first) {
+        rc = pyl::ep_budget_forest_select(bud, F, bud->level0 + l, tab, tab + tabw, bud->keys,
+                                          L->max_sugg, rec, s);
+        if (rc != PYL_OK) return rc;
+        const bool cut = bud->level_mask == nullptr || bud->level_mask[bud->level0 + l] != 0;
+        km = KMap{cut ? bud->kept : nullptr, tab, T};
+      }
     }
     if (tm) tm->end(s);
     if (tm) tm->begin(PYL_EP_PH_MATERIALIZE, s);
@@ -1232,11 +1298,11 @@ int circ_levels(const pyl_prune_params* p, const pyl_circ_params* q,
         <<<g_surv, kCBlk, materialize_lds(kComplex, k.nbins), s>>>(
             k, cc, leaves, folds, cl, C->valid, seg, L->c_parent, L->c_row, L->c_shift,
             L->c_phase, L->c_score, C->has_taylor, L->surv, 0, rec + kRecWords + kRecNpar,
-            L->leaves[o], L->folds[o], L->scores[o], bud ? bud->bt_tmp : L->hist,
-            bud ? nullptr : rec + kRecHist, ft);
+            L->leaves[o], L->folds[o], L->scores[o], pingpong ? L->hist : bud->bt_tmp,
+            pingpong ? rec + kRecHist : nullptr, ft, km);
     PYL_CHECK_LAUNCH("circ_materialize_kernel");
     if (tm) tm->end(s);
-    if (bud) {
+    if (bud && !cut_first) {
       if (tm) tm->begin(PYL_EP_PH_BUDGET, s);
       if constexpr (kForest)
         rc = pyl::ep_budget_forest_enqueue(bud, F, kP, bud->level0 + l, tab, tab + tabw,

@@ -241,8 +241,13 @@ struct SurvSink {
   int64_t* rec;  // this level's record; rec + kRecWords is the next level's
   int64_t max_sugg;
   int64_t cap_hist;
+  const float* score = nullptr;  // with keys: keys[j] = score of survivor j
+  float* keys = nullptr;
   __device__ void operator()(int64_t i, int64_t excl, int64_t x) const {
-    if (x && excl < max_sugg) surv[excl] = i;
+    if (x && excl < max_sugg) {
+      surv[excl] = i;
+      if (keys) keys[excl] = score[i];
+    }
   }
   __device__ void finish(int64_t npass) const {
     int64_t st = rec[kRecStatus];

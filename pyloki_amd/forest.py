@@ -27,12 +27,13 @@ import torch
 
 from pyloki_amd import _lib
 from pyloki_amd.device import ptr, stream_ptr
-from pyloki_amd.prune import (EP_COMPLEX, EP_FOREST_MAX, EP_FOREST_TAB, EP_SCAN_BLOCKS,
+from pyloki_amd.prune import (BUDGET_SCORE, EP_COMPLEX, EP_FOREST_MAX, EP_FOREST_TAB, EP_SCAN_BLOCKS,
                               REC_HIST, REC_NCHILD, REC_STATUS, REC_WORDS, ST_BUDGET,
                               ST_CHILD_CAP, ST_HIST_FULL, ST_OK, ST_OVERFLOW, TAB_CB, TAB_KO,
                               TAB_PO, TAB_SO, TAB_VB, BUDGET_HIST_BINS, BUDGET_SEL_WORDS,
                               BUDGET_STATS, DeviceTree, EpForest, LoopBuffers, PruneCoords,
-                              Pruning, PruneStats, PruneStatsCollection)
+                              Pruning, PruneStats, PruneStatsCollection,
+                              poly_taylor_report_batch)
 from pyloki_amd.utils.snail import MiddleOutScheme
 
 
@@ -149,14 +150,12 @@ class ForestSearch:
         n = self.n_seed
         state = {"l0": 1, "leaves": self.seed_leaves[:T * n], "folds": self.seed_folds[:T * n],
                  "sizes": [n] * T, "buf": -1}
+        self._last = None
+        self._fell_back = False
         while state is not None:
             run = self._enqueue(prs, state)
             state = self._finish(prs, run)
-        out = []
-        for pr in prs:
-            pr._do_ascend()
-            out.append(pr.report(self.ref_seg))
-        return out
+        return self._ascend_report(prs)
 
     def _seed(self, prs, dyps) -> None:
         """Every tree's seed: the reference segment's rows of its fold, scored in one
@@ -217,8 +216,8 @@ class ForestSearch:
         first_out = 1 - state["buf"] if state["buf"] >= 0 else 0
         self._f.ntree = T
         self._f.tree_stride = per_tree
-        if budget is not None and state["buf"] == 0:
-            msg = "a budgeted forest never reads tree buffer 0"
+        if not self._pingpong(budget) and state["buf"] == 0:
+            msg = "a forest with a network budget never reads tree buffer 0"
             raise RuntimeError(msg)
         _lib.check(lib.pyl_circ_forest_levels(
             ctypes.byref(pr0._pp), ctypes.byref(pr0._qq), EP_COMPLEX if pr0._complex else 0,
@@ -249,38 +248,13 @@ class ForestSearch:
         o1 = (nlev + 1) * REC_WORDS
         tab = info[o1:o1 + (nlev + 1) * tab_w].reshape(nlev + 1, EP_FOREST_TAB, T + 1)
         mm = info[o1 + (nlev + 1) * tab_w:].view(np.float32).reshape(nlev, T, 2)
-        stop, i_stop = ST_OK, nlev
-        for i, level in enumerate(levels):
-            r = rec[i]
-            stop = int(r[REC_STATUS])
-            if stop != ST_OK:
-                i_stop = i
-                break
-            o = 1 if budget is not None else (first_out + i) & 1
-            h = int(r[REC_HIST])
-            tb = tab[i]
-            po, cb, vb, so = tb[TAB_PO], tb[TAB_CB], tb[TAB_VB], tb[TAB_SO]
-            ko = tb[TAB_KO]
-            for t, pr in enumerate(prs):
-                if pr._prune_complete:
-                    continue
-                m = int(ko[t + 1] - ko[t])
-                nv = int(vb[t + 1] - vb[t])
-                a, b = int(ko[t]), int(ko[t + 1])
-                sc = buf.scores[o][a:b]
-                tree = DeviceTree(buf.leaves[o][a:b], buf.folds[o][a:b], sc,
-                                  buf.hist[h + a:h + b], scores_ep=sc)
-                pr.loop_levels[level] = (ST_OK, int(so[t + 1] - so[t]))
-                st = PruneStats(level=level, seg_idx=int(segs[i]), threshold=float(thr[i]),
-                                score_min=float(mm[i, t, 0]) if nv else 0.0,
-                                score_max=float(mm[i, t, 1]) if nv else 0.0,
-                                n_branches=int(po[t + 1] - po[t]),
-                                n_leaves=int(cb[t + 1] - cb[t]), n_leaves_phy=nv,
-                                n_leaves_surv=m)
-                pr._end_level(st, tree, None, tree.backtracks)
-            buf.hist_next = h + int(ko[T])
-            if all(pr._prune_complete for pr in prs):
-                return None
+        status = rec[:nlev, REC_STATUS]
+        bad = np.flatnonzero(status != ST_OK)
+        i_stop = int(bad[0]) if len(bad) else nlev
+        stop = int(status[i_stop]) if len(bad) else ST_OK
+        self._advance(prs, run, rec, tab, mm, i_stop)
+        if all(pr._prune_complete for pr in prs):
+            return None
         if stop == ST_OK:
             return None
         level = levels[i_stop]
@@ -292,7 +266,7 @@ class ForestSearch:
             st = dict(run["state"])
             leaves, folds, src = st["leaves"], st["folds"], st["buf"]
         else:
-            src = 1 if budget is not None else (first_out + i_stop - 1) & 1
+            src = (first_out + i_stop - 1) & 1 if self._pingpong(budget) else 1
             leaves, folds = buf.leaves[src], buf.folds[src]
         state = {"l0": level, "leaves": leaves, "folds": folds, "sizes": sizes, "buf": src}
         if stop == ST_CHILD_CAP:
@@ -319,8 +293,120 @@ class ForestSearch:
             return state
         # A branch error or a tree over its survivor limit: every tree goes on alone.
         self.stats["fallbacks"] += 1
+        self._fell_back = True
         self._fallback(prs)
         return None
+
+    def _advance(self, prs, run: dict, rec, tab, mm, nfin: int) -> None:
+        """Every tree over the run's first nfin levels, as ``Pruning._end_level`` would:
+        its level statistics and backtrack rows; its world tree (a view of the level's
+        output rows) after its last level."""
+        if nfin == 0:
+            return
+        buf = self._buf
+        T = len(prs)
+        levels, thr, segs = run["levels"], run["thr"], run["segs"]
+        budget, first_out = run["budget"], run["first_out"]
+        pingpong = self._pingpong(budget)
+        d = np.diff(tab[:nfin], axis=2)  # [level][table][tree]: per-tree counts
+        k0 = tab[:nfin, TAB_KO, :T]
+        hist0 = rec[:nfin, REC_HIST]
+        mmn = mm[:nfin].astype(np.float64)
+        for t, pr in enumerate(prs):
+            if pr._prune_complete:
+                continue
+            bts = pr._backtrack_hist
+            stats = pr._pstats
+            last = nfin - 1
+            for i in range(nfin):
+                m = int(d[i, TAB_KO, t])
+                nv = int(d[i, TAB_VB, t])
+                stats.append(PruneStats(
+                    level=levels[i], seg_idx=int(segs[i]), threshold=float(thr[i]),
+                    score_min=float(mmn[i, t, 0]) if nv else 0.0,
+                    score_max=float(mmn[i, t, 1]) if nv else 0.0,
+                    n_branches=int(d[i, TAB_PO, t]), n_leaves=int(d[i, TAB_CB, t]),
+                    n_leaves_phy=nv, n_leaves_surv=m))
+                pr.loop_levels[levels[i]] = (ST_OK, int(d[i, TAB_SO, t]))
+                if m == 0:
+                    last = i
+                    break
+                a = int(hist0[i] + k0[i, t])
+                bts.append(buf.hist.narrow(0, a, m))
+            pr._prune_level = levels[last]
+            o = (first_out + last) & 1 if pingpong else 1
+            a, m = int(k0[last, t]), int(d[last, TAB_KO, t])
+            sc = buf.scores[o].narrow(0, a, m)
+            pr._world_tree = DeviceTree(buf.leaves[o].narrow(0, a, m),
+                                        buf.folds[o].narrow(0, a, m), sc,
+                                        buf.hist.narrow(0, int(hist0[last]) + a, m),
+                                        scores_ep=sc)
+            pr._prune_complete = m == 0
+        buf.hist_next = int(hist0[nfin - 1] + tab[nfin - 1, TAB_KO, T])
+        self._last = ((first_out + nfin - 1) & 1 if pingpong else 1,
+                      tab[nfin - 1, TAB_KO].copy())
+
+    @staticmethod
+    def _pingpong(budget) -> bool:
+        """Whether level l writes tree buffer (first_out + l) % 2 (no budget, or an S/N
+        budget, which the loop cuts before materialising), else buffer 1 (a network's
+        budget: all survivors into buffer 0, the kept rows gathered into buffer 1)."""
+        return budget is None or budget.kind == BUDGET_SCORE
+
+    def _ascend_report(self, prs) -> list[dict]:
+        """The ascend of every tree in one launch, and the reports with one transfer, when
+        the trees left are the forest's last level's rows; else tree by tree."""
+        pr0 = prs[0]
+        T = len(prs)
+        live = [t for t, pr in enumerate(prs) if not pr._prune_complete]
+        last = getattr(self, "_last", None)
+        if self._fell_back or not live or last is None or any(
+                prs[t]._prune_level != self.nlast for t in live):
+            out = []
+            for pr in prs:
+                pr._do_ascend()
+                out.append(pr.report(self.ref_seg))
+            return out
+        o, ko = last
+        ko = np.ascontiguousarray(ko, dtype=np.int64)
+        n = int(ko[T])
+        buf = self._buf
+        leaves = buf.leaves[o][:n]
+        scores_ep = buf.scores[o][:n].clone()
+        folds = pr0._new_folds(n, leaves.device)
+        scores = torch.empty(n, dtype=torch.float32, device=leaves.device)
+        sch = prs[live[0]].scheme
+        seg_idx, seg_coords = sch.get_segment_coords_so_far(self.nlast)
+        t0_cur, _ = sch.get_previous_coord(self.nlast + 1, pr0._use_moving_grid)
+        idx = np.ascontiguousarray(seg_idx, dtype=np.int64)
+        t0 = np.ascontiguousarray(seg_coords[:, 0], dtype=np.float64)
+        per_tree = int(self.fold[0].numel()) * (2 if self.fold.is_complex() else 1)
+        lib = _lib.load()
+        _lib.check(lib.pyl_circ_forest_ascend(
+            ctypes.byref(pr0._pp), ctypes.byref(pr0._qq), EP_COMPLEX if pr0._complex else 0,
+            ptr(leaves), ko.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), T, ptr(self.fold),
+            per_tree, pr0._ncoords, idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+            t0.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(idx), t0_cur, ptr(folds),
+            ptr(scores), stream_ptr()))
+        host = torch.cat([leaves.reshape(-1).view(torch.int32),
+                          torch.stack([scores, scores_ep]).reshape(-1).view(torch.int32)]
+                         ).cpu().numpy()  # one transfer
+        nl = 2 * int(np.prod(leaves.shape[1:]))  # 32-bit words per leaf row
+        lv = host[:n * nl].view(np.float64).reshape(n, *leaves.shape[1:])
+        sc2 = host[n * nl:].view(np.float32).reshape(2, n)
+        rep = poly_taylor_report_batch(lv) if n else lv
+        out = []
+        for t, pr in enumerate(prs):
+            a, b = int(ko[t]), int(ko[t + 1])
+            tree = pr._world_tree
+            if not pr._prune_complete:
+                tree.scores_ep = scores_ep[a:b]
+                tree.folds = folds[a:b]
+                tree.scores = scores[a:b]
+            out.append({"run_name": f"{self.ref_seg:03d}_00", "scheme": pr.scheme.data.copy(),
+                        "param_sets": rep[a:b], "scores": sc2[0, a:b],
+                        "scores_ep": sc2[1, a:b], "level_stats": pr._pstats})
+        return out
 
     def _fallback(self, prs) -> None:
         """Every unfinished tree through its own Pruning from the stopped level, one tree
