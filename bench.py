@@ -203,7 +203,7 @@ EP_CNN = {"nsamps": 2**24, "tsamp": 64e-6, "nseg": 128, "nbins": 64, "freq": (24
           "thresholds": (0.5, 8.0), "max_sugg": 2**21, "budgets": (2**14, 12288),
           "survey_budget": 12288,
           "train_seeds": (1, 2, 3, 4), "test_seed": 9, "epochs": 16,
-          "batch_seeds": tuple(range(9, 17))}
+          "batch_seeds": tuple(range(9, 17)), "survey_seeds": tuple(range(9, 25))}
 
 
 def ep_circular_dyp(seed: int, w: dict = EP_CNN, use_fourier: bool = True):
@@ -245,28 +245,29 @@ def ep_circular_cnn() -> int:
     res = cnn.budget_experiment(ep_circular_dyp, w["train_seeds"], w["test_seed"], thr,
                                 w["max_sugg"], w["budgets"], epochs=w["epochs"])
     ranker = res["ranker"]
-    # The survey: 8 series of the configuration, 4 searches at a time on their own streams
-    # (each a captured graph): exhaustive vs the learned, hybrid and S/N budgets.
+    # The survey: 16 series of the configuration searched as the 16 trees of one forest
+    # (every launch of a level serves all of them): exhaustive vs the learned, hybrid and
+    # S/N budgets.
     b = w["survey_budget"]
-    series = [ep_circular_dyp(s) for s in w["batch_seeds"]]
+    series = [ep_circular_dyp(s) for s in w["survey_seeds"]]
     hooks = {"exhaustive": None,
              f"hybrid_{b}": lambda: cnn.SurvivorBudget(b, ranker, hybrid=True),
              f"cnn_{b}": lambda: cnn.SurvivorBudget(b, ranker),
              f"snr_{b}": lambda: cnn.SurvivorBudget(b)}
-    survey = cnn.batched_throughput(series, thr, w["max_sugg"], hooks, width=4)
+    survey = cnn.forest_throughput(series, thr, w["max_sugg"], hooks)
     r = survey[f"hybrid_{b}"]
     ex = survey["exhaustive"]
     line = {
         "metric": "EP recall of the exhaustive candidates under a learned survivor budget",
         "value": r["recall"], "unit": "fraction", "n_gpus": 1, "steps": len(series),
-        "warmup": 4, "ms_per_step": r["ms_per_search"], "higher_is_better": True,
+        "warmup": 1, "ms_per_step": r["ms_per_search"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32 (network inference bf16)",
         "data": "synthetic",
         "config": {"workload": "ep_circular_cnn", "nsamps": w["nsamps"], "nseg": w["nseg"],
                    "nbins": w["nbins"], "prune_poly_order": 5, "budget": b,
                    "key": "max(rank by logit, rank by S/N)", "max_sugg": w["max_sugg"],
                    "thresholds": list(w["thresholds"]), "train_seeds": list(w["train_seeds"]),
-                   "survey_seeds": list(w["batch_seeds"]), "streams": 4},
+                   "survey_seeds": list(w["survey_seeds"]), "forest_trees": len(series)},
         "trial_ratio": r["trial_ratio"], "speedup": r["speedup"],
         "survey": survey,
         "exhaustive_trials_per_s": ex["trials"] / len(series) / (ex["ms_per_search"] * 1e-3),

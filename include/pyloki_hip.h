@@ -495,7 +495,8 @@ int pyl_cheb_ascend_complex(const pyl_prune_params* p, const double* leaves_dev,
 #define PYL_EP_ST_OVERFLOW 3  /* passing children exceed max_sugg */
 #define PYL_EP_ST_HIST_FULL 4 /* backtrack history full */
 #define PYL_EP_ST_BUDGET 5    /* survivors exceed a budget the level was enqueued without */
-#define PYL_EP_ST_TREE_OVERFLOW 6 /* forest: one tree's passing children exceed its max_sugg */
+#define PYL_EP_ST_TREE_OVERFLOW 6 /* forest: a tree's passing children exceed its max_sugg,
+                                     or its held children times branch_max reach 2^31 */
 
 /* Optional per-phase timing of pyl_ep_levels: HIP events recorded on the stream around
  * each level's phases (PYL_EP_PH_*), read back by pyl_ep_timer_read (which waits for them
@@ -770,9 +771,10 @@ int pyl_circ_levels(const pyl_prune_params* p, const pyl_circ_params* q, int32_t
  * Each tree's results equal those of pyl_circ_levels on that tree alone, bit for bit: its
  * backtracks hold parent indices within its own parents. Besides the single loop's stops
  * (PYL_EP_ST_OVERFLOW: the forest's survivor buffer L->max_sugg), a level stops with
- * PYL_EP_ST_TREE_OVERFLOW when any tree's survivors exceed F->max_sugg (the caller then
- * runs the trees one by one from that level) and with PYL_EP_ST_BUDGET when a level
- * enqueued without the cut finds any tree's survivors above the budget.
+ * PYL_EP_ST_TREE_OVERFLOW when any tree's survivors exceed F->max_sugg or its held children
+ * times branch_max reach 2^31 (the caller then runs the trees one by one from that level)
+ * and with PYL_EP_ST_BUDGET when a level enqueued without the cut finds any tree's
+ * survivors above the budget.
  * The FFA folds of the trees are stacked: tree t's segment rows of level l start at
  * dyp_fold_dev + t * tree_stride + seg_idx[l] * seg_stride floats. L->max_sugg and
  * L->cap_* are the forest's capacities (all trees together); n_in: host [ntree], the

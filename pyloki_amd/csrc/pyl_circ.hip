@@ -952,9 +952,11 @@ struct HeldSrc {
     return expand[s] ? ((int64_t)cr_num[s1_parent[s]] << 32) : 1;
   }
 };
-struct PackedOffSink {
+struct PackedOffSink {  // prefix sums modulo 2^64: only their differences are used
   int64_t* off;
-  __device__ void operator()(int64_t i, int64_t excl, int64_t x) const { off[i + 1] = excl + x; }
+  __device__ void operator()(int64_t i, int64_t excl, int64_t x) const {
+    off[i + 1] = wadd(excl, x);
+  }
   __device__ void finish(int64_t) const { off[0] = 0; }
 };
 
@@ -973,10 +975,10 @@ __global__ void circ_batch_totals_kernel(const int64_t* __restrict__ off1,
     has_taylor[b] = 0;  // set again after the children are scored
     int64_t p0, p1;
     f_batch_parents(f, b, batch_size, n_par, &p0, &p1);
-    const int64_t d = held_off[off1[p1]] - held_off[off1[p0]];
-    const int64_t nk = d & 0xffffffffLL;
+    const uint64_t d = (uint64_t)wsub(held_off[off1[p1]], held_off[off1[p0]]);
+    const int64_t nk = (int64_t)(d & 0xffffffffull);
     batch_nk[b] = nk;
-    batch_tot[b] = nk + (d >> 32);
+    batch_tot[b] = nk + (int64_t)(d >> 32);
   }
 }
 
@@ -1009,8 +1011,9 @@ __global__ void circ_positions_kernel(const int64_t* __restrict__ off1,
     const int64_t b = f_batch(f, s1_parent[s], batch_size);
     int64_t p0, p1;
     f_batch_parents(f, b, batch_size, INT64_MAX, &p0, &p1);
-    const int64_t d = held_off[s] - held_off[off1[p0]];
-    s1_pos[s] = batch_off[b] + (s1_expand[s] ? batch_nk[b] + (d >> 32) : (d & 0xffffffffLL));
+    const uint64_t d = (uint64_t)wsub(held_off[s], held_off[off1[p0]]);
+    s1_pos[s] = batch_off[b] + (s1_expand[s] ? batch_nk[b] + (int64_t)(d >> 32)
+                                              : (int64_t)(d & 0xffffffffull));
   }
 }
 
@@ -1060,6 +1063,27 @@ __global__ void forest_begin_kernel(int64_t* __restrict__ tab, int T, int64_t bs
     acc += (po[t + 1] - po[t] + bs - 1) / bs;
   }
   bo[T] = acc;
+}
+
+// Every tree's held children times branch_max below 2^31 (the bound of the packed
+// per-batch sums, batches never straddling trees), else the level stops with
+// PYL_EP_ST_TREE_OVERFLOW (the forest's trees then go on alone).
+__global__ void forest_held_check_kernel(int64_t* __restrict__ rec,
+                                         const int64_t* __restrict__ tab, int T,
+                                         const int64_t* __restrict__ off1, int64_t branch_max) {
+  __shared__ int s_over;
+  const int t = threadIdx.x;
+  if (t == 0) s_over = 0;
+  __syncthreads();
+  const int64_t* po = tab + PYL_EP_TAB_PO * (T + 1);
+  if (rec[kRecStatus] == 0 && t < T && (off1[po[t + 1]] - off1[po[t]]) * branch_max > 0x7fffffffLL)
+    s_over = 1;
+  __syncthreads();
+  if (t == 0 && s_over) {
+    rec[kRecStatus] = PYL_EP_ST_TREE_OVERFLOW;
+    rec[PYL_EP_REC_NHELD] = 0;
+    rec[PYL_EP_REC_NBATCH] = 0;
+  }
 }
 
 // First index i in [0, n) with a[i] >= x (a increasing), else n.
@@ -1247,11 +1271,16 @@ int circ_levels(const pyl_prune_params* p, const pyl_circ_params* q,
     circ_branch_kernel<<<g_par, tpar, 0, s>>>(
         k, cc, leaves, cl, rec + kRecNpar, L->nchild,
         reinterpret_cast<unsigned long long*>(rec + kRecErr));
+    const int64_t bmax = std::max(p->branch_max, 1);
     chunked_scan(CountSrc{L->nchild},
                  HeldOffSink{C->off1, rec, L->cap_child, L->max_sugg, L->cap_hist, bs,
-                             (int64_t)std::max(p->branch_max, 1),
+                             kForest ? 0 : bmax,  // a forest checks its trees one by one
                              kForest ? tab + PYL_EP_TAB_BO * (T + 1) + T : nullptr},
                  rec + kRecNpar, L->part, s);
+    if constexpr (kForest) {
+      forest_held_check_kernel<<<1, 64, 0, s>>>(rec, tab, T, C->off1, bmax);
+      PYL_CHECK_LAUNCH("forest_held_check_kernel");
+    }
     circ_stage1_kernel<<<g_held, tpar, 0, s>>>(k, cc, leaves, cl, rec + kRecNpar,
                                               rec + PYL_EP_REC_NHELD);
     chunked_scan(HeldSrc{C->s1_expand, C->s1_parent, C->cr_num}, PackedOffSink{C->held_off},

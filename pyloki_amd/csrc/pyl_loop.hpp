@@ -6,7 +6,9 @@
 // [b*ceil(n/G), (b+1)*ceil(n/G)) of the n items; the first launch writes each chunk's sum,
 // the second re-scans each chunk from the sum of the chunks before it. Results are exact
 // integers, so they do not depend on the chunking; the second launch's last block also
-// hands the grand total to a "finish" step that updates the level record.
+// hands the grand total to a "finish" step that updates the level record. Sums are taken
+// modulo 2^64 (unsigned arithmetic): a packed source whose grand total wraps still gives
+// exact differences of prefix sums wherever those differences fit.
 #pragma once
 
 #include <cstdint>
@@ -51,39 +53,49 @@ constexpr int kRecHist = PYL_EP_REC_HIST;
 constexpr int kRecNvalid = PYL_EP_REC_NVALID;
 constexpr int kRecWords = PYL_EP_REC_WORDS;
 
+// Wrapping int64 arithmetic (through uint64).
+__device__ __forceinline__ int64_t wadd(int64_t a, int64_t b) {
+  return (int64_t)((uint64_t)a + (uint64_t)b);
+}
+__device__ __forceinline__ int64_t wsub(int64_t a, int64_t b) {
+  return (int64_t)((uint64_t)a - (uint64_t)b);
+}
+
 // Sum over the block (all threads get it).
-__device__ __forceinline__ int64_t block_sum_i64(int64_t x) {
-  __shared__ int64_t s_red[kScanThreads / 64];
+__device__ __forceinline__ int64_t block_sum_i64(int64_t x0) {
+  __shared__ uint64_t s_red[kScanThreads / 64];
+  uint64_t x = (uint64_t)x0;
   for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
   const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) s_red[w] = x;
   __syncthreads();
-  int64_t t = 0;
+  uint64_t t = 0;
   for (int i = 0; i < kScanThreads / 64; ++i) t += s_red[i];
   __syncthreads();
-  return t;
+  return (int64_t)t;
 }
 
 // Inclusive scan over the block; *total receives the block's sum.
-__device__ __forceinline__ int64_t block_scan_i64(int64_t x, int64_t* total) {
-  __shared__ int64_t s_wave[kScanThreads / 64];
+__device__ __forceinline__ int64_t block_scan_i64(int64_t x0, int64_t* total) {
+  __shared__ uint64_t s_wave[kScanThreads / 64];
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
+  uint64_t x = (uint64_t)x0;
   for (int o = 1; o < 64; o <<= 1) {
-    const int64_t y = __shfl_up(x, o);
+    const uint64_t y = __shfl_up(x, o);
     if (lane >= o) x += y;
   }
   if (lane == 63) s_wave[w] = x;
   __syncthreads();
-  int64_t pre = 0, tot = 0;
+  uint64_t pre = 0, tot = 0;
   for (int i = 0; i < kScanThreads / 64; ++i) {
-    const int64_t v = s_wave[i];
+    const uint64_t v = s_wave[i];
     pre += i < w ? v : 0;
     tot += v;
   }
   __syncthreads();
-  *total = tot;
-  return x + pre;
+  *total = (int64_t)tot;
+  return (int64_t)(x + pre);
 }
 
 __device__ __forceinline__ void chunk_bounds(int64_t n, int64_t* b0, int64_t* b1) {
@@ -113,7 +125,7 @@ chunk_sum_kernel(Src src, const int64_t* __restrict__ n_dev, int64_t* __restrict
   int64_t b0, b1;
   chunk_bounds(*n_dev, &b0, &b1);
   int64_t acc = 0;
-  for (int64_t i = b0 + threadIdx.x; i < b1; i += kScanThreads) acc += src(i);
+  for (int64_t i = b0 + threadIdx.x; i < b1; i += kScanThreads) acc = wadd(acc, src(i));
   acc = block_sum_i64(acc);
   if (threadIdx.x == 0) part[blockIdx.x] = acc;
 }
@@ -129,8 +141,8 @@ chunk_scan_kernel(Src src, Sink sink, const int64_t* __restrict__ n_dev,
   int64_t pre = 0, tot = 0;
   for (int i = threadIdx.x; i < (int)gridDim.x; i += kScanThreads) {
     const int64_t v = part[i];
-    pre += i < (int)blockIdx.x ? v : 0;
-    tot += v;
+    pre = wadd(pre, i < (int)blockIdx.x ? v : 0);
+    tot = wadd(tot, v);
   }
   pre = block_sum_i64(pre);
   tot = block_sum_i64(tot);
@@ -139,8 +151,8 @@ chunk_scan_kernel(Src src, Sink sink, const int64_t* __restrict__ n_dev,
     const int64_t x = i < b1 ? src(i) : 0;
     int64_t t;
     const int64_t inc = block_scan_i64(x, &t);
-    if (i < b1) sink(i, pre + inc - x, x);
-    pre += t;
+    if (i < b1) sink(i, wsub(wadd(pre, inc), x), x);
+    pre = wadd(pre, t);
   }
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) sink.finish(tot);
 }

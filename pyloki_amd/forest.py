@@ -14,8 +14,9 @@ A level the loop cannot finish on the device is handled as the single-tree loop 
 buffers too small (children, history, the forest's survivor buffer) are grown and the
 forest resumes at that level; a level enqueued without the budget's cut whose survivors
 exceed the budget is flagged and resumed with the cut; a branch error or a tree whose
-passing children exceed its ``max_sugg`` (the reference's overflow rule) sends every tree
-on through its own ``Pruning`` from that level.
+passing children exceed its ``max_sugg`` (the reference's overflow rule) or whose level is
+too large for the forest's packed per-batch sums sends every tree on through its own
+``Pruning`` from that level.
 """
 
 from __future__ import annotations
@@ -269,7 +270,7 @@ class ForestSearch:
             src = (first_out + i_stop - 1) & 1 if self._pingpong(budget) else 1
             leaves, folds = buf.leaves[src], buf.folds[src]
         state = {"l0": level, "leaves": leaves, "folds": folds, "sizes": sizes, "buf": src}
-        if stop == ST_CHILD_CAP:
+        if stop == ST_CHILD_CAP and self.cap_child < 2**30:
             nc = int(rec[i_stop][REC_NCHILD])
             self.cap_child = _pow2(2 * max(self.cap_child, nc))
             buf.set_children(self.cap_child)
