@@ -128,7 +128,7 @@ def make_series(nseries: int):
     for i, ln in enumerate(logn):
         sim = PulseSignalConfig(period=5e-3, dt=64e-6, nsamps=2 ** int(ln), snr=15.0, ducy=0.08,
                                 mod_kwargs={"acc": 20.0})
-        out.append(sim.generate(seed=1000 + i))
+        out.append(sim.generate_analytic(seed=1000 + i))
     return out
 
 
@@ -222,9 +222,10 @@ def ep_circular_dyp(seed: int, w: dict = EP_CNN, use_fourier: bool = True):
     rng = np.random.default_rng(seed)
     f0 = 0.5 * (w["freq"][0] + w["freq"][1]) * (1 + 1e-4 * rng.uniform(-1, 1))
     ts = PulseSignalConfig(period=1 / f0, dt=w["tsamp"], nsamps=n, snr=w["snr"], ducy=0.05,
+                           mod_type="circular",
                            mod_kwargs={"p_orb": w["p_orb"], "x_orb": w["x_orb"],
-                                       "phi": float(rng.uniform(0, 2 * np.pi))}
-                           ).generate(seed=seed)
+                                       "psi": float(rng.uniform(0, 2 * np.pi))}
+                           ).generate_analytic(seed=seed)
     dyp = DynamicProgramming(ts, cfg)
     dyp.initialize()
     dyp.execute()
@@ -299,7 +300,7 @@ def ep_jerk_dyp(w: dict = EP_JERK):
                              prune_poly_order=3, ducy_max=0.3, wtsp=1.5, branch_max=64,
                              use_fourier=False)
     ts = PulseSignalConfig(period=0.002, dt=w["tsamp"], nsamps=n, snr=w["snr"], ducy=0.05,
-                           mod_kwargs={"acc": 5.0, "jerk": 0.05}).generate(seed=w["seed"])
+                           mod_kwargs={"acc": 5.0, "jerk": 0.05}).generate_analytic(seed=w["seed"])
     dyp = DynamicProgramming(ts, cfg)
     dyp.initialize()
     dyp.execute()

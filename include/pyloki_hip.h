@@ -925,6 +925,56 @@ int pyl_rows_gather(const float* src_dev, const int64_t* idx_dev, int64_t n, int
                     float* dst_dev, void* stream);
 
 /* ------------------------------------------------------------------------------------
+ * Seeded signal simulation.
+ * Replaces simulation/pulse.py:310-406 PulseSignalConfig.generate (its sample-level
+ * parts: generate_pulse_template :39-66, the noise draw, and the two brutefold calls of
+ * the calibration, core/fold.py:57-105) with the proper time of simulation/modulate.py
+ * (DerivativeModulating :34-54, DerivativeSeriesModulating :92-97, CircularModulating
+ * :147-150, CircularT0Modulating :222-226, KeplerianModulating :240-250 with
+ * kepler.py:64-140).
+ *
+ * pyl_modulation.p by kind:
+ *   DERIVATIVE         {shift, vel, acc, jerk, snap}
+ *   DERIVATIVE_SERIES  coeffs[k] = d_k / k!, k < n_coeffs
+ *   CIRCULAR           {p_orb, psi, x_orb}
+ *   CIRCULAR_T0        {a, p_orb, t0}
+ *   KEPLERIAN          {p_orb, ecc, phi, a, om, inc}
+ * ------------------------------------------------------------------------------------ */
+#define PYL_MOD_DERIVATIVE 0
+#define PYL_MOD_DERIVATIVE_SERIES 1
+#define PYL_MOD_CIRCULAR 2
+#define PYL_MOD_CIRCULAR_T0 3
+#define PYL_MOD_KEPLERIAN 4
+#define PYL_MOD_MAX_COEFFS 24
+#define PYL_SIM_MAX_SAMPS (((int64_t)1) << 27)
+
+typedef struct {
+  int32_t kind;     /* PYL_MOD_* */
+  int32_t n_coeffs; /* DERIVATIVE_SERIES: order + 1 */
+  double t_ref;     /* reference epoch of the modulation (mod_tref) */
+  double p[6];
+  double coeffs[PYL_MOD_MAX_COEFFS];
+} pyl_modulation;
+
+/* For every sample i < nsamps (t = i dt, t' = proper time): tmpl_dev[i] = the pulse
+ * template (the CDF difference between the phases of t' and t' + dt, interpolated in
+ * cdf_lut_dev, f32 [ngrid + 1] with the wrap element last); noise_dev[i] = f32(z) with z the
+ * normal deviate of pyl_rng_normal(seed, stream_id) row 0, column i; and the calibration
+ * folds at f = 1/period into folds_dev int64 [3][nbins]: fixed-point sums (value * 2^32,
+ * rounded to nearest even) of the noise and of the template, and the sample count, per bin
+ * int(f32(phase * nbins) + 0.5f) (nbins -> 0). nsamps <= PYL_SIM_MAX_SAMPS keeps the sums
+ * inside int64. Deterministic: the same arguments give the same bits. */
+int pyl_sim_series(const pyl_modulation* mod, int64_t nsamps, double dt, double period,
+                   const float* cdf_lut_dev, int32_t ngrid, uint64_t seed, uint64_t stream_id,
+                   int32_t nbins, float* noise_dev, float* tmpl_dev, int64_t* folds_dev,
+                   void* stream);
+/* ts_e_dev[i] = f32(noise_scale * noise[i]) + f32(signal_scale * tmpl[i]) and
+ * ts_v_dev[i] = variance, in FP32 (pulse.py:402-405). */
+int pyl_sim_combine(const float* noise_dev, const float* tmpl_dev, int64_t nsamps,
+                    float noise_scale, float signal_scale, float variance, float* ts_e_dev,
+                    float* ts_v_dev, void* stream);
+
+/* ------------------------------------------------------------------------------------
  * Bandwidth probes of known size (no reference counterpart; measurement only):
  * pyl_hbm_read loads n_float4 16-byte elements once each (one float per block written to
  * partial_dev [nblocks]); pyl_hbm_copy copies n_float4 elements. Used to measure the

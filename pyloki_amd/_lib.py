@@ -91,6 +91,11 @@ SIGNATURES: list[tuple[str, object, list[object]]] = [
     ("pyl_rng_normal", ctypes.c_int,
      [ctypes.c_uint64, ctypes.c_uint64, _c_i64, _c_i32, _c_dbl, _vp, _vp]),
     ("pyl_rows_gather", ctypes.c_int, [_vp, _vp, _c_i64, _c_i32, _vp, _vp]),
+    ("pyl_sim_series", ctypes.c_int,
+     [_vp, _c_i64, _c_dbl, _c_dbl, _vp, _c_i32, ctypes.c_uint64, ctypes.c_uint64, _c_i32, _vp,
+      _vp, _vp, _vp]),
+    ("pyl_sim_combine", ctypes.c_int,
+     [_vp, _vp, _c_i64, ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp, _vp, _vp]),
     ("pyl_hbm_read", ctypes.c_int, [_vp, _c_i64, _vp, _c_i32, _vp]),
     ("pyl_hbm_copy", ctypes.c_int, [_vp, _vp, _c_i64, _c_i32, _vp]),
     ("pyl_score_boxcar", ctypes.c_int,

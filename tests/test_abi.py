@@ -66,6 +66,7 @@ def test_struct_layouts_match_the_header(tmp_path):
     import subprocess
 
     from pyloki_amd import prune
+    from pyloki_amd.simulation.modulate import PylModulation
 
     gcc = shutil.which("gcc")
     if gcc is None:
@@ -75,7 +76,8 @@ def test_struct_layouts_match_the_header(tmp_path):
     structs = {"pyl_prune_params": prune.PruneParams, "pyl_prune_coords": prune.PruneCoords,
                "pyl_circ_params": prune.CircParams, "pyl_circ_level": prune.CircLevel,
                "pyl_ep_loop": prune.EpLoop, "pyl_ep_budget": prune.EpBudget,
-               "pyl_circ_loop": prune.CircLoop, "pyl_ep_forest": prune.EpForest}
+               "pyl_circ_loop": prune.CircLoop, "pyl_ep_forest": prune.EpForest,
+               "pyl_modulation": PylModulation}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "pyloki_hip.h"',
              "int main(void) {"]
     for cname, py in structs.items():

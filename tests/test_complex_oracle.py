@@ -71,7 +71,7 @@ def test_shift_add_complex_matches_real_shift_add_at_integer_shifts():
 def test_complex_search_agrees_with_real_search():
     nsamps, tsamp = 2**17, 256e-6
     sim = PulseSignalConfig(period=0.5, dt=tsamp, nsamps=nsamps, snr=25.0, ducy=0.08)
-    ts = sim.generate(seed=5)
+    ts = sim.generate_analytic(seed=5)
     common = dict(nsamps=nsamps, tsamp=tsamp, nbins=64, eta=1.0,
                   param_limits=np.array([[1.8, 2.2]]), ducy_max=0.3, wtsp=1.5)
     cfg_c = PulsarSearchConfig(**common, use_fourier=True)
@@ -127,7 +127,7 @@ def test_complex_ep_search_finds_the_real_search_candidate():
                   ducy_max=0.3, wtsp=1.5, branch_max=32)
     sim = PulseSignalConfig(period=1 / 200.37, dt=tsamp, nsamps=nsamps, snr=30.0, ducy=0.1,
                             mod_kwargs={"acc": 12.0})
-    ts = sim.generate(seed=4)
+    ts = sim.generate_analytic(seed=4)
     out = {}
     for cplx in (False, True):
         cfg = PulsarSearchConfig(**common, use_fourier=cplx)

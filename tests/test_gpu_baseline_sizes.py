@@ -112,7 +112,7 @@ def test_ffa_jerk_2e24_fp32_bf16_vs_oracle(gpu_device):
     counts = [int(c) for c in cfg.get_param_grid_count(cfg.niters_ffa)]
     assert counts == [8, 10, 57553]
     ts = PulseSignalConfig(period=3e-3, dt=64e-6, nsamps=n, snr=30.0, ducy=0.08,
-                           mod_kwargs={"acc": 10.0, "jerk": 0.5}).generate(seed=5)
+                           mod_kwargs={"acc": 10.0, "jerk": 0.5}).generate_analytic(seed=5)
     widths = cfg.score_widths
     plan = FFAPlan(cfg)
     e = torch.from_numpy(ts.ts_e).cuda()

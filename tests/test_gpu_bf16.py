@@ -30,7 +30,7 @@ def _jerk_cfg(nsamps: int, nbins: int = 128, nseg_ffa: int = 1) -> PulsarSearchC
 def _series(cfg: PulsarSearchConfig, seed: int = 3):
     sim = PulseSignalConfig(period=3e-3, dt=cfg.tsamp, nsamps=cfg.nsamps, snr=20.0, ducy=0.08,
                             mod_kwargs={"acc": 10.0, "jerk": 0.5})
-    return sim.generate(seed=seed)
+    return sim.generate_analytic(seed=seed)
 
 
 @pytest.mark.parametrize("nbins,P", [(64, 2), (128, 3)])
@@ -41,7 +41,7 @@ def test_bf16_levels_bit_exact(gpu_device, nbins, P):
         cfg = PulsarSearchConfig(nsamps=2**18, tsamp=64e-6, nbins=nbins, eta=1.0,
                                  param_limits=lim, bseg_brute=4096, use_fourier=False)
         ts = PulseSignalConfig(period=5e-3, dt=64e-6, nsamps=cfg.nsamps, snr=20.0,
-                               ducy=0.08).generate(seed=1)
+                               ducy=0.08).generate_analytic(seed=1)
     else:
         cfg = _jerk_cfg(2**19, nbins)
         ts = _series(cfg)

@@ -41,7 +41,7 @@ def _setup(P: int, nsamps: int, nseg_ffa: int, use_fourier: bool = False, seed: 
                              use_fourier=use_fourier, branch_max=32)
     sim = PulseSignalConfig(period=1 / (lim[-1][0] + 5.37), dt=tsamp, nsamps=nsamps, snr=30.0,
                             ducy=0.1, mod_kwargs=mod)
-    dyp = DynamicProgramming(sim.generate(seed=seed), cfg)
+    dyp = DynamicProgramming(sim.generate_analytic(seed=seed), cfg)
     dyp.initialize()
     dyp.execute()
     pcfg = {"poly_order": P, "nbins": cfg.nbins, "eta": cfg.eta, "branch_max": cfg.branch_max,

@@ -33,8 +33,9 @@ def _setup(nsamps=2**18, nseg=32, p_orb_min=20.0, use_fourier=False):
                              prune_poly_order=5, p_orb_min=p_orb_min, m_c_max=0.2, m_p_min=1.4,
                              ducy_max=0.3, wtsp=1.5, use_fourier=use_fourier, branch_max=32)
     sim = PulseSignalConfig(period=1 / 200.17, dt=tsamp, nsamps=nsamps, snr=30.0, ducy=0.1,
-                            mod_kwargs={"p_orb": 30.0, "x_orb": 0.002, "phi": 0.7})
-    dyp = DynamicProgramming(sim.generate(seed=3), cfg)
+                            mod_type="circular",
+                            mod_kwargs={"p_orb": 30.0, "x_orb": 0.002, "psi": 0.7})
+    dyp = DynamicProgramming(sim.generate_analytic(seed=3), cfg)
     dyp.initialize()
     dyp.execute()
     pcfg = {"poly_order": 5, "nbins": cfg.nbins, "eta": cfg.eta,

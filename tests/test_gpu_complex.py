@@ -43,7 +43,7 @@ def _series(cfg: PulsarSearchConfig, seed: int = 2):
     period = 0.5 if cfg.nparams == 1 else 1 / 200.37
     mod = {} if cfg.nparams == 1 else {"acc": 12.0}
     return PulseSignalConfig(period=period, dt=cfg.tsamp, nsamps=cfg.nsamps, snr=25.0,
-                             ducy=0.08, mod_kwargs=mod).generate(seed=seed)
+                             ducy=0.08, mod_kwargs=mod).generate_analytic(seed=seed)
 
 
 @pytest.mark.parametrize("nbins,seg_len,nfreq", [(64, 4096, 37), (32, 2048, 9), (128, 1024, 5)])
@@ -137,19 +137,22 @@ def _ep_setup(circular: bool, use_fourier: bool = True):
         lim = ParamLimits.from_circular((199.0, 201.0), p_orb_min=p_orb_min, m_c_max=0.2,
                                         m_p_min=1.4).limits
         extra = dict(prune_poly_order=5, p_orb_min=p_orb_min, m_c_max=0.2, m_p_min=1.4)
-        mod = {"p_orb": 30.0, "x_orb": 0.002, "phi": 0.7}
+        mod = {"p_orb": 30.0, "x_orb": 0.002, "psi": 0.7}
+        mod_type = "circular"
         period = 1 / 200.17
     else:
         nsamps, nseg = 2**18, 16
         lim = ParamLimits.from_taylor((195.0, 205.0), accel=(-50.0, 50.0)).limits
         extra = dict(prune_poly_order=2)
         mod = {"acc": 12.0}
+        mod_type = "derivative"
         period = 1 / 200.37
     cfg = PulsarSearchConfig(nsamps=nsamps, tsamp=64e-6, nbins=32, eta=1.0, param_limits=lim,
                              bseg_brute=nsamps // 256, bseg_ffa=nsamps // nseg, ducy_max=0.3,
                              wtsp=1.5, use_fourier=use_fourier, branch_max=32, **extra)
     ts = PulseSignalConfig(period=period, dt=64e-6, nsamps=nsamps, snr=30.0, ducy=0.1,
-                           mod_kwargs=mod).generate(seed=3 if circular else 4)
+                           mod_type=mod_type,
+                           mod_kwargs=mod).generate_analytic(seed=3 if circular else 4)
     dyp = DynamicProgramming(ts, cfg)
     dyp.initialize()
     dyp.execute()

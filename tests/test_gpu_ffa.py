@@ -243,7 +243,7 @@ def test_ffa_search_config1_full_size(gpu_device):
     _case("ffa_search config 1 full size")
     nsamps, tsamp = 2**21, 256e-6
     sim = PulseSignalConfig(period=0.5, dt=tsamp, nsamps=nsamps, snr=15.0, ducy=0.05)
-    ts = sim.generate(seed=11)
+    ts = sim.generate_analytic(seed=11)
     cfg = PulsarSearchConfig(nsamps=nsamps, tsamp=tsamp, nbins=64, eta=1.0,
                              param_limits=np.array([[1.5, 2.5]]), ducy_max=0.5, wtsp=2.0,
                              use_fourier=False)

@@ -72,7 +72,7 @@ def test_pyl_ffa_level_matches_unify_fold(gpu_device, P, nbins):
     lib = _lib.load()
     cfg = _cfg(P, 2**16, nbins)
     ts = PulseSignalConfig(period=1 / 200.3, dt=cfg.tsamp, nsamps=cfg.nsamps, snr=20.0,
-                           ducy=0.1).generate(seed=P)
+                           ducy=0.1).generate_analytic(seed=P)
     pa0 = cfg.get_param_arr(cfg.get_dparams(0))
     t_ref = 0.0 if P == 1 else cfg.bseg_brute * cfg.tsamp / 2
     fold = orc.brutefold(ts.ts_e, ts.ts_v, pa0[-1], cfg.bseg_brute, nbins, cfg.tsamp, t_ref)

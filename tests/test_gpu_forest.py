@@ -30,8 +30,9 @@ def _series(seeds, use_fourier: bool):
     for s in seeds:
         sim = PulseSignalConfig(period=1 / 200.17, dt=cfg.tsamp, nsamps=cfg.nsamps, snr=30.0,
                                 ducy=0.1,
-                                mod_kwargs={"p_orb": 30.0, "x_orb": 0.002, "phi": 0.1 * s})
-        dyp = DynamicProgramming(sim.generate(seed=s), cfg)
+                                mod_type="circular",
+                                mod_kwargs={"p_orb": 30.0, "x_orb": 0.002, "psi": 0.1 * s})
+        dyp = DynamicProgramming(sim.generate_analytic(seed=s), cfg)
         dyp.initialize()
         dyp.execute()
         out.append(dyp)

@@ -35,7 +35,7 @@ def _setup(P: int, nsamps: int, nseg_ffa: int, nbins: int = 32, seed: int = 4):
                              branch_max=32)
     sim = PulseSignalConfig(period=1 / (lim[-1][0] + 5.37), dt=tsamp, nsamps=nsamps, snr=30.0,
                             ducy=0.1, mod_kwargs=mod)
-    ts = sim.generate(seed=seed)
+    ts = sim.generate_analytic(seed=seed)
     dyp = DynamicProgramming(ts, cfg)
     dyp.initialize()
     dyp.execute()

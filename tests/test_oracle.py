@@ -249,7 +249,7 @@ def test_ffa_recovers_injected_frequency():
                              param_limits=np.array([[3.0, 5.0]]), bseg_brute=512,
                              ducy_max=0.3, wtsp=1.5, use_fourier=False)
     sim = PulseSignalConfig(period=1 / 4.1234, dt=1e-3, nsamps=2**16, snr=40, ducy=0.08)
-    ts = sim.generate(seed=3)
+    ts = sim.generate_analytic(seed=3)
     snr = orc.ffa_search_snr(ts.ts_e, ts.ts_v, cfg)
     freqs = cfg.get_param_arr(cfg.get_dparams(cfg.niters_ffa))[0]
     fbest = freqs[np.unravel_index(np.argmax(snr), snr.shape)[0]]

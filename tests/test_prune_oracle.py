@@ -121,7 +121,7 @@ def _small_search(nsamps=2**17, nseg_ffa=8, seed=4):
                              branch_max=32)
     sim = PulseSignalConfig(period=1 / 200.37, dt=tsamp, nsamps=nsamps, snr=30.0, ducy=0.1,
                             mod_kwargs={"acc": 12.0})
-    ts = sim.generate(seed=seed)
+    ts = sim.generate_analytic(seed=seed)
     fold, pa = orc.ffa_fold(ts.ts_e, ts.ts_v, cfg)
     counts = [len(a) for a in pa]
     dyp_fold = fold.reshape(fold.shape[0], *counts, 2, cfg.nbins)
