@@ -81,8 +81,10 @@ int pyl_brutefold_set_segments_per_block(int32_t segs);
  *   ffa_taylor_init_complex fold.py:572-614, FFATaylorComplexDPFuncts.init
  *   dyn_pffa.py:82-140): fold_dev complex64 (float pairs) [nseg][nfreq][2][nbins/2 + 1],
  *   nsamps a multiple of seg_len. The plan holds the FP32 basis of the comb of
- *   harmonics; an execute is one FP32 GEMM per component (rocBLAS) plus the t_ref
- *   correction. Summation order is the GEMM's (the reference's is its BLAS library's).
+ *   harmonics; an execute is one launch of the library's FP32 matrix-core product of
+ *   the segments (both components) and the basis, with the t_ref correction and the fold
+ *   layout in its epilogue. Summation order is the matrix unit's (the reference's is its
+ *   BLAS library's). No scratch memory.
  * pyl_ffa_shift_add_complex: the segment-pair loop of unify_fold with
  *   common.py:344-372 shift_add_complex: out[k] = tail[k] phi_t^k + head[k] phi_h^k, the
  *   phasors by the reference's complex64 recurrence from the FP64 phases of
@@ -550,6 +552,10 @@ typedef struct pyl_ep_budget {
    * runs per level, where the budget always applies). Results do not depend on the flags;
    * the launches of a level without the cut are 1 instead of 15. */
   const uint8_t* level_mask;
+  /* Host budgets indexed by search level (0..nlevels), each >= 1, or null: every level
+   * keeps `budget`. A schedule keeps more survivors on the levels where the ranking is
+   * least reliable. */
+  const int64_t* level_budget;
 } pyl_ep_budget;
 
 /* The budget on one tree of n rows (the per-level path): writes the kept rows to out_*

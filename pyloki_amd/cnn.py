@@ -185,9 +185,86 @@ def ancestor_labels(recs: list[LevelRecord]) -> list[torch.Tensor]:
     return labels
 
 
+def descendant_counts(recs: list[LevelRecord]) -> list[torch.Tensor]:
+    """For the survivors of each level: how many of the last level's survivors (the final
+    candidates) descend from it."""
+    counts = [None] * len(recs)
+    if not recs:
+        return counts
+    dev = recs[-1].folds.device
+    cur = torch.ones(recs[-1].folds.shape[0], dtype=torch.float64, device=dev)
+    counts[-1] = cur
+    for k in range(len(recs) - 1, 0, -1):
+        prev = torch.zeros(recs[k - 1].folds.shape[0], dtype=torch.float64, device=dev)
+        prev.index_add_(0, recs[k].backtracks[:, 0].long(), cur)
+        counts[k - 1] = prev
+        cur = prev
+    return counts
+
+
+def level_needs(recs: list[LevelRecord], weights: list[torch.Tensor], key_fn,
+                loss: float = 0.0) -> dict:
+    """Per search level of an exhaustive run: the smallest budget whose cut, ranking the
+    survivors by ``key_fn(record)`` (ties keep the lowest rows, as the cut does), loses at
+    most the fraction ``loss`` of the final candidates (``weights``: each survivor's final
+    descendants, ``descendant_counts``; with loss 0 every ancestor is kept)."""
+    out = {}
+    for r, wt in zip(recs, weights):
+        total = float(wt.sum().item())
+        if total <= 0:
+            out[r.level] = 0
+            continue
+        key = key_fn(r).float()
+        order = torch.argsort(-key, stable=True)
+        w_sorted = wt[order]
+        # lost(b) = weight ranked at b or below; the smallest b with lost(b) <= loss*total
+        tail = torch.flip(torch.cumsum(torch.flip(w_sorted, [0]), 0), [0])  # tail[b]
+        ok = torch.nonzero(tail <= loss * total * (1 + 1e-12)).reshape(-1)
+        b = int(ok[0].item()) if len(ok) else len(order)
+        out[r.level] = b
+    return out
+
+
+def budget_schedule(needs: list[dict], nlevels: int, margin: float = 1.5, floor: int = 512,
+                    window: int = 2) -> np.ndarray:
+    """A budget per search level (0..nlevels) from the needs of calibration runs: the
+    largest need of any run within ``window`` levels, times ``margin``, at least
+    ``floor``."""
+    need = np.zeros(nlevels + 1, dtype=np.float64)
+    for nd in needs:
+        for level, v in nd.items():
+            need[level] = max(need[level], v)
+    sched = np.empty_like(need)
+    for level in range(nlevels + 1):
+        lo, hi = max(0, level - window), min(nlevels, level + window)
+        sched[level] = need[lo:hi + 1].max()
+    return np.maximum(np.ceil(margin * sched), floor).astype(np.int64)
+
+
+def snr_key(r: LevelRecord) -> torch.Tensor:
+    return r.feats[:, 0]
+
+
+def network_keys(ranker: BranchRanker, hybrid: bool):
+    """``key_fn`` of the network's ranking (BF16 inference, as the budget runs it); with
+    ``hybrid`` the larger of the rank fractions by logit and by S/N."""
+    def key(r: LevelRecord) -> torch.Tensor:
+        lg = ranker.logits(r.folds, r.feats, precision="bf16")
+        if not hybrid:
+            return lg
+        n = lg.numel()
+        order = torch.argsort(lg, stable=True)
+        rk = torch.empty(n, dtype=torch.float32, device=lg.device)
+        rk[order] = torch.arange(n, dtype=torch.float32, device=lg.device) / max(n - 1, 1)
+        return torch.maximum(rk, r.feats[:, 3])
+    return key
+
+
 def training_set(recs: list[LevelRecord], labels: list[torch.Tensor],
-                 neg_per_level: int = 2048, seed: int = 0):
-    """Every positive survivor and up to ``neg_per_level`` random negatives per level."""
+                 neg_per_level: int = 2048, seed: int = 0, pos_per_level: int = 0):
+    """Every positive survivor and up to ``neg_per_level`` random negatives per level;
+    with ``pos_per_level`` the positives of a level are repeated up to about that many
+    (the first levels hold one or two ancestors each, the last ones hundreds)."""
     dev = recs[0].folds.device
     g = torch.Generator(device=dev)
     g.manual_seed(seed)
@@ -197,6 +274,8 @@ def training_set(recs: list[LevelRecord], labels: list[torch.Tensor],
         neg = torch.nonzero(lab == 0).reshape(-1)
         if len(neg) > neg_per_level:
             neg = neg[torch.randperm(len(neg), device=dev, generator=g)[:neg_per_level]]
+        if 0 < len(pos) < pos_per_level:
+            pos = pos.repeat((pos_per_level + len(pos) - 1) // len(pos))
         idx = torch.cat([pos, neg])
         fo.append(r.folds[idx])
         fe.append(r.feats[idx])
@@ -229,18 +308,25 @@ def train(ranker: BranchRanker, folds: torch.Tensor, feats: torch.Tensor,
 class SurvivorBudget:
     """Keep, at every level, the ``budget`` survivors of highest key: the ranking
     network's logit when ``ranker`` is given, else the S/N (``pyl_ep_budget_apply``; ties
-    at the cut keep the lowest rows). Installed as ``Pruning.level_hook``: the sync-free
-    level loop applies it on the device (``device_budget``), the per-level path calls it."""
+    at the cut keep the lowest rows). ``budget`` is one number for every level, or a
+    schedule: one budget per search level 0..nsegments - 1 (``BudgetSchedule``).
+    Installed as ``Pruning.level_hook``: the sync-free level loop applies it on the device
+    (``device_budget``), the per-level path calls it."""
 
-    def __init__(self, budget: int, ranker: BranchRanker | None = None,
+    def __init__(self, budget, ranker: BranchRanker | None = None,
                  precision: str = "bf16", hybrid: bool = False) -> None:
-        if budget < 1:
+        sched = None
+        if np.ndim(budget) > 0:
+            sched = np.ascontiguousarray(budget, dtype=np.int64)
+            budget = int(sched.max()) if sched.size else 0
+        if budget < 1 or (sched is not None and int(sched.min()) < 1):
             msg = "budget must be >= 1"
             raise ValueError(msg)
         if precision not in ("fp32", "bf16"):
             msg = f"unknown precision {precision!r}"
             raise ValueError(msg)
-        self.budget = int(budget)
+        self.schedule = sched  # per search level, or None
+        self.budget = int(budget)  # the largest budget of any level
         self.ranker = ranker
         self.precision = precision  # of the network's forward pass (bf16: MFMA)
         # hybrid: rank by the larger of the rank fractions by logit and by S/N (keeps what
@@ -254,6 +340,12 @@ class SurvivorBudget:
         # Level loop: levels that get the ranking and the cut (None: all), see device_budget.
         self.level_mask: np.ndarray | None = None
         self._mask = None
+
+    def budget_at(self, level: int) -> int:
+        """The budget of search level ``level``."""
+        if self.schedule is None:
+            return self.budget
+        return int(self.schedule[min(int(level), len(self.schedule) - 1)])
 
     def _buffers(self, pr, cap: int):
         """The C struct with scratch for ``cap`` rows (kept alive here)."""
@@ -283,6 +375,14 @@ class SurvivorBudget:
                 setattr(s, name, ptr(ten))
             self._t, self._s, self._cap = t, s, cap
         self._s.nlevels = pr.dyp.nsegments - 1
+        if self.schedule is not None:
+            if len(self.schedule) != self._s.nlevels + 1:
+                msg = (f"the budget schedule needs {self._s.nlevels + 1} levels, "
+                       f"got {len(self.schedule)}")
+                raise ValueError(msg)
+            self._s.level_budget = self.schedule.ctypes.data
+        else:
+            self._s.level_budget = None
         return self._s
 
     def device_budget(self, pr, cap: int, level0: int):
@@ -313,7 +413,7 @@ class SurvivorBudget:
         lib = _lib.load()
         s = self._buffers(pr, n)
         P = int(tree.leaves.shape[1]) - 2
-        k = min(n, self.budget)
+        k = min(n, self.budget_at(pr.prune_level))
         dev = tree.leaves.device
         rec = torch.empty((2, _REC_WORDS), dtype=torch.int64, device=dev)
         out = (torch.empty((k, P + 2, 2), dtype=torch.float64, device=dev),
@@ -371,6 +471,68 @@ def compare_searches(pr_factory, ref_seg: int, hooks: dict) -> dict:
                      "n_final": len(r["final"]),
                      "trial_ratio": ex["trials"] / max(r["trials"], 1),
                      "speedup": ex["time_s"] / r["time_s"]}
+    return out
+
+
+def train_ranker(make_dyp, seeds, thresholds: np.ndarray, max_sugg: int, epochs: int = 16,
+                 batch_size: int = 1024, nbins: int = 64, seed: int = 0,
+                 pos_per_level: int = 0) -> tuple[BranchRanker, dict]:
+    """The network trained on the exhaustive runs of ``make_dyp(s)`` for ``seeds`` (the
+    method's own folds: every level's survivors, labelled by whether they are ancestors of
+    a final candidate)."""
+    from pyloki_amd.prune import Pruning
+
+    t0 = time.perf_counter()
+    parts = []
+    for s in seeds:
+        dyp = make_dyp(s)
+        pr = Pruning(dyp, thresholds, max_sugg=max_sugg, batch_size=batch_size)
+        recs = record_levels(pr, dyp.nsegments // 2)
+        if recs:
+            parts.append(training_set(recs, ancestor_labels(recs), seed=seed + s,
+                                      pos_per_level=pos_per_level))
+        del recs, pr
+    folds = torch.cat([p[0] for p in parts])
+    feats = torch.cat([p[1] for p in parts])
+    labels = torch.cat([p[2] for p in parts])
+    t_data = time.perf_counter() - t0
+    ranker = BranchRanker(nbins, seed=seed)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist = train(ranker, folds, feats, labels, epochs=epochs, seed=seed)
+    torch.cuda.synchronize()
+    info = {"samples": int(folds.shape[0]), "positives": int(labels.sum().item()),
+            "epochs": epochs, "loss_first": hist[0], "loss_last": hist[-1],
+            "data_s": t_data, "train_s": time.perf_counter() - t0, "seeds": list(seeds)}
+    return ranker, info
+
+
+def calibrate_schedules(make_dyp, seeds, thresholds: np.ndarray, max_sugg: int,
+                        ranker: BranchRanker | None = None, margin: float = 1.5,
+                        floor: int = 512, window: int = 2, batch_size: int = 1024,
+                        loss: float = 0.0) -> dict:
+    """Budget schedules of the S/N key (and, with ``ranker``, of the network's and the
+    hybrid key) from the exhaustive runs of calibration series, which the network has not
+    been trained on: ``budget_schedule`` of their ``level_needs`` at ``loss``."""
+    from pyloki_amd.prune import Pruning
+
+    keys = {"snr": snr_key}
+    if ranker is not None:
+        keys["cnn"] = network_keys(ranker, hybrid=False)
+        keys["hybrid"] = network_keys(ranker, hybrid=True)
+    needs = {k: [] for k in keys}
+    nlevels = 0
+    for s in seeds:
+        dyp = make_dyp(s)
+        nlevels = dyp.nsegments - 1
+        pr = Pruning(dyp, thresholds, max_sugg=max_sugg, batch_size=batch_size)
+        recs = record_levels(pr, dyp.nsegments // 2)
+        weights = descendant_counts(recs)
+        for k, fn in keys.items():
+            needs[k].append(level_needs(recs, weights, fn, loss))
+        del recs, weights, pr
+    out = {k: budget_schedule(v, nlevels, margin, floor, window) for k, v in needs.items()}
+    out["needs"] = needs
     return out
 
 

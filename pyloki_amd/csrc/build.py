@@ -35,9 +35,8 @@ HIPCC_FLAGS = [
     "-Wno-unused-result",
 ]
 
-# rocBLAS for the one plain GEMM of the path (the complex leaf fold). It resolves to the
-# rocBLAS already loaded by PyTorch-ROCm when that is present (same soname).
-LINK_FLAGS = ["-L/opt/rocm/lib", "-lrocblas"]
+# Every kernel of the path is the project's own: no vendor library is linked.
+LINK_FLAGS: list[str] = []
 
 
 def hipcc() -> str:

@@ -575,19 +575,19 @@ fgather_kernel(const int64_t* __restrict__ tab, int T, const int64_t* __restrict
 // The cut of every tree of a forest on `keys` (survivor order): its kept rows' indices,
 // the trees' kept offsets and the level's totals (the select state initialised first
 // unless `inited`).
-int forest_cut(const pyl_ep_budget* b, const pyl_ep_forest* F, int64_t* tab,
+int forest_cut(const pyl_ep_budget* b, const pyl_ep_forest* F, int64_t bud, int64_t* tab,
                int64_t* tab_next, const float* keys, int64_t n_cap, int64_t* rec, bool inited,
                hipStream_t s) {
   const int T = F->ntree;
   const int64_t* so = tab + PYL_EP_TAB_SO * (T + 1);
   const int64_t tree_cap = std::min<int64_t>(n_cap, F->max_sugg);
-  if (!inited) fsel_init_kernel<<<(unsigned)T, 256, 0, s>>>(rec, so, b->budget, F->sel);
+  if (!inited) fsel_init_kernel<<<(unsigned)T, 256, 0, s>>>(rec, so, bud, F->sel);
   const dim3 gsel(forest_grid(tree_cap, T), (unsigned)T);
   for (int shift = 24; shift >= 0; shift -= 8)
     sel_hist_kernel<<<gsel, kT, 0, s>>>(keys, nullptr, F->sel, shift, so);
   chunked_scan_seg(FKeepSrc{keys, F->sel}, FKeepSink{b->kept, F->sel, so, F->kcount}, so, T,
                    F->part, s);
-  fkeep_offsets_kernel<<<1, 64, 0, s>>>(rec, tab, tab_next, T, F->kcount, 0, b->budget);
+  fkeep_offsets_kernel<<<1, 64, 0, s>>>(rec, tab, tab_next, T, F->kcount, 0, bud);
   PYL_CHECK_LAUNCH("fkeep_offsets_kernel");
   return PYL_OK;
 }
@@ -605,15 +605,16 @@ int pyl::ep_budget_forest_enqueue(const pyl_ep_budget* b, const pyl_ep_forest* F
   PYL_REQUIRE(b->kind == PYL_BUDGET_SCORE || b->kind == PYL_BUDGET_CNN ||
                   b->kind == PYL_BUDGET_HYBRID,
               "unknown budget kind %d", b->kind);
-  PYL_REQUIRE(b->budget >= 1, "budget must be >= 1");
+  const int64_t bud = budget_of(b, level);
+  PYL_REQUIRE(bud >= 1, "budget must be >= 1");
   PYL_REQUIRE(n_cap >= 0 && n_cap < 0x7fffffffLL, "too many survivors for the budget");
   const int T = F->ntree;
   const int row_floats = b->complex_folds ? 4 * (b->nbins / 2 + 1) : 2 * b->nbins;
   const int64_t* so = tab + PYL_EP_TAB_SO * (T + 1);
-  const int64_t rows = std::min<int64_t>(n_cap, (int64_t)T * b->budget);
+  const int64_t rows = std::min<int64_t>(n_cap, (int64_t)T * bud);
   const unsigned g_rows = (unsigned)std::max<int64_t>(1, pyl::ceil_div(rows, kT / 64));
   if (b->level_mask != nullptr && b->level_mask[level] == 0) {
-    fkeep_offsets_kernel<<<1, 64, 0, s>>>(rec, tab, tab_next, T, F->kcount, 1, b->budget);
+    fkeep_offsets_kernel<<<1, 64, 0, s>>>(rec, tab, tab_next, T, F->kcount, 1, bud);
     fgather_kernel<<<g_rows, kT, 0, s>>>(tab, T, nullptr, rec, P, row_floats, leaves, folds,
                                          scores, bt, o_leaves, o_folds, o_scores, o_hist);
     PYL_CHECK_LAUNCH("fgather_kernel");
@@ -627,7 +628,7 @@ int pyl::ep_budget_forest_enqueue(const pyl_ep_budget* b, const pyl_ep_forest* F
   const int64_t tree_cap = std::min<int64_t>(n_cap, F->max_sugg);
   const float* keys = scores;
   if (net) {
-    fsel_init_kernel<<<(unsigned)T, 256, 0, s>>>(rec, so, b->budget, F->sel);
+    fsel_init_kernel<<<(unsigned)T, 256, 0, s>>>(rec, so, bud, F->sel);
     int rc = features_enqueue(leaves, scores, tree_cap, r_dev, P, level, b->nlevels, F->stats,
                               F->shist, b->feats, s, so, T);
     if (rc != PYL_OK) return rc;
@@ -645,7 +646,7 @@ int pyl::ep_budget_forest_enqueue(const pyl_ep_budget* b, const pyl_ep_forest* F
       PYL_CHECK_LAUNCH("hybrid_key_kernel");
     }
   }
-  int rc = forest_cut(b, F, tab, tab_next, keys, n_cap, rec, net, s);
+  int rc = forest_cut(b, F, bud, tab, tab_next, keys, n_cap, rec, net, s);
   if (rc != PYL_OK) return rc;
   fgather_kernel<<<g_rows, kT, 0, s>>>(tab, T, b->kept, rec, P, row_floats, leaves, folds,
                                        scores, bt, o_leaves, o_folds, o_scores, o_hist);
@@ -659,16 +660,17 @@ int pyl::ep_budget_forest_select(const pyl_ep_budget* b, const pyl_ep_forest* F,
   PYL_REQUIRE(b != nullptr && F != nullptr, "null budget");
   PYL_REQUIRE(level >= 0 && level <= b->nlevels, "level %d outside the search", level);
   PYL_REQUIRE(b->kind == PYL_BUDGET_SCORE, "the select-first cut ranks by S/N only");
-  PYL_REQUIRE(b->budget >= 1, "budget must be >= 1");
+  const int64_t bud = budget_of(b, level);
+  PYL_REQUIRE(bud >= 1, "budget must be >= 1");
   PYL_REQUIRE(n_cap >= 0 && n_cap < 0x7fffffffLL, "too many survivors for the budget");
   const int T = F->ntree;
   if (b->level_mask != nullptr && b->level_mask[level] == 0) {
-    fkeep_offsets_kernel<<<1, 64, 0, s>>>(rec, tab, tab_next, T, F->kcount, 1, b->budget);
+    fkeep_offsets_kernel<<<1, 64, 0, s>>>(rec, tab, tab_next, T, F->kcount, 1, bud);
     PYL_CHECK_LAUNCH("fkeep_offsets_kernel");
     return PYL_OK;
   }
   PYL_REQUIRE(b->kept && keys && F->sel && F->part && F->kcount, "null budget buffer");
-  return forest_cut(b, F, tab, tab_next, keys, n_cap, rec, false, s);
+  return forest_cut(b, F, bud, tab, tab_next, keys, n_cap, rec, false, s);
 }
 
 int pyl::ep_budget_enqueue(const pyl_ep_budget* b, int P, int level, const double* leaves,
@@ -679,7 +681,9 @@ int pyl::ep_budget_enqueue(const pyl_ep_budget* b, int P, int level, const doubl
   PYL_REQUIRE(b->kind == PYL_BUDGET_SCORE || b->kind == PYL_BUDGET_CNN ||
                   b->kind == PYL_BUDGET_HYBRID,
               "unknown budget kind %d", b->kind);
-  PYL_REQUIRE(b->budget >= 1, "budget must be >= 1");
+  PYL_REQUIRE(level >= 0 && level <= b->nlevels, "level %d outside the search", level);
+  const int64_t bud = budget_of(b, level);
+  PYL_REQUIRE(bud >= 1, "budget must be >= 1");
   PYL_REQUIRE(n_cap >= 0 && n_cap < 0x7fffffffLL, "too many survivors for the budget");
   const bool net = b->kind != PYL_BUDGET_SCORE;
   PYL_REQUIRE(b->kept && b->sel && b->part &&
@@ -687,7 +691,7 @@ int pyl::ep_budget_enqueue(const pyl_ep_budget* b, int P, int level, const doubl
               "null budget buffer");
   const int64_t* m_dev = rec + kRecNsurv;
   const int64_t* r_dev = rec + PYL_EP_REC_NRANK;  // 0 when every survivor is kept
-  sel_init_kernel<<<1, 256, 0, s>>>(rec, b->budget, b->sel);
+  sel_init_kernel<<<1, 256, 0, s>>>(rec, bud, b->sel);
   const float* keys = scores;
   if (net) {
     int rc = features_enqueue(leaves, scores, n_cap, r_dev, P, level, b->nlevels, b->stats,
@@ -712,7 +716,7 @@ int pyl::ep_budget_enqueue(const pyl_ep_budget* b, int P, int level, const doubl
   }
   chunked_scan(KeepSrc{keys, b->sel}, KeepSink{b->kept, b->sel, rec}, m_dev, b->part, s);
   const int row_floats = b->complex_folds ? 4 * (b->nbins / 2 + 1) : 2 * b->nbins;
-  const int64_t rows = std::min<int64_t>(n_cap, b->budget);
+  const int64_t rows = std::min<int64_t>(n_cap, bud);
   budget_gather_kernel<<<(unsigned)std::max<int64_t>(1, pyl::ceil_div(rows, kT / 64)), kT, 0,
                          s>>>(b->kept, rec, P, row_floats, leaves, folds, scores, bt,
                               o_leaves, o_folds, o_scores, o_hist);
@@ -730,11 +734,12 @@ int pyl::ep_budget_level_enqueue(const pyl_ep_budget* b, int P, int level,
   if (b->level_mask == nullptr || b->level_mask[level] != 0)
     return ep_budget_enqueue(b, P, level, leaves, folds, scores, bt, n_cap, rec, o_leaves,
                              o_folds, o_scores, o_hist, s);
-  PYL_REQUIRE(b->budget >= 1, "budget must be >= 1");
+  const int64_t bud = budget_of(b, level);
+  PYL_REQUIRE(bud >= 1, "budget must be >= 1");
   const int row_floats = b->complex_folds ? 4 * (b->nbins / 2 + 1) : 2 * b->nbins;
-  const int64_t rows = std::min<int64_t>(n_cap, b->budget);
+  const int64_t rows = std::min<int64_t>(n_cap, bud);
   budget_pass_kernel<<<(unsigned)std::max<int64_t>(1, pyl::ceil_div(rows, kT / 64)), kT, 0,
-                       s>>>(rec, b->budget, P, row_floats, leaves, folds, scores, bt,
+                       s>>>(rec, bud, P, row_floats, leaves, folds, scores, bt,
                             o_leaves, o_folds, o_scores, o_hist);
   PYL_CHECK_LAUNCH("budget_pass_kernel");
   return PYL_OK;

@@ -6,6 +6,11 @@
 
 namespace pyl {
 
+// The budget of search level `level`: the schedule's entry, or the single budget.
+inline int64_t budget_of(const pyl_ep_budget* b, int level) {
+  return b->level_budget != nullptr ? b->level_budget[level] : b->budget;
+}
+
 // The network's forward pass over at most n_cap samples, the count read from n_dev
 // when not null; bf16 selects the MFMA inference kernel.
 int cnn_forward_count(const float* params, const float* folds, int64_t n_cap,

@@ -5,9 +5,13 @@
 // dense contraction: the segments (rows of length seg_len) times a basis of n_comb =
 // nfreq * (B/2 + 1) complex exponentials. The basis is data-independent, so the plan
 // builds it once in HBM (FP32 cos / sin of the FP64 angle f32(f m) * t * f32(-2 pi tsamp),
-// as the reference does); each execute is one FP32 GEMM per component through rocBLAS
-// (MFMA on gfx950) followed by a kernel that applies the e^{+2 pi i f m t_ref} correction in
-// complex64 and writes the fold layout [nseg][nfreq][2][B/2 + 1].
+// as the reference does). Each execute is one launch of cfold_mfma_kernel, an FP32 product
+// on the matrix cores (v_mfma_f32_32x32x2f32): a block takes 64 segments and 64 comb
+// entries; its A tile holds the e and the v rows of the segments (so both components share
+// every basis tile) and its B tile the cos and the sin rows of the comb entries (so a lane
+// ends with the real and the imaginary part of the same output). The epilogue applies the
+// e^{+2 pi i f m t_ref} correction in complex64 and writes the fold layout
+// [nseg][nfreq][2][B/2 + 1]; nothing is staged in HBM between the product and the fold.
 //
 // Level shift-add. out[k] = tail[k] phi_t^k + head[k] phi_h^k with the reference's
 // complex64 recurrence phi^{k+1} = phi^k * delta, delta = (cos, sin)(f32(-2 pi f32(shift)/B)).
@@ -18,8 +22,6 @@
 // Scoring. irfft to B real bins (x[n] = (X_0 + (-1)^n X_{B/2} + 2 sum Re(X_k e^{2 pi i k n/B}))
 // / B, FP32 with an FP32 twiddle table), e / sqrt(v), then the boxcar scorer of
 // pyl_score.hpp (one thread per profile).
-#include <rocblas/rocblas.h>
-
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -34,7 +36,6 @@ struct pyl_cfold_plan {
   int64_t n_comb = 0;
   float* basis = nullptr;   // [2][n_comb][seg_len]: cos rows, then sin rows
   float2* corr = nullptr;   // [n_comb] complex64 t_ref correction
-  rocblas_handle blas = nullptr;
 };
 
 namespace {
@@ -55,22 +56,127 @@ __global__ void cfold_basis_kernel(const float* __restrict__ target, int64_t n_c
 
 using pyl::cmul;
 
-// C_comp[s][i] (real part, i < n_comb) and C_comp[s][n_comb + i] (imaginary part) times the
-// correction phasor, into fold[s][f][comp][m].
-__global__ void cfold_finish_kernel(const float* __restrict__ ce, const float* __restrict__ cv,
-                                    const float2* __restrict__ corr, int64_t nseg,
-                                    int64_t n_comb, int nb_f, float2* __restrict__ fold) {
-  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= nseg * 2 * n_comb) return;
-  const int64_t s = gid / (2 * n_comb);
-  const int64_t r = gid - s * 2 * n_comb;
-  const int comp = (int)(r / n_comb);
-  const int64_t i = r - (int64_t)comp * n_comb;
-  const float* c = comp == 0 ? ce : cv;
-  const float2 x = make_float2(c[s * 2 * n_comb + i], c[s * 2 * n_comb + n_comb + i]);
-  const int64_t f = i / nb_f;
-  const int m = (int)(i - f * nb_f);
-  fold[((s * (n_comb / nb_f) + f) * 2 + comp) * nb_f + m] = cmul(x, corr[i]);
+typedef float cf_f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kCfSeg = 64;            // segments per block (A rows: their e, then v rows)
+constexpr int kCfComb = 64;           // comb entries per block (B rows: cos, then sin)
+constexpr int kCfK = 16;              // K per LDS stage
+constexpr int kCfLd = kCfK + 1;       // padded LDS row (floats): conflict-free column reads
+constexpr int kCfThreads = 256;
+
+// One stage of the A and B tiles (128 rows x 16 K each) as 2 + 2 float4 per thread, zero
+// beyond the segments, the comb entries and seg_len. kVec: rows start 16-byte aligned.
+template <bool kVec>
+__device__ inline void cf_load(const float* __restrict__ xe, const float* __restrict__ xv,
+                               const float* __restrict__ basis, int64_t nseg, int64_t seg_len,
+                               int64_t n_comb, int64_t s0, int64_t i0, int64_t k0, int tid,
+                               float4 ra[2], float4 rb[2]) {
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int idx = tid + kCfThreads * j;  // 0..511
+    const int r = idx >> 2, q = idx & 3;   // tile row 0..127, K quad 0..3
+    const int64_t k = k0 + 4 * q;
+    const int64_t sg = s0 + (r & 63);
+    const int64_t ci = i0 + (r & 63);
+    const float* a = (r < 64 ? xe : xv) + sg * seg_len + k;
+    const float* b = basis + ((r < 64 ? 0 : n_comb) + ci) * seg_len + k;
+    const bool av = sg < nseg, bv = ci < n_comb;
+    if (kVec) {
+      const bool kin = k < seg_len;
+      ra[j] = av && kin ? *reinterpret_cast<const float4*>(a) : make_float4(0.f, 0.f, 0.f, 0.f);
+      rb[j] = bv && kin ? *reinterpret_cast<const float4*>(b) : make_float4(0.f, 0.f, 0.f, 0.f);
+    } else {
+      float va[4], vb[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const bool kin = k + c < seg_len;
+        va[c] = av && kin ? a[c] : 0.0f;
+        vb[c] = bv && kin ? b[c] : 0.0f;
+      }
+      ra[j] = make_float4(va[0], va[1], va[2], va[3]);
+      rb[j] = make_float4(vb[0], vb[1], vb[2], vb[3]);
+    }
+  }
+}
+
+// fold[s][f][c][m] = (sum_t x_c[s][t] cos_t, sum_t x_c[s][t] sin_t) * corr for the block's
+// 64 segments x 64 comb entries; wavefront w takes component w & 1 and comb half w >> 1,
+// i.e. two 32 x 32 tiles of segments times the cos and the sin tile of 32 comb entries.
+template <bool kVec>
+__global__ void __launch_bounds__(kCfThreads)
+cfold_mfma_kernel(const float* __restrict__ xe, const float* __restrict__ xv,
+                  const float* __restrict__ basis, const float2* __restrict__ corr,
+                  int64_t nseg, int64_t seg_len, int64_t n_comb, int nb_f, int m_tiles,
+                  int nblk, float2* __restrict__ fold) {
+  __shared__ float As[2 * kCfSeg * kCfLd];
+  __shared__ float Bs[2 * kCfComb * kCfLd];
+  // The grid is padded to a multiple of 8 and block p runs on XCD p % 8: logical blocks
+  // x * per .. x * per + per - 1 go to XCD x, so the blocks that share a basis tile (M
+  // fastest) share that XCD's L2.
+  const int per = (int)(gridDim.x / 8);
+  const int L = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+  if (L >= nblk) return;
+  const int64_t s0 = (int64_t)(L % m_tiles) * kCfSeg;
+  const int64_t i0 = (int64_t)(L / m_tiles) * kCfComb;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int row = lane & 31, half = lane >> 5;
+  cf_f32x16 acc_c[2], acc_s[2];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    acc_c[0][i] = 0.0f;
+    acc_c[1][i] = 0.0f;
+    acc_s[0][i] = 0.0f;
+    acc_s[1][i] = 0.0f;
+  }
+  float4 ra[2], rb[2];
+  cf_load<kVec>(xe, xv, basis, nseg, seg_len, n_comb, s0, i0, 0, tid, ra, rb);
+  const float* a0p = As + (wm * 64 + row) * kCfLd + half;
+  const float* a1p = a0p + 32 * kCfLd;
+  const float* bcp = Bs + (wn * 32 + row) * kCfLd + half;
+  const float* bsp = bcp + 64 * kCfLd;
+  for (int64_t k0 = 0; k0 < seg_len; k0 += kCfK) {
+    __syncthreads();  // the previous stage's reads are done
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int idx = tid + kCfThreads * j;
+      const int r = idx >> 2, q = idx & 3;
+      float* da = As + r * kCfLd + 4 * q;
+      float* db = Bs + r * kCfLd + 4 * q;
+      da[0] = ra[j].x; da[1] = ra[j].y; da[2] = ra[j].z; da[3] = ra[j].w;
+      db[0] = rb[j].x; db[1] = rb[j].y; db[2] = rb[j].z; db[3] = rb[j].w;
+    }
+    __syncthreads();
+    if (k0 + kCfK < seg_len)  // the next stage's loads overlap this stage's products
+      cf_load<kVec>(xe, xv, basis, nseg, seg_len, n_comb, s0, i0, k0 + kCfK, tid, ra, rb);
+#pragma unroll
+    for (int kk = 0; kk < kCfK; kk += 2) {
+      const float a0 = a0p[kk], a1 = a1p[kk], bc = bcp[kk], bs = bsp[kk];
+      acc_c[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, bc, acc_c[0], 0, 0, 0);
+      acc_s[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, bs, acc_s[0], 0, 0, 0);
+      acc_c[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, bc, acc_c[1], 0, 0, 0);
+      acc_s[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, bs, acc_s[1], 0, 0, 0);
+    }
+  }
+  // acc[i] of lane l: segment row 8 (i / 4) + 4 half + i % 4 of the tile, comb column l & 31.
+  const int64_t ci = i0 + wn * 32 + row;
+  if (ci >= n_comb) return;
+  const float2 cr = corr[ci];
+  const int64_t nfreq = n_comb / nb_f;
+  const int64_t f = ci / nb_f;
+  const int m = (int)(ci - f * nb_f);
+#pragma unroll
+  for (int mi = 0; mi < 2; ++mi) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int64_t sg = s0 + mi * 32 + 8 * (i / 4) + 4 * half + (i % 4);
+      if (sg < nseg) {
+        const float2 x = make_float2(acc_c[mi][i], acc_s[mi][i]);
+        fold[((sg * nfreq + f) * 2 + wm) * nb_f + m] = cmul(x, cr);
+      }
+    }
+  }
 }
 
 // common.py:344-372 with the rows of one level (segment pair q, set j).
@@ -153,7 +259,6 @@ extern "C" int pyl_cfold_plan_destroy(pyl_cfold_plan* plan) {
   if (!plan) return PYL_OK;
   if (plan->basis) (void)hipFree(plan->basis);
   if (plan->corr) (void)hipFree(plan->corr);
-  if (plan->blas) (void)rocblas_destroy_handle(plan->blas);
   delete plan;
   return PYL_OK;
 }
@@ -215,10 +320,6 @@ extern "C" int pyl_cfold_plan_create(const double* freqs, int64_t nfreq, int64_t
   if ((e = hipStreamSynchronize(s)) != hipSuccess) return fail_hip(e, "hipStreamSynchronize");
   (void)hipFree(d_target);
   d_target = nullptr;
-  if (rocblas_create_handle(&p->blas) != rocblas_status_success) {
-    (void)pyl_cfold_plan_destroy(p);
-    return pyl::fail(PYL_ERR_HIP, "rocblas_create_handle failed");
-  }
   *plan_out = p;
   return PYL_OK;
 }
@@ -231,39 +332,25 @@ extern "C" int pyl_cfold_execute(pyl_cfold_plan* p, const float* ts_e, const flo
               "the complex fold needs nsamps to be a multiple of segment_len");
   hipStream_t s = pyl::as_stream(stream);
   const int64_t nseg = nsamps / p->seg_len;
-  const int64_t ncols = 2 * p->n_comb;
-  float* ce = nullptr;
-  PYL_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&ce),
-                               sizeof(float) * 2 * nseg * ncols, s));
-  float* cv = ce + nseg * ncols;
-  int rc = PYL_OK;
-  if (rocblas_set_stream(p->blas, s) != rocblas_status_success) {
-    rc = pyl::fail(PYL_ERR_HIP, "rocblas_set_stream failed");
+  const int64_t m_tiles = pyl::ceil_div(nseg, kCfSeg);
+  const int64_t n_tiles = pyl::ceil_div(p->n_comb, kCfComb);
+  PYL_REQUIRE(m_tiles * n_tiles < ((int64_t)1 << 31) - 8, "complex fold grid too large");
+  const int nblk = (int)(m_tiles * n_tiles);
+  const unsigned grid = (unsigned)(pyl::ceil_div(nblk, 8) * 8);
+  const bool vec = p->seg_len % 4 == 0 && reinterpret_cast<uintptr_t>(ts_e) % 16 == 0 &&
+                   reinterpret_cast<uintptr_t>(ts_v) % 16 == 0;
+  auto* out = reinterpret_cast<float2*>(fold_out);
+  if (vec) {
+    cfold_mfma_kernel<true><<<grid, kCfThreads, 0, s>>>(ts_e, ts_v, p->basis, p->corr, nseg,
+                                                        p->seg_len, p->n_comb, p->nb_f,
+                                                        (int)m_tiles, nblk, out);
   } else {
-    // Row-major C[nseg][2 n_comb] = X[nseg][seg_len] * W[2 n_comb][seg_len]^T, i.e.
-    // column-major C^T = W^T' * X^T with W^T' the transpose of the stored W.
-    const float one = 1.0f, zero = 0.0f;
-    for (int c = 0; c < 2 && rc == PYL_OK; ++c) {
-      const float* x = c == 0 ? ts_e : ts_v;
-      float* out = c == 0 ? ce : cv;
-      const rocblas_status st = rocblas_sgemm(
-          p->blas, rocblas_operation_transpose, rocblas_operation_none, (rocblas_int)ncols,
-          (rocblas_int)nseg, (rocblas_int)p->seg_len, &one, p->basis, (rocblas_int)p->seg_len,
-          x, (rocblas_int)p->seg_len, &zero, out, (rocblas_int)ncols);
-      if (st != rocblas_status_success)
-        rc = pyl::fail(PYL_ERR_HIP, "rocblas_sgemm failed (status %d)", (int)st);
-    }
+    cfold_mfma_kernel<false><<<grid, kCfThreads, 0, s>>>(ts_e, ts_v, p->basis, p->corr, nseg,
+                                                         p->seg_len, p->n_comb, p->nb_f,
+                                                         (int)m_tiles, nblk, out);
   }
-  if (rc == PYL_OK) {
-    const int64_t total = nseg * 2 * p->n_comb;
-    cfold_finish_kernel<<<(unsigned)pyl::ceil_div(total, 256), 256, 0, s>>>(
-        ce, cv, p->corr, nseg, p->n_comb, p->nb_f, reinterpret_cast<float2*>(fold_out));
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess)
-      rc = pyl::fail(PYL_ERR_HIP, "cfold_finish_kernel launch failed: %s", hipGetErrorString(e));
-  }
-  PYL_CHECK_HIP(hipFreeAsync(ce, s));
-  return rc;
+  PYL_CHECK_LAUNCH("cfold_mfma_kernel");
+  return PYL_OK;
 }
 
 template <int NBF>

@@ -106,7 +106,7 @@ class ForestSearch:
         mask = np.zeros(self.nlast + 1, dtype=np.uint8)
         for level in range(1, self.nlast + 1):
             st = pr.loop_levels.get(level)
-            mask[level] = st is None or st[0] != 0 or st[1] > hook.budget // 4
+            mask[level] = st is None or st[0] != 0 or st[1] > hook.budget_at(level) // 4
         hook.level_mask = mask
 
     def _alloc(self) -> None:

@@ -79,7 +79,7 @@ class CapturedSearch:
         mask = np.zeros(nlast + 1, dtype=np.uint8)
         for level in range(1, nlast + 1):
             st = self.pr.loop_levels.get(level)
-            mask[level] = st is None or st[0] != 0 or st[1] > hook.budget // 4
+            mask[level] = st is None or st[0] != 0 or st[1] > hook.budget_at(level) // 4
         hook.level_mask = mask
 
     def _buffer_ptrs(self):
@@ -95,6 +95,8 @@ class CapturedSearch:
                                                      "bt_tmp", "sel", "stats", "shist",
                                                      "part")]
             key += [int(s.budget), int(s.kind), int(s.precision), int(s.nlevels)]
+            sched = getattr(hook, "schedule", None)
+            key.append(None if sched is None else bytes(sched))
             mask = getattr(hook, "level_mask", None)
             key.append(None if mask is None
                        else bytes(np.ascontiguousarray(mask, dtype=np.uint8)))

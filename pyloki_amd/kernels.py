@@ -110,7 +110,8 @@ class BruteFoldPlan:
 
 class ComplexFoldPlan:
     """Device plan of the complex (Fourier-domain) leaf fold (``fold.py:395-481``): the FP32
-    basis of the harmonic comb of ``freqs``, built once; ``fold`` is one GEMM per component.
+    basis of the harmonic comb of ``freqs``, built once; ``fold`` is one launch of the
+    library's FP32 matrix-core product of the segments and the basis (both components).
     """
 
     def __init__(self, freqs: np.ndarray, seg_len: int, nbins: int, tsamp: float,

@@ -184,6 +184,33 @@ def test_budgeted_search_pipeline(gpu_device):
     assert c["recall"] >= 0.8
 
 
+def test_network_gain_over_the_snr_budget(gpu_device):
+    """What the trained network adds, on held-out series of the benchmark's circular-orbit
+    configuration (2^24 samples, 128 segments): trained on the exhaustive runs of seeds
+    1-4, then 8 other series are searched as one forest under the same survivor budget
+    keyed by S/N alone and by the hybrid key (the better of the network's and the S/N's
+    rank). The hybrid budget recovers at least as many of the exhaustive candidates while
+    scoring fewer leaves."""
+    import bench
+
+    _case("network gain over the S/N budget (held-out series)")
+    w = bench.EP_CNN
+    thr = np.linspace(w["thresholds"][0], w["thresholds"][1], w["nseg"] - 1)
+    ranker, info = cnn.train_ranker(bench.ep_circular_dyp, w["train_seeds"], thr,
+                                    w["max_sugg"], epochs=w["epochs"])
+    series = [bench.ep_circular_dyp(s) for s in range(9, 17)]
+    b = w["survey_budget"]
+    res = cnn.forest_throughput(series, thr, w["max_sugg"], {
+        "exhaustive": None,
+        "snr": lambda: cnn.SurvivorBudget(b),
+        "hybrid": lambda: cnn.SurvivorBudget(b, ranker, hybrid=True)})
+    s, h = res["snr"], res["hybrid"]
+    print(f"[result] budget {b}: S/N recall {s['recall']:.4f} leaves {s['trials']}; "
+          f"hybrid recall {h['recall']:.4f} leaves {h['trials']}; train {info}", flush=True)
+    assert h["recall"] >= s["recall"]
+    assert h["trials"] <= s["trials"]
+
+
 @pytest.mark.parametrize("B,cplx", [(32, False), (64, False), (128, False), (64, True)])
 def test_cnn_forward_bf16_ranks_like_fp32(gpu_device, B, cplx):
     """The MFMA (BF16 operands) inference pass against the FP64 oracle: logits within

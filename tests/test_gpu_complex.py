@@ -46,11 +46,14 @@ def _series(cfg: PulsarSearchConfig, seed: int = 2):
                              ducy=0.08, mod_kwargs=mod).generate_analytic(seed=seed)
 
 
-@pytest.mark.parametrize("nbins,seg_len,nfreq", [(64, 4096, 37), (32, 2048, 9), (128, 1024, 5)])
-def test_complex_leaf_fold(gpu_device, nbins, seg_len, nfreq):
-    _case(f"complex leaf fold nbins={nbins} seg_len={seg_len} nfreq={nfreq}")
-    rng = np.random.default_rng(nbins)
-    nseg = 6
+@pytest.mark.parametrize("nbins,seg_len,nfreq,nseg", [
+    (64, 4096, 37, 6), (32, 2048, 9, 6), (128, 1024, 5, 6),
+    # several segment and comb tiles with tails; a K tail (1000 = 62.5 stages of 16); rows
+    # that are not 16-byte aligned (1002 samples: the scalar-load path)
+    (64, 1000, 11, 70), (32, 1002, 3, 9), (64, 2048, 40, 130)])
+def test_complex_leaf_fold(gpu_device, nbins, seg_len, nfreq, nseg):
+    _case(f"complex leaf fold nbins={nbins} seg_len={seg_len} nfreq={nfreq} nseg={nseg}")
+    rng = np.random.default_rng(nbins + nseg)
     ts_e = rng.normal(size=nseg * seg_len).astype(np.float32)
     ts_v = rng.uniform(0.5, 1.5, nseg * seg_len).astype(np.float32)
     freqs = np.linspace(150.0, 210.0, nfreq)
