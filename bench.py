@@ -61,6 +61,8 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--nseries", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="time eager FFA steps instead of replays of the captured step")
     ap.add_argument("--tile-rows", type=int, default=None,
                     help="outer rows per 256-row tile of the fused FFA kernels; default: "
                          "chosen by the autotuner (pyloki_amd/autotune.py) on this device")
@@ -524,19 +526,30 @@ def main() -> int:
     if args.tile_rows is not None:
         autotune.set_tile_rows(args.tile_rows)
         tuned = {"rows": args.tile_rows, "ms": None, "cached": False, "source": "--tile-rows"}
-    else:  # time every candidate tile on this workload, keep the fastest
+    else:  # time every candidate tile and brute-fold shape on this workload
         sig = (f"ffa_batch/B{nbins}/nw{nw}/lens{sorted(set(lens))}x{len(lens)}/"
                f"grid{[int(c) for c in cfg_max.get_param_grid_count(cfg_max.niters_ffa)]}")
-        tuned = autotune.tune_tile_rows(sig, lambda: ffa_batch(plan, dev_series, widths))
-        tuned["source"] = "autotune"
+        both = autotune.tune_all(sig, lambda: ffa_batch(plan, dev_series, widths))
+        t = both["ffa_tile_rows"]
+        tuned = {"rows": t["value"], "ms": t["ms"], "cached": t["cached"],
+                 "source": "autotune", "bf_segments_per_block": both["bf_segments_per_block"]}
     for _ in range(args.warmup):
         ffa_batch(plan, dev_series, widths)
     torch.cuda.synchronize()
+    # The step captured once as a HIP graph (brute fold, every level, the scoring); the
+    # timed steps replay it. --no-graph times eager steps instead.
+    step = (lambda: ffa_batch(plan, dev_series, widths))
+    if not args.no_graph:
+        from pyloki_amd.graph import CapturedFFABatch
 
+        cap = CapturedFFABatch(plan, dev_series, widths)
+        step = cap.replay
+        for _ in range(args.warmup):
+            step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        res = ffa_batch(plan, dev_series, widths)
+        res = step()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
 
@@ -623,9 +636,11 @@ def main() -> int:
             "trials_per_step": trials,
             "final_grid_2^23": [int(c) for c in cfg_max.get_param_grid_count(cfg_max.niters_ffa)],
             "parallelism": "1 GPU; series of one length in one launch per level pair",
+            "timed": "eager steps" if args.no_graph else "replays of the step captured as a HIP graph",
             "tile_outer_rows": tuned["rows"],
             "tile_source": tuned["source"],
             "autotune_ms": tuned["ms"],
+            "bf_segments_per_block": tuned.get("bf_segments_per_block"),
         },
         "roofline": roof,
         "boundary_evals": plan.boundary_counts(),

@@ -1,13 +1,19 @@
 """Launch shapes chosen by measurement on the device, cached per device and workload.
 
-The tunable shape today is the tile of the round-based fused FFA kernels (the last
-level, or the last two levels, fused with scoring): a block owns 256 output rows, ``rows``
-along the outer grid axis (e.g. accelerations) by ``256 / rows`` along the inner one
-(frequencies). Taller tiles reuse each source row across more outer indices from L2;
-wider ones keep more frequency neighbours together. Which wins depends on the grid and on
-the device, so ``tune`` times every candidate on the real workload (HIP events around a
-full step, the median of a few repetitions) and keeps the fastest. The tile never
-changes a result, only speed.
+Tunable shapes (``KNOBS``), none of which changes a result, only speed:
+
+- ``ffa_tile_rows``: the tile of the round-based fused FFA kernels (the last level, or
+  the last two levels, fused with scoring): a block owns 256 output rows, ``rows`` along
+  the outer grid axis (e.g. accelerations) by ``256 / rows`` along the inner one
+  (frequencies). Taller tiles reuse each source row across more outer indices from L2;
+  wider ones keep more frequency neighbours together.
+- ``bf_segments_per_block``: how many segments a brute-fold block stages in LDS (0: as
+  many as fit a 64 KB budget; 1, 2 or 4 up to the 160 KB a gfx950 workgroup holds). More
+  segments amortise each block's bin lists over more samples; fewer give more blocks.
+
+Which wins depends on the grid and on the device, so ``tune`` times every candidate on
+the real workload (HIP events around a full step, the median of a few repetitions) and
+keeps the fastest; ``tune_all`` tunes the knobs one after the other.
 
 Cache: a JSON file (``$PYLOKI_AMD_TUNE_CACHE``, default
 ``~/.cache/pyloki_amd/autotune.json``) keyed by device name and a caller-given workload
@@ -64,6 +70,17 @@ def get_tile_rows() -> int:
     return int(_lib.load().pyl_ffa_get_tile_rows())
 
 
+def set_bf_segments(segs: int) -> None:
+    _lib.check(_lib.load().pyl_brutefold_set_segments_per_block(int(segs)))
+
+
+# knob -> (setter, candidates, default)
+KNOBS = {
+    "ffa_tile_rows": (set_tile_rows, TILE_ROWS, 16),
+    "bf_segments_per_block": (set_bf_segments, (0, 1, 2, 4), 0),
+}
+
+
 def _time(step, reps: int, warmup: int) -> float:
     for _ in range(warmup):
         step()
@@ -79,34 +96,50 @@ def _time(step, reps: int, warmup: int) -> float:
     return float(np.median(ms))
 
 
-def tune_tile_rows(signature: str, step, candidates=TILE_ROWS, reps: int = 3,
-                   warmup: int = 1, use_cache: bool = True) -> dict:
+def tune(knob: str, signature: str, step, candidates=None, reps: int = 3, warmup: int = 1,
+         use_cache: bool = True) -> dict:
     """Time ``step()`` (one full unit of the workload on the current stream) with every
-    candidate tile, set the fastest and cache it under ``signature``. Returns
-    {"rows": best, "ms": {rows: median ms}, "cached": bool}."""
+    candidate value of ``knob``, set the fastest and cache it under ``signature``. Returns
+    {"value": best, "ms": {value: median ms}, "cached": bool}."""
+    setter, default_cands, _ = KNOBS[knob]
+    candidates = default_cands if candidates is None else candidates
     db = _load()
-    key = f"ffa_tile_rows/{signature}"
+    key = f"{knob}/{signature}"
     dev = _device()
     hit = db.get(dev, {}).get(key)
     if use_cache and hit is not None:
-        set_tile_rows(hit["rows"])
+        setter(hit["value"])
         return {**hit, "cached": True}
     timings = {}
-    for rows in candidates:
-        set_tile_rows(rows)
-        timings[int(rows)] = _time(step, reps, warmup)
+    for v in candidates:
+        setter(v)
+        timings[int(v)] = _time(step, reps, warmup)
     best = min(timings, key=timings.get)
-    set_tile_rows(best)
-    entry = {"rows": int(best), "ms": {str(k): v for k, v in timings.items()}}
+    setter(best)
+    entry = {"value": int(best), "ms": {str(k): v for k, v in timings.items()}}
     db.setdefault(dev, {})[key] = entry
     _store(db)
     return {**entry, "cached": False}
 
 
-def apply(signature: str) -> int | None:
-    """Set the cached tile for ``signature`` on this device, if any; returns it."""
-    hit = _load().get(_device(), {}).get(f"ffa_tile_rows/{signature}")
+def tune_all(signature: str, step, knobs=("ffa_tile_rows", "bf_segments_per_block"),
+             **kw) -> dict:
+    """``tune`` every knob in turn (each with the earlier knobs at their best)."""
+    return {k: tune(k, signature, step, **kw) for k in knobs}
+
+
+def tune_tile_rows(signature: str, step, candidates=TILE_ROWS, reps: int = 3,
+                   warmup: int = 1, use_cache: bool = True) -> dict:
+    """``tune`` of the fused FFA tile: {"rows": best, "ms": ..., "cached": ...}."""
+    r = tune("ffa_tile_rows", signature, step, candidates, reps, warmup, use_cache)
+    return {"rows": r["value"], "ms": r["ms"], "cached": r["cached"]}
+
+
+def apply(signature: str, knob: str = "ffa_tile_rows") -> int | None:
+    """Set the cached value of ``knob`` for ``signature`` on this device, if any."""
+    hit = _load().get(_device(), {}).get(f"{knob}/{signature}")
     if hit is None:
         return None
-    set_tile_rows(hit["rows"])
-    return int(hit["rows"])
+    value = hit["value"] if "value" in hit else hit["rows"]
+    KNOBS[knob][0](value)
+    return int(value)

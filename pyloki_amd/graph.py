@@ -207,3 +207,49 @@ class SearchBatch:
         for st in self.streams:
             torch.cuda.default_stream().wait_stream(st)
         return out
+
+
+class CapturedFFABatch:
+    """One step of the batched FFA search (``ffa.ffa_batch``: the brute fold of every
+    series, every level and the scoring) captured once as a HIP graph.
+
+    The series' samples live in static device buffers (one pair per series, the lengths
+    fixed at capture); ``run(series)`` copies new samples of the same lengths into them
+    (or ``replay()`` reuses the resident ones), replays the graph and returns the per-width
+    S/N grids, which are the graph's own output buffers (overwritten by the next replay).
+    The step's launches carry every constant by value (the brute-fold series table, the
+    level tables' addresses, the tile), allocate nothing outside the capture's memory pool
+    and record no memset node, so a replay equals an eager ``ffa_batch`` bit for bit.
+    """
+
+    def __init__(self, plan, series: list[tuple[torch.Tensor, torch.Tensor]],
+                 widths: np.ndarray, stdnoise: float = 1.0) -> None:
+        from pyloki_amd.ffa import ffa_batch
+
+        self.plan = plan
+        self.widths = np.asarray(widths)
+        self.stdnoise = float(stdnoise)
+        self.inputs = [(e.clone(), v.clone()) for e, v in series]
+        self.lengths = [int(e.shape[0]) for e, _ in series]
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):  # the warm-up fills torch's allocator before capture
+            ffa_batch(plan, self.inputs, self.widths, self.stdnoise)
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.outputs = ffa_batch(plan, self.inputs, self.widths, self.stdnoise)
+
+    def replay(self) -> list[torch.Tensor]:
+        self.graph.replay()
+        return self.outputs
+
+    def run(self, series: list[tuple[torch.Tensor, torch.Tensor]]) -> list[torch.Tensor]:
+        if [int(e.shape[0]) for e, _ in series] != self.lengths:
+            msg = "the series must have the lengths the graph was captured with"
+            raise ValueError(msg)
+        for (de, dv), (e, v) in zip(self.inputs, series):
+            de.copy_(e)
+            dv.copy_(v)
+        return self.replay()

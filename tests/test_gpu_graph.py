@@ -99,3 +99,37 @@ def test_search_batch_equals_eager(gpu_device):
             pr = Pruning(d, thresholds, max_sugg=2**20, batch_size=64)
             pr.level_hook = factory() if factory else None
             _same(got[i], pr.execute(3), f"series {i}")
+
+
+def test_captured_ffa_batch_replays(gpu_device):
+    """The batched FFA step (``graph.CapturedFFABatch``: brute fold, every level two at a
+    time, the fused last levels and scoring) captured once and replayed: every replay's
+    S/N grids equal an eager ``ffa_batch`` of the same series bit for bit, with eager work
+    and synchronisations between the replays and new samples copied in between."""
+    import torch
+
+    import bench
+    from pyloki_amd.ffa import FFAPlan, ffa_batch
+    from pyloki_amd.graph import CapturedFFABatch
+
+    _case("captured FFA batch (3 series of 2^18..2^20 samples)")
+    lens = [2**20, 2**18, 2**19]
+    cfg = bench.make_config(max(lens))
+    plan = FFAPlan(cfg)
+    widths = cfg.score_widths
+
+    def series(seed):
+        r = np.random.default_rng(seed)
+        return [(torch.from_numpy(r.standard_normal(n).astype(np.float32)).cuda(),
+                 torch.ones(n, dtype=torch.float32, device="cuda")) for n in lens]
+
+    base = series(1)
+    cap = CapturedFFABatch(plan, base, widths)
+    for k, seed in enumerate([1, 1, 2, 3, 2]):
+        data = series(seed)
+        exp = [x.cpu().numpy() for x in ffa_batch(plan, data, widths)]  # eager between
+        torch.cuda.synchronize()
+        got = [x.cpu().numpy() for x in cap.run(data)]
+        for i in range(len(lens)):
+            np.testing.assert_array_equal(got[i], exp[i], err_msg=f"replay {k} series {i}")
+    print("[result] 5 replays bit-equal to eager ffa_batch", flush=True)
