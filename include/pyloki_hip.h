@@ -981,6 +981,36 @@ int pyl_sim_combine(const float* noise_dev, const float* tmpl_dev, int64_t nsamp
                     float* ts_v_dev, void* stream);
 
 /* ------------------------------------------------------------------------------------
+ * Time-series conditioning.
+ * Replaces the sigpyproc calls of io/timeseries.py:197-235 TimeSeries.from_tim:
+ * deredden(method="median", window, fast=True) and normalise(loc_method="mean",
+ * scale_method) (sigpyproc is an unpinned git dependency of the reference, absent here).
+ *
+ * pyl_ts_quantiles: for each segment s of seg_len consecutive samples (the last may be
+ *   shorter), out_dev[s][t] = numpy's "linear" quantile q[t] (nq = 1 or 2) of the
+ *   segment's values, or of |x - center_dev[s]| when center_dev is not NULL: the order
+ *   statistics at floor and ceil of q (m - 1), found exactly by radix select, interpolated
+ *   in FP64. Deterministic.
+ * pyl_ts_deredden_median: medians_dev [ceil(n / window)] = the medians of consecutive
+ *   blocks of `window` samples; out_dev = x minus those medians interpolated linearly
+ *   between the block centres (held constant beyond the first and last centre).
+ * pyl_ts_normalise: out_dev = (x - mean) / scale in FP64 stored as FP32, scale =
+ *   PYL_TS_SCALE_STD the standard deviation, PYL_TS_SCALE_IQR the interquartile range /
+ *   1.3489795003921634, PYL_TS_SCALE_MAD the median absolute deviation x
+ *   1.482602218505602; loc_scale (host, 2 doubles) receives (mean, scale). Synchronises
+ *   the stream (the statistics come back to the host). A zero scale is a ValueError.
+ * ------------------------------------------------------------------------------------ */
+#define PYL_TS_SCALE_STD 0
+#define PYL_TS_SCALE_IQR 1
+#define PYL_TS_SCALE_MAD 2
+int pyl_ts_quantiles(const float* x_dev, int64_t n, int64_t seg_len, const double* q,
+                     int32_t nq, const float* center_dev, float* out_dev, void* stream);
+int pyl_ts_deredden_median(const float* x_dev, int64_t n, int64_t window, float* medians_dev,
+                           float* out_dev, void* stream);
+int pyl_ts_normalise(const float* x_dev, int64_t n, int32_t scale_method, float* out_dev,
+                     double* loc_scale, void* stream);
+
+/* ------------------------------------------------------------------------------------
  * Bandwidth probes of known size (no reference counterpart; measurement only):
  * pyl_hbm_read loads n_float4 16-byte elements once each (one float per block written to
  * partial_dev [nblocks]); pyl_hbm_copy copies n_float4 elements. Used to measure the
