@@ -981,6 +981,22 @@ int pyl_sim_combine(const float* noise_dev, const float* tmpl_dev, int64_t nsamp
                     float* ts_v_dev, void* stream);
 
 /* ------------------------------------------------------------------------------------
+ * Branching patterns on the device.
+ * Replaces the per-frequency loops of core/taylor.py:414-499 generate_bp_poly_taylor
+ * (kind 0) and core/circular.py:905-1010 generate_bp_circ_taylor (kind 1, n_params 5):
+ * nbr_out (host, f64 [nlev][nf]) = each final-grid frequency's branch count at every
+ * pruning level, from its cell widths [d_k .. d_1] (dinit, the last one scaled by C/f0).
+ * Per level l (host, FP64): dpf [nlev][n_params] the target step before the C/f0 scaling,
+ * fac [nlev][n_params] the bin shift per unit width (span powers / factorials, /2^k with
+ * Chebyshev coarsening), tmat [nlev][n_params + 1][n_params + 1] the Taylor shift matrix to
+ * the next level's epoch (moving grid; tiling 0 aggressive, 1 conservative, 2
+ * quadrature). The caller forms the weighted averages. Synchronises the stream.
+ * ------------------------------------------------------------------------------------ */
+int pyl_bp_counts(int32_t kind, int32_t n_params, int32_t nlev, int64_t nf, const double* f0,
+                  const double* dinit, const double* dpf, const double* fac, const double* tmat,
+                  int32_t moving, int32_t tiling, double eta, double* nbr_out, void* stream);
+
+/* ------------------------------------------------------------------------------------
  * Time-series conditioning.
  * Replaces the sigpyproc calls of io/timeseries.py:197-235 TimeSeries.from_tim:
  * deredden(method="median", window, fast=True) and normalise(loc_method="mean",

@@ -91,6 +91,9 @@ SIGNATURES: list[tuple[str, object, list[object]]] = [
     ("pyl_rng_normal", ctypes.c_int,
      [ctypes.c_uint64, ctypes.c_uint64, _c_i64, _c_i32, _c_dbl, _vp, _vp]),
     ("pyl_rows_gather", ctypes.c_int, [_vp, _vp, _c_i64, _c_i32, _vp, _vp]),
+    ("pyl_bp_counts", ctypes.c_int,
+     [_c_i32, _c_i32, _c_i32, _c_i64, _f64p, _f64p, _f64p, _f64p, _f64p, _c_i32, _c_i32, _c_dbl,
+      _f64p, _vp]),
     ("pyl_ts_quantiles", ctypes.c_int, [_vp, _c_i64, _c_i64, _f64p, _c_i32, _vp, _vp, _vp]),
     ("pyl_ts_deredden_median", ctypes.c_int, [_vp, _c_i64, _c_i64, _vp, _vp, _vp]),
     ("pyl_ts_normalise", ctypes.c_int, [_vp, _c_i64, _c_i32, _vp, _f64p, _vp]),

@@ -311,6 +311,8 @@ class PulsarSearchConfig:
         kind: str = "poly_taylor_moving",
         ref_seg: int = 0,
         use_cheby_coarsening: bool = True,
+        *,
+        device: bool = False,
     ) -> np.ndarray:
         """Expected children per parent at every pruning level (``config.py:758-819``).
 
@@ -318,7 +320,9 @@ class PulsarSearchConfig:
         Chebyshev kinds ``generate_bp_poly_chebyshev`` (``core/chebyshev.py:458-546``) and
         circular kinds ``generate_bp_circ_taylor`` (``core/circular.py:905-1010``): per
         final-grid frequency, the children per parent of each level, averaged with weights
-        equal to the running product of earlier branchings.
+        equal to the running product of earlier branchings. ``device=True`` runs the
+        per-frequency recurrence of the Taylor and circular kinds on the GPU
+        (``pyl_bp_counts``; the same values bit for bit).
         """
         funcs = {"poly_taylor_moving": (generate_bp_poly_taylor, True),
                  "poly_taylor_fixed": (generate_bp_poly_taylor, False),
@@ -336,6 +340,15 @@ class PulsarSearchConfig:
         dparams = self.get_dparams(self.niters_ffa, use_cheby_coarsening)
         param_arr = self.get_param_arr(dparams)
         dparams_act = self.get_dparams_actual(self.niters_ffa, use_cheby_coarsening)
+        if device:
+            if kind.startswith("poly_chebyshev"):
+                msg = "the device pattern covers the Taylor and circular kinds"
+                raise ValueError(msg)
+            return generate_bp_device(
+                param_arr, dparams_act, self.tseg_ffa, nsegments, self.nbins, self.eta,
+                ref_seg, use_moving_grid=use_moving_grid, tiling_strategy=self.tiling_strategy,
+                use_cheby_coarsening=use_cheby_coarsening,
+                circular=kind.startswith("circ"))
         return func(
             param_arr, dparams_act, self.tseg_ffa, nsegments, self.nbins, self.eta, ref_seg,
             use_moving_grid=use_moving_grid, tiling_strategy=self.tiling_strategy,
@@ -487,6 +500,67 @@ def generate_bp_circ_taylor(
                                                   tiling_strategy)[:, :-1]
         else:
             dcur = dnext
+    return bp
+
+
+def generate_bp_device(param_arr: list[np.ndarray], dparams_act: np.ndarray, tseg_ffa: float,
+                       nsegments: int, nbins: int, eta: float, ref_seg: int, *,
+                       use_moving_grid: bool, tiling_strategy: str,
+                       use_cheby_coarsening: bool = True, circular: bool = False
+                       ) -> np.ndarray:
+    """``generate_bp_poly_taylor`` / ``generate_bp_circ_taylor`` with the per-frequency
+    width recurrence on the GPU (``pyl_bp_counts``): the host supplies each level's
+    constants (the step, the shift factor and the Taylor shift matrix of the moving grid,
+    from the same helpers as the host generators) and forms the weighted averages with
+    the same sums, so the pattern equals the host one bit for bit."""
+    import ctypes
+
+    from pyloki_amd import _lib
+    from pyloki_amd.device import require_gpu, stream_ptr
+    from pyloki_amd.utils.snail import MiddleOutScheme
+
+    require_gpu()
+    n_params = len(dparams_act)
+    f0 = np.ascontiguousarray(param_arr[-1], dtype=np.float64)
+    nf = len(f0)
+    nlev = nsegments - 1
+    scheme = MiddleOutScheme(nsegments, ref_seg, tseg_ffa)
+    dpf = np.empty((max(nlev, 1), n_params))
+    fac = np.empty((max(nlev, 1), n_params))
+    tm = np.zeros((max(nlev, 1), n_params + 1, n_params + 1))
+    k = np.arange(n_params - 1, -1, -1)
+    for level in range(1, nsegments):
+        coord_next = scheme.get_coord(level)
+        coord_cur = scheme.get_current_coord(level, use_moving_grid)
+        tobs = coord_cur[1]
+        dpf[level - 1] = psr_utils.poly_taylor_step_f(n_params, tobs, nbins, eta, 0.0,
+                                                      use_cheby_coarsening)
+        fc = np.array([psr_utils.powi(tobs, int(kk) + 1) * nbins / psr_utils.fact(int(kk) + 1)
+                       for kk in k])
+        if use_cheby_coarsening:
+            fc = fc / (2.0**k)
+        fac[level - 1] = fc
+        if use_moving_grid:
+            tm[level - 1] = transforms.taylor_shift_matrix(n_params + 1,
+                                                           coord_next[0] - coord_cur[0])
+    tiling = {"aggressive": 0, "conservative": 1, "quadrature": 2}.get(tiling_strategy)
+    if tiling is None:
+        msg = f"Invalid tiling strategy: {tiling_strategy}"
+        raise ValueError(msg)
+    nbr = np.empty((max(nlev, 1), nf))
+    dinit = np.ascontiguousarray(dparams_act, dtype=np.float64)
+    p64 = ctypes.POINTER(ctypes.c_double)
+    _lib.check(_lib.load().pyl_bp_counts(
+        1 if circular else 0, n_params, nlev, nf, f0.ctypes.data_as(p64),
+        dinit.ctypes.data_as(p64), np.ascontiguousarray(dpf).ctypes.data_as(p64),
+        np.ascontiguousarray(fac).ctypes.data_as(p64),
+        np.ascontiguousarray(tm).ctypes.data_as(p64), int(use_moving_grid), tiling, float(eta),
+        nbr.ctypes.data_as(p64), stream_ptr()))
+    weights = np.ones(nf)
+    bp = np.empty(nlev)
+    for level in range(nlev):
+        bp[level] = np.sum(weights * nbr[level]) / np.sum(weights)
+        weights = weights * nbr[level]
     return bp
 
 
