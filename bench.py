@@ -66,7 +66,8 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--tile-rows", type=int, default=None,
                     help="outer rows per 256-row tile of the fused FFA kernels; default: "
                          "chosen by the autotuner (pyloki_amd/autotune.py) on this device")
-    ap.add_argument("--workload", choices=["ffa", "ep-taylor-jerk", "ep-circular-cnn"],
+    ap.add_argument("--workload", choices=["ffa", "ep-taylor-jerk", "ep-circular-cnn",
+                                           "screen"],
                     default="ffa",
                     help="ffa: the headline FFA batch (default); ep-taylor-jerk: the EP "
                          "search of a jerk pulsar (trials/s); ep-circular-cnn: the "
@@ -292,6 +293,59 @@ def ep_circular_cnn() -> int:
     return 0
 
 
+# The boxcar threshold screen on the matrix cores against the exact scorer: 2^22 stored
+# profiles of 64 bins (e, v), 3 % with a pulse of random strength, the FFA workload's
+# widths (ducy_max 0.2, wtsp 1.5), threshold 6.
+def screen_bench(args) -> int:
+    import torch
+
+    from pyloki_amd import _lib, kernels
+    from pyloki_amd.detection.scoring import generate_box_width_trials
+    from pyloki_amd.simulation.pulse import generate_folded_profile
+
+    n, B, thr = 2**22, 64, 6.0
+    widths = generate_box_width_trials(B, ducy_max=0.2, wtsp=1.5)
+    rng = np.random.default_rng(1)
+    e = rng.standard_normal((n, B), dtype=np.float32) * 4.0
+    pulsed = np.nonzero(rng.random(n) < 0.03)[0]
+    prof = generate_folded_profile(B, 0.1).astype(np.float32)
+    e[pulsed] += (rng.uniform(0, 10, len(pulsed))[:, None] * 4.0 * prof[None, :]).astype(
+        np.float32)
+    f = torch.from_numpy(np.stack([e, np.full_like(e, 16.0)], axis=1)).cuda()
+    del e
+
+    def timed(fn, reps=10):
+        fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        b.synchronize()
+        return a.elapsed_time(b) / reps
+
+    ms_exact = timed(lambda: kernels.score_boxcar(f, widths, mode=_lib.PYL_SCORE_MAX))
+    ms_screen = timed(lambda: kernels.score_boxcar_screen(f, widths, thr, count=False))
+    exact = kernels.score_boxcar(f, widths, mode=_lib.PYL_SCORE_MAX).cpu().numpy()
+    out, nres = kernels.score_boxcar_screen(f, widths, thr)
+    out = out.cpu().numpy()
+    sel = exact >= thr
+    line = {"metric": "profiles scored against a threshold per second", "unit": "profiles/s",
+            "value": n / (ms_screen * 1e-3), "n_gpus": 1, "higher_is_better": True,
+            "vs_baseline": None, "dtype": "bf16 screen, f32 exact rescoring",
+            "data": "synthetic",
+            "config": {"workload": "boxcar threshold screen", "profiles": n, "nbins": B,
+                       "widths": [int(w) for w in widths], "threshold": thr},
+            "exact_profiles_per_s": n / (ms_exact * 1e-3), "ms_exact": ms_exact,
+            "ms_screen": ms_screen, "speedup_vs_exact": ms_exact / ms_screen,
+            "rescored_fraction": nres / n, "survivors": int(sel.sum()),
+            "survivors_equal": bool(np.array_equal(out >= thr, sel)
+                                    and np.array_equal(out[sel], exact[sel]))}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 # EP search of a jerk pulsar (Taylor order 3): N = 2^23 samples at 64 us, 64 FFA segments
 # (63 EP levels), bseg_brute 1024, nbins 64, eta 1, f in [495, 505] Hz, accel +-30 m/s^2,
 # jerk +-2 m/s^3; injected P = 2 ms pulsar (a = 5 m/s^2, j = 0.05 m/s^3, so that its
@@ -432,6 +486,11 @@ def main() -> int:
             print("bench.py measures one GPU only", file=sys.stderr)
             return 2
         return ep_circular_cnn()
+    if args.workload == "screen":
+        if args.gpus != 1:
+            print("bench.py measures one GPU only", file=sys.stderr)
+            return 2
+        return screen_bench(args)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the FFA batch "
               "would shard by series across GPUs with no communication.", file=sys.stderr)

@@ -981,6 +981,21 @@ int pyl_sim_combine(const float* noise_dev, const float* tmpl_dev, int64_t nsamp
                     float* ts_v_dev, void* stream);
 
 /* ------------------------------------------------------------------------------------
+ * Boxcar S/N threshold screen on the matrix cores.
+ * The boxcar scores of detection/scoring.py:99-165 (snr_score_batch_func :249-282, max
+ * over widths, stdnoise 1) as a product of profiles and 0/1 box templates (the templates
+ * of scoring.py:383-471), BF16 operands, FP32 accumulation, with exact rescoring:
+ * out[i] = the exact score of pyl_score_boxcar (PYL_SCORE_MAX) for every profile whose
+ * screened upper bound reaches `threshold` (so out >= threshold selects the same profiles
+ * with the same scores, bit for bit), an upper bound below `threshold` for the others.
+ * folds_dev: f32 [n][2][nbins] (e, v), nbins 32 or 64, 1..16 widths; nrescored_dev
+ * (nullable, zeroed by the caller) is increased by the number of profiles scored exactly.
+ * ------------------------------------------------------------------------------------ */
+int pyl_score_boxcar_screen(const float* folds_dev, int64_t n, int32_t nbins,
+                            const int64_t* widths, int32_t nw, double threshold, float* out_dev,
+                            unsigned long long* nrescored_dev, void* stream);
+
+/* ------------------------------------------------------------------------------------
  * Branching patterns on the device.
  * Replaces the per-frequency loops of core/taylor.py:414-499 generate_bp_poly_taylor
  * (kind 0) and core/circular.py:905-1010 generate_bp_circ_taylor (kind 1, n_params 5):

@@ -104,6 +104,8 @@ SIGNATURES: list[tuple[str, object, list[object]]] = [
      [_vp, _vp, _c_i64, ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp, _vp, _vp]),
     ("pyl_hbm_read", ctypes.c_int, [_vp, _c_i64, _vp, _c_i32, _vp]),
     ("pyl_hbm_copy", ctypes.c_int, [_vp, _vp, _c_i64, _c_i32, _vp]),
+    ("pyl_score_boxcar_screen", ctypes.c_int,
+     [_vp, _c_i64, _c_i32, _i64p, _c_i32, _c_dbl, _vp, _vp, _vp]),
     ("pyl_score_boxcar", ctypes.c_int,
      [_vp, _c_i64, _c_i32, _c_i32, _i64p, _c_i32, _c_dbl, _c_i32, _vp, _vp]),
     ("pyl_circ_branch", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),

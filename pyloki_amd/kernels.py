@@ -416,3 +416,24 @@ def score_boxcar(folds: torch.Tensor, widths: np.ndarray, stdnoise: float = 1.0,
         ptr(folds), n, nbins, layout, _lib.i64_array(widths), len(widths), float(stdnoise), mode,
         ptr(out), stream_ptr()))
     return out
+
+
+def score_boxcar_screen(folds: torch.Tensor, widths: np.ndarray, threshold: float,
+                        count: bool = True) -> tuple[torch.Tensor, int | None]:
+    """Threshold screen of EV profiles [n, 2, B] (B 32 or 64, stdnoise 1) on the matrix
+    cores (``pyl_score_boxcar_screen``): out[i] is the exact maximum S/N of
+    ``score_boxcar(..., mode=PYL_SCORE_MAX)`` for every profile that can reach
+    ``threshold`` (so ``out >= threshold`` selects exactly the same profiles with the same
+    scores), and an upper bound below ``threshold`` for the others. Returns (out, the
+    number of profiles scored exactly) (the count read back only with ``count``)."""
+    widths = np.ascontiguousarray(widths, dtype=np.int64)
+    if folds.dim() != 3 or folds.shape[1] != 2 or folds.dtype != torch.float32:
+        msg = "folds must be float32 [n, 2, nbins]"
+        raise ValueError(msg)
+    n, _, nbins = folds.shape
+    out = torch.empty(n, dtype=torch.float32, device=folds.device)
+    cnt = torch.zeros(1, dtype=torch.int64, device=folds.device) if count else None
+    _lib.check(_lib.load().pyl_score_boxcar_screen(
+        ptr(folds.contiguous()), n, nbins, _lib.i64_array(widths), len(widths),
+        float(threshold), ptr(out), ptr(cnt), stream_ptr()))
+    return out, (int(cnt.item()) if count else None)
