@@ -15,9 +15,11 @@
 // (sample, bin). The conv layers keep the block's activations in LDS (32 KB) and each
 // thread accumulates its 32 output channels in registers; weights are read with
 // block-uniform indices (scalar loads, no LDS). The backward kernel walks tiles of samples
-// grid-stride; every thread owns a fixed set of weight-gradient outputs, accumulated in
-// registers over the block's tiles, and writes them once as the block's partial sum; a
-// second kernel adds the partials in block order, so the gradient is deterministic.
+// grid-stride; the weight gradients of the two 32 -> 32 layers are products on the matrix
+// cores (FP32, wgrad_mfma), every wavefront owning five 16 x 16 output tiles accumulated
+// over the block's tiles; the other gradients are owned by fixed threads. Each block writes
+// its partial sum once; a second kernel adds the partials in block order, so the gradient
+// is deterministic.
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
@@ -198,22 +200,74 @@ cnn_forward_kernel(const float* __restrict__ th, const float* __restrict__ folds
   }
 }
 
+typedef float cnn_f32x4 __attribute__((ext_vector_type(4)));
+
+// Weight gradient of a 32 -> 32 conv layer on the matrix cores: dW[co][n] (n = ci kK + t)
+// += sum over k = (sample ss, bin q) of dz[ss][co][q] in[ss][ci][(q + t - 2) mod B], a
+// (32 x K) x (K x 160) product with K = the tile's S B = 256 values. The 2 x 10 output
+// tiles of 16 x 16 are split over the four wavefronts, five each (tiles wave + 4 j);
+// v_mfma_f32_16x16x4f32, lane l: A = dz[..][16 mt + l % 16][k], B = in[..] at column
+// 16 nt + l % 16, k = k0 + l / 16; acc[j][r] is row 16 mt + 4 (l / 16) + r, column
+// 16 nt + l % 16. FP32 products and sums in the matrix unit's order.
+template <int B, int S>
+__device__ __forceinline__ void wgrad_mfma(const float (*D)[kC][B + 1], const float (*A)[kC][B + 1],
+                                           cnn_f32x4 (&acc)[5], int wave, int lane) {
+  const int m = lane & 15, kq = lane >> 4;
+  int mt[5], ci[5], tt[5];
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    const int T = wave + 4 * j;
+    mt[j] = T / 10;
+    const int n = (T % 10) * 16 + m;
+    ci[j] = n / kK;
+    tt[j] = n % kK;
+  }
+#pragma unroll 2
+  for (int k0 = 0; k0 < S * B; k0 += 4) {
+    const int k = k0 + kq;
+    const int ss = k / B, q = k % B;
+    const float a0 = D[ss][m][q], a1 = D[ss][16 + m][q];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+      const float b = A[ss][ci[j]][(q + tt[j] - 2) & (B - 1)];
+      acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(mt[j] == 0 ? a0 : a1, b, acc[j], 0, 0, 0);
+    }
+  }
+}
+
+template <int B>
+__device__ __forceinline__ void wgrad_store(float* __restrict__ out, const cnn_f32x4 (&acc)[5],
+                                            int wave, int lane) {
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    const int T = wave + 4 * j;
+    const int n = (T % 10) * 16 + (lane & 15);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int co = (T / 10) * 16 + 4 * (lane >> 4) + r;
+      out[co * kC * kK + n] = acc[j][r];
+    }
+  }
+}
+
 template <int B>
 __global__ void __launch_bounds__(kThreads)
 cnn_backward_kernel(const float* __restrict__ th, const float* __restrict__ cache,
                     const float* __restrict__ logits, const float* __restrict__ labels,
                     int64_t n, float w_pos, float w_neg, float* __restrict__ partial) {
   constexpr int S = kThreads / B;
-  __shared__ float Aa[S][kC][B];  // a2, then a1
-  __shared__ float D[S][kC][B];   // dz3, dz2, dz1
+  __shared__ float Aa[S][kC][B + 1];  // a2, then a1 (rows padded: the products' reads)
+  __shared__ float D[S][kC][B + 1];   // dz3, dz2, dz1
   __shared__ float X[S][B];
   __shared__ float DQ[S][kH], ZZ[S][kZ], UU[S][kH], DH[S][kC], DL[S];
   __shared__ int ARG[S][kC];
   const int tid = threadIdx.x;
   const int s = tid / B, p = tid % B;
-  float g3[kPerThread], g2[kPerThread], gf1[kF1PerThread];
+  const int wave = tid >> 6, lane = tid & 63;
+  cnn_f32x4 g3[5], g2[5];
+  float gf1[kF1PerThread];
 #pragma unroll
-  for (int r = 0; r < kPerThread; ++r) g3[r] = g2[r] = 0.0f;
+  for (int j = 0; j < 5; ++j) g3[j] = g2[j] = cnn_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
   for (int r = 0; r < kF1PerThread; ++r) gf1[r] = 0.0f;
   float g1 = 0.0f, gb1 = 0.0f, gb2 = 0.0f, gb3 = 0.0f, gbf1 = 0.0f, gf2 = 0.0f, gbf2 = 0.0f;
@@ -270,17 +324,7 @@ cnn_backward_kernel(const float* __restrict__ th, const float* __restrict__ cach
       }
     }
     // conv3: weight gradients from dz3 and a2, then da2 -> dz2 (kept in registers).
-#pragma unroll
-    for (int r = 0; r < kPerThread; ++r) {
-      const int o = tid + kThreads * r;
-      const int co = o / (kC * kK), rem = o % (kC * kK), ci = rem / kK, t = rem % kK;
-      float a = g3[r];
-#pragma unroll 1
-      for (int ss = 0; ss < S; ++ss)
-#pragma unroll 4
-        for (int q = 0; q < B; ++q) a += D[ss][co][q] * Aa[ss][ci][(q + t - 2) & (B - 1)];
-      g3[r] = a;
-    }
+    wgrad_mfma<B, S>(D, Aa, g3, wave, lane);
     if (tid < kC)
       for (int ss = 0; ss < S; ++ss)
 #pragma unroll 1
@@ -307,17 +351,7 @@ cnn_backward_kernel(const float* __restrict__ th, const float* __restrict__ cach
     }
     __syncthreads();
     // conv2: weight gradients from dz2 and a1, then da1 -> dz1.
-#pragma unroll
-    for (int r = 0; r < kPerThread; ++r) {
-      const int o = tid + kThreads * r;
-      const int co = o / (kC * kK), rem = o % (kC * kK), ci = rem / kK, t = rem % kK;
-      float a = g2[r];
-#pragma unroll 1
-      for (int ss = 0; ss < S; ++ss)
-#pragma unroll 4
-        for (int q = 0; q < B; ++q) a += D[ss][co][q] * Aa[ss][ci][(q + t - 2) & (B - 1)];
-      g2[r] = a;
-    }
+    wgrad_mfma<B, S>(D, Aa, g2, wave, lane);
     if (tid < kC)
       for (int ss = 0; ss < S; ++ss)
 #pragma unroll 1
@@ -354,11 +388,8 @@ cnn_backward_kernel(const float* __restrict__ th, const float* __restrict__ cach
     __syncthreads();
   }
   float* out = partial + (int64_t)blockIdx.x * kNParams;
-#pragma unroll
-  for (int r = 0; r < kPerThread; ++r) {
-    out[oW3 + tid + kThreads * r] = g3[r];
-    out[oW2 + tid + kThreads * r] = g2[r];
-  }
+  wgrad_store<B>(out + oW3, g3, wave, lane);
+  wgrad_store<B>(out + oW2, g2, wave, lane);
 #pragma unroll
   for (int r = 0; r < kF1PerThread; ++r) {
     const int o = tid + kThreads * r;
