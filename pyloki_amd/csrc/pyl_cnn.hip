@@ -235,6 +235,56 @@ __device__ __forceinline__ void wgrad_mfma(const float (*D)[kC][B + 1], const fl
   }
 }
 
+// Data gradient of a 32 -> 32 conv layer on the matrix cores: dz[ss][ci][p] = sum over
+// k = (co, t) of W[co][ci][t] d[ss][co][(p - t + 2) mod B], per sample a (32 x 160) x
+// (160 x B) product. The tile's S samples x 2 channel tiles x B/16 bin tiles (32 tiles of
+// 16 x 16 for every B) are split over the four wavefronts, eight each (tiles wave + 4 j);
+// lane l: A = W[co][16 mt + l % 16][t], B = d[ss][co][bin of column 16 nt + l % 16], with
+// (co, t) = k0 + l / 16; acc[j][r] is channel 16 mt + 4 (l / 16) + r at bin 16 nt + l % 16.
+template <int B, int S>
+__device__ __forceinline__ void dgrad_mfma(const float* __restrict__ W,
+                                           const float (*D)[kC][B + 1], cnn_f32x4 (&acc)[8],
+                                           int wave, int lane) {
+  constexpr int NT = B / 16;  // bin tiles
+  const int m = lane & 15, kq = lane >> 4;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) acc[j] = cnn_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 2
+  for (int k0 = 0; k0 < kC * kK; k0 += 4) {
+    const int k = k0 + kq;
+    const int co = k / kK, t = k - co * kK;
+    const float w0 = W[(co * kC + m) * kK + t], w1 = W[(co * kC + 16 + m) * kK + t];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int T = wave + 4 * j;
+      const int ss = T / (2 * NT), rem = T - ss * 2 * NT;
+      const int mt = rem / NT, nt = rem - mt * NT;
+      const float b = D[ss][co][(nt * 16 + m - t + 2) & (B - 1)];
+      acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(mt == 0 ? w0 : w1, b, acc[j], 0, 0, 0);
+    }
+  }
+}
+
+// dz masked by the layer input's relu (act > 0), stored over D (after every wavefront has
+// finished reading D).
+template <int B>
+__device__ __forceinline__ void dgrad_store(float (*D)[kC][B + 1], const float (*A)[kC][B + 1],
+                                            const cnn_f32x4 (&acc)[8], int wave, int lane) {
+  constexpr int NT = B / 16;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int T = wave + 4 * j;
+    const int ss = T / (2 * NT), rem = T - ss * 2 * NT;
+    const int mt = rem / NT, nt = rem - mt * NT;
+    const int pb = nt * 16 + (lane & 15);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int ci = mt * 16 + 4 * (lane >> 4) + r;
+      D[ss][ci][pb] = A[ss][ci][pb] > 0.0f ? acc[j][r] : 0.0f;
+    }
+  }
+}
+
 template <int B>
 __device__ __forceinline__ void wgrad_store(float* __restrict__ out, const cnn_f32x4 (&acc)[5],
                                             int wave, int lane) {
@@ -329,26 +379,13 @@ cnn_backward_kernel(const float* __restrict__ th, const float* __restrict__ cach
       for (int ss = 0; ss < S; ++ss)
 #pragma unroll 1
         for (int q = 0; q < B; ++q) gb3 += D[ss][tid][q];
-    float dz[kC];
-#pragma unroll
-    for (int ci = 0; ci < kC; ++ci) dz[ci] = 0.0f;
-#pragma unroll 1
-    for (int co = 0; co < kC; ++co) {
-#pragma unroll
-      for (int t = 0; t < kK; ++t) {
-        const float d = D[s][co][(p - t + 2) & (B - 1)];
-#pragma unroll
-        for (int ci = 0; ci < kC; ++ci) dz[ci] += th[oW3 + (co * kC + ci) * kK + t] * d;
-      }
-    }
-#pragma unroll
-    for (int ci = 0; ci < kC; ++ci) dz[ci] = Aa[s][ci][p] > 0.0f ? dz[ci] : 0.0f;
+    cnn_f32x4 dz[8];
+    dgrad_mfma<B, S>(th + oW3, D, dz, wave, lane);
+    __syncthreads();
+    dgrad_store<B>(D, Aa, dz, wave, lane);  // relu mask of a2
     __syncthreads();
 #pragma unroll
-    for (int c = 0; c < kC; ++c) {
-      D[s][c][p] = dz[c];
-      Aa[s][c][p] = valid ? cs[c * B + p] : 0.0f;  // a1
-    }
+    for (int c = 0; c < kC; ++c) Aa[s][c][p] = valid ? cs[c * B + p] : 0.0f;  // a1
     __syncthreads();
     // conv2: weight gradients from dz2 and a1, then da1 -> dz1.
     wgrad_mfma<B, S>(D, Aa, g2, wave, lane);
@@ -356,22 +393,9 @@ cnn_backward_kernel(const float* __restrict__ th, const float* __restrict__ cach
       for (int ss = 0; ss < S; ++ss)
 #pragma unroll 1
         for (int q = 0; q < B; ++q) gb2 += D[ss][tid][q];
-#pragma unroll
-    for (int ci = 0; ci < kC; ++ci) dz[ci] = 0.0f;
-#pragma unroll 1
-    for (int co = 0; co < kC; ++co) {
-#pragma unroll
-      for (int t = 0; t < kK; ++t) {
-        const float d = D[s][co][(p - t + 2) & (B - 1)];
-#pragma unroll
-        for (int ci = 0; ci < kC; ++ci) dz[ci] += th[oW2 + (co * kC + ci) * kK + t] * d;
-      }
-    }
-#pragma unroll
-    for (int ci = 0; ci < kC; ++ci) dz[ci] = Aa[s][ci][p] > 0.0f ? dz[ci] : 0.0f;
+    dgrad_mfma<B, S>(th + oW2, D, dz, wave, lane);
     __syncthreads();
-#pragma unroll
-    for (int c = 0; c < kC; ++c) D[s][c][p] = dz[c];
+    dgrad_store<B>(D, Aa, dz, wave, lane);  // relu mask of a1
     X[s][p] = valid ? cs[off_x0(B) + p] : 0.0f;
     __syncthreads();
     // conv1: weight gradients from dz1 and x0.
