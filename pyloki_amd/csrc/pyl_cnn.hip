@@ -35,10 +35,7 @@ constexpr int kThreads = 256;
 constexpr int oW1 = 0, oB1 = oW1 + kC * kK, oW2 = oB1 + kC, oB2 = oW2 + kC * kC * kK;
 constexpr int oW3 = oB2 + kC, oB3 = oW3 + kC * kC * kK, oF1 = oB3 + kC, oBF1 = oF1 + kH * kZ;
 constexpr int oF2 = oBF1 + kH, oBF2 = oF2 + kH, kNParams = oBF2 + 1;
-constexpr int kConvOut = kC * kC * kK;            // 5120 weights per 32 -> 32 layer
-constexpr int kPerThread = kConvOut / kThreads;   // 20
 constexpr int kF1PerThread = (kH * kZ + kThreads - 1) / kThreads;  // 5
-static_assert(kConvOut % kThreads == 0, "conv weight split");
 
 // Per-sample training cache (floats): a1 [kC][B], a2 [kC][B], x0 [B], argmax [kC] (int32
 // bits), z [kZ], q [kH].
