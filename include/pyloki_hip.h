@@ -198,6 +198,12 @@ int pyl_ffa_shift_add_score_layout(const float* fold_in_dev, int32_t in_layout,
  * counterpart: it changes no result, only which rows share a block's L2 reuse. */
 int pyl_ffa_set_tile_rows(int32_t rows);
 int pyl_ffa_get_tile_rows(void);
+/* Gather descriptors of the two-level kernel (pyl_ffa_shift_add2_score_layout): 1 (the
+ * default) packs each output row's four source rows and shifts into 16 B of LDS wherever
+ * an input segment holds fewer than 2^32 bytes, so four blocks fit a CU; 0 keeps the
+ * 48 B table entries. Process-wide. No reference counterpart: the results are identical,
+ * the switch exists so that tests can check both variants. */
+int pyl_ffa_set_packed_tables(int32_t enable);
 
 int pyl_ffa_shift_add2_score_layout(const float* fold_in_dev, int32_t in_layout,
                                     int64_t nseg_in, int64_t ncoords_prev2,

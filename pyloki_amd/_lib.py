@@ -67,6 +67,7 @@ SIGNATURES: list[tuple[str, object, list[object]]] = [
      [_c_i32, _vp, _i64p, _i64p, _f64p, _c_i32, _c_dbl, _c_i32, _vp, _vp]),
     ("pyl_ffa_shift_add", ctypes.c_int, [_vp, _c_i64, _c_i64, _vp, _c_i64, _c_i32, _vp, _vp]),
     ("pyl_ffa_set_tile_rows", ctypes.c_int, [_c_i32]),
+    ("pyl_ffa_set_packed_tables", ctypes.c_int, [_c_i32]),
     ("pyl_ffa_get_tile_rows", ctypes.c_int, []),
     ("pyl_ffa_level", ctypes.c_int,
      [_c_i32, _vp, _i64p, _i64p, _f64p, _c_i32, _c_dbl, _c_i32, _vp, _c_i64, _vp, _vp, _vp]),

@@ -391,7 +391,12 @@ ffa_shift_add_score_kernel(const T* __restrict__ in, int64_t ncoords_prev,
 // table2 entry), rotated by the tail shift, and likewise its head row. Each pair is added
 // and stored exactly as the intermediate level would store it, so the result is that of
 // the two levels run one after the other; the intermediate fold is never written.
-template <int B, typename T, int WC, bool kWrapIn, bool kTwo = false>
+//
+// kPack (kTwo, input segments of fewer than 2^32 bytes): a row's four source rows and their
+// composed shifts sit in LDS as one uint4 of (row << 6 | shift) words, 16 B per row
+// instead of the 48 B of its table and table2 entries. The block then needs 37 KB of LDS,
+// so four blocks fit a CU (the register limit) instead of three.
+template <int B, typename T, int WC, bool kWrapIn, bool kTwo = false, bool kPack = false>
 __global__ void __launch_bounds__(256)
 ffa_shift_add_score_rounds_kernel(const T* __restrict__ in, int64_t ncoords_prev,
                                   const int4* __restrict__ table, int64_t ncoords_cur,
@@ -400,6 +405,7 @@ ffa_shift_add_score_rounds_kernel(const T* __restrict__ in, int64_t ncoords_prev
                                   float* __restrict__ out,
                                   const int4* __restrict__ table2 = nullptr,
                                   int64_t ncoords_prev2 = 0) {
+  static_assert(!kPack || kTwo, "packed descriptors are for two fused levels");
   constexpr int kRows = 256;
   constexpr int kLds = kRows + 1;
   constexpr int kSeg = 16;            // scoring window step
@@ -413,7 +419,7 @@ ffa_shift_add_score_rounds_kernel(const T* __restrict__ in, int64_t ncoords_prev
   constexpr int kLdsRows = kWin > kSeg1 ? kWin : kSeg1;
   static_assert(B % kSeg1 == 0 && WC < B, "unsupported round layout");
   extern __shared__ float S[];  // [kLdsRows][kLds]
-  __shared__ int4 s_tab[kRows];
+  __shared__ int4 s_tab[kPack ? 1 : kRows];
   const int tid = threadIdx.x;
   const int64_t vb = xcd_slab_block(blockIdx.x, gridDim.x);
   const int64_t q = vb / tiles_per_pair;
@@ -427,12 +433,27 @@ ffa_shift_add_score_rounds_kernel(const T* __restrict__ in, int64_t ncoords_prev
   const bool ok = outer < n_outer && inner < n_inner;
   const int64_t j = outer * n_inner + inner;
   const int4 t0 = ok ? table[j] : make_int4(-1, 0, 0, 0);
-  s_tab[tid] = t0;
-  __shared__ int4 s_tab2[kTwo ? 2 * kRows : 1];
-  if constexpr (kTwo) {
+  __shared__ int4 s_tab2[kTwo && !kPack ? 2 * kRows : 1];
+  __shared__ uint4 s_pk[kPack ? kRows : 1];
+  constexpr uint32_t kNoRow = 0xffffffffu;  // above every packed word when rows < 2^26
+  if constexpr (kPack) {
+    uint4 pk = make_uint4(kNoRow, 0u, 0u, 0u);
     if (ok) {
-      s_tab2[2 * tid] = table2[t0.x];
-      s_tab2[2 * tid + 1] = table2[t0.z];
+      const int4 ta = table2[t0.x];
+      const int4 ha = table2[t0.z];
+      pk.x = ((uint32_t)ta.x << 6) | (uint32_t)((t0.y + ta.y) & (B - 1));
+      pk.y = ((uint32_t)ta.z << 6) | (uint32_t)((t0.y + ta.w) & (B - 1));
+      pk.z = ((uint32_t)ha.x << 6) | (uint32_t)((t0.w + ha.y) & (B - 1));
+      pk.w = ((uint32_t)ha.z << 6) | (uint32_t)((t0.w + ha.w) & (B - 1));
+    }
+    s_pk[tid] = pk;
+  } else {
+    s_tab[tid] = t0;
+    if constexpr (kTwo) {
+      if (ok) {
+        s_tab2[2 * tid] = table2[t0.x];
+        s_tab2[2 * tid + 1] = table2[t0.z];
+      }
     }
   }
   using Li = RowLayout<B, kWrapIn>;
@@ -450,14 +471,60 @@ ffa_shift_add_score_rounds_kernel(const T* __restrict__ in, int64_t ncoords_prev
 #pragma unroll
     for (int step = 0; step < kSteps; ++step) {
       const int r = wave * 64 + step * kRps + rsub;
-      const int4 tb = s_tab[r];
-      if (tb.x >= 0) {
+      int4 tb;
+      uint4 pk;
+      if constexpr (kPack) pk = s_pk[r];
+      else tb = s_tab[r];
+      if (kPack ? pk.x != kNoRow : tb.x >= 0) {
         const int cc = rd * kLpr + c4;
         // The level's fold element as it would be stored (identity for FP32, BF16 rounding
         // for BF16 storage), then e / sqrt(v).
         auto st = [](float v) { return pyl::Store<T>::load(pyl::Store<T>::store(v)); };
         float4 te, he, tv, hv;
-        if constexpr (kTwo) {
+        if constexpr (kPack) {
+          const int st1 = (int)(pk.x & 63u), st2 = (int)(pk.y & 63u);
+          const int sh1 = (int)(pk.z & 63u), sh2 = (int)(pk.w & 63u);
+          auto add = [&](float4 a, float4 b) {
+            return make_float4(st(a.x + b.x), st(a.y + b.y), st(a.z + b.z), st(a.w + b.w));
+          };
+          if constexpr (kWrapIn) {
+            // 32-bit byte offsets from the four segments' (uniform) bases: a segment holds
+            // fewer than 2^32 bytes here, so the loads take an SGPR base and a VGPR offset
+            // and the row address is one 24-bit multiply-add.
+            constexpr uint32_t kRowB = (uint32_t)(Li::kRow * sizeof(float));
+            constexpr uint32_t kCompB = (uint32_t)(Li::kComp * sizeof(float));
+            const char* bt1 = reinterpret_cast<const char*>(base_t);
+            const char* bt2 = reinterpret_cast<const char*>(base_t + seg_rows);
+            const char* bh1 = reinterpret_cast<const char*>(base_h);
+            const char* bh2 = reinterpret_cast<const char*>(base_h + seg_rows);
+            auto off = [&](uint32_t w, int s) {
+              return __umul24(w >> 6, kRowB) +
+                     (uint32_t)(((4 * cc - s) & (B - 1)) * (int)sizeof(float));
+            };
+            auto ld = [](const char* b, uint32_t o) {
+              float4 v;
+              __builtin_memcpy(&v, b + o, sizeof(float4));
+              return v;
+            };
+            const uint32_t o1 = off(pk.x, st1), o2 = off(pk.y, st2);
+            const uint32_t o3 = off(pk.z, sh1), o4 = off(pk.w, sh2);
+            te = add(ld(bt1, o1), ld(bt2, o2));
+            he = add(ld(bh1, o3), ld(bh2, o4));
+            tv = add(ld(bt1 + kCompB, o1), ld(bt2 + kCompB, o2));
+            hv = add(ld(bh1 + kCompB, o3), ld(bh2 + kCompB, o4));
+          } else {
+            const T* t1 = base_t + (int64_t)(pk.x >> 6) * Li::kRow;
+            const T* t2 = base_t + seg_rows + (int64_t)(pk.y >> 6) * Li::kRow;
+            const T* h1 = base_h + (int64_t)(pk.z >> 6) * Li::kRow;
+            const T* h2 = base_h + seg_rows + (int64_t)(pk.w >> 6) * Li::kRow;
+            te = add(row_chunk<B, T, kWrapIn>(t1, cc, st1), row_chunk<B, T, kWrapIn>(t2, cc, st2));
+            he = add(row_chunk<B, T, kWrapIn>(h1, cc, sh1), row_chunk<B, T, kWrapIn>(h2, cc, sh2));
+            tv = add(row_chunk<B, T, kWrapIn>(t1 + Li::kComp, cc, st1),
+                     row_chunk<B, T, kWrapIn>(t2 + Li::kComp, cc, st2));
+            hv = add(row_chunk<B, T, kWrapIn>(h1 + Li::kComp, cc, sh1),
+                     row_chunk<B, T, kWrapIn>(h2 + Li::kComp, cc, sh2));
+          }
+        } else if constexpr (kTwo) {
           const int4 ta = s_tab2[2 * r];
           const int4 ha = s_tab2[2 * r + 1];
           const T* t1 = base_t + (int64_t)ta.x * Li::kRow;
@@ -716,6 +783,11 @@ extern "C" int pyl_ffa_shift_add(const float* fold_in, int64_t nseg_in, int64_t 
 // rows on short outer axes. The rows a tile holds never change a result.
 static std::atomic<int> g_tile_outer{0};
 
+// Packed gather descriptors in the two-level kernel (kPack): on by default wherever the
+// row indices fit; pyl_ffa_set_packed_tables(0) selects the unpacked tables (a check that
+// both variants agree). Neither changes a result.
+static std::atomic<int> g_packed{1};
+
 static int tile_outer_rows(int64_t n_outer) {
   int to = g_tile_outer.load(std::memory_order_relaxed);
   if (to <= 0) to = 16;
@@ -748,10 +820,20 @@ static void launch_rounds2(const float* in, int64_t nprev2, const int4* tb2, int
   const int64_t tiles_outer = pyl::ceil_div(n_outer, to);
   const int64_t per_pair = tiles_inner * tiles_outer;
   const size_t lds = sizeof(float) * (size_t)std::max(16 + WC, 32) * 257;
-  ffa_shift_add_score_rounds_kernel<B, float, WC, kWrapIn, true>
-      <<<(unsigned)(per_pair * nseg_out), 256, lds, s>>>(in, nprev, tb, ncur, n_inner, to,
-                                                          tiles_inner, per_pair, k, out, tb2,
-                                                          nprev2);
+  const unsigned grid = (unsigned)(per_pair * nseg_out);
+  // Row indices fit the packed (row << 6 | shift) words, and a segment's bytes the 32-bit
+  // offsets of the wrap-layout gather (so a row index also fits the 24-bit multiply).
+  const bool fits = nprev2 < ((int64_t)1 << 26) &&
+                    nprev2 * RowLayout<B, kWrapIn>::kRow * (int64_t)sizeof(float) <
+                        ((int64_t)1 << 32);
+  if (fits && g_packed.load(std::memory_order_relaxed))
+    ffa_shift_add_score_rounds_kernel<B, float, WC, kWrapIn, true, true>
+        <<<grid, 256, lds, s>>>(in, nprev, tb, ncur, n_inner, to, tiles_inner, per_pair, k,
+                                out, tb2, nprev2);
+  else
+    ffa_shift_add_score_rounds_kernel<B, float, WC, kWrapIn, true, false>
+        <<<grid, 256, lds, s>>>(in, nprev, tb, ncur, n_inner, to, tiles_inner, per_pair, k,
+                                out, tb2, nprev2);
 }
 
 template <int B, bool kWrapIn>
@@ -992,6 +1074,12 @@ extern "C" int pyl_ffa_set_tile_rows(int32_t rows) {
 }
 
 extern "C" int pyl_ffa_get_tile_rows(void) { return g_tile_outer.load(std::memory_order_relaxed); }
+
+extern "C" int pyl_ffa_set_packed_tables(int32_t enable) {
+  PYL_REQUIRE(enable == 0 || enable == 1, "enable must be 0 or 1 (got %d)", enable);
+  g_packed.store(enable, std::memory_order_relaxed);
+  return PYL_OK;
+}
 
 // ---- One FFA level without any allocation -------------------------------------------
 // The resolve table of pyl_ffa_level lives in a file-scope device array instead of a

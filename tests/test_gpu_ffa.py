@@ -220,10 +220,17 @@ def test_two_fused_levels_bit_exact(gpu_device, nbins):
             np.testing.assert_array_equal(
                 exp, orc.boxcar_snr(orc.fold_norm(folds[-1].cpu().numpy()), widths))
         for part, lay in ((src, "dense"), (src_wrap, "wrap")):
-            got = kernels.ffa_shift_add2_score(part, lp_prev.table, lp.table, lp_prev.ncoords,
-                                               lp.ncoords, widths,
-                                               n_inner=int(lp.counts_cur[-1]), in_layout=lay)
-            np.testing.assert_array_equal(got.cpu().numpy(), exp)
+            # packed (default) and unpacked gather descriptors
+            for packed in (1, 0):
+                _lib.check(_lib.load().pyl_ffa_set_packed_tables(packed))
+                try:
+                    got = kernels.ffa_shift_add2_score(part, lp_prev.table, lp.table,
+                                                       lp_prev.ncoords, lp.ncoords, widths,
+                                                       n_inner=int(lp.counts_cur[-1]),
+                                                       in_layout=lay)
+                finally:
+                    _lib.check(_lib.load().pyl_ffa_set_packed_tables(1))
+                np.testing.assert_array_equal(got.cpu().numpy(), exp)
     with pytest.raises(ValueError, match="at most 16 widths"):
         kernels.ffa_shift_add2_score(src, lp_prev.table, lp.table, lp_prev.ncoords, lp.ncoords,
                                      _width_set(nbins, "many"))
