@@ -207,7 +207,7 @@ EP_CNN = {"nsamps": 2**24, "tsamp": 64e-6, "nseg": 128, "nbins": 64, "freq": (24
           "survey_budget": 12288,
           "train_seeds": (1, 2, 3, 4), "test_seed": 9, "epochs": 16,
           "batch_seeds": tuple(range(9, 17)), "survey_seeds": tuple(range(9, 25)),
-          "calib_seeds": tuple(range(101, 109)), "calib_loss": 0.005}
+          "calib_seeds": tuple(range(101, 109)), "calib_loss": 0.0}
 
 
 def ep_circular_dyp(seed: int, w: dict = EP_CNN, use_fourier: bool = True):
@@ -254,12 +254,14 @@ def ep_circular_cnn() -> int:
     # (every launch of a level serves all of them): exhaustive vs the learned, hybrid and
     # S/N budgets.
     b = w["survey_budget"]
-    # A per-level S/N budget schedule for recall >= 0.99: calibrated on the exhaustive runs
-    # of held-out calibration series (the budget that loses at most calib_loss of the final
-    # candidates at each level, the largest over the series within +-2 levels).
+    # A per-level S/N budget schedule from the exhaustive runs of held-out calibration
+    # series: each level's need (the budget that keeps every final candidate's ancestor,
+    # calib_loss 0), the recipe (margin, floor, window over levels) chosen by leave-one-out
+    # on the calibration series (cnn.select_schedule).
     cal = cnn.calibrate_schedules(ep_circular_dyp, w["calib_seeds"], thr, w["max_sugg"],
-                                  margin=1.0, floor=512, window=2, loss=w["calib_loss"])
-    sched = cal["snr"]
+                                  loss=w["calib_loss"])
+    sel = cnn.select_schedule(cal["needs"]["snr"], cal["survivors"], cal["nlevels"])
+    sched = sel["schedule"]
     series = [ep_circular_dyp(s) for s in w["survey_seeds"]]
     hooks = {"exhaustive": None,
              f"hybrid_{b}": lambda: cnn.SurvivorBudget(b, ranker, hybrid=True),
@@ -284,6 +286,7 @@ def ep_circular_cnn() -> int:
         "survey": survey,
         "exhaustive_trials_per_s": ex["trials"] / len(series) / (ex["ms_per_search"] * 1e-3),
         "schedule": {"calib_seeds": list(w["calib_seeds"]), "loss": w["calib_loss"],
+                     "recipe": {k: v for k, v in sel.items() if k != "schedule"},
                      "sum": int(sched.sum()), "max": int(sched.max()),
                      "levels_above_survey_budget": int((sched > b).sum())},
         "test_seed": {k: v for k, v in res.items() if k not in ("train", "ranker")},

@@ -521,6 +521,7 @@ def calibrate_schedules(make_dyp, seeds, thresholds: np.ndarray, max_sugg: int,
         keys["cnn"] = network_keys(ranker, hybrid=False)
         keys["hybrid"] = network_keys(ranker, hybrid=True)
     needs = {k: [] for k in keys}
+    survivors = []
     nlevels = 0
     for s in seeds:
         dyp = make_dyp(s)
@@ -530,10 +531,42 @@ def calibrate_schedules(make_dyp, seeds, thresholds: np.ndarray, max_sugg: int,
         weights = descendant_counts(recs)
         for k, fn in keys.items():
             needs[k].append(level_needs(recs, weights, fn, loss))
+        survivors.append({r.level: int(r.feats.shape[0]) for r in recs})
         del recs, weights, pr
     out = {k: budget_schedule(v, nlevels, margin, floor, window) for k, v in needs.items()}
     out["needs"] = needs
+    out["survivors"] = survivors
+    out["nlevels"] = nlevels
     return out
+
+
+def select_schedule(needs: list[dict], survivors: list[dict], nlevels: int,
+                    margins=(1.0, 1.25, 1.5, 2.0), floors=(512, 2048, 4096, 8192, 12288),
+                    windows=(0, 1, 2, 3)) -> dict:
+    """The ``budget_schedule`` recipe (margin, floor, window) chosen on the calibration
+    runs alone: the most calibration series kept when each is left out of the schedule it
+    is checked against (a series is kept when the schedule is at least its need at every
+    level, which is sufficient: cuts at earlier levels only raise the rank of a kept
+    ancestor), and among those the fewest survivors kept (sum over levels and series of
+    min(budget, exhaustive survivors)). Returns the schedule from all the series and the
+    recipe's figures."""
+    n = len(needs)
+    best = None
+    for m in margins:
+        for f in floors:
+            for w in windows:
+                kept = sum(
+                    all(v <= sched[lv] for lv, v in needs[x].items())
+                    for x in range(n)
+                    for sched in [budget_schedule([needs[y] for y in range(n) if y != x],
+                                                  nlevels, m, f, w)])
+                sched = budget_schedule(needs, nlevels, m, f, w)
+                cost = sum(min(int(sched[lv]), c) for sv in survivors for lv, c in sv.items())
+                key = (-kept, cost)
+                if best is None or key < best[0]:
+                    best = (key, {"margin": m, "floor": f, "window": w, "loo_kept": kept,
+                                  "loo_of": n, "kept_survivors": cost, "schedule": sched})
+    return best[1]
 
 
 def budget_experiment(make_dyp, train_seeds, test_seed: int, thresholds: np.ndarray,
