@@ -393,7 +393,7 @@ ffa_shift_add_score_kernel(const T* __restrict__ in, int64_t ncoords_prev,
 // the two levels run one after the other; the intermediate fold is never written.
 //
 // kPack (kTwo, input segments of fewer than 2^32 bytes): a row's four source rows and their
-// composed shifts sit in LDS as one uint4 of (row << 6 | shift) words, 16 B per row
+// composed shifts sit in LDS as one uint4 of (row << 8 | 4 * shift) words, 16 B per row
 // instead of the 48 B of its table and table2 entries. The block then needs 37 KB of LDS,
 // so four blocks fit a CU (the register limit) instead of three.
 template <int B, typename T, int WC, bool kWrapIn, bool kTwo = false, bool kPack = false>
@@ -435,16 +435,16 @@ ffa_shift_add_score_rounds_kernel(const T* __restrict__ in, int64_t ncoords_prev
   const int4 t0 = ok ? table[j] : make_int4(-1, 0, 0, 0);
   __shared__ int4 s_tab2[kTwo && !kPack ? 2 * kRows : 1];
   __shared__ uint4 s_pk[kPack ? kRows : 1];
-  constexpr uint32_t kNoRow = 0xffffffffu;  // above every packed word when rows < 2^26
+  constexpr uint32_t kNoRow = 0xffffffffu;  // above every packed word when rows < 2^24
   if constexpr (kPack) {
     uint4 pk = make_uint4(kNoRow, 0u, 0u, 0u);
     if (ok) {
       const int4 ta = table2[t0.x];
       const int4 ha = table2[t0.z];
-      pk.x = ((uint32_t)ta.x << 6) | (uint32_t)((t0.y + ta.y) & (B - 1));
-      pk.y = ((uint32_t)ta.z << 6) | (uint32_t)((t0.y + ta.w) & (B - 1));
-      pk.z = ((uint32_t)ha.x << 6) | (uint32_t)((t0.w + ha.y) & (B - 1));
-      pk.w = ((uint32_t)ha.z << 6) | (uint32_t)((t0.w + ha.w) & (B - 1));
+      pk.x = ((uint32_t)ta.x << 8) | (uint32_t)(4 * ((t0.y + ta.y) & (B - 1)));
+      pk.y = ((uint32_t)ta.z << 8) | (uint32_t)(4 * ((t0.y + ta.w) & (B - 1)));
+      pk.z = ((uint32_t)ha.x << 8) | (uint32_t)(4 * ((t0.w + ha.y) & (B - 1)));
+      pk.w = ((uint32_t)ha.z << 8) | (uint32_t)(4 * ((t0.w + ha.w) & (B - 1)));
     }
     s_pk[tid] = pk;
   } else {
@@ -482,41 +482,41 @@ ffa_shift_add_score_rounds_kernel(const T* __restrict__ in, int64_t ncoords_prev
         auto st = [](float v) { return pyl::Store<T>::load(pyl::Store<T>::store(v)); };
         float4 te, he, tv, hv;
         if constexpr (kPack) {
-          const int st1 = (int)(pk.x & 63u), st2 = (int)(pk.y & 63u);
-          const int sh1 = (int)(pk.z & 63u), sh2 = (int)(pk.w & 63u);
           auto add = [&](float4 a, float4 b) {
             return make_float4(st(a.x + b.x), st(a.y + b.y), st(a.z + b.z), st(a.w + b.w));
           };
           if constexpr (kWrapIn) {
             // 32-bit byte offsets from the four segments' (uniform) bases: a segment holds
             // fewer than 2^32 bytes here, so the loads take an SGPR base and a VGPR offset
-            // and the row address is one 24-bit multiply-add.
+            // and the row address is one 24-bit multiply-add. The byte offset of the
+            // rotated chunk, 4 ((4 cc - s) mod B), is (16 cc - w) mod 4B masked to whole
+            // words, since w = 256 row + 4 s and 4B divides 256.
             constexpr uint32_t kRowB = (uint32_t)(Li::kRow * sizeof(float));
             constexpr uint32_t kCompB = (uint32_t)(Li::kComp * sizeof(float));
             const char* bt1 = reinterpret_cast<const char*>(base_t);
             const char* bt2 = reinterpret_cast<const char*>(base_t + seg_rows);
             const char* bh1 = reinterpret_cast<const char*>(base_h);
             const char* bh2 = reinterpret_cast<const char*>(base_h + seg_rows);
-            auto off = [&](uint32_t w, int s) {
-              return __umul24(w >> 6, kRowB) +
-                     (uint32_t)(((4 * cc - s) & (B - 1)) * (int)sizeof(float));
+            auto off = [&](uint32_t w) {
+              return __umul24(w >> 8, kRowB) + ((16u * (uint32_t)cc - w) & (4u * (B - 1)));
             };
             auto ld = [](const char* b, uint32_t o) {
               float4 v;
               __builtin_memcpy(&v, b + o, sizeof(float4));
               return v;
             };
-            const uint32_t o1 = off(pk.x, st1), o2 = off(pk.y, st2);
-            const uint32_t o3 = off(pk.z, sh1), o4 = off(pk.w, sh2);
+            const uint32_t o1 = off(pk.x), o2 = off(pk.y), o3 = off(pk.z), o4 = off(pk.w);
             te = add(ld(bt1, o1), ld(bt2, o2));
             he = add(ld(bh1, o3), ld(bh2, o4));
             tv = add(ld(bt1 + kCompB, o1), ld(bt2 + kCompB, o2));
             hv = add(ld(bh1 + kCompB, o3), ld(bh2 + kCompB, o4));
           } else {
-            const T* t1 = base_t + (int64_t)(pk.x >> 6) * Li::kRow;
-            const T* t2 = base_t + seg_rows + (int64_t)(pk.y >> 6) * Li::kRow;
-            const T* h1 = base_h + (int64_t)(pk.z >> 6) * Li::kRow;
-            const T* h2 = base_h + seg_rows + (int64_t)(pk.w >> 6) * Li::kRow;
+            const T* t1 = base_t + (int64_t)(pk.x >> 8) * Li::kRow;
+            const T* t2 = base_t + seg_rows + (int64_t)(pk.y >> 8) * Li::kRow;
+            const T* h1 = base_h + (int64_t)(pk.z >> 8) * Li::kRow;
+            const T* h2 = base_h + seg_rows + (int64_t)(pk.w >> 8) * Li::kRow;
+            const int st1 = (int)((pk.x >> 2) & 63u), st2 = (int)((pk.y >> 2) & 63u);
+            const int sh1 = (int)((pk.z >> 2) & 63u), sh2 = (int)((pk.w >> 2) & 63u);
             te = add(row_chunk<B, T, kWrapIn>(t1, cc, st1), row_chunk<B, T, kWrapIn>(t2, cc, st2));
             he = add(row_chunk<B, T, kWrapIn>(h1, cc, sh1), row_chunk<B, T, kWrapIn>(h2, cc, sh2));
             tv = add(row_chunk<B, T, kWrapIn>(t1 + Li::kComp, cc, st1),
@@ -821,9 +821,9 @@ static void launch_rounds2(const float* in, int64_t nprev2, const int4* tb2, int
   const int64_t per_pair = tiles_inner * tiles_outer;
   const size_t lds = sizeof(float) * (size_t)std::max(16 + WC, 32) * 257;
   const unsigned grid = (unsigned)(per_pair * nseg_out);
-  // Row indices fit the packed (row << 6 | shift) words, and a segment's bytes the 32-bit
+  // Row indices fit the packed (row << 8 | 4 shift) words, and a segment's bytes the 32-bit
   // offsets of the wrap-layout gather (so a row index also fits the 24-bit multiply).
-  const bool fits = nprev2 < ((int64_t)1 << 26) &&
+  const bool fits = nprev2 < ((int64_t)1 << 24) &&
                     nprev2 * RowLayout<B, kWrapIn>::kRow * (int64_t)sizeof(float) <
                         ((int64_t)1 << 32);
   if (fits && g_packed.load(std::memory_order_relaxed))
