@@ -205,7 +205,7 @@ EP_CNN = {"nsamps": 2**24, "tsamp": 64e-6, "nseg": 128, "nbins": 64, "freq": (24
           "p_orb": 6 * 3600.0, "x_orb": 1.5, "m_c_max": 0.7, "m_p_min": 1.4, "snr": 15.0,
           "thresholds": (0.5, 8.0), "max_sugg": 2**21, "budgets": (2**14, 12288),
           "survey_budget": 12288,
-          "train_seeds": (1, 2, 3, 4), "test_seed": 9, "epochs": 16,
+          "train_seeds": tuple(range(1, 9)) + tuple(range(25, 49)), "test_seed": 9, "epochs": 16,
           "batch_seeds": tuple(range(9, 17)), "survey_seeds": tuple(range(9, 25)),
           "calib_seeds": tuple(range(101, 109)), "calib_loss": 0.0}
 

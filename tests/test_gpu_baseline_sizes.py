@@ -379,7 +379,7 @@ def test_ep_circular_2e24_learned_budget_recall(gpu_device):
     w = bench.EP_CNN
     thr = np.linspace(w["thresholds"][0], w["thresholds"][1], w["nseg"] - 1)
     b = w["survey_budget"]
-    res = cnn.budget_experiment(bench.ep_circular_dyp, w["train_seeds"], w["test_seed"], thr,
+    res = cnn.budget_experiment(bench.ep_circular_dyp, (1, 2, 3, 4), w["test_seed"], thr,
                                 w["max_sugg"], (b,), epochs=w["epochs"])
     ranker = res["ranker"]
     tot = {"hybrid": [0, 0, 0], "snr": [0, 0, 0], "cnn": [0, 0, 0]}
