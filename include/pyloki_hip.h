@@ -862,7 +862,8 @@ int pyl_cnn_forward(const float* params_dev, const float* folds_dev, int64_t n, 
                     float* cache_dev, void* stream);
 /* Gradient of the loss (summed over the n samples) into grad_dev [num_params], from the
  * cache and logits of pyl_cnn_forward; labels_dev f32 [n] in {0, 1}. work_dev holds the
- * per-block partial sums (at least num_params floats; more blocks with more room). */
+ * per-block partial sums and, past 32 blocks, one group sum per 32 blocks (at least
+ * num_params floats; more blocks with more room). */
 int pyl_cnn_backward(const float* params_dev, const float* cache_dev, const float* logits_dev,
                      const float* labels_dev, int64_t n, int nbins, float w_pos, float w_neg,
                      float* work_dev, int64_t work_floats, float* grad_dev, void* stream);

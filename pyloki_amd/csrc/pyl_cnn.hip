@@ -18,8 +18,8 @@
 // grid-stride; the weight gradients of the two 32 -> 32 layers are products on the matrix
 // cores (FP32, wgrad_mfma), every wavefront owning five 16 x 16 output tiles accumulated
 // over the block's tiles; the other gradients are owned by fixed threads. Each block writes
-// its partial sum once; a second kernel adds the partials in block order, so the gradient
-// is deterministic.
+// its partial sum once; a reduction adds the partials in groups of 32 blocks and then the
+// group sums, each in a fixed order, so the gradient is deterministic.
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
@@ -429,13 +429,20 @@ cnn_backward_kernel(const float* __restrict__ th, const float* __restrict__ cach
   if (tid == 0) out[oBF2] = gbf2;
 }
 
-__global__ void cnn_reduce_kernel(const float* __restrict__ partial, int nblk,
-                                  float* __restrict__ grad) {
+// Sums of the blocks' partial gradients in fixed order: blockIdx.y = g adds the partials
+// [g per, (g + 1) per) (of npart) in order into out[g]. The gradient takes one launch of
+// groups of kGroup partials and one over the group sums (one launch when npart <= kGroup),
+// so it depends only on the number of partials.
+constexpr int kGroup = 32;
+__global__ void cnn_reduce_kernel(const float* __restrict__ partial, int npart, int per,
+                                  float* __restrict__ out) {
   const int o = blockIdx.x * blockDim.x + threadIdx.x;
   if (o >= kNParams) return;
+  const int b0 = (int)blockIdx.y * per;
+  const int b1 = std::min(npart, b0 + per);
   float a = 0.0f;
-  for (int b = 0; b < nblk; ++b) a += partial[(int64_t)b * kNParams + o];
-  grad[o] = a;
+  for (int b = b0; b < b1; ++b) a += partial[(int64_t)b * kNParams + o];
+  out[(int64_t)blockIdx.y * kNParams + o] = a;
 }
 
 __global__ void cnn_adam_kernel(float* __restrict__ th, const float* __restrict__ g,
@@ -733,14 +740,27 @@ extern "C" int pyl_cnn_backward(const float* params, const float* cache, const f
   hipStream_t s = pyl::as_stream(stream);
   const int S = kThreads / nbins;
   const int64_t tiles = std::max<int64_t>(1, (n + S - 1) / S);
-  const int nblk = (int)std::min<int64_t>({tiles, (int64_t)2048, work_floats / kNParams});
+  // Blocks: one per tile up to 2048 and the workspace's room, which also holds the group
+  // sums of the two-level reduction (one per kGroup blocks) after the partials.
+  const int64_t room = work_floats / kNParams;
+  int nblk = (int)std::min<int64_t>({tiles, (int64_t)2048, room});
+  while (nblk > kGroup && nblk + (nblk + kGroup - 1) / kGroup > room) --nblk;
+  const int ngroup = nblk > kGroup ? (nblk + kGroup - 1) / kGroup : 1;
   switch (nbins) {
     case 32: cnn_backward_kernel<32><<<nblk, kThreads, 0, s>>>(params, cache, logits, labels, n, w_pos, w_neg, work); break;
     case 64: cnn_backward_kernel<64><<<nblk, kThreads, 0, s>>>(params, cache, logits, labels, n, w_pos, w_neg, work); break;
     default: cnn_backward_kernel<128><<<nblk, kThreads, 0, s>>>(params, cache, logits, labels, n, w_pos, w_neg, work); break;
   }
   PYL_CHECK_LAUNCH("cnn_backward_kernel");
-  cnn_reduce_kernel<<<(unsigned)pyl::ceil_div(kNParams, 256), 256, 0, s>>>(work, nblk, grad);
+  const unsigned pblk = (unsigned)pyl::ceil_div(kNParams, 256);
+  if (ngroup == 1) {
+    cnn_reduce_kernel<<<dim3(pblk, 1), 256, 0, s>>>(work, nblk, nblk, grad);
+  } else {
+    float* groups = work + (int64_t)nblk * kNParams;
+    cnn_reduce_kernel<<<dim3(pblk, (unsigned)ngroup), 256, 0, s>>>(work, nblk, kGroup, groups);
+    PYL_CHECK_LAUNCH("cnn_reduce_kernel");
+    cnn_reduce_kernel<<<dim3(pblk, 1), 256, 0, s>>>(groups, ngroup, ngroup, grad);
+  }
   PYL_CHECK_LAUNCH("cnn_reduce_kernel");
   return PYL_OK;
 }
