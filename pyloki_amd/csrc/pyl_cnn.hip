@@ -12,9 +12,10 @@
 // Loss: sum_n w_pos y softplus(-logit) + w_neg (1 - y) softplus(logit).
 //
 // Work decomposition: a block of 256 threads holds 256 / B samples, one thread per
-// (sample, bin). The conv layers keep the block's activations in LDS (32 KB) and each
-// thread accumulates its 32 output channels in registers; weights are read with
-// block-uniform indices (scalar loads, no LDS). The backward kernel walks tiles of samples
+// (sample, bin) for the profile and the first conv layer (32 output channels per thread
+// in registers, weights read with block-uniform indices); the two 32 -> 32 layers are FP32
+// products on the matrix cores (fwd_mfma) over the block's activations in LDS (33 KB).
+// The backward kernel walks tiles of samples
 // grid-stride; the weight gradients of the two 32 -> 32 layers are products on the matrix
 // cores (FP32, wgrad_mfma), every wavefront owning five 16 x 16 output tiles accumulated
 // over the block's tiles; the other gradients are owned by fixed threads. Each block writes
@@ -72,6 +73,69 @@ __device__ __forceinline__ void conv_layer(const float* __restrict__ W,
   }
 }
 
+typedef float cnn_f32x4 __attribute__((ext_vector_type(4)));
+
+// Forward 32 -> 32 conv layer on the matrix cores: y[ss][co][p] = b[co] + sum over
+// k = (ci, t) of W[co][ci][t] in[ss][ci][(p + t - 2) mod B], per sample a (32 x 160) x
+// (160 x B) product. The block's S samples x 2 channel tiles x B/16 bin tiles (32 tiles of
+// 16 x 16 for every B) are split over the four wavefronts, eight each (tiles wave + 4 j);
+// v_mfma_f32_16x16x4f32, lane l: A = W[16 mt + l % 16][ci][t], B = in[ss][ci][bin of
+// column 16 nt + l % 16 shifted by t], (ci, t) = k0 + l / 16; acc[j][r] starts at the bias
+// and ends as channel 16 mt + 4 (l / 16) + r at bin 16 nt + l % 16. FP32 products and sums
+// in the matrix unit's order.
+template <int B, int S>
+__device__ __forceinline__ void fwd_mfma(const float* __restrict__ W,
+                                         const float* __restrict__ bias,
+                                         const float (*In)[kC][B + 1], cnn_f32x4 (&acc)[8],
+                                         int wave, int lane) {
+  constexpr int NT = B / 16;
+  const int m = lane & 15, kq = lane >> 4;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int mt = ((wave + 4 * j) % (2 * NT)) / NT;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc[j][r] = bias[16 * mt + 4 * kq + r];
+  }
+#pragma unroll 2
+  for (int k0 = 0; k0 < kC * kK; k0 += 4) {
+    const int k = k0 + kq;
+    const int ci = k / kK, t = k - ci * kK;
+    const float w0 = W[(m * kC + ci) * kK + t], w1 = W[((16 + m) * kC + ci) * kK + t];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int T = wave + 4 * j;
+      const int ss = T / (2 * NT), rem = T - ss * 2 * NT;
+      const int mt = rem / NT, nt = rem - mt * NT;
+      const float b = In[ss][ci][(nt * 16 + m + t - 2) & (B - 1)];
+      acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(mt == 0 ? w0 : w1, b, acc[j], 0, 0, 0);
+    }
+  }
+}
+
+// relu of fwd_mfma's tiles into A (after every wavefront has finished reading it); with
+// cache, also into each valid sample's training cache at offset coff.
+template <int B, int S>
+__device__ __forceinline__ void fwd_store(float (*A)[kC][B + 1], const cnn_f32x4 (&acc)[8],
+                                          int wave, int lane, float* __restrict__ cache,
+                                          int64_t i0, int64_t n, int64_t coff) {
+  constexpr int NT = B / 16;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int T = wave + 4 * j;
+    const int ss = T / (2 * NT), rem = T - ss * 2 * NT;
+    const int mt = rem / NT, nt = rem - mt * NT;
+    const int pb = nt * 16 + (lane & 15);
+    const bool keep = cache != nullptr && i0 + ss < n;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int co = mt * 16 + 4 * (lane >> 4) + r;
+      const float a = fmaxf(acc[j][r], 0.0f);
+      A[ss][co][pb] = a;
+      if (keep) cache[(i0 + ss) * cache_stride(B) + coff + co * B + pb] = a;
+    }
+  }
+}
+
 template <int B, bool kComplex, bool kTrain>
 __global__ void __launch_bounds__(kThreads)
 cnn_forward_kernel(const float* __restrict__ th, const float* __restrict__ folds, int64_t n_arg,
@@ -80,7 +144,7 @@ cnn_forward_kernel(const float* __restrict__ th, const float* __restrict__ folds
   constexpr int S = kThreads / B;
   const int64_t n = n_dev ? *n_dev : n_arg;  // device count: the EP level loop
   if ((int64_t)blockIdx.x * S >= n) return;  // the whole block
-  __shared__ float A[S][kC][B];
+  __shared__ float A[S][kC][B + 1];
   __shared__ float X[S][B];
   __shared__ float stat[S][2];
   __shared__ float Z[S][kZ];
@@ -142,22 +206,20 @@ cnn_forward_kernel(const float* __restrict__ th, const float* __restrict__ folds
     }
   }
   __syncthreads();
-  conv_layer<B, kC>(th + oW2, th + oB2, &A[s][0][0], p, acc);
-  __syncthreads();
-#pragma unroll
-  for (int co = 0; co < kC; ++co) {
-    const float a = fmaxf(acc[co], 0.0f);
-    A[s][co][p] = a;
-    if constexpr (kTrain) {
-      if (valid) cs[off_a2(B) + co * B + p] = a;
-    }
+  {
+    // The two 32 -> 32 layers on the matrix cores.
+    const int wave = tid >> 6, lane = tid & 63;
+    const int64_t i0 = (int64_t)blockIdx.x * S;
+    cnn_f32x4 acc8[8];
+    fwd_mfma<B, S>(th + oW2, th + oB2, A, acc8, wave, lane);
+    __syncthreads();
+    fwd_store<B, S>(A, acc8, wave, lane, kTrain ? cache : nullptr, i0, n, off_a2(B));
+    __syncthreads();
+    fwd_mfma<B, S>(th + oW3, th + oB3, A, acc8, wave, lane);
+    __syncthreads();
+    fwd_store<B, S>(A, acc8, wave, lane, nullptr, i0, n, 0);
+    __syncthreads();
   }
-  __syncthreads();
-  conv_layer<B, kC>(th + oW3, th + oB3, &A[s][0][0], p, acc);
-  __syncthreads();
-#pragma unroll
-  for (int co = 0; co < kC; ++co) A[s][co][p] = fmaxf(acc[co], 0.0f);
-  __syncthreads();
   // Max-pool over bins (first maximum), then the features.
   if (p < kC) {
     float best = A[s][p][0];
@@ -196,8 +258,6 @@ cnn_forward_kernel(const float* __restrict__ th, const float* __restrict__ folds
     logits[i] = lg;
   }
 }
-
-typedef float cnn_f32x4 __attribute__((ext_vector_type(4)));
 
 // Weight gradient of a 32 -> 32 conv layer on the matrix cores: dW[co][n] (n = ci kK + t)
 // += sum over k = (sample ss, bin q) of dz[ss][co][q] in[ss][ci][(q + t - 2) mod B], a
