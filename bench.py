@@ -262,9 +262,15 @@ def ep_circular_cnn() -> int:
                                   loss=w["calib_loss"])
     sel = cnn.select_schedule(cal["needs"]["snr"], cal["survivors"], cal["nlevels"])
     sched = sel["schedule"]
+    # The hybrid key only where S/N ranking loses calibration candidates at this budget: up
+    # to the last level whose S/N need exceeds it on any calibration series; S/N after.
+    net_until = max((lv for nd in cal["needs"]["snr"] for lv, v in nd.items() if v > b),
+                    default=-1)
     series = [ep_circular_dyp(s) for s in w["survey_seeds"]]
     hooks = {"exhaustive": None,
              f"hybrid_{b}": lambda: cnn.SurvivorBudget(b, ranker, hybrid=True),
+             f"hybrid_{b}_until{net_until}": lambda: cnn.SurvivorBudget(
+                 b, ranker, hybrid=True, net_until=net_until),
              f"cnn_{b}": lambda: cnn.SurvivorBudget(b, ranker),
              f"snr_{b}": lambda: cnn.SurvivorBudget(b),
              "snr_schedule": lambda: cnn.SurvivorBudget(sched)}
@@ -287,6 +293,7 @@ def ep_circular_cnn() -> int:
         "exhaustive_trials_per_s": ex["trials"] / len(series) / (ex["ms_per_search"] * 1e-3),
         "schedule": {"calib_seeds": list(w["calib_seeds"]), "loss": w["calib_loss"],
                      "recipe": {k: v for k, v in sel.items() if k != "schedule"},
+                     "hybrid_net_until": net_until,
                      "sum": int(sched.sum()), "max": int(sched.max()),
                      "levels_above_survey_budget": int((sched > b).sum())},
         "test_seed": {k: v for k, v in res.items() if k not in ("train", "ranker")},

@@ -562,6 +562,10 @@ typedef struct pyl_ep_budget {
    * keeps `budget`. A schedule keeps more survivors on the levels where the ranking is
    * least reliable. */
   const int64_t* level_budget;
+  /* Host keys indexed by search level (0..nlevels), each PYL_BUDGET_SCORE or `kind`, or
+   * null: every level ranks by `kind`. A network budget can rank by S/N where S/N ranking
+   * suffices and skip the network's inference there. */
+  const int32_t* level_kind;
 } pyl_ep_budget;
 
 /* The budget on one tree of n rows (the per-level path): writes the kept rows to out_*

@@ -314,7 +314,8 @@ class SurvivorBudget:
     (``device_budget``), the per-level path calls it."""
 
     def __init__(self, budget, ranker: BranchRanker | None = None,
-                 precision: str = "bf16", hybrid: bool = False) -> None:
+                 precision: str = "bf16", hybrid: bool = False,
+                 net_until: int | None = None) -> None:
         sched = None
         if np.ndim(budget) > 0:
             sched = np.ascontiguousarray(budget, dtype=np.int64)
@@ -335,6 +336,13 @@ class SurvivorBudget:
             msg = "a hybrid budget needs the ranking network"
             raise ValueError(msg)
         self.hybrid = bool(hybrid)
+        # With the network: the last search level ranked by the network's key; later levels
+        # rank by S/N (no inference there). None: every level.
+        if net_until is not None and ranker is None:
+            msg = "net_until needs the ranking network"
+            raise ValueError(msg)
+        self.net_until = None if net_until is None else int(net_until)
+        self._level_kind = None
         self._cap = -1
         self._s = None
         # Level loop: levels that get the ranking and the cut (None: all), see device_budget.
@@ -383,6 +391,13 @@ class SurvivorBudget:
             self._s.level_budget = self.schedule.ctypes.data
         else:
             self._s.level_budget = None
+        if self.net_until is not None:
+            kinds = np.full(self._s.nlevels + 1, _BUDGET_SCORE, dtype=np.int32)
+            kinds[:self.net_until + 1] = self._s.kind
+            self._level_kind = kinds
+            self._s.level_kind = kinds.ctypes.data
+        else:
+            self._s.level_kind = None
         return self._s
 
     def device_budget(self, pr, cap: int, level0: int):

@@ -605,6 +605,9 @@ int pyl::ep_budget_forest_enqueue(const pyl_ep_budget* b, const pyl_ep_forest* F
   PYL_REQUIRE(b->kind == PYL_BUDGET_SCORE || b->kind == PYL_BUDGET_CNN ||
                   b->kind == PYL_BUDGET_HYBRID,
               "unknown budget kind %d", b->kind);
+  const int kind = kind_of(b, level);
+  PYL_REQUIRE(kind == PYL_BUDGET_SCORE || kind == b->kind,
+              "level %d ranks by kind %d, the budget by %d", level, kind, b->kind);
   const int64_t bud = budget_of(b, level);
   PYL_REQUIRE(bud >= 1, "budget must be >= 1");
   PYL_REQUIRE(n_cap >= 0 && n_cap < 0x7fffffffLL, "too many survivors for the budget");
@@ -620,7 +623,7 @@ int pyl::ep_budget_forest_enqueue(const pyl_ep_budget* b, const pyl_ep_forest* F
     PYL_CHECK_LAUNCH("fgather_kernel");
     return PYL_OK;
   }
-  const bool net = b->kind != PYL_BUDGET_SCORE;
+  const bool net = kind != PYL_BUDGET_SCORE;
   PYL_REQUIRE(b->kept && F->sel && F->part && F->kcount &&
                   (!net || (b->params && b->keys && b->feats && F->stats && F->shist)),
               "null budget buffer");
@@ -636,7 +639,7 @@ int pyl::ep_budget_forest_enqueue(const pyl_ep_budget* b, const pyl_ep_forest* F
                                 b->feats, b->keys, b->precision == PYL_BUDGET_BF16, s);
     if (rc != PYL_OK) return rc;
     keys = b->keys;
-    if (b->kind == PYL_BUDGET_HYBRID) {
+    if (kind == PYL_BUDGET_HYBRID) {
       const dim3 gs(kScanBlocks, (unsigned)T), gr(forest_grid(tree_cap, T), (unsigned)T);
       feat_stats_part_kernel<<<gs, kT, 0, s>>>(b->keys, tree_cap, r_dev, F->stats, F->shist,
                                                so);
@@ -682,10 +685,13 @@ int pyl::ep_budget_enqueue(const pyl_ep_budget* b, int P, int level, const doubl
                   b->kind == PYL_BUDGET_HYBRID,
               "unknown budget kind %d", b->kind);
   PYL_REQUIRE(level >= 0 && level <= b->nlevels, "level %d outside the search", level);
+  const int kind = kind_of(b, level);
+  PYL_REQUIRE(kind == PYL_BUDGET_SCORE || kind == b->kind,
+              "level %d ranks by kind %d, the budget by %d", level, kind, b->kind);
   const int64_t bud = budget_of(b, level);
   PYL_REQUIRE(bud >= 1, "budget must be >= 1");
   PYL_REQUIRE(n_cap >= 0 && n_cap < 0x7fffffffLL, "too many survivors for the budget");
-  const bool net = b->kind != PYL_BUDGET_SCORE;
+  const bool net = kind != PYL_BUDGET_SCORE;
   PYL_REQUIRE(b->kept && b->sel && b->part &&
                   (!net || (b->params && b->keys && b->feats && b->stats && b->shist)),
               "null budget buffer");
@@ -701,7 +707,7 @@ int pyl::ep_budget_enqueue(const pyl_ep_budget* b, int P, int level, const doubl
                                 b->feats, b->keys, b->precision == PYL_BUDGET_BF16, s);
     if (rc != PYL_OK) return rc;
     keys = b->keys;
-    if (b->kind == PYL_BUDGET_HYBRID) {  // the logits' histogram, then the hybrid key
+    if (kind == PYL_BUDGET_HYBRID) {  // the logits' histogram, then the hybrid key
       feat_stats_part_kernel<<<kScanBlocks, kT, 0, s>>>(b->keys, n_cap, r_dev, b->stats,
                                                          b->shist);
       feat_hist_kernel<<<stride_grid(n_cap), kT, 0, s>>>(b->keys, n_cap, r_dev, b->stats,

@@ -10,6 +10,10 @@ namespace pyl {
 inline int64_t budget_of(const pyl_ep_budget* b, int level) {
   return b->level_budget != nullptr ? b->level_budget[level] : b->budget;
 }
+// The ranking key of search level `level` (the level's entry of level_kind, else kind).
+inline int kind_of(const pyl_ep_budget* b, int level) {
+  return b->level_kind != nullptr ? b->level_kind[level] : b->kind;
+}
 
 // The network's forward pass over at most n_cap samples, the count read from n_dev
 // when not null; bf16 selects the MFMA inference kernel.

@@ -97,6 +97,8 @@ class CapturedSearch:
             key += [int(s.budget), int(s.kind), int(s.precision), int(s.nlevels)]
             sched = getattr(hook, "schedule", None)
             key.append(None if sched is None else bytes(sched))
+            lk = getattr(hook, "_level_kind", None)
+            key.append(None if lk is None else bytes(lk))
             mask = getattr(hook, "level_mask", None)
             key.append(None if mask is None
                        else bytes(np.ascontiguousarray(mask, dtype=np.uint8)))

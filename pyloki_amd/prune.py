@@ -121,7 +121,7 @@ class EpBudget(ctypes.Structure):
                 ("budget", ctypes.c_int64)] + [
         (name, ctypes.c_void_p) for name in (
             "params", "keys", "feats", "kept", "bt_tmp", "sel", "stats", "shist", "part",
-            "level_mask", "level_budget")]
+            "level_mask", "level_budget", "level_kind")]
 
 
 EP_FOREST_MAX, EP_FOREST_TAB = 64, 6

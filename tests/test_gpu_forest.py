@@ -68,11 +68,13 @@ def _budget(kind: str, nbins: int):
     if kind == "snr":
         return lambda: SurvivorBudget(256)
     ranker = BranchRanker(nbins, seed=2)
+    if kind == "hybrid_then_snr":  # the network's key on levels 0..3, S/N after
+        return lambda: SurvivorBudget(256, ranker, hybrid=True, net_until=3)
     return lambda: SurvivorBudget(256, ranker, hybrid=kind == "hybrid")
 
 
 @pytest.mark.parametrize("use_fourier", [False, True])
-@pytest.mark.parametrize("kind", ["none", "snr", "cnn", "hybrid"])
+@pytest.mark.parametrize("kind", ["none", "snr", "cnn", "hybrid", "hybrid_then_snr"])
 def test_forest_equals_eager(gpu_device, use_fourier, kind):
     _case(f"forest of 4 circular trees, complex={use_fourier}, budget={kind}")
     cfg, dyps = _series((3, 4, 5, 6), use_fourier)
@@ -122,3 +124,33 @@ def test_forest_tree_over_its_limit_goes_on_alone(gpu_device):
     print(f"[result] {fs.stats}", flush=True)
     assert fs.stats["fallbacks"] == 1
     _check(fs, got, dyps, thresholds, ref, None, **kw)
+
+
+@pytest.mark.parametrize("use_fourier", [False, True])
+def test_forest_net_until_switches_the_key(gpu_device, use_fourier):
+    """A network budget that ranks by S/N from level net_until + 1 on: with net_until = -1
+    it keeps exactly what the S/N budget keeps (the network never runs), and with net_until
+    past the last level exactly what the plain hybrid budget keeps."""
+    from pyloki_amd.cnn import BranchRanker, SurvivorBudget
+
+    _case(f"forest net_until limits, complex={use_fourier}")
+    cfg, dyps = _series((3, 4, 5), use_fourier)
+    thresholds = np.linspace(2.0, 8.0, dyps[0].nsegments - 1)
+    ref = dyps[0].nsegments // 2
+    kw = {"max_sugg": 2**20, "batch_size": 256}
+    ranker = BranchRanker(cfg.nbins, seed=2)
+    last = dyps[0].nsegments
+
+    def run(factory):
+        fs = ForestSearch(dyps[0], thresholds, ref, 3, hook_factory=factory, **kw)
+        return fs.run(dyps)
+
+    for a, b in ((lambda: SurvivorBudget(256, ranker, hybrid=True, net_until=-1),
+                  lambda: SurvivorBudget(256)),
+                 (lambda: SurvivorBudget(256, ranker, hybrid=True, net_until=last),
+                  lambda: SurvivorBudget(256, ranker, hybrid=True))):
+        ga, gb = run(a), run(b)
+        for t in range(len(dyps)):
+            np.testing.assert_array_equal(ga[t]["param_sets"], gb[t]["param_sets"])
+            np.testing.assert_array_equal(ga[t]["scores"], gb[t]["scores"])
+            assert ga[t]["level_stats"] == gb[t]["level_stats"]
