@@ -662,7 +662,7 @@ int pyl::ep_budget_forest_select(const pyl_ep_budget* b, const pyl_ep_forest* F,
                                  int64_t n_cap, int64_t* rec, hipStream_t s) {
   PYL_REQUIRE(b != nullptr && F != nullptr, "null budget");
   PYL_REQUIRE(level >= 0 && level <= b->nlevels, "level %d outside the search", level);
-  PYL_REQUIRE(b->kind == PYL_BUDGET_SCORE, "the select-first cut ranks by S/N only");
+  PYL_REQUIRE(kind_of(b, level) == PYL_BUDGET_SCORE, "the select-first cut ranks by S/N only");
   const int64_t bud = budget_of(b, level);
   PYL_REQUIRE(bud >= 1, "budget must be >= 1");
   PYL_REQUIRE(n_cap >= 0 && n_cap < 0x7fffffffLL, "too many survivors for the budget");

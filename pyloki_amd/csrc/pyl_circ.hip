@@ -1177,15 +1177,24 @@ int circ_levels(const pyl_prune_params* p, const pyl_circ_params* q,
                   C->has_taylor,
               "null device pointer");
   const pyl_ep_budget* bud = L->budget;
-  // A forest's S/N budget cuts before anything is materialised: only the kept survivors
-  // are materialised, into the tree buffers in turn as without a budget.
-  const bool cut_first = kForest && bud && bud->kind == PYL_BUDGET_SCORE;
-  const bool pingpong = !bud || cut_first;
-  const int out0 = pingpong ? first_out : 0;  // a budget: materialise into 0, keep into 1
-  PYL_REQUIRE(in_leaves != L->leaves[out0] && in_folds != L->folds[out0],
+  // Tree buffers, level by level. A level without a budget, or a forest level ranked by
+  // S/N (cut before anything is materialised, so only the kept survivors are), writes the
+  // buffer its input is not in (the first level: first_out). A level ranked by a network
+  // materialises every survivor into the buffer its input is not in, then gathers the
+  // kept rows into the other one (the input's, or 1 for an outside input). Host code
+  // follows the same rule to find a level's tree.
+  const auto cuts_first = [&](int l) {
+    return kForest && bud && pyl::kind_of(bud, bud->level0 + l) == PYL_BUDGET_SCORE;
+  };
+  int prev = in_leaves == L->leaves[0] ? 0 : (in_leaves == L->leaves[1] ? 1 : -1);
+  PYL_REQUIRE((prev == 0) == (in_folds == L->folds[0]) &&
+                  (prev == 1) == (in_folds == L->folds[1]),
+              "the input leaves and folds are in different tree buffers");
+  PYL_REQUIRE(prev < 0 || !(!bud || cuts_first(0)) || first_out == 1 - prev,
               "the input tree is the first output buffer");
+  PYL_REQUIRE(kForest || !bud || prev != 0, "the input tree is the first output buffer");
   PYL_REQUIRE(!bud || bud->bt_tmp, "null budget buffer");
-  PYL_REQUIRE(!cut_first || (bud->keys && bud->kept), "null budget buffer");
+  PYL_REQUIRE(!bud || !kForest || (bud->keys && bud->kept), "null budget buffer");
   PYL_REQUIRE((int64_t)p->n_accel * p->n_freq <= 0x7fffffff, "FFA grid too large");
   int T = 1;
   int64_t row_stride = 0;
@@ -1253,10 +1262,14 @@ int circ_levels(const pyl_prune_params* p, const pyl_circ_params* q,
                   "complex folds need a power-of-two nbins (%d)", k.nbins);
     PYL_REQUIRE(seg_idx[l] >= 0, "negative segment index");
     int64_t* rec = L->rec + (int64_t)l * kRecWords;
-    const int in_buf = pingpong ? (first_out + l - 1) & 1 : 1;
-    const double* leaves = l == 0 ? in_leaves : L->leaves[in_buf];
-    const float* folds = l == 0 ? in_folds : L->folds[in_buf];
-    const int o = pingpong ? (first_out + l) & 1 : 0;
+    const double* leaves = l == 0 ? in_leaves : L->leaves[prev];
+    const float* folds = l == 0 ? in_folds : L->folds[prev];
+    const bool cut_first = cuts_first(l);
+    const bool pingpong = !bud || cut_first;
+    // o: the buffer the materialize kernel writes; kb: a network level's kept rows.
+    const int o = pingpong ? (prev < 0 ? first_out : 1 - prev) : (prev == 0 ? 1 : 0);
+    const int kb = 1 - o;
+    prev = pingpong ? o : kb;
     const float* seg = fold + seg_idx[l] * seg_stride;
     const int64_t bs = cc.batch_size;
     FTab ft{};
@@ -1335,13 +1348,13 @@ int circ_levels(const pyl_prune_params* p, const pyl_circ_params* q,
       if (tm) tm->begin(PYL_EP_PH_BUDGET, s);
       if constexpr (kForest)
         rc = pyl::ep_budget_forest_enqueue(bud, F, kP, bud->level0 + l, tab, tab + tabw,
-                                           L->leaves[0], L->folds[0], L->scores[0],
-                                           bud->bt_tmp, L->max_sugg, rec, L->leaves[1],
-                                           L->folds[1], L->scores[1], L->hist, s);
+                                           L->leaves[o], L->folds[o], L->scores[o],
+                                           bud->bt_tmp, L->max_sugg, rec, L->leaves[kb],
+                                           L->folds[kb], L->scores[kb], L->hist, s);
       else
-        rc = pyl::ep_budget_level_enqueue(bud, kP, bud->level0 + l, L->leaves[0],
-                                          L->folds[0], L->scores[0], bud->bt_tmp, L->max_sugg,
-                                          rec, L->leaves[1], L->folds[1], L->scores[1],
+        rc = pyl::ep_budget_level_enqueue(bud, kP, bud->level0 + l, L->leaves[o],
+                                          L->folds[o], L->scores[o], bud->bt_tmp, L->max_sugg,
+                                          rec, L->leaves[kb], L->folds[kb], L->scores[kb],
                                           L->hist, s);
       if (rc != PYL_OK) return rc;
       if (tm) tm->end(s);

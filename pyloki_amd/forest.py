@@ -215,11 +215,9 @@ class ForestSearch:
             budget = self.hook.device_budget(pr0, buf.cap_par, l0)
         buf.s.budget = ctypes.addressof(budget) if budget is not None else None
         first_out = 1 - state["buf"] if state["buf"] >= 0 else 0
+        outs = self._out_buffers(budget, levels, first_out, state["buf"])
         self._f.ntree = T
         self._f.tree_stride = per_tree
-        if not self._pingpong(budget) and state["buf"] == 0:
-            msg = "a forest with a network budget never reads tree buffer 0"
-            raise RuntimeError(msg)
         _lib.check(lib.pyl_circ_forest_levels(
             ctypes.byref(pr0._pp), ctypes.byref(pr0._qq), EP_COMPLEX if pr0._complex else 0,
             coords, thr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
@@ -228,7 +226,7 @@ class ForestSearch:
             sizes.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), buf.hist_next, first_out,
             ctypes.byref(buf.s), ctypes.byref(buf.c), ctypes.byref(self._f), stream_ptr()))
         return {"levels": levels, "thr": thr, "segs": segs, "budget": budget,
-                "first_out": first_out, "state": state}
+                "first_out": first_out, "outs": outs, "state": state}
 
     def _finish(self, prs, run: dict) -> dict | None:
         """Advance every tree over the run's finished levels (one transfer); the state to
@@ -267,7 +265,7 @@ class ForestSearch:
             st = dict(run["state"])
             leaves, folds, src = st["leaves"], st["folds"], st["buf"]
         else:
-            src = (first_out + i_stop - 1) & 1 if self._pingpong(budget) else 1
+            src = run["outs"][i_stop - 1]
             leaves, folds = buf.leaves[src], buf.folds[src]
         state = {"l0": level, "leaves": leaves, "folds": folds, "sizes": sizes, "buf": src}
         if stop == ST_CHILD_CAP and self.cap_child < 2**30:
@@ -307,8 +305,7 @@ class ForestSearch:
         buf = self._buf
         T = len(prs)
         levels, thr, segs = run["levels"], run["thr"], run["segs"]
-        budget, first_out = run["budget"], run["first_out"]
-        pingpong = self._pingpong(budget)
+        outs = run["outs"]
         d = np.diff(tab[:nfin], axis=2)  # [level][table][tree]: per-tree counts
         k0 = tab[:nfin, TAB_KO, :T]
         hist0 = rec[:nfin, REC_HIST]
@@ -335,7 +332,7 @@ class ForestSearch:
                 a = int(hist0[i] + k0[i, t])
                 bts.append(buf.hist.narrow(0, a, m))
             pr._prune_level = levels[last]
-            o = (first_out + last) & 1 if pingpong else 1
+            o = outs[last]
             a, m = int(k0[last, t]), int(d[last, TAB_KO, t])
             sc = buf.scores[o].narrow(0, a, m)
             pr._world_tree = DeviceTree(buf.leaves[o].narrow(0, a, m),
@@ -344,15 +341,26 @@ class ForestSearch:
                                         scores_ep=sc)
             pr._prune_complete = m == 0
         buf.hist_next = int(hist0[nfin - 1] + tab[nfin - 1, TAB_KO, T])
-        self._last = ((first_out + nfin - 1) & 1 if pingpong else 1,
-                      tab[nfin - 1, TAB_KO].copy())
+        self._last = (outs[nfin - 1], tab[nfin - 1, TAB_KO].copy())
 
-    @staticmethod
-    def _pingpong(budget) -> bool:
-        """Whether level l writes tree buffer (first_out + l) % 2 (no budget, or an S/N
-        budget, which the loop cuts before materialising), else buffer 1 (a network's
-        budget: all survivors into buffer 0, the kept rows gathered into buffer 1)."""
-        return budget is None or budget.kind == BUDGET_SCORE
+    def _out_buffers(self, budget, levels, first_out: int, prev: int) -> list[int]:
+        """The tree buffer that holds each level's kept rows, by the level loop's rule: a
+        level without a budget, or one ranked by S/N (cut before materialising), writes the
+        buffer its input is not in (the first level: first_out); a level ranked by a
+        network materialises into the buffer its input is not in and gathers the kept rows
+        into the other (the input's, or 1 for an outside input). ``prev``: the buffer of
+        the run's input, or -1."""
+        lk = getattr(self.hook, "_level_kind", None) if budget is not None else None
+        outs = []
+        for lv in levels:
+            kind = None if budget is None else (int(lk[lv]) if lk is not None else budget.kind)
+            if budget is None or kind == BUDGET_SCORE:
+                out = first_out if prev < 0 else 1 - prev
+            else:
+                out = 0 if prev == 0 else 1
+            outs.append(out)
+            prev = out
+        return outs
 
     def _ascend_report(self, prs) -> list[dict]:
         """The ascend of every tree in one launch, and the reports with one transfer, when
