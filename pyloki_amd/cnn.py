@@ -655,10 +655,13 @@ def batched_throughput(dyps: list, thresholds: np.ndarray, max_sugg: int, hooks:
 
 
 def forest_throughput(dyps: list, thresholds: np.ndarray, max_sugg: int, hooks: dict,
-                      batch_size: int = 1024, ntree: int | None = None) -> dict:
+                      batch_size: int = 1024, ntree: int | None = None,
+                      capture: bool = False) -> dict:
     """``batched_throughput`` with the series searched ``ntree`` at a time (default: all)
     as the trees of one forest (``forest.ForestSearch``): every launch of a level serves
-    all of them. The first pass over the series (untimed) sizes the buffers."""
+    all of them. Untimed passes over the series first: one sizes the buffers and, with
+    ``capture``, a second captures the level loop as a HIP graph, which the timed pass
+    replays."""
     from pyloki_amd.forest import ForestSearch
 
     ref = dyps[0].nsegments // 2
@@ -666,9 +669,10 @@ def forest_throughput(dyps: list, thresholds: np.ndarray, max_sugg: int, hooks: 
     out, finals = {}, {}
     for name, factory in hooks.items():
         fs = ForestSearch(dyps[0], thresholds, ref, T, hook_factory=factory,
-                          max_sugg=max_sugg, batch_size=batch_size)
-        for b0 in range(0, len(dyps), T):  # warm-up
-            fs.run(dyps[b0:b0 + T])
+                          capture=capture, max_sugg=max_sugg, batch_size=batch_size)
+        for _ in range(2 if capture else 1):  # warm-up (and capture)
+            for b0 in range(0, len(dyps), T):
+                fs.run(dyps[b0:b0 + T])
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         res = []

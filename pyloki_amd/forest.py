@@ -49,7 +49,7 @@ class ForestSearch:
     The buffers are sized from an eager run of ``dyp``; a run that needs more grows them."""
 
     def __init__(self, dyp, threshold_scheme: np.ndarray, ref_seg: int, ntree: int,
-                 hook_factory=None, **kw) -> None:
+                 hook_factory=None, capture: bool = False, **kw) -> None:
         if not 1 <= ntree <= EP_FOREST_MAX:
             msg = f"ntree must be in [1, {EP_FOREST_MAX}]"
             raise ValueError(msg)
@@ -94,6 +94,14 @@ class ForestSearch:
         self.seed_scores = torch.empty(T * self.n_seed, dtype=torch.float32,
                                        device=seed.folds.device)
         self.stats = {"resumes": {}, "fallbacks": 0}
+        # capture: a run's first enqueue of the level loop (every level from the seed) is
+        # captured as a HIP graph once the buffers have settled, and replayed by later runs
+        # of as many trees; resumes run eagerly (see _first_enqueue).
+        self.capture = bool(capture)
+        self._graph = None
+        self._graph_key = None
+        self._graph_run = None
+        self._warm_key = None
 
     # -- buffers ---------------------------------------------------------------------
     def _fit_mask(self, pr) -> None:
@@ -153,10 +161,62 @@ class ForestSearch:
                  "sizes": [n] * T, "buf": -1}
         self._last = None
         self._fell_back = False
-        while state is not None:
-            run = self._enqueue(prs, state)
+        run = self._first_enqueue(prs, state)
+        while True:
             state = self._finish(prs, run)
+            if state is None:
+                break
+            run = self._enqueue(prs, state)
         return self._ascend_report(prs)
+
+    def _capture_key(self, T: int) -> tuple:
+        """Everything a captured first enqueue depends on: the trees, the loop's and the
+        forest's device buffers, and the budget's buffers, constants and host arrays."""
+        # Fields that _enqueue sets from the tree count and the hook, both in the key.
+        skip = {"ntree", "tree_stride", "budget", "timer"}
+
+        def vals(st) -> tuple:
+            out = []
+            for name, _ in st._fields_:
+                if name in skip:
+                    continue
+                v = getattr(st, name)
+                out.append(tuple(v) if isinstance(v, ctypes.Array) else v)
+            return tuple(out)
+
+        key = [T, vals(self._buf.s), vals(self._buf.c), vals(self._f)]
+        hook = self.hook
+        s = getattr(hook, "_s", None) if hook is not None else None
+        if s is not None:
+            key += [getattr(s, f) for f in ("kind", "nbins", "complex_folds", "nlevels",
+                                            "precision", "budget", "params", "keys", "feats",
+                                            "kept", "bt_tmp", "sel", "stats", "shist", "part")]
+            for arr in (getattr(hook, "schedule", None), getattr(hook, "_level_kind", None),
+                        getattr(hook, "level_mask", None)):
+                key.append(None if arr is None else bytes(np.ascontiguousarray(arr)))
+        return tuple(key)
+
+    def _first_enqueue(self, prs, state: dict) -> dict:
+        """The run's first enqueue: eager, or (capture) the replay of the graph captured
+        for the same key. A key is run eagerly once before it is captured (that run sizes
+        the buffers and allocates the budget's scratch); the capture is replayed at once."""
+        if not self.capture:
+            return self._enqueue(prs, state)
+        key = self._capture_key(len(prs))
+        if self._graph is not None and key == self._graph_key:
+            self._graph.replay()
+            return self._graph_run
+        self._graph = None
+        if key != self._warm_key:
+            self._warm_key = key
+            return self._enqueue(prs, state)
+        g = torch.cuda.CUDAGraph()
+        torch.cuda.synchronize()
+        with torch.cuda.graph(g):
+            run = self._enqueue(prs, state)
+        self._graph, self._graph_key, self._graph_run = g, key, run
+        g.replay()
+        return run
 
     def _seed(self, prs, dyps) -> None:
         """Every tree's seed: the reference segment's rows of its fold, scored in one

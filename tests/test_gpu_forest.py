@@ -154,3 +154,23 @@ def test_forest_net_until_switches_the_key(gpu_device, use_fourier):
             np.testing.assert_array_equal(ga[t]["param_sets"], gb[t]["param_sets"])
             np.testing.assert_array_equal(ga[t]["scores"], gb[t]["scores"])
             assert ga[t]["level_stats"] == gb[t]["level_stats"]
+
+
+@pytest.mark.parametrize("kind", ["none", "snr", "hybrid"])
+def test_captured_forest_replays_equal_eager(gpu_device, kind):
+    """A forest whose level loop is captured as a HIP graph: the first run eager, the second
+    captured and replayed, the third a replay; each run on other series equals the eager
+    per-tree searches bit for bit."""
+    _case(f"captured forest replays, budget={kind}")
+    cfg, dyps = _series((3, 4, 5, 6), False)
+    thresholds = np.linspace(2.0, 8.0, dyps[0].nsegments - 1)
+    ref = dyps[0].nsegments // 2
+    factory = _budget(kind, cfg.nbins)
+    kw = {"max_sugg": 2**20, "batch_size": 256}
+    fs = ForestSearch(dyps[0], thresholds, ref, 3, hook_factory=factory, capture=True, **kw)
+    for run, sub in enumerate(([dyps[0], dyps[1], dyps[2]], [dyps[3], dyps[1], dyps[0]],
+                               [dyps[2], dyps[3], dyps[1]])):
+        got = fs.run(sub)
+        print(f"[result] run {run}: graph {fs._graph is not None}, {fs.stats}", flush=True)
+        assert (fs._graph is not None) == (run >= 1)  # eager, captured, replayed
+        _check(fs, got, sub, thresholds, ref, factory, **kw)
